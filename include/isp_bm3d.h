@@ -1,0 +1,131 @@
+/*
+ * isp_bm3d.h - C ABI of the MI355X BM3D denoiser (libisp_bm3d.so).
+ *
+ * Drop-in replacement for the BM3D filter of ISP_MW: the Basic / Final /
+ * two-step filters of include/BM3D.h on Plane_FL-like float planes and on
+ * float RGB frames.  No C++ or device-library types cross this boundary:
+ * plain pointers, sizes and int status codes.  include/isp_bm3d.hpp layers
+ * the reference's C++ class names (BM3D_Para, BM3D_Basic, BM3D_Final, BM3D,
+ * Plane_FL, Frame) on top of these functions.
+ *
+ * Units follow the reference: sigma in 0..255, thMSE in MSE units of the
+ * 8-bit scale, float samples with luma range [0,1] and chroma [-0.5,0.5].
+ */
+#ifndef ISP_BM3D_H_
+#define ISP_BM3D_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  Functions return ISP_BM3D_OK (0) or a negative code and
+ * record a message readable with isp_bm3d_last_error(). */
+#define ISP_BM3D_OK 0
+#define ISP_BM3D_E_ARG -1         /* invalid argument (NULL pointer, bad mode, ...) */
+#define ISP_BM3D_E_UNSUPPORTED -2 /* parameters outside what the GPU kernels implement */
+#define ISP_BM3D_E_SIZE -3        /* image smaller than one block */
+#define ISP_BM3D_E_HIP -4         /* HIP runtime error */
+#define ISP_BM3D_E_PROFILE -5     /* unknown profile name */
+
+/* Steps (the `mode` argument). */
+#define ISP_BM3D_BASIC 1 /* BM3D_Basic: hard-threshold step only        */
+#define ISP_BM3D_FINAL 2 /* BM3D_Final: Wiener step only, ref = guide   */
+#define ISP_BM3D_BOTH 3  /* BM3D: basic then final                      */
+
+/* Where the caller's buffers live (the `mem` argument). */
+#define ISP_BM3D_MEM_HOST 0
+#define ISP_BM3D_MEM_DEVICE 1
+
+/* Arithmetic of the transforms and shrinkage.  Matching is always exact
+ * (FP64 distances) and aggregation always FP32. */
+#define ISP_BM3D_PREC_FP32 0 /* plain FP32 path                              */
+#define ISP_BM3D_PREC_FP16 1 /* FP16 MFMA transforms, FP32 accumulation      */
+#define ISP_BM3D_PREC_BF16 2 /* BF16 MFMA transforms, FP32 accumulation      */
+
+/* Mirror of BM3D_Para_Base (include/BM3D.h:15-45). */
+typedef struct isp_bm3d_step_params {
+    char profile[8];  /* "fast", "lc", "np", "high" or "vn"            */
+    double sigma[3];  /* per plane, 0..255 units                        */
+    int block_size;   /* BlockSize                                      */
+    int block_step;   /* BlockStep                                      */
+    int group_size;   /* GroupSize                                      */
+    int bm_range;     /* BMrange                                        */
+    int bm_step;      /* BMstep                                         */
+    double thMSE;     /* before the OPP-norm scaling of the constructor */
+    double lambda;    /* hard-threshold multiplier (basic step)         */
+} isp_bm3d_step_params;
+
+/* Mirror of BM3D_Para (include/BM3D.h:154-168). */
+typedef struct isp_bm3d_params {
+    isp_bm3d_step_params basic;
+    isp_bm3d_step_params final_;
+} isp_bm3d_params;
+
+typedef struct isp_bm3d_ctx isp_bm3d_ctx;
+
+/* BM3D_Para(profile) including thMSE_Default() (include/BM3D.h:27-168).
+ * profile NULL means "fast", the reference default. */
+int isp_bm3d_params_default(isp_bm3d_params *params, const char *profile);
+
+/* BM3D_Basic_Para(profile) / BM3D_Final_Para(profile) (include/BM3D.h:48-151).
+ * final_step = 0 for the basic step, 1 for the final step. */
+int isp_bm3d_step_params_default(isp_bm3d_step_params *p, const char *profile, int final_step);
+
+/* thMSE_Default() (include/BM3D.h:89-97 basic, 142-150 final): recompute
+ * thMSE from sigma[0] and the profile. */
+int isp_bm3d_thmse_default(isp_bm3d_step_params *p, int final_step);
+
+/* Creates a filter context on HIP device `device` (BM3D(const BM3D_Para&),
+ * include/BM3D.h:349-351; tables of source/BM3D.cpp:9-73).  The context owns
+ * its device tables and workspace; it is immutable after creation except for
+ * the precision set with isp_bm3d_set_precision(). */
+int isp_bm3d_create(isp_bm3d_ctx **ctx, const isp_bm3d_params *params, int device);
+int isp_bm3d_destroy(isp_bm3d_ctx *ctx);
+
+/* Message of the last failed call on ctx (or of the last failed create when
+ * ctx is NULL).  Never NULL. */
+const char *isp_bm3d_last_error(const isp_bm3d_ctx *ctx);
+
+/* Transform arithmetic; ISP_BM3D_PREC_FP32 by default. */
+int isp_bm3d_set_precision(isp_bm3d_ctx *ctx, int precision);
+
+/* One float plane (Plane_FL): FilterIF2::process(dst, src, ref) for
+ * BM3D_Basic (mode BASIC, source/BM3D.cpp:80-136,339-385), BM3D_Final (mode
+ * FINAL, :80-136,411-457) or BM3D (mode BOTH, :464-472, with the
+ * intermediate estimate sized like src).  ref may be NULL (= src).  stride is
+ * in elements and applies to dst, src and ref.  mem selects host or device
+ * pointers; stream is a hipStream_t (NULL = default stream).  Host calls
+ * return after the result is in dst; device calls are stream-ordered. */
+int isp_bm3d_plane_f32(isp_bm3d_ctx *ctx, float *dst, const float *src, const float *ref,
+                       int width, int height, size_t stride, int mode, int mem, void *stream);
+
+/* Float RGB frame (planar R, G, B in [0,1]) with the semantics of
+ * process_Frame (source/BM3D.cpp:282-313 for BASIC / FINAL, :501-527 for
+ * BOTH): OPP conversion without clip, matching on Y, OPP inverse with clip to
+ * [0,1].  ref_* may all be NULL (= src).  For BOTH a separate ref only guides
+ * step-1 matching (its Y); for FINAL it is the Wiener guide. */
+int isp_bm3d_rgb_f32(isp_bm3d_ctx *ctx, float *dst_r, float *dst_g, float *dst_b,
+                     const float *src_r, const float *src_g, const float *src_b,
+                     const float *ref_r, const float *ref_g, const float *ref_b,
+                     int width, int height, size_t stride, int mode, int mem, void *stream);
+
+/* Block matching of one step only (BM3D_Base::BlockMatching,
+ * source/BM3D.cpp:221-236, include/Block.h:658-703) on a float guide plane,
+ * for group-index parity checks.  For grid point g = gy*ngx + gx:
+ * counts[g] = N, pos[g*G + z] = (y << 16) | x, keys[g*G + z] = distance key
+ * (MSE units; may be NULL).  Outputs are host arrays; the guide follows mem.
+ * final_step selects which step's parameters are used. */
+int isp_bm3d_match(isp_bm3d_ctx *ctx, const float *guide, int width, int height, size_t stride,
+                   int final_step, int mem, int *counts, int *pos, double *keys);
+
+/* Grid sizes of a step for an image (source/BM3D.cpp:98-127). */
+int isp_bm3d_grid_size(const isp_bm3d_ctx *ctx, int width, int height, int final_step,
+                       int *ngx, int *ngy);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ISP_BM3D_H_ */

@@ -1,0 +1,108 @@
+"""ctypes binding of libisp_bm3d.so (include/isp_bm3d.h).
+
+The library is required: importing the binding fails loudly when it has not
+been built (``python -m isp_mw_amd.build``); there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libisp_bm3d.so")
+
+OK = 0
+E_ARG, E_UNSUPPORTED, E_SIZE, E_HIP, E_PROFILE = -1, -2, -3, -4, -5
+BASIC, FINAL, BOTH = 1, 2, 3
+MEM_HOST, MEM_DEVICE = 0, 1
+PREC_FP32, PREC_FP16, PREC_BF16 = 0, 1, 2
+
+# Every function include/isp_bm3d.h declares.
+EXPORTS = (
+    "isp_bm3d_params_default",
+    "isp_bm3d_step_params_default",
+    "isp_bm3d_thmse_default",
+    "isp_bm3d_create",
+    "isp_bm3d_destroy",
+    "isp_bm3d_last_error",
+    "isp_bm3d_set_precision",
+    "isp_bm3d_plane_f32",
+    "isp_bm3d_rgb_f32",
+    "isp_bm3d_match",
+    "isp_bm3d_grid_size",
+)
+
+
+class StepParams(ctypes.Structure):
+    """isp_bm3d_step_params (mirror of BM3D_Para_Base, include/BM3D.h:15-45)."""
+
+    _fields_ = [
+        ("profile", ctypes.c_char * 8),
+        ("sigma", ctypes.c_double * 3),
+        ("block_size", ctypes.c_int),
+        ("block_step", ctypes.c_int),
+        ("group_size", ctypes.c_int),
+        ("bm_range", ctypes.c_int),
+        ("bm_step", ctypes.c_int),
+        ("thMSE", ctypes.c_double),
+        ("lambda_", ctypes.c_double),
+    ]
+
+
+class Params(ctypes.Structure):
+    """isp_bm3d_params (mirror of BM3D_Para, include/BM3D.h:154-168)."""
+
+    _fields_ = [("basic", StepParams), ("final_", StepParams)]
+
+
+class BM3DError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"isp_bm3d error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m isp_mw_amd.build` "
+            "(the HIP library is required; there is no CPU fallback)")
+    # One HIP runtime per process: when torch is present its libamdhip64.so.7
+    # is loaded first, and the library's NEEDED entry (same soname) binds to
+    # it, so torch tensors and streams can be passed straight through.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t
+    sp = ctypes.POINTER(StepParams)
+    pp = ctypes.POINTER(Params)
+    L.isp_bm3d_params_default.argtypes = [pp, ctypes.c_char_p]
+    L.isp_bm3d_step_params_default.argtypes = [sp, ctypes.c_char_p, i]
+    L.isp_bm3d_thmse_default.argtypes = [sp, i]
+    L.isp_bm3d_create.argtypes = [ctypes.POINTER(vp), pp, i]
+    L.isp_bm3d_destroy.argtypes = [vp]
+    L.isp_bm3d_last_error.argtypes = [vp]
+    L.isp_bm3d_last_error.restype = ctypes.c_char_p
+    L.isp_bm3d_set_precision.argtypes = [vp, i]
+    L.isp_bm3d_plane_f32.argtypes = [vp, vp, vp, vp, i, i, sz, i, i, vp]
+    L.isp_bm3d_rgb_f32.argtypes = [vp] + [vp] * 9 + [i, i, sz, i, i, vp]
+    L.isp_bm3d_match.argtypes = [vp, vp, i, i, sz, i, i, vp, vp, vp]
+    L.isp_bm3d_grid_size.argtypes = [vp, i, i, i, ctypes.POINTER(i), ctypes.POINTER(i)]
+    for name in EXPORTS:
+        getattr(L, name).restype = getattr(L, name).restype or ctypes.c_int
+    L.isp_bm3d_last_error.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def check(rc: int, ctx=None) -> None:
+    if rc != OK:
+        msg = lib().isp_bm3d_last_error(ctx)
+        raise BM3DError(rc, msg.decode() if msg else "")

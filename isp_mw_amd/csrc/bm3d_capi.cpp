@@ -1,0 +1,721 @@
+// C ABI of the MI355X BM3D denoiser: parameters, context, workspace and the
+// orchestration of one or two BM3D steps on one GPU.
+//
+// Reference behaviour mirrored here:
+//   parameter presets and thMSE_Default   include/BM3D.h:15-168
+//   OPP-norm scaling of sigma and thMSE   include/BM3D.h:237-259
+//   copy-through on degenerate parameters source/BM3D.cpp:82-87, 143-151, 284-290
+//   step orchestration                    source/BM3D.cpp:464-527
+#include "../../include/isp_bm3d.h"
+#include "bm3d_internal.h"
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace ispbm3d;
+
+struct isp_bm3d_ctx {
+    isp_bm3d_params params;
+    int device = 0;
+    int precision = ISP_BM3D_PREC_FP32;
+    float *d_tables = nullptr;  // C8 (64) then the padded C_N matrices
+    int cn_off[kMaxGroup + 1] = {0};
+    void *ws = nullptr;
+    size_t ws_bytes = 0;
+    std::string err;
+};
+
+namespace {
+
+thread_local std::string g_create_err = "no error";
+
+int fail(isp_bm3d_ctx *ctx, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->err = buf;
+    else g_create_err = buf;
+    return code;
+}
+
+#define HIP_CHECK(ctx, expr)                                                                   \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail((ctx), ISP_BM3D_E_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                                   \
+    } while (0)
+
+bool known_profile(const char *p)
+{
+    return !strcmp(p, "fast") || !strcmp(p, "lc") || !strcmp(p, "np") || !strcmp(p, "high") ||
+           !strcmp(p, "vn");
+}
+
+// OPP norms (include/BM3D.h:241-248 with include/Specification.h:391-407).
+void opp_norms(double norm[3])
+{
+    const double Yr = (double)(1.0L / 3.0L), Yg = Yr, Yb = Yr;
+    const double Ur = (double)(1.0L / 2.0L), Ug = 0.0, Ub = (double)(-1.0L / 2.0L);
+    const double Vr = (double)(1.0L / 4.0L), Vg = (double)(-1.0L / 2.0L), Vb = (double)(1.0L / 4.0L);
+    norm[0] = std::sqrt(Yr * Yr + Yg * Yg + Yb * Yb);
+    norm[1] = std::sqrt(Ur * Ur + Ug * Ug + Ub * Ub);
+    norm[2] = std::sqrt(Vr * Vr + Vg * Vg + Vb * Vb);
+}
+
+int next_pow2(int n)
+{
+    int p = 1;
+    while (p < n) p <<= 1;
+    return p;
+}
+
+// Parameters of one step after the constructor's scaling.
+struct Derived {
+    bool degenerate;  // copy src through (Kernel's early return)
+    int range;
+    bool bypass;
+    double thSSE, distMul;
+    float thr[3], sig2[3];
+    int active[3];
+    int ngx, ngy;
+};
+
+int derive(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, int nplanes, int W, int H, Derived &d)
+{
+    double norm[3];
+    opp_norms(norm);
+    const double thMSE = p.thMSE * norm[0];
+    bool nosigma = true;
+    for (int c = 0; c < nplanes; ++c) nosigma = nosigma && !(p.sigma[c] > 0);
+    d.degenerate = nosigma || p.group_size == 0 || p.block_size <= 0 || p.bm_range <= 0 ||
+                   p.bm_range < p.bm_step || thMSE <= 0;
+    if (d.degenerate) return ISP_BM3D_OK;
+    if (p.block_size != kBlock)
+        return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "BlockSize %d: the GPU kernels implement BlockSize 8",
+                    p.block_size);
+    if (p.group_size < 0 || p.group_size > kMaxGroup)
+        return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "GroupSize %d outside 1..%d", p.group_size, kMaxGroup);
+    if (p.block_step <= 0 || p.bm_step <= 0)
+        return fail(ctx, ISP_BM3D_E_ARG, "BlockStep and BMstep must be positive");
+    if (W < kBlock || H < kBlock)
+        return fail(ctx, ISP_BM3D_E_SIZE, "image %dx%d is smaller than one %dx%d block", W, H, kBlock, kBlock);
+    if (W > 65535 || H > 65535)
+        return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "image %dx%d: width and height must be < 65536", W, H);
+    d.range = p.bm_range / p.bm_step * p.bm_step;
+    d.bypass = p.group_size == 1 || thMSE <= 0;
+    // include/Block.h:582-584 with range 1 (Plane_FL default Floor 0 / Ceil 1)
+    const double MSE2SSE = (double)(kBlock * kBlock) * 1.0 * 1.0 / (double)(255 * 255);
+    d.distMul = 1.0 / MSE2SSE;
+    d.thSSE = thMSE * MSE2SSE;
+    for (int c = 0; c < 3; ++c) {
+        d.active[c] = c < nplanes && p.sigma[c] > 0;
+        const double sigma = p.sigma[c] / 255.0 * norm[c];
+        d.thr[c] = (float)(sigma * p.lambda);
+        d.sig2[c] = (float)(sigma * sigma);
+    }
+    d.ngx = (W - kBlock) / p.block_step + 1 + ((W - kBlock) % p.block_step ? 1 : 0);
+    d.ngy = (H - kBlock) / p.block_step + 1 + ((H - kBlock) % p.block_step ? 1 : 0);
+    return ISP_BM3D_OK;
+}
+
+// Workspace carving: a bump allocator over one device allocation.
+struct Carver {
+    char *base;
+    size_t off = 0;
+    explicit Carver(void *b) : base((char *)b) {}
+    template <typename T>
+    T *take(size_t n)
+    {
+        off = (off + 255) & ~size_t(255);
+        T *p = reinterpret_cast<T *>(base ? base + off : nullptr);
+        off += n * sizeof(T);
+        return p;
+    }
+};
+
+int ensure_ws(isp_bm3d_ctx *ctx, size_t bytes)
+{
+    if (bytes <= ctx->ws_bytes) return ISP_BM3D_OK;
+    if (ctx->ws) {
+        HIP_CHECK(ctx, hipDeviceSynchronize());
+        HIP_CHECK(ctx, hipFree(ctx->ws));
+        ctx->ws = nullptr;
+        ctx->ws_bytes = 0;
+    }
+    bytes = (bytes + (1u << 20)) & ~size_t((1u << 20) - 1);
+    HIP_CHECK(ctx, hipMalloc(&ctx->ws, bytes));
+    ctx->ws_bytes = bytes;
+    return ISP_BM3D_OK;
+}
+
+// Per-call device buffers.
+struct Work {
+    float *plane[3];   // src YUV (RGB path) or uploaded src (host plane path)
+    float *refp[3];    // uploaded/converted guide planes
+    float *basic[3];   // step-1 estimate
+    float *num[3], *den[3];
+    float *out[3];     // host-mode output staging
+    double *y64;       // FP64 matching guide (RGB path)
+    double *ref64;     // FP64 matching guide of a separate ref frame
+    int *cnt, *pos;
+    double *keys;
+};
+
+size_t layout(Work *w, void *base, int P, int H, size_t refs, bool keys)
+{
+    Carver c(base);
+    const size_t np = (size_t)P * H;
+    for (int k = 0; k < 3; ++k) w->plane[k] = c.take<float>(np);
+    for (int k = 0; k < 3; ++k) w->refp[k] = c.take<float>(np);
+    for (int k = 0; k < 3; ++k) w->basic[k] = c.take<float>(np);
+    for (int k = 0; k < 3; ++k) w->num[k] = c.take<float>(np);
+    for (int k = 0; k < 3; ++k) w->den[k] = c.take<float>(np);
+    for (int k = 0; k < 3; ++k) w->out[k] = c.take<float>(np);
+    w->y64 = c.take<double>(np);
+    w->ref64 = c.take<double>(np);
+    w->cnt = c.take<int>(refs);
+    w->pos = c.take<int>(refs * kMaxGroup);
+    w->keys = keys ? c.take<double>(refs * kMaxGroup) : nullptr;
+    return c.off + 256;
+}
+
+int prepare(isp_bm3d_ctx *ctx, Work &w, int P, int H, int W, bool keys)
+{
+    // grid size bound over both steps
+    size_t refs = 1;
+    const isp_bm3d_step_params *ps[2] = {&ctx->params.basic, &ctx->params.final_};
+    for (auto *p : ps) {
+        const int s = p->block_step > 0 ? p->block_step : 1;
+        const size_t gx = (size_t)(W / s + 2), gy = (size_t)(H / s + 2);
+        refs = std::max(refs, gx * gy);
+    }
+    const size_t bytes = layout(&w, nullptr, P, H, refs, keys);
+    int rc = ensure_ws(ctx, bytes);
+    if (rc) return rc;
+    layout(&w, ctx->ws, P, H, refs, keys);
+    return ISP_BM3D_OK;
+}
+
+struct StepIO {
+    int nplanes;
+    const float *src[3];
+    const float *guide[3];     // Wiener guides (final step)
+    const float *mguide32;     // matching guide, FP32 ...
+    const double *mguide64;    // ... or FP64
+    int pitch;                 // of src, guide, mguide32 and the accumulators
+    int pitch64;
+    int to_rgb;
+    float *out[3];
+    size_t out_pitch;
+};
+
+int run_match(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, const Derived &d, Work &w,
+              const float *g32, const double *g64, int gpitch, int W, int H, double *keys,
+              hipStream_t s)
+{
+    MatchArgs a{};
+    a.W = W;
+    a.H = H;
+    a.ngx = d.ngx;
+    a.ngy = d.ngy;
+    a.bstep = p.block_step;
+    a.range = d.range;
+    a.step = p.bm_step;
+    a.G = p.group_size;
+    a.bypass = d.bypass;
+    a.thSSE = d.thSSE;
+    a.distMul = d.distMul;
+    a.tileW = (kTileRX - 1) * p.block_step + 2 * d.range + kBlock + (kStrip - 1) * p.bm_step;
+    a.tileH = (kTileRY - 1) * p.block_step + 2 * d.range + kBlock;
+    a.tilePitch = a.tileW | 1;
+    const int side = 2 * (d.range / p.bm_step) + 1;
+    a.maxCand = side * side;
+    a.cnt_out = w.cnt;
+    a.pos_out = w.pos;
+    a.key_out = keys;
+    if (match_lds_bytes(a) > 160 * 1024)
+        return fail(ctx, ISP_BM3D_E_UNSUPPORTED,
+                    "BMrange %d / BMstep %d / BlockStep %d need %zu bytes of LDS for matching (max 163840)",
+                    p.bm_range, p.bm_step, p.block_step, match_lds_bytes(a));
+    if (g64) HIP_CHECK(ctx, launch_match_f64(g64, gpitch, a, s));
+    else HIP_CHECK(ctx, launch_match_f32(g32, gpitch, a, s));
+    return ISP_BM3D_OK;
+}
+
+// One BM3D_Base::Kernel (source/BM3D.cpp:80-218) followed by finalize.
+int run_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, const StepIO &io, Work &w,
+             int W, int H, hipStream_t s)
+{
+    Derived d{};
+    int rc = derive(ctx, p, io.nplanes, W, H, d);
+    if (rc) return rc;
+    FinalizeArgs fa{};
+    fa.W = W;
+    fa.H = H;
+    fa.pitch = io.pitch;
+    fa.nplanes = io.nplanes;
+    fa.to_rgb = io.to_rgb;
+    fa.out_pitch = io.out_pitch;
+    for (int c = 0; c < 3; ++c) {
+        fa.src[c] = io.src[c];
+        fa.out[c] = io.out[c];
+        fa.num[c] = w.num[c];
+        fa.den[c] = w.den[c];
+        fa.active[c] = 0;
+    }
+    if (!d.degenerate) {
+        rc = run_match(ctx, p, d, w, io.mguide32, io.mguide64, io.mguide64 ? io.pitch64 : io.pitch, W, H,
+                       nullptr, s);
+        if (rc) return rc;
+        FilterArgs f{};
+        f.W = W;
+        f.H = H;
+        f.pitch = io.pitch;
+        f.ngx = d.ngx;
+        f.ngy = d.ngy;
+        f.bstep = p.block_step;
+        f.range = d.range;
+        f.G = p.group_size;
+        f.cnt = w.cnt;
+        f.pos = w.pos;
+        for (int c = 0; c < 3; ++c) {
+            f.src[c] = io.src[c];
+            f.guide[c] = io.guide[c];
+            f.num[c] = w.num[c];
+            f.den[c] = w.den[c];
+            f.active[c] = d.active[c];
+            f.thr[c] = d.thr[c];
+            f.sig2[c] = d.sig2[c];
+            fa.active[c] = d.active[c];
+            if (d.active[c]) {
+                HIP_CHECK(ctx, launch_fill(w.num[c], (size_t)io.pitch * H, 0.f, s));
+                HIP_CHECK(ctx, launch_fill(w.den[c], (size_t)io.pitch * H, 0.f, s));
+            }
+        }
+        f.FW = (kTileRX - 1) * p.block_step + 2 * d.range + kBlock;
+        f.FH = (kTileRY - 1) * p.block_step + 2 * d.range + kBlock;
+        f.c8 = ctx->d_tables;
+        f.cn = ctx->d_tables;
+        for (int n = 0; n <= kMaxGroup; ++n) f.cn_off[n] = ctx->cn_off[n];
+        if (filter_lds_bytes(f, wiener) > 160 * 1024)
+            return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "parameters need %zu bytes of LDS for filtering",
+                        filter_lds_bytes(f, wiener));
+        HIP_CHECK(ctx, launch_filter(f, wiener, s));
+    }
+    HIP_CHECK(ctx, launch_finalize(fa, s));
+    return ISP_BM3D_OK;
+}
+
+int check_common(isp_bm3d_ctx *ctx, int W, int H, size_t stride, int mode, int mem)
+{
+    if (!ctx) return fail(nullptr, ISP_BM3D_E_ARG, "null context");
+    if (mode < ISP_BM3D_BASIC || mode > ISP_BM3D_BOTH) return fail(ctx, ISP_BM3D_E_ARG, "bad mode %d", mode);
+    if (mem != ISP_BM3D_MEM_HOST && mem != ISP_BM3D_MEM_DEVICE)
+        return fail(ctx, ISP_BM3D_E_ARG, "bad mem %d", mem);
+    if (W <= 0 || H <= 0) return fail(ctx, ISP_BM3D_E_SIZE, "empty image %dx%d", W, H);
+    if (stride < (size_t)W) return fail(ctx, ISP_BM3D_E_ARG, "stride %zu < width %d", stride, W);
+    if (stride > (size_t)1 << 30) return fail(ctx, ISP_BM3D_E_ARG, "stride %zu too large", stride);
+    if (ctx->precision != ISP_BM3D_PREC_FP32)
+        return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "precision %d is not available in this build", ctx->precision);
+    HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    return ISP_BM3D_OK;
+}
+
+int internal_pitch(int W, size_t stride, int mem)
+{
+    return mem == ISP_BM3D_MEM_DEVICE ? (int)stride : ((W + 63) / 64) * 64;
+}
+
+void set_default_step(isp_bm3d_step_params *p, const char *profile, int final_step)
+{
+    std::memset(p, 0, sizeof(*p));
+    std::strncpy(p->profile, profile, sizeof(p->profile) - 1);
+    p->sigma[0] = p->sigma[1] = p->sigma[2] = 10.0;
+    p->block_size = 8;
+    p->bm_range = 16;
+    p->bm_step = 1;
+    const std::string pr = profile;
+    if (pr == "fast" || pr == "lc") p->bm_range = 9;
+    if (!final_step) {
+        p->block_step = 4;
+        p->group_size = 16;
+        p->lambda = 2.7;
+        if (pr == "fast") p->block_step = 8;
+        else if (pr == "lc") p->block_step = 6;
+        else if (pr == "high") p->block_step = 3;
+        else if (pr == "vn") {
+            p->block_step = 4;
+            p->group_size = 32;
+            p->lambda = 2.8;
+        }
+    } else {
+        p->block_step = 3;
+        p->group_size = 32;
+        // BM3D_Final_Para leaves lambda unset; the final step never reads it.
+        p->lambda = 2.7;
+        if (pr == "fast") {
+            p->block_step = 7;
+            p->group_size = 16;
+        } else if (pr == "lc") {
+            p->block_step = 5;
+            p->group_size = 16;
+        } else if (pr == "high") {
+            p->block_step = 2;
+        } else if (pr == "vn") {
+            p->block_size = 11;
+            p->block_step = 6;
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int isp_bm3d_thmse_default(isp_bm3d_step_params *p, int final_step)
+{
+    if (!p) return fail(nullptr, ISP_BM3D_E_ARG, "null params");
+    const bool vn = !std::strcmp(p->profile, "vn");
+    if (!final_step) p->thMSE = vn ? 1000 + p->sigma[0] * 150 : 400 + p->sigma[0] * 80;
+    else p->thMSE = vn ? 400 + p->sigma[0] * 40 : 200 + p->sigma[0] * 10;
+    return ISP_BM3D_OK;
+}
+
+int isp_bm3d_step_params_default(isp_bm3d_step_params *p, const char *profile, int final_step)
+{
+    if (!p) return fail(nullptr, ISP_BM3D_E_ARG, "null params");
+    if (!profile) profile = "fast";
+    if (!known_profile(profile))
+        return fail(nullptr, ISP_BM3D_E_PROFILE, "unrecognized profile \"%s\", should be \"fast\", \"lc\", "
+                    "\"np\", \"vn\" or \"high\"", profile);
+    set_default_step(p, profile, final_step);
+    return isp_bm3d_thmse_default(p, final_step);
+}
+
+int isp_bm3d_params_default(isp_bm3d_params *params, const char *profile)
+{
+    if (!params) return fail(nullptr, ISP_BM3D_E_ARG, "null params");
+    int rc = isp_bm3d_step_params_default(&params->basic, profile, 0);
+    if (rc) return rc;
+    return isp_bm3d_step_params_default(&params->final_, profile, 1);
+}
+
+const char *isp_bm3d_last_error(const isp_bm3d_ctx *ctx)
+{
+    return ctx ? ctx->err.c_str() : g_create_err.c_str();
+}
+
+int isp_bm3d_create(isp_bm3d_ctx **out, const isp_bm3d_params *params, int device)
+{
+    if (!out || !params) return fail(nullptr, ISP_BM3D_E_ARG, "null argument");
+    *out = nullptr;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= 0)
+        return fail(nullptr, ISP_BM3D_E_HIP, "no HIP device available (%s)", hipGetErrorString(e));
+    if (device < 0 || device >= ndev) return fail(nullptr, ISP_BM3D_E_ARG, "device %d of %d", device, ndev);
+    isp_bm3d_ctx *ctx = new isp_bm3d_ctx();
+    ctx->params = *params;
+    ctx->device = device;
+    ctx->err = "no error";
+    // Orthonormal DCT-II matrices: C8 [k][n], then C_N for N = 1..32, each
+    // padded to NP x NP (NP = next power of two) with zeros.
+    std::vector<float> tab(64);
+    const long double PI = 3.141592653589793238462643383279502884L;
+    for (int k = 0; k < 8; ++k)
+        for (int n = 0; n < 8; ++n)
+            tab[k * 8 + n] = (float)((k == 0 ? std::sqrt(1.0L / 8) : std::sqrt(2.0L / 8)) *
+                                     std::cos(PI * (2 * n + 1) * k / 16.0L));
+    ctx->cn_off[0] = 0;
+    for (int N = 1; N <= kMaxGroup; ++N) {
+        const int NP = next_pow2(N);
+        ctx->cn_off[N] = (int)tab.size();
+        const size_t base = tab.size();
+        tab.resize(base + (size_t)NP * NP, 0.f);
+        for (int k = 0; k < N; ++k)
+            for (int z = 0; z < N; ++z)
+                tab[base + (size_t)k * NP + z] =
+                    (float)((k == 0 ? std::sqrt(1.0L / N) : std::sqrt(2.0L / N)) *
+                            std::cos(PI * (2 * z + 1) * k / (2.0L * N)));
+    }
+    if ((e = hipSetDevice(device)) != hipSuccess ||
+        (e = hipMalloc(&ctx->d_tables, tab.size() * sizeof(float))) != hipSuccess ||
+        (e = hipMemcpy(ctx->d_tables, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice)) !=
+            hipSuccess) {
+        int rc = fail(nullptr, ISP_BM3D_E_HIP, "create: %s", hipGetErrorString(e));
+        if (ctx->d_tables) (void)hipFree(ctx->d_tables);
+        delete ctx;
+        return rc;
+    }
+    *out = ctx;
+    return ISP_BM3D_OK;
+}
+
+int isp_bm3d_destroy(isp_bm3d_ctx *ctx)
+{
+    if (!ctx) return ISP_BM3D_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();
+    if (ctx->ws) (void)hipFree(ctx->ws);
+    if (ctx->d_tables) (void)hipFree(ctx->d_tables);
+    delete ctx;
+    return ISP_BM3D_OK;
+}
+
+int isp_bm3d_set_precision(isp_bm3d_ctx *ctx, int precision)
+{
+    if (!ctx) return fail(nullptr, ISP_BM3D_E_ARG, "null context");
+    if (precision < ISP_BM3D_PREC_FP32 || precision > ISP_BM3D_PREC_BF16)
+        return fail(ctx, ISP_BM3D_E_ARG, "bad precision %d", precision);
+    ctx->precision = precision;
+    return ISP_BM3D_OK;
+}
+
+int isp_bm3d_grid_size(const isp_bm3d_ctx *ctx, int W, int H, int final_step, int *ngx, int *ngy)
+{
+    if (!ctx || !ngx || !ngy) return ISP_BM3D_E_ARG;
+    const isp_bm3d_step_params &p = final_step ? ctx->params.final_ : ctx->params.basic;
+    if (p.block_step <= 0 || p.block_size <= 0 || W < p.block_size || H < p.block_size) return ISP_BM3D_E_ARG;
+    const int lx = W - p.block_size, ly = H - p.block_size;
+    *ngx = lx / p.block_step + 1 + (lx % p.block_step ? 1 : 0);
+    *ngy = ly / p.block_step + 1 + (ly % p.block_step ? 1 : 0);
+    return ISP_BM3D_OK;
+}
+
+int isp_bm3d_plane_f32(isp_bm3d_ctx *ctx, float *dst, const float *src, const float *ref, int W, int H,
+                       size_t stride, int mode, int mem, void *stream)
+{
+    int rc = check_common(ctx, W, H, stride, mode, mem);
+    if (rc) return rc;
+    if (!dst || !src) return fail(ctx, ISP_BM3D_E_ARG, "null dst/src");
+    hipStream_t s = (hipStream_t)stream;
+    const bool host = mem == ISP_BM3D_MEM_HOST;
+    const int P = internal_pitch(W, stride, mem);
+    Work w{};
+    if ((rc = prepare(ctx, w, P, H, W, false))) return rc;
+    const bool sep_ref = ref && ref != src;
+    const float *dsrc = src, *dref = sep_ref ? ref : src;
+    float *ddst = dst;
+    if (host) {
+        HIP_CHECK(ctx, hipMemcpy2DAsync(w.plane[0], (size_t)P * 4, src, stride * 4, (size_t)W * 4, H,
+                                        hipMemcpyHostToDevice, s));
+        dsrc = w.plane[0];
+        dref = dsrc;
+        if (sep_ref) {
+            HIP_CHECK(ctx, hipMemcpy2DAsync(w.refp[0], (size_t)P * 4, ref, stride * 4, (size_t)W * 4, H,
+                                            hipMemcpyHostToDevice, s));
+            dref = w.refp[0];
+        }
+        ddst = w.out[0];
+    }
+    StepIO io{};
+    io.nplanes = 1;
+    io.pitch = P;
+    io.out_pitch = (size_t)P;
+    io.src[0] = dsrc;
+    if (mode == ISP_BM3D_BASIC || mode == ISP_BM3D_FINAL) {
+        const bool fin = mode == ISP_BM3D_FINAL;
+        io.guide[0] = dref;
+        io.mguide32 = dref;
+        io.out[0] = ddst;
+        rc = run_step(ctx, fin ? ctx->params.final_ : ctx->params.basic, fin, io, w, W, H, s);
+    } else {
+        io.mguide32 = dref;
+        io.guide[0] = dref;
+        io.out[0] = w.basic[0];
+        rc = run_step(ctx, ctx->params.basic, false, io, w, W, H, s);
+        if (!rc) {
+            io.mguide32 = w.basic[0];
+            io.guide[0] = w.basic[0];
+            io.out[0] = ddst;
+            rc = run_step(ctx, ctx->params.final_, true, io, w, W, H, s);
+        }
+    }
+    if (rc) return rc;
+    if (host) {
+        HIP_CHECK(ctx, hipMemcpy2DAsync(dst, stride * 4, ddst, (size_t)P * 4, (size_t)W * 4, H,
+                                        hipMemcpyDeviceToHost, s));
+        HIP_CHECK(ctx, hipStreamSynchronize(s));
+    }
+    return ISP_BM3D_OK;
+}
+
+int isp_bm3d_rgb_f32(isp_bm3d_ctx *ctx, float *dst_r, float *dst_g, float *dst_b, const float *src_r,
+                     const float *src_g, const float *src_b, const float *ref_r, const float *ref_g,
+                     const float *ref_b, int W, int H, size_t stride, int mode, int mem, void *stream)
+{
+    int rc = check_common(ctx, W, H, stride, mode, mem);
+    if (rc) return rc;
+    if (!dst_r || !dst_g || !dst_b || !src_r || !src_g || !src_b)
+        return fail(ctx, ISP_BM3D_E_ARG, "null dst/src plane");
+    const bool has_ref = ref_r || ref_g || ref_b;
+    if (has_ref && !(ref_r && ref_g && ref_b)) return fail(ctx, ISP_BM3D_E_ARG, "ref needs all three planes");
+    const bool sep_ref = has_ref && (ref_r != src_r || ref_g != src_g || ref_b != src_b);
+    hipStream_t s = (hipStream_t)stream;
+    const bool host = mem == ISP_BM3D_MEM_HOST;
+    const int P = internal_pitch(W, stride, mem);
+    Work w{};
+    if ((rc = prepare(ctx, w, P, H, W, false))) return rc;
+
+    const float *rgb[3] = {src_r, src_g, src_b};
+    const float *rrgb[3] = {ref_r, ref_g, ref_b};
+    float *drgb[3] = {dst_r, dst_g, dst_b};
+    size_t in_pitch = stride;
+    if (host) {
+        // stage RGB in the out/basic planes, which are free until later
+        for (int c = 0; c < 3; ++c) {
+            HIP_CHECK(ctx, hipMemcpy2DAsync(w.out[c], (size_t)P * 4, rgb[c], stride * 4, (size_t)W * 4, H,
+                                            hipMemcpyHostToDevice, s));
+            rgb[c] = w.out[c];
+            if (sep_ref) {
+                HIP_CHECK(ctx, hipMemcpy2DAsync(w.basic[c], (size_t)P * 4, rrgb[c], stride * 4, (size_t)W * 4,
+                                                H, hipMemcpyHostToDevice, s));
+                rrgb[c] = w.basic[c];
+            }
+        }
+        in_pitch = (size_t)P;
+    }
+    // BM3D_Base::process_Frame's own early return (source/BM3D.cpp:284-290)
+    if (mode != ISP_BM3D_BOTH) {
+        const isp_bm3d_step_params &p = mode == ISP_BM3D_BASIC ? ctx->params.basic : ctx->params.final_;
+        double norm[3];
+        opp_norms(norm);
+        if ((!(p.sigma[0] > 0) && !(p.sigma[1] > 0) && !(p.sigma[2] > 0)) || p.group_size == 1 ||
+            p.block_size <= 0 || p.bm_range <= 0 || p.bm_range < p.bm_step || p.thMSE * norm[0] <= 0) {
+            for (int c = 0; c < 3; ++c) {
+                if (host) {
+                    HIP_CHECK(ctx, hipMemcpy2DAsync(drgb[c], stride * 4, rgb[c], in_pitch * 4, (size_t)W * 4, H,
+                                                    hipMemcpyDeviceToHost, s));
+                } else if (drgb[c] != rgb[c]) {
+                    HIP_CHECK(ctx, launch_copy2d(drgb[c], stride, rgb[c], in_pitch, W, H, s));
+                }
+            }
+            if (host) HIP_CHECK(ctx, hipStreamSynchronize(s));
+            return ISP_BM3D_OK;
+        }
+    }
+    // RGB -> OPP of src (FP32 planes + FP64 luma guide)
+    OppArgs o{};
+    o.W = W;
+    o.H = H;
+    o.r = rgb[0];
+    o.g = rgb[1];
+    o.b = rgb[2];
+    o.in_pitch = in_pitch;
+    o.y = w.plane[0];
+    o.u = w.plane[1];
+    o.v = w.plane[2];
+    o.pitch = P;
+    o.y64 = w.y64;
+    o.pitch64 = P;
+    HIP_CHECK(ctx, launch_opp(o, s));
+    if (sep_ref) {
+        OppArgs q = o;
+        q.r = rrgb[0];
+        q.g = rrgb[1];
+        q.b = rrgb[2];
+        q.y64 = w.ref64;
+        if (mode == ISP_BM3D_FINAL) {  // BM3D_Final::RGB2YUV converts the whole guide
+            q.y = w.refp[0];
+            q.u = w.refp[1];
+            q.v = w.refp[2];
+        } else {                       // BM3D_Basic::RGB2YUV: ConvertToY only
+            q.y = q.u = q.v = nullptr;
+        }
+        HIP_CHECK(ctx, launch_opp(q, s));
+    }
+    float *out[3] = {drgb[0], drgb[1], drgb[2]};
+    size_t out_pitch = stride;
+    if (host) {
+        for (int c = 0; c < 3; ++c) out[c] = w.out[c];
+        out_pitch = (size_t)P;
+    }
+    StepIO io{};
+    io.nplanes = 3;
+    io.pitch = P;
+    io.pitch64 = P;
+    for (int c = 0; c < 3; ++c) io.src[c] = w.plane[c];
+    if (mode == ISP_BM3D_BASIC) {
+        io.mguide64 = sep_ref ? w.ref64 : w.y64;
+        for (int c = 0; c < 3; ++c) io.guide[c] = w.plane[c];
+        io.to_rgb = 1;
+        for (int c = 0; c < 3; ++c) io.out[c] = out[c];
+        io.out_pitch = out_pitch;
+        rc = run_step(ctx, ctx->params.basic, false, io, w, W, H, s);
+    } else if (mode == ISP_BM3D_FINAL) {
+        io.mguide64 = sep_ref ? w.ref64 : w.y64;
+        for (int c = 0; c < 3; ++c) io.guide[c] = sep_ref ? w.refp[c] : w.plane[c];
+        io.to_rgb = 1;
+        for (int c = 0; c < 3; ++c) io.out[c] = out[c];
+        io.out_pitch = out_pitch;
+        rc = run_step(ctx, ctx->params.final_, true, io, w, W, H, s);
+    } else {
+        io.mguide64 = sep_ref ? w.ref64 : w.y64;
+        for (int c = 0; c < 3; ++c) {
+            io.guide[c] = w.plane[c];
+            io.out[c] = w.basic[c];
+        }
+        io.to_rgb = 0;
+        io.out_pitch = (size_t)P;
+        rc = run_step(ctx, ctx->params.basic, false, io, w, W, H, s);
+        if (!rc) {
+            io.mguide64 = nullptr;
+            io.mguide32 = w.basic[0];
+            for (int c = 0; c < 3; ++c) {
+                io.guide[c] = w.basic[c];
+                io.out[c] = out[c];
+            }
+            io.to_rgb = 1;
+            io.out_pitch = out_pitch;
+            rc = run_step(ctx, ctx->params.final_, true, io, w, W, H, s);
+        }
+    }
+    if (rc) return rc;
+    if (host) {
+        for (int c = 0; c < 3; ++c)
+            HIP_CHECK(ctx, hipMemcpy2DAsync(drgb[c], stride * 4, out[c], (size_t)P * 4, (size_t)W * 4, H,
+                                            hipMemcpyDeviceToHost, s));
+        HIP_CHECK(ctx, hipStreamSynchronize(s));
+    }
+    return ISP_BM3D_OK;
+}
+
+int isp_bm3d_match(isp_bm3d_ctx *ctx, const float *guide, int W, int H, size_t stride, int final_step,
+                   int mem, int *counts, int *pos, double *keys)
+{
+    int rc = check_common(ctx, W, H, stride, ISP_BM3D_BASIC, mem);
+    if (rc) return rc;
+    if (!guide || !counts || !pos) return fail(ctx, ISP_BM3D_E_ARG, "null argument");
+    const isp_bm3d_step_params &p = final_step ? ctx->params.final_ : ctx->params.basic;
+    Derived d{};
+    if ((rc = derive(ctx, p, 1, W, H, d))) return rc;
+    if (d.degenerate) return fail(ctx, ISP_BM3D_E_ARG, "parameters are degenerate (no matching runs)");
+    const int P = internal_pitch(W, stride, mem);
+    Work w{};
+    if ((rc = prepare(ctx, w, P, H, W, true))) return rc;
+    hipStream_t s = nullptr;
+    const float *g = guide;
+    if (mem == ISP_BM3D_MEM_HOST) {
+        HIP_CHECK(ctx, hipMemcpy2DAsync(w.plane[0], (size_t)P * 4, guide, stride * 4, (size_t)W * 4, H,
+                                        hipMemcpyHostToDevice, s));
+        g = w.plane[0];
+    }
+    rc = run_match(ctx, p, d, w, g, nullptr, P, W, H, w.keys, s);
+    if (rc) return rc;
+    const size_t refs = (size_t)d.ngx * d.ngy, G = (size_t)p.group_size;
+    HIP_CHECK(ctx, hipMemcpyAsync(counts, w.cnt, refs * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(ctx, hipMemcpyAsync(pos, w.pos, refs * G * sizeof(int), hipMemcpyDeviceToHost, s));
+    if (keys) HIP_CHECK(ctx, hipMemcpyAsync(keys, w.keys, refs * G * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(ctx, hipStreamSynchronize(s));
+    return ISP_BM3D_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,91 @@
+// Internal interface between the C ABI (bm3d_capi.cpp) and the HIP kernels
+// (bm3d_kernels.hip).  Not installed; no reference counterpart.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ispbm3d {
+
+constexpr int kBlock = 8;        // BlockSize implemented by the GPU kernels
+constexpr int kMaxGroup = 32;    // largest GroupSize implemented
+constexpr int kTileRX = 8;       // reference blocks per workgroup tile, x
+constexpr int kTileRY = 4;       // reference blocks per workgroup tile, y
+constexpr int kStrip = 5;        // candidates per lane strip in matching
+constexpr int kWaves = 4;        // waves per workgroup
+constexpr int kThreads = 64 * kWaves;
+
+// Positions are packed (y << 16) | x.
+struct MatchArgs {
+    int W, H;
+    int ngx, ngy;
+    int bstep;               // BlockStep: grid position k is min(k*bstep, last)
+    int range;               // BMrange rounded down to a multiple of BMstep
+    int step;                // BMstep
+    int G;                   // GroupSize
+    int bypass;              // 1: every group is the reference block alone
+    double thSSE;            // thMSE * normY * MSE2SSE
+    double distMul;          // 1 / MSE2SSE
+    int tileW, tileH, tilePitch;  // staged guide tile (doubles)
+    int maxCand;             // window size bound (per-wave candidate list)
+    int *cnt_out;            // [ngy*ngx]
+    int *pos_out;            // [ngy*ngx*G]
+    double *key_out;         // [ngy*ngx*G] or nullptr
+};
+
+struct FilterArgs {
+    int W, H, pitch;          // geometry of src/guide/num/den planes
+    int ngx, ngy;
+    int bstep;
+    int range;
+    int G;
+    const int *cnt;           // groups from matching
+    const int *pos;
+    const float *src[3];      // noisy planes
+    const float *guide[3];    // Wiener guide planes (final step)
+    float *num[3];            // accumulators, zeroed before launch
+    float *den[3];
+    int active[3];            // plane filtered?
+    float thr[3];             // lambda * sigma_c (orthonormal domain)
+    float sig2[3];            // sigma_c^2 (orthonormal domain)
+    int FW, FH;               // accumulation footprint of a tile
+    const float *c8;          // 8x8 orthonormal DCT-II matrix, [k][n]
+    const float *cn;          // per-N orthonormal DCT-II matrices, padded to NP x NP
+    int cn_off[kMaxGroup + 1];
+};
+
+struct FinalizeArgs {
+    int W, H, pitch;
+    const float *num[3];
+    const float *den[3];
+    const float *src[3];      // copied through for inactive planes
+    int active[3];
+    int nplanes;
+    int to_rgb;               // OPP -> RGB with clip
+    float *out[3];
+    size_t out_pitch;
+};
+
+struct OppArgs {
+    int W, H;
+    const float *r, *g, *b;
+    size_t in_pitch;
+    float *y, *u, *v;          // may be null
+    int pitch;
+    double *y64;               // may be null (matching guide)
+    int pitch64;
+};
+
+size_t match_lds_bytes(const MatchArgs &a);
+size_t filter_lds_bytes(const FilterArgs &a, bool wiener);
+
+hipError_t launch_match_f32(const float *guide, int gpitch, const MatchArgs &a, hipStream_t s);
+hipError_t launch_match_f64(const double *guide, int gpitch, const MatchArgs &a, hipStream_t s);
+hipError_t launch_filter(const FilterArgs &a, bool wiener, hipStream_t s);
+hipError_t launch_finalize(const FinalizeArgs &a, hipStream_t s);
+hipError_t launch_opp(const OppArgs &a, hipStream_t s);
+hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s);
+hipError_t launch_copy2d(float *dst, size_t dpitch, const float *src, size_t spitch, int W, int H,
+                         hipStream_t s);
+
+}  // namespace ispbm3d

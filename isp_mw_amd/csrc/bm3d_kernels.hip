@@ -1,0 +1,686 @@
+// HIP kernels of the MI355X BM3D denoiser (gfx950, wave64).
+//
+// Stages of one BM3D step (source/BM3D.cpp:80-218):
+//   match_kernel     - exhaustive block matching in the clipped search window
+//                      (include/Block.h:578-751) with FP64 distances, threshold,
+//                      and top-(G-1) selection by a wave-level radix select.
+//   filter_kernel    - gather, separable orthonormal 3D DCT, hard threshold or
+//                      Wiener shrinkage, inverse, weighted aggregation into an
+//                      LDS tile, flushed to HBM with float atomics
+//                      (source/BM3D.cpp:339-385, 411-457; include/Block.h:1240-1357).
+//   finalize_kernel  - num/den (source/BM3D.cpp:132-135, 204-217), optionally
+//                      fused with OPP->RGB and clip (include/Conversion.hpp:739-762).
+//   opp_kernel       - RGB->OPP without clip (include/Conversion.hpp:547-578).
+#include "bm3d_internal.h"
+
+namespace ispbm3d {
+namespace {
+
+constexpr int ZS = 64;  // z-stride of a group staged in LDS (floats)
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Number of lanes below this one whose bit is set in mask.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ float wave_sum(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__device__ __forceinline__ int wave_sum(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane)
+{
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(v, o);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Block::_SearchBoundary (include/Block.h:716-745); range already a multiple of step.
+__device__ __forceinline__ int search_boundary(int pos, int boundary, int range, int step)
+{
+    int sb;
+    if (pos == boundary) {
+        sb = boundary;
+    } else if (pos > boundary) {
+        sb = pos - range;
+        while (sb < boundary) sb += step;
+    } else {
+        sb = pos + range;
+        while (sb > boundary) sb -= step;
+    }
+    return sb;
+}
+
+// Reference grid (source/BM3D.cpp:98-127): 0, s, 2s, ... with any position
+// past n-B replaced by n-B, i.e. position k is min(k*s, n-B).
+__device__ __forceinline__ int grid_pos(int k, int s, int last) { return min(k * s, last); }
+
+__host__ __device__ constexpr size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
+
+__host__ __device__ inline size_t match_wave_bytes(int maxCand)
+{
+    // keys u64[maxCand] + selk u64[32] + idxs u32[maxCand] + memb u32[maxCand]
+    // + hist u32[256] + seli u32[32]
+    return align16((size_t)maxCand * 8 + 32 * 8 + (size_t)maxCand * 8 + 256 * 4 + 32 * 4);
+}
+
+// ---------------------------------------------------------------------------
+// Block matching.  One workgroup stages the guide around a tile of
+// kTileRX x kTileRY reference blocks in LDS as FP64; each wave takes one
+// reference block at a time.  Lane strips of kStrip horizontally adjacent
+// candidates share loaded rows.  Distances are summed in FP64 in the
+// reference's row-major order; accepted candidates (SSE <= thSSE, SSE != 0)
+// are compacted into a per-wave LDS list, and the G-1 smallest by
+// (key, scan index) are found with an MSB-first radix select on the key bits.
+template <typename GT, bool ST1>
+__global__ __launch_bounds__(kThreads) void match_kernel(const GT *__restrict__ guide, int gpitch,
+                                                         MatchArgs a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tx0 = blockIdx.x * kTileRX, ty0 = blockIdx.y * kTileRY;
+    const int ox = grid_pos(tx0, a.bstep, a.W - kBlock) - a.range;
+    const int oy = grid_pos(ty0, a.bstep, a.H - kBlock) - a.range;
+
+    double *tile = reinterpret_cast<double *>(smem);
+    unsigned char *wb = smem + align16((size_t)a.tileH * a.tilePitch * 8) +
+                        (size_t)wave * match_wave_bytes(a.maxCand);
+    uint64_t *keys = reinterpret_cast<uint64_t *>(wb);
+    uint64_t *selk = keys + a.maxCand;
+    uint32_t *idxs = reinterpret_cast<uint32_t *>(selk + 32);
+    uint32_t *memb = idxs + a.maxCand;
+    uint32_t *hist = memb + a.maxCand;
+    uint32_t *seli = hist + 256;
+
+    for (int k = tid; k < a.tileH * a.tileW; k += kThreads) {
+        const int ty = k / a.tileW, tx = k - ty * a.tileW;
+        const int yy = oy + ty, xx = ox + tx;
+        double v = 0.0;
+        if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) v = (double)guide[(size_t)yy * gpitch + xx];
+        tile[ty * a.tilePitch + tx] = v;
+    }
+    __syncthreads();
+
+    for (int r = wave; r < kTileRX * kTileRY; r += kWaves) {
+        const int gxi = tx0 + (r % kTileRX), gyi = ty0 + (r / kTileRX);
+        if (gxi >= a.ngx || gyi >= a.ngy) continue;
+        const int i = grid_pos(gxi, a.bstep, a.W - kBlock), j = grid_pos(gyi, a.bstep, a.H - kBlock);
+        const size_t refIdx = (size_t)gyi * a.ngx + gxi;
+        const size_t base = refIdx * a.G;
+        if (a.bypass) {
+            if (lane == 0) {
+                a.cnt_out[refIdx] = 1;
+                a.pos_out[base] = (j << 16) | i;
+                if (a.key_out) a.key_out[base] = 0.0;
+            }
+            continue;
+        }
+        const int st = a.step;
+        const int l = search_boundary(i, 0, a.range, st);
+        const int rr = search_boundary(i, a.W - kBlock, a.range, st);
+        const int t = search_boundary(j, 0, a.range, st);
+        const int b = search_boundary(j, a.H - kBlock, a.range, st);
+        const int nx = (rr - l) / st + 1, ny = (b - t) / st + 1;
+        const int selfx = (i - l) / st, selfy = (j - t) / st;
+        const int nsx = (nx + kStrip - 1) / kStrip, nstrips = nsx * ny;
+        const double *tref = tile + (j - oy) * a.tilePitch + (i - ox);
+
+        uint32_t cnt = 0;
+        for (int s0 = 0; s0 < nstrips; s0 += 64) {
+            const int sidx = s0 + lane;
+            const bool valid = sidx < nstrips;
+            const int ky = valid ? sidx / nsx : 0;
+            const int kx0 = valid ? (sidx - ky * nsx) * kStrip : 0;
+            double acc[kStrip];
+#pragma unroll
+            for (int c = 0; c < kStrip; ++c) acc[c] = 0.0;
+            if (valid) {
+                const double *tc = tile + (t - oy + ky * st) * a.tilePitch + (l - ox + kx0 * st);
+#pragma unroll
+                for (int u = 0; u < kBlock; ++u) {
+                    double rv[kBlock];
+#pragma unroll
+                    for (int v = 0; v < kBlock; ++v) rv[v] = tref[u * a.tilePitch + v];
+                    if constexpr (ST1) {
+                        double cv[kBlock + kStrip - 1];
+#pragma unroll
+                        for (int q = 0; q < kBlock + kStrip - 1; ++q) cv[q] = tc[u * a.tilePitch + q];
+#pragma unroll
+                        for (int c = 0; c < kStrip; ++c)
+#pragma unroll
+                            for (int v = 0; v < kBlock; ++v) {
+                                const double d = rv[v] - cv[c + v];
+                                acc[c] = __builtin_fma(d, d, acc[c]);
+                            }
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < kStrip; ++c)
+#pragma unroll
+                            for (int v = 0; v < kBlock; ++v) {
+                                const double d = rv[v] - tc[u * a.tilePitch + c * st + v];
+                                acc[c] = __builtin_fma(d, d, acc[c]);
+                            }
+                    }
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < kStrip; ++c) {
+                const int kx = kx0 + c;
+                const bool ok = valid && kx < nx && !(kx == selfx && ky == selfy) &&
+                                acc[c] <= a.thSSE && acc[c] != 0.0;
+                const uint64_t m = __ballot(ok);
+                if (ok) {
+                    const uint32_t slot = cnt + lanes_below(m);
+                    keys[slot] = (uint64_t)__double_as_longlong(acc[c] * a.distMul);
+                    idxs[slot] = (uint32_t)(ky * nx + kx);
+                }
+                cnt += (uint32_t)__popcll(m);
+            }
+        }
+        wave_sync();
+
+        // ---- selection of the K = G-1 smallest (key, scan index) ----
+        const uint32_t K = (uint32_t)(a.G - 1);
+        uint32_t nsel = 0;
+        if (cnt <= K) {
+            for (uint32_t e = lane; e < cnt; e += 64) {
+                selk[e] = keys[e];
+                seli[e] = idxs[e];
+            }
+            nsel = cnt;
+        } else {
+            uint64_t prefix = 0, pmask = 0;
+            uint32_t need = K;
+            bool whole = false;
+            for (int shift = 56;; shift -= 8) {
+                for (int q = lane; q < 256; q += 64) hist[q] = 0;
+                wave_sync();
+                for (uint32_t e = lane; e < cnt; e += 64) {
+                    const uint64_t k = keys[e];
+                    if ((k & pmask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
+                }
+                wave_sync();
+                const uint32_t h0 = hist[4 * lane], h1 = hist[4 * lane + 1];
+                const uint32_t h2 = hist[4 * lane + 2], h3 = hist[4 * lane + 3];
+                const uint32_t s = h0 + h1 + h2 + h3;
+                const uint32_t incl = wave_incl_scan(s, lane);
+                const uint32_t excl = incl - s;
+                const uint64_t bm = __ballot(excl < need && need <= incl);
+                const int L = __ffsll((unsigned long long)bm) - 1;
+                uint32_t dig = 0, blw = 0, mm = 0;
+                if (lane == L) {
+                    const uint32_t hh[4] = {h0, h1, h2, h3};
+                    uint32_t c = excl;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        if (mm == 0 && c + hh[q] >= need) {
+                            dig = 4 * lane + q;
+                            blw = c;
+                            mm = hh[q];
+                        }
+                        c += hh[q];
+                    }
+                }
+                dig = __shfl(dig, L);
+                blw = __shfl(blw, L);
+                mm = __shfl(mm, L);
+                need -= blw;
+                prefix |= (uint64_t)dig << shift;
+                pmask |= (uint64_t)0xFF << shift;
+                if (mm == need) {
+                    whole = true;
+                    break;
+                }
+                if (mm <= 64 || shift == 0) break;
+                wave_sync();
+            }
+            uint32_t nm = 0;
+            for (uint32_t e0 = 0; e0 < cnt; e0 += 64) {
+                const uint32_t e = e0 + lane;
+                const bool inr = e < cnt;
+                const uint64_t k = inr ? keys[e] : ~0ull;
+                const bool below = inr && (k & pmask) < prefix;
+                const bool inb = inr && (k & pmask) == prefix;
+                const bool take = below || (whole && inb);
+                const uint64_t tm = __ballot(take);
+                if (take) {
+                    const uint32_t sl = nsel + lanes_below(tm);
+                    selk[sl] = k;
+                    seli[sl] = idxs[e];
+                }
+                nsel += (uint32_t)__popcll(tm);
+                if (!whole) {
+                    const uint64_t mb = __ballot(inb);
+                    if (inb) memb[nm + lanes_below(mb)] = e;
+                    nm += (uint32_t)__popcll(mb);
+                }
+            }
+            if (!whole) {
+                wave_sync();
+                for (uint32_t q0 = 0; q0 < nm; q0 += 64) {
+                    const uint32_t q = q0 + lane;
+                    const bool v = q < nm;
+                    uint64_t kq = 0;
+                    uint32_t iq = 0, rank = 0;
+                    if (v) {
+                        const uint32_t e = memb[q];
+                        kq = keys[e];
+                        iq = idxs[e];
+                        for (uint32_t r2 = 0; r2 < nm; ++r2) {
+                            const uint32_t e2 = memb[r2];
+                            const uint64_t k2 = keys[e2];
+                            const uint32_t i2 = idxs[e2];
+                            rank += (k2 < kq) || (k2 == kq && i2 < iq);
+                        }
+                    }
+                    const bool take = v && rank < need;
+                    const uint64_t tm = __ballot(take);
+                    if (take) {
+                        const uint32_t sl = nsel + lanes_below(tm);
+                        selk[sl] = kq;
+                        seli[sl] = iq;
+                    }
+                    nsel += (uint32_t)__popcll(tm);
+                }
+            }
+        }
+        wave_sync();
+
+        // ---- order the selection and write the group ----
+        if (lane == 0) {
+            a.cnt_out[refIdx] = (int)(1 + nsel);
+            a.pos_out[base] = (j << 16) | i;
+            if (a.key_out) a.key_out[base] = 0.0;
+        }
+        if ((uint32_t)lane < nsel) {
+            const uint64_t kq = selk[lane];
+            const uint32_t iq = seli[lane];
+            uint32_t rank = 0;
+            for (uint32_t q = 0; q < nsel; ++q) {
+                const uint64_t k2 = selk[q];
+                rank += (k2 < kq) || (k2 == kq && seli[q] < iq);
+            }
+            const int ky = (int)(iq / (uint32_t)nx), kx = (int)(iq - (uint32_t)ky * nx);
+            const int y = t + ky * st, x = l + kx * st;
+            a.pos_out[base + 1 + rank] = (y << 16) | x;
+            if (a.key_out) a.key_out[base + 1 + rank] = __longlong_as_double((long long)kq);
+        }
+        wave_sync();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Collaborative filtering, FP32.  The unnormalised FFTW transforms of the
+// reference are replaced by orthonormal DCT-II matrices; with them the
+// reference's thresholds become a uniform lambda*sigma and its Wiener noise
+// sigma^2*finalAMP becomes sigma^2/2^zeta (zeta = number of zero indices), and
+// the reconstruction needs no finalAMP division.
+__device__ __forceinline__ void rows_pass(float *g, int N, const float *__restrict__ c8, int lane,
+                                          bool inverse)
+{
+    for (int r = lane; r < N * kBlock; r += 64) {
+        float *p = g + (r >> 3) * ZS + (r & 7) * kBlock;
+        float in[kBlock];
+#pragma unroll
+        for (int v = 0; v < kBlock; ++v) in[v] = p[v];
+#pragma unroll
+        for (int k = 0; k < kBlock; ++k) {
+            float s = 0.f;
+#pragma unroll
+            for (int v = 0; v < kBlock; ++v)
+                s = fmaf(inverse ? c8[v * kBlock + k] : c8[k * kBlock + v], in[v], s);
+            p[k] = s;
+        }
+    }
+}
+
+__device__ __forceinline__ void cols_pass(float *g, int N, const float *__restrict__ c8, int lane,
+                                          bool inverse)
+{
+    for (int r = lane; r < N * kBlock; r += 64) {
+        float *p = g + (r >> 3) * ZS + (r & 7);
+        float in[kBlock];
+#pragma unroll
+        for (int u = 0; u < kBlock; ++u) in[u] = p[u * kBlock];
+#pragma unroll
+        for (int k = 0; k < kBlock; ++k) {
+            float s = 0.f;
+#pragma unroll
+            for (int u = 0; u < kBlock; ++u)
+                s = fmaf(inverse ? c8[u * kBlock + k] : c8[k * kBlock + u], in[u], s);
+            p[k * kBlock] = s;
+        }
+    }
+}
+
+// Group-axis DCT of the lane's pixel column, in place in LDS: the column is
+// held in registers and each output row is written back as soon as it is
+// formed.  inverse = 1 applies the transpose.
+template <int NP>
+__device__ __forceinline__ void zpass(float *g, int N, int lane, const float *__restrict__ cn, bool inverse)
+{
+    float x[NP];
+#pragma unroll
+    for (int z = 0; z < NP; ++z) x[z] = z < N ? g[z * ZS + lane] : 0.f;
+#pragma unroll 1
+    for (int k = 0; k < N; ++k) {
+        float s = 0.f;
+#pragma unroll
+        for (int z = 0; z < NP; ++z) s = fmaf(inverse ? cn[z * NP + k] : cn[k * NP + z], x[z], s);
+        g[k * ZS + lane] = s;
+    }
+}
+
+template <int NP, bool WIENER>
+__device__ __forceinline__ float group_core(float *g, float *gr, int N, int lane,
+                                            const float *__restrict__ c8,
+                                            const float *__restrict__ cn, float thr, float sig2)
+{
+    rows_pass(g, N, c8, lane, false);
+    if (WIENER) rows_pass(gr, N, c8, lane, false);
+    wave_sync();
+    cols_pass(g, N, c8, lane, false);
+    if (WIENER) cols_pass(gr, N, c8, lane, false);
+    wave_sync();
+    zpass<NP>(g, N, lane, cn, false);
+    if (WIENER) zpass<NP>(gr, N, lane, cn, false);
+
+    // shrinkage on the lane's own coefficients (no cross-lane dependency)
+    float weight;
+    if (!WIENER) {
+        int kept = 0;
+        for (int k = 0; k < N; ++k) {
+            const float y = g[k * ZS + lane];
+            if (fabsf(y) <= thr) g[k * ZS + lane] = 0.f;
+            else ++kept;
+        }
+        kept = wave_sum(kept);
+        weight = kept < 1 ? 1.f : 1.f / (float)kept;
+    } else {
+        const int u = lane >> 3, v = lane & 7;
+        const float s2 = sig2 * (u == 0 ? 0.5f : 1.f) * (v == 0 ? 0.5f : 1.f);
+        float l2 = 0.f;
+        for (int k = 0; k < N; ++k) {
+            const float yr = gr[k * ZS + lane];
+            const float yy = yr * yr;
+            const float wc = yy / (yy + (k == 0 ? 0.5f * s2 : s2));
+            g[k * ZS + lane] *= wc;
+            l2 = fmaf(wc, wc, l2);
+        }
+        l2 = wave_sum(l2);
+        weight = l2 <= 0.f ? 1.f : 1.f / l2;
+    }
+    zpass<NP>(g, N, lane, cn, true);
+    wave_sync();
+    cols_pass(g, N, c8, lane, true);
+    wave_sync();
+    rows_pass(g, N, c8, lane, true);
+    wave_sync();
+    return weight;
+}
+
+__host__ __device__ inline size_t filter_lds(int FW, int FH, int G, bool wiener)
+{
+    const int DW = FW - kBlock + 1, DH = FH - kBlock + 1;
+    size_t fl = (size_t)FW * FH + (size_t)DW * DH + (size_t)DH * FW;
+    fl = (fl + 3) & ~size_t(3);
+    return fl * 4 + (size_t)kWaves * (wiener ? 2 : 1) * G * ZS * 4;
+}
+
+template <bool WIENER>
+__global__ __launch_bounds__(kThreads) void filter_kernel(FilterArgs a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tx0 = blockIdx.x * kTileRX, ty0 = blockIdx.y * kTileRY;
+    const int ox = grid_pos(tx0, a.bstep, a.W - kBlock) - a.range;
+    const int oy = grid_pos(ty0, a.bstep, a.H - kBlock) - a.range;
+    const int FW = a.FW, FH = a.FH, DW = FW - kBlock + 1, DH = FH - kBlock + 1;
+
+    float *accNum = reinterpret_cast<float *>(smem);
+    float *accDen = accNum + FW * FH;
+    float *hsum = accDen + DW * DH;
+    size_t fl = (size_t)FW * FH + (size_t)DW * DH + (size_t)DH * FW;
+    fl = (fl + 3) & ~size_t(3);
+    float *g = accNum + fl + (size_t)wave * (WIENER ? 2 : 1) * a.G * ZS;
+    float *gr = g + a.G * ZS;
+    const int u = lane >> 3, v = lane & 7;
+
+    for (int c = 0; c < 3; ++c) {
+        if (!a.active[c]) continue;
+        for (int k = tid; k < FW * FH; k += kThreads) accNum[k] = 0.f;
+        for (int k = tid; k < DW * DH; k += kThreads) accDen[k] = 0.f;
+        __syncthreads();
+        const float *src = a.src[c];
+        const float *gsrc = a.guide[c];
+        for (int r = wave; r < kTileRX * kTileRY; r += kWaves) {
+            const int gxi = tx0 + (r % kTileRX), gyi = ty0 + (r / kTileRX);
+            if (gxi >= a.ngx || gyi >= a.ngy) continue;
+            const size_t refIdx = (size_t)gyi * a.ngx + gxi;
+            const int *pos = a.pos + refIdx * a.G;
+            const int N = a.cnt[refIdx];
+            for (int z = 0; z < N; ++z) {
+                const int pz = pos[z];
+                const size_t off = (size_t)((pz >> 16) + u) * a.pitch + (pz & 0xFFFF) + v;
+                g[z * ZS + lane] = src[off];
+                if (WIENER) gr[z * ZS + lane] = gsrc[off];
+            }
+            wave_sync();
+            const float thr = a.thr[c], sig2 = a.sig2[c];
+            float w;
+            if (N <= 1) w = group_core<1, WIENER>(g, gr, N, lane, a.c8, a.cn + a.cn_off[N], thr, sig2);
+            else if (N <= 2) w = group_core<2, WIENER>(g, gr, N, lane, a.c8, a.cn + a.cn_off[N], thr, sig2);
+            else if (N <= 4) w = group_core<4, WIENER>(g, gr, N, lane, a.c8, a.cn + a.cn_off[N], thr, sig2);
+            else if (N <= 8) w = group_core<8, WIENER>(g, gr, N, lane, a.c8, a.cn + a.cn_off[N], thr, sig2);
+            else if (N <= 16) w = group_core<16, WIENER>(g, gr, N, lane, a.c8, a.cn + a.cn_off[N], thr, sig2);
+            else w = group_core<32, WIENER>(g, gr, N, lane, a.c8, a.cn + a.cn_off[N], thr, sig2);
+            for (int z = 0; z < N; ++z) {
+                const int pz = pos[z];
+                const int py = (pz >> 16) - oy, px = (pz & 0xFFFF) - ox;
+                atomicAdd(&accNum[(py + u) * FW + px + v], w * g[z * ZS + lane]);
+            }
+            if (lane < N) {
+                const int pz = pos[lane];
+                atomicAdd(&accDen[((pz >> 16) - oy) * DW + (pz & 0xFFFF) - ox], w);
+            }
+            wave_sync();
+        }
+        __syncthreads();
+        // den(pixel) = sum of the weights of blocks whose origin lies in the
+        // 8x8 window above-left of it: row box sums, then column box sums.
+        for (int k = tid; k < DH * FW; k += kThreads) {
+            const int dy = k / FW, px = k - dy * FW;
+            float s = 0.f;
+#pragma unroll
+            for (int bq = 0; bq < kBlock; ++bq) {
+                const int q = px - bq;
+                if (q >= 0 && q < DW) s += accDen[dy * DW + q];
+            }
+            hsum[k] = s;
+        }
+        __syncthreads();
+        float *num = a.num[c], *den = a.den[c];
+        for (int k = tid; k < FW * FH; k += kThreads) {
+            const int py = k / FW, px = k - py * FW;
+            const int yy = oy + py, xx = ox + px;
+            if (yy < 0 || yy >= a.H || xx < 0 || xx >= a.W) continue;
+            float d = 0.f;
+#pragma unroll
+            for (int aq = 0; aq < kBlock; ++aq) {
+                const int q = py - aq;
+                if (q >= 0 && q < DH) d += hsum[q * FW + px];
+            }
+            if (d != 0.f) {
+                const size_t o = (size_t)yy * a.pitch + xx;
+                unsafeAtomicAdd(&num[o], accNum[k]);
+                unsafeAtomicAdd(&den[o], d);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+__global__ void finalize_kernel(FinalizeArgs a)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= a.W) return;
+    const size_t o = (size_t)y * a.pitch + x;
+    float val[3] = {0.f, 0.f, 0.f};
+    for (int c = 0; c < a.nplanes; ++c) val[c] = a.active[c] ? a.num[c][o] / a.den[c][o] : a.src[c][o];
+    const size_t oo = (size_t)y * a.out_pitch + x;
+    if (a.to_rgb) {
+        // MatrixConvert_YUV2RGB, OPP, float planes (include/Conversion.hpp:739-762)
+        const double Y = val[0], U = val[1], V = val[2];
+        const double Rv = (double)(2.0L / 3.0L), Gv = (double)(-4.0L / 3.0L), Bv = Rv;
+        double r = 1.0 * Y + 1.0 * U + Rv * V + 0.0;
+        double g = 1.0 * Y + Gv * V + 0.0;
+        double b = 1.0 * Y + -1.0 * U + Bv * V + 0.0;
+        r = r < 0.0 ? 0.0 : (r > 1.0 ? 1.0 : r);
+        g = g < 0.0 ? 0.0 : (g > 1.0 ? 1.0 : g);
+        b = b < 0.0 ? 0.0 : (b > 1.0 ? 1.0 : b);
+        a.out[0][oo] = (float)r;
+        a.out[1][oo] = (float)g;
+        a.out[2][oo] = (float)b;
+    } else {
+        for (int c = 0; c < a.nplanes; ++c) a.out[c][oo] = val[c];
+    }
+}
+
+// MatrixConvert_RGB2YUV / ConvertToY, OPP, float planes with Floor 0 and
+// Ceil 1 (include/Conversion.hpp:418-432, 547-578), in FP64.
+__global__ void opp_kernel(OppArgs a)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= a.W) return;
+    const size_t i = (size_t)y * a.in_pitch + x;
+    const double R = a.r[i], G = a.g[i], B = a.b[i];
+    const double gainY = 1.0 / (1.0 * 3.0), gainU = 1.0 / (1.0 * 2.0), gainV = 1.0 / (1.0 * 4.0);
+    const double Y = (R + G + B) * gainY + 0.0;
+    const size_t o = (size_t)y * a.pitch + x;
+    if (a.y64) a.y64[(size_t)y * a.pitch64 + x] = Y;
+    if (a.y) a.y[o] = (float)Y;
+    if (a.u) a.u[o] = (float)((R - B) * gainU + 0.0);
+    if (a.v) a.v[o] = (float)((R - G * 2.0 + B) * gainV + 0.0);
+}
+
+__global__ void fill_kernel(float *p, size_t n, float v)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void copy2d_kernel(float *dst, size_t dpitch, const float *src, size_t spitch, int W)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x < W) dst[(size_t)y * dpitch + x] = src[(size_t)y * spitch + x];
+}
+
+template <typename K>
+hipError_t set_lds(K kernel, size_t bytes)
+{
+    if (bytes <= 65536) return hipSuccess;
+    return hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+template <typename GT>
+hipError_t launch_match_t(const GT *guide, int gpitch, const MatchArgs &a, hipStream_t s)
+{
+    const size_t lds = match_lds_bytes(a);
+    const dim3 grid((a.ngx + kTileRX - 1) / kTileRX, (a.ngy + kTileRY - 1) / kTileRY);
+    hipError_t e;
+    if (a.step == 1) {
+        if ((e = set_lds(match_kernel<GT, true>, lds)) != hipSuccess) return e;
+        hipLaunchKernelGGL((match_kernel<GT, true>), grid, dim3(kThreads), lds, s, guide, gpitch, a);
+    } else {
+        if ((e = set_lds(match_kernel<GT, false>, lds)) != hipSuccess) return e;
+        hipLaunchKernelGGL((match_kernel<GT, false>), grid, dim3(kThreads), lds, s, guide, gpitch, a);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+size_t match_lds_bytes(const MatchArgs &a)
+{
+    return align16((size_t)a.tileH * a.tilePitch * 8) + (size_t)kWaves * match_wave_bytes(a.maxCand);
+}
+
+size_t filter_lds_bytes(const FilterArgs &a, bool wiener) { return filter_lds(a.FW, a.FH, a.G, wiener); }
+
+hipError_t launch_match_f32(const float *guide, int gpitch, const MatchArgs &a, hipStream_t s)
+{
+    return launch_match_t(guide, gpitch, a, s);
+}
+
+hipError_t launch_match_f64(const double *guide, int gpitch, const MatchArgs &a, hipStream_t s)
+{
+    return launch_match_t(guide, gpitch, a, s);
+}
+
+hipError_t launch_filter(const FilterArgs &a, bool wiener, hipStream_t s)
+{
+    const size_t lds = filter_lds_bytes(a, wiener);
+    const dim3 grid((a.ngx + kTileRX - 1) / kTileRX, (a.ngy + kTileRY - 1) / kTileRY);
+    hipError_t e;
+    if (wiener) {
+        if ((e = set_lds(filter_kernel<true>, lds)) != hipSuccess) return e;
+        hipLaunchKernelGGL(filter_kernel<true>, grid, dim3(kThreads), lds, s, a);
+    } else {
+        if ((e = set_lds(filter_kernel<false>, lds)) != hipSuccess) return e;
+        hipLaunchKernelGGL(filter_kernel<false>, grid, dim3(kThreads), lds, s, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize(const FinalizeArgs &a, hipStream_t s)
+{
+    hipLaunchKernelGGL(finalize_kernel, dim3((a.W + 255) / 256, a.H), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_opp(const OppArgs &a, hipStream_t s)
+{
+    hipLaunchKernelGGL(opp_kernel, dim3((a.W + 255) / 256, a.H), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy2d(float *dst, size_t dpitch, const float *src, size_t spitch, int W, int H,
+                         hipStream_t s)
+{
+    hipLaunchKernelGGL(copy2d_kernel, dim3((W + 255) / 256, H), dim3(256), 0, s, dst, dpitch, src, spitch, W);
+    return hipGetLastError();
+}
+
+}  // namespace ispbm3d

@@ -1,0 +1,638 @@
+/*
+ * FP64 CPU restatement of ISP_MW's BM3D filter (the oracle).
+ *
+ * TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py as the checker and the timed CPU baseline.  The
+ * product library never links or calls this code.
+ *
+ * The reference itself cannot be compiled here: it needs FFTW3 (absent) and
+ * MSVC-only headers and constructs.  This file restates its algorithm in C, in
+ * the reference's loop order and in double precision (FLType = double,
+ * include/Type.h:26-30).  The FFTW plans of source/BM3D.cpp:13-24 are replaced
+ * by the transforms FFTW documents for REDFT10 / REDFT01, computed here with a
+ * radix-2 fast DCT for power-of-two lengths and dense cosine sums otherwise.
+ *
+ * Deviations from the reference, each forced or documented:
+ *  - BM3D::process_Plane_FL passes a 0x0 intermediate plane to basic.Kernel
+ *    (source/BM3D.cpp:466-468), which the reference then overruns; here the
+ *    intermediate is sized like src (the evident intent).
+ *  - When more than GroupSize candidates pass the threshold the reference calls
+ *    std::partial_sort (include/Block.h:690-695), whose order among equal keys
+ *    is unspecified; here equal keys keep scan order (the stable outcome).
+ *  - A plane that the 3-plane Kernel does not filter (sigma <= 0) is left
+ *    unwritten by the reference (source/BM3D.cpp:200-217); here it is copied
+ *    from src, which is what the two-step Frame path then sees.
+ */
+#include "bm3d_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_MAXN 64
+
+static int g_threads = 1;
+
+int orc_set_threads(int n)
+{
+    int old = g_threads;
+    g_threads = n < 1 ? 1 : n;
+    return old;
+}
+
+/* ------------------------------------------------------------------------ */
+/* 1D transforms.  M_n[k][j] = cos(pi (2j+1) k / (2n)).
+ * dct2_sum: y = M x            (REDFT10 = 2 * dct2_sum)
+ * dct3_sum: x = M^T y          (REDFT01(X) = dct3_sum(diag(1,2,2,..) X))      */
+
+static double g_inv2cos[ORC_MAXN + 1][ORC_MAXN / 2]; /* 1 / (2 cos(pi(2j+1)/(2n))) */
+static double g_dense[ORC_MAXN + 1][ORC_MAXN * ORC_MAXN];
+static int g_tables_ready = 0;
+
+static void init_tables(void)
+{
+    if (g_tables_ready) return;
+    for (int n = 1; n <= ORC_MAXN; ++n) {
+        for (int j = 0; j < n / 2; ++j)
+            g_inv2cos[n][j] = (double)(1.0L / (2.0L * cosl(3.14159265358979323846264338327950288L * (2 * j + 1) / (2.0L * n))));
+        for (int k = 0; k < n; ++k)
+            for (int j = 0; j < n; ++j)
+                g_dense[n][k * n + j] = (double)cosl(3.14159265358979323846264338327950288L * (long double)((2 * j + 1) * k) / (2.0L * n));
+    }
+    g_tables_ready = 1;
+}
+
+static int is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+/* Lee's radix-2 factorisation of the DCT-II sum. */
+static void lee_dct2(double *x, int n, double *tmp)
+{
+    if (n == 1) return;
+    int h = n / 2;
+    double *a = tmp, *b = tmp + h;
+    for (int j = 0; j < h; ++j) {
+        a[j] = x[j] + x[n - 1 - j];
+        b[j] = (x[j] - x[n - 1 - j]) * g_inv2cos[n][j];
+    }
+    lee_dct2(a, h, tmp + n);
+    lee_dct2(b, h, tmp + n);
+    for (int k = 0; k < h; ++k) x[2 * k] = a[k];
+    for (int k = 0; k < h - 1; ++k) x[2 * k + 1] = b[k] + b[k + 1];
+    x[n - 1] = b[h - 1];
+}
+
+/* Transpose of lee_dct2. */
+static void lee_dct3(double *y, int n, double *tmp)
+{
+    if (n == 1) return;
+    int h = n / 2;
+    double *v = tmp, *u = tmp + h;
+    for (int k = 0; k < h; ++k) {
+        v[k] = y[2 * k];
+        u[k] = y[2 * k + 1] + (k > 0 ? y[2 * k - 1] : 0.0);
+    }
+    lee_dct3(v, h, tmp + n);
+    lee_dct3(u, h, tmp + n);
+    for (int j = 0; j < h; ++j) {
+        double t = u[j] * g_inv2cos[n][j];
+        y[j] = v[j] + t;
+        y[n - 1 - j] = v[j] - t;
+    }
+}
+
+/* REDFT10 (inverse = 0) or REDFT01 (inverse = 1) of one contiguous vector. */
+static void redft_1d(double *x, int n, int inverse, double *tmp)
+{
+    if (inverse)
+        for (int k = 1; k < n; ++k) x[k] *= 2.0;
+    if (is_pow2(n)) {
+        if (inverse) lee_dct3(x, n, tmp);
+        else lee_dct2(x, n, tmp);
+    } else {
+        const double *m = g_dense[n];
+        double *out = tmp;
+        for (int a = 0; a < n; ++a) {
+            double s = 0.0;
+            if (inverse)
+                for (int k = 0; k < n; ++k) s += m[k * n + a] * x[k];
+            else
+                for (int j = 0; j < n; ++j) s += m[a * n + j] * x[j];
+            out[a] = s;
+        }
+        memcpy(x, out, sizeof(double) * (size_t)n);
+    }
+    if (!inverse)
+        for (int k = 0; k < n; ++k) x[k] *= 2.0;
+}
+
+void orc_dct3d(double *data, int n0, int n1, int n2, int inverse)
+{
+    init_tables();
+    double vec[ORC_MAXN], tmp[4 * ORC_MAXN];
+    /* axis 2 (contiguous) */
+    for (int a = 0; a < n0 * n1; ++a) redft_1d(data + (size_t)a * n2, n2, inverse, tmp);
+    /* axis 1 */
+    for (int a = 0; a < n0; ++a)
+        for (int c = 0; c < n2; ++c) {
+            double *base = data + (size_t)a * n1 * n2 + c;
+            for (int b = 0; b < n1; ++b) vec[b] = base[(size_t)b * n2];
+            redft_1d(vec, n1, inverse, tmp);
+            for (int b = 0; b < n1; ++b) base[(size_t)b * n2] = vec[b];
+        }
+    /* axis 0 */
+    for (int bc = 0; bc < n1 * n2; ++bc) {
+        double *base = data + bc;
+        for (int a = 0; a < n0; ++a) vec[a] = base[(size_t)a * n1 * n2];
+        redft_1d(vec, n0, inverse, tmp);
+        for (int a = 0; a < n0; ++a) base[(size_t)a * n1 * n2] = vec[a];
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Parameters (include/BM3D.h:237-259; source/BM3D.cpp:9-73).              */
+
+static void opp_norms(double *normY, double *normU, double *normV)
+{
+    /* ColorMatrix_RGB2YUV_Parameter, OPP branch (include/Specification.h:391-407) */
+    double Yr = (double)(1.0L / 3.0L), Yg = (double)(1.0L / 3.0L), Yb = (double)(1.0L / 3.0L);
+    double Ur = (double)(1.0L / 2.0L), Ug = 0.0, Ub = (double)(-1.0L / 2.0L);
+    double Vr = (double)(1.0L / 4.0L), Vg = (double)(-1.0L / 2.0L), Vb = (double)(1.0L / 4.0L);
+    *normY = sqrt(Yr * Yr + Yg * Yg + Yb * Yb);
+    *normU = sqrt(Ur * Ur + Ug * Ug + Ub * Ub);
+    *normV = sqrt(Vr * Vr + Vg * Vg + Vb * Vb);
+}
+
+typedef struct {
+    orc_step_params p;   /* with thMSE scaled by normY */
+    double sigma[3];     /* sigma[c]/255*norm_c */
+    int wiener;
+    double finalAMP[ORC_MAXN];
+    double wienerSigmaSqr[3][ORC_MAXN];
+    double *thr[3][ORC_MAXN]; /* thrTable[c][n-1], n*B*B entries */
+} filter_data;
+
+static void filter_data_init(filter_data *f, const orc_step_params *p, int wiener)
+{
+    double norm[3];
+    opp_norms(&norm[0], &norm[1], &norm[2]);
+    memset(f, 0, sizeof(*f));
+    f->p = *p;
+    f->p.thMSE = p->thMSE * norm[0];
+    f->wiener = wiener;
+    int G = p->group_size, B = p->block_size;
+    for (int c = 0; c < 3; ++c) f->sigma[c] = p->sigma[c] / (double)255 * norm[c];
+    for (int i = 1; i <= G && i <= ORC_MAXN; ++i) {
+        f->finalAMP[i - 1] = 2 * i * 2 * B * 2 * B;
+        double forwardAMP = sqrt(f->finalAMP[i - 1]);
+        for (int c = 0; c < 3; ++c) {
+            if (!(p->sigma[c] > 0)) continue;
+            double sigma = f->sigma[c];
+            if (wiener) {
+                f->wienerSigmaSqr[c][i - 1] = sigma * forwardAMP * sigma * forwardAMP;
+            } else {
+                double thrBase = sigma * p->lambda * forwardAMP;
+                double thr[4] = {thrBase, thrBase * sqrt(2.0), thrBase * 2.0, thrBase * sqrt(8.0)};
+                double *t = (double *)malloc(sizeof(double) * (size_t)i * B * B);
+                double *d = t;
+                for (int z = 0; z < i; ++z)
+                    for (int y = 0; y < B; ++y)
+                        for (int x = 0; x < B; ++x, ++d) *d = thr[(x == 0) + (y == 0) + (z == 0)];
+                f->thr[c][i - 1] = t;
+            }
+        }
+    }
+}
+
+static void filter_data_free(filter_data *f)
+{
+    for (int c = 0; c < 3; ++c)
+        for (int i = 0; i < ORC_MAXN; ++i) free(f->thr[c][i]);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Grid and search window.                                                  */
+
+int orc_grid(int n, int block_size, int block_step, int *out, int cap)
+{
+    if (block_step <= 0) return -1;
+    int last = n - block_size, cnt = 0;
+    for (int j = 0;; j += block_step) {
+        if (j >= last + block_step) break;
+        else if (j > last) j = last;
+        if (cnt < cap && out) out[cnt] = j;
+        ++cnt;
+    }
+    return cnt;
+}
+
+int orc_search_boundary(int pos, int plane_boundary, int search_range, int search_step)
+{
+    int sb;
+    search_range = search_range / search_step * search_step;
+    if (pos == plane_boundary) {
+        sb = plane_boundary;
+    } else if (pos > plane_boundary) {
+        sb = pos - search_range;
+        while (sb < plane_boundary) sb += search_step;
+    } else {
+        sb = pos + search_range;
+        while (sb > plane_boundary) sb -= search_step;
+    }
+    return sb;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Block matching (include/Block.h:578-711).                                */
+
+typedef struct {
+    double key;
+    int seq;
+    int y, x;
+} cand;
+
+static int cand_cmp(const void *a, const void *b)
+{
+    const cand *p = (const cand *)a, *q = (const cand *)b;
+    if (p->key < q->key) return -1;
+    if (p->key > q->key) return 1;
+    return (p->seq > q->seq) - (p->seq < q->seq);
+}
+
+/* Returns N; writes N positions and keys.  buf holds at least window size + 1. */
+static int match_one(const filter_data *f, const double *ref, int w, int h, int stride,
+                     int j, int i, cand *buf, int32_t *pos_out, double *key_out)
+{
+    const orc_step_params *p = &f->p;
+    const int B = p->block_size;
+    int n = 0;
+    if (p->group_size == 1 || p->thMSE <= 0) {
+        pos_out[0] = (j << 16) | i;
+        if (key_out) key_out[0] = 0.0;
+        return 1;
+    }
+    double MSE2SSE = (double)(B * B) * 1.0 * 1.0 / (double)(255 * 255);
+    double distMul = 1.0 / MSE2SSE;
+    double thSSE = p->thMSE * MSE2SSE;
+
+    int step = p->bm_step;
+    int range = p->bm_range / step * step;
+    int l = orc_search_boundary(i, 0, range, step);
+    int r = orc_search_boundary(i, w - B, range, step);
+    int t = orc_search_boundary(j, 0, range, step);
+    int b = orc_search_boundary(j, h - B, range, step);
+
+    const double *rb = ref + (size_t)j * stride + i;
+    buf[n].key = 0.0; buf[n].seq = -1; buf[n].y = j; buf[n].x = i; ++n;
+    int seq = 0;
+    for (int y = t; y <= b; y += step) {
+        for (int x = l; x <= r; x += step) {
+            if (y == j && x == i) continue;
+            const double *cb = ref + (size_t)y * stride + x;
+            double dist = 0.0;
+            for (int u = 0; u < B; ++u) {
+                const double *rr = rb + (size_t)u * stride, *cc = cb + (size_t)u * stride;
+                for (int v = 0; v < B; ++v) {
+                    double temp = rr[v] - cc[v];
+                    dist += temp * temp;
+                }
+            }
+            if (dist <= thSSE && dist != 0) {
+                buf[n].key = dist * distMul; buf[n].seq = seq; buf[n].y = y; buf[n].x = x; ++n;
+            }
+            ++seq;
+        }
+    }
+    qsort(buf, (size_t)n, sizeof(cand), cand_cmp);
+    int G = p->group_size;
+    if (G > 0 && n > G) n = G;
+    for (int z = 0; z < n; ++z) {
+        pos_out[z] = (buf[z].y << 16) | buf[z].x;
+        if (key_out) key_out[z] = buf[z].key;
+    }
+    return n;
+}
+
+static int window_cap(const orc_step_params *p)
+{
+    int step = p->bm_step > 0 ? p->bm_step : 1;
+    int side = 2 * (p->bm_range / step) + 1;
+    return side * side + 1;
+}
+
+int orc_match_all(const orc_step_params *p, const double *ref, int w, int h, int stride,
+                  int *counts, int32_t *pos, double *keys)
+{
+    if (p->block_size <= 0 || w < p->block_size || h < p->block_size || p->bm_step <= 0 ||
+        p->group_size <= 0 || p->group_size > ORC_MAXN)
+        return -1;
+    filter_data f;
+    filter_data_init(&f, p, 0);
+    int ngx = orc_grid(w, p->block_size, p->block_step, NULL, 0);
+    int ngy = orc_grid(h, p->block_size, p->block_step, NULL, 0);
+    if (ngx < 0 || ngy < 0) { filter_data_free(&f); return -1; }
+    int *gx = (int *)malloc(sizeof(int) * (size_t)ngx), *gy = (int *)malloc(sizeof(int) * (size_t)ngy);
+    orc_grid(w, p->block_size, p->block_step, gx, ngx);
+    orc_grid(h, p->block_size, p->block_step, gy, ngy);
+    int G = p->group_size;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(g_threads)
+    for (int a = 0; a < ngy; ++a) {
+        cand *buf = (cand *)malloc(sizeof(cand) * (size_t)window_cap(p));
+        for (int c = 0; c < ngx; ++c) {
+            size_t g = (size_t)a * ngx + c;
+            counts[g] = match_one(&f, ref, w, h, stride, gy[a], gx[c], buf, pos + g * G,
+                                  keys ? keys + g * G : NULL);
+        }
+        free(buf);
+    }
+    free(gx); free(gy);
+    filter_data_free(&f);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Collaborative filtering (source/BM3D.cpp:339-385 and 411-457).           */
+
+static void gather(double *grp, const double *src, int stride, const int32_t *pos, int N, int B)
+{
+    double *d = grp;
+    for (int z = 0; z < N; ++z) {
+        int y = pos[z] >> 16, x = pos[z] & 0xFFFF;
+        const double *s = src + (size_t)y * stride + x;
+        for (int u = 0; u < B; ++u)
+            for (int v = 0; v < B; ++v) *d++ = s[(size_t)u * stride + v];
+    }
+}
+
+/* Accumulates into num/den, whose row 0 is image row `row0`. */
+static void collaborative_filter(const filter_data *f, int c, double *num, double *den, int row0,
+                                 int stride, const double *src, const double *ref,
+                                 const int32_t *pos, int N, double *grp, double *gref)
+{
+    const int B = f->p.block_size;
+    const int cnt = N * B * B;
+    gather(grp, src, stride, pos, N, B);
+    orc_dct3d(grp, N, B, B, 0);
+    double denWeight;
+    if (!f->wiener) {
+        const double *thr = f->thr[c][N - 1];
+        int retained = 0;
+        for (int k = 0; k < cnt; ++k) {
+            if (fabs(grp[k]) <= thr[k]) grp[k] = 0;
+            else ++retained;
+        }
+        denWeight = retained < 1 ? 1.0 : 1.0 / (double)retained;
+    } else {
+        gather(gref, ref, stride, pos, N, B);
+        orc_dct3d(gref, N, B, B, 0);
+        const double sigmaSquare = f->wienerSigmaSqr[c][N - 1];
+        double L2 = 0;
+        for (int k = 0; k < cnt; ++k) {
+            double ySquare = gref[k] * gref[k];
+            double wc = ySquare / (ySquare + sigmaSquare);
+            grp[k] *= wc;
+            L2 += wc * wc;
+        }
+        denWeight = L2 <= 0 ? 1.0 : 1.0 / L2;
+    }
+    orc_dct3d(grp, N, B, B, 1);
+    double numWeight = denWeight / f->finalAMP[N - 1];
+    const double *s = grp;
+    for (int z = 0; z < N; ++z) {
+        int y = (pos[z] >> 16) - row0, x = pos[z] & 0xFFFF;
+        double *dn = num + (size_t)y * stride + x;
+        for (int u = 0; u < B; ++u)
+            for (int v = 0; v < B; ++v) dn[(size_t)u * stride + v] += *s++ * numWeight;
+    }
+    for (int z = 0; z < N; ++z) {
+        int y = (pos[z] >> 16) - row0, x = pos[z] & 0xFFFF;
+        double *dd = den + (size_t)y * stride + x;
+        for (int u = 0; u < B; ++u)
+            for (int v = 0; v < B; ++v) dd[(size_t)u * stride + v] += denWeight;
+    }
+}
+
+static int degenerate(const filter_data *f, int nplanes)
+{
+    const orc_step_params *p = &f->p;
+    int nosigma = nplanes == 1 ? !(p->sigma[0] > 0)
+                               : (!(p->sigma[0] > 0) && !(p->sigma[1] > 0) && !(p->sigma[2] > 0));
+    return nosigma || p->group_size == 0 || p->block_size <= 0 || p->bm_range <= 0 ||
+           p->bm_range < p->bm_step || p->thMSE <= 0;
+}
+
+int orc_kernel(const orc_step_params *p, int wiener, int nplanes,
+               double *const *dst, const double *const *src, const double *const *ref,
+               int w, int h)
+{
+    if (nplanes != 1 && nplanes != 3) return -1;
+    init_tables();
+    filter_data f;
+    filter_data_init(&f, p, wiener);
+    size_t npx = (size_t)w * h;
+    if (degenerate(&f, nplanes)) {
+        for (int c = 0; c < nplanes; ++c)
+            if (dst[c] != src[c]) memcpy(dst[c], src[c], sizeof(double) * npx);
+        filter_data_free(&f);
+        return 1;
+    }
+    const int B = p->block_size, G = p->group_size;
+    if (w < B || h < B || p->block_step <= 0 || p->bm_step <= 0 || G < 0 || G > ORC_MAXN) {
+        filter_data_free(&f);
+        return -2;
+    }
+    int active[3] = {0, 0, 0};
+    for (int c = 0; c < nplanes; ++c) active[c] = p->sigma[c] > 0;
+
+    int ngx = orc_grid(w, B, p->block_step, NULL, 0);
+    int ngy = orc_grid(h, B, p->block_step, NULL, 0);
+    int *gx = (int *)malloc(sizeof(int) * (size_t)ngx), *gy = (int *)malloc(sizeof(int) * (size_t)ngy);
+    orc_grid(w, B, p->block_step, gx, ngx);
+    orc_grid(h, B, p->block_step, gy, ngy);
+
+    double *num[3] = {0, 0, 0}, *den[3] = {0, 0, 0};
+    for (int c = 0; c < nplanes; ++c)
+        if (active[c]) {
+            num[c] = (double *)calloc(npx, sizeof(double));
+            den[c] = (double *)calloc(npx, sizeof(double));
+        }
+
+    int range = p->bm_range / p->bm_step * p->bm_step;
+    int nthreads = g_threads;
+    if (nthreads > ngy) nthreads = ngy;
+    int Gcap = G > 0 ? G : ORC_MAXN;
+
+    if (nthreads <= 1) {
+        /* The reference's single loop: accumulate straight into num/den. */
+        cand *buf = (cand *)malloc(sizeof(cand) * (size_t)window_cap(p));
+        int32_t pos[ORC_MAXN];
+        double *grp = (double *)malloc(sizeof(double) * (size_t)Gcap * B * B);
+        double *gref = (double *)malloc(sizeof(double) * (size_t)Gcap * B * B);
+        for (int a = 0; a < ngy; ++a)
+            for (int b = 0; b < ngx; ++b) {
+                int N = match_one(&f, ref[0], w, h, w, gy[a], gx[b], buf, pos, NULL);
+                if (G > 0 && N > G) N = G;
+                for (int c = 0; c < nplanes; ++c)
+                    if (active[c])
+                        collaborative_filter(&f, c, num[c], den[c], 0, w, src[c], ref[c], pos, N, grp, gref);
+            }
+        free(buf); free(grp); free(gref);
+    } else {
+        /* Row bands of the grid, each with private num/den rows covering the
+         * bands' reach; summed afterwards in band order. */
+        int per = (ngy + nthreads - 1) / nthreads;
+#pragma omp parallel for schedule(static, 1) num_threads(nthreads)
+        for (int t = 0; t < nthreads; ++t) {
+            int a0 = t * per, a1 = a0 + per < ngy ? a0 + per : ngy;
+            if (a0 >= a1) continue;
+            int r0 = gy[a0] - range; if (r0 < 0) r0 = 0;
+            int r1 = gy[a1 - 1] + range + B; if (r1 > h) r1 = h;
+            size_t bpx = (size_t)(r1 - r0) * w;
+            double *bn[3] = {0, 0, 0}, *bd[3] = {0, 0, 0};
+            for (int c = 0; c < nplanes; ++c)
+                if (active[c]) { bn[c] = (double *)calloc(bpx, sizeof(double)); bd[c] = (double *)calloc(bpx, sizeof(double)); }
+            cand *buf = (cand *)malloc(sizeof(cand) * (size_t)window_cap(p));
+            int32_t pos[ORC_MAXN];
+            double *grp = (double *)malloc(sizeof(double) * (size_t)Gcap * B * B);
+            double *gref = (double *)malloc(sizeof(double) * (size_t)Gcap * B * B);
+            for (int a = a0; a < a1; ++a)
+                for (int b = 0; b < ngx; ++b) {
+                    int N = match_one(&f, ref[0], w, h, w, gy[a], gx[b], buf, pos, NULL);
+                    if (G > 0 && N > G) N = G;
+                    for (int c = 0; c < nplanes; ++c)
+                        if (active[c])
+                            collaborative_filter(&f, c, bn[c], bd[c], r0, w, src[c], ref[c], pos, N, grp, gref);
+                }
+            free(buf); free(grp); free(gref);
+#pragma omp critical
+            {
+                for (int c = 0; c < nplanes; ++c)
+                    if (active[c]) {
+                        double *dn = num[c] + (size_t)r0 * w, *dd = den[c] + (size_t)r0 * w;
+                        for (size_t k = 0; k < bpx; ++k) { dn[k] += bn[c][k]; dd[k] += bd[c][k]; }
+                    }
+            }
+            for (int c = 0; c < nplanes; ++c) { free(bn[c]); free(bd[c]); }
+        }
+    }
+
+    for (int c = 0; c < nplanes; ++c) {
+        if (active[c]) {
+            for (size_t k = 0; k < npx; ++k) dst[c][k] = num[c][k] / den[c][k];
+            free(num[c]); free(den[c]);
+        } else if (dst[c] != src[c]) {
+            memcpy(dst[c], src[c], sizeof(double) * npx);
+        }
+    }
+    free(gx); free(gy);
+    filter_data_free(&f);
+    return 0;
+}
+
+int orc_bm3d_plane(const orc_step_params *basic, const orc_step_params *final_,
+                   double *dst, const double *src, const double *ref, int w, int h)
+{
+    size_t npx = (size_t)w * h;
+    double *tmp = (double *)malloc(sizeof(double) * npx);
+    const double *s1[1] = {src}, *r1[1] = {ref ? ref : src};
+    double *d1[1] = {tmp};
+    int rc = orc_kernel(basic, 0, 1, d1, s1, r1, w, h);
+    if (rc >= 0) {
+        const double *r2[1] = {tmp};
+        double *d2[1] = {dst};
+        rc = orc_kernel(final_, 1, 1, d2, s1, r2, w, h);
+    }
+    free(tmp);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* OPP colour (include/Conversion.hpp:547-578, 418-432, 739-762) for float  */
+/* planes with Floor 0 / Ceil 1 (chroma Floor -0.5 / Neutral 0 / Ceil 0.5). */
+
+void orc_rgb2opp(double *y, double *u, double *v, const double *r, const double *g,
+                 const double *b, long n)
+{
+    double gainY = 1.0 / (1.0 * 3.0), offsetY = -0.0 * 3.0 * gainY + 0.0;
+    double gainU = 1.0 / (1.0 * 2.0), gainV = 1.0 / (1.0 * 4.0), offsetC = 0.0;
+    for (long k = 0; k < n; ++k) {
+        if (y) y[k] = (r[k] + g[k] + b[k]) * gainY + offsetY;
+        if (u) u[k] = (r[k] - b[k]) * gainU + offsetC;
+        if (v) v[k] = (r[k] - g[k] * 2.0 + b[k]) * gainV + offsetC;
+    }
+}
+
+static double clip01(double x) { return x < 0.0 ? 0.0 : (x > 1.0 ? 1.0 : x); }
+
+void orc_opp2rgb(double *r, double *g, double *b, const double *y, const double *u,
+                 const double *v, long n, int clip)
+{
+    double Ry = 1.0, Ru = 1.0, Rv = (double)(2.0L / 3.0L);
+    double Gy = 1.0, Gv = (double)(-4.0L / 3.0L);
+    double By = 1.0, Bu = -1.0, Bv = (double)(2.0L / 3.0L);
+    double offsetR = -0.0 * Ry - 0.0 * (Ru + Rv) + 0.0;
+    double offsetG = -0.0 * Gy - 0.0 * (0.0 + Gv) + 0.0;
+    double offsetB = -0.0 * By - 0.0 * (Bu + Bv) + 0.0;
+    for (long k = 0; k < n; ++k) {
+        double tr = Ry * y[k] + Ru * u[k] + Rv * v[k] + offsetR;
+        double tg = Gy * y[k] + Gv * v[k] + offsetG;
+        double tb = By * y[k] + Bu * u[k] + Bv * v[k] + offsetB;
+        r[k] = clip ? clip01(tr) : tr;
+        g[k] = clip ? clip01(tg) : tg;
+        b[k] = clip ? clip01(tb) : tb;
+    }
+}
+
+static int wrapper_degenerate(const orc_step_params *p)
+{
+    /* BM3D_Base::process_Frame (source/BM3D.cpp:284-290) */
+    return (!(p->sigma[0] > 0) && !(p->sigma[1] > 0) && !(p->sigma[2] > 0)) || p->group_size == 1 ||
+           p->block_size <= 0 || p->bm_range <= 0 || p->bm_range < p->bm_step || p->thMSE <= 0;
+}
+
+int orc_bm3d_rgb(const orc_step_params *basic, const orc_step_params *final_, int mode,
+                 double *const *dst, const double *const *src, const double *const *ref,
+                 int w, int h)
+{
+    long npx = (long)w * h;
+    if (mode != 1 && mode != 2 && mode != 3) return -1;
+    const orc_step_params *single = mode == 1 ? basic : final_;
+    if (mode != 3 && wrapper_degenerate(single)) {
+        for (int c = 0; c < 3; ++c) memcpy(dst[c], src[c], sizeof(double) * (size_t)npx);
+        return 1;
+    }
+    double *buf = (double *)malloc(sizeof(double) * (size_t)npx * 9);
+    double *s[3] = {buf, buf + npx, buf + 2 * npx};
+    double *t[3] = {buf + 3 * npx, buf + 4 * npx, buf + 5 * npx};
+    double *r[3] = {buf + 6 * npx, buf + 7 * npx, buf + 8 * npx};
+    orc_rgb2opp(s[0], s[1], s[2], src[0], src[1], src[2], npx);
+    int have_ref = ref && ref[0];
+    int rc;
+    const double *sc[3] = {s[0], s[1], s[2]};
+    if (mode == 3) {
+        const double *rb[3] = {s[0], s[1], s[2]};
+        if (have_ref) {
+            orc_rgb2opp(r[0], NULL, NULL, ref[0], ref[1], ref[2], npx);
+            rb[0] = r[0];
+        }
+        rc = orc_kernel(basic, 0, 3, t, sc, rb, w, h);
+        if (rc >= 0) {
+            const double *tc[3] = {t[0], t[1], t[2]};
+            rc = orc_kernel(final_, 1, 3, s, sc, tc, w, h);
+        }
+    } else {
+        const double *rb[3] = {s[0], s[1], s[2]};
+        if (have_ref) {
+            if (mode == 1) orc_rgb2opp(r[0], NULL, NULL, ref[0], ref[1], ref[2], npx);
+            else orc_rgb2opp(r[0], r[1], r[2], ref[0], ref[1], ref[2], npx);
+            rb[0] = r[0]; rb[1] = r[1]; rb[2] = r[2];
+        }
+        rc = orc_kernel(single, mode == 2, 3, s, sc, rb, w, h);
+    }
+    if (rc >= 0) orc_opp2rgb(dst[0], dst[1], dst[2], s[0], s[1], s[2], npx, 1);
+    free(buf);
+    return rc;
+}

@@ -1,0 +1,95 @@
+/*
+ * FP64 CPU restatement of ISP_MW's BM3D filter.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the correctness oracle for the
+ * HIP path and the timed CPU baseline.  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg may load it.  The product library
+ * (libisp_bm3d.so) never links or calls it.
+ *
+ * Everything is double precision, as in the reference CPU build where
+ * FLType is double (include/Type.h:26-30).
+ */
+#ifndef ISP_BM3D_ORACLE_H_
+#define ISP_BM3D_ORACLE_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Mirror of BM3D_Para_Base (include/BM3D.h:15-45): values as the user gives
+ * them; sigma in 0..255 units, thMSE in MSE units of the 8-bit scale. */
+typedef struct orc_step_params {
+    int block_size;
+    int block_step;
+    int group_size;
+    int bm_range;
+    int bm_step;
+    double thMSE;
+    double lambda;
+    double sigma[3];
+} orc_step_params;
+
+/* Reference grid of one axis (source/BM3D.cpp:98-127).  Writes at most
+ * `cap` positions, returns the count. */
+int orc_grid(int n, int block_size, int block_step, int *out, int cap);
+
+/* Block::_SearchBoundary (include/Block.h:716-745). */
+int orc_search_boundary(int pos, int plane_boundary, int search_range, int search_step);
+
+/* Block matching for every reference position of the grid
+ * (BM3D_Base::BlockMatching, source/BM3D.cpp:221-236, and
+ * Block::BlockMatchingMulti, include/Block.h:578-711) on one plane `ref`
+ * (range 1, row stride `stride`).  For grid point g = gy*ngx+gx:
+ * counts[g] = N (1..group_size), pos[g*group_size+z] = (y<<16)|x, and
+ * keys[g*group_size+z] = distance key (MSE units).  keys may be NULL. */
+int orc_match_all(const orc_step_params *p, const double *ref, int w, int h, int stride,
+                  int *counts, int32_t *pos, double *keys);
+
+/* BM3D_Base::Kernel (source/BM3D.cpp:80-218).  nplanes = 1 or 3; matching
+ * runs on ref[0]; plane c is filtered when sigma[c] > 0 (nplanes=3) or
+ * sigma[0] > 0 (nplanes=1).  wiener = 0 -> BM3D_Basic, 1 -> BM3D_Final.
+ * dst planes are written in full (planes that are not filtered are copied
+ * from src, the behaviour of the two-step Frame path).
+ * Returns 0 on success, 1 if the parameters made it copy src through,
+ * negative on error. */
+int orc_kernel(const orc_step_params *p, int wiener, int nplanes,
+               double *const *dst, const double *const *src, const double *const *ref,
+               int w, int h);
+
+/* Two-step BM3D on one plane with the intermediate estimate sized like src
+ * (BM3D::process_Plane_FL, source/BM3D.cpp:464-472, with tmp sized). */
+int orc_bm3d_plane(const orc_step_params *basic, const orc_step_params *final_,
+                   double *dst, const double *src, const double *ref, int w, int h);
+
+/* Two-step BM3D on a float RGB frame in [0,1] with the semantics of
+ * BM3D::process_Frame (source/BM3D.cpp:501-527): OPP conversion without clip,
+ * step 1 matched on Y of ref (or of src when ref is NULL), step 2 matched on
+ * the basic Y, OPP inverse with clip to [0,1].  mode: 1 basic only
+ * (BM3D_Basic::process_Frame), 2 final only (ref is then the guide frame,
+ * BM3D_Final::process_Frame), 3 both. */
+int orc_bm3d_rgb(const orc_step_params *basic, const orc_step_params *final_, int mode,
+                 double *const *dst, const double *const *src, const double *const *ref,
+                 int w, int h);
+
+/* OPP conversions on float planes (include/Conversion.hpp:547-578, 739-762)
+ * for a float RGB frame with Floor 0, Ceil 1. */
+void orc_rgb2opp(double *y, double *u, double *v, const double *r, const double *g,
+                 const double *b, long n);
+void orc_opp2rgb(double *r, double *g, double *b, const double *y, const double *u,
+                 const double *v, long n, int clip);
+
+/* Unnormalised 3D DCT on an n0 x n1 x n2 array (n2 fastest): FFTW REDFT10
+ * forward (inverse = 0) or REDFT01 backward (inverse = 1) on every axis. */
+void orc_dct3d(double *data, int n0, int n1, int n2, int inverse);
+
+/* Row-band threads used by orc_kernel / orc_bm3d_* (1 = the reference's
+ * single-threaded loop order).  Returns the previous value. */
+int orc_set_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
