@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""Throughput of two-step BM3D on one MI355X: 1920x1080 RGB, sigma = 25.
+
+Workload: the ISP_MW Frame path (OPP colour, step 1 hard threshold matched on
+Y, step 2 Wiener matched on the basic Y, OPP inverse with clip) on a seeded
+synthetic 1920x1080 RGB image (isp_mw_amd.synth, rectangles + gradient +
+gratings, channels mixed, AWGN sigma 25/255).  Parameters: BlockSize 8,
+BlockStep 3 (both steps), BMrange 12, BMstep 1, GroupSize 16 / 32, lambda 2.7,
+thMSE from thMSE_Default (profile "np" otherwise).
+
+One step = one two-step BM3D of the frame with input and output resident in
+HBM.  Prints one JSON line (rank 0, one GPU).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+W, H, SIGMA, SEED = 1920, 1080, 25.0, 1
+B = 8
+PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def make_para():
+    from isp_mw_amd import BM3D_Para
+    p = BM3D_Para("np")
+    for st in (p.basic, p.final):
+        st.BlockStep = 3
+        st.BMrange = 12
+        st.BMstep = 1
+    p.basic.GroupSize = 16
+    p.final.GroupSize = 32
+    p.basic.lambda_ = 2.7
+    p.set_sigma(SIGMA)
+    return p
+
+
+def grid(n: int, s: int) -> np.ndarray:
+    last = n - B
+    return np.minimum(np.arange(last // s + 1 + (1 if last % s else 0)) * s, last)
+
+
+def window_len(pos: np.ndarray, n: int, R: int) -> np.ndarray:
+    # clipped search window with BMstep 1 (include/Block.h:716-751)
+    return np.minimum(pos + R, n - B) - np.maximum(pos - R, 0) + 1
+
+
+def transform_flops(n) -> np.ndarray:
+    # dense separable N x 8 x 8 DCT: 2 B^2 N (2B + N)
+    n = np.asarray(n, np.float64)
+    return 2.0 * B * B * n * (2 * B + n)
+
+
+def cpu_baseline(para, clean_rgb_noisy, threads: int, target_s: float):
+    """Time the oracle (FP64 C restatement, multi-threaded row bands) on a crop."""
+    import oracle
+    fast = oracle.host_supports_fast()
+    oracle.set_threads(threads, fast=fast)
+    noisy = clean_rgb_noisy
+    # calibrate on a small crop, then size the timed crop to ~target_s per run
+    hc = 64
+    t0 = time.perf_counter()
+    oracle.bm3d_rgb(para.basic, para.final, noisy[:, :hc, :], fast=fast)
+    rate = W * hc / max(time.perf_counter() - t0, 1e-6)
+    hc = int(min(H, max(64, rate * target_s / W)))
+    times = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        oracle.bm3d_rgb(para.basic, para.final, noisy[:, :hc, :], fast=fast)
+        times.append(time.perf_counter() - t0)
+    dt = min(times)
+    return {
+        "value": round(W * hc / dt / 1e6, 4),
+        "unit": "Mpix/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"oracle (FP64 C restatement, {'x86-64-v3' if fast else 'baseline x86-64'} build, "
+                  f"{threads} threads, row bands) two-step BM3D on the top {W}x{hc} crop of the same "
+                  f"frame, best of 2 runs of {dt:.2f} s",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", default="fp32", choices=("fp32", "fp16", "bf16"))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=4.0, help="target seconds per CPU baseline run")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(images shard across GPUs with no collectives, see the design notes)", file=sys.stderr)
+        return 2
+
+    import torch
+    from isp_mw_amd import BM3D, synth
+
+    para = make_para()
+    clean, noisy = synth.rgb_pair(H, W, SIGMA, SEED)
+    dev = torch.device("cuda:0")
+    x = torch.from_numpy(noisy).to(dev)
+    y = torch.empty_like(x)
+    f = BM3D(para, device=0, precision=args.precision)
+
+    for _ in range(args.warmup):
+        f.process(y, x)
+    torch.cuda.synchronize()
+    f.set_timing(True)
+    f.timing(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        f.process(y, x)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    stages = f.timing()
+    f.set_timing(False)
+    dt = (t1 - t0) / args.steps
+    value = W * H / dt / 1e6
+    out = y.cpu().numpy()
+    psnr_out = synth.psnr(out, clean)
+    psnr_in = synth.psnr(noisy, clean)
+
+    # group sizes of the last run (actual-N work) and full-N work
+    gshape = f.grid_shape(W, H, False)
+    n1 = f.last_groups(False, gshape).astype(np.float64)
+    n2 = f.last_groups(True, f.grid_shape(W, H, True)).astype(np.float64)
+    refs = n1.size
+    gx, gy = grid(W, 3), grid(H, 3)
+    cand = float(window_len(gx, W, 12).sum() * window_len(gy, H, 12).sum() - refs)
+    match_flops = cand * B * B * 3              # per step, luma only
+    filt2_actual = 3 * 3 * transform_flops(n2).sum()   # 3 channels x 3 transforms
+    filt1_actual = 3 * 2 * transform_flops(n1).sum()
+    filt2_full = refs * 3 * 3 * transform_flops(32)
+    per_stage = {k: {"ms_per_launch": round(v[0] / v[1], 4) if v[1] else None, "launches": v[1]}
+                 for k, v in stages.items()}
+    # dominant kernel = the stage with the largest share of device time
+    dom = max(("filter_final", "filter_basic", "match_basic", "match_final"), key=lambda k: stages[k][0])
+    ms = stages[dom][0] / stages[dom][1]
+    if dom.startswith("filter"):
+        flops = filt2_actual if dom == "filter_final" else filt1_actual
+        achieved = flops / (ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None, "kernel": dom,
+                "engine": "fp32 VALU", "algorithmic_flops_per_launch": flops,
+                "flops_basis": "dense separable 2*B^2*N*(2B+N) per transform at each group's actual N"}
+    else:
+        achieved = match_flops / (ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None, "kernel": dom,
+                "engine": "fp64 VALU", "algorithmic_flops_per_launch": match_flops,
+                "flops_basis": "3 flop (sub+FMA) per pixel per candidate, clipped windows"}
+
+    res = {
+        "metric": "Mpix/s two-step BM3D 1080p RGB sigma=25 at <=0.05 dB PSNR vs plain FP32 GPU",
+        "value": round(value, 3),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32" if args.precision == "fp32" else f"{args.precision}+fp32",
+        "data": "synthetic (seeded rectangles+gradient+gratings, mixed RGB, AWGN sigma 25/255)",
+        "config": {"workload": "two-step BM3D, 1920x1080 RGB float32, sigma=25, OPP colour",
+                   "block": 8, "block_step": [3, 3], "bm_range": 12, "bm_step": 1, "group_size": [16, 32],
+                   "lambda": 2.7, "precision": args.precision},
+        "roofline": roof,
+        "quality": {"psnr_noisy_db": round(psnr_in, 3), "psnr_out_db": round(psnr_out, 3),
+                    "mean_group_size": [round(float(n1.mean()), 3), round(float(n2.mean()), 3)]},
+        "stages": per_stage,
+        "work": {"refs_per_step": refs, "match_candidates_per_step": cand, "match_flops_per_step": match_flops,
+                 "filter_final_flops_actual_n": filt2_actual, "filter_final_flops_full_n": filt2_full,
+                 "filter_basic_flops_actual_n": filt1_actual},
+    }
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        res["cpu_baseline"] = cpu_baseline(para, noisy, threads, args.cpu_seconds)
+        res["gpu_over_cpu"] = round(value / res["cpu_baseline"]["value"], 1)
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -120,6 +120,24 @@ int isp_bm3d_rgb_f32(isp_bm3d_ctx *ctx, float *dst_r, float *dst_g, float *dst_b
 int isp_bm3d_match(isp_bm3d_ctx *ctx, const float *guide, int width, int height, size_t stride,
                    int final_step, int mem, int *counts, int *pos, double *keys);
 
+/* Per-stage device timing (no reference counterpart; the reference has no
+ * timers).  When enabled, every kernel launch of the library is bracketed by
+ * hipEvents on the caller's stream.  isp_bm3d_get_timing() waits for the
+ * recorded events, adds their elapsed times into per-stage totals and copies
+ * up to `max` stages: names[i] (static strings), total_ms[i], launches[i].
+ * It returns the number of stages (ISP_BM3D_NUM_STAGES). */
+#define ISP_BM3D_NUM_STAGES 8
+int isp_bm3d_set_timing(isp_bm3d_ctx *ctx, int enable);
+int isp_bm3d_get_timing(isp_bm3d_ctx *ctx, const char **names, double *total_ms, long *launches, int max);
+int isp_bm3d_reset_timing(isp_bm3d_ctx *ctx);
+
+/* Groups formed by the most recent plane/RGB call for one step, copied to
+ * host arrays: counts[g] (capacity >= grid points) and, when pos is not NULL,
+ * pos[g*G + z] (capacity*G entries).  Waits for the device.  Returns the
+ * number of grid points.  No reference counterpart (the reference keeps its
+ * match codes local to Kernel, source/BM3D.cpp:122,190); for parity checks. */
+int isp_bm3d_last_groups(isp_bm3d_ctx *ctx, int final_step, int *counts, int *pos, size_t capacity);
+
 /* Grid sizes of a step for an image (source/BM3D.cpp:98-127). */
 int isp_bm3d_grid_size(const isp_bm3d_ctx *ctx, int width, int height, int final_step,
                        int *ngx, int *ngy);

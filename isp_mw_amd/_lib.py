@@ -30,7 +30,12 @@ EXPORTS = (
     "isp_bm3d_rgb_f32",
     "isp_bm3d_match",
     "isp_bm3d_grid_size",
+    "isp_bm3d_set_timing",
+    "isp_bm3d_get_timing",
+    "isp_bm3d_reset_timing",
+    "isp_bm3d_last_groups",
 )
+NUM_STAGES = 8
 
 
 class StepParams(ctypes.Structure):
@@ -95,6 +100,11 @@ def lib() -> ctypes.CDLL:
     L.isp_bm3d_rgb_f32.argtypes = [vp] + [vp] * 9 + [i, i, sz, i, i, vp]
     L.isp_bm3d_match.argtypes = [vp, vp, i, i, sz, i, i, vp, vp, vp]
     L.isp_bm3d_grid_size.argtypes = [vp, i, i, i, ctypes.POINTER(i), ctypes.POINTER(i)]
+    L.isp_bm3d_set_timing.argtypes = [vp, i]
+    L.isp_bm3d_get_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_long), i]
+    L.isp_bm3d_reset_timing.argtypes = [vp]
+    L.isp_bm3d_last_groups.argtypes = [vp, i, vp, vp, sz]
     for name in EXPORTS:
         getattr(L, name).restype = getattr(L, name).restype or ctypes.c_int
     L.isp_bm3d_last_error.restype = ctypes.c_char_p

@@ -221,6 +221,42 @@ class _Filter:
             dst = np.empty_like(src)
         return self.process(dst, src, ref)
 
+    def set_timing(self, enable: bool = True) -> None:
+        """Bracket every kernel launch with hipEvents (per-stage device time)."""
+        L.check(L.lib().isp_bm3d_set_timing(self._ctx, int(enable)), self._ctx)
+
+    def timing(self, reset: bool = False) -> dict:
+        """{stage: (total_ms, launches)} accumulated since the last reset."""
+        n = L.NUM_STAGES
+        names = (ctypes.c_char_p * n)()
+        ms = (ctypes.c_double * n)()
+        calls = (ctypes.c_long * n)()
+        rc = L.lib().isp_bm3d_get_timing(self._ctx, names, ms, calls, n)
+        if rc < 0:
+            L.check(rc, self._ctx)
+        out = {names[k].decode(): (ms[k], calls[k]) for k in range(n)}
+        if reset:
+            L.check(L.lib().isp_bm3d_reset_timing(self._ctx), self._ctx)
+        return out
+
+    def last_groups(self, final_step: bool, shape: tuple[int, int], with_pos: bool = False):
+        """Groups of one step of the last call: counts [ngy, ngx] (and pos)."""
+        ngy, ngx = shape
+        counts = np.zeros((ngy, ngx), np.int32)
+        G = self.group_size(final_step)
+        pos = np.zeros((ngy, ngx, G), np.int32) if with_pos else None
+        rc = L.lib().isp_bm3d_last_groups(self._ctx, int(final_step), ctypes.c_void_p(counts.ctypes.data),
+                                          ctypes.c_void_p(pos.ctypes.data) if with_pos else None, ngy * ngx)
+        if rc < 0:
+            L.check(rc, self._ctx)
+        return (counts, pos) if with_pos else counts
+
+    def grid_shape(self, width: int, height: int, final_step: bool = False) -> tuple[int, int]:
+        ngx, ngy = ctypes.c_int(), ctypes.c_int()
+        L.check(L.lib().isp_bm3d_grid_size(self._ctx, width, height, int(final_step), ctypes.byref(ngx),
+                                           ctypes.byref(ngy)), self._ctx)
+        return ngy.value, ngx.value
+
     def match(self, guide: np.ndarray, final_step: bool = False):
         """Groups of one step on a host float32 plane: (counts, pos, keys) shaped
         [ngy, ngx], [ngy, ngx, G], [ngy, ngx, G]; pos packs (y << 16) | x."""

@@ -27,11 +27,74 @@ struct isp_bm3d_ctx {
     void *ws = nullptr;
     size_t ws_bytes = 0;
     std::string err;
+    // groups of the last run of each step (device, inside ws)
+    int *last_cnt[2] = {nullptr, nullptr};
+    int *last_pos[2] = {nullptr, nullptr};
+    size_t last_refs[2] = {0, 0};
+    int last_G[2] = {0, 0};
+    // per-stage timing
+    bool timing = false;
+    std::vector<hipEvent_t> event_pool;
+    struct Pending {
+        int stage;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    double stage_ms[ISP_BM3D_NUM_STAGES] = {0};
+    long stage_calls[ISP_BM3D_NUM_STAGES] = {0};
 };
 
 namespace {
 
 thread_local std::string g_create_err = "no error";
+
+enum Stage {
+    ST_OPP = 0,
+    ST_FILL,
+    ST_MATCH_BASIC,
+    ST_FILTER_BASIC,
+    ST_MATCH_FINAL,
+    ST_FILTER_FINAL,
+    ST_FINALIZE,
+    ST_COPY
+};
+const char *const kStageNames[ISP_BM3D_NUM_STAGES] = {
+    "opp", "fill", "match_basic", "filter_basic", "match_final", "filter_final", "finalize", "copy"};
+
+hipEvent_t take_event(isp_bm3d_ctx *ctx)
+{
+    if (!ctx->event_pool.empty()) {
+        hipEvent_t e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Brackets the launches of its scope with events when timing is enabled.
+struct StageTimer {
+    isp_bm3d_ctx *ctx;
+    int stage;
+    hipStream_t s;
+    hipEvent_t a = nullptr;
+    StageTimer(isp_bm3d_ctx *c, int st, hipStream_t stream) : ctx(c), stage(st), s(stream)
+    {
+        if (ctx->timing && (a = take_event(ctx))) (void)hipEventRecord(a, s);
+    }
+    ~StageTimer()
+    {
+        if (!a) return;
+        hipEvent_t b = take_event(ctx);
+        if (!b) {
+            ctx->event_pool.push_back(a);
+            return;
+        }
+        (void)hipEventRecord(b, s);
+        ctx->pending.push_back({stage, a, b});
+    }
+};
 
 int fail(isp_bm3d_ctx *ctx, int code, const char *fmt, ...)
 {
@@ -149,6 +212,10 @@ int ensure_ws(isp_bm3d_ctx *ctx, size_t bytes)
         HIP_CHECK(ctx, hipFree(ctx->ws));
         ctx->ws = nullptr;
         ctx->ws_bytes = 0;
+        for (int k = 0; k < 2; ++k) {
+            ctx->last_cnt[k] = ctx->last_pos[k] = nullptr;
+            ctx->last_refs[k] = 0;
+        }
     }
     bytes = (bytes + (1u << 20)) & ~size_t((1u << 20) - 1);
     HIP_CHECK(ctx, hipMalloc(&ctx->ws, bytes));
@@ -165,7 +232,7 @@ struct Work {
     float *out[3];     // host-mode output staging
     double *y64;       // FP64 matching guide (RGB path)
     double *ref64;     // FP64 matching guide of a separate ref frame
-    int *cnt, *pos;
+    int *cnt[2], *pos[2];  // per step
     double *keys;
 };
 
@@ -181,8 +248,10 @@ size_t layout(Work *w, void *base, int P, int H, size_t refs, bool keys)
     for (int k = 0; k < 3; ++k) w->out[k] = c.take<float>(np);
     w->y64 = c.take<double>(np);
     w->ref64 = c.take<double>(np);
-    w->cnt = c.take<int>(refs);
-    w->pos = c.take<int>(refs * kMaxGroup);
+    for (int k = 0; k < 2; ++k) {
+        w->cnt[k] = c.take<int>(refs);
+        w->pos[k] = c.take<int>(refs * kMaxGroup);
+    }
     w->keys = keys ? c.take<double>(refs * kMaxGroup) : nullptr;
     return c.off + 256;
 }
@@ -219,7 +288,7 @@ struct StepIO {
 
 int run_match(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, const Derived &d, Work &w,
               const float *g32, const double *g64, int gpitch, int W, int H, double *keys,
-              hipStream_t s)
+              hipStream_t s, int stage)
 {
     MatchArgs a{};
     a.W = W;
@@ -238,13 +307,19 @@ int run_match(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, const Derived &d
     a.tilePitch = a.tileW | 1;
     const int side = 2 * (d.range / p.bm_step) + 1;
     a.maxCand = side * side;
-    a.cnt_out = w.cnt;
-    a.pos_out = w.pos;
+    const int k = stage == ST_MATCH_FINAL ? 1 : 0;
+    a.cnt_out = w.cnt[k];
+    a.pos_out = w.pos[k];
+    ctx->last_cnt[k] = w.cnt[k];
+    ctx->last_pos[k] = w.pos[k];
+    ctx->last_refs[k] = (size_t)d.ngx * d.ngy;
+    ctx->last_G[k] = p.group_size;
     a.key_out = keys;
     if (match_lds_bytes(a) > 160 * 1024)
         return fail(ctx, ISP_BM3D_E_UNSUPPORTED,
                     "BMrange %d / BMstep %d / BlockStep %d need %zu bytes of LDS for matching (max 163840)",
                     p.bm_range, p.bm_step, p.block_step, match_lds_bytes(a));
+    StageTimer tm(ctx, stage, s);
     if (g64) HIP_CHECK(ctx, launch_match_f64(g64, gpitch, a, s));
     else HIP_CHECK(ctx, launch_match_f32(g32, gpitch, a, s));
     return ISP_BM3D_OK;
@@ -273,7 +348,7 @@ int run_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, cons
     }
     if (!d.degenerate) {
         rc = run_match(ctx, p, d, w, io.mguide32, io.mguide64, io.mguide64 ? io.pitch64 : io.pitch, W, H,
-                       nullptr, s);
+                       nullptr, s, wiener ? ST_MATCH_FINAL : ST_MATCH_BASIC);
         if (rc) return rc;
         FilterArgs f{};
         f.W = W;
@@ -284,8 +359,8 @@ int run_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, cons
         f.bstep = p.block_step;
         f.range = d.range;
         f.G = p.group_size;
-        f.cnt = w.cnt;
-        f.pos = w.pos;
+        f.cnt = w.cnt[wiener ? 1 : 0];
+        f.pos = w.pos[wiener ? 1 : 0];
         for (int c = 0; c < 3; ++c) {
             f.src[c] = io.src[c];
             f.guide[c] = io.guide[c];
@@ -296,6 +371,7 @@ int run_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, cons
             f.sig2[c] = d.sig2[c];
             fa.active[c] = d.active[c];
             if (d.active[c]) {
+                StageTimer tm(ctx, ST_FILL, s);
                 HIP_CHECK(ctx, launch_fill(w.num[c], (size_t)io.pitch * H, 0.f, s));
                 HIP_CHECK(ctx, launch_fill(w.den[c], (size_t)io.pitch * H, 0.f, s));
             }
@@ -308,8 +384,10 @@ int run_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, cons
         if (filter_lds_bytes(f, wiener) > 160 * 1024)
             return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "parameters need %zu bytes of LDS for filtering",
                         filter_lds_bytes(f, wiener));
+        StageTimer tm(ctx, wiener ? ST_FILTER_FINAL : ST_FILTER_BASIC, s);
         HIP_CHECK(ctx, launch_filter(f, wiener, s));
     }
+    StageTimer tm(ctx, ST_FINALIZE, s);
     HIP_CHECK(ctx, launch_finalize(fa, s));
     return ISP_BM3D_OK;
 }
@@ -459,11 +537,58 @@ int isp_bm3d_create(isp_bm3d_ctx **out, const isp_bm3d_params *params, int devic
     return ISP_BM3D_OK;
 }
 
+int isp_bm3d_set_timing(isp_bm3d_ctx *ctx, int enable)
+{
+    if (!ctx) return fail(nullptr, ISP_BM3D_E_ARG, "null context");
+    ctx->timing = enable != 0;
+    return ISP_BM3D_OK;
+}
+
+int isp_bm3d_get_timing(isp_bm3d_ctx *ctx, const char **names, double *total_ms, long *launches, int max)
+{
+    if (!ctx) return fail(nullptr, ISP_BM3D_E_ARG, "null context");
+    HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    for (auto &p : ctx->pending) {
+        float ms = 0.f;
+        HIP_CHECK(ctx, hipEventSynchronize(p.b));
+        HIP_CHECK(ctx, hipEventElapsedTime(&ms, p.a, p.b));
+        ctx->stage_ms[p.stage] += ms;
+        ctx->stage_calls[p.stage] += 1;
+        ctx->event_pool.push_back(p.a);
+        ctx->event_pool.push_back(p.b);
+    }
+    ctx->pending.clear();
+    for (int i = 0; i < max && i < ISP_BM3D_NUM_STAGES; ++i) {
+        if (names) names[i] = kStageNames[i];
+        if (total_ms) total_ms[i] = ctx->stage_ms[i];
+        if (launches) launches[i] = ctx->stage_calls[i];
+    }
+    return ISP_BM3D_NUM_STAGES;
+}
+
+int isp_bm3d_reset_timing(isp_bm3d_ctx *ctx)
+{
+    if (!ctx) return fail(nullptr, ISP_BM3D_E_ARG, "null context");
+    const char *n[ISP_BM3D_NUM_STAGES];
+    int rc = isp_bm3d_get_timing(ctx, n, nullptr, nullptr, 0);
+    if (rc < 0) return rc;
+    for (int i = 0; i < ISP_BM3D_NUM_STAGES; ++i) {
+        ctx->stage_ms[i] = 0;
+        ctx->stage_calls[i] = 0;
+    }
+    return ISP_BM3D_OK;
+}
+
 int isp_bm3d_destroy(isp_bm3d_ctx *ctx)
 {
     if (!ctx) return ISP_BM3D_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipDeviceSynchronize();
+    for (auto &p : ctx->pending) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     if (ctx->ws) (void)hipFree(ctx->ws);
     if (ctx->d_tables) (void)hipFree(ctx->d_tables);
     delete ctx;
@@ -477,6 +602,21 @@ int isp_bm3d_set_precision(isp_bm3d_ctx *ctx, int precision)
         return fail(ctx, ISP_BM3D_E_ARG, "bad precision %d", precision);
     ctx->precision = precision;
     return ISP_BM3D_OK;
+}
+
+int isp_bm3d_last_groups(isp_bm3d_ctx *ctx, int final_step, int *counts, int *pos, size_t capacity)
+{
+    if (!ctx) return fail(nullptr, ISP_BM3D_E_ARG, "null context");
+    const int k = final_step ? 1 : 0;
+    if (!ctx->last_cnt[k]) return fail(ctx, ISP_BM3D_E_ARG, "no groups recorded for this step yet");
+    const size_t refs = ctx->last_refs[k];
+    if (capacity < refs) return fail(ctx, ISP_BM3D_E_ARG, "capacity %zu < %zu grid points", capacity, refs);
+    HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    HIP_CHECK(ctx, hipDeviceSynchronize());
+    if (counts) HIP_CHECK(ctx, hipMemcpy(counts, ctx->last_cnt[k], refs * sizeof(int), hipMemcpyDeviceToHost));
+    if (pos)
+        HIP_CHECK(ctx, hipMemcpy(pos, ctx->last_pos[k], refs * ctx->last_G[k] * sizeof(int), hipMemcpyDeviceToHost));
+    return (int)refs;
 }
 
 int isp_bm3d_grid_size(const isp_bm3d_ctx *ctx, int W, int H, int final_step, int *ngx, int *ngy)
@@ -616,6 +756,7 @@ int isp_bm3d_rgb_f32(isp_bm3d_ctx *ctx, float *dst_r, float *dst_g, float *dst_b
     o.pitch = P;
     o.y64 = w.y64;
     o.pitch64 = P;
+    StageTimer tm_opp(ctx, ST_OPP, s);
     HIP_CHECK(ctx, launch_opp(o, s));
     if (sep_ref) {
         OppArgs q = o;
@@ -708,11 +849,12 @@ int isp_bm3d_match(isp_bm3d_ctx *ctx, const float *guide, int W, int H, size_t s
                                         hipMemcpyHostToDevice, s));
         g = w.plane[0];
     }
-    rc = run_match(ctx, p, d, w, g, nullptr, P, W, H, w.keys, s);
+    rc = run_match(ctx, p, d, w, g, nullptr, P, W, H, w.keys, s, final_step ? ST_MATCH_FINAL : ST_MATCH_BASIC);
     if (rc) return rc;
     const size_t refs = (size_t)d.ngx * d.ngy, G = (size_t)p.group_size;
-    HIP_CHECK(ctx, hipMemcpyAsync(counts, w.cnt, refs * sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_CHECK(ctx, hipMemcpyAsync(pos, w.pos, refs * G * sizeof(int), hipMemcpyDeviceToHost, s));
+    const int k = final_step ? 1 : 0;
+    HIP_CHECK(ctx, hipMemcpyAsync(counts, w.cnt[k], refs * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(ctx, hipMemcpyAsync(pos, w.pos[k], refs * G * sizeof(int), hipMemcpyDeviceToHost, s));
     if (keys) HIP_CHECK(ctx, hipMemcpyAsync(keys, w.keys, refs * G * sizeof(double), hipMemcpyDeviceToHost, s));
     HIP_CHECK(ctx, hipStreamSynchronize(s));
     return ISP_BM3D_OK;

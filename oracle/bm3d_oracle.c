@@ -68,88 +68,122 @@ static void init_tables(void)
 
 static int is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
-/* Lee's radix-2 factorisation of the DCT-II sum. */
-static void lee_dct2(double *x, int n, double *tmp)
+/* Lee's radix-2 factorisation of the DCT-II sum, applied along the row index
+ * of an n x w array (every one of the w columns transformed at once). */
+static void lee_dct2(double *x, int n, int w, double *tmp)
 {
     if (n == 1) return;
     int h = n / 2;
-    double *a = tmp, *b = tmp + h;
+    double *a = tmp, *b = tmp + (size_t)h * w;
     for (int j = 0; j < h; ++j) {
-        a[j] = x[j] + x[n - 1 - j];
-        b[j] = (x[j] - x[n - 1 - j]) * g_inv2cos[n][j];
+        const double *p = x + (size_t)j * w, *q = x + (size_t)(n - 1 - j) * w;
+        double *pa = a + (size_t)j * w, *pb = b + (size_t)j * w;
+        const double s = g_inv2cos[n][j];
+        for (int c = 0; c < w; ++c) {
+            pa[c] = p[c] + q[c];
+            pb[c] = (p[c] - q[c]) * s;
+        }
     }
-    lee_dct2(a, h, tmp + n);
-    lee_dct2(b, h, tmp + n);
-    for (int k = 0; k < h; ++k) x[2 * k] = a[k];
-    for (int k = 0; k < h - 1; ++k) x[2 * k + 1] = b[k] + b[k + 1];
-    x[n - 1] = b[h - 1];
+    lee_dct2(a, h, w, tmp + (size_t)n * w);
+    lee_dct2(b, h, w, tmp + (size_t)n * w);
+    for (int k = 0; k < h; ++k) memcpy(x + (size_t)(2 * k) * w, a + (size_t)k * w, sizeof(double) * (size_t)w);
+    for (int k = 0; k < h - 1; ++k) {
+        double *o = x + (size_t)(2 * k + 1) * w;
+        const double *p = b + (size_t)k * w, *q = b + (size_t)(k + 1) * w;
+        for (int c = 0; c < w; ++c) o[c] = p[c] + q[c];
+    }
+    memcpy(x + (size_t)(n - 1) * w, b + (size_t)(h - 1) * w, sizeof(double) * (size_t)w);
 }
 
 /* Transpose of lee_dct2. */
-static void lee_dct3(double *y, int n, double *tmp)
+static void lee_dct3(double *y, int n, int w, double *tmp)
 {
     if (n == 1) return;
     int h = n / 2;
-    double *v = tmp, *u = tmp + h;
+    double *v = tmp, *u = tmp + (size_t)h * w;
     for (int k = 0; k < h; ++k) {
-        v[k] = y[2 * k];
-        u[k] = y[2 * k + 1] + (k > 0 ? y[2 * k - 1] : 0.0);
+        memcpy(v + (size_t)k * w, y + (size_t)(2 * k) * w, sizeof(double) * (size_t)w);
+        double *pu = u + (size_t)k * w;
+        const double *p = y + (size_t)(2 * k + 1) * w;
+        if (k > 0) {
+            const double *q = y + (size_t)(2 * k - 1) * w;
+            for (int c = 0; c < w; ++c) pu[c] = p[c] + q[c];
+        } else {
+            for (int c = 0; c < w; ++c) pu[c] = p[c] + 0.0;
+        }
     }
-    lee_dct3(v, h, tmp + n);
-    lee_dct3(u, h, tmp + n);
+    lee_dct3(v, h, w, tmp + (size_t)n * w);
+    lee_dct3(u, h, w, tmp + (size_t)n * w);
     for (int j = 0; j < h; ++j) {
-        double t = u[j] * g_inv2cos[n][j];
-        y[j] = v[j] + t;
-        y[n - 1 - j] = v[j] - t;
+        double *o1 = y + (size_t)j * w, *o2 = y + (size_t)(n - 1 - j) * w;
+        const double *pv = v + (size_t)j * w, *pu = u + (size_t)j * w;
+        const double s = g_inv2cos[n][j];
+        for (int c = 0; c < w; ++c) {
+            double t = pu[c] * s;
+            o1[c] = pv[c] + t;
+            o2[c] = pv[c] - t;
+        }
     }
 }
 
-/* REDFT10 (inverse = 0) or REDFT01 (inverse = 1) of one contiguous vector. */
-static void redft_1d(double *x, int n, int inverse, double *tmp)
+/* REDFT10 (inverse = 0) or REDFT01 (inverse = 1) along the row index of an
+ * n x w array.  tmp holds at least 2*n*w doubles. */
+static void redft_rows(double *x, int n, int w, int inverse, double *tmp)
 {
     if (inverse)
-        for (int k = 1; k < n; ++k) x[k] *= 2.0;
+        for (size_t k = (size_t)w; k < (size_t)n * w; ++k) x[k] *= 2.0;
     if (is_pow2(n)) {
-        if (inverse) lee_dct3(x, n, tmp);
-        else lee_dct2(x, n, tmp);
+        if (inverse) lee_dct3(x, n, w, tmp);
+        else lee_dct2(x, n, w, tmp);
     } else {
         const double *m = g_dense[n];
         double *out = tmp;
         for (int a = 0; a < n; ++a) {
-            double s = 0.0;
-            if (inverse)
-                for (int k = 0; k < n; ++k) s += m[k * n + a] * x[k];
-            else
-                for (int j = 0; j < n; ++j) s += m[a * n + j] * x[j];
-            out[a] = s;
+            double *o = out + (size_t)a * w;
+            for (int c = 0; c < w; ++c) o[c] = 0.0;
+            for (int k = 0; k < n; ++k) {
+                const double coef = inverse ? m[k * n + a] : m[a * n + k];
+                const double *p = x + (size_t)k * w;
+                for (int c = 0; c < w; ++c) o[c] += coef * p[c];
+            }
         }
-        memcpy(x, out, sizeof(double) * (size_t)n);
+        memcpy(x, out, sizeof(double) * (size_t)n * w);
     }
     if (!inverse)
-        for (int k = 0; k < n; ++k) x[k] *= 2.0;
+        for (size_t k = 0; k < (size_t)n * w; ++k) x[k] *= 2.0;
+}
+
+static void transpose(double *dst, const double *src, int rows, int cols)
+{
+    for (int r = 0; r < rows; ++r)
+        for (int c = 0; c < cols; ++c) dst[(size_t)c * rows + r] = src[(size_t)r * cols + c];
 }
 
 void orc_dct3d(double *data, int n0, int n1, int n2, int inverse)
 {
     init_tables();
-    double vec[ORC_MAXN], tmp[4 * ORC_MAXN];
-    /* axis 2 (contiguous) */
-    for (int a = 0; a < n0 * n1; ++a) redft_1d(data + (size_t)a * n2, n2, inverse, tmp);
-    /* axis 1 */
-    for (int a = 0; a < n0; ++a)
-        for (int c = 0; c < n2; ++c) {
-            double *base = data + (size_t)a * n1 * n2 + c;
-            for (int b = 0; b < n1; ++b) vec[b] = base[(size_t)b * n2];
-            redft_1d(vec, n1, inverse, tmp);
-            for (int b = 0; b < n1; ++b) base[(size_t)b * n2] = vec[b];
-        }
-    /* axis 0 */
-    for (int bc = 0; bc < n1 * n2; ++bc) {
-        double *base = data + bc;
-        for (int a = 0; a < n0; ++a) vec[a] = base[(size_t)a * n1 * n2];
-        redft_1d(vec, n0, inverse, tmp);
-        for (int a = 0; a < n0; ++a) base[(size_t)a * n1 * n2] = vec[a];
+    enum { CAP = ORC_MAXN * 64 };
+    double stackbuf[2 * CAP + CAP];
+    double *tmp = stackbuf, *tr = stackbuf + 2 * CAP;
+    double *heap = NULL;
+    size_t slice = (size_t)n1 * n2, total = (size_t)n0 * slice;
+    if (total > CAP || n0 > ORC_MAXN || n1 > ORC_MAXN || n2 > ORC_MAXN) {
+        heap = (double *)malloc(sizeof(double) * 3 * (total > slice ? total : slice) * 2);
+        tmp = heap;
+        tr = heap + 4 * total;
     }
+    /* axis 0: the n0 rows of width n1*n2 */
+    redft_rows(data, n0, (int)slice, inverse, tmp);
+    for (int a = 0; a < n0; ++a) {
+        double *s = data + (size_t)a * slice;
+        /* axis 1: n1 rows of width n2 */
+        redft_rows(s, n1, n2, inverse, tmp);
+        /* axis 2: via the transposed slice */
+        transpose(tr, s, n1, n2);
+        redft_rows(tr, n2, n1, inverse, tmp);
+        transpose(s, tr, n2, n1);
+    }
+    free(heap);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -289,21 +323,44 @@ static int match_one(const filter_data *f, const double *ref, int w, int h, int 
     buf[n].key = 0.0; buf[n].seq = -1; buf[n].y = j; buf[n].x = i; ++n;
     int seq = 0;
     for (int y = t; y <= b; y += step) {
-        for (int x = l; x <= r; x += step) {
-            if (y == j && x == i) continue;
-            const double *cb = ref + (size_t)y * stride + x;
-            double dist = 0.0;
-            for (int u = 0; u < B; ++u) {
-                const double *rr = rb + (size_t)u * stride, *cc = cb + (size_t)u * stride;
-                for (int v = 0; v < B; ++v) {
-                    double temp = rr[v] - cc[v];
-                    dist += temp * temp;
+        /* Four horizontally adjacent candidates at a time: each keeps its own
+         * accumulator, summed in the reference's row-major order, so every
+         * distance is bit-identical to the one-at-a-time loop. */
+        for (int x0 = l; x0 <= r; x0 += 4 * step) {
+            int nc = (r - x0) / step + 1;
+            if (nc > 4) nc = 4;
+            double dist[4] = {0.0, 0.0, 0.0, 0.0};
+            const double *cb = ref + (size_t)y * stride + x0;
+            if (nc == 4) {
+                for (int u = 0; u < B; ++u) {
+                    const double *rr = rb + (size_t)u * stride, *cc = cb + (size_t)u * stride;
+                    for (int v = 0; v < B; ++v) {
+                        double t0 = rr[v] - cc[v], t1 = rr[v] - cc[v + step];
+                        double t2 = rr[v] - cc[v + 2 * step], t3 = rr[v] - cc[v + 3 * step];
+                        dist[0] += t0 * t0;
+                        dist[1] += t1 * t1;
+                        dist[2] += t2 * t2;
+                        dist[3] += t3 * t3;
+                    }
                 }
+            } else {
+                for (int c = 0; c < nc; ++c)
+                    for (int u = 0; u < B; ++u) {
+                        const double *rr = rb + (size_t)u * stride, *cc = cb + (size_t)u * stride + c * step;
+                        for (int v = 0; v < B; ++v) {
+                            double temp = rr[v] - cc[v];
+                            dist[c] += temp * temp;
+                        }
+                    }
             }
-            if (dist <= thSSE && dist != 0) {
-                buf[n].key = dist * distMul; buf[n].seq = seq; buf[n].y = y; buf[n].x = x; ++n;
+            for (int c = 0; c < nc; ++c) {
+                int x = x0 + c * step;
+                if (y == j && x == i) continue;
+                if (dist[c] <= thSSE && dist[c] != 0) {
+                    buf[n].key = dist[c] * distMul; buf[n].seq = seq; buf[n].y = y; buf[n].x = x; ++n;
+                }
+                ++seq;
             }
-            ++seq;
         }
     }
     qsort(buf, (size_t)n, sizeof(cand), cand_cmp);

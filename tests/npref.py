@@ -1,0 +1,115 @@
+"""Independent numpy formulation of the BM3D collaborative filter, for
+pinning the oracle (test code only).
+
+It does NOT restate the reference line by line.  It uses orthonormal DCTs
+(scipy.fft.dctn norm="ortho") and the closed forms that the reference's
+unnormalised FFTW tables reduce to (source/BM3D.cpp:26-70, 339-457):
+  step 1: zero an orthonormal coefficient iff |O| <= lambda*sigma_c; weight
+          1/retained (1 if none);
+  step 2: W = O_r^2 / (O_r^2 + sigma_c^2 / 2^zeta), zeta = number of zero
+          indices (z, u, v); weight 1/sum(W^2) (1 if 0);
+  num += weight * block, den += weight over every block of the group.
+Groups come from a brute-force window search (brute_groups).
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.fft as sf
+
+NORM = np.array([np.sqrt(1 / 3), np.sqrt(1 / 2), np.sqrt(3 / 8)])
+B = 8
+
+
+def grid(n, s):
+    last = n - B
+    out, j = [], 0
+    while True:
+        if j >= last + s:
+            break
+        if j > last:
+            j = last
+        out.append(j)
+        j += s
+    return out
+
+
+def brute_groups(p, ref):
+    """All candidates of the clipped window (BMstep grid through the reference
+    position), SSE <= thSSE and != 0, sorted by (key, scan order), self first,
+    truncated to GroupSize.  Returns {(j, i): [(key, y, x), ...]}."""
+    ref = np.asarray(ref, np.float64)
+    h, w = ref.shape
+    st = p.BMstep
+    R = p.BMrange // st * st
+    thMSE = p.thMSE * NORM[0]
+    mse2sse = B * B / 255.0 ** 2
+    th = thMSE * mse2sse
+    out = {}
+    for j in grid(h, p.BlockStep):
+        for i in grid(w, p.BlockStep):
+            rb = ref[j:j + B, i:i + B]
+            if p.GroupSize == 1:
+                out[(j, i)] = [(0.0, j, i)]
+                continue
+            ys = [y for y in range(j - R, j + R + 1, st) if 0 <= y <= h - B]
+            xs = [x for x in range(i - R, i + R + 1, st) if 0 <= x <= w - B]
+            cands = []
+            for y in ys:
+                for x in xs:
+                    if (y, x) == (j, i):
+                        continue
+                    d = float(np.sum((rb - ref[y:y + B, x:x + B]) ** 2))
+                    if d <= th and d != 0:
+                        cands.append((d / mse2sse, len(cands), y, x))
+            cands.sort(key=lambda t: (t[0], t[1]))
+            g = [(0.0, j, i)] + [(k, y, x) for k, _, y, x in cands]
+            out[(j, i)] = g[:p.GroupSize] if p.GroupSize > 0 else g
+    return out
+
+
+def filter_plane(p, src, ref, groups, c, wiener):
+    """Aggregated num/den of one plane for the given groups."""
+    src = np.asarray(src, np.float64)
+    ref = np.asarray(ref, np.float64)
+    h, w = src.shape
+    sigma = p.sigma[c] / 255.0 * NORM[c]
+    num = np.zeros((h, w))
+    den = np.zeros((h, w))
+    for g in groups.values():
+        n = len(g)
+        grp = np.stack([src[y:y + B, x:x + B] for _, y, x in g])
+        O = sf.dctn(grp, type=2, norm="ortho")
+        if not wiener:
+            keep = np.abs(O) > p.lambda_ * sigma
+            O = np.where(keep, O, 0.0)
+            r = int(keep.sum())
+            wgt = 1.0 if r < 1 else 1.0 / r
+        else:
+            Or = sf.dctn(np.stack([ref[y:y + B, x:x + B] for _, y, x in g]), type=2, norm="ortho")
+            zeta = np.zeros((n, B, B))
+            zeta[0] += 1
+            zeta[:, 0, :] += 1
+            zeta[:, :, 0] += 1
+            Wc = Or ** 2 / (Or ** 2 + sigma ** 2 / 2.0 ** zeta)
+            O = O * Wc
+            l2 = float((Wc ** 2).sum())
+            wgt = 1.0 if l2 <= 0 else 1.0 / l2
+        blk = sf.idctn(O, type=2, norm="ortho")
+        for z, (_, y, x) in enumerate(g):
+            num[y:y + B, x:x + B] += wgt * blk[z]
+            den[y:y + B, x:x + B] += wgt
+    return num, den
+
+
+def kernel(p, src_planes, ref_planes, wiener):
+    """BM3D_Base::Kernel semantics: match on ref_planes[0], filter each plane
+    with sigma > 0, others copied from src."""
+    groups = brute_groups(p, ref_planes[0])
+    out = []
+    for c, s in enumerate(src_planes):
+        if p.sigma[c] > 0:
+            num, den = filter_plane(p, s, ref_planes[c], groups, c, wiener)
+            out.append(num / den)
+        else:
+            out.append(np.asarray(s, np.float64).copy())
+    return out

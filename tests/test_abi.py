@@ -1,0 +1,38 @@
+"""The C-ABI library loads and exports every function include/isp_bm3d.h
+declares (no GPU needed)."""
+import ctypes
+import os
+import re
+
+from isp_mw_amd import _lib as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    txt = open(os.path.join(ROOT, "include", "isp_bm3d.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(isp_bm3d_\w+)\s*\(", txt)))
+
+
+def test_header_and_binding_agree():
+    assert declared() == sorted(L.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(L.LIB_PATH)
+    for name in declared():
+        assert hasattr(lib, name), name
+
+
+def test_null_and_bad_arguments():
+    lib = L.lib()
+    assert lib.isp_bm3d_plane_f32(None, None, None, None, 8, 8, 8, L.BOTH, L.MEM_HOST, None) == L.E_ARG
+    assert lib.isp_bm3d_create(None, None, 0) == L.E_ARG
+    assert lib.isp_bm3d_destroy(None) == L.OK
+    assert lib.isp_bm3d_last_error(None)
+
+
+def test_sizes_of_structs():
+    # 8 (profile) + 24 (sigma) + 5 ints (20) + pad 4 + thMSE 8 + lambda 8
+    assert ctypes.sizeof(L.StepParams) == 72 and ctypes.sizeof(L.Params) == 144

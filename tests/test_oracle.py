@@ -1,0 +1,219 @@
+"""Pins the FP64 oracle (oracle/bm3d_oracle.c) against what fixes it
+independently of its own code: library DCTs, brute-force matching, an
+orthonormal numpy formulation of the filter, closed forms and identities."""
+import numpy as np
+import pytest
+import scipy.fft as sf
+
+import oracle
+from isp_mw_amd import BM3D_Para, synth
+from tests import npref
+
+
+def para(profile="np", sigma=25.0, step=3, R=12, G1=16, G2=32, bmstep=1):
+    p = BM3D_Para(profile)
+    for st in (p.basic, p.final):
+        st.BlockStep, st.BMrange, st.BMstep = step, R, bmstep
+    p.basic.GroupSize, p.final.GroupSize = G1, G2
+    p.set_sigma(sigma)
+    return p
+
+
+# --- transforms: FFTW REDFT10 / REDFT01 == scipy DCT-II / DCT-III, norm=None
+@pytest.mark.parametrize("n0", list(range(1, 33)))
+def test_dct3d_matches_scipy(n0):
+    a = np.random.default_rng(n0).standard_normal((n0, 8, 8))
+    f = oracle.dct3d(a)
+    np.testing.assert_allclose(f, sf.dctn(a, type=2, norm=None), rtol=0, atol=1e-12 * np.abs(f).max())
+    i = oracle.dct3d(a, inverse=True)
+    np.testing.assert_allclose(i, sf.dctn(a, type=3, norm=None), rtol=0, atol=1e-12 * np.abs(i).max())
+    # REDFT01(REDFT10(x)) = finalAMP * x with finalAMP = 2N*2B*2B (source/BM3D.cpp:26)
+    rt = oracle.dct3d(f, inverse=True)
+    np.testing.assert_allclose(rt, 8 * n0 * 64 * a, rtol=0, atol=1e-10 * n0 * 512)
+
+
+def test_dct3d_odd_shapes():
+    a = np.random.default_rng(7).standard_normal((5, 12, 7))
+    np.testing.assert_allclose(oracle.dct3d(a), sf.dctn(a, type=2), atol=1e-11)
+
+
+def test_dct_known_answer():
+    # REDFT10 of a unit impulse at j: Y_k = 2 cos(pi (2j+1) k / (2n)) (long double)
+    n = 6
+    a = np.zeros((n, 1, 1))
+    a[2, 0, 0] = 1.0
+    y = oracle.dct3d(a)[:, 0, 0] / 4.0  # the two length-1 axes each multiply by 2
+    k = np.arange(n, dtype=np.longdouble)
+    ref = 2 * np.cos(np.pi * (2 * 2 + 1) * k / (2 * n))
+    np.testing.assert_allclose(y, ref.astype(np.float64), atol=1e-15)
+
+
+# --- grid and search window
+@pytest.mark.parametrize("n,s", [(512, 3), (1920, 3), (1080, 3), (37, 4), (8, 3), (9, 8), (100, 7), (64, 8)])
+def test_grid(n, s):
+    g = oracle.grid(n, 8, s)
+    assert list(g) == npref.grid(n, s)
+    last = n - 8
+    assert g[0] == 0 and g[-1] == last and np.all(np.diff(g) > 0) and np.all(np.diff(g) <= s)
+    np.testing.assert_array_equal(g, np.minimum(np.arange(len(g)) * s, last))
+
+
+def test_grid_sizes_of_workloads():
+    assert len(oracle.grid(512, 8, 3)) == 169
+    assert len(oracle.grid(1920, 8, 3)) * len(oracle.grid(1080, 8, 3)) == 229401
+
+
+def test_search_boundary_brute_force():
+    for step in (1, 2, 3):
+        for R in (4, 9, 12):
+            Ra = R // step * step
+            for bound in (0, 30):
+                for pos in range(0, 31):
+                    lo = oracle.search_boundary(pos, 0, R, step)
+                    hi = oracle.search_boundary(pos, bound, R, step)
+                    reach = [q for q in range(pos - Ra, pos + Ra + 1, step) if 0 <= q]
+                    assert lo == min(reach) if pos > 0 else lo == 0
+                    if pos < bound:
+                        assert hi == max(q for q in range(pos - Ra, pos + Ra + 1, step) if q <= bound)
+                    elif pos == bound:
+                        assert hi == bound
+
+
+# --- block matching vs brute force
+CASES = [((24, 20), dict(step=3, R=4)), ((37, 41), dict(step=3, R=6)), ((40, 32), dict(step=4, R=9, G1=8)),
+         ((33, 29), dict(step=2, R=6, bmstep=2)), ((30, 26), dict(step=3, R=5, G1=1))]
+
+
+@pytest.mark.parametrize("shape,kw", CASES)
+def test_match_vs_brute_force(shape, kw):
+    h, w = shape
+    _, n = synth.grey_pair(h, w, 25.0, h * 100 + w)
+    p = para(**kw)
+    counts, pos, keys = oracle.match_all(p.basic, n)
+    bf = npref.brute_groups(p.basic, n)
+    gy, gx = npref.grid(h, p.basic.BlockStep), npref.grid(w, p.basic.BlockStep)
+    assert counts.shape == (len(gy), len(gx))
+    for a, j in enumerate(gy):
+        for b, i in enumerate(gx):
+            g = bf[(j, i)]
+            N = counts[a, b]
+            assert N == len(g)
+            got = [(int(v) >> 16, int(v) & 0xFFFF) for v in pos[a, b, :N]]
+            assert got == [(y, x) for _, y, x in g]
+            np.testing.assert_allclose(keys[a, b, :N], [k for k, _, _ in g], rtol=1e-12, atol=1e-12)
+
+
+def test_match_invariants():
+    _, n = synth.grey_pair(64, 72, 25.0, 3)
+    p = para()
+    counts, pos, keys = oracle.match_all(p.final, n)
+    gy, gx = oracle.grid(64, 8, 3), oracle.grid(72, 8, 3)
+    thkey = p.final.thMSE * np.sqrt(1 / 3)
+    for a, j in enumerate(gy):
+        for b, i in enumerate(gx):
+            N = counts[a, b]
+            assert 1 <= N <= p.final.GroupSize
+            assert pos[a, b, 0] == (j << 16) | i and keys[a, b, 0] == 0
+            k = keys[a, b, 1:N]
+            assert np.all(k > 0) and np.all(k <= thkey * (1 + 1e-12)) and np.all(np.diff(k) >= 0)
+            ys, xs = pos[a, b, :N] >> 16, pos[a, b, :N] & 0xFFFF
+            assert np.all(np.abs(ys - j) <= 12) and np.all(np.abs(xs - i) <= 12)
+            assert np.all(ys <= 64 - 8) and np.all(xs <= 72 - 8)
+            assert len(set(zip(ys.tolist(), xs.tolist()))) == N
+
+
+# --- collaborative filter and aggregation vs the orthonormal numpy formulation
+@pytest.mark.parametrize("wiener", [False, True])
+@pytest.mark.parametrize("shape,kw", [((24, 32), dict(step=3, R=4, G1=8, G2=8)),
+                                      ((29, 27), dict(step=2, R=5, G1=16, G2=16, sigma=40.0))])
+def test_kernel_vs_orthonormal_numpy(shape, kw, wiener):
+    h, w = shape
+    clean, n = synth.grey_pair(h, w, kw.get("sigma", 25.0), 5)
+    p = para(**kw)
+    st = p.final if wiener else p.basic
+    ref = clean.astype(np.float32) if wiener else n  # Wiener guide: a clean-ish estimate
+    got = oracle.kernel(st, n, ref, wiener=wiener)
+    want = npref.kernel(st, [n], [ref], wiener)[0]
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-10)
+
+
+def test_kernel_three_planes_vs_numpy():
+    c, n = synth.rgb_pair(24, 28, 30.0, 9)
+    p = para(step=3, R=4, G1=8, G2=8)
+    yuv = oracle.rgb2opp(n)
+    got = oracle.kernel(p.basic, list(yuv), list(yuv), wiener=False)
+    want = npref.kernel(p.basic, list(yuv), list(yuv), False)
+    for c_ in range(3):
+        np.testing.assert_allclose(got[c_], want[c_], atol=1e-10)
+
+
+def test_constant_image_closed_form():
+    # A constant image matches nothing (every SSE is 0, include/Block.h:608),
+    # so every group is the lone reference block (N = 1).  Step 1 keeps the
+    # DC (8c > lambda*sigma) and reproduces c exactly; step 2 scales it by the
+    # Wiener factor of the DC, W = 64c^2 / (64c^2 + sigma^2 / 2^3).
+    p = para()
+    c = 0.4
+    src = np.full((40, 48), c)
+    counts, _, _ = oracle.match_all(p.final, src)
+    assert counts.max() == 1
+    np.testing.assert_allclose(oracle.kernel(p.basic, src, src, wiener=False), c, atol=1e-15)
+    s = 25 / 255 * np.sqrt(1 / 3)
+    W = 64 * c * c / (64 * c * c + s * s / 8)
+    np.testing.assert_allclose(oracle.kernel(p.final, src, src, wiener=True), c * W, rtol=1e-13)
+    # Two steps: the basic estimate carries ~1e-17 rounding, so step-2 SSEs
+    # are no longer exactly 0 and groups can fill; the Wiener DC factor then
+    # lies between the N = 1 value and 1.
+    two = oracle.bm3d_plane(p.basic, p.final, src)
+    assert np.ptp(two) < 1e-13 and c * W - 1e-15 <= two.min() and two.max() <= c
+    rgb = np.stack([np.full((40, 48), v) for v in (0.2, 0.5, 0.7)])
+    out = oracle.bm3d_rgb(p.basic, p.final, rgb)
+    assert np.ptp(out, axis=(1, 2)).max() < 1e-13  # stays spatially constant
+    np.testing.assert_allclose(out[:, 0, 0], [0.2, 0.5, 0.7], atol=1e-4)
+
+
+def test_tiny_sigma_is_identity():
+    _, n = synth.grey_pair(40, 40, 25.0, 2)
+    p = para(sigma=1e-9)
+    np.testing.assert_allclose(oracle.kernel(p.basic, n, n, wiener=False), n, atol=1e-12)
+    np.testing.assert_allclose(oracle.kernel(p.final, n, n, wiener=True), n, atol=1e-9)
+
+
+def test_degenerate_params_copy_through():
+    _, n = synth.grey_pair(32, 32, 25.0, 2)
+    p = para(sigma=0.0)
+    np.testing.assert_array_equal(oracle.kernel(p.basic, n, n, wiener=False), n.astype(np.float64))
+    p = para()
+    p.basic.GroupSize = 0
+    np.testing.assert_array_equal(oracle.kernel(p.basic, n, n, wiener=False), n.astype(np.float64))
+
+
+def test_denoises():
+    c, n = synth.grey_pair(96, 96, 25.0, 4)
+    p = para(sigma=25.0 / np.sqrt(1 / 3))  # effective sigma 25 after the normY scaling
+    out = oracle.bm3d_plane(p.basic, p.final, n)
+    assert synth.psnr(out, c) > synth.psnr(n, c) + 5
+
+
+# --- OPP colour
+def test_opp_matrices_and_inverse():
+    rgb = np.random.default_rng(1).uniform(0, 1, (3, 5, 7))
+    yuv = oracle.rgb2opp(rgb)
+    r, g, b = rgb
+    np.testing.assert_allclose(yuv[0], (r + g + b) / 3, atol=1e-15)
+    np.testing.assert_allclose(yuv[1], (r - b) / 2, atol=1e-15)
+    np.testing.assert_allclose(yuv[2], (r - 2 * g + b) / 4, atol=1e-15)
+    np.testing.assert_allclose(oracle.opp2rgb(yuv, clip=False), rgb, atol=1e-14)
+    out = oracle.opp2rgb(np.stack([np.full((2, 2), 1.2), np.zeros((2, 2)), np.zeros((2, 2))]), clip=True)
+    assert out.max() == 1.0
+
+
+def test_threads_agree():
+    c, n = synth.rgb_pair(60, 70, 25.0, 8)
+    p = para()
+    oracle.set_threads(1)
+    a = oracle.bm3d_rgb(p.basic, p.final, n)
+    oracle.set_threads(4)
+    b = oracle.bm3d_rgb(p.basic, p.final, n)
+    oracle.set_threads(1)
+    np.testing.assert_allclose(a, b, atol=1e-12)
