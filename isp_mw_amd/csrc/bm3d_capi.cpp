@@ -24,6 +24,8 @@ struct isp_bm3d_ctx {
     int precision = ISP_BM3D_PREC_FP32;
     float *d_tables = nullptr;  // C8 (64) then the padded C_N matrices
     int cn_off[kMaxGroup + 1] = {0};
+    double *d_tables64 = nullptr;  // C8 (64) then the unpadded C_N matrices, FP64
+    int cnd_off[kMaxGroup + 1] = {0};
     void *ws = nullptr;
     size_t ws_bytes = 0;
     std::string err;
@@ -147,6 +149,8 @@ struct Derived {
     bool bypass;
     double thSSE, distMul;
     float thr[3], sig2[3];
+    double thr64[3];
+    float band[3];
     int active[3];
     int ngx, ngy;
 };
@@ -183,6 +187,10 @@ int derive(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, int nplanes, int W,
         const double sigma = p.sigma[c] / 255.0 * norm[c];
         d.thr[c] = (float)(sigma * p.lambda);
         d.sig2[c] = (float)(sigma * sigma);
+        d.thr64[c] = sigma * p.lambda;
+        // FP32 coefficients of [0,1] data carry absolute errors of a few 1e-6;
+        // decisions closer than this to the threshold are re-made in FP64.
+        d.band[c] = (float)(3e-5 + 1e-4 * d.thr64[c]);
     }
     d.ngx = (W - kBlock) / p.block_step + 1 + ((W - kBlock) % p.block_step ? 1 : 0);
     d.ngy = (H - kBlock) / p.block_step + 1 + ((H - kBlock) % p.block_step ? 1 : 0);
@@ -380,7 +388,16 @@ int run_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, cons
         f.FH = (kTileRY - 1) * p.block_step + 2 * d.range + kBlock;
         f.c8 = ctx->d_tables;
         f.cn = ctx->d_tables;
-        for (int n = 0; n <= kMaxGroup; ++n) f.cn_off[n] = ctx->cn_off[n];
+        for (int n = 0; n <= kMaxGroup; ++n) {
+            f.cn_off[n] = ctx->cn_off[n];
+            f.cnd_off[n] = ctx->cnd_off[n];
+        }
+        f.c8d = ctx->d_tables64;
+        f.cnd = ctx->d_tables64;
+        for (int c = 0; c < 3; ++c) {
+            f.thr64[c] = d.thr64[c];
+            f.band[c] = d.band[c];
+        }
         if (filter_lds_bytes(f, wiener) > 160 * 1024)
             return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "parameters need %zu bytes of LDS for filtering",
                         filter_lds_bytes(f, wiener));
@@ -524,12 +541,28 @@ int isp_bm3d_create(isp_bm3d_ctx **out, const isp_bm3d_params *params, int devic
                     (float)((k == 0 ? std::sqrt(1.0L / N) : std::sqrt(2.0L / N)) *
                             std::cos(PI * (2 * z + 1) * k / (2.0L * N)));
     }
+    std::vector<double> tab64(64);
+    for (int k = 0; k < 8; ++k)
+        for (int n = 0; n < 8; ++n)
+            tab64[k * 8 + n] = (double)((k == 0 ? std::sqrt(1.0L / 8) : std::sqrt(2.0L / 8)) *
+                                        std::cos(PI * (2 * n + 1) * k / 16.0L));
+    for (int N = 1; N <= kMaxGroup; ++N) {
+        ctx->cnd_off[N] = (int)tab64.size();
+        for (int k = 0; k < N; ++k)
+            for (int z = 0; z < N; ++z)
+                tab64.push_back((double)((k == 0 ? std::sqrt(1.0L / N) : std::sqrt(2.0L / N)) *
+                                         std::cos(PI * (2 * z + 1) * k / (2.0L * N))));
+    }
     if ((e = hipSetDevice(device)) != hipSuccess ||
         (e = hipMalloc(&ctx->d_tables, tab.size() * sizeof(float))) != hipSuccess ||
         (e = hipMemcpy(ctx->d_tables, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice)) !=
+            hipSuccess ||
+        (e = hipMalloc(&ctx->d_tables64, tab64.size() * sizeof(double))) != hipSuccess ||
+        (e = hipMemcpy(ctx->d_tables64, tab64.data(), tab64.size() * sizeof(double), hipMemcpyHostToDevice)) !=
             hipSuccess) {
         int rc = fail(nullptr, ISP_BM3D_E_HIP, "create: %s", hipGetErrorString(e));
         if (ctx->d_tables) (void)hipFree(ctx->d_tables);
+        if (ctx->d_tables64) (void)hipFree(ctx->d_tables64);
         delete ctx;
         return rc;
     }
@@ -591,6 +624,7 @@ int isp_bm3d_destroy(isp_bm3d_ctx *ctx)
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     if (ctx->ws) (void)hipFree(ctx->ws);
     if (ctx->d_tables) (void)hipFree(ctx->d_tables);
+    if (ctx->d_tables64) (void)hipFree(ctx->d_tables64);
     delete ctx;
     return ISP_BM3D_OK;
 }
@@ -756,8 +790,10 @@ int isp_bm3d_rgb_f32(isp_bm3d_ctx *ctx, float *dst_r, float *dst_g, float *dst_b
     o.pitch = P;
     o.y64 = w.y64;
     o.pitch64 = P;
-    StageTimer tm_opp(ctx, ST_OPP, s);
-    HIP_CHECK(ctx, launch_opp(o, s));
+    {
+        StageTimer tm_opp(ctx, ST_OPP, s);
+        HIP_CHECK(ctx, launch_opp(o, s));
+    }
     if (sep_ref) {
         OppArgs q = o;
         q.r = rrgb[0];

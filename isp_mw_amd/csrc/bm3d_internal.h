@@ -52,6 +52,13 @@ struct FilterArgs {
     const float *c8;          // 8x8 orthonormal DCT-II matrix, [k][n]
     const float *cn;          // per-N orthonormal DCT-II matrices, padded to NP x NP
     int cn_off[kMaxGroup + 1];
+    // Hard-threshold decisions within `band` of the threshold are re-made
+    // from the coefficient recomputed in FP64 (FP64 matrices, unpadded N x N).
+    const double *c8d;
+    const double *cnd;
+    int cnd_off[kMaxGroup + 1];
+    double thr64[3];
+    float band[3];
 };
 
 struct FinalizeArgs {

@@ -16,7 +16,6 @@
 namespace ispbm3d {
 namespace {
 
-constexpr int ZS = 64;  // z-stride of a group staged in LDS (floats)
 
 __device__ __forceinline__ void wave_sync()
 {
@@ -336,11 +335,20 @@ __global__ __launch_bounds__(kThreads) void match_kernel(const GT *__restrict__ 
 // reference's thresholds become a uniform lambda*sigma and its Wiener noise
 // sigma^2*finalAMP becomes sigma^2/2^zeta (zeta = number of zero indices), and
 // the reconstruction needs no finalAMP division.
+//
+// A group is staged per wave in LDS as rows of kRow = 9 floats (element
+// (z, u, v) at z*ZS + u*kRow + v): the odd row stride keeps the row pass, the
+// column pass and the per-pixel group-axis pass free of bank conflicts.
+constexpr int kRow = kBlock + 1;
+constexpr int ZS = kBlock * kRow;  // 72 floats per block
+
+__device__ __forceinline__ int gidx(int z, int pix) { return z * ZS + (pix >> 3) * kRow + (pix & 7); }
+
 __device__ __forceinline__ void rows_pass(float *g, int N, const float *__restrict__ c8, int lane,
                                           bool inverse)
 {
     for (int r = lane; r < N * kBlock; r += 64) {
-        float *p = g + (r >> 3) * ZS + (r & 7) * kBlock;
+        float *p = g + r * kRow;  // row (z, u) = r
         float in[kBlock];
 #pragma unroll
         for (int v = 0; v < kBlock; ++v) in[v] = p[v];
@@ -359,60 +367,133 @@ __device__ __forceinline__ void cols_pass(float *g, int N, const float *__restri
                                           bool inverse)
 {
     for (int r = lane; r < N * kBlock; r += 64) {
-        float *p = g + (r >> 3) * ZS + (r & 7);
+        float *p = g + (r >> 3) * ZS + (r & 7);  // column (z, v)
         float in[kBlock];
 #pragma unroll
-        for (int u = 0; u < kBlock; ++u) in[u] = p[u * kBlock];
+        for (int u = 0; u < kBlock; ++u) in[u] = p[u * kRow];
 #pragma unroll
         for (int k = 0; k < kBlock; ++k) {
             float s = 0.f;
 #pragma unroll
             for (int u = 0; u < kBlock; ++u)
                 s = fmaf(inverse ? c8[u * kBlock + k] : c8[k * kBlock + u], in[u], s);
-            p[k * kBlock] = s;
+            p[k * kRow] = s;
         }
     }
 }
 
 // Group-axis DCT of the lane's pixel column, in place in LDS: the column is
 // held in registers and each output row is written back as soon as it is
-// formed.  inverse = 1 applies the transpose.
+// formed.  inverse applies the transpose.
 template <int NP>
 __device__ __forceinline__ void zpass(float *g, int N, int lane, const float *__restrict__ cn, bool inverse)
 {
     float x[NP];
+    const int o = gidx(0, lane);
 #pragma unroll
-    for (int z = 0; z < NP; ++z) x[z] = z < N ? g[z * ZS + lane] : 0.f;
+    for (int z = 0; z < NP; ++z) x[z] = z < N ? g[o + z * ZS] : 0.f;
 #pragma unroll 1
     for (int k = 0; k < N; ++k) {
         float s = 0.f;
 #pragma unroll
         for (int z = 0; z < NP; ++z) s = fmaf(inverse ? cn[z * NP + k] : cn[k * NP + z], x[z], s);
-        g[k * ZS + lane] = s;
+        g[o + k * ZS] = s;
     }
 }
 
-template <int NP, bool WIENER>
-__device__ __forceinline__ float group_core(float *g, float *gr, int N, int lane,
-                                            const float *__restrict__ c8,
-                                            const float *__restrict__ cn, float thr, float sig2)
+__device__ __forceinline__ double wave_sum(double v)
 {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// One 3D orthonormal DCT coefficient (k, u, v) of a group, in FP64 from the
+// original samples, computed by the whole wave (lane = pixel (y, x)):
+// sum_{z,y,x} CN[k][z] C8[u][y] C8[v][x] X[z][y][x].  Result in every lane.
+__device__ __noinline__ double exact_coef_wave(int k, int u, int v, int N, int lane,
+                                               const int *__restrict__ pos, const float *__restrict__ src,
+                                               int pitch, const double *__restrict__ c8d,
+                                               const double *__restrict__ cnd)
+{
+    const int y = lane >> 3, x = lane & 7;
+    double s = 0.0;
+    for (int z = 0; z < N; ++z) {
+        const int pz = pos[z];
+        s = fma(cnd[k * N + z], (double)src[(size_t)((pz >> 16) + y) * pitch + (pz & 0xFFFF) + x], s);
+    }
+    return wave_sum(s * c8d[u * kBlock + y] * c8d[v * kBlock + x]);
+}
+
+struct ExactCtx {
+    const int *pos;
+    const float *src;
+    int pitch;
+    const double *c8d;
+    const double *cnd;  // N x N for this group
+    double thr64;
+    float band;
+};
+
+__device__ __forceinline__ void gather(float *g, const float *__restrict__ plane, int pitch,
+                                       const int *__restrict__ pos, int N, int lane)
+{
+    const int u = lane >> 3, v = lane & 7;
+    const int o = gidx(0, lane);
+    for (int z = 0; z < N; ++z) {
+        const int pz = pos[z];
+        g[o + z * ZS] = plane[(size_t)((pz >> 16) + u) * pitch + (pz & 0xFFFF) + v];
+    }
+}
+
+// Forward transform, shrinkage and inverse of one group of one plane; the
+// filtered blocks are left in g.  Returns the group's aggregation weight.
+template <int NP, bool WIENER>
+__device__ __forceinline__ float group_core(float *g, int N, int lane, const float *__restrict__ c8,
+                                            const float *__restrict__ cn, const float *__restrict__ src,
+                                            const float *__restrict__ guide, int pitch, float thr, float sig2,
+                                            const ExactCtx &ex)
+{
+    const int o = gidx(0, lane);
+    float yr[WIENER ? NP : 1];
+    if (WIENER) {
+        // the guide group first; its coefficients stay in registers
+        gather(g, guide, pitch, ex.pos, N, lane);
+        wave_sync();
+        rows_pass(g, N, c8, lane, false);
+        wave_sync();
+        cols_pass(g, N, c8, lane, false);
+        wave_sync();
+        zpass<NP>(g, N, lane, cn, false);
+#pragma unroll
+        for (int k = 0; k < NP; ++k) yr[k] = k < N ? g[o + k * ZS] : 0.f;
+        wave_sync();
+    }
+    gather(g, src, pitch, ex.pos, N, lane);
+    wave_sync();
     rows_pass(g, N, c8, lane, false);
-    if (WIENER) rows_pass(gr, N, c8, lane, false);
     wave_sync();
     cols_pass(g, N, c8, lane, false);
-    if (WIENER) cols_pass(gr, N, c8, lane, false);
     wave_sync();
     zpass<NP>(g, N, lane, cn, false);
-    if (WIENER) zpass<NP>(gr, N, lane, cn, false);
 
     // shrinkage on the lane's own coefficients (no cross-lane dependency)
     float weight;
     if (!WIENER) {
         int kept = 0;
         for (int k = 0; k < N; ++k) {
-            const float y = g[k * ZS + lane];
-            if (fabsf(y) <= thr) g[k * ZS + lane] = 0.f;
+            const float ay = fabsf(g[o + k * ZS]);
+            bool zero = ay <= thr;
+            // decisions too close to call in FP32 are re-made in FP64
+            uint64_t close = __ballot(fabsf(ay - thr) <= ex.band);
+            while (close) {
+                const int L = __ffsll((unsigned long long)close) - 1;
+                close &= close - 1;
+                const double oc = exact_coef_wave(k, L >> 3, L & 7, N, lane, ex.pos, ex.src, ex.pitch,
+                                                  ex.c8d, ex.cnd);
+                if (lane == L) zero = fabs(oc) <= ex.thr64;
+            }
+            if (zero) g[o + k * ZS] = 0.f;
             else ++kept;
         }
         kept = wave_sum(kept);
@@ -421,12 +502,14 @@ __device__ __forceinline__ float group_core(float *g, float *gr, int N, int lane
         const int u = lane >> 3, v = lane & 7;
         const float s2 = sig2 * (u == 0 ? 0.5f : 1.f) * (v == 0 ? 0.5f : 1.f);
         float l2 = 0.f;
-        for (int k = 0; k < N; ++k) {
-            const float yr = gr[k * ZS + lane];
-            const float yy = yr * yr;
-            const float wc = yy / (yy + (k == 0 ? 0.5f * s2 : s2));
-            g[k * ZS + lane] *= wc;
-            l2 = fmaf(wc, wc, l2);
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+            if (k < N) {
+                const float yy = yr[k] * yr[k];
+                const float wc = yy / (yy + (k == 0 ? 0.5f * s2 : s2));
+                g[o + k * ZS] *= wc;
+                l2 = fmaf(wc, wc, l2);
+            }
         }
         l2 = wave_sum(l2);
         weight = l2 <= 0.f ? 1.f : 1.f / l2;
@@ -440,12 +523,22 @@ __device__ __forceinline__ float group_core(float *g, float *gr, int N, int lane
     return weight;
 }
 
-__host__ __device__ inline size_t filter_lds(int FW, int FH, int G, bool wiener)
+// LDS of the filter kernel: accumulators of the tile footprint (num per pixel,
+// den per block origin) and one group buffer per wave; the row box sums of
+// the flush reuse the group buffers.
+__host__ __device__ inline size_t filter_acc_floats(int FW, int FH)
 {
     const int DW = FW - kBlock + 1, DH = FH - kBlock + 1;
-    size_t fl = (size_t)FW * FH + (size_t)DW * DH + (size_t)DH * FW;
-    fl = (fl + 3) & ~size_t(3);
-    return fl * 4 + (size_t)kWaves * (wiener ? 2 : 1) * G * ZS * 4;
+    return (((size_t)FW * FH + (size_t)DW * DH) + 3) & ~size_t(3);
+}
+
+__host__ __device__ inline size_t filter_lds(int FW, int FH, int G)
+{
+    const int DH = FH - kBlock + 1;
+    size_t groups = (size_t)kWaves * G * ZS;
+    const size_t hs = (size_t)DH * FW;
+    if (hs > groups) groups = hs;
+    return (filter_acc_floats(FW, FH) + groups) * 4;
 }
 
 template <bool WIENER>
@@ -460,11 +553,9 @@ __global__ __launch_bounds__(kThreads) void filter_kernel(FilterArgs a)
 
     float *accNum = reinterpret_cast<float *>(smem);
     float *accDen = accNum + FW * FH;
-    float *hsum = accDen + DW * DH;
-    size_t fl = (size_t)FW * FH + (size_t)DW * DH + (size_t)DH * FW;
-    fl = (fl + 3) & ~size_t(3);
-    float *g = accNum + fl + (size_t)wave * (WIENER ? 2 : 1) * a.G * ZS;
-    float *gr = g + a.G * ZS;
+    float *gbase = accNum + filter_acc_floats(FW, FH);
+    float *g = gbase + (size_t)wave * a.G * ZS;
+    float *hsum = gbase;  // flush only, after every wave is done with its groups
     const int u = lane >> 3, v = lane & 7;
 
     for (int c = 0; c < 3; ++c) {
@@ -480,25 +571,21 @@ __global__ __launch_bounds__(kThreads) void filter_kernel(FilterArgs a)
             const size_t refIdx = (size_t)gyi * a.ngx + gxi;
             const int *pos = a.pos + refIdx * a.G;
             const int N = a.cnt[refIdx];
-            for (int z = 0; z < N; ++z) {
-                const int pz = pos[z];
-                const size_t off = (size_t)((pz >> 16) + u) * a.pitch + (pz & 0xFFFF) + v;
-                g[z * ZS + lane] = src[off];
-                if (WIENER) gr[z * ZS + lane] = gsrc[off];
-            }
-            wave_sync();
             const float thr = a.thr[c], sig2 = a.sig2[c];
+            const ExactCtx ex{pos, src, a.pitch, a.c8d, a.cnd + a.cnd_off[N], a.thr64[c], a.band[c]};
+            const float *cn = a.cn + a.cn_off[N];
             float w;
-            if (N <= 1) w = group_core<1, WIENER>(g, gr, N, lane, a.c8, a.cn + a.cn_off[N], thr, sig2);
-            else if (N <= 2) w = group_core<2, WIENER>(g, gr, N, lane, a.c8, a.cn + a.cn_off[N], thr, sig2);
-            else if (N <= 4) w = group_core<4, WIENER>(g, gr, N, lane, a.c8, a.cn + a.cn_off[N], thr, sig2);
-            else if (N <= 8) w = group_core<8, WIENER>(g, gr, N, lane, a.c8, a.cn + a.cn_off[N], thr, sig2);
-            else if (N <= 16) w = group_core<16, WIENER>(g, gr, N, lane, a.c8, a.cn + a.cn_off[N], thr, sig2);
-            else w = group_core<32, WIENER>(g, gr, N, lane, a.c8, a.cn + a.cn_off[N], thr, sig2);
+            if (N <= 1) w = group_core<1, WIENER>(g, N, lane, a.c8, cn, src, gsrc, a.pitch, thr, sig2, ex);
+            else if (N <= 2) w = group_core<2, WIENER>(g, N, lane, a.c8, cn, src, gsrc, a.pitch, thr, sig2, ex);
+            else if (N <= 4) w = group_core<4, WIENER>(g, N, lane, a.c8, cn, src, gsrc, a.pitch, thr, sig2, ex);
+            else if (N <= 8) w = group_core<8, WIENER>(g, N, lane, a.c8, cn, src, gsrc, a.pitch, thr, sig2, ex);
+            else if (N <= 16) w = group_core<16, WIENER>(g, N, lane, a.c8, cn, src, gsrc, a.pitch, thr, sig2, ex);
+            else w = group_core<32, WIENER>(g, N, lane, a.c8, cn, src, gsrc, a.pitch, thr, sig2, ex);
+            const int o = gidx(0, lane);
             for (int z = 0; z < N; ++z) {
                 const int pz = pos[z];
                 const int py = (pz >> 16) - oy, px = (pz & 0xFFFF) - ox;
-                atomicAdd(&accNum[(py + u) * FW + px + v], w * g[z * ZS + lane]);
+                atomicAdd(&accNum[(py + u) * FW + px + v], w * g[o + z * ZS]);
             }
             if (lane < N) {
                 const int pz = pos[lane];
@@ -628,7 +715,7 @@ size_t match_lds_bytes(const MatchArgs &a)
     return align16((size_t)a.tileH * a.tilePitch * 8) + (size_t)kWaves * match_wave_bytes(a.maxCand);
 }
 
-size_t filter_lds_bytes(const FilterArgs &a, bool wiener) { return filter_lds(a.FW, a.FH, a.G, wiener); }
+size_t filter_lds_bytes(const FilterArgs &a, bool) { return filter_lds(a.FW, a.FH, a.G); }
 
 hipError_t launch_match_f32(const float *guide, int gpitch, const MatchArgs &a, hipStream_t s)
 {
