@@ -1,0 +1,245 @@
+// isp_bm3d.hpp - header-only C++ layer over the C ABI (isp_bm3d.h) with the
+// class names of ISP_MW's BM3D interface, so that reference-style code reads
+// the same:
+//
+//     isp_bm3d::BM3D_Para para("np");             // BM3D_Para, include/BM3D.h:154-168
+//     para.basic.sigma = {25, 25, 25};
+//     para.thMSE_Default();
+//     isp_bm3d::BM3D filter(para);                // BM3D, include/BM3D.h:337-357
+//     isp_bm3d::Plane_FL dst = filter(src);       // FilterIF2::operator(), include/Filter.h:211-216
+//
+// Plane_FL holds float samples (luma range [0,1]) with stride == width, like
+// the reference's Plane_FL (include/Image_Type.h:186-290).  Frame_FL holds
+// three float RGB planes in [0,1]; it stands in for the reference's integer
+// Frame (include/Image_Type.h:294-403) with the same process_Frame semantics.
+// Errors throw isp_bm3d::Error (the reference prints and exits instead).
+#pragma once
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "isp_bm3d.h"
+
+namespace isp_bm3d {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+inline void check(int rc, const isp_bm3d_ctx *ctx = nullptr)
+{
+    if (rc < 0) throw Error(rc, isp_bm3d_last_error(ctx));
+}
+
+// BM3D_Para_Base (include/BM3D.h:15-45) and its two step presets.
+struct BM3D_Para_Base {
+    std::string profile;
+    std::vector<double> sigma;
+    int BlockSize, BlockStep, GroupSize, BMrange, BMstep;
+    double thMSE, lambda;
+    bool final_step;
+
+    BM3D_Para_Base(const std::string &prof, bool fin) : profile(prof), final_step(fin)
+    {
+        isp_bm3d_step_params p;
+        check(isp_bm3d_step_params_default(&p, prof.c_str(), fin ? 1 : 0));
+        from_c(p);
+    }
+    void thMSE_Default()
+    {
+        isp_bm3d_step_params p = to_c();
+        check(isp_bm3d_thmse_default(&p, final_step ? 1 : 0));
+        thMSE = p.thMSE;
+    }
+    isp_bm3d_step_params to_c() const
+    {
+        isp_bm3d_step_params p{};
+        profile.copy(p.profile, sizeof(p.profile) - 1);
+        for (int c = 0; c < 3; ++c)
+            p.sigma[c] = sigma.empty() ? 0.0 : sigma[c < (int)sigma.size() ? c : sigma.size() - 1];
+        p.block_size = BlockSize;
+        p.block_step = BlockStep;
+        p.group_size = GroupSize;
+        p.bm_range = BMrange;
+        p.bm_step = BMstep;
+        p.thMSE = thMSE;
+        p.lambda = lambda;
+        return p;
+    }
+    void from_c(const isp_bm3d_step_params &p)
+    {
+        sigma.assign(p.sigma, p.sigma + 3);
+        BlockSize = p.block_size;
+        BlockStep = p.block_step;
+        GroupSize = p.group_size;
+        BMrange = p.bm_range;
+        BMstep = p.bm_step;
+        thMSE = p.thMSE;
+        lambda = p.lambda;
+    }
+};
+
+struct BM3D_Basic_Para : BM3D_Para_Base {
+    explicit BM3D_Basic_Para(const std::string &profile = "fast") : BM3D_Para_Base(profile, false) {}
+};
+
+struct BM3D_Final_Para : BM3D_Para_Base {
+    explicit BM3D_Final_Para(const std::string &profile = "fast") : BM3D_Para_Base(profile, true) {}
+};
+
+struct BM3D_Para {
+    BM3D_Basic_Para basic;
+    BM3D_Final_Para final;
+    explicit BM3D_Para(const std::string &profile = "fast") : basic(profile), final(profile) {}
+    void thMSE_Default()
+    {
+        basic.thMSE_Default();
+        final.thMSE_Default();
+    }
+};
+
+class Plane_FL {
+public:
+    Plane_FL() = default;
+    Plane_FL(int width, int height, float value = 0.f)
+        : w_(width), h_(height), data_((size_t)width * height, value) {}
+    int Width() const { return w_; }
+    int Height() const { return h_; }
+    int Stride() const { return w_; }
+    float *data() { return data_.data(); }
+    const float *data() const { return data_.data(); }
+    float &operator()(int y, int x) { return data_[(size_t)y * w_ + x]; }
+    float operator()(int y, int x) const { return data_[(size_t)y * w_ + x]; }
+
+private:
+    int w_ = 0, h_ = 0;
+    std::vector<float> data_;
+};
+
+class Frame_FL {
+public:
+    Frame_FL() = default;
+    Frame_FL(int width, int height) : p_{Plane_FL(width, height), Plane_FL(width, height), Plane_FL(width, height)} {}
+    Plane_FL &R() { return p_[0]; }
+    Plane_FL &G() { return p_[1]; }
+    Plane_FL &B() { return p_[2]; }
+    const Plane_FL &R() const { return p_[0]; }
+    const Plane_FL &G() const { return p_[1]; }
+    const Plane_FL &B() const { return p_[2]; }
+    int Width() const { return p_[0].Width(); }
+    int Height() const { return p_[0].Height(); }
+
+private:
+    Plane_FL p_[3];
+};
+
+// FilterIF2-style wrapper (include/Filter.h:138-217) around one context.
+class FilterBase {
+public:
+    FilterBase(const FilterBase &) = delete;
+    FilterBase &operator=(const FilterBase &) = delete;
+    ~FilterBase() { isp_bm3d_destroy(ctx_); }
+
+    Plane_FL &process(Plane_FL &dst, const Plane_FL &src, const Plane_FL &ref)
+    {
+        size_check(dst, src);
+        size_check(ref, src);
+        check(isp_bm3d_plane_f32(ctx_, dst.data(), src.data(), ref.data(), src.Width(), src.Height(), src.Stride(),
+                                 mode_, ISP_BM3D_MEM_HOST, nullptr),
+              ctx_);
+        return dst;
+    }
+    Plane_FL &process(Plane_FL &dst, const Plane_FL &src) { return process(dst, src, src); }
+    Frame_FL &process(Frame_FL &dst, const Frame_FL &src, const Frame_FL &ref)
+    {
+        size_check(dst.R(), src.R());
+        size_check(ref.R(), src.R());
+        check(isp_bm3d_rgb_f32(ctx_, dst.R().data(), dst.G().data(), dst.B().data(), src.R().data(), src.G().data(),
+                               src.B().data(), ref.R().data(), ref.G().data(), ref.B().data(), src.Width(),
+                               src.Height(), src.R().Stride(), mode_, ISP_BM3D_MEM_HOST, nullptr),
+              ctx_);
+        return dst;
+    }
+    Frame_FL &process(Frame_FL &dst, const Frame_FL &src) { return process(dst, src, src); }
+
+    template <typename T>
+    T operator()(const T &src)
+    {
+        T dst(src.Width(), src.Height());
+        process(dst, src);
+        return dst;
+    }
+    template <typename T>
+    T operator()(const T &src, const T &ref)
+    {
+        T dst(src.Width(), src.Height());
+        process(dst, src, ref);
+        return dst;
+    }
+    isp_bm3d_ctx *context() { return ctx_; }
+
+protected:
+    FilterBase(const isp_bm3d_params &p, int mode, int device) : mode_(mode)
+    {
+        check(isp_bm3d_create(&ctx_, &p, device));
+    }
+    static void size_check(const Plane_FL &a, const Plane_FL &b)
+    {
+        if (a.Width() != b.Width() || a.Height() != b.Height())
+            throw Error(ISP_BM3D_E_ARG, "planes of different sizes");
+    }
+    isp_bm3d_ctx *ctx_ = nullptr;
+    int mode_;
+};
+
+// BM3D_Basic (include/BM3D.h:287-309)
+class BM3D_Basic : public FilterBase {
+public:
+    explicit BM3D_Basic(const BM3D_Basic_Para &para = BM3D_Basic_Para(), int device = 0)
+        : FilterBase(make(para), ISP_BM3D_BASIC, device) {}
+
+private:
+    static isp_bm3d_params make(const BM3D_Basic_Para &para)
+    {
+        isp_bm3d_params p;
+        check(isp_bm3d_params_default(&p, para.profile.c_str()));
+        p.basic = para.to_c();
+        return p;
+    }
+};
+
+// BM3D_Final (include/BM3D.h:312-334)
+class BM3D_Final : public FilterBase {
+public:
+    explicit BM3D_Final(const BM3D_Final_Para &para = BM3D_Final_Para(), int device = 0)
+        : FilterBase(make(para), ISP_BM3D_FINAL, device) {}
+
+private:
+    static isp_bm3d_params make(const BM3D_Final_Para &para)
+    {
+        isp_bm3d_params p;
+        check(isp_bm3d_params_default(&p, para.profile.c_str()));
+        p.final_ = para.to_c();
+        return p;
+    }
+};
+
+// BM3D (include/BM3D.h:337-357)
+class BM3D : public FilterBase {
+public:
+    explicit BM3D(const BM3D_Para &para = BM3D_Para(), int device = 0)
+        : FilterBase(make(para), ISP_BM3D_BOTH, device) {}
+
+private:
+    static isp_bm3d_params make(const BM3D_Para &para)
+    {
+        isp_bm3d_params p;
+        p.basic = para.basic.to_c();
+        p.final_ = para.final.to_c();
+        return p;
+    }
+};
+
+}  // namespace isp_bm3d

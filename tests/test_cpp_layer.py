@@ -1,0 +1,35 @@
+"""The header-only C++ layer (include/isp_bm3d.hpp) compiles and links against
+libisp_bm3d.so; on a GPU box the example program runs."""
+import os
+import subprocess
+
+import pytest
+
+from isp_mw_amd import _lib as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SRC = os.path.join(ROOT, "tests", "cpp", "bm3d_example.cpp")
+
+
+def build_example(tmp_path):
+    exe = str(tmp_path / "bm3d_example")
+    libdir = os.path.dirname(L.LIB_PATH)
+    cmd = ["g++", "-std=c++17", "-O1", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), SRC,
+           "-L", libdir, "-lisp_bm3d", f"-Wl,-rpath,{libdir}", "-o", exe]
+    subprocess.run(cmd, check=True)
+    return exe
+
+
+def test_cpp_layer_compiles(tmp_path):
+    assert os.path.exists(build_example(tmp_path))
+
+
+@pytest.mark.gpu
+def test_cpp_example_runs(gpu, tmp_path):
+    exe = build_example(tmp_path)
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(out.stdout, out.stderr)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "expected error -2" in out.stdout
+    err = float(out.stdout.split("mean abs err")[1].split()[0])
+    assert err < 0.05
