@@ -26,6 +26,7 @@ struct isp_bm3d_ctx {
     int cn_off[kMaxGroup + 1] = {0};
     double *d_tables64 = nullptr;  // C8 (64) then the unpadded C_N matrices, FP64
     int cnd_off[kMaxGroup + 1] = {0};
+    uint16_t *d_mfma[2] = {nullptr, nullptr};  // MFMA operand tables, [0] FP16, [1] BF16
     void *ws = nullptr;
     size_t ws_bytes = 0;
     std::string err;
@@ -398,11 +399,20 @@ int run_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, cons
             f.thr64[c] = d.thr64[c];
             f.band[c] = d.band[c];
         }
-        if (filter_lds_bytes(f, wiener) > 160 * 1024)
-            return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "parameters need %zu bytes of LDS for filtering",
-                        filter_lds_bytes(f, wiener));
+        const bool mfma = ctx->precision != ISP_BM3D_PREC_FP32;
+        if (mfma) {
+            const uint16_t *t = ctx->d_mfma[ctx->precision == ISP_BM3D_PREC_FP16 ? 0 : 1];
+            f.kb = t;
+            f.ka = t + 8 * 64 * 8;
+            f.cna = t + 16 * 64 * 8;
+            f.cnb = f.cna + (size_t)kMaxGroup * 2 * 64 * 8;
+        }
+        const size_t lds = mfma ? filter_mfma_lds_bytes(f) : filter_lds_bytes(f, wiener);
+        if (lds > 160 * 1024)
+            return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "parameters need %zu bytes of LDS for filtering", lds);
         StageTimer tm(ctx, wiener ? ST_FILTER_FINAL : ST_FILTER_BASIC, s);
-        HIP_CHECK(ctx, launch_filter(f, wiener, s));
+        if (mfma) HIP_CHECK(ctx, launch_filter_mfma(f, wiener, ctx->precision, s));
+        else HIP_CHECK(ctx, launch_filter(f, wiener, s));
     }
     StageTimer tm(ctx, ST_FINALIZE, s);
     HIP_CHECK(ctx, launch_finalize(fa, s));
@@ -418,8 +428,6 @@ int check_common(isp_bm3d_ctx *ctx, int W, int H, size_t stride, int mode, int m
     if (W <= 0 || H <= 0) return fail(ctx, ISP_BM3D_E_SIZE, "empty image %dx%d", W, H);
     if (stride < (size_t)W) return fail(ctx, ISP_BM3D_E_ARG, "stride %zu < width %d", stride, W);
     if (stride > (size_t)1 << 30) return fail(ctx, ISP_BM3D_E_ARG, "stride %zu too large", stride);
-    if (ctx->precision != ISP_BM3D_PREC_FP32)
-        return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "precision %d is not available in this build", ctx->precision);
     HIP_CHECK(ctx, hipSetDevice(ctx->device));
     return ISP_BM3D_OK;
 }
@@ -469,6 +477,61 @@ void set_default_step(isp_bm3d_step_params *p, const char *profile, int final_st
             p->block_step = 6;
         }
     }
+}
+
+uint16_t to_bf16(float f)
+{
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+
+uint16_t to_f16(float f)
+{
+    _Float16 hv = (_Float16)f;
+    uint16_t u;
+    std::memcpy(&u, &hv, 2);
+    return u;
+}
+
+// MFMA operand tables for filter_mfma_kernel (layouts in bm3d_internal.h).
+// h = lane / 32, c = lane % 32; a 16-long k chunk m of an accumulator-fed
+// operand is ordered 8*(e/4) + 4h + e%4 to match the accumulator rows.
+std::vector<float> mfma_tables_f32(size_t *n_kb, size_t *n_ka, size_t *n_cn)
+{
+    const long double PI = 3.141592653589793238462643383279502884L;
+    auto c8 = [&](int k, int n) {
+        return (double)((k == 0 ? std::sqrt(1.0L / 8) : std::sqrt(2.0L / 8)) * std::cos(PI * (2 * n + 1) * k / 16.0L));
+    };
+    auto k64 = [&](int p, int q) { return c8(q >> 3, p >> 3) * c8(q & 7, p & 7); };
+    auto cn = [&](int N, int k, int z) -> double {
+        if (k >= N || z >= N) return 0.0;
+        return (double)((k == 0 ? std::sqrt(1.0L / N) : std::sqrt(2.0L / N)) *
+                        std::cos(PI * (2 * z + 1) * k / (2.0L * N)));
+    };
+    auto perm = [](int m, int h, int e) { return 16 * m + 8 * (e >> 2) + 4 * h + (e & 3); };
+    std::vector<float> t;
+    for (int s = 0; s < 4; ++s)
+        for (int q = 0; q < 2; ++q)
+            for (int l = 0; l < 64; ++l)
+                for (int e = 0; e < 8; ++e) t.push_back((float)k64(16 * s + 8 * (l >> 5) + e, 32 * q + (l & 31)));
+    *n_kb = t.size();
+    for (int r = 0; r < 2; ++r)
+        for (int s = 0; s < 4; ++s)
+            for (int l = 0; l < 64; ++l)
+                for (int e = 0; e < 8; ++e) t.push_back((float)k64(32 * r + (l & 31), perm(s, l >> 5, e)));
+    *n_ka = t.size() - *n_kb;
+    for (int N = 1; N <= kMaxGroup; ++N)
+        for (int m = 0; m < 2; ++m)
+            for (int l = 0; l < 64; ++l)
+                for (int e = 0; e < 8; ++e) t.push_back((float)cn(N, l & 31, perm(m, l >> 5, e)));
+    for (int N = 1; N <= kMaxGroup; ++N)
+        for (int m = 0; m < 2; ++m)
+            for (int l = 0; l < 64; ++l)
+                for (int e = 0; e < 8; ++e) t.push_back((float)cn(N, perm(m, l >> 5, e), l & 31));
+    *n_cn = (t.size() - *n_kb - *n_ka) / 2;
+    return t;
 }
 
 }  // namespace
@@ -566,6 +629,24 @@ int isp_bm3d_create(isp_bm3d_ctx **out, const isp_bm3d_params *params, int devic
         delete ctx;
         return rc;
     }
+    size_t n_kb, n_ka, n_cn;
+    std::vector<float> mt = mfma_tables_f32(&n_kb, &n_ka, &n_cn);
+    for (int k = 0; k < 2; ++k) {
+        std::vector<uint16_t> bits(mt.size());
+        for (size_t i = 0; i < mt.size(); ++i) bits[i] = k == 0 ? to_f16(mt[i]) : to_bf16(mt[i]);
+        if ((e = hipMalloc(&ctx->d_mfma[k], bits.size() * 2)) != hipSuccess ||
+            (e = hipMemcpy(ctx->d_mfma[k], bits.data(), bits.size() * 2, hipMemcpyHostToDevice)) != hipSuccess)
+            break;
+    }
+    if (e != hipSuccess) {
+        int rc = fail(nullptr, ISP_BM3D_E_HIP, "create: %s", hipGetErrorString(e));
+        (void)hipFree(ctx->d_tables);
+        (void)hipFree(ctx->d_tables64);
+        for (int k = 0; k < 2; ++k)
+            if (ctx->d_mfma[k]) (void)hipFree(ctx->d_mfma[k]);
+        delete ctx;
+        return rc;
+    }
     *out = ctx;
     return ISP_BM3D_OK;
 }
@@ -625,6 +706,8 @@ int isp_bm3d_destroy(isp_bm3d_ctx *ctx)
     if (ctx->ws) (void)hipFree(ctx->ws);
     if (ctx->d_tables) (void)hipFree(ctx->d_tables);
     if (ctx->d_tables64) (void)hipFree(ctx->d_tables64);
+    for (int k = 0; k < 2; ++k)
+        if (ctx->d_mfma[k]) (void)hipFree(ctx->d_mfma[k]);
     delete ctx;
     return ISP_BM3D_OK;
 }

@@ -59,6 +59,12 @@ struct FilterArgs {
     int cnd_off[kMaxGroup + 1];
     double thr64[3];
     float band[3];
+    // MFMA operand tables (16-bit elements, f16 or bf16 by precision):
+    // kb[4][2][64][8]  K64 as B of T = X K64 (forward, spatial)
+    // ka[2][4][64][8]  K64 as A of X^T = K64 V^T (inverse, spatial)
+    // cna[N][2][64][8] C_N as A of Y = C_N T (forward, group axis)
+    // cnb[N][2][64][8] C_N as B of V^T = Y^T C_N (inverse, group axis)
+    const uint16_t *kb, *ka, *cna, *cnb;
 };
 
 struct FinalizeArgs {
@@ -89,6 +95,9 @@ size_t filter_lds_bytes(const FilterArgs &a, bool wiener);
 hipError_t launch_match_f32(const float *guide, int gpitch, const MatchArgs &a, hipStream_t s);
 hipError_t launch_match_f64(const double *guide, int gpitch, const MatchArgs &a, hipStream_t s);
 hipError_t launch_filter(const FilterArgs &a, bool wiener, hipStream_t s);
+// precision: 1 = FP16, 2 = BF16 operands with FP32 accumulation
+hipError_t launch_filter_mfma(const FilterArgs &a, bool wiener, int precision, hipStream_t s);
+size_t filter_mfma_lds_bytes(const FilterArgs &a);
 hipError_t launch_finalize(const FinalizeArgs &a, hipStream_t s);
 hipError_t launch_opp(const OppArgs &a, hipStream_t s);
 hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s);

@@ -541,6 +541,44 @@ __host__ __device__ inline size_t filter_lds(int FW, int FH, int G)
     return (filter_acc_floats(FW, FH) + groups) * 4;
 }
 
+
+// Adds a tile's LDS accumulators into the num/den planes of plane c.  den of
+// a pixel is the sum of the weights of the blocks whose origin lies in the
+// 8x8 window above-left of it: row box sums (into hsum), then column sums.
+__device__ __forceinline__ void flush_tile(const FilterArgs &a, int c, const float *accNum, const float *accDen,
+                                           float *hsum, int ox, int oy, int tid)
+{
+    const int FW = a.FW, FH = a.FH, DW = FW - kBlock + 1, DH = FH - kBlock + 1;
+    for (int k = tid; k < DH * FW; k += kThreads) {
+        const int dy = k / FW, px = k - dy * FW;
+        float s = 0.f;
+#pragma unroll
+        for (int bq = 0; bq < kBlock; ++bq) {
+            const int q = px - bq;
+            if (q >= 0 && q < DW) s += accDen[dy * DW + q];
+        }
+        hsum[k] = s;
+    }
+    __syncthreads();
+    float *num = a.num[c], *den = a.den[c];
+    for (int k = tid; k < FW * FH; k += kThreads) {
+        const int py = k / FW, px = k - py * FW;
+        const int yy = oy + py, xx = ox + px;
+        if (yy < 0 || yy >= a.H || xx < 0 || xx >= a.W) continue;
+        float d = 0.f;
+#pragma unroll
+        for (int aq = 0; aq < kBlock; ++aq) {
+            const int q = py - aq;
+            if (q >= 0 && q < DH) d += hsum[q * FW + px];
+        }
+        if (d != 0.f) {
+            const size_t o = (size_t)yy * a.pitch + xx;
+            unsafeAtomicAdd(&num[o], accNum[k]);
+            unsafeAtomicAdd(&den[o], d);
+        }
+    }
+}
+
 template <bool WIENER>
 __global__ __launch_bounds__(kThreads) void filter_kernel(FilterArgs a)
 {
@@ -594,36 +632,226 @@ __global__ __launch_bounds__(kThreads) void filter_kernel(FilterArgs a)
             wave_sync();
         }
         __syncthreads();
-        // den(pixel) = sum of the weights of blocks whose origin lies in the
-        // 8x8 window above-left of it: row box sums, then column box sums.
-        for (int k = tid; k < DH * FW; k += kThreads) {
-            const int dy = k / FW, px = k - dy * FW;
-            float s = 0.f;
+        flush_tile(a, c, accNum, accDen, hsum, ox, oy, tid);
+        __syncthreads();
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// Collaborative filtering on the matrix cores (FP16 or BF16 operands, FP32
+// accumulation).  A group is the 32 x 64 matrix X (block z, pixel p = 8y+x),
+// zero-padded to 32 blocks.  With K64 = C8 (x) C8 (the 2D DCT on p) and C_N
+// padded to 32 x 32:
+//   forward  T = X K64,   Y = C_N T                  (12 MFMAs 32x32x16)
+//   inverse  V^T = Y'^T C_N,   Xhat^T = K64 V^T       (12 MFMAs)
+// Every product takes the previous accumulator straight from registers: an
+// accumulator 32x32 tile holds rows 8i+4h+j (h = lane/32) of column lane%32,
+// which is an operand fragment of the next product for a fixed permutation of
+// its 16-long k chunks; the constant operand (C_N or K64) is stored with the
+// same permutation, so no data moves between the stages.
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+template <typename T>
+struct MfmaT;
+template <>
+struct MfmaT<_Float16> {
+    typedef f16x8 v8;
+    static __device__ __forceinline__ f32x16 mma(v8 a, v8 b, f32x16 c)
+    {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct MfmaT<__bf16> {
+    typedef bf16x8 v8;
+    static __device__ __forceinline__ f32x16 mma(v8 a, v8 b, f32x16 c)
+    {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    }
+};
+
+template <typename T>
+__device__ __forceinline__ typename MfmaT<T>::v8 load_frag(const uint16_t *__restrict__ tab, int frag, int lane)
+{
+    return *reinterpret_cast<const typename MfmaT<T>::v8 *>(tab + ((size_t)frag * 64 + lane) * 8);
+}
+
+// accumulator registers 8m..8m+7 as an operand fragment
+template <typename T>
+__device__ __forceinline__ typename MfmaT<T>::v8 acc_frag(const f32x16 &acc, int m)
+{
+    typename MfmaT<T>::v8 f;
 #pragma unroll
-            for (int bq = 0; bq < kBlock; ++bq) {
-                const int q = px - bq;
-                if (q >= 0 && q < DW) s += accDen[dy * DW + q];
+    for (int e = 0; e < 8; ++e) f[e] = (T)acc[8 * m + e];
+    return f;
+}
+
+// Forward 3D DCT of one group: y[t] = (C_N X K64) columns 32t..32t+31.
+template <typename T>
+__device__ __forceinline__ void mfma_forward(const float *__restrict__ plane, int pitch, const int *__restrict__ pos,
+                                             int N, int lane, const typename MfmaT<T>::v8 (&kb)[4][2],
+                                             const typename MfmaT<T>::v8 (&cna)[2], f32x16 (&y)[2])
+{
+    typedef typename MfmaT<T>::v8 v8;
+    const int c = lane & 31, h = lane >> 5;
+    // A fragments of X: block z = c, pixel rows y = 2s + h
+    v8 xa[4];
+    if (c < N) {
+        const int pz = pos[c];
+        const float *b = plane + (size_t)(pz >> 16) * pitch + (pz & 0xFFFF);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const float *r = b + (size_t)(2 * s + h) * pitch;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) xa[s][e] = (T)r[e];
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) xa[s][e] = (T)0.f;
+    }
+    f32x16 t[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        t[q] = (f32x16){};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) t[q] = MfmaT<T>::mma(xa[s], kb[s][q], t[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        y[q] = (f32x16){};
+#pragma unroll
+        for (int m = 0; m < 2; ++m) y[q] = MfmaT<T>::mma(cna[m], acc_frag<T>(t[q], m), y[q]);
+    }
+}
+
+// Inverse: xh[r] = rows 32r..32r+31 (pixels) of Xhat^T = K64 (Y'^T C_N).
+template <typename T>
+__device__ __forceinline__ void mfma_inverse(const f32x16 (&y)[2], const typename MfmaT<T>::v8 (&ka)[2][4],
+                                             const typename MfmaT<T>::v8 (&cnb)[2], f32x16 (&xh)[2])
+{
+    f32x16 vt[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        vt[q] = (f32x16){};
+#pragma unroll
+        for (int m = 0; m < 2; ++m) vt[q] = MfmaT<T>::mma(acc_frag<T>(y[q], m), cnb[m], vt[q]);
+    }
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        xh[r] = (f32x16){};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) xh[r] = MfmaT<T>::mma(ka[r][s], acc_frag<T>(vt[s >> 1], s & 1), xh[r]);
+    }
+}
+
+__host__ __device__ inline size_t filter_mfma_lds(int FW, int FH)
+{
+    const int DH = FH - kBlock + 1;
+    return (filter_acc_floats(FW, FH) + (size_t)DH * FW) * 4;
+}
+
+template <typename T, bool WIENER>
+__global__ __launch_bounds__(kThreads) void filter_mfma_kernel(FilterArgs a)
+{
+    typedef typename MfmaT<T>::v8 v8;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int c = lane & 31, h = lane >> 5;
+    const int tx0 = blockIdx.x * kTileRX, ty0 = blockIdx.y * kTileRY;
+    const int ox = grid_pos(tx0, a.bstep, a.W - kBlock) - a.range;
+    const int oy = grid_pos(ty0, a.bstep, a.H - kBlock) - a.range;
+    const int FW = a.FW, FH = a.FH, DW = FW - kBlock + 1, DH = FH - kBlock + 1;
+
+    float *accNum = reinterpret_cast<float *>(smem);
+    float *accDen = accNum + FW * FH;
+    float *hsum = accNum + filter_acc_floats(FW, FH);
+
+    v8 kb[4][2], ka[2][4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) kb[s][q] = load_frag<T>(a.kb, s * 2 + q, lane);
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) ka[r][s] = load_frag<T>(a.ka, r * 4 + s, lane);
+
+    for (int pl = 0; pl < 3; ++pl) {
+        if (!a.active[pl]) continue;
+        for (int k = tid; k < FW * FH; k += kThreads) accNum[k] = 0.f;
+        for (int k = tid; k < DW * DH; k += kThreads) accDen[k] = 0.f;
+        __syncthreads();
+        const float *src = a.src[pl];
+        const float *gsrc = a.guide[pl];
+        const float thr = a.thr[pl], sig2 = a.sig2[pl];
+        for (int r = wave; r < kTileRX * kTileRY; r += kWaves) {
+            const int gxi = tx0 + (r % kTileRX), gyi = ty0 + (r / kTileRX);
+            if (gxi >= a.ngx || gyi >= a.ngy) continue;
+            const size_t refIdx = (size_t)gyi * a.ngx + gxi;
+            const int *pos = a.pos + refIdx * a.G;
+            const int N = a.cnt[refIdx];
+            v8 cna[2], cnb[2];
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                cna[m] = load_frag<T>(a.cna, (N - 1) * 2 + m, lane);
+                cnb[m] = load_frag<T>(a.cnb, (N - 1) * 2 + m, lane);
             }
-            hsum[k] = s;
+            f32x16 y[2];
+            mfma_forward<T>(src, a.pitch, pos, N, lane, kb, cna, y);
+            float w;
+            if (!WIENER) {
+                int kept = 0;
+#pragma unroll
+                for (int q = 0; q < 2; ++q)
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) {
+                        if (fabsf(y[q][k]) <= thr) y[q][k] = 0.f;
+                        else ++kept;
+                    }
+                kept = wave_sum(kept);
+                w = kept < 1 ? 1.f : 1.f / (float)kept;
+            } else {
+                f32x16 yg[2];
+                mfma_forward<T>(gsrc, a.pitch, pos, N, lane, kb, cna, yg);
+                float l2 = 0.f;
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    // column 32q + c = frequency (u, v): u = 4q + c/8, v = c%8
+                    const float su = sig2 * ((4 * q + (c >> 3)) == 0 ? 0.5f : 1.f) * ((c & 7) == 0 ? 0.5f : 1.f);
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) {
+                        const int kz = 8 * (k >> 2) + 4 * h + (k & 3);  // group-axis frequency
+                        const float yy = yg[q][k] * yg[q][k];
+                        const float wc = yy / (yy + (kz == 0 ? 0.5f * su : su));
+                        y[q][k] *= wc;
+                        l2 = fmaf(wc, wc, l2);
+                    }
+                }
+                l2 = wave_sum(l2);
+                w = l2 <= 0.f ? 1.f : 1.f / l2;
+            }
+            f32x16 xh[2];
+            mfma_inverse<T>(y, ka, cnb, xh);
+            if (c < N) {
+                const int pz = pos[c];
+                const int py = (pz >> 16) - oy, px = (pz & 0xFFFF) - ox + 4 * h;
+#pragma unroll
+                for (int rr = 0; rr < 2; ++rr)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            atomicAdd(&accNum[(py + 4 * rr + i) * FW + px + j], w * xh[rr][4 * i + j]);
+                if (h == 0) atomicAdd(&accDen[py * DW + (pz & 0xFFFF) - ox], w);
+            }
         }
         __syncthreads();
-        float *num = a.num[c], *den = a.den[c];
-        for (int k = tid; k < FW * FH; k += kThreads) {
-            const int py = k / FW, px = k - py * FW;
-            const int yy = oy + py, xx = ox + px;
-            if (yy < 0 || yy >= a.H || xx < 0 || xx >= a.W) continue;
-            float d = 0.f;
-#pragma unroll
-            for (int aq = 0; aq < kBlock; ++aq) {
-                const int q = py - aq;
-                if (q >= 0 && q < DH) d += hsum[q * FW + px];
-            }
-            if (d != 0.f) {
-                const size_t o = (size_t)yy * a.pitch + xx;
-                unsafeAtomicAdd(&num[o], accNum[k]);
-                unsafeAtomicAdd(&den[o], d);
-            }
-        }
+        flush_tile(a, pl, accNum, accDen, hsum, ox, oy, tid);
         __syncthreads();
     }
 }
@@ -739,6 +967,29 @@ hipError_t launch_filter(const FilterArgs &a, bool wiener, hipStream_t s)
         if ((e = set_lds(filter_kernel<false>, lds)) != hipSuccess) return e;
         hipLaunchKernelGGL(filter_kernel<false>, grid, dim3(kThreads), lds, s, a);
     }
+    return hipGetLastError();
+}
+
+size_t filter_mfma_lds_bytes(const FilterArgs &a) { return filter_mfma_lds(a.FW, a.FH); }
+
+hipError_t launch_filter_mfma(const FilterArgs &a, bool wiener, int precision, hipStream_t s)
+{
+    const size_t lds = filter_mfma_lds_bytes(a);
+    const dim3 grid((a.ngx + kTileRX - 1) / kTileRX, (a.ngy + kTileRY - 1) / kTileRY);
+    hipError_t e;
+#define LAUNCH_MFMA(T, W)                                                                     \
+    do {                                                                                      \
+        if ((e = set_lds(filter_mfma_kernel<T, W>, lds)) != hipSuccess) return e;            \
+        hipLaunchKernelGGL((filter_mfma_kernel<T, W>), grid, dim3(kThreads), lds, s, a);     \
+    } while (0)
+    if (precision == 1) {
+        if (wiener) LAUNCH_MFMA(_Float16, true);
+        else LAUNCH_MFMA(_Float16, false);
+    } else {
+        if (wiener) LAUNCH_MFMA(__bf16, true);
+        else LAUNCH_MFMA(__bf16, false);
+    }
+#undef LAUNCH_MFMA
     return hipGetLastError();
 }
 
