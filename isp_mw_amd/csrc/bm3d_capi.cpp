@@ -237,7 +237,7 @@ struct Work {
     float *plane[3];   // src YUV (RGB path) or uploaded src (host plane path)
     float *refp[3];    // uploaded/converted guide planes
     float *basic[3];   // step-1 estimate
-    float *num[3], *den[3];
+    unsigned long long *num[3], *den[3];  // fixed-point accumulators
     float *out[3];     // host-mode output staging
     double *y64;       // FP64 matching guide (RGB path)
     double *ref64;     // FP64 matching guide of a separate ref frame
@@ -252,8 +252,8 @@ size_t layout(Work *w, void *base, int P, int H, size_t refs, bool keys)
     for (int k = 0; k < 3; ++k) w->plane[k] = c.take<float>(np);
     for (int k = 0; k < 3; ++k) w->refp[k] = c.take<float>(np);
     for (int k = 0; k < 3; ++k) w->basic[k] = c.take<float>(np);
-    for (int k = 0; k < 3; ++k) w->num[k] = c.take<float>(np);
-    for (int k = 0; k < 3; ++k) w->den[k] = c.take<float>(np);
+    for (int k = 0; k < 3; ++k) w->num[k] = c.take<unsigned long long>(np);
+    for (int k = 0; k < 3; ++k) w->den[k] = c.take<unsigned long long>(np);
     for (int k = 0; k < 3; ++k) w->out[k] = c.take<float>(np);
     w->y64 = c.take<double>(np);
     w->ref64 = c.take<double>(np);
@@ -381,12 +381,15 @@ int run_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, cons
             fa.active[c] = d.active[c];
             if (d.active[c]) {
                 StageTimer tm(ctx, ST_FILL, s);
-                HIP_CHECK(ctx, launch_fill(w.num[c], (size_t)io.pitch * H, 0.f, s));
-                HIP_CHECK(ctx, launch_fill(w.den[c], (size_t)io.pitch * H, 0.f, s));
+                // zero bits of the 64-bit accumulators
+                HIP_CHECK(ctx, launch_fill(reinterpret_cast<float *>(w.num[c]), (size_t)io.pitch * H * 2, 0.f, s));
+                HIP_CHECK(ctx, launch_fill(reinterpret_cast<float *>(w.den[c]), (size_t)io.pitch * H * 2, 0.f, s));
             }
         }
-        f.FW = (kTileRX - 1) * p.block_step + 2 * d.range + kBlock;
-        f.FH = (kTileRY - 1) * p.block_step + 2 * d.range + kBlock;
+        const bool mfma = ctx->precision != ISP_BM3D_PREC_FP32;
+        const int trx = mfma ? kMTileRX : kTileRX, try_ = mfma ? kMTileRY : kTileRY;
+        f.FW = (trx - 1) * p.block_step + 2 * d.range + kBlock;
+        f.FH = (try_ - 1) * p.block_step + 2 * d.range + kBlock;
         f.c8 = ctx->d_tables;
         f.cn = ctx->d_tables;
         for (int n = 0; n <= kMaxGroup; ++n) {
@@ -399,7 +402,6 @@ int run_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, cons
             f.thr64[c] = d.thr64[c];
             f.band[c] = d.band[c];
         }
-        const bool mfma = ctx->precision != ISP_BM3D_PREC_FP32;
         if (mfma) {
             const uint16_t *t = ctx->d_mfma[ctx->precision == ISP_BM3D_PREC_FP16 ? 0 : 1];
             f.kb = t;
@@ -407,7 +409,7 @@ int run_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, cons
             f.cna = t + 16 * 64 * 8;
             f.cnb = f.cna + (size_t)kMaxGroup * 2 * 64 * 8;
         }
-        const size_t lds = mfma ? filter_mfma_lds_bytes(f) : filter_lds_bytes(f, wiener);
+        const size_t lds = mfma ? filter_mfma_lds_bytes(f, wiener) : filter_lds_bytes(f, wiener);
         if (lds > 160 * 1024)
             return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "parameters need %zu bytes of LDS for filtering", lds);
         StageTimer tm(ctx, wiener ? ST_FILTER_FINAL : ST_FILTER_BASIC, s);

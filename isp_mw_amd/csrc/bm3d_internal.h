@@ -14,6 +14,18 @@ constexpr int kTileRY = 4;       // reference blocks per workgroup tile, y
 constexpr int kStrip = 5;        // candidates per lane strip in matching
 constexpr int kWaves = 4;        // waves per workgroup
 constexpr int kThreads = 64 * kWaves;
+// MFMA filter: 8 waves over a tile of 8 x 8 reference blocks
+constexpr int kMTileRX = 8;
+constexpr int kMTileRY = 8;
+constexpr int kMWaves = 8;
+constexpr int kMThreads = 64 * kMWaves;
+
+// Fixed-point scale of the aggregation sums: value * 2^36 rounded to int64.
+// A pixel collects at most 64 block origins x (2R/s+1)^2 groups of weight
+// <= 1 times |sample| <= 2, below 2^19 for every supported setting, so sums
+// stay below 2^55.
+constexpr float kFixScale = 68719476736.0f;        // 2^36
+constexpr double kFixInv = 1.0 / 68719476736.0;    // 2^-36
 
 // Positions are packed (y << 16) | x.
 struct MatchArgs {
@@ -43,8 +55,10 @@ struct FilterArgs {
     const int *pos;
     const float *src[3];      // noisy planes
     const float *guide[3];    // Wiener guide planes (final step)
-    float *num[3];            // accumulators, zeroed before launch
-    float *den[3];
+    // Aggregation accumulators, zeroed before launch: 64-bit fixed point
+    // with kFixScale fractional bits (exact, order-independent sums).
+    unsigned long long *num[3];
+    unsigned long long *den[3];
     int active[3];            // plane filtered?
     float thr[3];             // lambda * sigma_c (orthonormal domain)
     float sig2[3];            // sigma_c^2 (orthonormal domain)
@@ -69,8 +83,8 @@ struct FilterArgs {
 
 struct FinalizeArgs {
     int W, H, pitch;
-    const float *num[3];
-    const float *den[3];
+    const unsigned long long *num[3];
+    const unsigned long long *den[3];
     const float *src[3];      // copied through for inactive planes
     int active[3];
     int nplanes;
@@ -97,7 +111,7 @@ hipError_t launch_match_f64(const double *guide, int gpitch, const MatchArgs &a,
 hipError_t launch_filter(const FilterArgs &a, bool wiener, hipStream_t s);
 // precision: 1 = FP16, 2 = BF16 operands with FP32 accumulation
 hipError_t launch_filter_mfma(const FilterArgs &a, bool wiener, int precision, hipStream_t s);
-size_t filter_mfma_lds_bytes(const FilterArgs &a);
+size_t filter_mfma_lds_bytes(const FilterArgs &a, bool wiener);
 hipError_t launch_finalize(const FinalizeArgs &a, hipStream_t s);
 hipError_t launch_opp(const OppArgs &a, hipStream_t s);
 hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s);

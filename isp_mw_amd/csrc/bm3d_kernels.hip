@@ -77,6 +77,15 @@ __device__ __forceinline__ int grid_pos(int k, int s, int last) { return min(k *
 
 __host__ __device__ constexpr size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
 
+typedef unsigned long long u64;
+
+// Adds v to a 64-bit fixed-point accumulator in LDS (integer LDS atomics are
+// full rate; two's complement wraps negative values correctly).
+__device__ __forceinline__ void fix_add(u64 *acc, float v)
+{
+    atomicAdd(acc, (u64)__float2ll_rn(v * kFixScale));
+}
+
 __host__ __device__ inline size_t match_wave_bytes(int maxCand)
 {
     // keys u64[maxCand] + selk u64[32] + idxs u32[maxCand] + memb u32[maxCand]
@@ -526,32 +535,34 @@ __device__ __forceinline__ float group_core(float *g, int N, int lane, const flo
 // LDS of the filter kernel: accumulators of the tile footprint (num per pixel,
 // den per block origin) and one group buffer per wave; the row box sums of
 // the flush reuse the group buffers.
+// accumulators as 32-bit words: num per pixel and den per origin, 64 bits each
 __host__ __device__ inline size_t filter_acc_floats(int FW, int FH)
 {
     const int DW = FW - kBlock + 1, DH = FH - kBlock + 1;
-    return (((size_t)FW * FH + (size_t)DW * DH) + 3) & ~size_t(3);
+    return ((((size_t)FW * FH + (size_t)DW * DH) * 2) + 3) & ~size_t(3);
 }
 
 __host__ __device__ inline size_t filter_lds(int FW, int FH, int G)
 {
     const int DH = FH - kBlock + 1;
     size_t groups = (size_t)kWaves * G * ZS;
-    const size_t hs = (size_t)DH * FW;
+    const size_t hs = (size_t)DH * FW * 2;
     if (hs > groups) groups = hs;
     return (filter_acc_floats(FW, FH) + groups) * 4;
 }
 
 
-// Adds a tile's LDS accumulators into the num/den planes of plane c.  den of
+// Adds a tile's LDS accumulators into the num/den planes of plane c (all in
+// 64-bit fixed point, so the sums do not depend on the order of the adds).  den of
 // a pixel is the sum of the weights of the blocks whose origin lies in the
 // 8x8 window above-left of it: row box sums (into hsum), then column sums.
-__device__ __forceinline__ void flush_tile(const FilterArgs &a, int c, const float *accNum, const float *accDen,
-                                           float *hsum, int ox, int oy, int tid)
+__device__ __forceinline__ void flush_tile(const FilterArgs &a, int c, const u64 *accNum, const u64 *accDen,
+                                           u64 *hsum, int ox, int oy, int tid, int nthreads = kThreads)
 {
     const int FW = a.FW, FH = a.FH, DW = FW - kBlock + 1, DH = FH - kBlock + 1;
-    for (int k = tid; k < DH * FW; k += kThreads) {
+    for (int k = tid; k < DH * FW; k += nthreads) {
         const int dy = k / FW, px = k - dy * FW;
-        float s = 0.f;
+        u64 s = 0;
 #pragma unroll
         for (int bq = 0; bq < kBlock; ++bq) {
             const int q = px - bq;
@@ -560,21 +571,21 @@ __device__ __forceinline__ void flush_tile(const FilterArgs &a, int c, const flo
         hsum[k] = s;
     }
     __syncthreads();
-    float *num = a.num[c], *den = a.den[c];
-    for (int k = tid; k < FW * FH; k += kThreads) {
+    u64 *num = a.num[c], *den = a.den[c];
+    for (int k = tid; k < FW * FH; k += nthreads) {
         const int py = k / FW, px = k - py * FW;
         const int yy = oy + py, xx = ox + px;
         if (yy < 0 || yy >= a.H || xx < 0 || xx >= a.W) continue;
-        float d = 0.f;
+        u64 d = 0;
 #pragma unroll
         for (int aq = 0; aq < kBlock; ++aq) {
             const int q = py - aq;
             if (q >= 0 && q < DH) d += hsum[q * FW + px];
         }
-        if (d != 0.f) {
+        if (d != 0) {
             const size_t o = (size_t)yy * a.pitch + xx;
-            unsafeAtomicAdd(&num[o], accNum[k]);
-            unsafeAtomicAdd(&den[o], d);
+            atomicAdd(&num[o], accNum[k]);
+            atomicAdd(&den[o], d);
         }
     }
 }
@@ -589,17 +600,17 @@ __global__ __launch_bounds__(kThreads) void filter_kernel(FilterArgs a)
     const int oy = grid_pos(ty0, a.bstep, a.H - kBlock) - a.range;
     const int FW = a.FW, FH = a.FH, DW = FW - kBlock + 1, DH = FH - kBlock + 1;
 
-    float *accNum = reinterpret_cast<float *>(smem);
-    float *accDen = accNum + FW * FH;
-    float *gbase = accNum + filter_acc_floats(FW, FH);
+    u64 *accNum = reinterpret_cast<u64 *>(smem);
+    u64 *accDen = accNum + FW * FH;
+    float *gbase = reinterpret_cast<float *>(smem) + filter_acc_floats(FW, FH);
     float *g = gbase + (size_t)wave * a.G * ZS;
-    float *hsum = gbase;  // flush only, after every wave is done with its groups
+    u64 *hsum = reinterpret_cast<u64 *>(gbase);  // flush only, after every wave is done with its groups
     const int u = lane >> 3, v = lane & 7;
 
     for (int c = 0; c < 3; ++c) {
         if (!a.active[c]) continue;
-        for (int k = tid; k < FW * FH; k += kThreads) accNum[k] = 0.f;
-        for (int k = tid; k < DW * DH; k += kThreads) accDen[k] = 0.f;
+        for (int k = tid; k < FW * FH; k += kThreads) accNum[k] = 0;
+        for (int k = tid; k < DW * DH; k += kThreads) accDen[k] = 0;
         __syncthreads();
         const float *src = a.src[c];
         const float *gsrc = a.guide[c];
@@ -623,11 +634,11 @@ __global__ __launch_bounds__(kThreads) void filter_kernel(FilterArgs a)
             for (int z = 0; z < N; ++z) {
                 const int pz = pos[z];
                 const int py = (pz >> 16) - oy, px = (pz & 0xFFFF) - ox;
-                atomicAdd(&accNum[(py + u) * FW + px + v], w * g[o + z * ZS]);
+                fix_add(&accNum[(py + u) * FW + px + v], w * g[o + z * ZS]);
             }
             if (lane < N) {
                 const int pz = pos[lane];
-                atomicAdd(&accDen[((pz >> 16) - oy) * DW + (pz & 0xFFFF) - ox], w);
+                fix_add(&accDen[((pz >> 16) - oy) * DW + (pz & 0xFFFF) - ox], w);
             }
             wave_sync();
         }
@@ -691,8 +702,15 @@ __device__ __forceinline__ typename MfmaT<T>::v8 acc_frag(const f32x16 &acc, int
 
 // Forward 3D DCT of one group: y[t] = (C_N X K64) columns 32t..32t+31.
 template <typename T>
-__device__ __forceinline__ void mfma_forward(const float *__restrict__ plane, int pitch, const int *__restrict__ pos,
-                                             int N, int lane, const typename MfmaT<T>::v8 (&kb)[4][2],
+__device__ __forceinline__ typename MfmaT<T>::v8 lds_frag(const uint16_t *tab, int frag, int lane)
+{
+    return *reinterpret_cast<const typename MfmaT<T>::v8 *>(tab + ((size_t)frag * 64 + lane) * 8);
+}
+
+// plane: the tile footprint staged in LDS (row pitch `pitch`, origin ox, oy).
+template <typename T>
+__device__ __forceinline__ void mfma_forward(const float *plane, int pitch, int ox, int oy,
+                                             const int *__restrict__ pos, int N, int lane, const uint16_t *kb,
                                              const typename MfmaT<T>::v8 (&cna)[2], f32x16 (&y)[2])
 {
     typedef typename MfmaT<T>::v8 v8;
@@ -701,10 +719,10 @@ __device__ __forceinline__ void mfma_forward(const float *__restrict__ plane, in
     v8 xa[4];
     if (c < N) {
         const int pz = pos[c];
-        const float *b = plane + (size_t)(pz >> 16) * pitch + (pz & 0xFFFF);
+        const float *b = plane + ((pz >> 16) - oy) * pitch + (pz & 0xFFFF) - ox;
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            const float *r = b + (size_t)(2 * s + h) * pitch;
+            const float *r = b + (2 * s + h) * pitch;
 #pragma unroll
             for (int e = 0; e < 8; ++e) xa[s][e] = (T)r[e];
         }
@@ -719,7 +737,7 @@ __device__ __forceinline__ void mfma_forward(const float *__restrict__ plane, in
     for (int q = 0; q < 2; ++q) {
         t[q] = (f32x16){};
 #pragma unroll
-        for (int s = 0; s < 4; ++s) t[q] = MfmaT<T>::mma(xa[s], kb[s][q], t[q]);
+        for (int s = 0; s < 4; ++s) t[q] = MfmaT<T>::mma(xa[s], lds_frag<T>(kb, s * 2 + q, lane), t[q]);
     }
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
@@ -731,7 +749,7 @@ __device__ __forceinline__ void mfma_forward(const float *__restrict__ plane, in
 
 // Inverse: xh[r] = rows 32r..32r+31 (pixels) of Xhat^T = K64 (Y'^T C_N).
 template <typename T>
-__device__ __forceinline__ void mfma_inverse(const f32x16 (&y)[2], const typename MfmaT<T>::v8 (&ka)[2][4],
+__device__ __forceinline__ void mfma_inverse(const f32x16 (&y)[2], const uint16_t *ka, int lane,
                                              const typename MfmaT<T>::v8 (&cnb)[2], f32x16 (&xh)[2])
 {
     f32x16 vt[2];
@@ -745,52 +763,74 @@ __device__ __forceinline__ void mfma_inverse(const f32x16 (&y)[2], const typenam
     for (int r = 0; r < 2; ++r) {
         xh[r] = (f32x16){};
 #pragma unroll
-        for (int s = 0; s < 4; ++s) xh[r] = MfmaT<T>::mma(ka[r][s], acc_frag<T>(vt[s >> 1], s & 1), xh[r]);
+        for (int s = 0; s < 4; ++s)
+            xh[r] = MfmaT<T>::mma(lds_frag<T>(ka, r * 4 + s, lane), acc_frag<T>(vt[s >> 1], s & 1), xh[r]);
     }
 }
 
-__host__ __device__ inline size_t filter_mfma_lds(int FW, int FH)
+// LDS: accumulators, flush row sums, then the K64 operand fragments
+// (forward B and inverse A, 16 x 64 lanes x 8 elements of 16 bits).
+// LDS of the MFMA filter: accumulators, the K64 operand fragments (forward B
+// and inverse A, 16 x 64 lanes x 8 elements of 16 bits), then the staged
+// source (and guide) footprint; the flush's row sums reuse the staging area.
+constexpr int kK64Elems = 16 * 64 * 8;
+__host__ __device__ inline size_t filter_mfma_stage_floats(int FW, int FH, bool wiener)
 {
     const int DH = FH - kBlock + 1;
-    return (filter_acc_floats(FW, FH) + (size_t)DH * FW) * 4;
+    size_t st = (size_t)FW * FH * (wiener ? 2 : 1);
+    const size_t hs = (size_t)DH * FW * 2;
+    return st > hs ? st : hs;
+}
+__host__ __device__ inline size_t filter_mfma_lds(int FW, int FH, bool wiener)
+{
+    return filter_acc_floats(FW, FH) * 4 + kK64Elems * 2 + filter_mfma_stage_floats(FW, FH, wiener) * 4;
 }
 
 template <typename T, bool WIENER>
-__global__ __launch_bounds__(kThreads) void filter_mfma_kernel(FilterArgs a)
+__global__ __launch_bounds__(kMThreads) void filter_mfma_kernel(FilterArgs a)
 {
     typedef typename MfmaT<T>::v8 v8;
     extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int c = lane & 31, h = lane >> 5;
-    const int tx0 = blockIdx.x * kTileRX, ty0 = blockIdx.y * kTileRY;
+    const int tx0 = blockIdx.x * kMTileRX, ty0 = blockIdx.y * kMTileRY;
     const int ox = grid_pos(tx0, a.bstep, a.W - kBlock) - a.range;
     const int oy = grid_pos(ty0, a.bstep, a.H - kBlock) - a.range;
     const int FW = a.FW, FH = a.FH, DW = FW - kBlock + 1, DH = FH - kBlock + 1;
 
-    float *accNum = reinterpret_cast<float *>(smem);
-    float *accDen = accNum + FW * FH;
-    float *hsum = accNum + filter_acc_floats(FW, FH);
-
-    v8 kb[4][2], ka[2][4];
-#pragma unroll
-    for (int s = 0; s < 4; ++s)
-#pragma unroll
-        for (int q = 0; q < 2; ++q) kb[s][q] = load_frag<T>(a.kb, s * 2 + q, lane);
-#pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-        for (int s = 0; s < 4; ++s) ka[r][s] = load_frag<T>(a.ka, r * 4 + s, lane);
+    u64 *accNum = reinterpret_cast<u64 *>(smem);
+    u64 *accDen = accNum + FW * FH;
+    uint16_t *kb = reinterpret_cast<uint16_t *>(reinterpret_cast<float *>(smem) + filter_acc_floats(FW, FH));
+    uint16_t *ka = kb + kK64Elems / 2;
+    float *stage = reinterpret_cast<float *>(kb + kK64Elems);
+    float *lsrc = stage;
+    float *lgd = stage + FW * FH;
+    u64 *hsum = reinterpret_cast<u64 *>(stage);  // flush only
+    {
+        // the K64 fragments (kb then ka are contiguous in the table)
+        const uint4 *srcq = reinterpret_cast<const uint4 *>(a.kb);
+        uint4 *dstq = reinterpret_cast<uint4 *>(kb);
+        for (int k = tid; k < kK64Elems / 8; k += kMThreads) dstq[k] = srcq[k];
+    }
 
     for (int pl = 0; pl < 3; ++pl) {
         if (!a.active[pl]) continue;
-        for (int k = tid; k < FW * FH; k += kThreads) accNum[k] = 0.f;
-        for (int k = tid; k < DW * DH; k += kThreads) accDen[k] = 0.f;
-        __syncthreads();
         const float *src = a.src[pl];
         const float *gsrc = a.guide[pl];
+        for (int k = tid; k < FW * FH; k += kMThreads) {
+            accNum[k] = 0;
+            const int py = k / FW, px = k - py * FW;
+            const int yy = oy + py, xx = ox + px;
+            const bool in = yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
+            const size_t o = (size_t)yy * a.pitch + xx;
+            lsrc[k] = in ? src[o] : 0.f;
+            if (WIENER) lgd[k] = in ? gsrc[o] : 0.f;
+        }
+        for (int k = tid; k < DW * DH; k += kMThreads) accDen[k] = 0;
+        __syncthreads();
         const float thr = a.thr[pl], sig2 = a.sig2[pl];
-        for (int r = wave; r < kTileRX * kTileRY; r += kWaves) {
-            const int gxi = tx0 + (r % kTileRX), gyi = ty0 + (r / kTileRX);
+        for (int r = wave; r < kMTileRX * kMTileRY; r += kMWaves) {
+            const int gxi = tx0 + (r % kMTileRX), gyi = ty0 + (r / kMTileRX);
             if (gxi >= a.ngx || gyi >= a.ngy) continue;
             const size_t refIdx = (size_t)gyi * a.ngx + gxi;
             const int *pos = a.pos + refIdx * a.G;
@@ -802,9 +842,9 @@ __global__ __launch_bounds__(kThreads) void filter_mfma_kernel(FilterArgs a)
                 cnb[m] = load_frag<T>(a.cnb, (N - 1) * 2 + m, lane);
             }
             f32x16 y[2];
-            mfma_forward<T>(src, a.pitch, pos, N, lane, kb, cna, y);
             float w;
             if (!WIENER) {
+                mfma_forward<T>(lsrc, FW, ox, oy, pos, N, lane, kb, cna, y);
                 int kept = 0;
 #pragma unroll
                 for (int q = 0; q < 2; ++q)
@@ -817,7 +857,8 @@ __global__ __launch_bounds__(kThreads) void filter_mfma_kernel(FilterArgs a)
                 w = kept < 1 ? 1.f : 1.f / (float)kept;
             } else {
                 f32x16 yg[2];
-                mfma_forward<T>(gsrc, a.pitch, pos, N, lane, kb, cna, yg);
+                mfma_forward<T>(lgd, FW, ox, oy, pos, N, lane, kb, cna, yg);
+                mfma_forward<T>(lsrc, FW, ox, oy, pos, N, lane, kb, cna, y);
                 float l2 = 0.f;
 #pragma unroll
                 for (int q = 0; q < 2; ++q) {
@@ -827,7 +868,8 @@ __global__ __launch_bounds__(kThreads) void filter_mfma_kernel(FilterArgs a)
                     for (int k = 0; k < 16; ++k) {
                         const int kz = 8 * (k >> 2) + 4 * h + (k & 3);  // group-axis frequency
                         const float yy = yg[q][k] * yg[q][k];
-                        const float wc = yy / (yy + (kz == 0 ? 0.5f * su : su));
+                        // 1-ulp reciprocal: the operands already carry 16-bit rounding
+                        const float wc = yy * __builtin_amdgcn_rcpf(yy + (kz == 0 ? 0.5f * su : su));
                         y[q][k] *= wc;
                         l2 = fmaf(wc, wc, l2);
                     }
@@ -836,7 +878,7 @@ __global__ __launch_bounds__(kThreads) void filter_mfma_kernel(FilterArgs a)
                 w = l2 <= 0.f ? 1.f : 1.f / l2;
             }
             f32x16 xh[2];
-            mfma_inverse<T>(y, ka, cnb, xh);
+            mfma_inverse<T>(y, ka, lane, cnb, xh);
             if (c < N) {
                 const int pz = pos[c];
                 const int py = (pz >> 16) - oy, px = (pz & 0xFFFF) - ox + 4 * h;
@@ -846,12 +888,12 @@ __global__ __launch_bounds__(kThreads) void filter_mfma_kernel(FilterArgs a)
                     for (int i = 0; i < 4; ++i)
 #pragma unroll
                         for (int j = 0; j < 4; ++j)
-                            atomicAdd(&accNum[(py + 4 * rr + i) * FW + px + j], w * xh[rr][4 * i + j]);
-                if (h == 0) atomicAdd(&accDen[py * DW + (pz & 0xFFFF) - ox], w);
+                            fix_add(&accNum[(py + 4 * rr + i) * FW + px + j], w * xh[rr][4 * i + j]);
+                if (h == 0) fix_add(&accDen[py * DW + (pz & 0xFFFF) - ox], w);
             }
         }
         __syncthreads();
-        flush_tile(a, pl, accNum, accDen, hsum, ox, oy, tid);
+        flush_tile(a, pl, accNum, accDen, hsum, ox, oy, tid, kMThreads);
         __syncthreads();
     }
 }
@@ -863,7 +905,8 @@ __global__ void finalize_kernel(FinalizeArgs a)
     if (x >= a.W) return;
     const size_t o = (size_t)y * a.pitch + x;
     float val[3] = {0.f, 0.f, 0.f};
-    for (int c = 0; c < a.nplanes; ++c) val[c] = a.active[c] ? a.num[c][o] / a.den[c][o] : a.src[c][o];
+    for (int c = 0; c < a.nplanes; ++c)
+        val[c] = a.active[c] ? (float)((double)(long long)a.num[c][o] / (double)(long long)a.den[c][o]) : a.src[c][o];
     const size_t oo = (size_t)y * a.out_pitch + x;
     if (a.to_rgb) {
         // MatrixConvert_YUV2RGB, OPP, float planes (include/Conversion.hpp:739-762)
@@ -970,17 +1013,17 @@ hipError_t launch_filter(const FilterArgs &a, bool wiener, hipStream_t s)
     return hipGetLastError();
 }
 
-size_t filter_mfma_lds_bytes(const FilterArgs &a) { return filter_mfma_lds(a.FW, a.FH); }
+size_t filter_mfma_lds_bytes(const FilterArgs &a, bool wiener) { return filter_mfma_lds(a.FW, a.FH, wiener); }
 
 hipError_t launch_filter_mfma(const FilterArgs &a, bool wiener, int precision, hipStream_t s)
 {
-    const size_t lds = filter_mfma_lds_bytes(a);
-    const dim3 grid((a.ngx + kTileRX - 1) / kTileRX, (a.ngy + kTileRY - 1) / kTileRY);
+    const size_t lds = filter_mfma_lds_bytes(a, wiener);
+    const dim3 grid((a.ngx + kMTileRX - 1) / kMTileRX, (a.ngy + kMTileRY - 1) / kMTileRY);
     hipError_t e;
 #define LAUNCH_MFMA(T, W)                                                                     \
     do {                                                                                      \
         if ((e = set_lds(filter_mfma_kernel<T, W>, lds)) != hipSuccess) return e;            \
-        hipLaunchKernelGGL((filter_mfma_kernel<T, W>), grid, dim3(kThreads), lds, s, a);     \
+        hipLaunchKernelGGL((filter_mfma_kernel<T, W>), grid, dim3(kMThreads), lds, s, a);     \
     } while (0)
     if (precision == 1) {
         if (wiener) LAUNCH_MFMA(_Float16, true);
