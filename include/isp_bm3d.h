@@ -111,6 +111,23 @@ int isp_bm3d_rgb_f32(isp_bm3d_ctx *ctx, float *dst_r, float *dst_g, float *dst_b
                      const float *ref_r, const float *ref_g, const float *ref_b,
                      int width, int height, size_t stride, int mode, int mem, void *stream);
 
+/* Integer Plane (int32 samples with the quantisation of a Plane:
+ * Floor / Neutral / Ceil, include/Image_Type.h:51-180; chroma iff
+ * Floor < Neutral): BM3D_Base::process_Plane / BM3D::process_Plane
+ * (source/BM3D.cpp:248-279, 475-498): RangeConvert to float, the filter,
+ * RangeConvert back with +0.5 rounding and clip.  Otherwise as
+ * isp_bm3d_plane_f32. */
+int isp_bm3d_plane_i32(isp_bm3d_ctx *ctx, int *dst, const int *src, const int *ref, int width, int height,
+                       size_t stride, int floor_, int neutral, int ceil_, int mode, int mem, void *stream);
+
+/* Integer RGB Frame (int32 R, G, B with Floor / Ceil): process_Frame
+ * (source/BM3D.cpp:282-313, 501-527) exactly, including OPP from int
+ * (include/Conversion.hpp:547-578) and OPP to int with +0.5 rounding and clip
+ * (:660-762).  Otherwise as isp_bm3d_rgb_f32. */
+int isp_bm3d_rgb_i32(isp_bm3d_ctx *ctx, int *dst_r, int *dst_g, int *dst_b, const int *src_r, const int *src_g,
+                     const int *src_b, const int *ref_r, const int *ref_g, const int *ref_b, int width, int height,
+                     size_t stride, int floor_, int ceil_, int mode, int mem, void *stream);
+
 /* Block matching of one step only (BM3D_Base::BlockMatching,
  * source/BM3D.cpp:221-236, include/Block.h:658-703) on a float guide plane,
  * for group-index parity checks.  For grid point g = gy*ngx + gx:

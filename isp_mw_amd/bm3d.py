@@ -7,10 +7,11 @@ reference reads the same:
     out = BM3D(para)(noisy)            # FilterIF2::operator()(src)
     BM3D_Basic(para.basic).process(dst, src, ref)
 
-Planes are 2-D float32 arrays (Plane_FL, luma range [0,1]); frames are
-[3, H, W] float32 RGB arrays in [0,1] (Frame semantics on float samples).
-numpy arrays run through host memory; contiguous float32 torch tensors on a
-HIP device run in place on the current torch stream.
+Planes are 2-D float32 arrays (Plane_FL, luma range [0,1]) or int32 arrays
+(Plane: integer samples with Floor / Neutral / Ceil); frames are [3, H, W]
+RGB arrays, float32 in [0,1] or int32 with Floor / Ceil (Frame).  numpy
+arrays run through host memory; contiguous torch tensors on a HIP device run
+in place on the current torch stream.
 """
 from __future__ import annotations
 
@@ -178,48 +179,64 @@ class _Filter:
             pass
 
     # FilterIF2::process(dst, src[, ref]) (include/Filter.h:173-209)
-    def process(self, dst, src, ref=None):
+    def process(self, dst, src, ref=None, floor=None, neutral=None, ceil=None, bit_depth=8):
+        """Integer samples use Floor / Neutral / Ceil (defaults: full range of
+        bit_depth, luma, i.e. 0 / 0 / 2^bit_depth - 1)."""
         torch_io = _is_torch(src)
         if torch_io:
             import torch
-            if not (src.is_cuda and src.dtype == torch.float32 and src.is_contiguous()):
-                raise ValueError("torch inputs must be contiguous float32 tensors on a HIP device")
+            if not (src.is_cuda and src.dtype in (torch.float32, torch.int32) and src.is_contiguous()):
+                raise ValueError("torch inputs must be contiguous float32 or int32 tensors on a HIP device")
+            is_int = src.dtype == torch.int32
             ptr = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
             stream = ctypes.c_void_p(torch.cuda.current_stream(src.device).cuda_stream)
             mem = L.MEM_DEVICE
         else:
+            is_int = isinstance(src, np.ndarray) and src.dtype == np.int32
+            want = np.int32 if is_int else np.float32
             for a in (dst, src, ref):
-                if a is not None and not (isinstance(a, np.ndarray) and a.dtype == np.float32
-                                          and a.flags.c_contiguous):
-                    raise ValueError("numpy inputs must be C-contiguous float32 arrays")
+                if a is not None and not (isinstance(a, np.ndarray) and a.dtype == want and a.flags.c_contiguous):
+                    raise ValueError("numpy inputs must be C-contiguous float32 (or all int32) arrays")
             ptr = lambda a: ctypes.c_void_p(a.ctypes.data) if a is not None else None  # noqa: E731
             stream = None
             mem = L.MEM_HOST
         if tuple(dst.shape) != tuple(src.shape) or (ref is not None and tuple(ref.shape) != tuple(src.shape)):
             raise ValueError("dst, src and ref must have the same shape")
+        if is_int:
+            floor = 0 if floor is None else int(floor)
+            ceil = (1 << bit_depth) - 1 if ceil is None else int(ceil)
+            neutral = floor if neutral is None else int(neutral)
         lib = L.lib()
         if src.ndim == 2:
             h, w = src.shape
-            rc = lib.isp_bm3d_plane_f32(self._ctx, ptr(dst), ptr(src), ptr(ref), w, h, w, self._mode, mem, stream)
+            if is_int:
+                rc = lib.isp_bm3d_plane_i32(self._ctx, ptr(dst), ptr(src), ptr(ref), w, h, w, floor, neutral, ceil,
+                                            self._mode, mem, stream)
+            else:
+                rc = lib.isp_bm3d_plane_f32(self._ctx, ptr(dst), ptr(src), ptr(ref), w, h, w, self._mode, mem,
+                                            stream)
         elif src.ndim == 3 and src.shape[0] == 3:
             _, h, w = src.shape
             d = [ptr(dst[c]) for c in range(3)]
-            s = [ptr(src[c]) for c in range(3)]
+            s_ = [ptr(src[c]) for c in range(3)]
             r = [ptr(ref[c]) for c in range(3)] if ref is not None else [None] * 3
-            rc = lib.isp_bm3d_rgb_f32(self._ctx, *d, *s, *r, w, h, w, self._mode, mem, stream)
+            if is_int:
+                rc = lib.isp_bm3d_rgb_i32(self._ctx, *d, *s_, *r, w, h, w, floor, ceil, self._mode, mem, stream)
+            else:
+                rc = lib.isp_bm3d_rgb_f32(self._ctx, *d, *s_, *r, w, h, w, self._mode, mem, stream)
         else:
             raise ValueError("expected a [H, W] plane or a [3, H, W] RGB frame")
         L.check(rc, self._ctx)
         return dst
 
     # FilterIF2::operator()(src[, ref]) (include/Filter.h:203-216)
-    def __call__(self, src, ref=None):
+    def __call__(self, src, ref=None, **quant):
         if _is_torch(src):
             import torch
             dst = torch.empty_like(src)
         else:
             dst = np.empty_like(src)
-        return self.process(dst, src, ref)
+        return self.process(dst, src, ref, **quant)
 
     def set_timing(self, enable: bool = True) -> None:
         """Bracket every kernel launch with hipEvents (per-stage device time)."""

@@ -290,8 +290,10 @@ struct StepIO {
     const double *mguide64;    // ... or FP64
     int pitch;                 // of src, guide, mguide32 and the accumulators
     int pitch64;
-    int to_rgb;
-    float *out[3];
+    // finalize: output format (see FinalizeArgs)
+    int to_rgb, out_int, clip;
+    double gain_y, gain_c, offset, q_gain, lo, hi;
+    void *out[3];
     size_t out_pitch;
 };
 
@@ -311,6 +313,9 @@ int run_match(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, const Derived &d
     a.bypass = d.bypass;
     a.thSSE = d.thSSE;
     a.distMul = d.distMul;
+    // the plain FP32 path reproduces the reference's tie order bit for bit;
+    // the mixed-precision forms make no parity claim and use fused FMA
+    a.exact = ctx->precision == ISP_BM3D_PREC_FP32;
     a.tileW = (kTileRX - 1) * p.block_step + 2 * d.range + kBlock + (kStrip - 1) * p.bm_step;
     a.tileH = (kTileRY - 1) * p.block_step + 2 * d.range + kBlock;
     a.tilePitch = a.tileW | 1;
@@ -347,6 +352,14 @@ int run_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, cons
     fa.pitch = io.pitch;
     fa.nplanes = io.nplanes;
     fa.to_rgb = io.to_rgb;
+    fa.out_int = io.out_int;
+    fa.clip = io.clip;
+    fa.gain_y = io.gain_y;
+    fa.gain_c = io.gain_c;
+    fa.offset = io.offset;
+    fa.q_gain = io.q_gain;
+    fa.lo = io.lo;
+    fa.hi = io.hi;
     fa.out_pitch = io.out_pitch;
     for (int c = 0; c < 3; ++c) {
         fa.src[c] = io.src[c];
@@ -749,94 +762,205 @@ int isp_bm3d_grid_size(const isp_bm3d_ctx *ctx, int W, int H, int final_step, in
     return ISP_BM3D_OK;
 }
 
-int isp_bm3d_plane_f32(isp_bm3d_ctx *ctx, float *dst, const float *src, const float *ref, int W, int H,
-                       size_t stride, int mode, int mem, void *stream)
+}  // extern "C"
+
+namespace {
+
+// Sample format of the caller's planes: float (Plane_FL: luma [0,1]) or
+// int32 with the quantisation of a Plane (include/Image_Type.hpp:85-117).
+struct Fmt {
+    bool is_int;
+    int floor_, neutral, ceil_;
+    bool chroma() const { return floor_ < neutral; }                                      // isChroma
+    bool pc_chroma() const { return floor_ < neutral && (floor_ + ceil_) % 2 == 1; }       // isPCChroma
+};
+
+void set_plain_out(StepIO &io, void *out, size_t pitch)
+{
+    io.to_rgb = 0;
+    io.out_int = 0;
+    io.clip = 0;
+    io.out[0] = out;
+    io.out_pitch = pitch;
+}
+
+// RangeConvert float -> int with clip (include/Conversion.hpp:241-256)
+void set_int_plane_out(StepIO &io, const Fmt &f, void *out, size_t pitch)
+{
+    io.to_rgb = 0;
+    io.out_int = 1;
+    io.q_gain = (double)(f.ceil_ - f.floor_) / 1.0;
+    io.offset = -0.0 * io.q_gain + (double)(f.chroma() ? f.neutral : f.floor_) + (f.pc_chroma() ? 0.499999 : 0.5);
+    io.clip = 1;
+    io.lo = f.floor_;
+    io.hi = f.ceil_;
+    io.out[0] = out;
+    io.out_pitch = pitch;
+}
+
+int copy_plane(isp_bm3d_ctx *ctx, void *dst, size_t dpitch, const void *src, size_t spitch, int W, int H,
+               hipStream_t s)
+{
+    StageTimer tm(ctx, ST_COPY, s);
+    HIP_CHECK(ctx, launch_copy2d((float *)dst, dpitch, (const float *)src, spitch, W, H, s));
+    return ISP_BM3D_OK;
+}
+
+// One plane: Plane_FL (source/BM3D.cpp:80-136, 239-245, 464-472) or, for int
+// samples, Plane (RangeConvert in and out, :248-279, 475-498).
+int plane_common(isp_bm3d_ctx *ctx, void *dst, const void *src, const void *ref, int W, int H, size_t stride,
+                 const Fmt &fmt, int mode, int mem, void *stream)
 {
     int rc = check_common(ctx, W, H, stride, mode, mem);
     if (rc) return rc;
     if (!dst || !src) return fail(ctx, ISP_BM3D_E_ARG, "null dst/src");
+    if (fmt.is_int && !(fmt.floor_ < fmt.ceil_ && fmt.floor_ <= fmt.neutral && fmt.neutral <= fmt.ceil_))
+        return fail(ctx, ISP_BM3D_E_ARG, "bad quantisation floor %d neutral %d ceil %d", fmt.floor_, fmt.neutral,
+                    fmt.ceil_);
     hipStream_t s = (hipStream_t)stream;
     const bool host = mem == ISP_BM3D_MEM_HOST;
     const int P = internal_pitch(W, stride, mem);
     Work w{};
     if ((rc = prepare(ctx, w, P, H, W, false))) return rc;
     const bool sep_ref = ref && ref != src;
-    const float *dsrc = src, *dref = sep_ref ? ref : src;
-    float *ddst = dst;
-    if (host) {
-        HIP_CHECK(ctx, hipMemcpy2DAsync(w.plane[0], (size_t)P * 4, src, stride * 4, (size_t)W * 4, H,
-                                        hipMemcpyHostToDevice, s));
-        dsrc = w.plane[0];
-        dref = dsrc;
-        if (sep_ref) {
-            HIP_CHECK(ctx, hipMemcpy2DAsync(w.refp[0], (size_t)P * 4, ref, stride * 4, (size_t)W * 4, H,
-                                            hipMemcpyHostToDevice, s));
-            dref = w.refp[0];
+    const size_t bw = (size_t)W * 4;
+    // BM3D_Base::process_Plane's early return for the single steps on int planes
+    if (fmt.is_int && mode != ISP_BM3D_BOTH) {
+        const isp_bm3d_step_params &p = mode == ISP_BM3D_BASIC ? ctx->params.basic : ctx->params.final_;
+        double norm[3];
+        opp_norms(norm);
+        if (!(p.sigma[0] > 0) || p.group_size == 1 || p.block_size <= 0 || p.bm_range <= 0 ||
+            p.bm_range < p.bm_step || p.thMSE * norm[0] <= 0) {
+            if (host) {
+                HIP_CHECK(ctx, hipMemcpy2DAsync(w.out[0], (size_t)P * 4, src, stride * 4, bw, H,
+                                                hipMemcpyHostToDevice, s));
+                HIP_CHECK(ctx, hipMemcpy2DAsync(dst, stride * 4, w.out[0], (size_t)P * 4, bw, H,
+                                                hipMemcpyDeviceToHost, s));
+                HIP_CHECK(ctx, hipStreamSynchronize(s));
+                return ISP_BM3D_OK;
+            }
+            return dst == src ? ISP_BM3D_OK : copy_plane(ctx, dst, stride, src, stride, W, H, s);
         }
-        ddst = w.out[0];
     }
+    // source and guide as float planes of pitch P
+    const void *rsrc = src, *rref = sep_ref ? ref : src;
+    size_t rpitch = stride;
+    if (host) {
+        float *stage_s = fmt.is_int ? w.out[1] : w.plane[0];
+        float *stage_r = fmt.is_int ? w.out[2] : w.refp[0];
+        HIP_CHECK(ctx, hipMemcpy2DAsync(stage_s, (size_t)P * 4, src, stride * 4, bw, H, hipMemcpyHostToDevice, s));
+        rsrc = stage_s;
+        rref = stage_s;
+        if (sep_ref) {
+            HIP_CHECK(ctx, hipMemcpy2DAsync(stage_r, (size_t)P * 4, ref, stride * 4, bw, H, hipMemcpyHostToDevice, s));
+            rref = stage_r;
+        }
+        rpitch = (size_t)P;
+    }
+    const float *fsrc = (const float *)rsrc, *fref = (const float *)rref;
+    if (fmt.is_int) {
+        // RangeConvert int -> float, no clip except for PC-range chroma
+        ToFloatArgs t{};
+        t.W = W;
+        t.H = H;
+        t.spitch = rpitch;
+        t.dpitch = P;
+        t.gain = 1.0 / (double)(fmt.ceil_ - fmt.floor_);
+        t.offset = -(double)(fmt.chroma() ? fmt.neutral : fmt.floor_) * t.gain + 0.0;
+        t.clip = fmt.pc_chroma();
+        t.lo = fmt.chroma() ? -0.5 : 0.0;
+        t.hi = fmt.chroma() ? 0.5 : 1.0;
+        {
+            // the matching guide also in FP64, as the reference holds it
+            StageTimer tm(ctx, ST_OPP, s);
+            t.src = (const int *)rsrc;
+            t.dst = w.plane[0];
+            t.dst64 = sep_ref ? nullptr : w.y64;
+            HIP_CHECK(ctx, launch_to_float(t, s));
+            fsrc = w.plane[0];
+            fref = fsrc;
+            if (sep_ref) {
+                t.src = (const int *)rref;
+                t.dst = w.refp[0];
+                t.dst64 = w.y64;
+                HIP_CHECK(ctx, launch_to_float(t, s));
+                fref = w.refp[0];
+            }
+        }
+    }
+    void *out = host ? (void *)w.out[0] : dst;
+    const size_t out_pitch = host ? (size_t)P : stride;
     StepIO io{};
     io.nplanes = 1;
     io.pitch = P;
-    io.out_pitch = (size_t)P;
-    io.src[0] = dsrc;
+    io.src[0] = fsrc;
+    auto final_out = [&](StepIO &x) {
+        if (fmt.is_int) set_int_plane_out(x, fmt, out, out_pitch);
+        else set_plain_out(x, out, out_pitch);
+    };
+    io.pitch64 = P;
+    const double *g64 = fmt.is_int ? w.y64 : nullptr;
     if (mode == ISP_BM3D_BASIC || mode == ISP_BM3D_FINAL) {
         const bool fin = mode == ISP_BM3D_FINAL;
-        io.guide[0] = dref;
-        io.mguide32 = dref;
-        io.out[0] = ddst;
+        io.guide[0] = fref;
+        io.mguide32 = fref;
+        io.mguide64 = g64;
+        final_out(io);
         rc = run_step(ctx, fin ? ctx->params.final_ : ctx->params.basic, fin, io, w, W, H, s);
     } else {
-        io.mguide32 = dref;
-        io.guide[0] = dref;
-        io.out[0] = w.basic[0];
+        io.mguide32 = fref;
+        io.mguide64 = g64;
+        io.guide[0] = fref;
+        set_plain_out(io, w.basic[0], (size_t)P);
         rc = run_step(ctx, ctx->params.basic, false, io, w, W, H, s);
         if (!rc) {
             io.mguide32 = w.basic[0];
+            io.mguide64 = nullptr;
             io.guide[0] = w.basic[0];
-            io.out[0] = ddst;
+            final_out(io);
             rc = run_step(ctx, ctx->params.final_, true, io, w, W, H, s);
         }
     }
     if (rc) return rc;
     if (host) {
-        HIP_CHECK(ctx, hipMemcpy2DAsync(dst, stride * 4, ddst, (size_t)P * 4, (size_t)W * 4, H,
-                                        hipMemcpyDeviceToHost, s));
+        HIP_CHECK(ctx, hipMemcpy2DAsync(dst, stride * 4, out, (size_t)P * 4, bw, H, hipMemcpyDeviceToHost, s));
         HIP_CHECK(ctx, hipStreamSynchronize(s));
     }
     return ISP_BM3D_OK;
 }
 
-int isp_bm3d_rgb_f32(isp_bm3d_ctx *ctx, float *dst_r, float *dst_g, float *dst_b, const float *src_r,
-                     const float *src_g, const float *src_b, const float *ref_r, const float *ref_g,
-                     const float *ref_b, int W, int H, size_t stride, int mode, int mem, void *stream)
+// RGB frame: process_Frame (source/BM3D.cpp:282-313, 501-527) on float RGB in
+// [0,1] or on int32 RGB with Floor / Ceil.
+int rgb_common(isp_bm3d_ctx *ctx, void *const *dst, const void *const *src, const void *const *ref, int W, int H,
+               size_t stride, const Fmt &fmt, int mode, int mem, void *stream)
 {
     int rc = check_common(ctx, W, H, stride, mode, mem);
     if (rc) return rc;
-    if (!dst_r || !dst_g || !dst_b || !src_r || !src_g || !src_b)
+    if (!dst[0] || !dst[1] || !dst[2] || !src[0] || !src[1] || !src[2])
         return fail(ctx, ISP_BM3D_E_ARG, "null dst/src plane");
-    const bool has_ref = ref_r || ref_g || ref_b;
-    if (has_ref && !(ref_r && ref_g && ref_b)) return fail(ctx, ISP_BM3D_E_ARG, "ref needs all three planes");
-    const bool sep_ref = has_ref && (ref_r != src_r || ref_g != src_g || ref_b != src_b);
+    if (fmt.is_int && !(fmt.floor_ < fmt.ceil_)) return fail(ctx, ISP_BM3D_E_ARG, "bad quantisation");
+    const bool has_ref = ref[0] || ref[1] || ref[2];
+    if (has_ref && !(ref[0] && ref[1] && ref[2])) return fail(ctx, ISP_BM3D_E_ARG, "ref needs all three planes");
+    const bool sep_ref = has_ref && (ref[0] != src[0] || ref[1] != src[1] || ref[2] != src[2]);
     hipStream_t s = (hipStream_t)stream;
     const bool host = mem == ISP_BM3D_MEM_HOST;
     const int P = internal_pitch(W, stride, mem);
     Work w{};
     if ((rc = prepare(ctx, w, P, H, W, false))) return rc;
+    const size_t bw = (size_t)W * 4;
 
-    const float *rgb[3] = {src_r, src_g, src_b};
-    const float *rrgb[3] = {ref_r, ref_g, ref_b};
-    float *drgb[3] = {dst_r, dst_g, dst_b};
+    const void *rgb[3] = {src[0], src[1], src[2]};
+    const void *rrgb[3] = {ref[0], ref[1], ref[2]};
     size_t in_pitch = stride;
     if (host) {
         // stage RGB in the out/basic planes, which are free until later
         for (int c = 0; c < 3; ++c) {
-            HIP_CHECK(ctx, hipMemcpy2DAsync(w.out[c], (size_t)P * 4, rgb[c], stride * 4, (size_t)W * 4, H,
+            HIP_CHECK(ctx, hipMemcpy2DAsync(w.out[c], (size_t)P * 4, rgb[c], stride * 4, bw, H,
                                             hipMemcpyHostToDevice, s));
             rgb[c] = w.out[c];
             if (sep_ref) {
-                HIP_CHECK(ctx, hipMemcpy2DAsync(w.basic[c], (size_t)P * 4, rrgb[c], stride * 4, (size_t)W * 4,
-                                                H, hipMemcpyHostToDevice, s));
+                HIP_CHECK(ctx, hipMemcpy2DAsync(w.basic[c], (size_t)P * 4, rrgb[c], stride * 4, bw, H,
+                                                hipMemcpyHostToDevice, s));
                 rrgb[c] = w.basic[c];
             }
         }
@@ -851,23 +975,33 @@ int isp_bm3d_rgb_f32(isp_bm3d_ctx *ctx, float *dst_r, float *dst_g, float *dst_b
             p.block_size <= 0 || p.bm_range <= 0 || p.bm_range < p.bm_step || p.thMSE * norm[0] <= 0) {
             for (int c = 0; c < 3; ++c) {
                 if (host) {
-                    HIP_CHECK(ctx, hipMemcpy2DAsync(drgb[c], stride * 4, rgb[c], in_pitch * 4, (size_t)W * 4, H,
+                    HIP_CHECK(ctx, hipMemcpy2DAsync(dst[c], stride * 4, rgb[c], in_pitch * 4, bw, H,
                                                     hipMemcpyDeviceToHost, s));
-                } else if (drgb[c] != rgb[c]) {
-                    HIP_CHECK(ctx, launch_copy2d(drgb[c], stride, rgb[c], in_pitch, W, H, s));
+                } else if (dst[c] != rgb[c]) {
+                    if ((rc = copy_plane(ctx, dst[c], stride, rgb[c], in_pitch, W, H, s))) return rc;
                 }
             }
             if (host) HIP_CHECK(ctx, hipStreamSynchronize(s));
             return ISP_BM3D_OK;
         }
     }
-    // RGB -> OPP of src (FP32 planes + FP64 luma guide)
+    // RGB -> OPP (float planes + FP64 luma guide)
     OppArgs o{};
     o.W = W;
     o.H = H;
     o.r = rgb[0];
     o.g = rgb[1];
     o.b = rgb[2];
+    o.in_int = fmt.is_int;
+    {
+        const double sRange = fmt.is_int ? (double)(fmt.ceil_ - fmt.floor_) : 1.0;
+        const double sFloor = fmt.is_int ? (double)fmt.floor_ : 0.0;
+        o.gainY = 1.0 / (sRange * 3.0);
+        o.offsetY = -sFloor * 3.0 * o.gainY + 0.0;
+        o.gainU = 1.0 / (sRange * 2.0);
+        o.gainV = 1.0 / (sRange * 4.0);
+        o.offsetC = 0.0;
+    }
     o.in_pitch = in_pitch;
     o.y = w.plane[0];
     o.u = w.plane[1];
@@ -878,28 +1012,42 @@ int isp_bm3d_rgb_f32(isp_bm3d_ctx *ctx, float *dst_r, float *dst_g, float *dst_b
     {
         StageTimer tm_opp(ctx, ST_OPP, s);
         HIP_CHECK(ctx, launch_opp(o, s));
-    }
-    if (sep_ref) {
-        OppArgs q = o;
-        q.r = rrgb[0];
-        q.g = rrgb[1];
-        q.b = rrgb[2];
-        q.y64 = w.ref64;
-        if (mode == ISP_BM3D_FINAL) {  // BM3D_Final::RGB2YUV converts the whole guide
-            q.y = w.refp[0];
-            q.u = w.refp[1];
-            q.v = w.refp[2];
-        } else {                       // BM3D_Basic::RGB2YUV: ConvertToY only
-            q.y = q.u = q.v = nullptr;
+        if (sep_ref) {
+            OppArgs q = o;
+            q.r = rrgb[0];
+            q.g = rrgb[1];
+            q.b = rrgb[2];
+            q.y64 = w.ref64;
+            if (mode == ISP_BM3D_FINAL) {  // BM3D_Final::RGB2YUV converts the whole guide
+                q.y = w.refp[0];
+                q.u = w.refp[1];
+                q.v = w.refp[2];
+            } else {                       // BM3D_Basic::RGB2YUV: ConvertToY only
+                q.y = q.u = q.v = nullptr;
+            }
+            HIP_CHECK(ctx, launch_opp(q, s));
         }
-        HIP_CHECK(ctx, launch_opp(q, s));
     }
-    float *out[3] = {drgb[0], drgb[1], drgb[2]};
+    void *out[3] = {dst[0], dst[1], dst[2]};
     size_t out_pitch = stride;
     if (host) {
         for (int c = 0; c < 3; ++c) out[c] = w.out[c];
         out_pitch = (size_t)P;
     }
+    auto rgb_out = [&](StepIO &x) {
+        // MatrixConvert_YUV2RGB OPP with clip (include/Conversion.hpp:660-762)
+        x.to_rgb = 1;
+        x.out_int = fmt.is_int;
+        x.clip = 1;
+        const double dRange = fmt.is_int ? (double)(fmt.ceil_ - fmt.floor_) : 1.0;
+        x.gain_y = dRange / 1.0;
+        x.gain_c = dRange / 1.0;
+        x.offset = fmt.is_int ? (double)fmt.floor_ + 0.5 : 0.0;
+        x.lo = fmt.is_int ? fmt.floor_ : 0.0;
+        x.hi = fmt.is_int ? fmt.ceil_ : 1.0;
+        for (int c = 0; c < 3; ++c) x.out[c] = out[c];
+        x.out_pitch = out_pitch;
+    };
     StepIO io{};
     io.nplanes = 3;
     io.pitch = P;
@@ -908,16 +1056,12 @@ int isp_bm3d_rgb_f32(isp_bm3d_ctx *ctx, float *dst_r, float *dst_g, float *dst_b
     if (mode == ISP_BM3D_BASIC) {
         io.mguide64 = sep_ref ? w.ref64 : w.y64;
         for (int c = 0; c < 3; ++c) io.guide[c] = w.plane[c];
-        io.to_rgb = 1;
-        for (int c = 0; c < 3; ++c) io.out[c] = out[c];
-        io.out_pitch = out_pitch;
+        rgb_out(io);
         rc = run_step(ctx, ctx->params.basic, false, io, w, W, H, s);
     } else if (mode == ISP_BM3D_FINAL) {
         io.mguide64 = sep_ref ? w.ref64 : w.y64;
         for (int c = 0; c < 3; ++c) io.guide[c] = sep_ref ? w.refp[c] : w.plane[c];
-        io.to_rgb = 1;
-        for (int c = 0; c < 3; ++c) io.out[c] = out[c];
-        io.out_pitch = out_pitch;
+        rgb_out(io);
         rc = run_step(ctx, ctx->params.final_, true, io, w, W, H, s);
     } else {
         io.mguide64 = sep_ref ? w.ref64 : w.y64;
@@ -926,28 +1070,64 @@ int isp_bm3d_rgb_f32(isp_bm3d_ctx *ctx, float *dst_r, float *dst_g, float *dst_b
             io.out[c] = w.basic[c];
         }
         io.to_rgb = 0;
+        io.out_int = 0;
+        io.clip = 0;
         io.out_pitch = (size_t)P;
         rc = run_step(ctx, ctx->params.basic, false, io, w, W, H, s);
         if (!rc) {
             io.mguide64 = nullptr;
             io.mguide32 = w.basic[0];
-            for (int c = 0; c < 3; ++c) {
-                io.guide[c] = w.basic[c];
-                io.out[c] = out[c];
-            }
-            io.to_rgb = 1;
-            io.out_pitch = out_pitch;
+            for (int c = 0; c < 3; ++c) io.guide[c] = w.basic[c];
+            rgb_out(io);
             rc = run_step(ctx, ctx->params.final_, true, io, w, W, H, s);
         }
     }
     if (rc) return rc;
     if (host) {
         for (int c = 0; c < 3; ++c)
-            HIP_CHECK(ctx, hipMemcpy2DAsync(drgb[c], stride * 4, out[c], (size_t)P * 4, (size_t)W * 4, H,
+            HIP_CHECK(ctx, hipMemcpy2DAsync(dst[c], stride * 4, out[c], (size_t)P * 4, bw, H,
                                             hipMemcpyDeviceToHost, s));
         HIP_CHECK(ctx, hipStreamSynchronize(s));
     }
     return ISP_BM3D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int isp_bm3d_plane_f32(isp_bm3d_ctx *ctx, float *dst, const float *src, const float *ref, int W, int H,
+                       size_t stride, int mode, int mem, void *stream)
+{
+    const Fmt f{false, 0, 0, 1};
+    return plane_common(ctx, dst, src, ref, W, H, stride, f, mode, mem, stream);
+}
+
+int isp_bm3d_plane_i32(isp_bm3d_ctx *ctx, int *dst, const int *src, const int *ref, int W, int H, size_t stride,
+                       int floor_, int neutral, int ceil_, int mode, int mem, void *stream)
+{
+    const Fmt f{true, floor_, neutral, ceil_};
+    return plane_common(ctx, dst, src, ref, W, H, stride, f, mode, mem, stream);
+}
+
+int isp_bm3d_rgb_f32(isp_bm3d_ctx *ctx, float *dst_r, float *dst_g, float *dst_b, const float *src_r,
+                     const float *src_g, const float *src_b, const float *ref_r, const float *ref_g,
+                     const float *ref_b, int W, int H, size_t stride, int mode, int mem, void *stream)
+{
+    void *d[3] = {dst_r, dst_g, dst_b};
+    const void *sv[3] = {src_r, src_g, src_b}, *r[3] = {ref_r, ref_g, ref_b};
+    const Fmt f{false, 0, 0, 1};
+    return rgb_common(ctx, d, sv, r, W, H, stride, f, mode, mem, stream);
+}
+
+int isp_bm3d_rgb_i32(isp_bm3d_ctx *ctx, int *dst_r, int *dst_g, int *dst_b, const int *src_r, const int *src_g,
+                     const int *src_b, const int *ref_r, const int *ref_g, const int *ref_b, int W, int H,
+                     size_t stride, int floor_, int ceil_, int mode, int mem, void *stream)
+{
+    void *d[3] = {dst_r, dst_g, dst_b};
+    const void *sv[3] = {src_r, src_g, src_b}, *r[3] = {ref_r, ref_g, ref_b};
+    const Fmt f{true, floor_, floor_, ceil_};
+    return rgb_common(ctx, d, sv, r, W, H, stride, f, mode, mem, stream);
 }
 
 int isp_bm3d_match(isp_bm3d_ctx *ctx, const float *guide, int W, int H, size_t stride, int final_step,

@@ -36,6 +36,7 @@ struct MatchArgs {
     int step;                // BMstep
     int G;                   // GroupSize
     int bypass;              // 1: every group is the reference block alone
+    int exact;               // 1: distances rounded term by term as the reference does
     double thSSE;            // thMSE * normY * MSE2SSE
     double distMul;          // 1 / MSE2SSE
     int tileW, tileH, tilePitch;  // staged guide tile (doubles)
@@ -88,19 +89,40 @@ struct FinalizeArgs {
     const float *src[3];      // copied through for inactive planes
     int active[3];
     int nplanes;
-    int to_rgb;               // OPP -> RGB with clip
-    float *out[3];
+    int to_rgb;               // OPP -> RGB (include/Conversion.hpp:660-762)
+    int out_int;              // 0: float outputs, 1: int32 outputs (+0.5 rounding)
+    double gain_y, gain_c;    // RGB: YUV->RGB gains (dRange / sRange)
+    double offset;            // RGB: dFloor (+0.5 for int); plane int: Neutral + 0.5
+    double q_gain;            // plane int: range
+    int clip;                 // clip outputs to [lo, hi]
+    double lo, hi;
+    void *out[3];
     size_t out_pitch;
 };
 
 struct OppArgs {
     int W, H;
-    const float *r, *g, *b;
+    const void *r, *g, *b;     // float or int32 samples
+    int in_int;
+    double gainY, offsetY, gainU, gainV, offsetC;
     size_t in_pitch;
     float *y, *u, *v;          // may be null
     int pitch;
     double *y64;               // may be null (matching guide)
     int pitch64;
+};
+
+// int32 plane -> float plane: v * gain + offset (RangeConvert), optional clip
+struct ToFloatArgs {
+    int W, H;
+    const int *src;
+    size_t spitch;
+    float *dst;
+    int dpitch;
+    double *dst64;             // may be null: the same values in FP64 (matching guide)
+    double gain, offset;
+    int clip;
+    double lo, hi;
 };
 
 size_t match_lds_bytes(const MatchArgs &a);
@@ -114,6 +136,7 @@ hipError_t launch_filter_mfma(const FilterArgs &a, bool wiener, int precision, h
 size_t filter_mfma_lds_bytes(const FilterArgs &a, bool wiener);
 hipError_t launch_finalize(const FinalizeArgs &a, hipStream_t s);
 hipError_t launch_opp(const OppArgs &a, hipStream_t s);
+hipError_t launch_to_float(const ToFloatArgs &a, hipStream_t s);
 hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s);
 hipError_t launch_copy2d(float *dst, size_t dpitch, const float *src, size_t spitch, int W, int H,
                          hipStream_t s);

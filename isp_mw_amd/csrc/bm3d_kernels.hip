@@ -101,7 +101,24 @@ __host__ __device__ inline size_t match_wave_bytes(int maxCand)
 // reference's row-major order; accepted candidates (SSE <= thSSE, SSE != 0)
 // are compacted into a per-wave LDS list, and the G-1 smallest by
 // (key, scan index) are found with an MSB-first radix select on the key bits.
-template <typename GT, bool ST1>
+// EXACT: each term rounded as d*d then added, as the reference's FP64 loop
+// does (include/Block.h:600-603), so equal-looking distances of quantised
+// data tie exactly as they do there; otherwise fused multiply-add.
+__device__ __forceinline__ double sq_acc_exact(double d, double acc)
+{
+#pragma clang fp contract(off)
+    const double t = d * d;
+    return acc + t;
+}
+
+template <bool EXACT>
+__device__ __forceinline__ double sq_acc(double d, double acc)
+{
+    if (EXACT) return sq_acc_exact(d, acc);
+    return __builtin_fma(d, d, acc);
+}
+
+template <typename GT, bool ST1, bool EXACT>
 __global__ __launch_bounds__(kThreads) void match_kernel(const GT *__restrict__ guide, int gpitch,
                                                          MatchArgs a)
 {
@@ -179,7 +196,7 @@ __global__ __launch_bounds__(kThreads) void match_kernel(const GT *__restrict__ 
 #pragma unroll
                             for (int v = 0; v < kBlock; ++v) {
                                 const double d = rv[v] - cv[c + v];
-                                acc[c] = __builtin_fma(d, d, acc[c]);
+                                acc[c] = sq_acc<EXACT>(d, acc[c]);
                             }
                     } else {
 #pragma unroll
@@ -187,7 +204,7 @@ __global__ __launch_bounds__(kThreads) void match_kernel(const GT *__restrict__ 
 #pragma unroll
                             for (int v = 0; v < kBlock; ++v) {
                                 const double d = rv[v] - tc[u * a.tilePitch + c * st + v];
-                                acc[c] = __builtin_fma(d, d, acc[c]);
+                                acc[c] = sq_acc<EXACT>(d, acc[c]);
                             }
                     }
                 }
@@ -899,48 +916,80 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_kernel(FilterArgs a)
 }
 
 // ---------------------------------------------------------------------------
+// num/den, then either the plane output (float, or int with RangeConvert
+// rounding and clip, include/Conversion.hpp:241-256) or OPP -> RGB
+// (MatrixConvert_YUV2RGB, include/Conversion.hpp:660-762) with clip, float or
+// int.  FP64 arithmetic.
 __global__ void finalize_kernel(FinalizeArgs a)
 {
+    // separate mul and add roundings, as the reference's double code
+#pragma clang fp contract(off)
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= a.W) return;
     const size_t o = (size_t)y * a.pitch + x;
-    float val[3] = {0.f, 0.f, 0.f};
+    double val[3] = {0.0, 0.0, 0.0};
     for (int c = 0; c < a.nplanes; ++c)
-        val[c] = a.active[c] ? (float)((double)(long long)a.num[c][o] / (double)(long long)a.den[c][o]) : a.src[c][o];
+        val[c] = a.active[c] ? (double)(long long)a.num[c][o] / (double)(long long)a.den[c][o] : (double)a.src[c][o];
     const size_t oo = (size_t)y * a.out_pitch + x;
+    double res[3];
+    int n = a.nplanes;
     if (a.to_rgb) {
-        // MatrixConvert_YUV2RGB, OPP, float planes (include/Conversion.hpp:739-762)
         const double Y = val[0], U = val[1], V = val[2];
-        const double Rv = (double)(2.0L / 3.0L), Gv = (double)(-4.0L / 3.0L), Bv = Rv;
-        double r = 1.0 * Y + 1.0 * U + Rv * V + 0.0;
-        double g = 1.0 * Y + Gv * V + 0.0;
-        double b = 1.0 * Y + -1.0 * U + Bv * V + 0.0;
-        r = r < 0.0 ? 0.0 : (r > 1.0 ? 1.0 : r);
-        g = g < 0.0 ? 0.0 : (g > 1.0 ? 1.0 : g);
-        b = b < 0.0 ? 0.0 : (b > 1.0 ? 1.0 : b);
-        a.out[0][oo] = (float)r;
-        a.out[1][oo] = (float)g;
-        a.out[2][oo] = (float)b;
+        const double Ry = 1.0 * a.gain_y, Ru = 1.0 * a.gain_c, Rv = (double)(2.0L / 3.0L) * a.gain_c;
+        const double Gy = 1.0 * a.gain_y, Gv = (double)(-4.0L / 3.0L) * a.gain_c;
+        const double By = 1.0 * a.gain_y, Bu = -1.0 * a.gain_c, Bv = (double)(2.0L / 3.0L) * a.gain_c;
+        res[0] = Ry * Y + Ru * U + Rv * V + a.offset;
+        res[1] = Gy * Y + Gv * V + a.offset;
+        res[2] = By * Y + Bu * U + Bv * V + a.offset;
+        n = 3;
     } else {
-        for (int c = 0; c < a.nplanes; ++c) a.out[c][oo] = val[c];
+        for (int c = 0; c < n; ++c) res[c] = a.out_int ? val[c] * a.q_gain + a.offset : val[c];
+    }
+    for (int c = 0; c < n; ++c) {
+        double v = res[c];
+        if (a.clip) v = v < a.lo ? a.lo : (v > a.hi ? a.hi : v);
+        if (a.out_int) static_cast<int *>(a.out[c])[oo] = (int)v;
+        else static_cast<float *>(a.out[c])[oo] = (float)v;
     }
 }
 
-// MatrixConvert_RGB2YUV / ConvertToY, OPP, float planes with Floor 0 and
-// Ceil 1 (include/Conversion.hpp:418-432, 547-578), in FP64.
+// MatrixConvert_RGB2YUV / ConvertToY, OPP (include/Conversion.hpp:418-432,
+// 547-578) from float or int32 RGB, in FP64.
 __global__ void opp_kernel(OppArgs a)
 {
+    // separate mul and add roundings, as the reference's double code
+#pragma clang fp contract(off)
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= a.W) return;
     const size_t i = (size_t)y * a.in_pitch + x;
-    const double R = a.r[i], G = a.g[i], B = a.b[i];
-    const double gainY = 1.0 / (1.0 * 3.0), gainU = 1.0 / (1.0 * 2.0), gainV = 1.0 / (1.0 * 4.0);
-    const double Y = (R + G + B) * gainY + 0.0;
+    double R, G, B;
+    if (a.in_int) {
+        R = static_cast<const int *>(a.r)[i];
+        G = static_cast<const int *>(a.g)[i];
+        B = static_cast<const int *>(a.b)[i];
+    } else {
+        R = static_cast<const float *>(a.r)[i];
+        G = static_cast<const float *>(a.g)[i];
+        B = static_cast<const float *>(a.b)[i];
+    }
+    const double Y = (R + G + B) * a.gainY + a.offsetY;
     const size_t o = (size_t)y * a.pitch + x;
     if (a.y64) a.y64[(size_t)y * a.pitch64 + x] = Y;
     if (a.y) a.y[o] = (float)Y;
-    if (a.u) a.u[o] = (float)((R - B) * gainU + 0.0);
-    if (a.v) a.v[o] = (float)((R - G * 2.0 + B) * gainV + 0.0);
+    if (a.u) a.u[o] = (float)((R - B) * a.gainU + a.offsetC);
+    if (a.v) a.v[o] = (float)((R - G * 2.0 + B) * a.gainV + a.offsetC);
+}
+
+__global__ void to_float_kernel(ToFloatArgs a)
+{
+    // separate mul and add roundings, as the reference's double code
+#pragma clang fp contract(off)
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= a.W) return;
+    double v = (double)a.src[(size_t)y * a.spitch + x] * a.gain + a.offset;
+    if (a.clip) v = v < a.lo ? a.lo : (v > a.hi ? a.hi : v);
+    a.dst[(size_t)y * a.dpitch + x] = (float)v;
+    if (a.dst64) a.dst64[(size_t)y * a.dpitch + x] = v;
 }
 
 __global__ void fill_kernel(float *p, size_t n, float v)
@@ -963,18 +1012,18 @@ hipError_t set_lds(K kernel, size_t bytes)
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-template <typename GT>
+template <typename GT, bool EXACT>
 hipError_t launch_match_t(const GT *guide, int gpitch, const MatchArgs &a, hipStream_t s)
 {
     const size_t lds = match_lds_bytes(a);
     const dim3 grid((a.ngx + kTileRX - 1) / kTileRX, (a.ngy + kTileRY - 1) / kTileRY);
     hipError_t e;
     if (a.step == 1) {
-        if ((e = set_lds(match_kernel<GT, true>, lds)) != hipSuccess) return e;
-        hipLaunchKernelGGL((match_kernel<GT, true>), grid, dim3(kThreads), lds, s, guide, gpitch, a);
+        if ((e = set_lds(match_kernel<GT, true, EXACT>, lds)) != hipSuccess) return e;
+        hipLaunchKernelGGL((match_kernel<GT, true, EXACT>), grid, dim3(kThreads), lds, s, guide, gpitch, a);
     } else {
-        if ((e = set_lds(match_kernel<GT, false>, lds)) != hipSuccess) return e;
-        hipLaunchKernelGGL((match_kernel<GT, false>), grid, dim3(kThreads), lds, s, guide, gpitch, a);
+        if ((e = set_lds(match_kernel<GT, false, EXACT>, lds)) != hipSuccess) return e;
+        hipLaunchKernelGGL((match_kernel<GT, false, EXACT>), grid, dim3(kThreads), lds, s, guide, gpitch, a);
     }
     return hipGetLastError();
 }
@@ -990,12 +1039,13 @@ size_t filter_lds_bytes(const FilterArgs &a, bool) { return filter_lds(a.FW, a.F
 
 hipError_t launch_match_f32(const float *guide, int gpitch, const MatchArgs &a, hipStream_t s)
 {
-    return launch_match_t(guide, gpitch, a, s);
+    return a.exact ? launch_match_t<float, true>(guide, gpitch, a, s) : launch_match_t<float, false>(guide, gpitch, a, s);
 }
 
 hipError_t launch_match_f64(const double *guide, int gpitch, const MatchArgs &a, hipStream_t s)
 {
-    return launch_match_t(guide, gpitch, a, s);
+    return a.exact ? launch_match_t<double, true>(guide, gpitch, a, s)
+                   : launch_match_t<double, false>(guide, gpitch, a, s);
 }
 
 hipError_t launch_filter(const FilterArgs &a, bool wiener, hipStream_t s)
@@ -1045,6 +1095,12 @@ hipError_t launch_finalize(const FinalizeArgs &a, hipStream_t s)
 hipError_t launch_opp(const OppArgs &a, hipStream_t s)
 {
     hipLaunchKernelGGL(opp_kernel, dim3((a.W + 255) / 256, a.H), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_float(const ToFloatArgs &a, hipStream_t s)
+{
+    hipLaunchKernelGGL(to_float_kernel, dim3((a.W + 255) / 256, a.H), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

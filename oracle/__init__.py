@@ -75,6 +75,14 @@ def _lib(fast: bool = False) -> ctypes.CDLL:
     lib.orc_rgb2opp.argtypes = [dp] * 6 + [ctypes.c_long]
     lib.orc_opp2rgb.argtypes = [dp] * 6 + [ctypes.c_long, ctypes.c_int]
     lib.orc_dct3d.argtypes = [dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    ip = ctypes.POINTER(ctypes.c_int32)
+    ipp = ctypes.POINTER(ip)
+    lib.orc_range_to_float.argtypes = [dp, ip, ctypes.c_long, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.orc_range_to_int.argtypes = [ip, dp, ctypes.c_long, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.orc_bm3d_plane_int.argtypes = [sp, sp, ctypes.c_int, ip, ip, ip, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.orc_bm3d_rgb_int.argtypes = [sp, sp, ctypes.c_int, ipp, ipp, ipp, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_int]
     lib.orc_set_threads.argtypes = [ctypes.c_int]
     lib.orc_set_threads.restype = ctypes.c_int
     _libs[name] = lib
@@ -230,3 +238,58 @@ def dct3d(a: np.ndarray, inverse: bool = False) -> np.ndarray:
     n0, n1, n2 = out.shape
     _lib().orc_dct3d(_dptr(out), n0, n1, n2, int(inverse))
     return out
+
+
+def _iptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+def range_to_float(a, floor, neutral, ceil) -> np.ndarray:
+    """RangeConvert int -> float (include/Conversion.hpp:134-256)."""
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    out = np.empty(a.shape)
+    _lib().orc_range_to_float(_dptr(out), _iptr(a), a.size, floor, neutral, ceil)
+    return out
+
+
+def range_to_int(a, floor, neutral, ceil) -> np.ndarray:
+    """RangeConvert float -> int with clip (include/Conversion.hpp:241-256)."""
+    a = _f64(a)
+    out = np.empty(a.shape, np.int32)
+    _lib().orc_range_to_int(_iptr(out), _dptr(a), a.size, floor, neutral, ceil)
+    return out
+
+
+def bm3d_plane_int(basic, final, src, ref=None, mode: int = 3, floor: int = 0, neutral: int = 0,
+                   ceil: int = 255, fast: bool = False) -> np.ndarray:
+    """Integer Plane path (source/BM3D.cpp:248-279, 475-498)."""
+    b, f = step_params(basic), step_params(final)
+    src = np.ascontiguousarray(src, dtype=np.int32)
+    h, w = src.shape
+    r = np.ascontiguousarray(ref, dtype=np.int32) if ref is not None else src
+    dst = np.empty_like(src)
+    rc = _lib(fast).orc_bm3d_plane_int(ctypes.byref(b), ctypes.byref(f), mode, _iptr(dst), _iptr(src), _iptr(r),
+                                       w, h, floor, neutral, ceil)
+    if rc < 0:
+        raise ValueError(f"orc_bm3d_plane_int failed ({rc})")
+    return dst
+
+
+def bm3d_rgb_int(basic, final, src, ref=None, mode: int = 3, floor: int = 0, ceil: int = 255,
+                 fast: bool = False) -> np.ndarray:
+    """Integer Frame path (source/BM3D.cpp:282-313, 501-527).  src: int [3,H,W]."""
+    b, f = step_params(basic), step_params(final)
+    src = [np.ascontiguousarray(p, dtype=np.int32) for p in src]
+    h, w = src[0].shape
+    sp_ = (ctypes.POINTER(ctypes.c_int32) * 3)(*[_iptr(p) for p in src])
+    if ref is not None:
+        ref = [np.ascontiguousarray(p, dtype=np.int32) for p in ref]
+        rp = (ctypes.POINTER(ctypes.c_int32) * 3)(*[_iptr(p) for p in ref])
+    else:
+        rp = None
+    dst = [np.empty((h, w), np.int32) for _ in range(3)]
+    dp_ = (ctypes.POINTER(ctypes.c_int32) * 3)(*[_iptr(p) for p in dst])
+    rc = _lib(fast).orc_bm3d_rgb_int(ctypes.byref(b), ctypes.byref(f), mode, dp_, sp_, rp, w, h, floor, ceil)
+    if rc < 0:
+        raise ValueError(f"orc_bm3d_rgb_int failed ({rc})")
+    return np.stack(dst)

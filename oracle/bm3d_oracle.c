@@ -693,3 +693,153 @@ int orc_bm3d_rgb(const orc_step_params *basic, const orc_step_params *final_, in
     free(buf);
     return rc;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Integer planes and frames (Plane / Frame of include/Image_Type.h:51-403). */
+
+/* isChroma / isPCChroma on integer quantisation (include/Image_Type.hpp:144-167). */
+static int q_chroma(int floor_, int neutral) { return floor_ < neutral; }
+static int q_pc_chroma(int floor_, int neutral, int ceil_) { return floor_ < neutral && (floor_ + ceil_) % 2 == 1; }
+
+/* RangeConvert int -> float plane (include/Conversion.hpp:134-256, general
+ * branch): luma to [0,1] (dst Floor 0, Neutral 0, Ceil 1), chroma to
+ * [-0.5,0.5] (dst Neutral 0).  No clip, except that a PC-range chroma source
+ * is always clipped (:241-242). */
+void orc_range_to_float(double *dst, const int32_t *src, long n, int floor_, int neutral, int ceil_)
+{
+    int chroma = q_chroma(floor_, neutral);
+    double sRange = (double)(ceil_ - floor_);
+    double gain = 1.0 / sRange;
+    double offset = -(double)neutral * gain + 0.0;
+    int clip = q_pc_chroma(floor_, neutral, ceil_);
+    double lo = chroma ? -0.5 : 0.0, hi = chroma ? 0.5 : 1.0;
+    for (long k = 0; k < n; ++k) {
+        double v = (double)src[k] * gain + offset;
+        dst[k] = clip ? (v < lo ? lo : (v > hi ? hi : v)) : v;
+    }
+}
+
+/* RangeConvert float -> int plane with clip (include/Conversion.hpp:241-256):
+ * v * range + Neutral + 0.5 (0.499999 for PC-range chroma), clipped to
+ * [Floor, Ceil], truncated. */
+void orc_range_to_int(int32_t *dst, const double *src, long n, int floor_, int neutral, int ceil_)
+{
+    double dRange = (double)(ceil_ - floor_);
+    double gain = dRange / 1.0;
+    double offset = -0.0 * gain + (double)neutral + (q_pc_chroma(floor_, neutral, ceil_) ? 0.499999 : 0.5);
+    for (long k = 0; k < n; ++k) {
+        double v = src[k] * gain + offset;
+        v = v < floor_ ? floor_ : (v > ceil_ ? ceil_ : v);
+        dst[k] = (int32_t)v;
+    }
+}
+
+static int plane_wrapper_degenerate(const orc_step_params *p)
+{
+    /* BM3D_Base::process_Plane (source/BM3D.cpp:250-255) */
+    double normY, normU, normV;
+    opp_norms(&normY, &normU, &normV);
+    return !(p->sigma[0] > 0) || p->group_size == 1 || p->block_size <= 0 || p->bm_range <= 0 ||
+           p->bm_range < p->bm_step || p->thMSE * normY <= 0;
+}
+
+int orc_bm3d_plane_int(const orc_step_params *basic, const orc_step_params *final_, int mode,
+                       int32_t *dst, const int32_t *src, const int32_t *ref, int w, int h,
+                       int floor_, int neutral, int ceil_)
+{
+    long npx = (long)w * h;
+    const orc_step_params *single = mode == 1 ? basic : final_;
+    if (mode != 3 && plane_wrapper_degenerate(single)) {
+        memcpy(dst, src, sizeof(int32_t) * (size_t)npx);
+        return 1;
+    }
+    double *s = (double *)malloc(sizeof(double) * (size_t)npx * 3);
+    double *r = s + npx, *d = s + 2 * npx;
+    orc_range_to_float(s, src, npx, floor_, neutral, ceil_);
+    const double *rr = s;
+    if (ref && ref != src) {
+        orc_range_to_float(r, ref, npx, floor_, neutral, ceil_);
+        rr = r;
+    }
+    int rc;
+    if (mode == 3) {
+        rc = orc_bm3d_plane(basic, final_, d, s, rr, w, h);
+    } else {
+        const double *s1[1] = {s}, *r1[1] = {rr};
+        double *d1[1] = {d};
+        rc = orc_kernel(single, mode == 2, 1, d1, s1, r1, w, h);
+    }
+    if (rc >= 0) orc_range_to_int(dst, d, npx, floor_, neutral, ceil_);
+    free(s);
+    return rc;
+}
+
+/* Frame (int RGB, Floor/Ceil) two-step or single-step BM3D:
+ * MatrixConvert_RGB2YUV OPP from int (include/Conversion.hpp:547-578),
+ * filtering, MatrixConvert_YUV2RGB OPP to int with +0.5 and clip
+ * (include/Conversion.hpp:739-762). */
+int orc_bm3d_rgb_int(const orc_step_params *basic, const orc_step_params *final_, int mode,
+                     int32_t *const *dst, const int32_t *const *src, const int32_t *const *ref,
+                     int w, int h, int floor_, int ceil_)
+{
+    long npx = (long)w * h;
+    if (mode != 1 && mode != 2 && mode != 3) return -1;
+    const orc_step_params *single = mode == 1 ? basic : final_;
+    if (mode != 3 && wrapper_degenerate(single)) {
+        for (int c = 0; c < 3; ++c) memcpy(dst[c], src[c], sizeof(int32_t) * (size_t)npx);
+        return 1;
+    }
+    double sRange = (double)(ceil_ - floor_);
+    double gainY = 1.0 / (sRange * 3.0), offsetY = -(double)floor_ * 3.0 * gainY + 0.0;
+    double gainU = 1.0 / (sRange * 2.0), gainV = 1.0 / (sRange * 4.0), offsetC = 0.0;
+    double *buf = (double *)malloc(sizeof(double) * (size_t)npx * 9);
+    double *s[3] = {buf, buf + npx, buf + 2 * npx};
+    double *t[3] = {buf + 3 * npx, buf + 4 * npx, buf + 5 * npx};
+    double *r[3] = {buf + 6 * npx, buf + 7 * npx, buf + 8 * npx};
+    for (long k = 0; k < npx; ++k) {
+        double R = src[0][k], G = src[1][k], B = src[2][k];
+        s[0][k] = (R + G + B) * gainY + offsetY;
+        s[1][k] = (R - B) * gainU + offsetC;
+        s[2][k] = (R - G * 2.0 + B) * gainV + offsetC;
+    }
+    int have_ref = ref && ref[0] && (ref[0] != src[0] || ref[1] != src[1] || ref[2] != src[2]);
+    if (have_ref)
+        for (long k = 0; k < npx; ++k) {
+            double R = ref[0][k], G = ref[1][k], B = ref[2][k];
+            r[0][k] = (R + G + B) * gainY + offsetY;
+            r[1][k] = (R - B) * gainU + offsetC;
+            r[2][k] = (R - G * 2.0 + B) * gainV + offsetC;
+        }
+    const double *sc[3] = {s[0], s[1], s[2]};
+    int rc;
+    if (mode == 3) {
+        const double *rb[3] = {have_ref ? r[0] : s[0], s[1], s[2]};
+        rc = orc_kernel(basic, 0, 3, t, sc, rb, w, h);
+        if (rc >= 0) {
+            const double *tc[3] = {t[0], t[1], t[2]};
+            rc = orc_kernel(final_, 1, 3, s, sc, tc, w, h);
+        }
+    } else {
+        const double *rb[3] = {have_ref ? r[0] : s[0], have_ref ? r[1] : s[1], have_ref ? r[2] : s[2]};
+        rc = orc_kernel(single, mode == 2, 3, s, sc, rb, w, h);
+    }
+    if (rc >= 0) {
+        double dRange = sRange, gY = dRange / 1.0, gC = dRange / 1.0;
+        double Ry = 1.0 * gY, Ru = 1.0 * gC, Rv = (double)(2.0L / 3.0L) * gC;
+        double Gy = 1.0 * gY, Gv = (double)(-4.0L / 3.0L) * gC;
+        double By = 1.0 * gY, Bu = -1.0 * gC, Bv = (double)(2.0L / 3.0L) * gC;
+        double offR = -0.0 * Ry - 0.0 * (Ru + Rv) + (double)floor_ + 0.5;
+        double offG = -0.0 * Gy - 0.0 * (0.0 + Gv) + (double)floor_ + 0.5;
+        double offB = -0.0 * By - 0.0 * (Bu + Bv) + (double)floor_ + 0.5;
+        for (long k = 0; k < npx; ++k) {
+            double Y = s[0][k], U = s[1][k], V = s[2][k];
+            double v[3] = {Ry * Y + Ru * U + Rv * V + offR, Gy * Y + Gv * V + offG, By * Y + Bu * U + Bv * V + offB};
+            for (int c = 0; c < 3; ++c) {
+                double x = v[c] < floor_ ? floor_ : (v[c] > ceil_ ? ceil_ : v[c]);
+                dst[c][k] = (int32_t)x;
+            }
+        }
+    }
+    free(buf);
+    return rc;
+}

@@ -80,6 +80,19 @@ void orc_rgb2opp(double *y, double *u, double *v, const double *r, const double 
 void orc_opp2rgb(double *r, double *g, double *b, const double *y, const double *u,
                  const double *v, long n, int clip);
 
+/* Integer Plane / Frame paths (BM3D_Base::process_Plane, BM3D::process_Plane,
+ * source/BM3D.cpp:248-279, 475-498; process_Frame on an int Frame,
+ * :282-313, 501-527).  Quantisation Floor / Neutral / Ceil as in Plane
+ * (include/Image_Type.hpp:85-117; chroma iff Floor < Neutral); mode as orc_bm3d_rgb. */
+void orc_range_to_float(double *dst, const int32_t *src, long n, int floor_, int neutral, int ceil_);
+void orc_range_to_int(int32_t *dst, const double *src, long n, int floor_, int neutral, int ceil_);
+int orc_bm3d_plane_int(const orc_step_params *basic, const orc_step_params *final_, int mode,
+                       int32_t *dst, const int32_t *src, const int32_t *ref, int w, int h,
+                       int floor_, int neutral, int ceil_);
+int orc_bm3d_rgb_int(const orc_step_params *basic, const orc_step_params *final_, int mode,
+                     int32_t *const *dst, const int32_t *const *src, const int32_t *const *ref,
+                     int w, int h, int floor_, int ceil_);
+
 /* Unnormalised 3D DCT on an n0 x n1 x n2 array (n2 fastest): FFTW REDFT10
  * forward (inverse = 0) or REDFT01 backward (inverse = 1) on every axis. */
 void orc_dct3d(double *data, int n0, int n1, int n2, int inverse);

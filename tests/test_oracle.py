@@ -217,3 +217,95 @@ def test_threads_agree():
     b = oracle.bm3d_rgb(p.basic, p.final, n)
     oracle.set_threads(1)
     np.testing.assert_allclose(a, b, atol=1e-12)
+
+
+# --- integer quantisation (RangeConvert, include/Conversion.hpp:134-256)
+QUANT = [(0, 0, 255), (16, 16, 235), (0, 0, 1023), (0, 128, 255), (16, 128, 240), (0, 32768, 65535)]
+
+
+@pytest.mark.parametrize("q", QUANT, ids=[str(q) for q in QUANT])
+def test_range_convert_closed_forms(q):
+    floor, neutral, ceil = q
+    lv = np.arange(floor, ceil + 1, dtype=np.int32)
+    f = oracle.range_to_float(lv, *q)
+    chroma = floor < neutral
+    pc_chroma = chroma and (floor + ceil) % 2 == 1
+    zero = neutral if chroma else floor
+    # the zero level maps to 0, one level is one 1/(Ceil-Floor) step
+    assert f[zero - floor] == 0.0
+    np.testing.assert_allclose(np.diff(f[1:-1] if pc_chroma else f), 1.0 / (ceil - floor), rtol=0, atol=1e-15)
+    if pc_chroma:
+        # an odd PC range has no centre level: Floor lies below -0.5 and is clipped
+        assert f[0] == -0.5 and f[-1] < 0.5
+    else:
+        lo = -0.5 if chroma else 0.0
+        np.testing.assert_allclose([f[0], f[-1]], [lo, lo + 1.0], atol=1e-15)
+    # every level survives the round trip, the clipped PC-chroma Floor included
+    # (the 0.499999 offset: with 0.5 it would come back as Floor + 1)
+    np.testing.assert_array_equal(oracle.range_to_int(f, *q), lv)
+
+
+def test_range_to_int_rounding_and_clip():
+    q = (16, 16, 235)
+    step = 1.0 / 219
+    k = np.arange(10)
+    # halfway values round up, just below halfway rounds down
+    np.testing.assert_array_equal(oracle.range_to_int((k + 0.5) * step, *q), 16 + k + 1)
+    np.testing.assert_array_equal(oracle.range_to_int((k + 0.49) * step, *q), 16 + k)
+    np.testing.assert_array_equal(oracle.range_to_int(np.array([-1.0, -1e-9, 1.0 + 1e-9, 7.0]), *q),
+                                  [16, 16, 235, 235])
+    q = (0, 128, 255)
+    np.testing.assert_array_equal(oracle.range_to_int(np.array([-0.9, 0.0, 0.9]), *q), [0, 128, 255])
+
+
+def _quant(x, floor, ceil):
+    return np.clip(np.floor(x * (ceil - floor) + floor + 0.5), floor, ceil).astype(np.int32)
+
+
+@pytest.mark.parametrize("mode", [1, 3])
+def test_plane_int_agrees_with_float_path(mode):
+    # same data as a float plane: at most one level apart (the conversions round
+    # v*(1/range) where the float input was v/range), on very few pixels
+    clean, n = synth.grey_pair(48, 56, 25.0, 21)
+    src = _quant(np.clip(n, 0, 1), 0, 255)
+    p = para()
+    got = oracle.bm3d_plane_int(p.basic, p.final, src, mode=mode)
+    x = src.astype(np.float64) / 255
+    ref = oracle.kernel(p.basic, x, x, wiener=False) if mode == 1 else oracle.bm3d_plane(p.basic, p.final, x)
+    want = _quant(ref, 0, 255)
+    d = np.abs(got.astype(np.int64) - want)
+    assert d.max() <= 1 and (d > 0).mean() < 0.01
+    assert synth.psnr(got / 255.0, clean) > synth.psnr(src / 255.0, clean) + 1
+
+
+def test_plane_int_chroma_shift():
+    # a chroma plane is the luma plane moved by Neutral: the filter sees the
+    # same data minus a constant, so outputs agree up to that shift except where
+    # a group's DC coefficient falls under the hard threshold
+    _, n = synth.grey_pair(40, 48, 10.0, 5)
+    src = _quant(0.5 + 0.2 * (np.clip(n, 0, 1) - 0.5), 0, 255)
+    p = para(sigma=10.0)
+    lum = oracle.bm3d_plane_int(p.basic, p.final, src, mode=3, floor=16, neutral=16, ceil=240)
+    chr_ = oracle.bm3d_plane_int(p.basic, p.final, src, mode=3, floor=16, neutral=128, ceil=240)
+    d = np.abs(lum.astype(np.int64) - chr_)
+    assert np.median(d) == 0 and (d > 1).mean() < 0.05
+
+
+def test_rgb_int_agrees_with_float_path():
+    clean, n = synth.rgb_pair(40, 48, 25.0, 22)
+    src = _quant(np.clip(n, 0, 1), 0, 255)
+    p = para()
+    got = oracle.bm3d_rgb_int(p.basic, p.final, src, mode=3)
+    want = _quant(oracle.bm3d_rgb(p.basic, p.final, src.astype(np.float64) / 255, mode=3), 0, 255)
+    d = np.abs(got.astype(np.int64) - want)
+    assert d.max() <= 1 and (d > 0).mean() < 0.01
+
+
+def test_int_degenerate_copy_through():
+    src = _quant(synth.grey_pair(32, 32, 25.0, 2)[1].clip(0, 1), 16, 235)
+    p = para(sigma=0.0)
+    for mode in (1, 2):
+        np.testing.assert_array_equal(oracle.bm3d_plane_int(p.basic, p.final, src, src, mode=mode,
+                                                            floor=16, neutral=16, ceil=235), src)
+    rgb = np.stack([src, src[::-1], src[:, ::-1]])
+    np.testing.assert_array_equal(oracle.bm3d_rgb_int(p.basic, p.final, rgb, mode=1), rgb)
