@@ -88,15 +88,67 @@ def cpu_baseline(para, clean_rgb_noisy, threads: int, target_s: float):
     }
 
 
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                            "bm3d_1080p_rgb_{prec}_hbm_traffic.txt")
+
+
+def measured_traffic(prec: str, kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE runs, each scaled by a copy kernel's reading).
+    Lines of the form `traffic <stage> <fetch_bytes> <write_bytes>`."""
+    path = TRAFFIC_FILE.format(prec=prec)
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            f = line.split()
+            if len(f) == 4 and f[0] == "traffic" and f[1] == kernel:
+                return {"bytes": float(f[2]) + float(f[3]), "fetch_bytes": float(f[2]),
+                        "write_bytes": float(f[3]), "source": os.path.relpath(path, os.path.dirname(TRAFFIC_FILE))}
+    return None
+
+
+def timed_steps(f, y, x, steps: int, warmup: int):
+    import torch
+    for _ in range(warmup):
+        f.process(y, x)
+    torch.cuda.synchronize()
+    f.set_timing(True)
+    f.timing(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        f.process(y, x)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    stages = f.timing()
+    f.set_timing(False)
+    return dt, stages
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", default="fp32", choices=("fp32", "fp16", "bf16"))
+    ap.add_argument("--precision", default="fp16", choices=("fp32", "fp16", "bf16"),
+                    help="transform/shrinkage precision of the timed form (fp16: the metric's mixed form)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=4.0, help="target seconds per CPU baseline run")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 256 MiB device-to-device copy (3 launches): the known-traffic "
+                         "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        a = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda:0").fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_each_way": a.numel() * 4, "launches": 3}), flush=True)
+        return 0
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(images shard across GPUs with no collectives, see the design notes)", file=sys.stderr)
@@ -112,31 +164,34 @@ def main() -> int:
     y = torch.empty_like(x)
     f = BM3D(para, device=0, precision=args.precision)
 
-    for _ in range(args.warmup):
-        f.process(y, x)
-    torch.cuda.synchronize()
-    f.set_timing(True)
-    f.timing(reset=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        f.process(y, x)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    stages = f.timing()
-    f.set_timing(False)
-    dt = (t1 - t0) / args.steps
+    dt, stages = timed_steps(f, y, x, args.steps, args.warmup)
     value = W * H / dt / 1e6
     out = y.cpu().numpy()
     psnr_out = synth.psnr(out, clean)
     psnr_in = synth.psnr(noisy, clean)
     quality_extra = {}
+    fp32_form = None
     if args.precision != "fp32":
-        # the metric's bar: PSNR loss against the plain FP32 GPU path
-        ref32 = BM3D(para, device=0, precision="fp32")(x).cpu().numpy()
+        # the metric's bar: PSNR loss against the plain FP32 GPU path, timed beside it
+        f32 = BM3D(para, device=0, precision="fp32")
+        y32 = torch.empty_like(x)
+        dt32, st32 = timed_steps(f32, y32, x, max(2, args.steps // 2), 1)
+        ref32 = y32.cpu().numpy()
         quality_extra = {"psnr_fp32_db": round(synth.psnr(ref32, clean), 4),
                          "psnr_loss_vs_fp32_db": round(synth.psnr(ref32, clean) - psnr_out, 4),
                          "max_abs_vs_fp32": float(np.abs(out - ref32).max())}
+        fp32_form = {"value": round(W * H / dt32 / 1e6, 3), "ms_per_step": round(dt32 * 1e3, 3),
+                     "stages_ms": {k: round(v[0] / v[1], 4) for k, v in st32.items() if v[1]}}
+    # end-to-end rate with host buffers (PCIe both ways); never the headline value
+    hin = noisy
+    hout = np.empty_like(hin)
+    f.process(hout, hin)
+    host_dev = [float(np.abs(hout - out).max())]
+    t0 = time.perf_counter()
+    for _ in range(2):
+        f.process(hout, hin)
+    host_rate = W * H / ((time.perf_counter() - t0) / 2) / 1e6
+    host_dev.append(float(np.abs(hout - out).max()))
 
     # group sizes of the last run (actual-N work) and full-N work
     gshape = f.grid_shape(W, H, False)
@@ -160,6 +215,7 @@ def main() -> int:
         peak = PEAK_FP32_TFLOPS if args.precision == "fp32" else PEAK_F16_TFLOPS
         roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
                 "frac": round(achieved / peak, 4), "traffic": None, "kernel": dom,
+                "ms_per_launch": round(ms, 4),
                 "engine": "fp32 VALU" if args.precision == "fp32" else f"{args.precision} MFMA 32x32x16",
                 "algorithmic_flops_per_launch": flops,
                 "flops_basis": "dense separable 2*B^2*N*(2B+N) per transform at each group's actual N"}
@@ -172,8 +228,15 @@ def main() -> int:
         achieved = match_flops / (ms * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None, "kernel": dom,
+                "ms_per_launch": round(ms, 4),
                 "engine": "fp64 VALU", "algorithmic_flops_per_launch": match_flops,
                 "flops_basis": "3 flop (sub+FMA) per pixel per candidate, clipped windows"}
+
+    tr = measured_traffic(args.precision, dom)
+    if tr is not None:
+        roof["traffic"] = tr["bytes"]
+        roof["traffic_detail"] = tr
+        roof["traffic_gbs_at_measured_time"] = round(tr["bytes"] / (ms * 1e-3) / 1e9, 1)
 
     res = {
         "metric": "Mpix/s two-step BM3D 1080p RGB sigma=25 at <=0.05 dB PSNR vs plain FP32 GPU",
@@ -196,6 +259,9 @@ def main() -> int:
                     "mean_group_size": [round(float(n1.mean()), 3), round(float(n2.mean()), 3)],
                     **quality_extra},
         "stages": per_stage,
+        "fp32_form": fp32_form,
+        "host_buffers_mpix_s": round(host_rate, 3),
+        "host_buffers_max_abs_vs_device": host_dev,
         "work": {"refs_per_step": refs, "match_candidates_per_step": cand, "match_flops_per_step": match_flops,
                  "filter_final_flops_actual_n": filt2_actual, "filter_final_flops_full_n": filt2_full,
                  "filter_basic_flops_actual_n": filt1_actual},

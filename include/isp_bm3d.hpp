@@ -9,12 +9,16 @@
 //     isp_bm3d::Plane_FL dst = filter(src);       // FilterIF2::operator(), include/Filter.h:211-216
 //
 // Plane_FL holds float samples (luma range [0,1]) with stride == width, like
-// the reference's Plane_FL (include/Image_Type.h:186-290).  Frame_FL holds
-// three float RGB planes in [0,1]; it stands in for the reference's integer
-// Frame (include/Image_Type.h:294-403) with the same process_Frame semantics.
+// the reference's Plane_FL (include/Image_Type.h:186-290).  Plane holds int32
+// samples with BitDepth and Floor / Neutral / Ceil (include/Image_Type.h:51-183)
+// and Frame three such RGB planes (include/Image_Type.h:294-403); BM3D on them
+// converts with RangeConvert / the OPP matrices and rounds back exactly as the
+// reference's process_Plane / process_Frame do.  Frame_FL holds three float RGB
+// planes in [0,1] with the same process_Frame semantics.
 // Errors throw isp_bm3d::Error (the reference prints and exits instead).
 #pragma once
 
+#include <cstdint>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -118,6 +122,87 @@ private:
     std::vector<float> data_;
 };
 
+// QuantRange and the quantisation of a bit depth (include/Image_Type.h:39-44,
+// include/Image_Type.hpp:94-109).
+enum class QuantRange { TV, PC };
+
+inline void Quantize_Value(int &Floor, int &Neutral, int &Ceil, int BitDepth, QuantRange qr, bool Chroma)
+{
+    if (BitDepth < 8 || BitDepth > 30) throw Error(ISP_BM3D_E_ARG, "bit depth must be 8..30");
+    if (qr == QuantRange::PC) {
+        Floor = 0;
+        Ceil = (1 << BitDepth) - 1;
+    } else {
+        Floor = 16 << (BitDepth - 8);
+        Ceil = (Chroma ? 240 : 235) << (BitDepth - 8);
+    }
+    Neutral = Chroma ? 1 << (BitDepth - 1) : Floor;
+}
+
+class Plane {
+public:
+    Plane() = default;
+    // Plane(Value, Width, Height, BitDepth): full-range luma quantisation
+    explicit Plane(int Value, int width = 1920, int height = 1080, int BitDepth = 16)
+        : w_(width), h_(height), bd_(BitDepth), data_((size_t)width * height, Value)
+    {
+        Quantize_Value(floor_, neutral_, ceil_, BitDepth, QuantRange::PC, false);
+    }
+    Plane(int Value, int width, int height, int BitDepth, int Floor, int Neutral, int Ceil)
+        : w_(width), h_(height), bd_(BitDepth), floor_(Floor), neutral_(Neutral), ceil_(Ceil),
+          data_((size_t)width * height, Value) {}
+    int Width() const { return w_; }
+    int Height() const { return h_; }
+    int Stride() const { return w_; }
+    int BitDepth() const { return bd_; }
+    int Floor() const { return floor_; }
+    int Neutral() const { return neutral_; }
+    int Ceil() const { return ceil_; }
+    int ValueRange() const { return ceil_ - floor_; }
+    bool isChroma() const { return floor_ < neutral_; }
+    bool isPCChroma() const { return floor_ < neutral_ && (floor_ + ceil_) % 2 == 1; }
+    int32_t *data() { return data_.data(); }
+    const int32_t *data() const { return data_.data(); }
+    int32_t &operator()(int y, int x) { return data_[(size_t)y * w_ + x]; }
+    int32_t operator()(int y, int x) const { return data_[(size_t)y * w_ + x]; }
+    // a plane of the same size and quantisation (what the reference's operator() allocates)
+    Plane like() const { return Plane(0, w_, h_, bd_, floor_, neutral_, ceil_); }
+
+private:
+    int w_ = 0, h_ = 0, bd_ = 16, floor_ = 0, neutral_ = 0, ceil_ = 65535;
+    std::vector<int32_t> data_;
+};
+
+// RGB frame of int32 planes (R, G, B are luma-type: Floor == Neutral).
+class Frame {
+public:
+    Frame() = default;
+    Frame(int width, int height, int BitDepth = 16, QuantRange qr = QuantRange::PC)
+    {
+        int fl, ne, ce;
+        Quantize_Value(fl, ne, ce, BitDepth, qr, false);
+        for (auto &p : p_) p = Plane(fl, width, height, BitDepth, fl, ne, ce);
+    }
+    Plane &R() { return p_[0]; }
+    Plane &G() { return p_[1]; }
+    Plane &B() { return p_[2]; }
+    const Plane &R() const { return p_[0]; }
+    const Plane &G() const { return p_[1]; }
+    const Plane &B() const { return p_[2]; }
+    int Width() const { return p_[0].Width(); }
+    int Height() const { return p_[0].Height(); }
+    int BitDepth() const { return p_[0].BitDepth(); }
+    Frame like() const
+    {
+        Frame f;
+        for (int c = 0; c < 3; ++c) f.p_[c] = p_[c].like();
+        return f;
+    }
+
+private:
+    Plane p_[3];
+};
+
 class Frame_FL {
 public:
     Frame_FL() = default;
@@ -164,17 +249,43 @@ public:
     }
     Frame_FL &process(Frame_FL &dst, const Frame_FL &src) { return process(dst, src, src); }
 
+    // integer Plane: RangeConvert in and out (source/BM3D.cpp:248-279)
+    Plane &process(Plane &dst, const Plane &src, const Plane &ref)
+    {
+        size_check(dst, src);
+        size_check(ref, src);
+        check(isp_bm3d_plane_i32(ctx_, dst.data(), src.data(), ref.data(), src.Width(), src.Height(), src.Stride(),
+                                 src.Floor(), src.Neutral(), src.Ceil(), mode_, ISP_BM3D_MEM_HOST, nullptr),
+              ctx_);
+        return dst;
+    }
+    Plane &process(Plane &dst, const Plane &src) { return process(dst, src, src); }
+    // integer RGB Frame: OPP in, OPP out with +0.5 and clip (source/BM3D.cpp:282-313)
+    Frame &process(Frame &dst, const Frame &src, const Frame &ref)
+    {
+        size_check(dst.R(), src.R());
+        size_check(ref.R(), src.R());
+        const Plane &q = src.R();
+        if (q.Floor() != q.Neutral()) throw Error(ISP_BM3D_E_ARG, "an RGB frame has luma-type planes");
+        check(isp_bm3d_rgb_i32(ctx_, dst.R().data(), dst.G().data(), dst.B().data(), src.R().data(), src.G().data(),
+                               src.B().data(), ref.R().data(), ref.G().data(), ref.B().data(), src.Width(),
+                               src.Height(), q.Stride(), q.Floor(), q.Ceil(), mode_, ISP_BM3D_MEM_HOST, nullptr),
+              ctx_);
+        return dst;
+    }
+    Frame &process(Frame &dst, const Frame &src) { return process(dst, src, src); }
+
     template <typename T>
     T operator()(const T &src)
     {
-        T dst(src.Width(), src.Height());
+        T dst = make_like(src);
         process(dst, src);
         return dst;
     }
     template <typename T>
     T operator()(const T &src, const T &ref)
     {
-        T dst(src.Width(), src.Height());
+        T dst = make_like(src);
         process(dst, src, ref);
         return dst;
     }
@@ -185,11 +296,16 @@ protected:
     {
         check(isp_bm3d_create(&ctx_, &p, device));
     }
-    static void size_check(const Plane_FL &a, const Plane_FL &b)
+    template <typename P>
+    static void size_check(const P &a, const P &b)
     {
         if (a.Width() != b.Width() || a.Height() != b.Height())
             throw Error(ISP_BM3D_E_ARG, "planes of different sizes");
     }
+    static Plane_FL make_like(const Plane_FL &p) { return Plane_FL(p.Width(), p.Height()); }
+    static Frame_FL make_like(const Frame_FL &f) { return Frame_FL(f.Width(), f.Height()); }
+    static Plane make_like(const Plane &p) { return p.like(); }
+    static Frame make_like(const Frame &f) { return f.like(); }
     isp_bm3d_ctx *ctx_ = nullptr;
     int mode_;
 };

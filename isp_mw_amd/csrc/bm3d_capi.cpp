@@ -29,6 +29,8 @@ struct isp_bm3d_ctx {
     uint16_t *d_mfma[2] = {nullptr, nullptr};  // MFMA operand tables, [0] FP16, [1] BF16
     void *ws = nullptr;
     size_t ws_bytes = 0;
+    char *pin = nullptr;  // page-locked staging for host-memory calls
+    size_t pin_bytes = 0;
     std::string err;
     // groups of the last run of each step (device, inside ws)
     int *last_cnt[2] = {nullptr, nullptr};
@@ -230,6 +232,59 @@ int ensure_ws(isp_bm3d_ctx *ctx, size_t bytes)
     HIP_CHECK(ctx, hipMalloc(&ctx->ws, bytes));
     ctx->ws_bytes = bytes;
     return ISP_BM3D_OK;
+}
+
+// Host-memory calls stage each plane through page-locked memory: the CPU packs
+// rows at the device pitch, one DMA per plane runs on the caller's stream, and
+// the call synchronises that stream before it returns (so no DMA from an
+// earlier call is in flight when the next one writes the staging area).
+struct HostStage {
+    isp_bm3d_ctx *ctx;
+    hipStream_t s;
+    int P, W, H;
+    size_t plane_bytes() const { return (size_t)P * H * 4; }
+    char *slot(int k) const { return ctx->pin + (size_t)k * plane_bytes(); }
+
+    static int reserve(isp_bm3d_ctx *ctx, size_t bytes)
+    {
+        if (bytes <= ctx->pin_bytes) return ISP_BM3D_OK;
+        if (ctx->pin) {
+            HIP_CHECK(ctx, hipHostFree(ctx->pin));
+            ctx->pin = nullptr;
+            ctx->pin_bytes = 0;
+        }
+        HIP_CHECK(ctx, hipHostMalloc((void **)&ctx->pin, bytes, hipHostMallocDefault));
+        ctx->pin_bytes = bytes;
+        return ISP_BM3D_OK;
+    }
+    // host plane (row stride `stride` elements of 4 bytes) -> device plane of pitch P
+    int upload(int k, void *dev, const void *host, size_t stride) const
+    {
+        char *p = slot(k);
+        for (int y = 0; y < H; ++y)
+            std::memcpy(p + (size_t)y * P * 4, (const char *)host + (size_t)y * stride * 4, (size_t)W * 4);
+        HIP_CHECK(ctx, hipMemcpyAsync(dev, p, plane_bytes(), hipMemcpyHostToDevice, s));
+        return ISP_BM3D_OK;
+    }
+    int download_start(int k, const void *dev) const
+    {
+        HIP_CHECK(ctx, hipMemcpyAsync(slot(k), dev, plane_bytes(), hipMemcpyDeviceToHost, s));
+        return ISP_BM3D_OK;
+    }
+    // after hipStreamSynchronize(s)
+    void download_finish(int k, void *host, size_t stride) const
+    {
+        const char *p = slot(k);
+        for (int y = 0; y < H; ++y)
+            std::memcpy((char *)host + (size_t)y * stride * 4, p + (size_t)y * P * 4, (size_t)W * 4);
+    }
+};
+
+void host_copy_plane(void *dst, const void *src, size_t stride, int W, int H)
+{
+    if (dst == src) return;
+    for (int y = 0; y < H; ++y)
+        std::memcpy((char *)dst + (size_t)y * stride * 4, (const char *)src + (size_t)y * stride * 4, (size_t)W * 4);
 }
 
 // Per-call device buffers.
@@ -719,6 +774,7 @@ int isp_bm3d_destroy(isp_bm3d_ctx *ctx)
     }
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     if (ctx->ws) (void)hipFree(ctx->ws);
+    if (ctx->pin) (void)hipHostFree(ctx->pin);
     if (ctx->d_tables) (void)hipFree(ctx->d_tables);
     if (ctx->d_tables64) (void)hipFree(ctx->d_tables64);
     for (int k = 0; k < 2; ++k)
@@ -823,7 +879,6 @@ int plane_common(isp_bm3d_ctx *ctx, void *dst, const void *src, const void *ref,
     Work w{};
     if ((rc = prepare(ctx, w, P, H, W, false))) return rc;
     const bool sep_ref = ref && ref != src;
-    const size_t bw = (size_t)W * 4;
     // BM3D_Base::process_Plane's early return for the single steps on int planes
     if (fmt.is_int && mode != ISP_BM3D_BOTH) {
         const isp_bm3d_step_params &p = mode == ISP_BM3D_BASIC ? ctx->params.basic : ctx->params.final_;
@@ -832,11 +887,7 @@ int plane_common(isp_bm3d_ctx *ctx, void *dst, const void *src, const void *ref,
         if (!(p.sigma[0] > 0) || p.group_size == 1 || p.block_size <= 0 || p.bm_range <= 0 ||
             p.bm_range < p.bm_step || p.thMSE * norm[0] <= 0) {
             if (host) {
-                HIP_CHECK(ctx, hipMemcpy2DAsync(w.out[0], (size_t)P * 4, src, stride * 4, bw, H,
-                                                hipMemcpyHostToDevice, s));
-                HIP_CHECK(ctx, hipMemcpy2DAsync(dst, stride * 4, w.out[0], (size_t)P * 4, bw, H,
-                                                hipMemcpyDeviceToHost, s));
-                HIP_CHECK(ctx, hipStreamSynchronize(s));
+                host_copy_plane(dst, src, stride, W, H);
                 return ISP_BM3D_OK;
             }
             return dst == src ? ISP_BM3D_OK : copy_plane(ctx, dst, stride, src, stride, W, H, s);
@@ -845,14 +896,16 @@ int plane_common(isp_bm3d_ctx *ctx, void *dst, const void *src, const void *ref,
     // source and guide as float planes of pitch P
     const void *rsrc = src, *rref = sep_ref ? ref : src;
     size_t rpitch = stride;
+    const HostStage hs{ctx, s, P, W, H};
     if (host) {
+        if ((rc = HostStage::reserve(ctx, 2 * hs.plane_bytes()))) return rc;
         float *stage_s = fmt.is_int ? w.out[1] : w.plane[0];
         float *stage_r = fmt.is_int ? w.out[2] : w.refp[0];
-        HIP_CHECK(ctx, hipMemcpy2DAsync(stage_s, (size_t)P * 4, src, stride * 4, bw, H, hipMemcpyHostToDevice, s));
+        if ((rc = hs.upload(0, stage_s, src, stride))) return rc;
         rsrc = stage_s;
         rref = stage_s;
         if (sep_ref) {
-            HIP_CHECK(ctx, hipMemcpy2DAsync(stage_r, (size_t)P * 4, ref, stride * 4, bw, H, hipMemcpyHostToDevice, s));
+            if ((rc = hs.upload(1, stage_r, ref, stride))) return rc;
             rref = stage_r;
         }
         rpitch = (size_t)P;
@@ -923,10 +976,21 @@ int plane_common(isp_bm3d_ctx *ctx, void *dst, const void *src, const void *ref,
     }
     if (rc) return rc;
     if (host) {
-        HIP_CHECK(ctx, hipMemcpy2DAsync(dst, stride * 4, out, (size_t)P * 4, bw, H, hipMemcpyDeviceToHost, s));
+        if ((rc = hs.download_start(0, out))) return rc;
         HIP_CHECK(ctx, hipStreamSynchronize(s));
+        hs.download_finish(0, dst, stride);
     }
     return ISP_BM3D_OK;
+}
+
+// BM3D_Base::process_Frame's wrapper test for a single step (source/BM3D.cpp:284-290)
+bool frame_degenerate(const isp_bm3d_ctx *ctx, int mode)
+{
+    const isp_bm3d_step_params &p = mode == ISP_BM3D_BASIC ? ctx->params.basic : ctx->params.final_;
+    double norm[3];
+    opp_norms(norm);
+    return (!(p.sigma[0] > 0) && !(p.sigma[1] > 0) && !(p.sigma[2] > 0)) || p.group_size == 1 ||
+           p.block_size <= 0 || p.bm_range <= 0 || p.bm_range < p.bm_step || p.thMSE * norm[0] <= 0;
 }
 
 // RGB frame: process_Frame (source/BM3D.cpp:282-313, 501-527) on float RGB in
@@ -947,43 +1011,34 @@ int rgb_common(isp_bm3d_ctx *ctx, void *const *dst, const void *const *src, cons
     const int P = internal_pitch(W, stride, mem);
     Work w{};
     if ((rc = prepare(ctx, w, P, H, W, false))) return rc;
-    const size_t bw = (size_t)W * 4;
 
     const void *rgb[3] = {src[0], src[1], src[2]};
     const void *rrgb[3] = {ref[0], ref[1], ref[2]};
     size_t in_pitch = stride;
+    const HostStage hs{ctx, s, P, W, H};
     if (host) {
+        // BM3D_Base::process_Frame's own early return needs no device work
+        if (mode != ISP_BM3D_BOTH && frame_degenerate(ctx, mode)) {
+            for (int c = 0; c < 3; ++c) host_copy_plane(dst[c], src[c], stride, W, H);
+            return ISP_BM3D_OK;
+        }
         // stage RGB in the out/basic planes, which are free until later
+        if ((rc = HostStage::reserve(ctx, (sep_ref ? 6 : 3) * hs.plane_bytes()))) return rc;
         for (int c = 0; c < 3; ++c) {
-            HIP_CHECK(ctx, hipMemcpy2DAsync(w.out[c], (size_t)P * 4, rgb[c], stride * 4, bw, H,
-                                            hipMemcpyHostToDevice, s));
+            if ((rc = hs.upload(c, w.out[c], rgb[c], stride))) return rc;
             rgb[c] = w.out[c];
             if (sep_ref) {
-                HIP_CHECK(ctx, hipMemcpy2DAsync(w.basic[c], (size_t)P * 4, rrgb[c], stride * 4, bw, H,
-                                                hipMemcpyHostToDevice, s));
+                if ((rc = hs.upload(3 + c, w.basic[c], rrgb[c], stride))) return rc;
                 rrgb[c] = w.basic[c];
             }
         }
         in_pitch = (size_t)P;
     }
     // BM3D_Base::process_Frame's own early return (source/BM3D.cpp:284-290)
-    if (mode != ISP_BM3D_BOTH) {
-        const isp_bm3d_step_params &p = mode == ISP_BM3D_BASIC ? ctx->params.basic : ctx->params.final_;
-        double norm[3];
-        opp_norms(norm);
-        if ((!(p.sigma[0] > 0) && !(p.sigma[1] > 0) && !(p.sigma[2] > 0)) || p.group_size == 1 ||
-            p.block_size <= 0 || p.bm_range <= 0 || p.bm_range < p.bm_step || p.thMSE * norm[0] <= 0) {
-            for (int c = 0; c < 3; ++c) {
-                if (host) {
-                    HIP_CHECK(ctx, hipMemcpy2DAsync(dst[c], stride * 4, rgb[c], in_pitch * 4, bw, H,
-                                                    hipMemcpyDeviceToHost, s));
-                } else if (dst[c] != rgb[c]) {
-                    if ((rc = copy_plane(ctx, dst[c], stride, rgb[c], in_pitch, W, H, s))) return rc;
-                }
-            }
-            if (host) HIP_CHECK(ctx, hipStreamSynchronize(s));
-            return ISP_BM3D_OK;
-        }
+    if (mode != ISP_BM3D_BOTH && frame_degenerate(ctx, mode)) {
+        for (int c = 0; c < 3; ++c)
+            if (dst[c] != rgb[c] && (rc = copy_plane(ctx, dst[c], stride, rgb[c], in_pitch, W, H, s))) return rc;
+        return ISP_BM3D_OK;
     }
     // RGB -> OPP (float planes + FP64 luma guide)
     OppArgs o{};
@@ -1085,9 +1140,9 @@ int rgb_common(isp_bm3d_ctx *ctx, void *const *dst, const void *const *src, cons
     if (rc) return rc;
     if (host) {
         for (int c = 0; c < 3; ++c)
-            HIP_CHECK(ctx, hipMemcpy2DAsync(dst[c], stride * 4, out[c], (size_t)P * 4, bw, H,
-                                            hipMemcpyDeviceToHost, s));
+            if ((rc = hs.download_start(c, out[c]))) return rc;
         HIP_CHECK(ctx, hipStreamSynchronize(s));
+        for (int c = 0; c < 3; ++c) hs.download_finish(c, dst[c], stride);
     }
     return ISP_BM3D_OK;
 }
@@ -1146,8 +1201,9 @@ int isp_bm3d_match(isp_bm3d_ctx *ctx, const float *guide, int W, int H, size_t s
     hipStream_t s = nullptr;
     const float *g = guide;
     if (mem == ISP_BM3D_MEM_HOST) {
-        HIP_CHECK(ctx, hipMemcpy2DAsync(w.plane[0], (size_t)P * 4, guide, stride * 4, (size_t)W * 4, H,
-                                        hipMemcpyHostToDevice, s));
+        const HostStage hs{ctx, s, P, W, H};
+        if ((rc = HostStage::reserve(ctx, hs.plane_bytes()))) return rc;
+        if ((rc = hs.upload(0, w.plane[0], guide, stride))) return rc;
         g = w.plane[0];
     }
     rc = run_match(ctx, p, d, w, g, nullptr, P, W, H, w.keys, s, final_step ? ST_MATCH_FINAL : ST_MATCH_BASIC);

@@ -6,8 +6,13 @@
 //                      and top-(G-1) selection by a wave-level radix select.
 //   filter_kernel    - gather, separable orthonormal 3D DCT, hard threshold or
 //                      Wiener shrinkage, inverse, weighted aggregation into an
-//                      LDS tile, flushed to HBM with float atomics
-//                      (source/BM3D.cpp:339-385, 411-457; include/Block.h:1240-1357).
+//                      LDS tile of 64-bit fixed-point sums, flushed to HBM with
+//                      integer atomics (source/BM3D.cpp:339-385, 411-457;
+//                      include/Block.h:1240-1357).  FP32 on the vector ALUs.
+//   filter_mfma_kernel - the same stage with the transforms as chained FP16 or
+//                      BF16 matrix-core products (FP32 accumulation).
+//   to_float_kernel  - RangeConvert of integer samples to float
+//                      (include/Conversion.hpp:134-256).
 //   finalize_kernel  - num/den (source/BM3D.cpp:132-135, 204-217), optionally
 //                      fused with OPP->RGB and clip (include/Conversion.hpp:739-762).
 //   opp_kernel       - RGB->OPP without clip (include/Conversion.hpp:547-578).

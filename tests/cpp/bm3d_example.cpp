@@ -38,6 +38,24 @@ int main()
         Frame_FL g = filter(f);
         std::printf("plane mean %.6f mean abs err %.6f frame R(0,0) %.6f\n", sum / (W * H), err / (W * H),
                     g.R()(0, 0));
+        // integer frame, 8-bit full range: the same denoising through RangeConvert / OPP rounding
+        Frame fi(W, H, 8, QuantRange::PC);
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                fi.R()(y, x) = (int)std::lround(255.0 * std::fmin(1.0, std::fmax(0.0, src(y, x))));
+                fi.G()(y, x) = 128;
+                fi.B()(y, x) = 255 - fi.R()(y, x);
+            }
+        Frame go = filter(fi);
+        double ierr = 0;
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) ierr += std::fabs(go.R()(y, x) - 255.0 * (x < W / 2 ? 0.3 : 0.7));
+        Plane pi(0, W, H, 10);  // 10-bit full-range luma plane
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) pi(y, x) = (int)std::lround(1023.0 * std::fmin(1.0, std::fmax(0.0, src(y, x))));
+        Plane po = filter(pi);
+        std::printf("int frame R mean abs err %.3f levels; int plane Ceil %d out(0,0) %d\n", ierr / (W * H), po.Ceil(),
+                    po(0, 0));
         BM3D_Final_Para fp("np");
         fp.BlockSize = 11;  // not implemented by the GPU kernels: reported, not crashed
         BM3D_Final fin(fp);

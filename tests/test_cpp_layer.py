@@ -33,3 +33,6 @@ def test_cpp_example_runs(gpu, tmp_path):
     assert "expected error -2" in out.stdout
     err = float(out.stdout.split("mean abs err")[1].split()[0])
     assert err < 0.05
+    ierr = float(out.stdout.split("int frame R mean abs err")[1].split()[0])
+    assert ierr < 0.05 * 255
+    assert "int plane Ceil 1023" in out.stdout

@@ -230,6 +230,22 @@ def test_device_and_host_agree(gpu):
     np.testing.assert_allclose(dev.cpu().numpy(), host, atol=1e-5)
 
 
+def test_large_host_call_after_device_calls(gpu):
+    # host buffers (staged through the context's workspace) after device calls
+    # on the same context give the device result
+    import torch
+    clean, n = synth.rgb_pair(540, 960, 25.0, 19)
+    f = BM3D(para(), precision="fp16")
+    x = torch.from_numpy(n).to(gpu)
+    dev = f(x).cpu().numpy()
+    dev2 = f(x).cpu().numpy()
+    np.testing.assert_array_equal(dev, dev2)
+    for k in range(2):
+        host = f(n)
+        say(f"  host call {k}: max |host - device| = {np.abs(host - dev).max():.3g}")
+        np.testing.assert_array_equal(host, dev)
+
+
 @pytest.mark.parametrize("prec", ["fp32", "fp16"])
 def test_bit_repeatable(gpu, prec):
     # aggregation sums are 64-bit fixed point: the order of the atomic adds
