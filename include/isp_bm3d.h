@@ -159,6 +159,65 @@ int isp_bm3d_last_groups(isp_bm3d_ctx *ctx, int final_step, int *counts, int *po
 int isp_bm3d_grid_size(const isp_bm3d_ctx *ctx, int width, int height, int final_step,
                        int *ngx, int *ngy);
 
+/* ---------------------------------------------------------------------------
+ * Batches: many images of any sizes filtered together, one launch per stage
+ * for the whole batch, replayed as one hipGraph.  No reference counterpart
+ * (ISP_MW filters one image per call, include/Filter.h:173-216); each image's
+ * result equals a single call on it with the same parameters and tiles, bit
+ * for bit.
+ * ------------------------------------------------------------------------- */
+typedef struct isp_bm3d_image {
+    float *dst[3];          /* device planes: [0] for grey (nplanes 1), [0..2] R, G, B */
+    const float *src[3];
+    const float *ref[3];    /* step-1 matching guide (grey) / guide frame (RGB), or NULL: src */
+    int width, height;
+    size_t stride;          /* row pitch of dst, src and ref in elements */
+    double sigma;           /* > 0: this image's sigma (0..255 units) for every plane of both
+                               steps, thMSE from thMSE_Default; <= 0: the context's parameters */
+} isp_bm3d_image;
+
+typedef struct isp_bm3d_batch isp_bm3d_batch;
+
+/* flags */
+#define ISP_BM3D_BATCH_AUTO_GROUP 1  /* the final step's GroupSize of each image from the
+                                        tuning table of its size class, when one is set */
+
+/* mode: ISP_BM3D_BASIC or ISP_BM3D_BOTH.  Buffers are device memory; the plan
+ * keeps the pointers (change them with isp_bm3d_batch_update). */
+int isp_bm3d_batch_create(isp_bm3d_ctx *ctx, isp_bm3d_batch **batch, const isp_bm3d_image *images, int count,
+                          int nplanes, int mode, int flags);
+/* Runs the whole batch on `stream` (hipStream_t): use_graph 1 replays the
+ * captured hipGraph, 0 enqueues the same launches one by one. */
+int isp_bm3d_batch_run(isp_bm3d_batch *batch, void *stream, int use_graph);
+/* New buffers and sigmas for images of the same sizes (waits for the device). */
+int isp_bm3d_batch_update(isp_bm3d_batch *batch, const isp_bm3d_image *images);
+/* kernel launches per run, device bytes held, workgroups of the largest launch */
+int isp_bm3d_batch_info(const isp_bm3d_batch *batch, int *launches, size_t *device_bytes, int *max_workgroups);
+int isp_bm3d_batch_destroy(isp_bm3d_batch *batch);
+
+/* ---------------------------------------------------------------------------
+ * Autotuning: tile shapes (reference blocks per workgroup) per size class,
+ * measured on a batch of seeded synthetic images; used by every later call
+ * and batch on the context.  Tile shapes never change results.  With
+ * ISP_BM3D_BATCH_AUTO_GROUP the tuner also tries a final-step GroupSize of 16
+ * (against the context's) and keeps it when it is faster and its PSNR on the
+ * synthetic images is within 0.05 dB.  No reference counterpart.
+ * ------------------------------------------------------------------------- */
+typedef struct isp_bm3d_tuning {
+    int match_tile[2];      /* x, y */
+    int filter_tile[2];     /* of the engine the context's precision selects */
+    int final_group_size;   /* 0: the context's */
+    double ms_per_image;    /* measured with the chosen shapes */
+    double psnr_db;         /* of the synthetic images with the chosen shapes */
+} isp_bm3d_tuning;
+
+int isp_bm3d_size_class(int width, int height);  /* ceil(log2(width * height)) */
+int isp_bm3d_autotune(isp_bm3d_ctx *ctx, int width, int height, double sigma, int flags, isp_bm3d_tuning *out);
+/* t NULL clears the entry */
+int isp_bm3d_set_tuning(isp_bm3d_ctx *ctx, int size_class, const isp_bm3d_tuning *t);
+/* returns 1 and fills t when the class has an entry, 0 otherwise */
+int isp_bm3d_get_tuning(const isp_bm3d_ctx *ctx, int size_class, isp_bm3d_tuning *t);
+
 #ifdef __cplusplus
 }
 #endif

@@ -36,7 +36,17 @@ EXPORTS = (
     "isp_bm3d_get_timing",
     "isp_bm3d_reset_timing",
     "isp_bm3d_last_groups",
+    "isp_bm3d_batch_create",
+    "isp_bm3d_batch_run",
+    "isp_bm3d_batch_update",
+    "isp_bm3d_batch_info",
+    "isp_bm3d_batch_destroy",
+    "isp_bm3d_size_class",
+    "isp_bm3d_autotune",
+    "isp_bm3d_set_tuning",
+    "isp_bm3d_get_tuning",
 )
+BATCH_AUTO_GROUP = 1
 NUM_STAGES = 8
 
 
@@ -60,6 +70,32 @@ class Params(ctypes.Structure):
     """isp_bm3d_params (mirror of BM3D_Para, include/BM3D.h:154-168)."""
 
     _fields_ = [("basic", StepParams), ("final_", StepParams)]
+
+
+class Image(ctypes.Structure):
+    """isp_bm3d_image: one image of a batch (device planes)."""
+
+    _fields_ = [
+        ("dst", ctypes.c_void_p * 3),
+        ("src", ctypes.c_void_p * 3),
+        ("ref", ctypes.c_void_p * 3),
+        ("width", ctypes.c_int),
+        ("height", ctypes.c_int),
+        ("stride", ctypes.c_size_t),
+        ("sigma", ctypes.c_double),
+    ]
+
+
+class Tuning(ctypes.Structure):
+    """isp_bm3d_tuning: tile shapes (and final GroupSize) of one size class."""
+
+    _fields_ = [
+        ("match_tile", ctypes.c_int * 2),
+        ("filter_tile", ctypes.c_int * 2),
+        ("final_group_size", ctypes.c_int),
+        ("ms_per_image", ctypes.c_double),
+        ("psnr_db", ctypes.c_double),
+    ]
 
 
 class BM3DError(RuntimeError):
@@ -109,6 +145,15 @@ def lib() -> ctypes.CDLL:
                                       ctypes.POINTER(ctypes.c_long), i]
     L.isp_bm3d_reset_timing.argtypes = [vp]
     L.isp_bm3d_last_groups.argtypes = [vp, i, vp, vp, sz]
+    L.isp_bm3d_batch_create.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(Image), i, i, i, i]
+    L.isp_bm3d_batch_run.argtypes = [vp, vp, i]
+    L.isp_bm3d_batch_update.argtypes = [vp, ctypes.POINTER(Image)]
+    L.isp_bm3d_batch_info.argtypes = [vp, ctypes.POINTER(i), ctypes.POINTER(sz), ctypes.POINTER(i)]
+    L.isp_bm3d_batch_destroy.argtypes = [vp]
+    L.isp_bm3d_size_class.argtypes = [i, i]
+    L.isp_bm3d_autotune.argtypes = [vp, i, i, ctypes.c_double, i, ctypes.POINTER(Tuning)]
+    L.isp_bm3d_set_tuning.argtypes = [vp, i, ctypes.POINTER(Tuning)]
+    L.isp_bm3d_get_tuning.argtypes = [vp, i, ctypes.POINTER(Tuning)]
     for name in EXPORTS:
         getattr(L, name).restype = getattr(L, name).restype or ctypes.c_int
     L.isp_bm3d_last_error.restype = ctypes.c_char_p

@@ -152,6 +152,11 @@ class BM3D_Para:
         return p
 
 
+def _tuning_dict(t) -> dict:
+    return {"match_tile": (t.match_tile[0], t.match_tile[1]), "filter_tile": (t.filter_tile[0], t.filter_tile[1]),
+            "final_group_size": t.final_group_size, "ms_per_image": t.ms_per_image, "psnr_db": t.psnr_db}
+
+
 def _is_torch(x) -> bool:
     return type(x).__module__.startswith("torch")
 
@@ -267,6 +272,37 @@ class _Filter:
         if rc < 0:
             L.check(rc, self._ctx)
         return (counts, pos) if with_pos else counts
+
+    # ---- tuning (tile shapes per size class; no reference counterpart)
+    @staticmethod
+    def size_class(width: int, height: int) -> int:
+        return int(L.lib().isp_bm3d_size_class(int(width), int(height)))
+
+    def autotune(self, width: int, height: int, sigma: float = 0.0, auto_group: bool = False) -> dict:
+        """Measures tile shapes (and, with auto_group, a final GroupSize of 16)
+        on seeded synthetic images of this size; the choice applies to every
+        later call and batch of the size class."""
+        t = L.Tuning()
+        L.check(L.lib().isp_bm3d_autotune(self._ctx, int(width), int(height), float(sigma),
+                                          L.BATCH_AUTO_GROUP if auto_group else 0, ctypes.byref(t)), self._ctx)
+        return _tuning_dict(t)
+
+    def set_tuning(self, size_class: int, tuning: dict | None) -> None:
+        if tuning is None:
+            L.check(L.lib().isp_bm3d_set_tuning(self._ctx, int(size_class), None), self._ctx)
+            return
+        t = L.Tuning()
+        t.match_tile[0], t.match_tile[1] = tuning["match_tile"]
+        t.filter_tile[0], t.filter_tile[1] = tuning["filter_tile"]
+        t.final_group_size = int(tuning.get("final_group_size", 0))
+        L.check(L.lib().isp_bm3d_set_tuning(self._ctx, int(size_class), ctypes.byref(t)), self._ctx)
+
+    def get_tuning(self, size_class: int) -> dict | None:
+        t = L.Tuning()
+        rc = L.lib().isp_bm3d_get_tuning(self._ctx, int(size_class), ctypes.byref(t))
+        if rc < 0:
+            L.check(rc, self._ctx)
+        return _tuning_dict(t) if rc == 1 else None
 
     def grid_shape(self, width: int, height: int, final_step: bool = False) -> tuple[int, int]:
         ngx, ngy = ctypes.c_int(), ctypes.c_int()

@@ -12,7 +12,9 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -31,6 +33,13 @@ struct isp_bm3d_ctx {
     size_t ws_bytes = 0;
     char *pin = nullptr;  // page-locked staging for host-memory calls
     size_t pin_bytes = 0;
+    // tile shapes (reference blocks per workgroup) of the single-image launches
+    struct Tiles {
+        int match_x = ispbm3d::kTileRX, match_y = ispbm3d::kTileRY;
+        int filter_x = ispbm3d::kTileRX, filter_y = ispbm3d::kTileRY;
+        int mfma_x = ispbm3d::kMTileRX, mfma_y = ispbm3d::kMTileRY;
+    } tiles;
+    std::map<int, isp_bm3d_tuning> tune;  // size class -> tuned tiles (and group size)
     std::string err;
     // groups of the last run of each step (device, inside ws)
     int *last_cnt[2] = {nullptr, nullptr};
@@ -352,11 +361,42 @@ struct StepIO {
     size_t out_pitch;
 };
 
-int run_match(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, const Derived &d, Work &w,
-              const float *g32, const double *g64, int gpitch, int W, int H, double *keys,
-              hipStream_t s, int stage)
+int size_class(int W, int H)
+{
+    const unsigned long long n = (unsigned long long)W * (unsigned long long)H;
+    int k = 0;
+    while (k < 63 && (1ull << k) < n) ++k;
+    return k;
+}
+
+// Tile shapes of an image: the tuning entry of its size class, else the defaults.
+isp_bm3d_ctx::Tiles tiles_for(const isp_bm3d_ctx *ctx, int W, int H)
+{
+    isp_bm3d_ctx::Tiles t = ctx->tiles;
+    auto it = ctx->tune.find(size_class(W, H));
+    if (it != ctx->tune.end()) {
+        t.match_x = it->second.match_tile[0];
+        t.match_y = it->second.match_tile[1];
+        if (ctx->precision != ISP_BM3D_PREC_FP32) {
+            t.mfma_x = it->second.filter_tile[0];
+            t.mfma_y = it->second.filter_tile[1];
+        } else {
+            t.filter_x = it->second.filter_tile[0];
+            t.filter_y = it->second.filter_tile[1];
+        }
+    }
+    return t;
+}
+
+// Arguments of one matching launch on one image (guide: FP32 or FP64 plane).
+MatchArgs build_match(const isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, const Derived &d, int W, int H,
+                      const void *guide, int gpitch, int trx, int try_, int *cnt, int *pos, double *keys)
 {
     MatchArgs a{};
+    a.guide = guide;
+    a.gpitch = gpitch;
+    a.trx = trx;
+    a.try_ = try_;
     a.W = W;
     a.H = H;
     a.ngx = d.ngx;
@@ -371,36 +411,93 @@ int run_match(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, const Derived &d
     // the plain FP32 path reproduces the reference's tie order bit for bit;
     // the mixed-precision forms make no parity claim and use fused FMA
     a.exact = ctx->precision == ISP_BM3D_PREC_FP32;
-    a.tileW = (kTileRX - 1) * p.block_step + 2 * d.range + kBlock + (kStrip - 1) * p.bm_step;
-    a.tileH = (kTileRY - 1) * p.block_step + 2 * d.range + kBlock;
+    a.tileW = (trx - 1) * p.block_step + 2 * d.range + kBlock + (kStrip - 1) * p.bm_step;
+    a.tileH = (try_ - 1) * p.block_step + 2 * d.range + kBlock;
     a.tilePitch = a.tileW | 1;
     const int side = 2 * (d.range / p.bm_step) + 1;
     a.maxCand = side * side;
-    const int k = stage == ST_MATCH_FINAL ? 1 : 0;
-    a.cnt_out = w.cnt[k];
-    a.pos_out = w.pos[k];
-    ctx->last_cnt[k] = w.cnt[k];
-    ctx->last_pos[k] = w.pos[k];
-    ctx->last_refs[k] = (size_t)d.ngx * d.ngy;
-    ctx->last_G[k] = p.group_size;
+    a.cnt_out = cnt;
+    a.pos_out = pos;
     a.key_out = keys;
+    return a;
+}
+
+int check_match_lds(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, const MatchArgs &a)
+{
     if (match_lds_bytes(a) > 160 * 1024)
         return fail(ctx, ISP_BM3D_E_UNSUPPORTED,
                     "BMrange %d / BMstep %d / BlockStep %d need %zu bytes of LDS for matching (max 163840)",
                     p.bm_range, p.bm_step, p.block_step, match_lds_bytes(a));
-    StageTimer tm(ctx, stage, s);
-    if (g64) HIP_CHECK(ctx, launch_match_f64(g64, gpitch, a, s));
-    else HIP_CHECK(ctx, launch_match_f32(g32, gpitch, a, s));
     return ISP_BM3D_OK;
 }
 
-// One BM3D_Base::Kernel (source/BM3D.cpp:80-218) followed by finalize.
-int run_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, const StepIO &io, Work &w,
-             int W, int H, hipStream_t s)
+// Arguments of one filtering launch on one image; the engine (FP32 VALU or
+// the matrix cores) follows the context's precision.
+FilterArgs build_filter(const isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, const Derived &d, int W, int H,
+                        int pitch, const float *const src[3], const float *const guide[3],
+                        unsigned long long *const num[3], unsigned long long *const den[3], const int *cnt,
+                        const int *pos, int trx, int try_)
 {
-    Derived d{};
-    int rc = derive(ctx, p, io.nplanes, W, H, d);
-    if (rc) return rc;
+    FilterArgs f{};
+    f.trx = trx;
+    f.try_ = try_;
+    f.W = W;
+    f.H = H;
+    f.pitch = pitch;
+    f.ngx = d.ngx;
+    f.ngy = d.ngy;
+    f.bstep = p.block_step;
+    f.range = d.range;
+    f.G = p.group_size;
+    f.cnt = cnt;
+    f.pos = pos;
+    for (int c = 0; c < 3; ++c) {
+        f.src[c] = src[c];
+        f.guide[c] = guide[c];
+        f.num[c] = num[c];
+        f.den[c] = den[c];
+        f.active[c] = d.active[c];
+        f.thr[c] = d.thr[c];
+        f.sig2[c] = d.sig2[c];
+        f.thr64[c] = d.thr64[c];
+        f.band[c] = d.band[c];
+    }
+    f.FW = (trx - 1) * p.block_step + 2 * d.range + kBlock;
+    f.FH = (try_ - 1) * p.block_step + 2 * d.range + kBlock;
+    f.c8 = ctx->d_tables;
+    f.cn = ctx->d_tables;
+    for (int n = 0; n <= kMaxGroup; ++n) {
+        f.cn_off[n] = ctx->cn_off[n];
+        f.cnd_off[n] = ctx->cnd_off[n];
+    }
+    f.c8d = ctx->d_tables64;
+    f.cnd = ctx->d_tables64;
+    if (ctx->precision != ISP_BM3D_PREC_FP32) {
+        const uint16_t *t = ctx->d_mfma[ctx->precision == ISP_BM3D_PREC_FP16 ? 0 : 1];
+        f.kb = t;
+        f.ka = t + 8 * 64 * 8;
+        f.cna = t + 16 * 64 * 8;
+        f.cnb = f.cna + (size_t)kMaxGroup * 2 * 64 * 8;
+    }
+    return f;
+}
+
+size_t filter_lds(const isp_bm3d_ctx *ctx, const FilterArgs &f, bool wiener)
+{
+    return ctx->precision != ISP_BM3D_PREC_FP32 ? filter_mfma_lds_bytes(f, wiener) : filter_lds_bytes(f, wiener);
+}
+
+hipError_t launch_filter_any(const isp_bm3d_ctx *ctx, const FilterArgs &f, bool wiener, hipStream_t s,
+                             const BatchLaunch *b = nullptr)
+{
+    if (ctx->precision != ISP_BM3D_PREC_FP32) return launch_filter_mfma(f, wiener, ctx->precision, s, b);
+    return launch_filter(f, wiener, s, b);
+}
+
+// The output side of one step: num/den, then the plane / RGB conversion.
+FinalizeArgs build_finalize(int W, int H, const StepIO &io, const Derived &d, unsigned long long *const num[3],
+                            unsigned long long *const den[3])
+{
     FinalizeArgs fa{};
     fa.W = W;
     fa.H = H;
@@ -419,34 +516,45 @@ int run_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, cons
     for (int c = 0; c < 3; ++c) {
         fa.src[c] = io.src[c];
         fa.out[c] = io.out[c];
-        fa.num[c] = w.num[c];
-        fa.den[c] = w.den[c];
-        fa.active[c] = 0;
+        fa.num[c] = num[c];
+        fa.den[c] = den[c];
+        fa.active[c] = d.degenerate ? 0 : d.active[c];
     }
+    return fa;
+}
+
+int run_match(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, const Derived &d, Work &w,
+              const float *g32, const double *g64, int gpitch, int W, int H, double *keys,
+              hipStream_t s, int stage)
+{
+    const int k = stage == ST_MATCH_FINAL ? 1 : 0;
+    const isp_bm3d_ctx::Tiles t = tiles_for(ctx, W, H);
+    const MatchArgs a = build_match(ctx, p, d, W, H, g64 ? (const void *)g64 : (const void *)g32, gpitch,
+                                    t.match_x, t.match_y, w.cnt[k], w.pos[k], keys);
+    ctx->last_cnt[k] = w.cnt[k];
+    ctx->last_pos[k] = w.pos[k];
+    ctx->last_refs[k] = (size_t)d.ngx * d.ngy;
+    ctx->last_G[k] = p.group_size;
+    int rc = check_match_lds(ctx, p, a);
+    if (rc) return rc;
+    StageTimer tm(ctx, stage, s);
+    HIP_CHECK(ctx, launch_match(a, g64 != nullptr, s));
+    return ISP_BM3D_OK;
+}
+
+// One BM3D_Base::Kernel (source/BM3D.cpp:80-218) followed by finalize.
+int run_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, const StepIO &io, Work &w,
+             int W, int H, hipStream_t s)
+{
+    Derived d{};
+    int rc = derive(ctx, p, io.nplanes, W, H, d);
+    if (rc) return rc;
+    const FinalizeArgs fa = build_finalize(W, H, io, d, w.num, w.den);
     if (!d.degenerate) {
         rc = run_match(ctx, p, d, w, io.mguide32, io.mguide64, io.mguide64 ? io.pitch64 : io.pitch, W, H,
                        nullptr, s, wiener ? ST_MATCH_FINAL : ST_MATCH_BASIC);
         if (rc) return rc;
-        FilterArgs f{};
-        f.W = W;
-        f.H = H;
-        f.pitch = io.pitch;
-        f.ngx = d.ngx;
-        f.ngy = d.ngy;
-        f.bstep = p.block_step;
-        f.range = d.range;
-        f.G = p.group_size;
-        f.cnt = w.cnt[wiener ? 1 : 0];
-        f.pos = w.pos[wiener ? 1 : 0];
         for (int c = 0; c < 3; ++c) {
-            f.src[c] = io.src[c];
-            f.guide[c] = io.guide[c];
-            f.num[c] = w.num[c];
-            f.den[c] = w.den[c];
-            f.active[c] = d.active[c];
-            f.thr[c] = d.thr[c];
-            f.sig2[c] = d.sig2[c];
-            fa.active[c] = d.active[c];
             if (d.active[c]) {
                 StageTimer tm(ctx, ST_FILL, s);
                 // zero bits of the 64-bit accumulators
@@ -455,34 +563,15 @@ int run_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, cons
             }
         }
         const bool mfma = ctx->precision != ISP_BM3D_PREC_FP32;
-        const int trx = mfma ? kMTileRX : kTileRX, try_ = mfma ? kMTileRY : kTileRY;
-        f.FW = (trx - 1) * p.block_step + 2 * d.range + kBlock;
-        f.FH = (try_ - 1) * p.block_step + 2 * d.range + kBlock;
-        f.c8 = ctx->d_tables;
-        f.cn = ctx->d_tables;
-        for (int n = 0; n <= kMaxGroup; ++n) {
-            f.cn_off[n] = ctx->cn_off[n];
-            f.cnd_off[n] = ctx->cnd_off[n];
-        }
-        f.c8d = ctx->d_tables64;
-        f.cnd = ctx->d_tables64;
-        for (int c = 0; c < 3; ++c) {
-            f.thr64[c] = d.thr64[c];
-            f.band[c] = d.band[c];
-        }
-        if (mfma) {
-            const uint16_t *t = ctx->d_mfma[ctx->precision == ISP_BM3D_PREC_FP16 ? 0 : 1];
-            f.kb = t;
-            f.ka = t + 8 * 64 * 8;
-            f.cna = t + 16 * 64 * 8;
-            f.cnb = f.cna + (size_t)kMaxGroup * 2 * 64 * 8;
-        }
-        const size_t lds = mfma ? filter_mfma_lds_bytes(f, wiener) : filter_lds_bytes(f, wiener);
+        const int k = wiener ? 1 : 0;
+        const isp_bm3d_ctx::Tiles t = tiles_for(ctx, W, H);
+        const FilterArgs f = build_filter(ctx, p, d, W, H, io.pitch, io.src, io.guide, w.num, w.den, w.cnt[k],
+                                          w.pos[k], mfma ? t.mfma_x : t.filter_x, mfma ? t.mfma_y : t.filter_y);
+        const size_t lds = filter_lds(ctx, f, wiener);
         if (lds > 160 * 1024)
             return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "parameters need %zu bytes of LDS for filtering", lds);
         StageTimer tm(ctx, wiener ? ST_FILTER_FINAL : ST_FILTER_BASIC, s);
-        if (mfma) HIP_CHECK(ctx, launch_filter_mfma(f, wiener, ctx->precision, s));
-        else HIP_CHECK(ctx, launch_filter(f, wiener, s));
+        HIP_CHECK(ctx, launch_filter_any(ctx, f, wiener, s));
     }
     StageTimer tm(ctx, ST_FINALIZE, s);
     HIP_CHECK(ctx, launch_finalize(fa, s));
@@ -604,6 +693,559 @@ std::vector<float> mfma_tables_f32(size_t *n_kb, size_t *n_ka, size_t *n_cn)
     return t;
 }
 
+
+// ---------------------------------------------------------------------------
+// Batches.  Every image gets its own workspace; per step, one launch of each
+// stage covers all images through a workgroup -> (image, tile) map and an
+// array of per-image arguments in device memory.  The launches of a run are
+// captured once into a hipGraph.
+struct BatchImg {
+    int W, H, P;            // P: row pitch of the caller's planes and of the workspace planes
+    isp_bm3d_params prm;
+    Derived d[2];
+    isp_bm3d_ctx::Tiles tiles;
+    float *yuv[3];          // RGB: OPP planes
+    double *y64;            // RGB: FP64 luma guide (of ref when given)
+    float *basic[3];
+    unsigned long long *num[3], *den[3];
+    int *cnt[2], *pos[2];
+};
+
+struct BatchStepPlan {
+    bool any = false;          // some image runs matching/filtering in this step
+    MatchArgs hm{};            // launch header: largest LDS
+    FilterArgs hf{};
+    FinalizeArgs hfin{};
+    std::vector<MatchArgs> m;
+    std::vector<FilterArgs> f;
+    std::vector<FinalizeArgs> fin;
+    std::vector<uint32_t> mt, ft;
+    MatchArgs *dm = nullptr;
+    FilterArgs *df = nullptr;
+    FinalizeArgs *dfin = nullptr;
+    uint32_t *dmt = nullptr, *dft = nullptr;
+};
+
+}  // namespace
+
+struct isp_bm3d_batch {
+    isp_bm3d_ctx *ctx = nullptr;
+    int count = 0, nplanes = 1, mode = ISP_BM3D_BOTH, flags = 0;
+    bool guide_y64 = false;          // RGB: step-1 guide is FP64
+    std::vector<isp_bm3d_image> img;
+    std::vector<BatchImg> bi;
+    BatchStepPlan st[2];
+    OppArgs ho{};
+    std::vector<OppArgs> o;          // per image: RGB -> OPP
+    std::vector<OppArgs> oref;       // per image: ConvertToY of its guide frame (when it has one)
+    OppArgs *dop = nullptr, *dopref = nullptr;
+    std::vector<uint32_t> rows;      // (image << 20) | y over all images
+    std::vector<uint32_t> rows_ref;  // the same over the images with a guide frame
+    uint32_t *drows = nullptr, *drows_ref = nullptr;
+    int maxW = 0;
+    void *mem = nullptr;
+    size_t mem_bytes = 0;
+    unsigned long long *acc = nullptr;
+    size_t acc_words = 0;
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int launches = 0;
+    unsigned max_wg = 0;
+    // tile overrides of the autotuner (x <= 0: none)
+    int ov_match[2] = {0, 0}, ov_filter[2] = {0, 0}, ov_group = 0;
+};
+
+namespace {
+
+void batch_free(isp_bm3d_batch *b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->cap) (void)hipStreamDestroy(b->cap);
+    if (b->mem) (void)hipFree(b->mem);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->cap = nullptr;
+    b->mem = nullptr;
+}
+
+// Per-image parameters: the context's, with the image's sigma and thMSE_Default.
+int batch_params(isp_bm3d_batch *b, int k)
+{
+    isp_bm3d_ctx *ctx = b->ctx;
+    const isp_bm3d_image &im = b->img[k];
+    BatchImg &x = b->bi[k];
+    x.prm = ctx->params;
+    if (im.sigma > 0) {
+        for (int c = 0; c < 3; ++c) x.prm.basic.sigma[c] = x.prm.final_.sigma[c] = im.sigma;
+        int rc = isp_bm3d_thmse_default(&x.prm.basic, 0);
+        if (!rc) rc = isp_bm3d_thmse_default(&x.prm.final_, 1);
+        if (rc) return fail(ctx, ISP_BM3D_E_ARG, "image %d: thMSE_Default failed", k);
+    }
+    x.tiles = tiles_for(ctx, x.W, x.H);
+    if (b->ov_match[0] > 0) {
+        x.tiles.match_x = b->ov_match[0];
+        x.tiles.match_y = b->ov_match[1];
+    }
+    if (b->ov_filter[0] > 0) {
+        x.tiles.filter_x = x.tiles.mfma_x = b->ov_filter[0];
+        x.tiles.filter_y = x.tiles.mfma_y = b->ov_filter[1];
+    }
+    int g = 0;
+    if (b->ov_group > 0) g = b->ov_group;
+    else if (b->flags & ISP_BM3D_BATCH_AUTO_GROUP) {
+        auto it = ctx->tune.find(size_class(x.W, x.H));
+        if (it != ctx->tune.end()) g = it->second.final_group_size;
+    }
+    if (g > 0) x.prm.final_.group_size = g;
+    for (int s = 0; s < 2; ++s) {
+        int rc = derive(ctx, s ? x.prm.final_ : x.prm.basic, b->nplanes, x.W, x.H, x.d[s]);
+        if (rc) return rc;
+    }
+    return ISP_BM3D_OK;
+}
+
+// Builds the per-image launch arguments of both steps (pointers included).
+int batch_args(isp_bm3d_batch *b)
+{
+    isp_bm3d_ctx *ctx = b->ctx;
+    const bool mfma = ctx->precision != ISP_BM3D_PREC_FP32;
+    const int nsteps = b->mode == ISP_BM3D_BOTH ? 2 : 1;
+    b->o.clear();
+    b->oref.clear();
+    for (int s = 0; s < 2; ++s) {
+        BatchStepPlan &sp = b->st[s];
+        sp.m.clear();
+        sp.f.clear();
+        sp.fin.clear();
+    }
+    for (int k = 0; k < b->count; ++k) {
+        const isp_bm3d_image &im = b->img[k];
+        BatchImg &x = b->bi[k];
+        const bool rgb = b->nplanes == 3;
+        const bool sep = im.ref[0] && (im.ref[0] != im.src[0]);
+        const float *src[3] = {nullptr, nullptr, nullptr};
+        if (rgb) {
+            OppArgs q{};
+            q.W = x.W;
+            q.H = x.H;
+            q.r = im.src[0];
+            q.g = im.src[1];
+            q.b = im.src[2];
+            q.in_int = 0;
+            q.gainY = 1.0 / 3.0;
+            q.offsetY = 0.0;
+            q.gainU = 1.0 / 2.0;
+            q.gainV = 1.0 / 4.0;
+            q.offsetC = 0.0;
+            q.in_pitch = (size_t)x.P;
+            q.y = x.yuv[0];
+            q.u = x.yuv[1];
+            q.v = x.yuv[2];
+            q.pitch = x.P;
+            q.y64 = sep ? nullptr : x.y64;
+            q.pitch64 = x.P;
+            b->o.push_back(q);
+            // BM3D_Basic::RGB2YUV: ConvertToY of the guide (source/BM3D.cpp:320-336)
+            OppArgs r = q;
+            r.r = sep ? im.ref[0] : im.src[0];
+            r.g = sep ? im.ref[1] : im.src[1];
+            r.b = sep ? im.ref[2] : im.src[2];
+            r.y = r.u = r.v = nullptr;
+            r.y64 = sep ? x.y64 : nullptr;
+            b->oref.push_back(r);
+            for (int c = 0; c < 3; ++c) src[c] = x.yuv[c];
+        } else {
+            src[0] = im.src[0];
+        }
+        for (int s = 0; s < nsteps; ++s) {
+            BatchStepPlan &sp = b->st[s];
+            const bool fin = s == 1;
+            const isp_bm3d_step_params &p = fin ? x.prm.final_ : x.prm.basic;
+            const Derived &d = x.d[s];
+            // step input / guides / output (source/BM3D.cpp:501-527, 464-472)
+            const float *guide[3] = {src[0], src[1], src[2]};
+            const void *mg;
+            if (fin) {
+                for (int c = 0; c < 3; ++c) guide[c] = x.basic[c];
+                mg = x.basic[0];
+            } else if (rgb) {
+                mg = x.y64;
+            } else {
+                mg = sep ? im.ref[0] : im.src[0];
+            }
+            StepIO io{};
+            io.nplanes = b->nplanes;
+            io.pitch = x.P;
+            for (int c = 0; c < 3; ++c) io.src[c] = src[c];
+            const bool last = s == nsteps - 1;
+            if (last) {
+                io.out_pitch = (size_t)x.P;
+                for (int c = 0; c < 3; ++c) io.out[c] = im.dst[c];
+                if (rgb) {
+                    // MatrixConvert_YUV2RGB OPP with clip (include/Conversion.hpp:660-762)
+                    io.to_rgb = 1;
+                    io.clip = 1;
+                    io.gain_y = io.gain_c = 1.0;
+                    io.offset = 0.0;
+                    io.lo = 0.0;
+                    io.hi = 1.0;
+                }
+            } else {
+                io.out_pitch = (size_t)x.P;
+                for (int c = 0; c < 3; ++c) io.out[c] = x.basic[c];
+            }
+            sp.fin.push_back(build_finalize(x.W, x.H, io, d, x.num, x.den));
+            if (d.degenerate) {
+                sp.m.push_back(MatchArgs{});
+                sp.f.push_back(FilterArgs{});
+                continue;
+            }
+            const MatchArgs ma = build_match(ctx, p, d, x.W, x.H, mg, x.P, x.tiles.match_x, x.tiles.match_y,
+                                             x.cnt[s], x.pos[s], nullptr);
+            int rc = check_match_lds(ctx, p, ma);
+            if (rc) return rc;
+            sp.m.push_back(ma);
+            const FilterArgs fa =
+                build_filter(ctx, p, d, x.W, x.H, x.P, src, guide, x.num, x.den, x.cnt[s], x.pos[s],
+                             mfma ? x.tiles.mfma_x : x.tiles.filter_x, mfma ? x.tiles.mfma_y : x.tiles.filter_y);
+            if (filter_lds(ctx, fa, fin) > 160 * 1024)
+                return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "image %d needs %zu bytes of LDS for filtering", k,
+                            filter_lds(ctx, fa, fin));
+            sp.f.push_back(fa);
+        }
+    }
+    return ISP_BM3D_OK;
+}
+
+// Tile maps, launch headers and the device copies of the argument arrays.
+int batch_upload(isp_bm3d_batch *b)
+{
+    isp_bm3d_ctx *ctx = b->ctx;
+    const int nsteps = b->mode == ISP_BM3D_BOTH ? 2 : 1;
+    for (int s = 0; s < nsteps; ++s) {
+        BatchStepPlan &sp = b->st[s];
+        sp.mt.clear();
+        sp.ft.clear();
+        sp.any = false;
+        size_t lm = 0, lf = 0;
+        for (int k = 0; k < b->count; ++k) {
+            if (b->bi[k].d[s].degenerate) continue;
+            const MatchArgs &m = sp.m[k];
+            const FilterArgs &f = sp.f[k];
+            const int mx = (m.ngx + m.trx - 1) / m.trx, my = (m.ngy + m.try_ - 1) / m.try_;
+            const int fx = (f.ngx + f.trx - 1) / f.trx, fy = (f.ngy + f.try_ - 1) / f.try_;
+            if (mx > kMaxBatchTiles || my > kMaxBatchTiles || fx > kMaxBatchTiles || fy > kMaxBatchTiles)
+                return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "image %d: more than %d tiles along an axis", k,
+                            kMaxBatchTiles);
+            for (int ty = 0; ty < my; ++ty)
+                for (int tx = 0; tx < mx; ++tx)
+                    sp.mt.push_back(((uint32_t)k << kBatchImageShift) | ((uint32_t)ty << 10) | (uint32_t)tx);
+            for (int ty = 0; ty < fy; ++ty)
+                for (int tx = 0; tx < fx; ++tx)
+                    sp.ft.push_back(((uint32_t)k << kBatchImageShift) | ((uint32_t)ty << 10) | (uint32_t)tx);
+            const size_t a = match_lds_bytes(m), c = filter_lds(ctx, f, s == 1);
+            if (!sp.any || a > lm) {
+                sp.hm = m;
+                lm = a;
+            }
+            if (!sp.any || c > lf) {
+                sp.hf = f;
+                lf = c;
+            }
+            sp.any = true;
+        }
+        sp.hfin = sp.fin[0];
+        b->max_wg = std::max<unsigned>(b->max_wg, (unsigned)std::max(sp.mt.size(), sp.ft.size()));
+    }
+    // device copies
+    auto put = [&](void *dst, const void *src, size_t bytes) -> int {
+        if (bytes) HIP_CHECK(ctx, hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+        return ISP_BM3D_OK;
+    };
+    int rc;
+    for (int s = 0; s < nsteps; ++s) {
+        BatchStepPlan &sp = b->st[s];
+        if ((rc = put(sp.dm, sp.m.data(), sp.m.size() * sizeof(MatchArgs)))) return rc;
+        if ((rc = put(sp.df, sp.f.data(), sp.f.size() * sizeof(FilterArgs)))) return rc;
+        if ((rc = put(sp.dfin, sp.fin.data(), sp.fin.size() * sizeof(FinalizeArgs)))) return rc;
+        if ((rc = put(sp.dmt, sp.mt.data(), sp.mt.size() * 4))) return rc;
+        if ((rc = put(sp.dft, sp.ft.data(), sp.ft.size() * 4))) return rc;
+    }
+    if (b->nplanes == 3) {
+        if ((rc = put(b->dop, b->o.data(), b->o.size() * sizeof(OppArgs)))) return rc;
+        if ((rc = put(b->dopref, b->oref.data(), b->oref.size() * sizeof(OppArgs)))) return rc;
+        if ((rc = put(b->drows_ref, b->rows_ref.data(), b->rows_ref.size() * 4))) return rc;
+        b->ho = b->o[0];
+    }
+    return put(b->drows, b->rows.data(), b->rows.size() * 4);
+}
+
+// All launches of one run, in stream order on s.
+int batch_enqueue(isp_bm3d_batch *b, hipStream_t s)
+{
+    isp_bm3d_ctx *ctx = b->ctx;
+    int n = 0;
+    const unsigned nrows = (unsigned)b->rows.size();
+    if (b->nplanes == 3) {
+        // OPP of every image, then ConvertToY of the guides that differ from src
+        BatchLaunch bl{b->drows, b->dop, nrows, b->maxW};
+        HIP_CHECK(ctx, launch_opp(b->ho, s, &bl));
+        ++n;
+        if (!b->rows_ref.empty()) {
+            BatchLaunch bg{b->drows_ref, b->dopref, (unsigned)b->rows_ref.size(), b->maxW};
+            HIP_CHECK(ctx, launch_opp(b->ho, s, &bg));
+            ++n;
+        }
+    }
+    const int nsteps = b->mode == ISP_BM3D_BOTH ? 2 : 1;
+    for (int st = 0; st < nsteps; ++st) {
+        BatchStepPlan &sp = b->st[st];
+        const bool fin = st == 1;
+        if (sp.any) {
+            HIP_CHECK(ctx, launch_fill(reinterpret_cast<float *>(b->acc), b->acc_words * 2, 0.f, s));
+            BatchLaunch bm{sp.dmt, sp.dm, (unsigned)sp.mt.size(), 0};
+            HIP_CHECK(ctx, launch_match(sp.hm, b->nplanes == 3 && !fin, s, &bm));
+            BatchLaunch bf{sp.dft, sp.df, (unsigned)sp.ft.size(), 0};
+            HIP_CHECK(ctx, launch_filter_any(ctx, sp.hf, fin, s, &bf));
+            n += 3;
+        }
+        BatchLaunch bz{b->drows, sp.dfin, nrows, b->maxW};
+        HIP_CHECK(ctx, launch_finalize(sp.hfin, s, &bz));
+        ++n;
+    }
+    b->launches = n;
+    return ISP_BM3D_OK;
+}
+
+// Device memory of a batch: accumulators of all images first (one zero-fill
+// span), then per-image planes and groups, then the argument arrays and maps.
+size_t batch_layout(isp_bm3d_batch *b, void *base)
+{
+    Carver c(base);
+    const int nsteps = b->mode == ISP_BM3D_BOTH ? 2 : 1;
+    size_t acc0 = 0;
+    for (int k = 0; k < b->count; ++k) {
+        BatchImg &x = b->bi[k];
+        const size_t np = (size_t)x.P * x.H;
+        for (int pl = 0; pl < 3; ++pl) {
+            x.num[pl] = pl < b->nplanes ? c.take<unsigned long long>(np) : nullptr;
+            x.den[pl] = pl < b->nplanes ? c.take<unsigned long long>(np) : nullptr;
+            if (k == 0 && pl == 0) acc0 = (size_t)((char *)x.num[0] - (char *)base);
+        }
+    }
+    b->acc = reinterpret_cast<unsigned long long *>(base ? (char *)base + acc0 : nullptr);
+    b->acc_words = (c.off - acc0 + 7) / 8;
+    size_t mt[2] = {0, 0}, ft[2] = {0, 0};
+    for (int k = 0; k < b->count; ++k) {
+        BatchImg &x = b->bi[k];
+        const size_t np = (size_t)x.P * x.H;
+        for (int pl = 0; pl < 3; ++pl) {
+            x.yuv[pl] = b->nplanes == 3 ? c.take<float>(np) : nullptr;
+            x.basic[pl] = (pl < b->nplanes && nsteps == 2) ? c.take<float>(np) : nullptr;
+        }
+        x.y64 = b->nplanes == 3 ? c.take<double>(np) : nullptr;
+        for (int s = 0; s < 2; ++s) {
+            const Derived &d = x.d[s];
+            const bool used = s < nsteps && !d.degenerate;
+            const size_t refs = used ? (size_t)d.ngx * d.ngy : 0;
+            const int G = s ? x.prm.final_.group_size : x.prm.basic.group_size;
+            x.cnt[s] = used ? c.take<int>(refs) : nullptr;
+            x.pos[s] = used ? c.take<int>(refs * (size_t)std::max(G, 1)) : nullptr;
+            if (used) {
+                const bool mf = b->ctx->precision != ISP_BM3D_PREC_FP32;
+                const int tfx = mf ? x.tiles.mfma_x : x.tiles.filter_x, tfy = mf ? x.tiles.mfma_y : x.tiles.filter_y;
+                mt[s] += (size_t)((d.ngx + x.tiles.match_x - 1) / x.tiles.match_x) *
+                         ((d.ngy + x.tiles.match_y - 1) / x.tiles.match_y);
+                ft[s] += (size_t)((d.ngx + tfx - 1) / tfx) * ((d.ngy + tfy - 1) / tfy);
+            }
+        }
+    }
+    for (int s = 0; s < nsteps; ++s) {
+        BatchStepPlan &sp = b->st[s];
+        sp.dm = c.take<MatchArgs>(b->count);
+        sp.df = c.take<FilterArgs>(b->count);
+        sp.dfin = c.take<FinalizeArgs>(b->count);
+        sp.dmt = c.take<uint32_t>(std::max<size_t>(mt[s], 1));
+        sp.dft = c.take<uint32_t>(std::max<size_t>(ft[s], 1));
+    }
+    b->dop = b->nplanes == 3 ? c.take<OppArgs>(b->count) : nullptr;
+    b->dopref = b->nplanes == 3 ? c.take<OppArgs>(b->count) : nullptr;
+    b->drows = c.take<uint32_t>(b->rows.size());
+    b->drows_ref = c.take<uint32_t>(std::max<size_t>(b->rows_ref.size(), 1));
+    return c.off + 256;
+}
+
+int batch_build(isp_bm3d_batch *b, const isp_bm3d_image *images)
+{
+    isp_bm3d_ctx *ctx = b->ctx;
+    batch_free(b);
+    b->img.assign(images, images + b->count);
+    b->bi.assign(b->count, BatchImg{});
+    b->rows.clear();
+    b->rows_ref.clear();
+    b->maxW = 0;
+    b->max_wg = 0;
+    for (int k = 0; k < b->count; ++k) {
+        const isp_bm3d_image &im = b->img[k];
+        for (int c = 0; c < b->nplanes; ++c)
+            if (!im.dst[c] || !im.src[c]) return fail(ctx, ISP_BM3D_E_ARG, "image %d: null plane %d", k, c);
+        if (b->nplanes == 3 && im.ref[0] && !(im.ref[1] && im.ref[2]))
+            return fail(ctx, ISP_BM3D_E_ARG, "image %d: ref needs all three planes", k);
+        if (im.width <= 0 || im.height <= 0) return fail(ctx, ISP_BM3D_E_SIZE, "image %d is empty", k);
+        if (im.stride < (size_t)im.width || im.stride > ((size_t)1 << 30))
+            return fail(ctx, ISP_BM3D_E_ARG, "image %d: bad stride %zu", k, im.stride);
+        BatchImg &x = b->bi[k];
+        x.W = im.width;
+        x.H = im.height;
+        x.P = (int)im.stride;
+        int rc = batch_params(b, k);
+        if (rc) return rc;
+        b->maxW = std::max(b->maxW, x.W);
+        const bool sep = b->nplanes == 3 && im.ref[0] && im.ref[0] != im.src[0];
+        for (int y = 0; y < x.H; ++y) {
+            b->rows.push_back(((uint32_t)k << kBatchImageShift) | (uint32_t)y);
+            if (sep) b->rows_ref.push_back(((uint32_t)k << kBatchImageShift) | (uint32_t)y);
+        }
+    }
+    if (b->rows.size() > (size_t)kMaxBatchRows)
+        return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "a batch holds at most %d rows in all", kMaxBatchRows);
+    const size_t bytes = batch_layout(b, nullptr);
+    HIP_CHECK(ctx, hipMalloc(&b->mem, bytes));
+    b->mem_bytes = bytes;
+    batch_layout(b, b->mem);
+    int rc = batch_args(b);
+    if (!rc) rc = batch_upload(b);
+    if (rc) return rc;
+    HIP_CHECK(ctx, hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+    HIP_CHECK(ctx, hipStreamBeginCapture(b->cap, hipStreamCaptureModeRelaxed));
+    rc = batch_enqueue(b, b->cap);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->cap, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    HIP_CHECK(ctx, e);
+    b->graph = g;
+    HIP_CHECK(ctx, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    return ISP_BM3D_OK;
+}
+
+int batch_make(isp_bm3d_ctx *ctx, isp_bm3d_batch **out, const isp_bm3d_image *images, int count, int nplanes,
+               int mode, int flags, const int *ov_match, const int *ov_filter, int ov_group)
+{
+    if (!ctx) return fail(nullptr, ISP_BM3D_E_ARG, "null context");
+    if (!out || !images) return fail(ctx, ISP_BM3D_E_ARG, "null argument");
+    *out = nullptr;
+    if (count < 1 || count > kMaxBatchImages)
+        return fail(ctx, ISP_BM3D_E_ARG, "batch of %d images (1..%d)", count, kMaxBatchImages);
+    if (nplanes != 1 && nplanes != 3) return fail(ctx, ISP_BM3D_E_ARG, "nplanes %d (1 or 3)", nplanes);
+    if (mode != ISP_BM3D_BASIC && mode != ISP_BM3D_BOTH)
+        return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "batches run ISP_BM3D_BASIC or ISP_BM3D_BOTH");
+    HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    isp_bm3d_batch *b = new isp_bm3d_batch;
+    b->ctx = ctx;
+    b->count = count;
+    b->nplanes = nplanes;
+    b->mode = mode;
+    b->flags = flags;
+    if (ov_match) {
+        b->ov_match[0] = ov_match[0];
+        b->ov_match[1] = ov_match[1];
+    }
+    if (ov_filter) {
+        b->ov_filter[0] = ov_filter[0];
+        b->ov_filter[1] = ov_filter[1];
+    }
+    b->ov_group = ov_group;
+    const int rc = batch_build(b, images);
+    if (rc) {
+        batch_free(b);
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return ISP_BM3D_OK;
+}
+
+// Seeded synthetic grey image of the measurement workloads: a gradient,
+// 20 flat rectangles and 4 gratings, clipped to [0,1].
+struct SplitMix {
+    uint64_t s;
+    uint64_t next()
+    {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uni() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+    double normal()
+    {
+        const double u1 = 1.0 - uni(), u2 = uni();
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    }
+};
+
+void synth_clean(std::vector<float> &img, int W, int H, uint64_t seed)
+{
+    SplitMix r{seed};
+    img.assign((size_t)W * H, 0.f);
+    const double gx = r.uni() * 0.6 - 0.3, gy = r.uni() * 0.6 - 0.3;
+    std::vector<double> v((size_t)W * H);
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) v[(size_t)y * W + x] = 0.25 + gx * ((double)x / W - 0.5) + gy * ((double)y / H - 0.5);
+    for (int k = 0; k < 20; ++k) {
+        const int x0 = (int)(r.uni() * W), y0 = (int)(r.uni() * H);
+        const int w = (int)(W / 32.0 + r.uni() * (W / 4.0 - W / 32.0)), h = (int)(H / 32.0 + r.uni() * (H / 4.0 - H / 32.0));
+        const double val = r.uni();
+        for (int y = y0; y < std::min(H, y0 + h); ++y)
+            for (int x = x0; x < std::min(W, x0 + w); ++x) v[(size_t)y * W + x] = val;
+    }
+    for (int k = 0; k < 4; ++k) {
+        const double a = 0.05 + 0.15 * r.uni(), T = 4.0 + 28.0 * r.uni(), th = 3.141592653589793 * r.uni();
+        const double ph = 6.283185307179586 * r.uni(), cx = std::cos(th), sy = std::sin(th);
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) v[(size_t)y * W + x] += a * std::sin(6.283185307179586 * (x * cx + y * sy) / T + ph);
+    }
+    for (size_t i = 0; i < v.size(); ++i) img[i] = (float)std::min(1.0, std::max(0.0, v[i]));
+}
+
+double psnr01(const float *a, const float *b, size_t n)
+{
+    double se = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const double d = (double)a[i] - (double)b[i];
+        se += d * d;
+    }
+    return se > 0 ? 10.0 * std::log10((double)n / se) : 200.0;
+}
+
+// Milliseconds per run of a batch (graph replay), best of `reps` after a warm-up.
+int batch_time(isp_bm3d_ctx *ctx, isp_bm3d_batch *b, hipStream_t s, int reps, double *ms)
+{
+    hipEvent_t e0, e1;
+    HIP_CHECK(ctx, hipEventCreate(&e0));
+    HIP_CHECK(ctx, hipEventCreate(&e1));
+    double best = 1e30;
+    hipError_t err = hipGraphLaunch(b->exec, s);
+    for (int r = 0; r < reps && err == hipSuccess; ++r) {
+        (void)hipEventRecord(e0, s);
+        err = hipGraphLaunch(b->exec, s);
+        (void)hipEventRecord(e1, s);
+        if (err == hipSuccess) err = hipEventSynchronize(e1);
+        float t = 0;
+        if (err == hipSuccess) err = hipEventElapsedTime(&t, e0, e1);
+        best = std::min(best, (double)t);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIP_CHECK(ctx, err);
+    *ms = best;
+    return ISP_BM3D_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -1217,4 +1859,203 @@ int isp_bm3d_match(isp_bm3d_ctx *ctx, const float *guide, int W, int H, size_t s
     return ISP_BM3D_OK;
 }
 
+
+int isp_bm3d_batch_create(isp_bm3d_ctx *ctx, isp_bm3d_batch **batch, const isp_bm3d_image *images, int count,
+                          int nplanes, int mode, int flags)
+{
+    return batch_make(ctx, batch, images, count, nplanes, mode, flags, nullptr, nullptr, 0);
+}
+
+int isp_bm3d_batch_run(isp_bm3d_batch *b, void *stream, int use_graph)
+{
+    if (!b) return fail(nullptr, ISP_BM3D_E_ARG, "null batch");
+    isp_bm3d_ctx *ctx = b->ctx;
+    HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    if (use_graph) {
+        HIP_CHECK(ctx, hipGraphLaunch(b->exec, (hipStream_t)stream));
+        return ISP_BM3D_OK;
+    }
+    return batch_enqueue(b, (hipStream_t)stream);
+}
+
+int isp_bm3d_batch_update(isp_bm3d_batch *b, const isp_bm3d_image *images)
+{
+    if (!b || !images) return fail(b ? b->ctx : nullptr, ISP_BM3D_E_ARG, "null argument");
+    isp_bm3d_ctx *ctx = b->ctx;
+    for (int k = 0; k < b->count; ++k)
+        if (images[k].width != b->img[k].width || images[k].height != b->img[k].height)
+            return fail(ctx, ISP_BM3D_E_ARG, "image %d changed size; create a new batch", k);
+    HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    HIP_CHECK(ctx, hipDeviceSynchronize());
+    return batch_build(b, images);
+}
+
+int isp_bm3d_batch_info(const isp_bm3d_batch *b, int *launches, size_t *device_bytes, int *max_workgroups)
+{
+    if (!b) return fail(nullptr, ISP_BM3D_E_ARG, "null batch");
+    if (launches) *launches = b->launches;
+    if (device_bytes) *device_bytes = b->mem_bytes;
+    if (max_workgroups) *max_workgroups = (int)b->max_wg;
+    return ISP_BM3D_OK;
+}
+
+int isp_bm3d_batch_destroy(isp_bm3d_batch *b)
+{
+    if (!b) return ISP_BM3D_OK;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipDeviceSynchronize();
+    batch_free(b);
+    delete b;
+    return ISP_BM3D_OK;
+}
+
+int isp_bm3d_size_class(int width, int height)
+{
+    if (width <= 0 || height <= 0) return ISP_BM3D_E_ARG;
+    return size_class(width, height);
+}
+
+int isp_bm3d_set_tuning(isp_bm3d_ctx *ctx, int cls, const isp_bm3d_tuning *t)
+{
+    if (!ctx) return fail(nullptr, ISP_BM3D_E_ARG, "null context");
+    if (cls < 0 || cls > 62) return fail(ctx, ISP_BM3D_E_ARG, "size class %d", cls);
+    if (!t) {
+        ctx->tune.erase(cls);
+        return ISP_BM3D_OK;
+    }
+    for (int k = 0; k < 2; ++k)
+        if (t->match_tile[k] < 1 || t->filter_tile[k] < 1 || t->match_tile[0] * t->match_tile[1] > kMaxTileRefs ||
+            t->filter_tile[0] * t->filter_tile[1] > kMaxTileRefs)
+            return fail(ctx, ISP_BM3D_E_ARG, "tile shapes must be positive with at most %d blocks", kMaxTileRefs);
+    if (t->final_group_size < 0 || t->final_group_size > kMaxGroup)
+        return fail(ctx, ISP_BM3D_E_ARG, "final_group_size %d", t->final_group_size);
+    ctx->tune[cls] = *t;
+    return ISP_BM3D_OK;
+}
+
+int isp_bm3d_get_tuning(const isp_bm3d_ctx *ctx, int cls, isp_bm3d_tuning *t)
+{
+    if (!ctx || !t) return ISP_BM3D_E_ARG;
+    auto it = ctx->tune.find(cls);
+    if (it == ctx->tune.end()) return 0;
+    *t = it->second;
+    return 1;
+}
+
+int isp_bm3d_autotune(isp_bm3d_ctx *ctx, int W, int H, double sigma, int flags, isp_bm3d_tuning *out)
+{
+    if (!ctx) return fail(nullptr, ISP_BM3D_E_ARG, "null context");
+    if (W < kBlock || H < kBlock) return fail(ctx, ISP_BM3D_E_SIZE, "image %dx%d", W, H);
+    if ((size_t)W * H > ((size_t)1 << 26)) return fail(ctx, ISP_BM3D_E_ARG, "tuning image too large");
+    HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    const int cls = size_class(W, H);
+    ctx->tune.erase(cls);
+    // the synthetic workload: K noisy copies of one image, enough tiles to fill the GPU
+    std::vector<float> clean, noisy;
+    synth_clean(clean, W, H, 0x5EED0000ull + (uint64_t)cls);
+    noisy = clean;
+    SplitMix rn{0xA5A5ull + (uint64_t)W * 131 + (uint64_t)H};
+    const double sg = (sigma > 0 ? sigma : ctx->params.basic.sigma[0]) / 255.0;
+    for (auto &v : noisy) v = (float)(v + sg * rn.normal());
+    Derived d{};
+    int rc = derive(ctx, ctx->params.basic, 1, W, H, d);
+    if (rc) return rc;
+    const long tiles = (long)((d.ngx + 7) / 8) * ((d.ngy + 7) / 8);
+    const int K = (int)std::max(1L, std::min(32L, (4096 + tiles - 1) / std::max(1L, tiles)));
+    const size_t np = (size_t)W * H;
+    float *dmem = nullptr;
+    HIP_CHECK(ctx, hipMalloc(&dmem, np * 4 * (K + 1)));
+    struct Guard {
+        float *p;
+        ~Guard() { (void)hipFree(p); }
+    } guard{dmem};
+    HIP_CHECK(ctx, hipMemcpy(dmem, noisy.data(), np * 4, hipMemcpyHostToDevice));
+    std::vector<isp_bm3d_image> imgs(K);
+    for (int k = 0; k < K; ++k) {
+        isp_bm3d_image &im = imgs[k];
+        std::memset(&im, 0, sizeof(im));
+        im.dst[0] = dmem + np * (k + 1);
+        im.src[0] = dmem;
+        im.width = W;
+        im.height = H;
+        im.stride = (size_t)W;
+        im.sigma = sigma;
+    }
+    hipStream_t s = nullptr;
+    HIP_CHECK(ctx, hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    struct SGuard {
+        hipStream_t s;
+        ~SGuard() { (void)hipStreamDestroy(s); }
+    } sguard{s};
+    auto measure = [&](const int *mt, const int *ft, int g, double *ms, double *psnr) -> int {
+        isp_bm3d_batch *b = nullptr;
+        int r = batch_make(ctx, &b, imgs.data(), K, 1, ISP_BM3D_BOTH, 0, mt, ft, g);
+        if (r) return r;
+        r = batch_time(ctx, b, s, 3, ms);
+        if (!r && psnr) {
+            std::vector<float> outp(np);
+            hipError_t e = hipMemcpy(outp.data(), imgs[0].dst[0], np * 4, hipMemcpyDeviceToHost);
+            if (e != hipSuccess) r = fail(ctx, ISP_BM3D_E_HIP, "copy: %s", hipGetErrorString(e));
+            else *psnr = psnr01(outp.data(), clean.data(), np);
+        }
+        (void)isp_bm3d_batch_destroy(b);
+        return r;
+    };
+    const bool mf = ctx->precision != ISP_BM3D_PREC_FP32;
+    const int mcands[][2] = {{8, 4}, {8, 8}, {16, 4}, {4, 8}, {16, 8}};
+    const int fcands_m[][2] = {{8, 8}, {16, 8}, {8, 16}, {16, 16}};
+    const int fcands_v[][2] = {{8, 4}, {8, 8}, {16, 4}, {4, 8}};
+    int best_m[2] = {ctx->tiles.match_x, ctx->tiles.match_y};
+    int best_f[2] = {mf ? ctx->tiles.mfma_x : ctx->tiles.filter_x, mf ? ctx->tiles.mfma_y : ctx->tiles.filter_y};
+    double best = 1e30, psnr_base = 0;
+    for (const auto &c : mcands) {
+        double ms = 0;
+        rc = measure(c, best_f, 0, &ms, nullptr);
+        if (rc == ISP_BM3D_E_UNSUPPORTED) continue;
+        if (rc) return rc;
+        if (ms < best) {
+            best = ms;
+            best_m[0] = c[0];
+            best_m[1] = c[1];
+        }
+    }
+    const int nf = 4;
+    for (int i = 0; i < nf; ++i) {
+        const int *c = mf ? fcands_m[i] : fcands_v[i];
+        double ms = 0;
+        rc = measure(best_m, c, 0, &ms, nullptr);
+        if (rc == ISP_BM3D_E_UNSUPPORTED) continue;
+        if (rc) return rc;
+        if (ms < best) {
+            best = ms;
+            best_f[0] = c[0];
+            best_f[1] = c[1];
+        }
+    }
+    rc = measure(best_m, best_f, 0, &best, &psnr_base);
+    if (rc) return rc;
+    int group = 0;
+    double psnr = psnr_base;
+    if ((flags & ISP_BM3D_BATCH_AUTO_GROUP) && ctx->params.final_.group_size > 16) {
+        double ms16 = 0, p16 = 0;
+        rc = measure(best_m, best_f, 16, &ms16, &p16);
+        if (rc) return rc;
+        if (ms16 < best && psnr_base - p16 <= 0.05) {
+            group = 16;
+            best = ms16;
+            psnr = p16;
+        }
+    }
+    isp_bm3d_tuning t{};
+    t.match_tile[0] = best_m[0];
+    t.match_tile[1] = best_m[1];
+    t.filter_tile[0] = best_f[0];
+    t.filter_tile[1] = best_f[1];
+    t.final_group_size = group;
+    t.ms_per_image = best / K;
+    t.psnr_db = psnr;
+    ctx->tune[cls] = t;
+    if (out) *out = t;
+    return ISP_BM3D_OK;
+}
 }  // extern "C"

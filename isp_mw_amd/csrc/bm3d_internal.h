@@ -9,16 +9,29 @@ namespace ispbm3d {
 
 constexpr int kBlock = 8;        // BlockSize implemented by the GPU kernels
 constexpr int kMaxGroup = 32;    // largest GroupSize implemented
-constexpr int kTileRX = 8;       // reference blocks per workgroup tile, x
-constexpr int kTileRY = 4;       // reference blocks per workgroup tile, y
+// Default tile shapes (reference blocks per workgroup, x by y); the tile
+// shape is a launch parameter (MatchArgs / FilterArgs trx, try_) that the
+// autotuner may change.
+constexpr int kTileRX = 8;       // matching and the FP32 filter
+constexpr int kTileRY = 4;
 constexpr int kStrip = 5;        // candidates per lane strip in matching
 constexpr int kWaves = 4;        // waves per workgroup
 constexpr int kThreads = 64 * kWaves;
-// MFMA filter: 8 waves over a tile of 8 x 8 reference blocks
+// MFMA filter: 8 waves over a tile of 8 x 8 reference blocks by default
 constexpr int kMTileRX = 8;
 constexpr int kMTileRY = 8;
 constexpr int kMWaves = 8;
 constexpr int kMThreads = 64 * kMWaves;
+constexpr int kMaxTileRefs = 256;  // trx * try_ bound
+
+// Batched launches: one launch covers the tiles (or rows) of many images.
+// Workgroup b reads unit = map[b]: image (unit >> 20) and, for tiled kernels,
+// tile ((unit >> 10) & 1023, unit & 1023) as (y, x), for row kernels row
+// (unit & 0xFFFFF); its arguments are args[image].
+constexpr int kBatchImageShift = 20;
+constexpr int kMaxBatchImages = 4096;
+constexpr int kMaxBatchTiles = 1024;  // tiles per image per axis
+constexpr int kMaxBatchRows = 1 << 20;
 
 // Fixed-point scale of the aggregation sums: value * 2^36 rounded to int64.
 // A pixel collects at most 64 block origins x (2R/s+1)^2 groups of weight
@@ -29,6 +42,9 @@ constexpr double kFixInv = 1.0 / 68719476736.0;    // 2^-36
 
 // Positions are packed (y << 16) | x.
 struct MatchArgs {
+    const void *guide;       // matching guide plane (float or double by launch)
+    int gpitch;              // its row pitch in elements
+    int trx, try_;           // tile shape in reference blocks
     int W, H;
     int ngx, ngy;
     int bstep;               // BlockStep: grid position k is min(k*bstep, last)
@@ -47,6 +63,7 @@ struct MatchArgs {
 };
 
 struct FilterArgs {
+    int trx, try_;            // tile shape in reference blocks
     int W, H, pitch;          // geometry of src/guide/num/den planes
     int ngx, ngy;
     int bstep;
@@ -128,14 +145,23 @@ struct ToFloatArgs {
 size_t match_lds_bytes(const MatchArgs &a);
 size_t filter_lds_bytes(const FilterArgs &a, bool wiener);
 
-hipError_t launch_match_f32(const float *guide, int gpitch, const MatchArgs &a, hipStream_t s);
-hipError_t launch_match_f64(const double *guide, int gpitch, const MatchArgs &a, hipStream_t s);
-hipError_t launch_filter(const FilterArgs &a, bool wiener, hipStream_t s);
+// A batched launch: `units` workgroups (or rows), map and args in device memory.
+struct BatchLaunch {
+    const uint32_t *map;
+    const void *args;      // device array of the kernel's argument struct
+    unsigned units;
+    int maxW;              // row kernels: widest image
+};
+
+// guide64: the guide is FP64 (else FP32).  `b` null: a single-image launch.
+hipError_t launch_match(const MatchArgs &a, bool guide64, hipStream_t s, const BatchLaunch *b = nullptr);
+hipError_t launch_filter(const FilterArgs &a, bool wiener, hipStream_t s, const BatchLaunch *b = nullptr);
 // precision: 1 = FP16, 2 = BF16 operands with FP32 accumulation
-hipError_t launch_filter_mfma(const FilterArgs &a, bool wiener, int precision, hipStream_t s);
+hipError_t launch_filter_mfma(const FilterArgs &a, bool wiener, int precision, hipStream_t s,
+                              const BatchLaunch *b = nullptr);
 size_t filter_mfma_lds_bytes(const FilterArgs &a, bool wiener);
-hipError_t launch_finalize(const FinalizeArgs &a, hipStream_t s);
-hipError_t launch_opp(const OppArgs &a, hipStream_t s);
+hipError_t launch_finalize(const FinalizeArgs &a, hipStream_t s, const BatchLaunch *b = nullptr);
+hipError_t launch_opp(const OppArgs &a, hipStream_t s, const BatchLaunch *b = nullptr);
 hipError_t launch_to_float(const ToFloatArgs &a, hipStream_t s);
 hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s);
 hipError_t launch_copy2d(float *dst, size_t dpitch, const float *src, size_t spitch, int W, int H,

@@ -36,6 +36,39 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
+// The image and tile (or row) a workgroup works on: its own launch's
+// arguments and blockIdx, or, in a batched launch, the map entry of this
+// workgroup and that image's arguments.
+template <bool BATCHED, typename A>
+__device__ __forceinline__ const A &tile_unit(const A &a0, const A *__restrict__ bargs,
+                                              const uint32_t *__restrict__ bmap, int &tx, int &ty)
+{
+    if constexpr (BATCHED) {
+        const uint32_t u = bmap[blockIdx.x];
+        tx = (int)(u & 1023u);
+        ty = (int)((u >> 10) & 1023u);
+        return bargs[u >> kBatchImageShift];
+    } else {
+        tx = (int)blockIdx.x;
+        ty = (int)blockIdx.y;
+        return a0;
+    }
+}
+
+template <bool BATCHED, typename A>
+__device__ __forceinline__ const A &row_unit(const A &a0, const A *__restrict__ bargs,
+                                             const uint32_t *__restrict__ bmap, int &y)
+{
+    if constexpr (BATCHED) {
+        const uint32_t u = bmap[blockIdx.y];
+        y = (int)(u & 0xFFFFFu);
+        return bargs[u >> kBatchImageShift];
+    } else {
+        y = (int)blockIdx.y;
+        return a0;
+    }
+}
+
 __device__ __forceinline__ float wave_sum(float v)
 {
 #pragma unroll
@@ -123,13 +156,18 @@ __device__ __forceinline__ double sq_acc(double d, double acc)
     return __builtin_fma(d, d, acc);
 }
 
-template <typename GT, bool ST1, bool EXACT>
-__global__ __launch_bounds__(kThreads) void match_kernel(const GT *__restrict__ guide, int gpitch,
-                                                         MatchArgs a)
+template <typename GT, bool ST1, bool EXACT, bool BATCHED>
+__global__ __launch_bounds__(kThreads) void match_kernel(MatchArgs a0, const uint32_t *__restrict__ bmap,
+                                                         const MatchArgs *__restrict__ bargs)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int tx0 = blockIdx.x * kTileRX, ty0 = blockIdx.y * kTileRY;
+    int tx, ty;
+    const MatchArgs &a = tile_unit<BATCHED>(a0, bargs, bmap, tx, ty);
+    const GT *__restrict__ guide = static_cast<const GT *>(a.guide);
+    const int gpitch = a.gpitch;
+    const int trx = a.trx, tnum = a.trx * a.try_;
+    const int tx0 = tx * trx, ty0 = ty * a.try_;
     const int ox = grid_pos(tx0, a.bstep, a.W - kBlock) - a.range;
     const int oy = grid_pos(ty0, a.bstep, a.H - kBlock) - a.range;
 
@@ -152,8 +190,8 @@ __global__ __launch_bounds__(kThreads) void match_kernel(const GT *__restrict__ 
     }
     __syncthreads();
 
-    for (int r = wave; r < kTileRX * kTileRY; r += kWaves) {
-        const int gxi = tx0 + (r % kTileRX), gyi = ty0 + (r / kTileRX);
+    for (int r = wave; r < tnum; r += kWaves) {
+        const int gxi = tx0 + (r % trx), gyi = ty0 + (r / trx);
         if (gxi >= a.ngx || gyi >= a.ngy) continue;
         const int i = grid_pos(gxi, a.bstep, a.W - kBlock), j = grid_pos(gyi, a.bstep, a.H - kBlock);
         const size_t refIdx = (size_t)gyi * a.ngx + gxi;
@@ -612,12 +650,16 @@ __device__ __forceinline__ void flush_tile(const FilterArgs &a, int c, const u64
     }
 }
 
-template <bool WIENER>
-__global__ __launch_bounds__(kThreads) void filter_kernel(FilterArgs a)
+template <bool WIENER, bool BATCHED>
+__global__ __launch_bounds__(kThreads) void filter_kernel(FilterArgs a0, const uint32_t *__restrict__ bmap,
+                                                          const FilterArgs *__restrict__ bargs)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int tx0 = blockIdx.x * kTileRX, ty0 = blockIdx.y * kTileRY;
+    int tx, ty;
+    const FilterArgs &a = tile_unit<BATCHED>(a0, bargs, bmap, tx, ty);
+    const int trx = a.trx, tnum = a.trx * a.try_;
+    const int tx0 = tx * trx, ty0 = ty * a.try_;
     const int ox = grid_pos(tx0, a.bstep, a.W - kBlock) - a.range;
     const int oy = grid_pos(ty0, a.bstep, a.H - kBlock) - a.range;
     const int FW = a.FW, FH = a.FH, DW = FW - kBlock + 1, DH = FH - kBlock + 1;
@@ -636,8 +678,8 @@ __global__ __launch_bounds__(kThreads) void filter_kernel(FilterArgs a)
         __syncthreads();
         const float *src = a.src[c];
         const float *gsrc = a.guide[c];
-        for (int r = wave; r < kTileRX * kTileRY; r += kWaves) {
-            const int gxi = tx0 + (r % kTileRX), gyi = ty0 + (r / kTileRX);
+        for (int r = wave; r < tnum; r += kWaves) {
+            const int gxi = tx0 + (r % trx), gyi = ty0 + (r / trx);
             if (gxi >= a.ngx || gyi >= a.ngy) continue;
             const size_t refIdx = (size_t)gyi * a.ngx + gxi;
             const int *pos = a.pos + refIdx * a.G;
@@ -808,14 +850,18 @@ __host__ __device__ inline size_t filter_mfma_lds(int FW, int FH, bool wiener)
     return filter_acc_floats(FW, FH) * 4 + kK64Elems * 2 + filter_mfma_stage_floats(FW, FH, wiener) * 4;
 }
 
-template <typename T, bool WIENER>
-__global__ __launch_bounds__(kMThreads) void filter_mfma_kernel(FilterArgs a)
+template <typename T, bool WIENER, bool BATCHED>
+__global__ __launch_bounds__(kMThreads) void filter_mfma_kernel(FilterArgs a0, const uint32_t *__restrict__ bmap,
+                                                                const FilterArgs *__restrict__ bargs)
 {
     typedef typename MfmaT<T>::v8 v8;
     extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int c = lane & 31, h = lane >> 5;
-    const int tx0 = blockIdx.x * kMTileRX, ty0 = blockIdx.y * kMTileRY;
+    int tx, ty;
+    const FilterArgs &a = tile_unit<BATCHED>(a0, bargs, bmap, tx, ty);
+    const int trx = a.trx, tnum = a.trx * a.try_;
+    const int tx0 = tx * trx, ty0 = ty * a.try_;
     const int ox = grid_pos(tx0, a.bstep, a.W - kBlock) - a.range;
     const int oy = grid_pos(ty0, a.bstep, a.H - kBlock) - a.range;
     const int FW = a.FW, FH = a.FH, DW = FW - kBlock + 1, DH = FH - kBlock + 1;
@@ -851,8 +897,8 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_kernel(FilterArgs a)
         for (int k = tid; k < DW * DH; k += kMThreads) accDen[k] = 0;
         __syncthreads();
         const float thr = a.thr[pl], sig2 = a.sig2[pl];
-        for (int r = wave; r < kMTileRX * kMTileRY; r += kMWaves) {
-            const int gxi = tx0 + (r % kMTileRX), gyi = ty0 + (r / kMTileRX);
+        for (int r = wave; r < tnum; r += kMWaves) {
+            const int gxi = tx0 + (r % trx), gyi = ty0 + (r / trx);
             if (gxi >= a.ngx || gyi >= a.ngy) continue;
             const size_t refIdx = (size_t)gyi * a.ngx + gxi;
             const int *pos = a.pos + refIdx * a.G;
@@ -925,11 +971,15 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_kernel(FilterArgs a)
 // rounding and clip, include/Conversion.hpp:241-256) or OPP -> RGB
 // (MatrixConvert_YUV2RGB, include/Conversion.hpp:660-762) with clip, float or
 // int.  FP64 arithmetic.
-__global__ void finalize_kernel(FinalizeArgs a)
+template <bool BATCHED>
+__global__ void finalize_kernel(FinalizeArgs a0, const uint32_t *__restrict__ bmap,
+                                const FinalizeArgs *__restrict__ bargs)
 {
     // separate mul and add roundings, as the reference's double code
 #pragma clang fp contract(off)
-    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    int y;
+    const FinalizeArgs &a = row_unit<BATCHED>(a0, bargs, bmap, y);
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= a.W) return;
     const size_t o = (size_t)y * a.pitch + x;
     double val[3] = {0.0, 0.0, 0.0};
@@ -960,11 +1010,14 @@ __global__ void finalize_kernel(FinalizeArgs a)
 
 // MatrixConvert_RGB2YUV / ConvertToY, OPP (include/Conversion.hpp:418-432,
 // 547-578) from float or int32 RGB, in FP64.
-__global__ void opp_kernel(OppArgs a)
+template <bool BATCHED>
+__global__ void opp_kernel(OppArgs a0, const uint32_t *__restrict__ bmap, const OppArgs *__restrict__ bargs)
 {
     // separate mul and add roundings, as the reference's double code
 #pragma clang fp contract(off)
-    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    int y;
+    const OppArgs &a = row_unit<BATCHED>(a0, bargs, bmap, y);
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= a.W) return;
     const size_t i = (size_t)y * a.in_pitch + x;
     double R, G, B;
@@ -1017,20 +1070,29 @@ hipError_t set_lds(K kernel, size_t bytes)
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-template <typename GT, bool EXACT>
-hipError_t launch_match_t(const GT *guide, int gpitch, const MatchArgs &a, hipStream_t s)
+template <typename GT, bool EXACT, bool ST1>
+hipError_t launch_match_t(const MatchArgs &a, hipStream_t s, const BatchLaunch *b)
 {
     const size_t lds = match_lds_bytes(a);
-    const dim3 grid((a.ngx + kTileRX - 1) / kTileRX, (a.ngy + kTileRY - 1) / kTileRY);
     hipError_t e;
-    if (a.step == 1) {
-        if ((e = set_lds(match_kernel<GT, true, EXACT>, lds)) != hipSuccess) return e;
-        hipLaunchKernelGGL((match_kernel<GT, true, EXACT>), grid, dim3(kThreads), lds, s, guide, gpitch, a);
+    if (b) {
+        auto k = match_kernel<GT, ST1, EXACT, true>;
+        if ((e = set_lds(k, lds)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k, dim3(b->units), dim3(kThreads), lds, s, a, b->map,
+                           static_cast<const MatchArgs *>(b->args));
     } else {
-        if ((e = set_lds(match_kernel<GT, false, EXACT>, lds)) != hipSuccess) return e;
-        hipLaunchKernelGGL((match_kernel<GT, false, EXACT>), grid, dim3(kThreads), lds, s, guide, gpitch, a);
+        auto k = match_kernel<GT, ST1, EXACT, false>;
+        if ((e = set_lds(k, lds)) != hipSuccess) return e;
+        const dim3 grid((a.ngx + a.trx - 1) / a.trx, (a.ngy + a.try_ - 1) / a.try_);
+        hipLaunchKernelGGL(k, grid, dim3(kThreads), lds, s, a, nullptr, nullptr);
     }
     return hipGetLastError();
+}
+
+template <typename GT, bool EXACT>
+hipError_t launch_match_g(const MatchArgs &a, hipStream_t s, const BatchLaunch *b)
+{
+    return a.step == 1 ? launch_match_t<GT, EXACT, true>(a, s, b) : launch_match_t<GT, EXACT, false>(a, s, b);
 }
 
 }  // namespace
@@ -1042,65 +1104,70 @@ size_t match_lds_bytes(const MatchArgs &a)
 
 size_t filter_lds_bytes(const FilterArgs &a, bool) { return filter_lds(a.FW, a.FH, a.G); }
 
-hipError_t launch_match_f32(const float *guide, int gpitch, const MatchArgs &a, hipStream_t s)
+hipError_t launch_match(const MatchArgs &a, bool guide64, hipStream_t s, const BatchLaunch *b)
 {
-    return a.exact ? launch_match_t<float, true>(guide, gpitch, a, s) : launch_match_t<float, false>(guide, gpitch, a, s);
+    if (guide64)
+        return a.exact ? launch_match_g<double, true>(a, s, b) : launch_match_g<double, false>(a, s, b);
+    return a.exact ? launch_match_g<float, true>(a, s, b) : launch_match_g<float, false>(a, s, b);
 }
 
-hipError_t launch_match_f64(const double *guide, int gpitch, const MatchArgs &a, hipStream_t s)
+template <typename K, typename A>
+hipError_t launch_tiled(K kernel, unsigned threads, size_t lds, const A &a, hipStream_t s, const BatchLaunch *b,
+                        dim3 grid)
 {
-    return a.exact ? launch_match_t<double, true>(guide, gpitch, a, s)
-                   : launch_match_t<double, false>(guide, gpitch, a, s);
+    hipError_t e;
+    if ((e = set_lds(kernel, lds)) != hipSuccess) return e;
+    if (b) hipLaunchKernelGGL(kernel, dim3(b->units), dim3(threads), lds, s, a, b->map, static_cast<const A *>(b->args));
+    else hipLaunchKernelGGL(kernel, grid, dim3(threads), lds, s, a, nullptr, nullptr);
+    return hipGetLastError();
 }
 
-hipError_t launch_filter(const FilterArgs &a, bool wiener, hipStream_t s)
+hipError_t launch_filter(const FilterArgs &a, bool wiener, hipStream_t s, const BatchLaunch *b)
 {
     const size_t lds = filter_lds_bytes(a, wiener);
-    const dim3 grid((a.ngx + kTileRX - 1) / kTileRX, (a.ngy + kTileRY - 1) / kTileRY);
-    hipError_t e;
-    if (wiener) {
-        if ((e = set_lds(filter_kernel<true>, lds)) != hipSuccess) return e;
-        hipLaunchKernelGGL(filter_kernel<true>, grid, dim3(kThreads), lds, s, a);
-    } else {
-        if ((e = set_lds(filter_kernel<false>, lds)) != hipSuccess) return e;
-        hipLaunchKernelGGL(filter_kernel<false>, grid, dim3(kThreads), lds, s, a);
-    }
-    return hipGetLastError();
+    const dim3 grid((a.ngx + a.trx - 1) / a.trx, (a.ngy + a.try_ - 1) / a.try_);
+    if (wiener)
+        return b ? launch_tiled(filter_kernel<true, true>, kThreads, lds, a, s, b, grid)
+                 : launch_tiled(filter_kernel<true, false>, kThreads, lds, a, s, b, grid);
+    return b ? launch_tiled(filter_kernel<false, true>, kThreads, lds, a, s, b, grid)
+             : launch_tiled(filter_kernel<false, false>, kThreads, lds, a, s, b, grid);
 }
 
 size_t filter_mfma_lds_bytes(const FilterArgs &a, bool wiener) { return filter_mfma_lds(a.FW, a.FH, wiener); }
 
-hipError_t launch_filter_mfma(const FilterArgs &a, bool wiener, int precision, hipStream_t s)
+template <typename T, bool W>
+hipError_t launch_mfma_t(const FilterArgs &a, size_t lds, dim3 grid, hipStream_t s, const BatchLaunch *b)
+{
+    return b ? launch_tiled(filter_mfma_kernel<T, W, true>, kMThreads, lds, a, s, b, grid)
+             : launch_tiled(filter_mfma_kernel<T, W, false>, kMThreads, lds, a, s, b, grid);
+}
+
+hipError_t launch_filter_mfma(const FilterArgs &a, bool wiener, int precision, hipStream_t s, const BatchLaunch *b)
 {
     const size_t lds = filter_mfma_lds_bytes(a, wiener);
-    const dim3 grid((a.ngx + kMTileRX - 1) / kMTileRX, (a.ngy + kMTileRY - 1) / kMTileRY);
-    hipError_t e;
-#define LAUNCH_MFMA(T, W)                                                                     \
-    do {                                                                                      \
-        if ((e = set_lds(filter_mfma_kernel<T, W>, lds)) != hipSuccess) return e;            \
-        hipLaunchKernelGGL((filter_mfma_kernel<T, W>), grid, dim3(kMThreads), lds, s, a);     \
-    } while (0)
-    if (precision == 1) {
-        if (wiener) LAUNCH_MFMA(_Float16, true);
-        else LAUNCH_MFMA(_Float16, false);
-    } else {
-        if (wiener) LAUNCH_MFMA(__bf16, true);
-        else LAUNCH_MFMA(__bf16, false);
-    }
-#undef LAUNCH_MFMA
+    const dim3 grid((a.ngx + a.trx - 1) / a.trx, (a.ngy + a.try_ - 1) / a.try_);
+    if (precision == 1)
+        return wiener ? launch_mfma_t<_Float16, true>(a, lds, grid, s, b) : launch_mfma_t<_Float16, false>(a, lds, grid, s, b);
+    return wiener ? launch_mfma_t<__bf16, true>(a, lds, grid, s, b) : launch_mfma_t<__bf16, false>(a, lds, grid, s, b);
+}
+
+template <typename K, typename A>
+hipError_t launch_rows(K k1, K kb, const A &a, hipStream_t s, const BatchLaunch *b)
+{
+    if (b) hipLaunchKernelGGL(kb, dim3((b->maxW + 255) / 256, b->units), dim3(256), 0, s, a, b->map,
+                              static_cast<const A *>(b->args));
+    else hipLaunchKernelGGL(k1, dim3((a.W + 255) / 256, a.H), dim3(256), 0, s, a, nullptr, nullptr);
     return hipGetLastError();
 }
 
-hipError_t launch_finalize(const FinalizeArgs &a, hipStream_t s)
+hipError_t launch_finalize(const FinalizeArgs &a, hipStream_t s, const BatchLaunch *b)
 {
-    hipLaunchKernelGGL(finalize_kernel, dim3((a.W + 255) / 256, a.H), dim3(256), 0, s, a);
-    return hipGetLastError();
+    return launch_rows(finalize_kernel<false>, finalize_kernel<true>, a, s, b);
 }
 
-hipError_t launch_opp(const OppArgs &a, hipStream_t s)
+hipError_t launch_opp(const OppArgs &a, hipStream_t s, const BatchLaunch *b)
 {
-    hipLaunchKernelGGL(opp_kernel, dim3((a.W + 255) / 256, a.H), dim3(256), 0, s, a);
-    return hipGetLastError();
+    return launch_rows(opp_kernel<false>, opp_kernel<true>, a, s, b);
 }
 
 hipError_t launch_to_float(const ToFloatArgs &a, hipStream_t s)

@@ -36,3 +36,25 @@ def test_null_and_bad_arguments():
 def test_sizes_of_structs():
     # 8 (profile) + 24 (sigma) + 5 ints (20) + pad 4 + thMSE 8 + lambda 8
     assert ctypes.sizeof(L.StepParams) == 72 and ctypes.sizeof(L.Params) == 144
+
+
+def test_batch_struct_sizes():
+    # 9 plane pointers, width, height, stride, sigma
+    assert ctypes.sizeof(L.Image) == 9 * 8 + 4 + 4 + 8 + 8
+    # 5 ints, pad, 2 doubles
+    assert ctypes.sizeof(L.Tuning) == 5 * 4 + 4 + 16
+
+
+def test_batch_and_tuning_argument_checks():
+    lib = L.lib()
+    plan = ctypes.c_void_p()
+    assert lib.isp_bm3d_batch_create(None, ctypes.byref(plan), None, 1, 1, L.BOTH, 0) == L.E_ARG
+    assert lib.isp_bm3d_batch_run(None, None, 1) == L.E_ARG
+    assert lib.isp_bm3d_batch_destroy(None) == L.OK
+    assert lib.isp_bm3d_autotune(None, 64, 64, 25.0, 0, None) == L.E_ARG
+    assert lib.isp_bm3d_set_tuning(None, 16, None) == L.E_ARG
+    # size class = ceil(log2(width * height))
+    assert lib.isp_bm3d_size_class(256, 256) == 16
+    assert lib.isp_bm3d_size_class(257, 256) == 17
+    assert lib.isp_bm3d_size_class(1920, 1080) == 21
+    assert lib.isp_bm3d_size_class(0, 5) == L.E_ARG
