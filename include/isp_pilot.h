@@ -1,0 +1,73 @@
+/*
+ * isp_pilot.h - C ABI of the learned pilot kernels in libisp_bm3d.so.
+ *
+ * A small residual CNN (3x3 convolutions, 48 channels, ReLU) can stand in for
+ * BM3D_Basic's estimate (source/BM3D.cpp:339-385) as the guide of the Wiener
+ * step BM3D_Final (source/BM3D.cpp:411-457); these are its building blocks.
+ * None of them has a counterpart in ISP_MW.
+ *
+ * Activations are BF16 (raw 16-bit patterns) in NHWC layout [n][h][w][c] with
+ * c a multiple of 16; arithmetic accumulates in FP32 on the matrix cores.
+ * Every pointer is device memory; work is enqueued on `stream` (a
+ * hipStream_t, NULL for the null stream) and nothing synchronises.
+ *
+ * Weights: FP32 master copy [cout][cin][9], tap = ky*3 + kx, cross-correlation
+ *   out[y][x][co] = b[co] + sum w[co][ci][ky*3+kx] * in[y+ky-1][x+kx-1][ci]
+ * with zeros outside the image.  isp_pilot_pack writes the BF16 forms the
+ * convolution reads: wf [cout_pad][kpad(cin_pad)] and wb [cin_pad][kpad(cout_pad)]
+ * (flipped and transposed, for the input gradient).
+ */
+#ifndef ISP_PILOT_H_
+#define ISP_PILOT_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ISP_PILOT_OK 0
+#define ISP_PILOT_E_ARG (-1)
+#define ISP_PILOT_E_UNSUPPORTED (-2)
+#define ISP_PILOT_E_HIP (-4)
+
+/* K extent of a packed weight row: 9 * cin_pad rounded up to 32 */
+int isp_pilot_kpad(int cin_pad);
+
+/* One 3x3 convolution.  (cin_pad, cout_pad) in {(16,48), (48,48), (48,16)}.
+ * mask (may be NULL): input values whose mask value is not > 0 read as 0 (the
+ * ReLU derivative of a post-ReLU activation, for the input gradient).
+ * bias may be NULL; relu != 0 applies max(., 0); out_f32 (may be NULL) gets
+ * the FP32 results too. */
+int isp_pilot_conv3x3(const void *in, const void *mask, const void *weights, const float *bias, void *out,
+                      float *out_f32, int n, int h, int w, int cin_pad, int cout_pad, int relu, void *stream);
+
+/* Weight and bias gradients of a convolution with real channel counts cin,
+ * cout: dw [cout][cin][9], db [cout] (may be NULL), with dz = dout * (mask > 0)
+ * (mask may be NULL).  partial: scratch of isp_pilot_wgrad_partial_floats
+ * floats.  The sums are reduced in a fixed order (run to run identical). */
+size_t isp_pilot_wgrad_partial_floats(int n, int h, int w, int cin, int cout);
+int isp_pilot_conv3x3_wgrad(const void *in, const void *dout, const void *mask, float *partial, float *dw, float *db,
+                            int n, int h, int w, int cin_pad, int cout_pad, int cin, int cout, void *stream);
+
+int isp_pilot_pack(const float *w, void *wf, void *wb, int cin, int cout, int cin_pad, int cout_pad, void *stream);
+
+/* Adam (bias-corrected) on n parameters at step >= 1 */
+int isp_pilot_adam(float *p, const float *g, float *m, float *v, long n, float lr, float beta1, float beta2,
+                   float eps, int step, void *stream);
+
+/* Network input [n][h][w][16]: channel 0 the noisy plane, channel 1 sigma[i]/255
+ * (sigma per image in 0..255 units), the rest 0. */
+int isp_pilot_input(const float *noisy, const float *sigma, void *x, int n, int h, int w, void *stream);
+
+/* out = noisy - residual (residual: channel 0 of res_f32 [n][h][w][16]).  With a
+ * target: *loss += mean((out - target)^2) and dres [n][h][w][16] (bf16) gets
+ * d(loss)/d(residual) in channel 0.  out may be NULL. */
+int isp_pilot_output(const float *noisy, const float *res_f32, float *out, const float *target, void *dres,
+                     float *loss, int n, int h, int w, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ISP_PILOT_H_ */

@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libisp_bm3d.so")
 
-SOURCES = ["bm3d_kernels.hip", "bm3d_capi.cpp"]
+SOURCES = ["bm3d_kernels.hip", "bm3d_capi.cpp", "pilot.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -309,3 +309,76 @@ def test_int_degenerate_copy_through():
                                                             floor=16, neutral=16, ceil=235), src)
     rgb = np.stack([src, src[::-1], src[:, ::-1]])
     np.testing.assert_array_equal(oracle.bm3d_rgb_int(p.basic, p.final, rgb, mode=1), rgb)
+
+
+# --- the learned pilot's FP64 oracle (oracle/pilot_oracle.py)
+def _pilot_params(rng, scale=1.0):
+    from oracle import pilot_oracle as po
+    out = []
+    for cin, cout, relu in po.LAYERS:
+        out.append((rng.standard_normal((cout, cin, 9)) * np.sqrt(2.0 / (cin * 9)) * scale,
+                    rng.standard_normal(cout) * 0.05))
+    return out
+
+
+def test_pilot_conv_matches_scipy_correlate2d():
+    from scipy.signal import correlate2d
+    from oracle import pilot_oracle as po
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((2, 7, 9, 3))
+    w = rng.standard_normal((4, 3, 9))
+    b = rng.standard_normal(4)
+    got = po.conv3x3(x, w, b)
+    for n in range(2):
+        for co in range(4):
+            want = b[co] + sum(correlate2d(x[n, :, :, ci], w[co, ci].reshape(3, 3), mode="same", boundary="fill")
+                               for ci in range(3))
+            np.testing.assert_allclose(got[n, :, :, co], want, atol=1e-12)
+
+
+def test_pilot_conv_adjoint_identity():
+    # <conv(x), y> == <x, dinput(y)>: pins the flipped, transposed kernel of the input gradient
+    from oracle import pilot_oracle as po
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal((1, 6, 5, 4))
+    y = rng.standard_normal((1, 6, 5, 3))
+    w = rng.standard_normal((3, 4, 9))
+    lhs = np.sum(po.conv3x3(x, w) * y)
+    rhs = np.sum(x * po.conv3x3_dinput(y, w))
+    assert abs(lhs - rhs) < 1e-9 * max(1.0, abs(lhs))
+
+
+def test_pilot_gradients_match_finite_differences():
+    from oracle import pilot_oracle as po
+    rng = np.random.default_rng(2)
+    params = _pilot_params(rng)
+    noisy = rng.uniform(0, 1, (1, 5, 6))
+    sigma = np.array([25.0])
+    target = rng.uniform(0, 1, (1, 5, 6))
+    _, grads = po.loss_and_grads(params, noisy, sigma, target)
+    h = 1e-6
+    for l in range(len(params)):
+        w, b = params[l]
+        for idx in [tuple(rng.integers(0, s) for s in w.shape) for _ in range(3)]:
+            wp = [(ww.copy(), bb.copy()) for ww, bb in params]
+            wm = [(ww.copy(), bb.copy()) for ww, bb in params]
+            wp[l][0][idx] += h
+            wm[l][0][idx] -= h
+            fd = (po.loss(wp, noisy, sigma, target) - po.loss(wm, noisy, sigma, target)) / (2 * h)
+            assert abs(fd - grads[l][0][idx]) <= 1e-6 * max(1.0, abs(fd)) + 1e-9, (l, idx, fd, grads[l][0][idx])
+        j = int(rng.integers(0, b.shape[0]))
+        bp = [(ww.copy(), bb.copy()) for ww, bb in params]
+        bm = [(ww.copy(), bb.copy()) for ww, bb in params]
+        bp[l][1][j] += h
+        bm[l][1][j] -= h
+        fd = (po.loss(bp, noisy, sigma, target) - po.loss(bm, noisy, sigma, target)) / (2 * h)
+        assert abs(fd - grads[l][1][j]) <= 1e-6 * max(1.0, abs(fd)) + 1e-9, (l, j, fd, grads[l][1][j])
+
+
+def test_bf16_round_known_values():
+    from oracle import pilot_oracle as po
+    v = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -9, -2.5, 1e-3, 0.0])
+    got = po.bf16_round(v)
+    # 1 + 2^-8 is a tie between 1 and 1 + 2^-7: to even (1); 1 + 1.5 * 2^-8 rounds up
+    np.testing.assert_array_equal(got[:4], [1.0, 1.0, 1.0 + 2 ** -7, -2.5])
+    assert abs(got[4] - 1e-3) <= 1e-3 * 2 ** -8 and got[5] == 0.0
