@@ -212,6 +212,210 @@ def bench_batch(args) -> int:
     return 0
 
 
+def bench_pilot(args) -> int:
+    """Learned pilot: train the residual CNN on 20k random 128x128 crops of
+    synthetic images (sigma in [10, 40]) against GPU step-1 targets (Adam,
+    batch 32), then denoise a 1920x1080 grey frame at sigma 25 with the pilot
+    as the Wiener guide, against plain two-step BM3D."""
+    import torch
+    from isp_mw_amd import pilot as P, synth
+
+    t0 = time.perf_counter()
+    noisy, sigma, target = P.make_training_set(20000, crop=128, seed=0)
+    torch.cuda.synchronize()
+    gen_s = time.perf_counter() - t0
+    net = P.PilotCNN(seed=0)
+    t0 = time.perf_counter()
+    losses = P.train(net, noisy, sigma, target, args.pilot_steps, batch=32, lr=1e-3, seed=0,
+                     log_every=max(1, args.pilot_steps // 10))
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    del noisy, target
+    clean, n = synth.grey_pair(H, W, SIGMA, SEED)
+    x = torch.from_numpy(n).cuda()
+    para = P.default_para(SIGMA)
+    plain = P.PilotBM3D(para, precision=args.precision)
+    piloted = P.PilotBM3D(para, net, precision=args.precision)
+
+    def timeit(fn, k):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(k):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / k
+
+    dt_pilot = timeit(lambda: piloted(x), args.steps)
+    dt_plain = timeit(lambda: plain(x), args.steps)
+    dt_guide_pilot = timeit(lambda: piloted.guide(x), args.steps)
+    dt_guide_step1 = timeit(lambda: plain.guide(x), args.steps)
+    out_p = piloted(x).cpu().numpy()
+    out_b = plain(x).cpu().numpy()
+    g_p = piloted.guide(x).cpu().numpy()
+    g_b = plain.guide(x).cpu().numpy()
+    p_noisy, p_plain, p_pilot = synth.psnr(n, clean), synth.psnr(out_b, clean), synth.psnr(out_p, clean)
+    # the network's flops: 2*9*cin*cout per pixel per layer
+    from isp_mw_amd.pilot import LAYERS
+    fl = 2.0 * 9 * sum(ci * co for ci, co, _, _, _ in LAYERS) * W * H
+    res = {
+        "metric": "Mpix/s learned-pilot + Wiener BM3D, 1080p grey sigma=25 (vs plain two-step)",
+        "value": round(W * H / dt_pilot / 1e6, 3),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 2,
+        "ms_per_step": round(dt_pilot * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16 network, " + ("fp32" if args.precision == "fp32" else f"{args.precision}+fp32") + " Wiener",
+        "data": "synthetic (seeded rectangles+gradient+gratings, AWGN)",
+        "config": {"workload": "pilot CNN (8 layers 3x3, 48 ch, ReLU, residual) as Wiener guide, 1920x1080 grey",
+                   "train_crops": 20000, "crop": 128, "batch": 32, "train_steps": args.pilot_steps,
+                   "sigma_train": [10, 40], "sigma": SIGMA, "bm3d_sigma_param": round(SIGMA * P.GREY_SIGMA_SCALE, 3),
+                   "precision": args.precision},
+        "plain_two_step": {"value": round(W * H / dt_plain / 1e6, 3), "ms": round(dt_plain * 1e3, 3)},
+        "guide_ms": {"pilot_cnn": round(dt_guide_pilot * 1e3, 3), "bm3d_step1": round(dt_guide_step1 * 1e3, 3)},
+        "pilot_cnn_tflops": round(fl / dt_guide_pilot / 1e12, 2),
+        "quality": {"psnr_noisy_db": round(p_noisy, 3), "psnr_plain_two_step_db": round(p_plain, 3),
+                    "psnr_pilot_db": round(p_pilot, 3), "gain_db": round(p_pilot - p_plain, 3),
+                    "psnr_guide_pilot_db": round(synth.psnr(g_p, clean), 3),
+                    "psnr_guide_step1_db": round(synth.psnr(g_b, clean), 3)},
+        "training": {"target_generation_s": round(gen_s, 2), "train_s": round(train_s, 2),
+                     "ms_per_step": round(train_s / args.pilot_steps * 1e3, 3),
+                     "loss_first": losses[0], "loss_last_mean": float(np.mean(losses[-50:]))},
+    }
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+def bench_full(args) -> int:
+    """Full form: 8 frames of 3840x2160 RGB (one seeded base image, frames
+    shifted by random 0..7 px), sigma 25, BMrange 19, GroupSize 32 in both
+    steps.  The learned pilot (trained as in the pilot workload) gives each
+    frame's guide, per RGB channel; the Wiener step runs on all 8 frames in one
+    batched graph launch with FP16 matrix-core transforms.  Compared with the
+    plain FP32 GPU two-step path on every frame."""
+    import torch
+    from isp_mw_amd import BM3D, BM3D_Final, BM3D_Para, BM3DBatch, pilot as P, synth
+
+    FW_, FH_, NF = 3840, 2160, 8
+    rng = np.random.default_rng(4)
+    base = synth.clean_rgb(FH_ + 8, FW_ + 8, 99)
+    frames, cleans = [], []
+    for k in range(NF):
+        dy, dx = rng.integers(0, 8, 2)
+        c = np.ascontiguousarray(base[:, dy:dy + FH_, dx:dx + FW_])
+        cleans.append(c)
+        frames.append((c + rng.standard_normal(c.shape) * (SIGMA / 255.0)).astype(np.float32))
+    del base
+    para = BM3D_Para("np")
+    for st in (para.basic, para.final):
+        st.BlockStep, st.BMrange, st.BMstep, st.GroupSize = 3, 19, 1, 32
+    para.basic.lambda_ = 2.7
+    para.set_sigma(SIGMA)
+    # the pilot
+    t0 = time.perf_counter()
+    noisy_t, sig_t, tgt_t = P.make_training_set(20000, crop=128, seed=0)
+    net = P.PilotCNN(seed=0)
+    P.train(net, noisy_t, sig_t, tgt_t, args.pilot_steps, batch=32, lr=1e-3, seed=0)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    del noisy_t, tgt_t
+    xs = [torch.from_numpy(f).cuda() for f in frames]
+    guides = [torch.empty_like(x) for x in xs]
+    outs = [torch.empty_like(x) for x in xs]
+    fin = BM3D_Final(para.final, precision="fp16")
+    batch = BM3DBatch(fin, outs, xs, refs=guides)
+
+    def guides_of_frames():
+        for x, g in zip(xs, guides):
+            g.copy_(P.rgb_guide(net, x, SIGMA))   # the network on the frame's OPP planes
+
+    def full_form():
+        guides_of_frames()
+        batch.run()
+
+    full_form()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        full_form()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        guides_of_frames()
+    torch.cuda.synchronize()
+    dt_pilot = (time.perf_counter() - t0) / args.steps
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.run()
+    torch.cuda.synchronize()
+    dt_wiener = (time.perf_counter() - t0) / args.steps
+    psnr_full = float(np.mean([synth.psnr(o.cpu().numpy(), c) for o, c in zip(outs, cleans)]))
+    # plain FP32 GPU two-step, frame by frame
+    plain = BM3D(para, precision="fp32")
+    ref_out = [torch.empty_like(x) for x in xs]
+    t0 = time.perf_counter()
+    for x, o in zip(xs, ref_out):
+        plain.process(o, x)
+    torch.cuda.synchronize()
+    dt_plain = time.perf_counter() - t0
+    psnr_plain = float(np.mean([synth.psnr(o.cpu().numpy(), c) for o, c in zip(ref_out, cleans)]))
+    # the FP16 two-step path without the pilot, batched, for reference
+    both16 = BM3D(para, precision="fp16")
+    out16 = [torch.empty_like(x) for x in xs]
+    b16 = BM3DBatch(both16, out16, xs)
+    b16.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b16.run()
+    torch.cuda.synchronize()
+    dt16 = time.perf_counter() - t0
+    psnr16 = float(np.mean([synth.psnr(o.cpu().numpy(), c) for o, c in zip(out16, cleans)]))
+    px = float(NF * FW_ * FH_)
+    # algorithmic work of the full form: the CNN on 3 planes per frame, and the
+    # Wiener step's 3 transforms x 3 planes at full N plus its matching
+    from isp_mw_amd.pilot import LAYERS
+    cnn = 2.0 * 9 * sum(ci * co for ci, co, _, _, _ in LAYERS) * 3 * px
+    gx, gy = grid(FW_, 3), grid(FH_, 3)
+    refs = gx.size * gy.size
+    cand = float(window_len(gx, FW_, 19).sum() * window_len(gy, FH_, 19).sum() - refs)
+    wiener = NF * (refs * 3 * 3 * transform_flops(32) + cand * B * B * 3)
+    achieved = (cnn + wiener) / dt / 1e12
+    res = {
+        "metric": "Mpix/s full form: 8 x 4K RGB, pilot + batched FP16 Wiener, R=19, G=32",
+        "value": round(px / dt / 1e6, 3),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 1,
+        "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16 network + fp16 Wiener transforms (fp32 accumulation)",
+        "data": "synthetic RGB base, 8 frames shifted by random 0..7 px, AWGN sigma 25/255",
+        "config": {"workload": "8 frames 3840x2160 RGB, pilot guide + one batched Wiener launch",
+                   "block_step": 3, "bm_range": 19, "group_size": [32, 32], "sigma": SIGMA},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_F16_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_F16_TFLOPS, 4), "traffic": None,
+                     "basis": "CNN flops + Wiener transforms at full N + matching, over the whole form"},
+        "stages_ms": {"pilot_cnn_8_frames": round(dt_pilot * 1e3, 2), "wiener_batch": round(dt_wiener * 1e3, 2)},
+        "quality": {"psnr_full_form_db": round(psnr_full, 3), "psnr_plain_fp32_two_step_db": round(psnr_plain, 3),
+                    "psnr_delta_vs_plain_fp32_db": round(psnr_full - psnr_plain, 3),
+                    "psnr_fp16_two_step_batch_db": round(psnr16, 3)},
+        "plain_fp32_two_step": {"value": round(px / dt_plain / 1e6, 3), "ms": round(dt_plain * 1e3, 1)},
+        "fp16_two_step_batch": {"value": round(px / dt16 / 1e6, 3), "ms": round(dt16 * 1e3, 1)},
+        "pilot_training_s": round(train_s, 1),
+    }
+    print(json.dumps(res), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -221,8 +425,11 @@ def main() -> int:
                     help="transform/shrinkage precision of the timed form (fp16: the metric's mixed form)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=4.0, help="target seconds per CPU baseline run")
-    ap.add_argument("--workload", default="rgb1080", choices=("rgb1080", "batch64"),
-                    help="rgb1080: the headline 1080p RGB frame; batch64: 64 uneven grey images as one batch")
+    ap.add_argument("--workload", default="rgb1080", choices=("rgb1080", "batch64", "pilot1080", "full4k"),
+                    help="rgb1080: the headline 1080p RGB frame; batch64: 64 uneven grey images as one batch; "
+                         "pilot1080: train the learned pilot, then pilot + Wiener on 1080p grey; "
+                         "full4k: 8 x 4K RGB frames, pilot guides + one batched FP16 Wiener launch")
+    ap.add_argument("--pilot-steps", type=int, default=1250, help="training steps of the pilot workload")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy (3 launches): the known-traffic "
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
@@ -243,6 +450,10 @@ def main() -> int:
         return 2
     if args.workload == "batch64":
         return bench_batch(args)
+    if args.workload == "pilot1080":
+        return bench_pilot(args)
+    if args.workload == "full4k":
+        return bench_full(args)
 
     import torch
     from isp_mw_amd import BM3D, synth

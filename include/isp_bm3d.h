@@ -159,6 +159,15 @@ int isp_bm3d_last_groups(isp_bm3d_ctx *ctx, int final_step, int *counts, int *po
 int isp_bm3d_grid_size(const isp_bm3d_ctx *ctx, int width, int height, int final_step,
                        int *ngx, int *ngy);
 
+/* OPP colour conversions of float device planes (MatrixConvert_RGB2YUV /
+ * MatrixConvert_YUV2RGB with ColorMatrix::OPP, include/Conversion.hpp:547-578,
+ * 660-762): Y=(R+G+B)/3, U=(R-B)/2, V=(R-2G+B)/4 without clip, and the inverse
+ * with clip to [0,1] when clip != 0.  Enqueued on `stream`, nothing synchronises. */
+int isp_bm3d_rgb_to_opp(float *y, float *u, float *v, const float *r, const float *g, const float *b, int width,
+                        int height, size_t stride, void *stream);
+int isp_bm3d_opp_to_rgb(float *r, float *g, float *b, const float *y, const float *u, const float *v, int width,
+                        int height, size_t stride, int clip, void *stream);
+
 /* ---------------------------------------------------------------------------
  * Batches: many images of any sizes filtered together, one launch per stage
  * for the whole batch, replayed as one hipGraph.  No reference counterpart
@@ -169,7 +178,8 @@ int isp_bm3d_grid_size(const isp_bm3d_ctx *ctx, int width, int height, int final
 typedef struct isp_bm3d_image {
     float *dst[3];          /* device planes: [0] for grey (nplanes 1), [0..2] R, G, B */
     const float *src[3];
-    const float *ref[3];    /* step-1 matching guide (grey) / guide frame (RGB), or NULL: src */
+    const float *ref[3];    /* or NULL (= src): BASIC and BOTH, the step-1 matching guide (its luma
+                               for RGB); FINAL, the Wiener guide of BM3D_Final::process(dst, src, ref) */
     int width, height;
     size_t stride;          /* row pitch of dst, src and ref in elements */
     double sigma;           /* > 0: this image's sigma (0..255 units) for every plane of both
@@ -182,7 +192,8 @@ typedef struct isp_bm3d_batch isp_bm3d_batch;
 #define ISP_BM3D_BATCH_AUTO_GROUP 1  /* the final step's GroupSize of each image from the
                                         tuning table of its size class, when one is set */
 
-/* mode: ISP_BM3D_BASIC or ISP_BM3D_BOTH.  Buffers are device memory; the plan
+/* mode: ISP_BM3D_BASIC, ISP_BM3D_FINAL (ref: the guide frames) or ISP_BM3D_BOTH.
+ * Buffers are device memory; the plan
  * keeps the pointers (change them with isp_bm3d_batch_update). */
 int isp_bm3d_batch_create(isp_bm3d_ctx *ctx, isp_bm3d_batch **batch, const isp_bm3d_image *images, int count,
                           int nplanes, int mode, int flags);

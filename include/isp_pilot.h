@@ -56,9 +56,12 @@ int isp_pilot_pack(const float *w, void *wf, void *wb, int cin, int cout, int ci
 int isp_pilot_adam(float *p, const float *g, float *m, float *v, long n, float lr, float beta1, float beta2,
                    float eps, int step, void *stream);
 
-/* Network input [n][h][w][16]: channel 0 the noisy plane, channel 1 sigma[i]/255
- * (sigma per image in 0..255 units), the rest 0. */
-int isp_pilot_input(const float *noisy, const float *sigma, void *x, int n, int h, int w, void *stream);
+/* Network input [n][h][w][16]: channel 0 the noisy plane plus offset[i] (offset
+ * may be NULL: 0; chroma planes centred on 0 pass 0.5), channel 1 sigma[i]/255
+ * (sigma per plane in 0..255 units), the rest 0.  The residual output makes
+ * the offset cancel: out = noisy - r. */
+int isp_pilot_input(const float *noisy, const float *sigma, const float *offset, void *x, int n, int h, int w,
+                    void *stream);
 
 /* out = noisy - residual (residual: channel 0 of res_f32 [n][h][w][16]).  With a
  * target: *loss += mean((out - target)^2) and dres [n][h][w][16] (bf16) gets

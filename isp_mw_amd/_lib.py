@@ -36,6 +36,8 @@ EXPORTS = (
     "isp_bm3d_get_timing",
     "isp_bm3d_reset_timing",
     "isp_bm3d_last_groups",
+    "isp_bm3d_rgb_to_opp",
+    "isp_bm3d_opp_to_rgb",
     "isp_bm3d_batch_create",
     "isp_bm3d_batch_run",
     "isp_bm3d_batch_update",
@@ -145,6 +147,8 @@ def lib() -> ctypes.CDLL:
                                       ctypes.POINTER(ctypes.c_long), i]
     L.isp_bm3d_reset_timing.argtypes = [vp]
     L.isp_bm3d_last_groups.argtypes = [vp, i, vp, vp, sz]
+    L.isp_bm3d_rgb_to_opp.argtypes = [vp] * 6 + [i, i, sz, vp]
+    L.isp_bm3d_opp_to_rgb.argtypes = [vp] * 6 + [i, i, sz, i, vp]
     L.isp_bm3d_batch_create.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(Image), i, i, i, i]
     L.isp_bm3d_batch_run.argtypes = [vp, vp, i]
     L.isp_bm3d_batch_update.argtypes = [vp, ctypes.POINTER(Image)]

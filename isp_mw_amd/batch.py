@@ -8,7 +8,7 @@ from __future__ import annotations
 import ctypes
 
 from . import _lib as L
-from .bm3d import BM3D, BM3D_Basic
+from .bm3d import _Filter
 
 
 class BM3DBatch:
@@ -16,9 +16,11 @@ class BM3DBatch:
     or [3, H, W] (RGB in [0, 1]), contiguous.  `sigmas[k]` > 0 gives image k
     its own sigma (0..255 units, both steps, thMSE from thMSE_Default)."""
 
-    def __init__(self, filt: BM3D | BM3D_Basic, dsts, srcs, sigmas=None, refs=None, auto_group: bool = False):
-        if not isinstance(filt, (BM3D, BM3D_Basic)):
-            raise TypeError("a batch runs BM3D (both steps) or BM3D_Basic")
+    def __init__(self, filt: _Filter, dsts, srcs, sigmas=None, refs=None, auto_group: bool = False):
+        """filt: BM3D (both steps), BM3D_Basic or BM3D_Final (refs are then the
+        Wiener guides, as BM3D_Final::process(dst, src, ref))."""
+        if not isinstance(filt, _Filter):
+            raise TypeError("a batch runs a BM3D, BM3D_Basic or BM3D_Final filter")
         self._filt = filt
         self._flags = L.BATCH_AUTO_GROUP if auto_group else 0
         self._plan = ctypes.c_void_p()

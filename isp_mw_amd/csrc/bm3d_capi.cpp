@@ -705,6 +705,7 @@ struct BatchImg {
     Derived d[2];
     isp_bm3d_ctx::Tiles tiles;
     float *yuv[3];          // RGB: OPP planes
+    float *refyuv[3];       // RGB, final step alone: OPP planes of the guide frame
     double *y64;            // RGB: FP64 luma guide (of ref when given)
     float *basic[3];
     unsigned long long *num[3], *den[3];
@@ -758,6 +759,18 @@ struct isp_bm3d_batch {
 
 namespace {
 
+// The steps a batch runs, as kinds (0 basic, 1 final), in order.
+int batch_kinds(const isp_bm3d_batch *b, int kinds[2])
+{
+    if (b->mode == ISP_BM3D_BOTH) {
+        kinds[0] = 0;
+        kinds[1] = 1;
+        return 2;
+    }
+    kinds[0] = b->mode == ISP_BM3D_FINAL ? 1 : 0;
+    return 1;
+}
+
 void batch_free(isp_bm3d_batch *b)
 {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
@@ -806,12 +819,14 @@ int batch_params(isp_bm3d_batch *b, int k)
     return ISP_BM3D_OK;
 }
 
-// Builds the per-image launch arguments of both steps (pointers included).
+// Builds the per-image launch arguments of every step (pointers included).
 int batch_args(isp_bm3d_batch *b)
 {
     isp_bm3d_ctx *ctx = b->ctx;
     const bool mfma = ctx->precision != ISP_BM3D_PREC_FP32;
-    const int nsteps = b->mode == ISP_BM3D_BOTH ? 2 : 1;
+    int kinds[2];
+    const int nst = batch_kinds(b, kinds);
+    const bool final_only = b->mode == ISP_BM3D_FINAL;
     b->o.clear();
     b->oref.clear();
     for (int s = 0; s < 2; ++s) {
@@ -826,6 +841,9 @@ int batch_args(isp_bm3d_batch *b)
         const bool rgb = b->nplanes == 3;
         const bool sep = im.ref[0] && (im.ref[0] != im.src[0]);
         const float *src[3] = {nullptr, nullptr, nullptr};
+        // guide planes of a final step run alone: BM3D_Final::RGB2YUV converts
+        // the whole guide frame (source/BM3D.cpp:392-408); without one, src
+        const float *fguide[3] = {nullptr, nullptr, nullptr};
         if (rgb) {
             OppArgs q{};
             q.W = x.W;
@@ -847,29 +865,43 @@ int batch_args(isp_bm3d_batch *b)
             q.y64 = sep ? nullptr : x.y64;
             q.pitch64 = x.P;
             b->o.push_back(q);
-            // BM3D_Basic::RGB2YUV: ConvertToY of the guide (source/BM3D.cpp:320-336)
+            // the guide frame: ConvertToY for step 1 (BM3D_Basic::RGB2YUV,
+            // source/BM3D.cpp:320-336), the whole OPP for a final step alone
             OppArgs r = q;
             r.r = sep ? im.ref[0] : im.src[0];
             r.g = sep ? im.ref[1] : im.src[1];
             r.b = sep ? im.ref[2] : im.src[2];
             r.y = r.u = r.v = nullptr;
+            if (final_only && sep) {
+                r.y = x.refyuv[0];
+                r.u = x.refyuv[1];
+                r.v = x.refyuv[2];
+            }
             r.y64 = sep ? x.y64 : nullptr;
             b->oref.push_back(r);
-            for (int c = 0; c < 3; ++c) src[c] = x.yuv[c];
+            for (int c = 0; c < 3; ++c) {
+                src[c] = x.yuv[c];
+                fguide[c] = sep && final_only ? x.refyuv[c] : x.yuv[c];
+            }
         } else {
             src[0] = im.src[0];
+            fguide[0] = sep ? im.ref[0] : im.src[0];
         }
-        for (int s = 0; s < nsteps; ++s) {
-            BatchStepPlan &sp = b->st[s];
-            const bool fin = s == 1;
+        for (int j = 0; j < nst; ++j) {
+            const int kind = kinds[j];
+            BatchStepPlan &sp = b->st[kind];
+            const bool fin = kind == 1;
             const isp_bm3d_step_params &p = fin ? x.prm.final_ : x.prm.basic;
-            const Derived &d = x.d[s];
-            // step input / guides / output (source/BM3D.cpp:501-527, 464-472)
+            const Derived &d = x.d[kind];
+            // step input / guides / output (source/BM3D.cpp:501-527, 464-472, 392-408)
             const float *guide[3] = {src[0], src[1], src[2]};
             const void *mg;
-            if (fin) {
+            if (fin && !final_only) {
                 for (int c = 0; c < 3; ++c) guide[c] = x.basic[c];
                 mg = x.basic[0];
+            } else if (fin) {
+                for (int c = 0; c < 3; ++c) guide[c] = fguide[c];
+                mg = rgb ? (const void *)x.y64 : (const void *)fguide[0];
             } else if (rgb) {
                 mg = x.y64;
             } else {
@@ -879,9 +911,9 @@ int batch_args(isp_bm3d_batch *b)
             io.nplanes = b->nplanes;
             io.pitch = x.P;
             for (int c = 0; c < 3; ++c) io.src[c] = src[c];
-            const bool last = s == nsteps - 1;
+            const bool last = j == nst - 1;
+            io.out_pitch = (size_t)x.P;
             if (last) {
-                io.out_pitch = (size_t)x.P;
                 for (int c = 0; c < 3; ++c) io.out[c] = im.dst[c];
                 if (rgb) {
                     // MatrixConvert_YUV2RGB OPP with clip (include/Conversion.hpp:660-762)
@@ -893,7 +925,6 @@ int batch_args(isp_bm3d_batch *b)
                     io.hi = 1.0;
                 }
             } else {
-                io.out_pitch = (size_t)x.P;
                 for (int c = 0; c < 3; ++c) io.out[c] = x.basic[c];
             }
             sp.fin.push_back(build_finalize(x.W, x.H, io, d, x.num, x.den));
@@ -903,12 +934,12 @@ int batch_args(isp_bm3d_batch *b)
                 continue;
             }
             const MatchArgs ma = build_match(ctx, p, d, x.W, x.H, mg, x.P, x.tiles.match_x, x.tiles.match_y,
-                                             x.cnt[s], x.pos[s], nullptr);
+                                             x.cnt[kind], x.pos[kind], nullptr);
             int rc = check_match_lds(ctx, p, ma);
             if (rc) return rc;
             sp.m.push_back(ma);
             const FilterArgs fa =
-                build_filter(ctx, p, d, x.W, x.H, x.P, src, guide, x.num, x.den, x.cnt[s], x.pos[s],
+                build_filter(ctx, p, d, x.W, x.H, x.P, src, guide, x.num, x.den, x.cnt[kind], x.pos[kind],
                              mfma ? x.tiles.mfma_x : x.tiles.filter_x, mfma ? x.tiles.mfma_y : x.tiles.filter_y);
             if (filter_lds(ctx, fa, fin) > 160 * 1024)
                 return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "image %d needs %zu bytes of LDS for filtering", k,
@@ -923,8 +954,10 @@ int batch_args(isp_bm3d_batch *b)
 int batch_upload(isp_bm3d_batch *b)
 {
     isp_bm3d_ctx *ctx = b->ctx;
-    const int nsteps = b->mode == ISP_BM3D_BOTH ? 2 : 1;
-    for (int s = 0; s < nsteps; ++s) {
+    int kinds[2];
+    const int nst = batch_kinds(b, kinds);
+    for (int j = 0; j < nst; ++j) {
+        const int s = kinds[j];
         BatchStepPlan &sp = b->st[s];
         sp.mt.clear();
         sp.ft.clear();
@@ -965,8 +998,8 @@ int batch_upload(isp_bm3d_batch *b)
         return ISP_BM3D_OK;
     };
     int rc;
-    for (int s = 0; s < nsteps; ++s) {
-        BatchStepPlan &sp = b->st[s];
+    for (int j = 0; j < nst; ++j) {
+        BatchStepPlan &sp = b->st[kinds[j]];
         if ((rc = put(sp.dm, sp.m.data(), sp.m.size() * sizeof(MatchArgs)))) return rc;
         if ((rc = put(sp.df, sp.f.data(), sp.f.size() * sizeof(FilterArgs)))) return rc;
         if ((rc = put(sp.dfin, sp.fin.data(), sp.fin.size() * sizeof(FinalizeArgs)))) return rc;
@@ -999,14 +1032,17 @@ int batch_enqueue(isp_bm3d_batch *b, hipStream_t s)
             ++n;
         }
     }
-    const int nsteps = b->mode == ISP_BM3D_BOTH ? 2 : 1;
-    for (int st = 0; st < nsteps; ++st) {
-        BatchStepPlan &sp = b->st[st];
-        const bool fin = st == 1;
+    int kinds[2];
+    const int nst = batch_kinds(b, kinds);
+    for (int j = 0; j < nst; ++j) {
+        BatchStepPlan &sp = b->st[kinds[j]];
+        const bool fin = kinds[j] == 1;
+        // FP64 guides: the RGB luma of step 1, and of the guide frame of a final step alone
+        const bool g64 = b->nplanes == 3 && (!fin || b->mode == ISP_BM3D_FINAL);
         if (sp.any) {
             HIP_CHECK(ctx, launch_fill(reinterpret_cast<float *>(b->acc), b->acc_words * 2, 0.f, s));
             BatchLaunch bm{sp.dmt, sp.dm, (unsigned)sp.mt.size(), 0};
-            HIP_CHECK(ctx, launch_match(sp.hm, b->nplanes == 3 && !fin, s, &bm));
+            HIP_CHECK(ctx, launch_match(sp.hm, g64, s, &bm));
             BatchLaunch bf{sp.dft, sp.df, (unsigned)sp.ft.size(), 0};
             HIP_CHECK(ctx, launch_filter_any(ctx, sp.hf, fin, s, &bf));
             n += 3;
@@ -1024,7 +1060,10 @@ int batch_enqueue(isp_bm3d_batch *b, hipStream_t s)
 size_t batch_layout(isp_bm3d_batch *b, void *base)
 {
     Carver c(base);
-    const int nsteps = b->mode == ISP_BM3D_BOTH ? 2 : 1;
+    int kinds[2];
+    const int nst = batch_kinds(b, kinds);
+    bool used_kind[2] = {false, false};
+    for (int j = 0; j < nst; ++j) used_kind[kinds[j]] = true;
     size_t acc0 = 0;
     for (int k = 0; k < b->count; ++k) {
         BatchImg &x = b->bi[k];
@@ -1041,14 +1080,16 @@ size_t batch_layout(isp_bm3d_batch *b, void *base)
     for (int k = 0; k < b->count; ++k) {
         BatchImg &x = b->bi[k];
         const size_t np = (size_t)x.P * x.H;
+        const bool sep = b->img[k].ref[0] && b->img[k].ref[0] != b->img[k].src[0];
         for (int pl = 0; pl < 3; ++pl) {
             x.yuv[pl] = b->nplanes == 3 ? c.take<float>(np) : nullptr;
-            x.basic[pl] = (pl < b->nplanes && nsteps == 2) ? c.take<float>(np) : nullptr;
+            x.refyuv[pl] = (b->nplanes == 3 && b->mode == ISP_BM3D_FINAL && sep) ? c.take<float>(np) : nullptr;
+            x.basic[pl] = (pl < b->nplanes && b->mode == ISP_BM3D_BOTH) ? c.take<float>(np) : nullptr;
         }
         x.y64 = b->nplanes == 3 ? c.take<double>(np) : nullptr;
         for (int s = 0; s < 2; ++s) {
             const Derived &d = x.d[s];
-            const bool used = s < nsteps && !d.degenerate;
+            const bool used = used_kind[s] && !d.degenerate;
             const size_t refs = used ? (size_t)d.ngx * d.ngy : 0;
             const int G = s ? x.prm.final_.group_size : x.prm.basic.group_size;
             x.cnt[s] = used ? c.take<int>(refs) : nullptr;
@@ -1062,7 +1103,8 @@ size_t batch_layout(isp_bm3d_batch *b, void *base)
             }
         }
     }
-    for (int s = 0; s < nsteps; ++s) {
+    for (int j = 0; j < nst; ++j) {
+        const int s = kinds[j];
         BatchStepPlan &sp = b->st[s];
         sp.dm = c.take<MatchArgs>(b->count);
         sp.df = c.take<FilterArgs>(b->count);
@@ -1142,8 +1184,7 @@ int batch_make(isp_bm3d_ctx *ctx, isp_bm3d_batch **out, const isp_bm3d_image *im
     if (count < 1 || count > kMaxBatchImages)
         return fail(ctx, ISP_BM3D_E_ARG, "batch of %d images (1..%d)", count, kMaxBatchImages);
     if (nplanes != 1 && nplanes != 3) return fail(ctx, ISP_BM3D_E_ARG, "nplanes %d (1 or 3)", nplanes);
-    if (mode != ISP_BM3D_BASIC && mode != ISP_BM3D_BOTH)
-        return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "batches run ISP_BM3D_BASIC or ISP_BM3D_BOTH");
+    if (mode < ISP_BM3D_BASIC || mode > ISP_BM3D_BOTH) return fail(ctx, ISP_BM3D_E_ARG, "bad mode %d", mode);
     HIP_CHECK(ctx, hipSetDevice(ctx->device));
     isp_bm3d_batch *b = new isp_bm3d_batch;
     b->ctx = ctx;
@@ -1859,6 +1900,37 @@ int isp_bm3d_match(isp_bm3d_ctx *ctx, const float *guide, int W, int H, size_t s
     return ISP_BM3D_OK;
 }
 
+
+int isp_bm3d_rgb_to_opp(float *y, float *u, float *v, const float *r, const float *g, const float *b, int W, int H,
+                        size_t stride, void *stream)
+{
+    if (!y || !u || !v || !r || !g || !b || W <= 0 || H <= 0 || stride < (size_t)W) return ISP_BM3D_E_ARG;
+    OppArgs o{};
+    o.W = W;
+    o.H = H;
+    o.r = r;
+    o.g = g;
+    o.b = b;
+    o.in_int = 0;
+    o.gainY = 1.0 / 3.0;
+    o.gainU = 1.0 / 2.0;
+    o.gainV = 1.0 / 4.0;
+    o.in_pitch = stride;
+    o.y = y;
+    o.u = u;
+    o.v = v;
+    o.pitch = (int)stride;
+    return launch_opp(o, (hipStream_t)stream) == hipSuccess ? ISP_BM3D_OK : ISP_BM3D_E_HIP;
+}
+
+int isp_bm3d_opp_to_rgb(float *r, float *g, float *b, const float *y, const float *u, const float *v, int W, int H,
+                        size_t stride, int clip, void *stream)
+{
+    if (!y || !u || !v || !r || !g || !b || W <= 0 || H <= 0 || stride < (size_t)W) return ISP_BM3D_E_ARG;
+    return launch_opp_inverse(r, g, b, y, u, v, W, H, stride, clip, (hipStream_t)stream) == hipSuccess
+               ? ISP_BM3D_OK
+               : ISP_BM3D_E_HIP;
+}
 
 int isp_bm3d_batch_create(isp_bm3d_ctx *ctx, isp_bm3d_batch **batch, const isp_bm3d_image *images, int count,
                           int nplanes, int mode, int flags)

@@ -163,6 +163,8 @@ size_t filter_mfma_lds_bytes(const FilterArgs &a, bool wiener);
 hipError_t launch_finalize(const FinalizeArgs &a, hipStream_t s, const BatchLaunch *b = nullptr);
 hipError_t launch_opp(const OppArgs &a, hipStream_t s, const BatchLaunch *b = nullptr);
 hipError_t launch_to_float(const ToFloatArgs &a, hipStream_t s);
+hipError_t launch_opp_inverse(float *r, float *g, float *b, const float *y, const float *u, const float *v, int W,
+                              int H, size_t pitch, int clip, hipStream_t s);
 hipError_t launch_fill(float *p, size_t n, float v, hipStream_t s);
 hipError_t launch_copy2d(float *dst, size_t dpitch, const float *src, size_t spitch, int W, int H,
                          hipStream_t s);

@@ -1038,6 +1038,27 @@ __global__ void opp_kernel(OppArgs a0, const uint32_t *__restrict__ bmap, const 
     if (a.v) a.v[o] = (float)((R - G * 2.0 + B) * a.gainV + a.offsetC);
 }
 
+// MatrixConvert_YUV2RGB with the OPP matrix on float planes
+// (include/Conversion.hpp:660-762), optional clip to [0, 1].
+__global__ void opp_inverse_kernel(float *r, float *g, float *b, const float *y, const float *u, const float *v,
+                                   int W, size_t pitch, int clip)
+{
+    // separate mul and add roundings, as the reference's double code
+#pragma clang fp contract(off)
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= W) return;
+    const size_t o = (size_t)blockIdx.y * pitch + x;
+    const double Y = y[o], U = u[o], V = v[o];
+    double res[3] = {Y + U + (double)(2.0L / 3.0L) * V, Y + (double)(-4.0L / 3.0L) * V,
+                     Y - U + (double)(2.0L / 3.0L) * V};
+    float *out[3] = {r, g, b};
+    for (int c = 0; c < 3; ++c) {
+        double t = res[c];
+        if (clip) t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+        out[c][o] = (float)t;
+    }
+}
+
 __global__ void to_float_kernel(ToFloatArgs a)
 {
     // separate mul and add roundings, as the reference's double code
@@ -1168,6 +1189,14 @@ hipError_t launch_finalize(const FinalizeArgs &a, hipStream_t s, const BatchLaun
 hipError_t launch_opp(const OppArgs &a, hipStream_t s, const BatchLaunch *b)
 {
     return launch_rows(opp_kernel<false>, opp_kernel<true>, a, s, b);
+}
+
+hipError_t launch_opp_inverse(float *r, float *g, float *b, const float *y, const float *u, const float *v, int W,
+                              int H, size_t pitch, int clip, hipStream_t s)
+{
+    hipLaunchKernelGGL(opp_inverse_kernel, dim3((W + 255) / 256, H), dim3(256), 0, s, r, g, b, y, u, v, W, pitch,
+                       clip);
+    return hipGetLastError();
 }
 
 hipError_t launch_to_float(const ToFloatArgs &a, hipStream_t s)

@@ -25,9 +25,14 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int TH = 16, TW = 32;          // output tile of a convolution workgroup
+// output tile of a convolution workgroup: 8 x 32 pixels keeps the 48-channel
+// kernel's LDS (81,856 bytes with padding) within half of the 160 KB of a CU,
+// so two workgroups overlap their staging with each other's products
+constexpr int TH = 8, TW = 32;
+constexpr int RPW = TH / 4;              // rows per wave
+constexpr int MT = RPW * (TW / 16);      // 16-pixel M-tiles per wave
 constexpr int SH = TH + 2, SW = TW + 2;  // staged input tile with halo
-constexpr int THREADS = 256;             // 4 waves x 128 pixels
+constexpr int THREADS = 256;             // 4 waves x RPW rows
 constexpr int GTH = 8, GTW = 32;         // pixel tile of the weight-gradient kernel
 
 __host__ __device__ constexpr int kpad(int cin) { return (9 * cin + 31) / 32 * 32; }
@@ -57,7 +62,8 @@ __device__ __forceinline__ uint4 apply_mask(uint4 v, uint4 m)
 }
 
 // One output tile of TH x TW pixels and all COUT channels per workgroup; wave
-// w computes rows 4w..4w+3 (8 M-tiles of 16 pixels) against COUT/16 N-tiles.
+// w computes rows RPW*w .. RPW*w+RPW-1 (MT M-tiles of 16 pixels) against
+// COUT/16 N-tiles.
 // LDS rows are padded by 8 elements so that the 16-byte operand reads of 8
 // neighbouring lanes fall in distinct banks.
 template <int CIN, int COUT>
@@ -94,9 +100,9 @@ __global__ __launch_bounds__(THREADS) void conv3x3_kernel(const uint16_t *__rest
     __syncthreads();
 
     const int m = lane & 15, g = lane >> 4;
-    f32x4 acc[8][NT];
+    f32x4 acc[MT][NT];
 #pragma unroll
-    for (int a = 0; a < 8; ++a)
+    for (int a = 0; a < MT; ++a)
 #pragma unroll
         for (int b = 0; b < NT; ++b) acc[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
 
@@ -111,8 +117,8 @@ __global__ __launch_bounds__(THREADS) void conv3x3_kernel(const uint16_t *__rest
         for (int nt = 0; nt < NT; ++nt)
             b[nt] = *reinterpret_cast<const bf16x8 *>(tw + (size_t)(16 * nt + m) * KS + k);
 #pragma unroll
-        for (int mt = 0; mt < 8; ++mt) {
-            const int r = 4 * wave + (mt >> 1), xs = (mt & 1) * 16 + m;
+        for (int mt = 0; mt < MT; ++mt) {
+            const int r = RPW * wave + (mt >> 1), xs = (mt & 1) * 16 + m;
             const bf16x8 a = *reinterpret_cast<const bf16x8 *>(tin + ((size_t)(r + ty) * SW + xs + tx) * SP + ci0);
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt)
@@ -121,8 +127,8 @@ __global__ __launch_bounds__(THREADS) void conv3x3_kernel(const uint16_t *__rest
     }
     // D layout: lane holds rows 4g..4g+3 (pixels) of column lane % 16 (channel)
 #pragma unroll
-    for (int mt = 0; mt < 8; ++mt) {
-        const int r = 4 * wave + (mt >> 1), yy = y0 + r;
+    for (int mt = 0; mt < MT; ++mt) {
+        const int r = RPW * wave + (mt >> 1), yy = y0 + r;
         if (yy >= H) continue;
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
@@ -272,15 +278,17 @@ __global__ void adam_kernel(float *__restrict__ p, const float *__restrict__ g, 
     }
 }
 
-// network input: channel 0 the noisy sample, channel 1 sigma/255, 2..15 zero
+// network input: channel 0 the noisy sample plus the plane's offset, channel 1
+// sigma/255, 2..15 zero
 __global__ void input_kernel(const float *__restrict__ noisy, const float *__restrict__ sigma,
-                             uint16_t *__restrict__ x, int N, int H, int W)
+                             const float *__restrict__ offset, uint16_t *__restrict__ x, int N, int H, int W)
 {
     const long total = (long)N * H * W;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
         const int n = (int)(i / ((long)H * W));
         uint4 *d = reinterpret_cast<uint4 *>(x + (size_t)i * 16);
-        const uint32_t c01 = (uint32_t)f2bf(noisy[i]) | ((uint32_t)f2bf(sigma[n] / 255.f) << 16);
+        const float v = noisy[i] + (offset ? offset[n] : 0.f);
+        const uint32_t c01 = (uint32_t)f2bf(v) | ((uint32_t)f2bf(sigma[n] / 255.f) << 16);
         d[0] = make_uint4(c01, 0, 0, 0);
         d[1] = make_uint4(0, 0, 0, 0);
     }
@@ -430,12 +438,13 @@ int isp_pilot_adam(float *p, const float *g, float *m, float *v, long n, float l
     return hipGetLastError() == hipSuccess ? ISP_PILOT_OK : ISP_PILOT_E_HIP;
 }
 
-int isp_pilot_input(const float *noisy, const float *sigma, void *x, int n, int h, int w, void *stream)
+int isp_pilot_input(const float *noisy, const float *sigma, const float *offset, void *x, int n, int h, int w,
+                    void *stream)
 {
     if (!noisy || !sigma || !x || n <= 0 || h <= 0 || w <= 0) return ISP_PILOT_E_ARG;
     const long total = (long)n * h * w;
     const long blocks = (total + 255) / 256 < 4096 ? (total + 255) / 256 : 4096;
-    hipLaunchKernelGGL(input_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, noisy, sigma,
+    hipLaunchKernelGGL(input_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, noisy, sigma, offset,
                        (uint16_t *)x, n, h, w);
     return hipGetLastError() == hipSuccess ? ISP_PILOT_OK : ISP_PILOT_E_HIP;
 }

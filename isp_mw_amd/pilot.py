@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import sys
 
 import numpy as np
 
@@ -41,7 +42,7 @@ def _plib():
         lib.isp_pilot_conv3x3_wgrad.argtypes = [vp, vp, vp, vp, vp, vp, i, i, i, i, i, i, i, vp]
         lib.isp_pilot_pack.argtypes = [vp, vp, vp, i, i, i, i, vp]
         lib.isp_pilot_adam.argtypes = [vp, vp, vp, vp, ctypes.c_long, f, f, f, f, i, vp]
-        lib.isp_pilot_input.argtypes = [vp, vp, vp, i, i, i, vp]
+        lib.isp_pilot_input.argtypes = [vp, vp, vp, vp, i, i, i, vp]
         lib.isp_pilot_output.argtypes = [vp, vp, vp, vp, vp, vp, i, i, i, vp]
         lib._pilot_typed = True
     return lib
@@ -123,16 +124,17 @@ class PilotCNN:
             self._bufs[name] = t
         return t[:numel]
 
-    def forward(self, noisy, sigma, keep: bool = True):
-        """noisy [N,H,W] fp32 device, sigma [N] fp32 device (0..255).
-        Returns (acts, res32): acts[0] the input, acts[l+1] layer l's output."""
+    def forward(self, noisy, sigma, keep: bool = True, offset=None):
+        """noisy [N,H,W] fp32 device, sigma [N] fp32 device (0..255), offset
+        [N] fp32 device or None.  Returns (acts, res32): acts[0] the input,
+        acts[l+1] layer l's output."""
         torch = self.torch
         lib = _plib()
         s = self._stream()
         N, H, W = noisy.shape
         P = N * H * W
         x = self._buf("act0", P * 16, torch.int16)
-        _check(lib.isp_pilot_input(_ptr(noisy), _ptr(sigma), _ptr(x), N, H, W, s), "input")
+        _check(lib.isp_pilot_input(_ptr(noisy), _ptr(sigma), _ptr(offset), _ptr(x), N, H, W, s), "input")
         acts = [x]
         res32 = self._buf("res32", P * 16, torch.float32)
         cur = x
@@ -180,13 +182,17 @@ class PilotCNN:
                                       self.params.numel(), lr, beta1, beta2, eps, self.step, self._stream()), "adam")
         self.pack()
 
-    def infer(self, noisy, sigma: float):
-        """noisy [H,W] or [N,H,W] fp32 device -> the estimate (same shape)."""
+    def infer(self, noisy, sigma, offset=None):
+        """noisy [H,W] or [N,H,W] fp32 device -> the estimate (same shape).
+        sigma: one value or one per plane (0..255 units); offset: None or one
+        value per plane added to the network input (0.5 for chroma centred on 0)."""
         torch = self.torch
         x = noisy if noisy.dim() == 3 else noisy[None]
         N, H, W = x.shape
-        sg = torch.full((N,), float(sigma), device=self.dev)
-        _, res32 = self.forward(x.contiguous(), sg, keep=False)
+        sg = torch.tensor(np.broadcast_to(np.asarray(sigma, np.float32), (N,)).copy(), device=self.dev)
+        off = None if offset is None else torch.tensor(np.broadcast_to(np.asarray(offset, np.float32), (N,)).copy(),
+                                                       device=self.dev)
+        _, res32 = self.forward(x.contiguous(), sg, keep=False, offset=off)
         out = torch.empty_like(x)
         _check(_plib().isp_pilot_output(_ptr(x), _ptr(res32), _ptr(out), None, None, None, N, H, W,
                                         self._stream()), "output")
@@ -275,23 +281,54 @@ def train(net: PilotCNN, noisy, sigma, target, steps: int, batch: int = 32, lr: 
         net.adam(lr_t)
         losses.append(loss)
         if log_every and (it + 1) % log_every == 0:
-            print(f"pilot step {it + 1}/{steps} loss {float(loss.item()):.3e}", flush=True)
+            print(f"pilot step {it + 1}/{steps} loss {float(loss.item()):.3e}", file=sys.stderr, flush=True)
     return [float(v.item()) for v in losses]
 
 
-class PilotBM3D:
-    """Two-step BM3D on a grey plane with the pilot's estimate as the Wiener
-    guide.  Without a network it runs plain step 1 (the fallback)."""
+# OPP norms of Y, U, V (include/BM3D.h:241-248): the noise of an OPP plane is
+# sigma * norm for RGB noise sigma
+OPP_NORMS = (1.0 / math.sqrt(3.0), 1.0 / math.sqrt(2.0), math.sqrt(6.0) / 4.0)
 
-    def __init__(self, para: BM3D_Para, net: PilotCNN | None = None, device: int = 0, precision: str = "fp32"):
+
+def rgb_guide(net: PilotCNN, rgb, sigma: float):
+    """The pilot's estimate of an RGB frame [3,H,W] (device fp32, noise sigma
+    per channel in 0..255 units): the network runs on the OPP planes (Y, U, V
+    with their own noise levels, chroma offset by 0.5), converted back to RGB
+    without clip, so that BM3D_Final's own OPP conversion recovers them."""
+    import torch
+    lib = L.lib()
+    _, H, W = rgb.shape
+    s = ctypes.c_void_p(torch.cuda.current_stream(rgb.device).cuda_stream)
+    yuv = torch.empty_like(rgb)
+    _check(lib.isp_bm3d_rgb_to_opp(_ptr(yuv[0]), _ptr(yuv[1]), _ptr(yuv[2]), _ptr(rgb[0]), _ptr(rgb[1]), _ptr(rgb[2]),
+                                   W, H, W, s), "rgb_to_opp")
+    est = net.infer(yuv, [sigma * n for n in OPP_NORMS], [0.0, 0.5, 0.5])
+    out = torch.empty_like(rgb)
+    _check(lib.isp_bm3d_opp_to_rgb(_ptr(out[0]), _ptr(out[1]), _ptr(out[2]), _ptr(est[0]), _ptr(est[1]), _ptr(est[2]),
+                                   W, H, W, 0, s), "opp_to_rgb")
+    return out
+
+
+class PilotBM3D:
+    """Two-step BM3D on a grey plane [H,W] or an RGB frame [3,H,W] with the
+    pilot's estimate as the Wiener guide.  Without a network it runs plain
+    step 1 (the fallback).  para: the BM3D parameters of the plain two-step
+    form (for grey planes with sigma scaled by GREY_SIGMA_SCALE, see
+    default_para)."""
+
+    def __init__(self, para: BM3D_Para, net: PilotCNN | None = None, device: int = 0, precision: str = "fp32",
+                 rgb: bool = False):
         self.net = net
+        self.rgb = rgb
         self.final = BM3D_Final(para.final, device=device, precision=precision)
         self.basic = BM3D_Basic(para.basic, device=device, precision=precision)
-        self.noise_sigma = float(para.basic.sigma[0]) / GREY_SIGMA_SCALE
+        self.noise_sigma = float(para.basic.sigma[0]) / (1.0 if rgb else GREY_SIGMA_SCALE)
 
     def guide(self, noisy):
         if self.net is None:
             return self.basic(noisy)
+        if noisy.dim() == 3:
+            return rgb_guide(self.net, noisy, self.noise_sigma)
         return self.net.infer(noisy, self.noise_sigma)
 
     def __call__(self, noisy):
@@ -301,5 +338,5 @@ class PilotBM3D:
         return self.final.process(out, noisy, g)
 
 
-__all__ = ["PilotCNN", "PilotBM3D", "make_training_set", "train", "default_para", "LAYERS", "GREY_SIGMA_SCALE",
-           "BM3D"]
+__all__ = ["PilotCNN", "PilotBM3D", "make_training_set", "train", "default_para", "rgb_guide", "LAYERS",
+           "GREY_SIGMA_SCALE", "OPP_NORMS", "BM3D"]

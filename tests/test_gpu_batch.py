@@ -157,3 +157,24 @@ def test_autotune_keeps_results_and_auto_group(gpu):
     BM3DBatch(f, [d2], [x]).run()
     torch.cuda.synchronize()
     np.testing.assert_array_equal(d2.cpu().numpy(), before)
+
+
+@pytest.mark.parametrize("rgb", [False, True])
+def test_final_step_batch_with_guides(gpu, rgb):
+    import torch
+    from isp_mw_amd import BM3D_Final
+    shapes = [(48, 64), (56, 40), (40, 72)]
+    pairs = [(synth.rgb_pair if rgb else synth.grey_pair)(h, w, 25.0, 110 + k) for k, (h, w) in enumerate(shapes)]
+    p = para()
+    f = BM3D_Final(p.final, precision="fp16")
+    srcs = [torch.from_numpy(n).to(gpu) for _, n in pairs]
+    # guides: the clean image for the first two, none (src) for the last
+    refs = [torch.from_numpy(c.astype(np.float32)).to(gpu) for c, _ in pairs[:2]] + [None]
+    dsts = [torch.empty_like(t) for t in srcs]
+    BM3DBatch(f, dsts, srcs, refs=refs).run()
+    torch.cuda.synchronize()
+    g = BM3D_Final(p.final, precision="fp16")
+    for k, (c, n) in enumerate(pairs):
+        ref = c.astype(np.float32) if k < 2 else n
+        want = g.process(np.empty_like(n), n, ref)
+        np.testing.assert_array_equal(dsts[k].cpu().numpy(), want, err_msg=f"image {k}")

@@ -204,3 +204,48 @@ def test_pilot_as_wiener_guide(P):
         f"two-step {p_plain:.2f}, pilot + Wiener {p_pilot:.2f}: gain {p_pilot - p_plain:+.3f} dB")
     assert p_pilot > p_noisy + 5.0
     assert p_pilot > p_plain - 1.0
+
+
+def test_opp_conversions_vs_oracle(gpu):
+    import oracle
+    import torch
+    from isp_mw_amd import _lib as L
+    rng = np.random.default_rng(8)
+    H, W = 37, 61
+    rgb = rng.uniform(-0.1, 1.1, (3, H, W)).astype(np.float32)
+    x = torch.from_numpy(rgb).cuda()
+    yuv = torch.empty_like(x)
+    back = torch.empty_like(x)
+    p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    lib = L.lib()
+    assert lib.isp_bm3d_rgb_to_opp(p(yuv[0]), p(yuv[1]), p(yuv[2]), p(x[0]), p(x[1]), p(x[2]), W, H, W, None) == 0
+    torch.cuda.synchronize()
+    want = oracle.rgb2opp(rgb.astype(np.float64))
+    np.testing.assert_allclose(yuv.cpu().numpy(), want, rtol=0, atol=2e-7)
+    for clip in (0, 1):
+        assert lib.isp_bm3d_opp_to_rgb(p(back[0]), p(back[1]), p(back[2]), p(yuv[0]), p(yuv[1]), p(yuv[2]), W, H, W,
+                                       clip, None) == 0
+        torch.cuda.synchronize()
+        ref = oracle.opp2rgb(yuv.cpu().numpy().astype(np.float64), clip=bool(clip))
+        np.testing.assert_allclose(back.cpu().numpy(), ref, rtol=0, atol=2e-7)
+        if not clip:
+            np.testing.assert_allclose(back.cpu().numpy(), rgb, rtol=0, atol=1e-6)
+
+
+def test_pilot_guides_an_rgb_frame(P):
+    import torch
+    from isp_mw_amd import BM3D_Para, synth
+    noisy, sigma, target = P.make_training_set(1024, crop=64, seed=3)
+    net = P.PilotCNN(seed=2)
+    P.train(net, noisy, sigma, target, steps=300, batch=32, lr=2e-3, seed=2)
+    clean, n = synth.rgb_pair(96, 128, 25.0, 5)
+    para = BM3D_Para("np")
+    for st in (para.basic, para.final):
+        st.BlockStep, st.BMrange = 3, 12
+    para.set_sigma(25.0)
+    x = torch.from_numpy(n).cuda()
+    plain = P.PilotBM3D(para, rgb=True)(x).cpu().numpy()
+    out = P.PilotBM3D(para, net, rgb=True)(x).cpu().numpy()
+    pn, pp, po_ = synth.psnr(n, clean), synth.psnr(plain, clean), synth.psnr(out, clean)
+    say(f"RGB: noisy {pn:.2f} dB, two-step {pp:.2f} dB, pilot + Wiener {po_:.2f} dB ({po_ - pp:+.3f})")
+    assert po_ > pn + 4.0
