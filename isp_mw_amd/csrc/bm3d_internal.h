@@ -52,7 +52,8 @@ struct MatchArgs {
     int step;                // BMstep
     int G;                   // GroupSize
     int bypass;              // 1: every group is the reference block alone
-    int exact;               // 1: distances rounded term by term as the reference does
+    int exact;               // 1: FP64 distances rounded term by term as the reference does;
+                             // 0: FP32 distances (fused multiply-add) and 32-bit keys
     double thSSE;            // thMSE * normY * MSE2SSE
     double distMul;          // 1 / MSE2SSE
     int tileW, tileH, tilePitch;  // staged guide tile (doubles)

@@ -124,11 +124,11 @@ __device__ __forceinline__ void fix_add(u64 *acc, float v)
     atomicAdd(acc, (u64)__float2ll_rn(v * kFixScale));
 }
 
-__host__ __device__ inline size_t match_wave_bytes(int maxCand)
+__host__ __device__ inline size_t match_wave_bytes(int maxCand, int keyBytes)
 {
-    // keys u64[maxCand] + selk u64[32] + idxs u32[maxCand] + memb u32[maxCand]
+    // keys K[maxCand] + selk K[32] + idxs u32[maxCand] + memb u32[maxCand]
     // + hist u32[256] + seli u32[32]
-    return align16((size_t)maxCand * 8 + 32 * 8 + (size_t)maxCand * 8 + 256 * 4 + 32 * 4);
+    return align16((size_t)maxCand * keyBytes + 32 * (size_t)keyBytes + (size_t)maxCand * 8 + 256 * 4 + 32 * 4);
 }
 
 // ---------------------------------------------------------------------------
@@ -142,6 +142,9 @@ __host__ __device__ inline size_t match_wave_bytes(int maxCand)
 // EXACT: each term rounded as d*d then added, as the reference's FP64 loop
 // does (include/Block.h:600-603), so equal-looking distances of quantised
 // data tie exactly as they do there; otherwise fused multiply-add.
+// DT: the distance type.  double (the plain form: the reference's FP64
+// distances, 64-bit keys) or float (the mixed-precision forms: FP32 distances
+// and 32-bit keys, half the LDS and half the radix passes).
 __device__ __forceinline__ double sq_acc_exact(double d, double acc)
 {
 #pragma clang fp contract(off)
@@ -156,10 +159,35 @@ __device__ __forceinline__ double sq_acc(double d, double acc)
     return __builtin_fma(d, d, acc);
 }
 
-template <typename GT, bool ST1, bool EXACT, bool BATCHED>
+template <bool EXACT>
+__device__ __forceinline__ float sq_acc(float d, float acc)
+{
+    return fmaf(d, d, acc);
+}
+
+// Sort keys of non-negative distances: their IEEE bit patterns.
+template <typename DT>
+struct KeyOf;
+template <>
+struct KeyOf<double> {
+    typedef uint64_t T;
+    static constexpr int BITS = 64;
+    static __device__ __forceinline__ T key(double v) { return (uint64_t)__double_as_longlong(v); }
+    static __device__ __forceinline__ double value(T k) { return __longlong_as_double((long long)k); }
+};
+template <>
+struct KeyOf<float> {
+    typedef uint32_t T;
+    static constexpr int BITS = 32;
+    static __device__ __forceinline__ T key(float v) { return __float_as_uint(v); }
+    static __device__ __forceinline__ double value(T k) { return (double)__uint_as_float(k); }
+};
+
+template <typename GT, typename DT, bool ST1, bool EXACT, bool BATCHED>
 __global__ __launch_bounds__(kThreads) void match_kernel(MatchArgs a0, const uint32_t *__restrict__ bmap,
                                                          const MatchArgs *__restrict__ bargs)
 {
+    typedef typename KeyOf<DT>::T KT;
     extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int tx, ty;
@@ -171,11 +199,11 @@ __global__ __launch_bounds__(kThreads) void match_kernel(MatchArgs a0, const uin
     const int ox = grid_pos(tx0, a.bstep, a.W - kBlock) - a.range;
     const int oy = grid_pos(ty0, a.bstep, a.H - kBlock) - a.range;
 
-    double *tile = reinterpret_cast<double *>(smem);
-    unsigned char *wb = smem + align16((size_t)a.tileH * a.tilePitch * 8) +
-                        (size_t)wave * match_wave_bytes(a.maxCand);
-    uint64_t *keys = reinterpret_cast<uint64_t *>(wb);
-    uint64_t *selk = keys + a.maxCand;
+    DT *tile = reinterpret_cast<DT *>(smem);
+    unsigned char *wb = smem + align16((size_t)a.tileH * a.tilePitch * sizeof(DT)) +
+                        (size_t)wave * match_wave_bytes(a.maxCand, sizeof(KT));
+    KT *keys = reinterpret_cast<KT *>(wb);
+    KT *selk = keys + a.maxCand;
     uint32_t *idxs = reinterpret_cast<uint32_t *>(selk + 32);
     uint32_t *memb = idxs + a.maxCand;
     uint32_t *hist = memb + a.maxCand;
@@ -184,8 +212,8 @@ __global__ __launch_bounds__(kThreads) void match_kernel(MatchArgs a0, const uin
     for (int k = tid; k < a.tileH * a.tileW; k += kThreads) {
         const int ty = k / a.tileW, tx = k - ty * a.tileW;
         const int yy = oy + ty, xx = ox + tx;
-        double v = 0.0;
-        if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) v = (double)guide[(size_t)yy * gpitch + xx];
+        DT v = 0;
+        if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) v = (DT)guide[(size_t)yy * gpitch + xx];
         tile[ty * a.tilePitch + tx] = v;
     }
     __syncthreads();
@@ -212,7 +240,8 @@ __global__ __launch_bounds__(kThreads) void match_kernel(MatchArgs a0, const uin
         const int nx = (rr - l) / st + 1, ny = (b - t) / st + 1;
         const int selfx = (i - l) / st, selfy = (j - t) / st;
         const int nsx = (nx + kStrip - 1) / kStrip, nstrips = nsx * ny;
-        const double *tref = tile + (j - oy) * a.tilePitch + (i - ox);
+        const DT *tref = tile + (j - oy) * a.tilePitch + (i - ox);
+        const DT thSSE = (DT)a.thSSE, distMul = (DT)a.distMul;
 
         uint32_t cnt = 0;
         for (int s0 = 0; s0 < nstrips; s0 += 64) {
@@ -220,25 +249,25 @@ __global__ __launch_bounds__(kThreads) void match_kernel(MatchArgs a0, const uin
             const bool valid = sidx < nstrips;
             const int ky = valid ? sidx / nsx : 0;
             const int kx0 = valid ? (sidx - ky * nsx) * kStrip : 0;
-            double acc[kStrip];
+            DT acc[kStrip];
 #pragma unroll
-            for (int c = 0; c < kStrip; ++c) acc[c] = 0.0;
+            for (int c = 0; c < kStrip; ++c) acc[c] = 0;
             if (valid) {
-                const double *tc = tile + (t - oy + ky * st) * a.tilePitch + (l - ox + kx0 * st);
+                const DT *tc = tile + (t - oy + ky * st) * a.tilePitch + (l - ox + kx0 * st);
 #pragma unroll
                 for (int u = 0; u < kBlock; ++u) {
-                    double rv[kBlock];
+                    DT rv[kBlock];
 #pragma unroll
                     for (int v = 0; v < kBlock; ++v) rv[v] = tref[u * a.tilePitch + v];
                     if constexpr (ST1) {
-                        double cv[kBlock + kStrip - 1];
+                        DT cv[kBlock + kStrip - 1];
 #pragma unroll
                         for (int q = 0; q < kBlock + kStrip - 1; ++q) cv[q] = tc[u * a.tilePitch + q];
 #pragma unroll
                         for (int c = 0; c < kStrip; ++c)
 #pragma unroll
                             for (int v = 0; v < kBlock; ++v) {
-                                const double d = rv[v] - cv[c + v];
+                                const DT d = rv[v] - cv[c + v];
                                 acc[c] = sq_acc<EXACT>(d, acc[c]);
                             }
                     } else {
@@ -246,7 +275,7 @@ __global__ __launch_bounds__(kThreads) void match_kernel(MatchArgs a0, const uin
                         for (int c = 0; c < kStrip; ++c)
 #pragma unroll
                             for (int v = 0; v < kBlock; ++v) {
-                                const double d = rv[v] - tc[u * a.tilePitch + c * st + v];
+                                const DT d = rv[v] - tc[u * a.tilePitch + c * st + v];
                                 acc[c] = sq_acc<EXACT>(d, acc[c]);
                             }
                     }
@@ -255,12 +284,11 @@ __global__ __launch_bounds__(kThreads) void match_kernel(MatchArgs a0, const uin
 #pragma unroll
             for (int c = 0; c < kStrip; ++c) {
                 const int kx = kx0 + c;
-                const bool ok = valid && kx < nx && !(kx == selfx && ky == selfy) &&
-                                acc[c] <= a.thSSE && acc[c] != 0.0;
+                const bool ok = valid && kx < nx && !(kx == selfx && ky == selfy) && acc[c] <= thSSE && acc[c] != 0;
                 const uint64_t m = __ballot(ok);
                 if (ok) {
                     const uint32_t slot = cnt + lanes_below(m);
-                    keys[slot] = (uint64_t)__double_as_longlong(acc[c] * a.distMul);
+                    keys[slot] = KeyOf<DT>::key(acc[c] * distMul);
                     idxs[slot] = (uint32_t)(ky * nx + kx);
                 }
                 cnt += (uint32_t)__popcll(m);
@@ -278,14 +306,14 @@ __global__ __launch_bounds__(kThreads) void match_kernel(MatchArgs a0, const uin
             }
             nsel = cnt;
         } else {
-            uint64_t prefix = 0, pmask = 0;
+            KT prefix = 0, pmask = 0;
             uint32_t need = K;
             bool whole = false;
-            for (int shift = 56;; shift -= 8) {
+            for (int shift = KeyOf<DT>::BITS - 8;; shift -= 8) {
                 for (int q = lane; q < 256; q += 64) hist[q] = 0;
                 wave_sync();
                 for (uint32_t e = lane; e < cnt; e += 64) {
-                    const uint64_t k = keys[e];
+                    const KT k = keys[e];
                     if ((k & pmask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
                 }
                 wave_sync();
@@ -314,8 +342,8 @@ __global__ __launch_bounds__(kThreads) void match_kernel(MatchArgs a0, const uin
                 blw = __shfl(blw, L);
                 mm = __shfl(mm, L);
                 need -= blw;
-                prefix |= (uint64_t)dig << shift;
-                pmask |= (uint64_t)0xFF << shift;
+                prefix |= (KT)dig << shift;
+                pmask |= (KT)0xFF << shift;
                 if (mm == need) {
                     whole = true;
                     break;
@@ -327,7 +355,7 @@ __global__ __launch_bounds__(kThreads) void match_kernel(MatchArgs a0, const uin
             for (uint32_t e0 = 0; e0 < cnt; e0 += 64) {
                 const uint32_t e = e0 + lane;
                 const bool inr = e < cnt;
-                const uint64_t k = inr ? keys[e] : ~0ull;
+                const KT k = inr ? keys[e] : ~(KT)0;
                 const bool below = inr && (k & pmask) < prefix;
                 const bool inb = inr && (k & pmask) == prefix;
                 const bool take = below || (whole && inb);
@@ -349,7 +377,7 @@ __global__ __launch_bounds__(kThreads) void match_kernel(MatchArgs a0, const uin
                 for (uint32_t q0 = 0; q0 < nm; q0 += 64) {
                     const uint32_t q = q0 + lane;
                     const bool v = q < nm;
-                    uint64_t kq = 0;
+                    KT kq = 0;
                     uint32_t iq = 0, rank = 0;
                     if (v) {
                         const uint32_t e = memb[q];
@@ -357,7 +385,7 @@ __global__ __launch_bounds__(kThreads) void match_kernel(MatchArgs a0, const uin
                         iq = idxs[e];
                         for (uint32_t r2 = 0; r2 < nm; ++r2) {
                             const uint32_t e2 = memb[r2];
-                            const uint64_t k2 = keys[e2];
+                            const KT k2 = keys[e2];
                             const uint32_t i2 = idxs[e2];
                             rank += (k2 < kq) || (k2 == kq && i2 < iq);
                         }
@@ -382,17 +410,17 @@ __global__ __launch_bounds__(kThreads) void match_kernel(MatchArgs a0, const uin
             if (a.key_out) a.key_out[base] = 0.0;
         }
         if ((uint32_t)lane < nsel) {
-            const uint64_t kq = selk[lane];
+            const KT kq = selk[lane];
             const uint32_t iq = seli[lane];
             uint32_t rank = 0;
             for (uint32_t q = 0; q < nsel; ++q) {
-                const uint64_t k2 = selk[q];
+                const KT k2 = selk[q];
                 rank += (k2 < kq) || (k2 == kq && seli[q] < iq);
             }
             const int ky = (int)(iq / (uint32_t)nx), kx = (int)(iq - (uint32_t)ky * nx);
             const int y = t + ky * st, x = l + kx * st;
             a.pos_out[base + 1 + rank] = (y << 16) | x;
-            if (a.key_out) a.key_out[base + 1 + rank] = __longlong_as_double((long long)kq);
+            if (a.key_out) a.key_out[base + 1 + rank] = KeyOf<DT>::value(kq);
         }
         wave_sync();
     }
@@ -1091,18 +1119,18 @@ hipError_t set_lds(K kernel, size_t bytes)
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-template <typename GT, bool EXACT, bool ST1>
+template <typename GT, typename DT, bool EXACT, bool ST1>
 hipError_t launch_match_t(const MatchArgs &a, hipStream_t s, const BatchLaunch *b)
 {
     const size_t lds = match_lds_bytes(a);
     hipError_t e;
     if (b) {
-        auto k = match_kernel<GT, ST1, EXACT, true>;
+        auto k = match_kernel<GT, DT, ST1, EXACT, true>;
         if ((e = set_lds(k, lds)) != hipSuccess) return e;
         hipLaunchKernelGGL(k, dim3(b->units), dim3(kThreads), lds, s, a, b->map,
                            static_cast<const MatchArgs *>(b->args));
     } else {
-        auto k = match_kernel<GT, ST1, EXACT, false>;
+        auto k = match_kernel<GT, DT, ST1, EXACT, false>;
         if ((e = set_lds(k, lds)) != hipSuccess) return e;
         const dim3 grid((a.ngx + a.trx - 1) / a.trx, (a.ngy + a.try_ - 1) / a.try_);
         hipLaunchKernelGGL(k, grid, dim3(kThreads), lds, s, a, nullptr, nullptr);
@@ -1110,17 +1138,20 @@ hipError_t launch_match_t(const MatchArgs &a, hipStream_t s, const BatchLaunch *
     return hipGetLastError();
 }
 
-template <typename GT, bool EXACT>
+template <typename GT, typename DT, bool EXACT>
 hipError_t launch_match_g(const MatchArgs &a, hipStream_t s, const BatchLaunch *b)
 {
-    return a.step == 1 ? launch_match_t<GT, EXACT, true>(a, s, b) : launch_match_t<GT, EXACT, false>(a, s, b);
+    return a.step == 1 ? launch_match_t<GT, DT, EXACT, true>(a, s, b)
+                       : launch_match_t<GT, DT, EXACT, false>(a, s, b);
 }
 
 }  // namespace
 
+// exact (the plain form): FP64 distances and keys; otherwise FP32
 size_t match_lds_bytes(const MatchArgs &a)
 {
-    return align16((size_t)a.tileH * a.tilePitch * 8) + (size_t)kWaves * match_wave_bytes(a.maxCand);
+    const int db = a.exact ? 8 : 4;
+    return align16((size_t)a.tileH * a.tilePitch * db) + (size_t)kWaves * match_wave_bytes(a.maxCand, db);
 }
 
 size_t filter_lds_bytes(const FilterArgs &a, bool) { return filter_lds(a.FW, a.FH, a.G); }
@@ -1128,8 +1159,8 @@ size_t filter_lds_bytes(const FilterArgs &a, bool) { return filter_lds(a.FW, a.F
 hipError_t launch_match(const MatchArgs &a, bool guide64, hipStream_t s, const BatchLaunch *b)
 {
     if (guide64)
-        return a.exact ? launch_match_g<double, true>(a, s, b) : launch_match_g<double, false>(a, s, b);
-    return a.exact ? launch_match_g<float, true>(a, s, b) : launch_match_g<float, false>(a, s, b);
+        return a.exact ? launch_match_g<double, double, true>(a, s, b) : launch_match_g<double, float, false>(a, s, b);
+    return a.exact ? launch_match_g<float, double, true>(a, s, b) : launch_match_g<float, float, false>(a, s, b);
 }
 
 template <typename K, typename A>
