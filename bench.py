@@ -424,6 +424,9 @@ def main() -> int:
     ap.add_argument("--precision", default="fp16", choices=("fp32", "fp16", "bf16"),
                     help="transform/shrinkage precision of the timed form (fp16: the metric's mixed form)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-tune", action="store_true", help="default tile shapes instead of the autotuner's")
+    ap.add_argument("--tiles", default="", help="mx,my,fx,fy: given tile shapes of the timed form (no autotuning; "
+                                                "for profiler runs, whose averages the tuner's trials would mix)")
     ap.add_argument("--cpu-seconds", type=float, default=4.0, help="target seconds per CPU baseline run")
     ap.add_argument("--workload", default="rgb1080", choices=("rgb1080", "batch64", "pilot1080", "full4k"),
                     help="rgb1080: the headline 1080p RGB frame; batch64: 64 uneven grey images as one batch; "
@@ -464,6 +467,14 @@ def main() -> int:
     x = torch.from_numpy(noisy).to(dev)
     y = torch.empty_like(x)
     f = BM3D(para, device=0, precision=args.precision)
+    # tile shapes of this size class from the autotuner, or given (tiles never change results)
+    tuning = None
+    if args.tiles:
+        mx, my, fx, fy = (int(v) for v in args.tiles.split(","))
+        tuning = {"match_tile": (mx, my), "filter_tile": (fx, fy), "final_group_size": 0, "given": True}
+        f.set_tuning(f.size_class(W, H), tuning)
+    elif not args.no_tune:
+        tuning = f.autotune(W, H, SIGMA, rgb=True)
 
     dt, stages = timed_steps(f, y, x, args.steps, args.warmup)
     value = W * H / dt / 1e6
@@ -475,6 +486,8 @@ def main() -> int:
     if args.precision != "fp32":
         # the metric's bar: PSNR loss against the plain FP32 GPU path, timed beside it
         f32 = BM3D(para, device=0, precision="fp32")
+        if not args.no_tune and not args.tiles:
+            f32.autotune(W, H, SIGMA, rgb=True)
         y32 = torch.empty_like(x)
         dt32, st32 = timed_steps(f32, y32, x, max(2, args.steps // 2), 1)
         ref32 = y32.cpu().numpy()
@@ -561,6 +574,7 @@ def main() -> int:
                     **quality_extra},
         "stages": per_stage,
         "fp32_form": fp32_form,
+        "tuning": tuning,
         "host_buffers_mpix_s": round(host_rate, 3),
         "host_buffers_max_abs_vs_device": host_dev,
         "work": {"refs_per_step": refs, "match_candidates_per_step": cand, "match_flops_per_step": match_flops,

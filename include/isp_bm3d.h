@@ -191,6 +191,7 @@ typedef struct isp_bm3d_batch isp_bm3d_batch;
 /* flags */
 #define ISP_BM3D_BATCH_AUTO_GROUP 1  /* the final step's GroupSize of each image from the
                                         tuning table of its size class, when one is set */
+#define ISP_BM3D_TUNE_RGB 2          /* isp_bm3d_autotune: measure RGB frames (three planes) */
 
 /* mode: ISP_BM3D_BASIC, ISP_BM3D_FINAL (ref: the guide frames) or ISP_BM3D_BOTH.
  * Buffers are device memory; the plan

@@ -49,6 +49,7 @@ EXPORTS = (
     "isp_bm3d_get_tuning",
 )
 BATCH_AUTO_GROUP = 1
+TUNE_RGB = 2
 NUM_STAGES = 8
 
 

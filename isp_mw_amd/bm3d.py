@@ -278,13 +278,15 @@ class _Filter:
     def size_class(width: int, height: int) -> int:
         return int(L.lib().isp_bm3d_size_class(int(width), int(height)))
 
-    def autotune(self, width: int, height: int, sigma: float = 0.0, auto_group: bool = False) -> dict:
+    def autotune(self, width: int, height: int, sigma: float = 0.0, auto_group: bool = False,
+                 rgb: bool = False) -> dict:
         """Measures tile shapes (and, with auto_group, a final GroupSize of 16)
-        on seeded synthetic images of this size; the choice applies to every
-        later call and batch of the size class."""
+        on seeded synthetic images (rgb: RGB frames) of this size; the choice
+        applies to every later call and batch of the size class."""
         t = L.Tuning()
-        L.check(L.lib().isp_bm3d_autotune(self._ctx, int(width), int(height), float(sigma),
-                                          L.BATCH_AUTO_GROUP if auto_group else 0, ctypes.byref(t)), self._ctx)
+        flags = (L.BATCH_AUTO_GROUP if auto_group else 0) | (L.TUNE_RGB if rgb else 0)
+        L.check(L.lib().isp_bm3d_autotune(self._ctx, int(width), int(height), float(sigma), flags, ctypes.byref(t)),
+                self._ctx)
         return _tuning_dict(t)
 
     def set_tuning(self, size_class: int, tuning: dict | None) -> None:

@@ -2022,9 +2022,14 @@ int isp_bm3d_autotune(isp_bm3d_ctx *ctx, int W, int H, double sigma, int flags, 
     HIP_CHECK(ctx, hipSetDevice(ctx->device));
     const int cls = size_class(W, H);
     ctx->tune.erase(cls);
-    // the synthetic workload: K noisy copies of one image, enough tiles to fill the GPU
-    std::vector<float> clean, noisy;
-    synth_clean(clean, W, H, 0x5EED0000ull + (uint64_t)cls);
+    // the synthetic workload: K noisy copies of one image (RGB: three planes),
+    // enough tiles to fill the GPU
+    const int np_ = (flags & ISP_BM3D_TUNE_RGB) ? 3 : 1;
+    std::vector<float> clean, noisy, plane;
+    for (int c = 0; c < np_; ++c) {
+        synth_clean(plane, W, H, 0x5EED0000ull + (uint64_t)cls * 8 + (uint64_t)c);
+        clean.insert(clean.end(), plane.begin(), plane.end());
+    }
     noisy = clean;
     SplitMix rn{0xA5A5ull + (uint64_t)W * 131 + (uint64_t)H};
     const double sg = (sigma > 0 ? sigma : ctx->params.basic.sigma[0]) / 255.0;
@@ -2036,18 +2041,20 @@ int isp_bm3d_autotune(isp_bm3d_ctx *ctx, int W, int H, double sigma, int flags, 
     const int K = (int)std::max(1L, std::min(32L, (4096 + tiles - 1) / std::max(1L, tiles)));
     const size_t np = (size_t)W * H;
     float *dmem = nullptr;
-    HIP_CHECK(ctx, hipMalloc(&dmem, np * 4 * (K + 1)));
+    HIP_CHECK(ctx, hipMalloc(&dmem, np * 4 * np_ * (K + 1)));
     struct Guard {
         float *p;
         ~Guard() { (void)hipFree(p); }
     } guard{dmem};
-    HIP_CHECK(ctx, hipMemcpy(dmem, noisy.data(), np * 4, hipMemcpyHostToDevice));
+    HIP_CHECK(ctx, hipMemcpy(dmem, noisy.data(), np * 4 * np_, hipMemcpyHostToDevice));
     std::vector<isp_bm3d_image> imgs(K);
     for (int k = 0; k < K; ++k) {
         isp_bm3d_image &im = imgs[k];
         std::memset(&im, 0, sizeof(im));
-        im.dst[0] = dmem + np * (k + 1);
-        im.src[0] = dmem;
+        for (int c = 0; c < np_; ++c) {
+            im.dst[c] = dmem + np * ((size_t)np_ * (k + 1) + c);
+            im.src[c] = dmem + np * c;
+        }
         im.width = W;
         im.height = H;
         im.stride = (size_t)W;
@@ -2061,14 +2068,14 @@ int isp_bm3d_autotune(isp_bm3d_ctx *ctx, int W, int H, double sigma, int flags, 
     } sguard{s};
     auto measure = [&](const int *mt, const int *ft, int g, double *ms, double *psnr) -> int {
         isp_bm3d_batch *b = nullptr;
-        int r = batch_make(ctx, &b, imgs.data(), K, 1, ISP_BM3D_BOTH, 0, mt, ft, g);
+        int r = batch_make(ctx, &b, imgs.data(), K, np_, ISP_BM3D_BOTH, 0, mt, ft, g);
         if (r) return r;
         r = batch_time(ctx, b, s, 3, ms);
         if (!r && psnr) {
-            std::vector<float> outp(np);
-            hipError_t e = hipMemcpy(outp.data(), imgs[0].dst[0], np * 4, hipMemcpyDeviceToHost);
+            std::vector<float> outp(np * np_);
+            hipError_t e = hipMemcpy(outp.data(), imgs[0].dst[0], np * 4 * np_, hipMemcpyDeviceToHost);
             if (e != hipSuccess) r = fail(ctx, ISP_BM3D_E_HIP, "copy: %s", hipGetErrorString(e));
-            else *psnr = psnr01(outp.data(), clean.data(), np);
+            else *psnr = psnr01(outp.data(), clean.data(), np * np_);
         }
         (void)isp_bm3d_batch_destroy(b);
         return r;

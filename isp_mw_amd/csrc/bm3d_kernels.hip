@@ -118,10 +118,17 @@ __host__ __device__ constexpr size_t align16(size_t x) { return (x + 15) & ~size
 typedef unsigned long long u64;
 
 // Adds v to a 64-bit fixed-point accumulator in LDS (integer LDS atomics are
-// full rate; two's complement wraps negative values correctly).
+// full rate; two's complement wraps negative values correctly).  The value
+// added is floor(v * 2^36), exactly: hi = floor(v * 2^4) is the part above
+// 2^32, and the fraction (v * 2^4 - hi) in [0, 1) scaled by 2^32 is an exact
+// FP32 value whose truncation is the low word.  Six ALU operations against
+// the eleven of a rounding float -> int64 conversion.
 __device__ __forceinline__ void fix_add(u64 *acc, float v)
 {
-    atomicAdd(acc, (u64)__float2ll_rn(v * kFixScale));
+    const float s = v * (kFixScale / 4294967296.0f);
+    const float hi = floorf(s);
+    const uint32_t lo = (uint32_t)((s - hi) * 4294967296.0f);
+    atomicAdd(acc, ((u64)(long long)(int)hi << 32) + lo);
 }
 
 __host__ __device__ inline size_t match_wave_bytes(int maxCand, int keyBytes)

@@ -178,3 +178,14 @@ def test_final_step_batch_with_guides(gpu, rgb):
         ref = c.astype(np.float32) if k < 2 else n
         want = g.process(np.empty_like(n), n, ref)
         np.testing.assert_array_equal(dsts[k].cpu().numpy(), want, err_msg=f"image {k}")
+
+
+def test_rgb_autotune_keeps_results(gpu):
+    import torch
+    f = BM3D(para(), precision="fp16")
+    _, n = synth.rgb_pair(120, 200, 25.0, 9)
+    x = torch.from_numpy(n).to(gpu)
+    before = f(x).cpu().numpy()
+    t = f.autotune(200, 120, 25.0, rgb=True)
+    say(f"rgb autotune 200x120: {t}")
+    np.testing.assert_array_equal(f(x).cpu().numpy(), before)
