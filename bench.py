@@ -424,7 +424,8 @@ def main() -> int:
     ap.add_argument("--precision", default="fp16", choices=("fp32", "fp16", "bf16"),
                     help="transform/shrinkage precision of the timed form (fp16: the metric's mixed form)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-tune", action="store_true", help="default tile shapes instead of the autotuner's")
+    ap.add_argument("--tune", action="store_true", help="autotune the tile shapes of the frame's size class first "
+                                                        "(default: the built-in shapes, which measure as fast)")
     ap.add_argument("--tiles", default="", help="mx,my,fx,fy: given tile shapes of the timed form (no autotuning; "
                                                 "for profiler runs, whose averages the tuner's trials would mix)")
     ap.add_argument("--cpu-seconds", type=float, default=4.0, help="target seconds per CPU baseline run")
@@ -473,7 +474,7 @@ def main() -> int:
         mx, my, fx, fy = (int(v) for v in args.tiles.split(","))
         tuning = {"match_tile": (mx, my), "filter_tile": (fx, fy), "final_group_size": 0, "given": True}
         f.set_tuning(f.size_class(W, H), tuning)
-    elif not args.no_tune:
+    elif args.tune:
         tuning = f.autotune(W, H, SIGMA, rgb=True)
 
     dt, stages = timed_steps(f, y, x, args.steps, args.warmup)
@@ -486,7 +487,7 @@ def main() -> int:
     if args.precision != "fp32":
         # the metric's bar: PSNR loss against the plain FP32 GPU path, timed beside it
         f32 = BM3D(para, device=0, precision="fp32")
-        if not args.no_tune and not args.tiles:
+        if args.tune and not args.tiles:
             f32.autotune(W, H, SIGMA, rgb=True)
         y32 = torch.empty_like(x)
         dt32, st32 = timed_steps(f32, y32, x, max(2, args.steps // 2), 1)
