@@ -1,7 +1,8 @@
-"""Build libisp_bm3d.so (HIP kernels + C ABI) for gfx950 with hipcc.
+"""Build libisp_bm3d.so (HIP kernels + C ABI) for gfx950 with hipcc, and the
+`isp_mw` command-line front end on it (g++, zlib).
 
-Run ``python -m isp_mw_amd.build`` (or ``__graft_entry__.build()``).  The
-library is written in-tree to ``isp_mw_amd/lib/libisp_bm3d.so``.
+Run ``python -m isp_mw_amd.build`` (or ``__graft_entry__.build()``).  Both are
+written in-tree to ``isp_mw_amd/lib/``.
 """
 from __future__ import annotations
 
@@ -14,6 +15,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libisp_bm3d.so")
+CLI = os.path.join(LIBDIR, "isp_mw")
+CLI_SRC = os.path.join(HERE, "cli", "isp_mw.cpp")
 
 SOURCES = ["bm3d_kernels.hip", "bm3d_capi.cpp", "pilot.hip"]
 
@@ -49,7 +52,22 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("hipcc failed building libisp_bm3d.so")
     os.replace(tmp, LIB)
+    build_cli(verbose)
     return LIB
+
+
+def build_cli(verbose: bool = False) -> str:
+    tmp = CLI + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), CLI_SRC,
+           "-L", LIBDIR, "-lisp_bm3d", "-lz", "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("g++ failed building isp_mw")
+    os.replace(tmp, CLI)
+    return CLI
 
 
 if __name__ == "__main__":
