@@ -143,7 +143,7 @@ int isp_bm3d_match(isp_bm3d_ctx *ctx, const float *guide, int width, int height,
  * recorded events, adds their elapsed times into per-stage totals and copies
  * up to `max` stages: names[i] (static strings), total_ms[i], launches[i].
  * It returns the number of stages (ISP_BM3D_NUM_STAGES). */
-#define ISP_BM3D_NUM_STAGES 8
+#define ISP_BM3D_NUM_STAGES 10
 int isp_bm3d_set_timing(isp_bm3d_ctx *ctx, int enable);
 int isp_bm3d_get_timing(isp_bm3d_ctx *ctx, const char **names, double *total_ms, long *launches, int max);
 int isp_bm3d_reset_timing(isp_bm3d_ctx *ctx);

@@ -161,6 +161,35 @@ def _is_torch(x) -> bool:
     return type(x).__module__.startswith("torch")
 
 
+def io_args(dst, src, ref, floor, neutral, ceil, bit_depth):
+    """Pointers, stream and memory kind of a FilterIF2::process call on numpy
+    (host) or torch (device) arrays, and the integer quantisation."""
+    if _is_torch(src):
+        import torch
+        if not (src.is_cuda and src.dtype in (torch.float32, torch.int32) and src.is_contiguous()):
+            raise ValueError("torch inputs must be contiguous float32 or int32 tensors on a HIP device")
+        is_int = src.dtype == torch.int32
+        ptr = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
+        stream = ctypes.c_void_p(torch.cuda.current_stream(src.device).cuda_stream)
+        mem = L.MEM_DEVICE
+    else:
+        is_int = isinstance(src, np.ndarray) and src.dtype == np.int32
+        want = np.int32 if is_int else np.float32
+        for a in (dst, src, ref):
+            if a is not None and not (isinstance(a, np.ndarray) and a.dtype == want and a.flags.c_contiguous):
+                raise ValueError("numpy inputs must be C-contiguous float32 (or all int32) arrays")
+        ptr = lambda a: ctypes.c_void_p(a.ctypes.data) if a is not None else None  # noqa: E731
+        stream = None
+        mem = L.MEM_HOST
+    if tuple(dst.shape) != tuple(src.shape) or (ref is not None and tuple(ref.shape) != tuple(src.shape)):
+        raise ValueError("dst, src and ref must have the same shape")
+    if is_int:
+        floor = 0 if floor is None else int(floor)
+        ceil = (1 << bit_depth) - 1 if ceil is None else int(ceil)
+        neutral = floor if neutral is None else int(neutral)
+    return ptr, stream, mem, is_int, (floor, neutral, ceil)
+
+
 class _Filter:
     _mode = L.BOTH
 
@@ -187,30 +216,7 @@ class _Filter:
     def process(self, dst, src, ref=None, floor=None, neutral=None, ceil=None, bit_depth=8):
         """Integer samples use Floor / Neutral / Ceil (defaults: full range of
         bit_depth, luma, i.e. 0 / 0 / 2^bit_depth - 1)."""
-        torch_io = _is_torch(src)
-        if torch_io:
-            import torch
-            if not (src.is_cuda and src.dtype in (torch.float32, torch.int32) and src.is_contiguous()):
-                raise ValueError("torch inputs must be contiguous float32 or int32 tensors on a HIP device")
-            is_int = src.dtype == torch.int32
-            ptr = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
-            stream = ctypes.c_void_p(torch.cuda.current_stream(src.device).cuda_stream)
-            mem = L.MEM_DEVICE
-        else:
-            is_int = isinstance(src, np.ndarray) and src.dtype == np.int32
-            want = np.int32 if is_int else np.float32
-            for a in (dst, src, ref):
-                if a is not None and not (isinstance(a, np.ndarray) and a.dtype == want and a.flags.c_contiguous):
-                    raise ValueError("numpy inputs must be C-contiguous float32 (or all int32) arrays")
-            ptr = lambda a: ctypes.c_void_p(a.ctypes.data) if a is not None else None  # noqa: E731
-            stream = None
-            mem = L.MEM_HOST
-        if tuple(dst.shape) != tuple(src.shape) or (ref is not None and tuple(ref.shape) != tuple(src.shape)):
-            raise ValueError("dst, src and ref must have the same shape")
-        if is_int:
-            floor = 0 if floor is None else int(floor)
-            ceil = (1 << bit_depth) - 1 if ceil is None else int(ceil)
-            neutral = floor if neutral is None else int(neutral)
+        ptr, stream, mem, is_int, (floor, neutral, ceil) = io_args(dst, src, ref, floor, neutral, ceil, bit_depth)
         lib = L.lib()
         if src.ndim == 2:
             h, w = src.shape

@@ -7,6 +7,7 @@
 //   copy-through on degenerate parameters source/BM3D.cpp:82-87, 143-151, 284-290
 //   step orchestration                    source/BM3D.cpp:464-527
 #include "../../include/isp_bm3d.h"
+#include "../../include/isp_nlm.h"
 #include "bm3d_internal.h"
 
 #include <cmath>
@@ -70,10 +71,13 @@ enum Stage {
     ST_MATCH_FINAL,
     ST_FILTER_FINAL,
     ST_FINALIZE,
-    ST_COPY
+    ST_COPY,
+    ST_NLM_MATCH,
+    ST_NLM_AVERAGE
 };
 const char *const kStageNames[ISP_BM3D_NUM_STAGES] = {
-    "opp", "fill", "match_basic", "filter_basic", "match_final", "filter_final", "finalize", "copy"};
+    "opp", "fill", "match_basic", "filter_basic", "match_final", "filter_final", "finalize", "copy",
+    "nlm_match", "nlm_average"};
 
 hipEvent_t take_event(isp_bm3d_ctx *ctx)
 {
@@ -2137,4 +2141,342 @@ int isp_bm3d_autotune(isp_bm3d_ctx *ctx, int W, int H, double sigma, int flags, 
     if (out) *out = t;
     return ISP_BM3D_OK;
 }
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// NL-means (include/NLMeans.h, source/NLMeans.cpp) on the matching kernel.
+
+namespace {
+
+// NLMeans::process_* early return (source/NLMeans.cpp:152-158, 234-240, 263-269)
+bool nlm_degenerate(const isp_nlm_params &p)
+{
+    return p.strength <= 0 || p.group_size == 1 || p.block_size <= 0 || p.bm_range <= 0 ||
+           p.bm_range < p.bm_step || p.thMSE <= 0;
+}
+
+struct NlmWork {
+    double *g64;                  // FP64 matching guide (int forms)
+    double *s64;                  // FP64 source plane (int plane form)
+    float *stage[6];              // host uploads: src planes, then ref planes (4-byte samples)
+    unsigned long long *num[3];   // fixed-point sums
+    float *out[3];                // host-mode output staging (4-byte samples)
+    int *cnt, *pos;
+    double *keys;
+};
+
+size_t nlm_layout(NlmWork *w, void *base, int P, int H, size_t refs, int G)
+{
+    Carver c(base);
+    const size_t np = (size_t)P * H;
+    w->g64 = c.take<double>(np);
+    w->s64 = c.take<double>(np);
+    for (int k = 0; k < 6; ++k) w->stage[k] = c.take<float>(np);
+    for (int k = 0; k < 3; ++k) w->num[k] = c.take<unsigned long long>(np);
+    for (int k = 0; k < 3; ++k) w->out[k] = c.take<float>(np);
+    w->cnt = c.take<int>(refs);
+    w->pos = c.take<int>(refs * G);
+    w->keys = c.take<double>(refs * G);
+    return c.off + 256;
+}
+
+// The three forms: nplanes 1 (float plane, or int plane when fmt.is_int) or
+// 3 (int RGB frame).
+int nlm_common(isp_bm3d_ctx *ctx, const isp_nlm_params *pp, int nplanes, void *const *dst,
+               const void *const *src, const void *const *ref, int W, int H, size_t stride, const Fmt &fmt,
+               int mem, void *stream)
+{
+    int rc = check_common(ctx, W, H, stride, ISP_BM3D_BASIC, mem);
+    if (rc) return rc;
+    if (!pp) return fail(ctx, ISP_BM3D_E_ARG, "null NL-means parameters");
+    const isp_nlm_params p = *pp;
+    for (int c = 0; c < nplanes; ++c)
+        if (!dst[c] || !src[c]) return fail(ctx, ISP_BM3D_E_ARG, "null dst/src plane");
+    if (fmt.is_int && !(fmt.floor_ < fmt.ceil_ && fmt.floor_ <= fmt.neutral && fmt.neutral <= fmt.ceil_))
+        return fail(ctx, ISP_BM3D_E_ARG, "bad quantisation floor %d neutral %d ceil %d", fmt.floor_, fmt.neutral,
+                    fmt.ceil_);
+    bool has_ref = false, all_ref = true, sep_ref = false;
+    for (int c = 0; c < nplanes; ++c) {
+        has_ref = has_ref || ref[c];
+        all_ref = all_ref && ref[c];
+        sep_ref = sep_ref || (ref[c] && ref[c] != src[c]);
+    }
+    if (has_ref && !all_ref) return fail(ctx, ISP_BM3D_E_ARG, "ref needs every plane");
+    hipStream_t s = (hipStream_t)stream;
+    const bool host = mem == ISP_BM3D_MEM_HOST;
+    if (nlm_degenerate(p)) {
+        for (int c = 0; c < nplanes; ++c) {
+            if (host) host_copy_plane(dst[c], src[c], stride, W, H);
+            else if (dst[c] != src[c] && (rc = copy_plane(ctx, dst[c], stride, src[c], stride, W, H, s)))
+                return rc;
+        }
+        return ISP_BM3D_OK;
+    }
+    if (p.block_size != kBlock)
+        return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "BlockSize %d: the GPU kernels implement BlockSize 8",
+                    p.block_size);
+    if (p.group_size < 2 || p.group_size > kMaxGroup)
+        return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "GroupSize %d outside 2..%d", p.group_size, kMaxGroup);
+    if (p.block_step <= 0 || p.bm_step <= 0)
+        return fail(ctx, ISP_BM3D_E_ARG, "BlockStep and BMstep must be positive");
+    if (W < kBlock || H < kBlock)
+        return fail(ctx, ISP_BM3D_E_SIZE, "image %dx%d is smaller than one %dx%d block", W, H, kBlock, kBlock);
+    if (W > 65535 || H > 65535)
+        return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "image %dx%d: width and height must be < 65536", W, H);
+
+    // matching: BlockMatchingMulti with the unscaled thMSE (source/NLMeans.cpp:203, 332)
+    isp_bm3d_step_params mp{};
+    mp.block_size = kBlock;
+    mp.block_step = p.block_step;
+    mp.group_size = p.group_size;
+    mp.bm_range = p.bm_range;
+    mp.bm_step = p.bm_step;
+    mp.thMSE = p.thMSE;
+    Derived d{};
+    d.range = p.bm_range / p.bm_step * p.bm_step;
+    d.bypass = false;
+    const double MSE2SSE = (double)(kBlock * kBlock) * 1.0 * 1.0 / (double)(255 * 255);
+    d.distMul = 1.0 / MSE2SSE;
+    d.thSSE = p.thMSE * MSE2SSE;
+    d.ngx = (W - kBlock) / p.block_step + 1 + ((W - kBlock) % p.block_step ? 1 : 0);
+    d.ngy = (H - kBlock) / p.block_step + 1 + ((H - kBlock) % p.block_step ? 1 : 0);
+    const size_t refs = (size_t)d.ngx * d.ngy;
+
+    const int P = internal_pitch(W, stride, mem);
+    NlmWork w{};
+    if ((rc = ensure_ws(ctx, nlm_layout(&w, nullptr, P, H, refs, p.group_size)))) return rc;
+    nlm_layout(&w, ctx->ws, P, H, refs, p.group_size);
+
+    const void *in[3] = {src[0], nplanes > 1 ? src[1] : nullptr, nplanes > 1 ? src[2] : nullptr};
+    const void *rin[3] = {ref[0], nplanes > 1 ? ref[1] : nullptr, nplanes > 1 ? ref[2] : nullptr};
+    size_t ipitch = stride;
+    const HostStage hs{ctx, s, P, W, H};
+    if (host) {
+        if ((rc = HostStage::reserve(ctx, (size_t)(sep_ref ? 2 : 1) * nplanes * hs.plane_bytes()))) return rc;
+        for (int c = 0; c < nplanes; ++c) {
+            if ((rc = hs.upload(c, w.stage[c], in[c], stride))) return rc;
+            in[c] = w.stage[c];
+            if (sep_ref) {
+                if ((rc = hs.upload(nplanes + c, w.stage[3 + c], rin[c], stride))) return rc;
+                rin[c] = w.stage[3 + c];
+            }
+        }
+        ipitch = (size_t)P;
+    }
+    if (!sep_ref)
+        for (int c = 0; c < nplanes; ++c) rin[c] = in[c];
+
+    NlmArgs a{};
+    a.W = W;
+    a.H = H;
+    a.ngx = d.ngx;
+    a.ngy = d.ngy;
+    a.bstep = p.block_step;
+    a.G = p.group_size;
+    a.cnt = w.cnt;
+    a.pos = w.pos;
+    a.keys = w.keys;
+    a.nplanes = nplanes;
+    a.correction = p.correction != 0;
+    a.exp_mul = -1 / (p.strength * p.strength);
+    a.pitch = P;
+    for (int c = 0; c < nplanes; ++c) a.num[c] = w.num[c];
+    NlmFinalizeArgs fa{};
+    fa.W = W;
+    fa.H = H;
+    fa.pitch = P;
+    fa.bstep = p.block_step;
+    fa.nplanes = nplanes;
+    const void *mguide = nullptr;
+    bool g64 = false;
+    if (nplanes == 3) {
+        // R, G, B averaged as int samples; matching on ConvertToY(ref) with clip
+        OppArgs o{};
+        o.W = W;
+        o.H = H;
+        o.r = rin[0];
+        o.g = rin[1];
+        o.b = rin[2];
+        o.in_int = 1;
+        const double sRange = (double)(fmt.ceil_ - fmt.floor_);
+        o.gainY = 1.0 / (sRange * 3.0);
+        o.offsetY = -(double)fmt.floor_ * 3.0 * o.gainY + 0.0;
+        o.in_pitch = ipitch;
+        o.pitch = P;
+        o.y64 = w.g64;
+        o.pitch64 = P;
+        o.clipY = 1;
+        {
+            StageTimer tm(ctx, ST_OPP, s);
+            HIP_CHECK(ctx, launch_opp(o, s));
+        }
+        mguide = w.g64;
+        g64 = true;
+        a.src_type = 2;
+        for (int c = 0; c < 3; ++c) a.src[c] = in[c];
+        a.spitch = ipitch;
+        const double sigma = p.sigma * (double)(fmt.ceil_ - fmt.floor_) / 255;
+        a.varN = sigma * sigma;
+        // |samples| < 2^(e+5): 64 blocks per pixel stay below 2^63
+        const int M = std::max({std::abs(fmt.floor_), std::abs(fmt.ceil_), 1});
+        int e = 0;
+        while (e < 31 && ((int64_t)1 << e) < (int64_t)M) ++e;
+        a.fix = std::ldexp(1.0, 52 - e);
+        fa.out_int = 1;
+        fa.gain = 1.0;
+        fa.offset = 0.5;
+        fa.clip = 0;
+    } else if (fmt.is_int) {
+        // Plane_FL src_data(src): RangeConvert with clip (source/NLMeans.cpp:242-243,
+        // include/Conversion.hpp:277-296)
+        ToFloatArgs t{};
+        t.W = W;
+        t.H = H;
+        t.spitch = ipitch;
+        t.dpitch = P;
+        t.gain = 1.0 / (double)(fmt.ceil_ - fmt.floor_);
+        t.offset = -(double)(fmt.chroma() ? fmt.neutral : fmt.floor_) * t.gain + 0.0;
+        t.clip = 1;
+        t.lo = fmt.chroma() ? -0.5 : 0.0;
+        t.hi = fmt.chroma() ? 0.5 : 1.0;
+        {
+            StageTimer tm(ctx, ST_OPP, s);
+            t.src = (const int *)in[0];
+            t.dst = w.out[2];  // FP32 copy unused
+            t.dst64 = w.s64;
+            HIP_CHECK(ctx, launch_to_float(t, s));
+            mguide = w.s64;
+            if (sep_ref) {
+                t.src = (const int *)rin[0];
+                t.dst64 = w.g64;
+                HIP_CHECK(ctx, launch_to_float(t, s));
+                mguide = w.g64;
+            }
+        }
+        g64 = true;
+        a.src_type = 1;
+        a.src[0] = w.s64;
+        a.spitch = (size_t)P;
+        const double sigma = p.sigma * 1.0 / 255;
+        a.varN = sigma * sigma;
+        a.fix = std::ldexp(1.0, 52);
+        // RangeConvert back, +0.5 (0.499999 for PC-range chroma); values are
+        // convex combinations of clipped samples, so the clip changes nothing
+        fa.out_int = 1;
+        fa.gain = (double)(fmt.ceil_ - fmt.floor_) / 1.0;
+        fa.offset = -0.0 * fa.gain + (double)(fmt.chroma() ? fmt.neutral : fmt.floor_) +
+                    (fmt.pc_chroma() ? 0.499999 : 0.5);
+        fa.clip = 1;
+        fa.lo = fmt.floor_;
+        fa.hi = fmt.ceil_;
+    } else {
+        mguide = rin[0];
+        g64 = false;
+        a.src_type = 0;
+        a.src[0] = in[0];
+        a.spitch = ipitch;
+        const double sigma = p.sigma * 1.0 / 255;
+        a.varN = sigma * sigma;
+        a.fix = std::ldexp(1.0, 36);
+        fa.out_int = 0;
+    }
+    fa.inv_fix = 1.0 / a.fix;
+    const int gpitch = g64 ? P : (int)ipitch;
+
+    const isp_bm3d_ctx::Tiles tl = tiles_for(ctx, W, H);
+    const MatchArgs ma = build_match(ctx, mp, d, W, H, mguide, gpitch, tl.match_x, tl.match_y, w.cnt, w.pos,
+                                     w.keys);
+    if ((rc = check_match_lds(ctx, mp, ma))) return rc;
+    {
+        StageTimer tm(ctx, ST_NLM_MATCH, s);
+        HIP_CHECK(ctx, launch_match(ma, g64, s));
+    }
+    {
+        StageTimer tm(ctx, ST_FILL, s);
+        for (int c = 0; c < nplanes; ++c) HIP_CHECK(ctx, launch_fill((float *)w.num[c], (size_t)P * H * 2, 0.0f, s));
+    }
+    // tile of reference blocks per workgroup: 8 x 8, smaller where the
+    // footprint's accumulators would not fit in 64 KiB of LDS
+    int t = 8;
+    for (;; --t) {
+        a.trx = a.try_ = t;
+        a.FW = (t - 1) * p.block_step + kBlock;
+        a.FH = a.FW;
+        if (t == 1 || nlm_lds_bytes(a) <= 65536) break;
+    }
+    {
+        StageTimer tm(ctx, ST_NLM_AVERAGE, s);
+        HIP_CHECK(ctx, launch_nlm(a, s));
+    }
+    void *out[3] = {dst[0], nplanes > 1 ? dst[1] : nullptr, nplanes > 1 ? dst[2] : nullptr};
+    fa.out_pitch = stride;
+    if (host) {
+        for (int c = 0; c < nplanes; ++c) out[c] = w.out[c];
+        fa.out_pitch = (size_t)P;
+    }
+    for (int c = 0; c < nplanes; ++c) {
+        fa.num[c] = w.num[c];
+        fa.out[c] = out[c];
+    }
+    {
+        StageTimer tm(ctx, ST_FINALIZE, s);
+        HIP_CHECK(ctx, launch_nlm_finalize(fa, s));
+    }
+    if (host) {
+        for (int c = 0; c < nplanes; ++c)
+            if ((rc = hs.download_start(c, out[c]))) return rc;
+        HIP_CHECK(ctx, hipStreamSynchronize(s));
+        for (int c = 0; c < nplanes; ++c) hs.download_finish(c, dst[c], stride);
+    }
+    return ISP_BM3D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int isp_nlm_params_default(isp_nlm_params *p, int correction, double sigma)
+{
+    if (!p) return ISP_BM3D_E_ARG;
+    // NLMeans_Para (include/NLMeans.h:11-22)
+    p->correction = correction != 0;
+    p->sigma = sigma;
+    p->strength = correction ? sigma * 5 : sigma * 1.5;
+    p->block_size = 8;
+    p->block_step = 5;
+    p->group_size = 16;
+    p->bm_range = 24;
+    p->bm_step = 3;
+    p->thMSE = correction ? sigma * 50 : sigma * 25;
+    return ISP_BM3D_OK;
+}
+
+int isp_nlm_plane_f32(isp_bm3d_ctx *ctx, const isp_nlm_params *p, float *dst, const float *src, const float *ref,
+                      int W, int H, size_t stride, int mem, void *stream)
+{
+    void *d[3] = {dst, nullptr, nullptr};
+    const void *sv[3] = {src, nullptr, nullptr}, *r[3] = {ref, nullptr, nullptr};
+    const Fmt f{false, 0, 0, 1};
+    return nlm_common(ctx, p, 1, d, sv, r, W, H, stride, f, mem, stream);
+}
+
+int isp_nlm_plane_i32(isp_bm3d_ctx *ctx, const isp_nlm_params *p, int *dst, const int *src, const int *ref, int W,
+                      int H, size_t stride, int floor_, int neutral, int ceil_, int mem, void *stream)
+{
+    void *d[3] = {dst, nullptr, nullptr};
+    const void *sv[3] = {src, nullptr, nullptr}, *r[3] = {ref, nullptr, nullptr};
+    const Fmt f{true, floor_, neutral, ceil_};
+    return nlm_common(ctx, p, 1, d, sv, r, W, H, stride, f, mem, stream);
+}
+
+int isp_nlm_rgb_i32(isp_bm3d_ctx *ctx, const isp_nlm_params *p, int *dst_r, int *dst_g, int *dst_b,
+                    const int *src_r, const int *src_g, const int *src_b, const int *ref_r, const int *ref_g,
+                    const int *ref_b, int W, int H, size_t stride, int floor_, int ceil_, int mem, void *stream)
+{
+    void *d[3] = {dst_r, dst_g, dst_b};
+    const void *sv[3] = {src_r, src_g, src_b}, *r[3] = {ref_r, ref_g, ref_b};
+    const Fmt f{true, floor_, floor_, ceil_};
+    return nlm_common(ctx, p, 3, d, sv, r, W, H, stride, f, mem, stream);
+}
+
 }  // extern "C"

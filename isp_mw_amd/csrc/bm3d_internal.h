@@ -128,6 +128,7 @@ struct OppArgs {
     int pitch;
     double *y64;               // may be null (matching guide)
     int pitch64;
+    int clipY;                 // clip Y to [0, 1] (ConvertToY's default)
 };
 
 // int32 plane -> float plane: v * gain + offset (RangeConvert), optional clip
@@ -142,6 +143,44 @@ struct ToFloatArgs {
     int clip;
     double lo, hi;
 };
+
+// NL-means weighted averaging of the groups of one match launch.
+struct NlmArgs {
+    int trx, try_;            // tile shape in reference blocks
+    int W, H;
+    int ngx, ngy, bstep;
+    int G;                    // GroupSize: row length of pos / keys
+    const int *cnt;
+    const int *pos;
+    const double *keys;       // distance keys (MSE units), member 0 = the reference block
+    int nplanes;
+    int src_type;             // 0 float, 1 double, 2 int32 samples
+    const void *src[3];
+    size_t spitch;            // of src, in elements
+    int correction;           // WeightedAverage_Correction
+    double exp_mul;           // -1 / strength^2
+    double varN;              // (sigma * range / 255)^2
+    double fix;               // fixed-point scale of the accumulators (a power of two)
+    unsigned long long *num[3];
+    int pitch;                // of num
+    int FW, FH;               // footprint of a tile: (trx - 1) * bstep + 8 by (try_ - 1) * bstep + 8
+};
+
+struct NlmFinalizeArgs {
+    int W, H, pitch, bstep, nplanes;
+    const unsigned long long *num[3];
+    double inv_fix;           // 1 / NlmArgs::fix
+    int out_int;              // 0: float outputs; 1: int32 of val * gain + offset (truncated)
+    double gain, offset;
+    int clip;
+    double lo, hi;
+    void *out[3];
+    size_t out_pitch;
+};
+
+size_t nlm_lds_bytes(const NlmArgs &a);
+hipError_t launch_nlm(const NlmArgs &a, hipStream_t s);
+hipError_t launch_nlm_finalize(const NlmFinalizeArgs &a, hipStream_t s);
 
 size_t match_lds_bytes(const MatchArgs &a);
 size_t filter_lds_bytes(const FilterArgs &a, bool wiener);

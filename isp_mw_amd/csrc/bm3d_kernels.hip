@@ -1065,7 +1065,8 @@ __global__ void opp_kernel(OppArgs a0, const uint32_t *__restrict__ bmap, const 
         G = static_cast<const float *>(a.g)[i];
         B = static_cast<const float *>(a.b)[i];
     }
-    const double Y = (R + G + B) * a.gainY + a.offsetY;
+    double Y = (R + G + B) * a.gainY + a.offsetY;
+    if (a.clipY) Y = Y < 0.0 ? 0.0 : (Y > 1.0 ? 1.0 : Y);
     const size_t o = (size_t)y * a.pitch + x;
     if (a.y64) a.y64[(size_t)y * a.pitch64 + x] = Y;
     if (a.y) a.y[o] = (float)Y;

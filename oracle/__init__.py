@@ -54,6 +54,34 @@ def build() -> None:
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+class NlmParams(ctypes.Structure):
+    """Mirror of NLMeans_Para (include/NLMeans.h:11-22)."""
+
+    _fields_ = [
+        ("correction", ctypes.c_int),
+        ("sigma", ctypes.c_double),
+        ("strength", ctypes.c_double),
+        ("block_size", ctypes.c_int),
+        ("block_step", ctypes.c_int),
+        ("group_size", ctypes.c_int),
+        ("bm_range", ctypes.c_int),
+        ("bm_step", ctypes.c_int),
+        ("thMSE", ctypes.c_double),
+    ]
+
+
+def nlm_params(correction=True, sigma=8.0, strength=None, BlockSize=8, BlockStep=5, GroupSize=16, BMrange=24,
+               BMstep=3, thMSE=None) -> NlmParams:
+    """NLMeans_Para defaults; strength and thMSE follow sigma as NLMeans_IO
+    derives them (include/NLMeans.h:150-151) unless given."""
+    if strength is None:
+        strength = sigma * 5 if correction else sigma * 1.5
+    if thMSE is None:
+        thMSE = sigma * 50 if correction else sigma * 25
+    return NlmParams(int(bool(correction)), float(sigma), float(strength), int(BlockSize), int(BlockStep),
+                     int(GroupSize), int(BMrange), int(BMstep), float(thMSE))
+
+
 def _lib(fast: bool = False) -> ctypes.CDLL:
     name = "liborc_fast.so" if fast else "liborc.so"
     if name in _libs:
@@ -83,6 +111,11 @@ def _lib(fast: bool = False) -> ctypes.CDLL:
                                        ctypes.c_int, ctypes.c_int, ctypes.c_int]
     lib.orc_bm3d_rgb_int.argtypes = [sp, sp, ctypes.c_int, ipp, ipp, ipp, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, ctypes.c_int]
+    np_ = ctypes.POINTER(NlmParams)
+    lib.orc_nlm_plane.argtypes = [np_, dp, dp, dp, ctypes.c_int, ctypes.c_int]
+    lib.orc_nlm_rgb_int.argtypes = [np_, ipp, ipp, ipp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.orc_nlm_plane_int.argtypes = [np_, ip, ip, ip, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int]
     lib.orc_set_threads.argtypes = [ctypes.c_int]
     lib.orc_set_threads.restype = ctypes.c_int
     _libs[name] = lib
@@ -292,4 +325,42 @@ def bm3d_rgb_int(basic, final, src, ref=None, mode: int = 3, floor: int = 0, cei
     rc = _lib(fast).orc_bm3d_rgb_int(ctypes.byref(b), ctypes.byref(f), mode, dp_, sp_, rp, w, h, floor, ceil)
     if rc < 0:
         raise ValueError(f"orc_bm3d_rgb_int failed ({rc})")
+    return np.stack(dst)
+
+
+def nlm_plane(p: NlmParams, src, ref=None) -> np.ndarray:
+    """NLMeans::process_Plane_FL (source/NLMeans.cpp:150-228)."""
+    src = _f64(src)
+    h, w = src.shape
+    r = _f64(ref) if ref is not None else src
+    dst = np.empty_like(src)
+    rc = _lib().orc_nlm_plane(ctypes.byref(p), _dptr(dst), _dptr(src), _dptr(r), w, h)
+    if rc < 0:
+        raise ValueError(f"orc_nlm_plane failed ({rc})")
+    return dst
+
+
+def nlm_plane_int(p: NlmParams, src, ref=None, floor: int = 0, neutral: int = 0, ceil: int = 255) -> np.ndarray:
+    """NLMeans::process_Plane on an int plane (source/NLMeans.cpp:232-257)."""
+    src = np.ascontiguousarray(src, dtype=np.int32)
+    h, w = src.shape
+    r = np.ascontiguousarray(ref, dtype=np.int32) if ref is not None else src
+    dst = np.empty_like(src)
+    rc = _lib().orc_nlm_plane_int(ctypes.byref(p), _iptr(dst), _iptr(src), _iptr(r), w, h, floor, neutral, ceil)
+    if rc < 0:
+        raise ValueError(f"orc_nlm_plane_int failed ({rc})")
+    return dst
+
+
+def nlm_rgb_int(p: NlmParams, src, ref=None, floor: int = 0, ceil: int = 255) -> np.ndarray:
+    """NLMeans::process_Frame on an int RGB frame [3,H,W] (source/NLMeans.cpp:279-375)."""
+    src = [np.ascontiguousarray(c, dtype=np.int32) for c in src]
+    h, w = src[0].shape
+    mk = lambda planes: (ctypes.POINTER(ctypes.c_int32) * 3)(*[_iptr(c) for c in planes])  # noqa: E731
+    rr = [np.ascontiguousarray(c, dtype=np.int32) for c in ref] if ref is not None else None
+    dst = [np.empty((h, w), np.int32) for _ in range(3)]
+    rc = _lib().orc_nlm_rgb_int(ctypes.byref(p), mk(dst), mk(src), mk(rr) if rr is not None else None, w, h,
+                                floor, ceil)
+    if rc < 0:
+        raise ValueError(f"orc_nlm_rgb_int failed ({rc})")
     return np.stack(dst)

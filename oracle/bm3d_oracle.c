@@ -843,3 +843,182 @@ int orc_bm3d_rgb_int(const orc_step_params *basic, const orc_step_params *final_
     free(buf);
     return rc;
 }
+
+/* ------------------------------------------------------------------------ */
+/* NLMeans (include/NLMeans.h:11-64, source/NLMeans.cpp:6-375): the same     */
+/* block matching (thMSE not scaled by normY), a weighted average of each    */
+/* group with weights exp(-key / strength^2), optionally corrected by a      */
+/* stationarity test, and uniform aggregation.                               */
+
+static int nlm_degenerate(const orc_nlm_params *p)
+{
+    /* NLMeans::process_Plane_FL / process_Frame (source/NLMeans.cpp:152-158, 281-287) */
+    return p->strength <= 0 || p->group_size == 1 || p->block_size <= 0 || p->bm_range <= 0 ||
+           p->bm_range < p->bm_step || p->thMSE <= 0;
+}
+
+/* WeightedAverage / WeightedAverage_Correction (source/NLMeans.cpp:6-147) of
+ * one group on one plane (sample range `range`), into blk[B*B]. */
+static void nlm_average(const orc_nlm_params *p, const double *src, int stride, double range, int j, int i,
+                        const int32_t *pos, const double *keys, int N, double *blk)
+{
+    const int B = p->block_size;
+    int G = N;
+    if (p->group_size > 0 && G > p->group_size) G = p->group_size;
+    if (G < 2) {
+        for (int y = 0; y < B; ++y)
+            for (int x = 0; x < B; ++x) blk[y * B + x] = src[(size_t)(j + y) * stride + i + x];
+        return;
+    }
+    double *s1 = (double *)calloc((size_t)B * B * 2, sizeof(double)), *s2 = s1 + B * B;
+    const double exponentMul = -1 / (p->strength * p->strength);
+    double wsum = 0;
+    for (int k = 0; k < G; ++k) {
+        const double w = exp(keys[k] * exponentMul);
+        wsum += w;
+        const int py = pos[k] >> 16, px = pos[k] & 0xFFFF;
+        for (int y = 0; y < B; ++y)
+            for (int x = 0; x < B; ++x) {
+                const double v = src[(size_t)(py + y) * stride + px + x];
+                if (p->correction) {
+                    const double t = v * w;
+                    s1[y * B + x] += t;
+                    s2[y * B + x] += v * t;
+                } else {
+                    s1[y * B + x] += v * w;
+                }
+            }
+    }
+    const double rec = 1.0 / wsum;
+    if (p->correction) {
+        const double sigma = p->sigma * range / 255, VarN = sigma * sigma;
+        for (int y = 0; y < B; ++y)
+            for (int x = 0; x < B; ++x) {
+                const double X = src[(size_t)(j + y) * stride + i + x];
+                const double EX = s1[y * B + x] * rec, VarX = s2[y * B + x] * rec - EX * EX;
+                blk[y * B + x] = VarX > VarN ? X - (VarN / VarX) * (X - EX) : EX;
+            }
+    } else {
+        for (int q = 0; q < B * B; ++q) blk[q] = s1[q] * rec;
+    }
+    free(s1);
+}
+
+/* The grid loop shared by the plane and frame forms: matching on ref (range
+ * 1), nplanes source planes of sample range `range`, num / den sums. */
+static int nlm_sums(const orc_nlm_params *p, int nplanes, const double *const *src, const double *ref, double range,
+                    int w, int h, double *const *num, double *den)
+{
+    const int B = p->block_size;
+    if (w < B || h < B || p->bm_step <= 0 || p->block_step <= 0 || p->group_size < 0 || p->group_size > 4096) return -1;
+    filter_data f;
+    memset(&f, 0, sizeof(f));
+    f.p.block_size = B;
+    f.p.block_step = p->block_step;
+    f.p.group_size = p->group_size;
+    f.p.bm_range = p->bm_range;
+    f.p.bm_step = p->bm_step;
+    f.p.thMSE = p->thMSE;
+    const int cap = window_cap(&f.p);
+    cand *buf = (cand *)malloc(sizeof(cand) * (size_t)cap);
+    int32_t *pos = (int32_t *)malloc(sizeof(int32_t) * (size_t)cap);
+    double *keys = (double *)malloc(sizeof(double) * (size_t)cap), *blk = (double *)malloc(sizeof(double) * B * B);
+    const int right = w - B, bottom = h - B;
+    for (int j = 0;; j += p->block_step) {
+        if (j >= bottom + p->block_step) break;
+        if (j > bottom) j = bottom;
+        for (int i = 0;; i += p->block_step) {
+            if (i >= right + p->block_step) break;
+            if (i > right) i = right;
+            const int n = match_one(&f, ref, w, h, w, j, i, buf, pos, keys);
+            for (int c = 0; c < nplanes; ++c) {
+                nlm_average(p, src[c], w, range, j, i, pos, keys, n, blk);
+                for (int y = 0; y < B; ++y)
+                    for (int x = 0; x < B; ++x) num[c][(size_t)(j + y) * w + i + x] += blk[y * B + x];
+            }
+            for (int y = 0; y < B; ++y)
+                for (int x = 0; x < B; ++x) den[(size_t)(j + y) * w + i + x] += 1.0;
+        }
+    }
+    free(buf);
+    free(pos);
+    free(keys);
+    free(blk);
+    return 0;
+}
+
+int orc_nlm_plane(const orc_nlm_params *p, double *dst, const double *src, const double *ref, int w, int h)
+{
+    const long npx = (long)w * h;
+    if (nlm_degenerate(p)) {
+        memcpy(dst, src, sizeof(double) * (size_t)npx);
+        return 1;
+    }
+    double *num = (double *)calloc((size_t)npx * 2, sizeof(double)), *den = num + npx;
+    double *nums[1] = {num};
+    const double *srcs[1] = {src};
+    const int rc = nlm_sums(p, 1, srcs, ref ? ref : src, 1.0, w, h, nums, den);
+    if (rc == 0)
+        for (long k = 0; k < npx; ++k) dst[k] = num[k] / den[k];
+    free(num);
+    return rc;
+}
+
+int orc_nlm_rgb_int(const orc_nlm_params *p, int32_t *const *dst, const int32_t *const *src,
+                    const int32_t *const *ref, int w, int h, int floor_, int ceil_)
+{
+    const long npx = (long)w * h;
+    if (nlm_degenerate(p)) {
+        for (int c = 0; c < 3; ++c) memcpy(dst[c], src[c], sizeof(int32_t) * (size_t)npx);
+        return 1;
+    }
+    /* ConvertToY of the guide, OPP (include/Conversion.hpp:418-432) */
+    const double sRange = (double)(ceil_ - floor_), gainY = 1.0 / (sRange * 3.0);
+    const double offsetY = -(double)floor_ * 3.0 * gainY + 0.0;
+    const int32_t *const *g = (ref && ref[0]) ? ref : src;
+    double *buf = (double *)calloc((size_t)npx * 8, sizeof(double));
+    double *y = buf, *s[3] = {buf + npx, buf + 2 * npx, buf + 3 * npx};
+    double *num[3] = {buf + 4 * npx, buf + 5 * npx, buf + 6 * npx}, *den = buf + 7 * npx;
+    for (long k = 0; k < npx; ++k) {
+        const double t = ((double)g[0][k] + (double)g[1][k] + (double)g[2][k]) * gainY + offsetY;
+        y[k] = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);  /* clip = true (:487, :502) */
+        for (int c = 0; c < 3; ++c) s[c][k] = (double)src[c][k];
+    }
+    const double *sc[3] = {s[0], s[1], s[2]};
+    const int rc = nlm_sums(p, 3, sc, y, sRange, w, h, num, den);
+    if (rc == 0)
+        for (int c = 0; c < 3; ++c)
+            for (long k = 0; k < npx; ++k) dst[c][k] = (int32_t)(num[c][k] / den[k] + 0.5);
+    free(buf);
+    return rc;
+}
+
+/* NLMeans::process_Plane (source/NLMeans.cpp:232-257): Plane_FL src_data(src)
+ * converts with clip (include/Image_Type.cpp Plane_FL(const Plane&) ->
+ * RangeConvert(dst, src, clip = true), include/Conversion.hpp:277), then the
+ * float filter and RangeConvert back with +0.5 rounding.  The output clip of
+ * orc_range_to_int changes nothing here: outputs are convex combinations of
+ * clipped samples. */
+int orc_nlm_plane_int(const orc_nlm_params *p, int32_t *dst, const int32_t *src, const int32_t *ref, int w, int h,
+                      int floor_, int neutral, int ceil_)
+{
+    const long npx = (long)w * h;
+    if (nlm_degenerate(p)) {
+        memcpy(dst, src, sizeof(int32_t) * (size_t)npx);
+        return 1;
+    }
+    double *buf = (double *)malloc(sizeof(double) * (size_t)npx * 3), *s = buf, *r = buf + npx, *d = buf + 2 * npx;
+    const int chroma = q_chroma(floor_, neutral);
+    const double lo = chroma ? -0.5 : 0.0, hi = chroma ? 0.5 : 1.0;
+    const int32_t *in[2] = {src, ref && ref != src ? ref : NULL};
+    double *out[2] = {s, r};
+    for (int j = 0; j < 2; ++j) {
+        if (!in[j]) continue;
+        orc_range_to_float(out[j], in[j], npx, floor_, neutral, ceil_);
+        for (long k = 0; k < npx; ++k) out[j][k] = out[j][k] < lo ? lo : (out[j][k] > hi ? hi : out[j][k]);
+    }
+    const int rc = orc_nlm_plane(p, d, s, in[1] ? r : s, w, h);
+    if (rc == 0) orc_range_to_int(dst, d, npx, floor_, neutral, ceil_);
+    free(buf);
+    return rc;
+}

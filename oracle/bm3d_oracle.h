@@ -93,6 +93,27 @@ int orc_bm3d_rgb_int(const orc_step_params *basic, const orc_step_params *final_
                      int32_t *const *dst, const int32_t *const *src, const int32_t *const *ref,
                      int w, int h, int floor_, int ceil_);
 
+/* NLMeans (include/NLMeans.h:11-22 NLMeans_Para): the same block matching
+ * with thMSE not scaled by normY, weights exp(-key / strength^2), and the
+ * stationarity correction when `correction` is set. */
+typedef struct orc_nlm_params {
+    int correction;
+    double sigma, strength;
+    int block_size, block_step, group_size, bm_range, bm_step;
+    double thMSE;
+} orc_nlm_params;
+
+/* NLMeans::process_Plane_FL (source/NLMeans.cpp:150-228); ref may be NULL (src). */
+int orc_nlm_plane(const orc_nlm_params *p, double *dst, const double *src, const double *ref, int w, int h);
+/* NLMeans::process_Frame on an int RGB frame (source/NLMeans.cpp:279-375):
+ * matching on the OPP luma of ref (or src), averaging of the R, G, B samples,
+ * output (int)(num/den + 0.5). */
+int orc_nlm_rgb_int(const orc_nlm_params *p, int32_t *const *dst, const int32_t *const *src,
+                    const int32_t *const *ref, int w, int h, int floor_, int ceil_);
+/* NLMeans::process_Plane on an int plane (source/NLMeans.cpp:232-257). */
+int orc_nlm_plane_int(const orc_nlm_params *p, int32_t *dst, const int32_t *src, const int32_t *ref, int w, int h,
+                      int floor_, int neutral, int ceil_);
+
 /* Unnormalised 3D DCT on an n0 x n1 x n2 array (n2 fastest): FFTW REDFT10
  * forward (inverse = 0) or REDFT01 backward (inverse = 1) on every axis. */
 void orc_dct3d(double *data, int n0, int n1, int n2, int inverse);

@@ -33,15 +33,16 @@ def grid(n, s):
     return out
 
 
-def brute_groups(p, ref):
+def brute_groups(p, ref, norm=NORM[0]):
     """All candidates of the clipped window (BMstep grid through the reference
     position), SSE <= thSSE and != 0, sorted by (key, scan order), self first,
-    truncated to GroupSize.  Returns {(j, i): [(key, y, x), ...]}."""
+    truncated to GroupSize.  Returns {(j, i): [(key, y, x), ...]}.  thMSE is
+    scaled by `norm` (BM3D's luma OPP norm; NL-means passes 1)."""
     ref = np.asarray(ref, np.float64)
     h, w = ref.shape
     st = p.BMstep
     R = p.BMrange // st * st
-    thMSE = p.thMSE * NORM[0]
+    thMSE = p.thMSE * norm
     mse2sse = B * B / 255.0 ** 2
     th = thMSE * mse2sse
     out = {}
@@ -113,3 +114,37 @@ def kernel(p, src_planes, ref_planes, wiener):
         else:
             out.append(np.asarray(s, np.float64).copy())
     return out
+
+
+def nlm(p, src_planes, ref, value_range=1.0):
+    """NL-means from its definition (source/NLMeans.cpp:6-375) on the groups
+    of brute_groups with the unscaled thMSE: per group, weights
+    w_k = exp(-key_k / strength^2) and the weighted moments
+    EX = sum w X / sum w, VarX = sum w X^2 / sum w - EX^2 as array reductions;
+    the block estimate is EX (no correction) or, with correction,
+    X - VarN/VarX (X - EX) where VarX > VarN (VarN = (sigma * range/255)^2)
+    and EX elsewhere; a group of one returns the source block.  Every grid
+    block adds its estimate with weight 1.  Returns (num, den) per plane."""
+    groups = brute_groups(p, ref, norm=1.0)
+    h, w = np.asarray(ref).shape
+    src_planes = [np.asarray(s_, np.float64) for s_ in src_planes]
+    nums = [np.zeros((h, w)) for _ in src_planes]
+    den = np.zeros((h, w))
+    varN = (p.sigma * value_range / 255.0) ** 2
+    for (j, i), g in groups.items():
+        wts = np.exp(-np.array([k for k, _, _ in g]) / p.strength ** 2)
+        for c, s_ in enumerate(src_planes):
+            X = s_[j:j + B, i:i + B]
+            if len(g) < 2:
+                est = X
+            else:
+                blocks = np.stack([s_[y:y + B, x:x + B] for _, y, x in g])
+                EX = np.tensordot(wts, blocks, 1) / wts.sum()
+                if p.correction:
+                    VarX = np.tensordot(wts, blocks ** 2, 1) / wts.sum() - EX ** 2
+                    est = np.where(VarX > varN, X - varN / np.where(VarX > 0, VarX, 1) * (X - EX), EX)
+                else:
+                    est = EX
+            nums[c][j:j + B, i:i + B] += est
+        den[j:j + B, i:i + B] += 1
+    return nums, den
