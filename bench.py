@@ -416,6 +416,129 @@ def bench_full(args) -> int:
     return 0
 
 
+def nlm_window(pos: np.ndarray, n: int, R: int, s: int) -> np.ndarray:
+    # clipped search window with BMstep s (include/Block.h:716-751): k*s offsets
+    # with |k*s| <= R and 0 <= pos + k*s <= n - B
+    last = n - B
+    return np.minimum(R // s, (last - pos) // s) + np.minimum(R // s, pos // s) + 1
+
+
+def bench_nlm(args) -> int:
+    """NL-means with NLMeans_Para's defaults (correction, sigma 8: strength 40,
+    thMSE 400, BlockStep 5, GroupSize 16, BMrange 24, BMstep 3) of a 1920x1080
+    RGB frame of 16-bit samples (8-bit values * 257, the reference's
+    ImageReader frame), AWGN sigma 8/255, frames resident in HBM."""
+    import torch
+    from isp_mw_amd import NLMeans, NLMeans_Para, synth
+
+    sigma = 8.0
+    para = NLMeans_Para(correction=True, sigma=sigma)
+    clean, noisy = synth.rgb_pair(H, W, sigma, SEED)
+    frame = np.clip(np.rint(noisy * 255), 0, 255).astype(np.int32) * 257
+    dev = torch.device("cuda:0")
+    x = torch.from_numpy(frame).to(dev)
+
+    def timed(prec, steps, warmup):
+        f = NLMeans(para, precision=prec)
+        y = torch.empty_like(x)
+        for _ in range(warmup):
+            f.process(y, x, floor=0, ceil=65535)
+        torch.cuda.synchronize()
+        f.set_timing(True)
+        f.timing(reset=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            f.process(y, x, floor=0, ceil=65535)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        st = f.timing()
+        f.set_timing(False)
+        return dt, st, y.cpu().numpy()
+
+    dt, stages, out = timed(args.precision, args.steps, args.warmup)
+    value = W * H / dt / 1e6
+    c16 = clean * 255 * 257
+    psnr = lambda a: 10 * np.log10(65535.0 ** 2 / np.mean((np.asarray(a, np.float64) - c16) ** 2))  # noqa: E731
+    other = None
+    if args.precision != "fp32":
+        dt32, st32, out32 = timed("fp32", max(2, args.steps // 2), 1)
+        other = {"form": "exact (FP64 distances)", "value": round(W * H / dt32 / 1e6, 3),
+                 "ms_per_step": round(dt32 * 1e3, 3), "psnr_db": round(psnr(out32), 4),
+                 "stages_ms": {k: round(v[0] / v[1], 4) for k, v in st32.items() if v[1]}}
+    gx, gy = grid(W, para.BlockStep), grid(H, para.BlockStep)
+    refs = gx.size * gy.size
+    R = para.BMrange // para.BMstep * para.BMstep
+    cand = float(nlm_window(gx, W, R, para.BMstep).sum() * nlm_window(gy, H, R, para.BMstep).sum() - refs)
+    match_flops = cand * B * B * 3            # difference, square, add per pixel per candidate (luma)
+    # average: each pixel's 3 samples read once, 3 fixed-point sums written once,
+    # the group lists (count, G positions, G keys) read once
+    avg_bytes = W * H * 3 * 4 + W * H * 3 * 8 + refs * (4 + para.GroupSize * 12)
+    dom = max(("nlm_match", "nlm_average"), key=lambda k: stages[k][0])
+    ms = stages[dom][0] / stages[dom][1]
+    if dom == "nlm_match":
+        achieved = match_flops / (ms * 1e-3) / 1e12
+        peak = PEAK_FP64_TFLOPS if args.precision == "fp32" else PEAK_FP32_TFLOPS
+        roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(achieved / peak, 4), "traffic": None, "kernel": dom, "ms_per_launch": round(ms, 4),
+                "engine": "fp64 VALU" if args.precision == "fp32" else "fp32 VALU",
+                "algorithmic_flops_per_launch": match_flops,
+                "flops_basis": "3 flop per pixel per candidate, clipped BMstep-3 windows, luma"}
+    else:
+        achieved = avg_bytes / (ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None, "kernel": dom,
+                "ms_per_launch": round(ms, 4), "algorithmic_bytes_per_launch": avg_bytes}
+    res = {
+        "metric": "Mpix/s NL-means 1080p RGB 16-bit, NLMeans_Para defaults",
+        "value": round(value, 3),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp32" else "f32 matching + f64 averaging",
+        "data": "synthetic (seeded rectangles+gradient+gratings, mixed RGB, AWGN sigma 8/255, 8-bit * 257)",
+        "config": {"workload": "NL-means, 1920x1080 RGB int32 frame (Floor 0, Ceil 65535)",
+                   "correction": True, "sigma": sigma, "strength": para.strength, "thMSE": para.thMSE,
+                   "block": 8, "block_step": para.BlockStep, "group_size": para.GroupSize,
+                   "bm_range": para.BMrange, "bm_step": para.BMstep, "precision": args.precision},
+        "roofline": roof,
+        "quality": {"psnr_noisy_db": round(psnr(frame), 4), "psnr_out_db": round(psnr(out), 4)},
+        "stages": {k: {"ms_per_launch": round(v[0] / v[1], 4) if v[1] else None, "launches": v[1]}
+                   for k, v in stages.items()},
+        "exact_form": other,
+        "work": {"refs": refs, "match_candidates": cand, "match_flops": match_flops,
+                 "average_algorithmic_bytes": avg_bytes},
+    }
+    if not args.no_cpu:
+        import oracle
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        oracle.set_threads(threads)
+        op = oracle.nlm_params(True, sigma)
+        hc = 64
+        t0 = time.perf_counter()
+        oracle.nlm_rgb_int(op, frame[:, :hc], floor=0, ceil=65535)
+        rate = W * hc / max(time.perf_counter() - t0, 1e-6)
+        hc = int(min(H, max(64, rate * args.cpu_seconds / W)))
+        times = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            oracle.nlm_rgb_int(op, frame[:, :hc], floor=0, ceil=65535)
+            times.append(time.perf_counter() - t0)
+        t = min(times)
+        res["cpu_baseline"] = {"value": round(W * hc / t / 1e6, 4), "unit": "Mpix/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"oracle (FP64 C restatement, {threads} threads over grid rows) NL-means "
+                                         f"on the top {W}x{hc} crop of the same frame, best of 2 runs of {t:.2f} s"}
+        res["gpu_over_cpu"] = round(value / res["cpu_baseline"]["value"], 1)
+    print(json.dumps(res), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -429,10 +552,12 @@ def main() -> int:
     ap.add_argument("--tiles", default="", help="mx,my,fx,fy: given tile shapes of the timed form (no autotuning; "
                                                 "for profiler runs, whose averages the tuner's trials would mix)")
     ap.add_argument("--cpu-seconds", type=float, default=4.0, help="target seconds per CPU baseline run")
-    ap.add_argument("--workload", default="rgb1080", choices=("rgb1080", "batch64", "pilot1080", "full4k"),
+    ap.add_argument("--workload", default="rgb1080",
+                    choices=("rgb1080", "batch64", "pilot1080", "full4k", "nlm1080"),
                     help="rgb1080: the headline 1080p RGB frame; batch64: 64 uneven grey images as one batch; "
                          "pilot1080: train the learned pilot, then pilot + Wiener on 1080p grey; "
-                         "full4k: 8 x 4K RGB frames, pilot guides + one batched FP16 Wiener launch")
+                         "full4k: 8 x 4K RGB frames, pilot guides + one batched FP16 Wiener launch; "
+                         "nlm1080: NL-means of a 1080p 16-bit RGB frame")
     ap.add_argument("--pilot-steps", type=int, default=1250, help="training steps of the pilot workload")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy (3 launches): the known-traffic "
@@ -458,6 +583,8 @@ def main() -> int:
         return bench_pilot(args)
     if args.workload == "full4k":
         return bench_full(args)
+    if args.workload == "nlm1080":
+        return bench_nlm(args)
 
     import torch
     from isp_mw_amd import BM3D, synth

@@ -15,6 +15,8 @@
 // converts with RangeConvert / the OPP matrices and rounds back exactly as the
 // reference's process_Plane / process_Frame do.  Frame_FL holds three float RGB
 // planes in [0,1] with the same process_Frame semantics.
+// NLMeans_Para / NLMeans (include/NLMeans.h) run NL-means on the same
+// matching kernel, on Plane_FL, Plane and Frame.
 // Errors throw isp_bm3d::Error (the reference prints and exits instead).
 #pragma once
 
@@ -24,6 +26,7 @@
 #include <vector>
 
 #include "isp_bm3d.h"
+#include "isp_nlm.h"
 
 namespace isp_bm3d {
 
@@ -356,6 +359,112 @@ private:
         p.final_ = para.final.to_c();
         return p;
     }
+};
+
+// NLMeans_Para (include/NLMeans.h:11-22): strength and thMSE follow sigma and
+// correction unless set; Derive() recomputes them as NLMeans_IO does (:148-149).
+struct NLMeans_Para {
+    bool correction = true;
+    double sigma = 8.0;
+    double strength = 40.0;
+    int BlockSize = 8, BlockStep = 5, GroupSize = 16, BMrange = 24, BMstep = 3;
+    double thMSE = 400.0;
+
+    NLMeans_Para() = default;
+    NLMeans_Para(bool corr, double s) : correction(corr), sigma(s) { Derive(); }
+    void Derive()
+    {
+        strength = correction ? sigma * 5 : sigma * 1.5;
+        thMSE = correction ? sigma * 50 : sigma * 25;
+    }
+    isp_nlm_params to_c() const
+    {
+        isp_nlm_params p{};
+        p.correction = correction;
+        p.sigma = sigma;
+        p.strength = strength;
+        p.block_size = BlockSize;
+        p.block_step = BlockStep;
+        p.group_size = GroupSize;
+        p.bm_range = BMrange;
+        p.bm_step = BMstep;
+        p.thMSE = thMSE;
+        return p;
+    }
+};
+
+// NLMeans (include/NLMeans.h:25-59): FilterIF2-style process / operator() on
+// Plane_FL (process_Plane_FL), Plane (process_Plane) and RGB Frame
+// (process_Frame, matching on the OPP luma of ref).
+class NLMeans {
+public:
+    explicit NLMeans(const NLMeans_Para &para = NLMeans_Para(), int device = 0) : para_(para.to_c())
+    {
+        isp_bm3d_params p;
+        check(isp_bm3d_params_default(&p, "fast"));
+        check(isp_bm3d_create(&ctx_, &p, device));
+    }
+    NLMeans(const NLMeans &) = delete;
+    NLMeans &operator=(const NLMeans &) = delete;
+    ~NLMeans() { isp_bm3d_destroy(ctx_); }
+
+    Plane_FL &process(Plane_FL &dst, const Plane_FL &src, const Plane_FL &ref)
+    {
+        same(dst, src);
+        same(ref, src);
+        check(isp_nlm_plane_f32(ctx_, &para_, dst.data(), src.data(), ref.data(), src.Width(), src.Height(),
+                                src.Stride(), ISP_BM3D_MEM_HOST, nullptr),
+              ctx_);
+        return dst;
+    }
+    Plane &process(Plane &dst, const Plane &src, const Plane &ref)
+    {
+        same(dst, src);
+        same(ref, src);
+        check(isp_nlm_plane_i32(ctx_, &para_, dst.data(), src.data(), ref.data(), src.Width(), src.Height(),
+                                src.Stride(), src.Floor(), src.Neutral(), src.Ceil(), ISP_BM3D_MEM_HOST, nullptr),
+              ctx_);
+        return dst;
+    }
+    Frame &process(Frame &dst, const Frame &src, const Frame &ref)
+    {
+        same(dst.R(), src.R());
+        same(ref.R(), src.R());
+        const Plane &q = src.R();
+        check(isp_nlm_rgb_i32(ctx_, &para_, dst.R().data(), dst.G().data(), dst.B().data(), src.R().data(),
+                              src.G().data(), src.B().data(), ref.R().data(), ref.G().data(), ref.B().data(),
+                              src.Width(), src.Height(), q.Stride(), q.Floor(), q.Ceil(), ISP_BM3D_MEM_HOST, nullptr),
+              ctx_);
+        return dst;
+    }
+    template <typename T>
+    T &process(T &dst, const T &src) { return process(dst, src, src); }
+    Plane_FL operator()(const Plane_FL &src) { return run(Plane_FL(src.Width(), src.Height()), src, src); }
+    Plane_FL operator()(const Plane_FL &src, const Plane_FL &ref)
+    {
+        return run(Plane_FL(src.Width(), src.Height()), src, ref);
+    }
+    Plane operator()(const Plane &src) { return run(src.like(), src, src); }
+    Plane operator()(const Plane &src, const Plane &ref) { return run(src.like(), src, ref); }
+    Frame operator()(const Frame &src) { return run(src.like(), src, src); }
+    Frame operator()(const Frame &src, const Frame &ref) { return run(src.like(), src, ref); }
+    isp_bm3d_ctx *context() { return ctx_; }
+
+private:
+    template <typename T>
+    T run(T dst, const T &src, const T &ref)
+    {
+        process(dst, src, ref);
+        return dst;
+    }
+    template <typename P>
+    static void same(const P &a, const P &b)
+    {
+        if (a.Width() != b.Width() || a.Height() != b.Height())
+            throw Error(ISP_BM3D_E_ARG, "planes of different sizes");
+    }
+    isp_nlm_params para_;
+    isp_bm3d_ctx *ctx_ = nullptr;
 };
 
 }  // namespace isp_bm3d

@@ -1,25 +1,34 @@
-// isp_mw - command-line front end of the BM3D filter, with the flags and file
-// behaviour of ISP_MW's `--bm3d` (Filtering, source/ISP_MW.cpp:201-289;
-// FilterIO, include/Filter.h:220-340; BM3D_IO, include/BM3D.h:363-526), on
+// isp_mw - command-line front end of the BM3D and NL-means filters, with the
+// flags and file behaviour of ISP_MW's `--bm3d` and `--nlm` (Filtering,
+// source/ISP_MW.cpp:201-289; FilterIO, include/Filter.h:220-340; BM3D_IO,
+// include/BM3D.h:363-526; NLMeans_IO, include/NLMeans.h:62-171), on
 // libisp_bm3d.so and without OpenCV.
 //
 //   isp_mw --bm3d <input> [-T tag] [-F format] [-P profile] [--ref path]
 //          [-S sigma]... [-BS1 n] [-BSP1 n] [-GS1 n] [-MR1 n] [-MS1 n] [-TH1 x] [-LD x]
 //          [-BS2 n] [-BSP2 n] [-GS2 n] [-MR2 n] [-MS2 n] [-TH2 x]
 //          [-D device] [--precision fp32|fp16|bf16] [--print-params]
+//   isp_mw --nlm|--nlmeans|--nonlocalmeans <input> [-T tag] [-F format] [--ref path]
+//          [-C true|false] [-S sigma] [-H strength] [-BS n] [-BSP n] [-GS n] [-MR n] [-MS n]
+//          [-TH x] [-D device] [--precision fp32|fp16|bf16] [--print-params]
 //
-// As the reference: the profile defaults to "fast"; -S may repeat (one value
-// per plane, the last one repeated to three) and sets both steps; thMSE
+// BM3D as the reference: the profile defaults to "fast"; -S may repeat (one
+// value per plane, the last one repeated to three) and sets both steps; thMSE
 // follows thMSE_Default unless -TH1 / -TH2 give it; the output is
 // <input dir>/<input stem><tag><format>, tag ".BM3D", format ".png".
+// NL-means as the reference: strength and thMSE follow the final sigma and
+// correction (5 / 50 sigma, 1.5 / 25 sigma without correction) unless -H /
+// -TH give them; tag ".NLMeans".  The filter name is matched case-blind.
 // 8-bit images become 16-bit RGB frames (value * 257, grey replicated to
 // R, G, B, include/ImageIO.h ImageReader) filtered by the integer Frame path
 // and written as 8-bit RGB with RoundDiv(value, 257) (ImageWriter).
 // Files: PNG (8/16-bit grey, grey+alpha, RGB, RGBA, palette; not interlaced),
-// PPM/PGM (P5/P6, 8/16-bit) and PFM (float RGB or grey, filtered as a float
-// frame in [0, 1] and written as PFM).  16-bit PNG / PNM samples are kept as
+// PPM/PGM (P5/P6, 8/16-bit) and PFM (float RGB or grey, filtered by BM3D as a
+// float frame in [0, 1] and written as PFM; NL-means filters integer frames
+// only, as the reference's Frame).  16-bit PNG / PNM samples are kept as
 // 16-bit values (the reference's OpenCV reader reduces them to 8 bits).
-// The extra flags: -D the HIP device, --precision the transform precision,
+// The extra flags: -D the HIP device, --precision the transform precision
+// (NL-means: FP64 distances for fp32, FP32 distances otherwise),
 // --print-params the parsed parameters as JSON without filtering, --copy the
 // file conversion alone (read, 16-bit frame, write) without filtering.
 #include <zlib.h>
@@ -38,6 +47,7 @@
 #include <vector>
 
 #include "isp_bm3d.h"
+#include "isp_nlm.h"
 
 namespace {
 
@@ -354,11 +364,13 @@ Image load(const std::string &path)
 
 // ---- arguments (BM3D_IO::arguments_process, include/BM3D.h:372-505) ----
 struct Options {
+    std::string filter = "--bm3d";
     std::string input, ref, tag = ".BM3D", format = ".png", profile = "fast", precision = "fp32";
     int device = 0;
     bool print_params = false;
     bool copy_only = false;   // read, convert and write without filtering (no GPU)
     isp_bm3d_params para{};
+    isp_nlm_params nlm{};
 };
 
 int to_int(const std::string &opt, const std::vector<std::string> &a, size_t &i)
@@ -377,6 +389,15 @@ std::string to_str(const std::string &opt, const std::vector<std::string> &a, si
 {
     if (++i >= a.size()) die("No argument specified for option " + opt + "!");
     return a[i];
+}
+
+// Args::GetPara(int&, bool&) (include/Args.h:167-190): "true" / "false", any case
+bool to_bool(const std::string &opt, const std::vector<std::string> &a, size_t &i)
+{
+    const std::string v = lower(to_str(opt, a, i));
+    if (v == "true") return true;
+    if (v == "false") return false;
+    die("Invalid argument specified for option " + opt + ", must be \"true\" or \"false\"!");
 }
 
 Options parse(const std::vector<std::string> &a)
@@ -433,6 +454,44 @@ Options parse(const std::vector<std::string> &a)
     return o;
 }
 
+// NLMeans_IO::arguments_process (include/NLMeans.h:76-150) and FilterIO's -T / -F
+Options parse_nlm(const std::vector<std::string> &a)
+{
+    Options o;
+    o.filter = "--nlm";
+    o.tag = ".NLMeans";
+    isp_nlm_params_default(&o.nlm, 1, 8.0);
+    isp_nlm_params &p = o.nlm;
+    bool strength_def = false, th_def = false;
+    for (size_t i = 0; i < a.size(); ++i) {
+        const std::string &s = a[i];
+        if (s == "-T" || s == "--tag") o.tag = to_str(s, a, i);
+        else if (s == "-F" || s == "--format") o.format = to_str(s, a, i);
+        else if (s == "--ref") o.ref = to_str(s, a, i);
+        else if (s == "-C" || s == "--correction") p.correction = to_bool(s, a, i);
+        else if (s == "-S" || s == "--sigma") p.sigma = to_double(s, a, i);
+        else if (s == "-H" || s == "--strength") p.strength = to_double(s, a, i), strength_def = true;
+        else if (s == "-BS" || s == "--BlockSize") p.block_size = to_int(s, a, i);
+        else if (s == "-BSP" || s == "--BlockStep") p.block_step = to_int(s, a, i);
+        else if (s == "-GS" || s == "--GroupSize") p.group_size = to_int(s, a, i);
+        else if (s == "-MR" || s == "--BMrange") p.bm_range = to_int(s, a, i);
+        else if (s == "-MS" || s == "--BMstep") p.bm_step = to_int(s, a, i);
+        else if (s == "-TH" || s == "--thMSE") p.thMSE = to_double(s, a, i), th_def = true;
+        else if (s == "-D" || s == "--device") o.device = to_int(s, a, i);
+        else if (s == "--precision") o.precision = to_str(s, a, i);
+        else if (s == "--print-params") o.print_params = true;
+        else if (s == "--copy") o.copy_only = true;
+        else if (!s.empty() && s[0] == '-') die("unknown option " + s);
+        else o.input = s;
+    }
+    // derived after parsing, from the final sigma and correction (:148-149)
+    if (!strength_def) p.strength = p.correction ? p.sigma * 5 : p.sigma * 1.5;
+    if (!th_def) p.thMSE = p.correction ? p.sigma * 50 : p.sigma * 25;
+    if (o.input.empty() && !o.print_params) die("no input image given");
+    if (o.format.empty() || o.format[0] != '.') o.format = "." + o.format;
+    return o;
+}
+
 std::string output_path(const Options &o)
 {
     const size_t slash = o.input.find_last_of("/\\");
@@ -445,6 +504,17 @@ std::string output_path(const Options &o)
 
 void print_params(const Options &o)
 {
+    if (o.filter != "--bm3d") {
+        const isp_nlm_params &p = o.nlm;
+        std::cout << "{\"filter\": \"nlm\", \"correction\": " << (p.correction ? "true" : "false")
+                  << ", \"sigma\": " << p.sigma << ", \"strength\": " << p.strength << ", \"BlockSize\": "
+                  << p.block_size << ", \"BlockStep\": " << p.block_step << ", \"GroupSize\": " << p.group_size
+                  << ", \"BMrange\": " << p.bm_range << ", \"BMstep\": " << p.bm_step << ", \"thMSE\": " << p.thMSE
+                  << ", \"input\": \"" << o.input << "\", \"ref\": \"" << o.ref << "\", \"output\": \""
+                  << (o.input.empty() ? "" : output_path(o)) << "\", \"precision\": \"" << o.precision << "\"}"
+                  << std::endl;
+        return;
+    }
     auto step = [](const isp_bm3d_step_params &p) {
         std::ostringstream s;
         s << "{\"sigma\": [" << p.sigma[0] << ", " << p.sigma[1] << ", " << p.sigma[2] << "], \"BlockSize\": "
@@ -477,29 +547,26 @@ int write_out(const Options &o, Image &out)
     return 0;
 }
 
-int run_bm3d(const Options &o)
+int precision_code(const std::string &p)
 {
-    Image src = load(o.input);
-    Image ref;
-    const bool has_ref = !o.ref.empty();
-    if (has_ref) {
-        ref = load(o.ref);
-        if (ref.w != src.w || ref.h != src.h || (ref.depth == 32) != (src.depth == 32))
-            die("the reference image differs from the input in size or type");
-    }
-    if (o.copy_only) {
-        Image out = src;
-        if (out.depth != 32) {
-            const int scale = out.depth == 8 ? 257 : 1;
-            for (auto &v : out.px) v *= scale;
-        }
-        return write_out(o, out);
-    }
+    if (p == "fp32") return ISP_BM3D_PREC_FP32;
+    if (p == "fp16") return ISP_BM3D_PREC_FP16;
+    if (p == "bf16") return ISP_BM3D_PREC_BF16;
+    die("unknown precision " + p);
+}
+
+// ImageReader (include/ImageIO.h): 8-bit samples * 257 in a 16-bit frame (Floor 0, Ceil 65535)
+void to_frame16(Image &im)
+{
+    const int scale = im.depth == 8 ? 257 : 1;
+    for (auto &v : im.px) v *= scale;
+}
+
+int run_bm3d(const Options &o, Image &src, Image &ref, bool has_ref)
+{
     isp_bm3d_ctx *ctx = nullptr;
+    const int prec = precision_code(o.precision);
     if (isp_bm3d_create(&ctx, &o.para, o.device) != ISP_BM3D_OK) die(isp_bm3d_last_error(nullptr));
-    const int prec = o.precision == "fp16" ? ISP_BM3D_PREC_FP16 : o.precision == "bf16" ? ISP_BM3D_PREC_BF16
-                                                                                       : ISP_BM3D_PREC_FP32;
-    if (o.precision != "fp32" && o.precision != "fp16" && o.precision != "bf16") die("unknown precision " + o.precision);
     int rc = isp_bm3d_set_precision(ctx, prec);
     const size_t np = (size_t)src.w * src.h;
     Image out = src;
@@ -514,11 +581,8 @@ int run_bm3d(const Options &o)
         rc = isp_bm3d_rgb_f32(ctx, d[0], d[1], d[2], s[0], s[1], s[2], r[0], r[1], r[2], src.w, src.h, (size_t)src.w,
                               ISP_BM3D_BOTH, ISP_BM3D_MEM_HOST, nullptr);
     } else if (!rc) {
-        // ImageReader: 8-bit samples * 257 in a 16-bit frame (Floor 0, Ceil 65535)
-        const int scale = src.depth == 8 ? 257 : 1;
-        for (auto &v : src.px) v *= scale;
-        if (has_ref)
-            for (auto &v : ref.px) v *= (ref.depth == 8 ? 257 : 1);
+        to_frame16(src);
+        if (has_ref) to_frame16(ref);
         int32_t *d[3], *s[3], *r[3];
         for (int c = 0; c < 3; ++c) {
             d[c] = out.px.data() + c * np;
@@ -537,6 +601,59 @@ int run_bm3d(const Options &o)
     return write_out(o, out);
 }
 
+// NLMeans_IO::process (include/NLMeans.h:153-166): NLMeans::process_Frame on
+// the 16-bit frame, guided by --ref when given
+int run_nlm(const Options &o, Image &src, Image &ref, bool has_ref)
+{
+    if (src.depth == 32) die("NL-means filters integer frames (PNG / PNM), not float images");
+    isp_bm3d_params bp{};
+    isp_bm3d_params_default(&bp, "fast");
+    const int prec = precision_code(o.precision);
+    isp_bm3d_ctx *ctx = nullptr;
+    if (isp_bm3d_create(&ctx, &bp, o.device) != ISP_BM3D_OK) die(isp_bm3d_last_error(nullptr));
+    int rc = isp_bm3d_set_precision(ctx, prec);
+    to_frame16(src);
+    if (has_ref) to_frame16(ref);
+    const size_t np = (size_t)src.w * src.h;
+    Image out = src;
+    if (!rc) {
+        int32_t *d[3], *s[3], *r[3];
+        for (int c = 0; c < 3; ++c) {
+            d[c] = out.px.data() + c * np;
+            s[c] = src.px.data() + c * np;
+            r[c] = has_ref ? ref.px.data() + c * np : nullptr;
+        }
+        rc = isp_nlm_rgb_i32(ctx, &o.nlm, d[0], d[1], d[2], s[0], s[1], s[2], r[0], r[1], r[2], src.w, src.h,
+                             (size_t)src.w, 0, 65535, ISP_BM3D_MEM_HOST, nullptr);
+    }
+    if (rc) {
+        const std::string m = isp_bm3d_last_error(ctx);
+        isp_bm3d_destroy(ctx);
+        die(m);
+    }
+    isp_bm3d_destroy(ctx);
+    return write_out(o, out);
+}
+
+// FilterIO::operator() (include/Filter.h:300-340): read, filter, write
+int run(const Options &o)
+{
+    Image src = load(o.input);
+    Image ref;
+    const bool has_ref = !o.ref.empty();
+    if (has_ref) {
+        ref = load(o.ref);
+        if (ref.w != src.w || ref.h != src.h || (ref.depth == 32) != (src.depth == 32))
+            die("the reference image differs from the input in size or type");
+    }
+    if (o.copy_only) {
+        Image out = src;
+        if (out.depth != 32) to_frame16(out);
+        return write_out(o, out);
+    }
+    return o.filter == "--bm3d" ? run_bm3d(o, src, ref, has_ref) : run_nlm(o, src, ref, has_ref);
+}
+
 }  // namespace
 
 int main(int argc, char **argv)
@@ -546,18 +663,20 @@ int main(int argc, char **argv)
             std::cout << "Not enough arguments specified.\n";
             return 0;
         }
+        // Filtering (source/ISP_MW.cpp:201-289): the filter name, lower-cased
         const std::string filter = lower(argv[1]);
-        if (filter != "--bm3d") {
-            std::cerr << "isp_mw: only --bm3d is part of this build\n";
+        const bool nlm = filter == "--nlm" || filter == "--nlmeans" || filter == "--nonlocalmeans";
+        if (filter != "--bm3d" && !nlm) {
+            std::cerr << "isp_mw: only --bm3d and --nlm are part of this build\n";
             return 1;
         }
         const std::vector<std::string> args(argv + 2, argv + argc);
-        const Options o = parse(args);
+        const Options o = nlm ? parse_nlm(args) : parse(args);
         if (o.print_params) {
             print_params(o);
             return 0;
         }
-        return run_bm3d(o);
+        return run(o);
     } catch (const std::exception &e) {
         std::cerr << "isp_mw: " << e.what() << std::endl;
         return 2;

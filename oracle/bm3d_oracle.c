@@ -905,7 +905,10 @@ static void nlm_average(const orc_nlm_params *p, const double *src, int stride, 
 }
 
 /* The grid loop shared by the plane and frame forms: matching on ref (range
- * 1), nplanes source planes of sample range `range`, num / den sums. */
+ * 1), nplanes source planes of sample range `range`, num / den sums.  Groups
+ * and their block estimates are formed in parallel over grid rows; the sums
+ * are then taken in the reference's raster order, so the result does not
+ * depend on the thread count. */
 static int nlm_sums(const orc_nlm_params *p, int nplanes, const double *const *src, const double *ref, double range,
                     int w, int h, double *const *num, double *den)
 {
@@ -920,30 +923,51 @@ static int nlm_sums(const orc_nlm_params *p, int nplanes, const double *const *s
     f.p.bm_step = p->bm_step;
     f.p.thMSE = p->thMSE;
     const int cap = window_cap(&f.p);
-    cand *buf = (cand *)malloc(sizeof(cand) * (size_t)cap);
-    int32_t *pos = (int32_t *)malloc(sizeof(int32_t) * (size_t)cap);
-    double *keys = (double *)malloc(sizeof(double) * (size_t)cap), *blk = (double *)malloc(sizeof(double) * B * B);
+    /* reference grid (source/NLMeans.cpp:179-201) */
     const int right = w - B, bottom = h - B;
+    int *gy = (int *)malloc(sizeof(int) * (size_t)(h + 2)), *gx = (int *)malloc(sizeof(int) * (size_t)(w + 2));
+    int ngy = 0, ngx = 0;
     for (int j = 0;; j += p->block_step) {
         if (j >= bottom + p->block_step) break;
         if (j > bottom) j = bottom;
-        for (int i = 0;; i += p->block_step) {
-            if (i >= right + p->block_step) break;
-            if (i > right) i = right;
-            const int n = match_one(&f, ref, w, h, w, j, i, buf, pos, keys);
+        gy[ngy++] = j;
+    }
+    for (int i = 0;; i += p->block_step) {
+        if (i >= right + p->block_step) break;
+        if (i > right) i = right;
+        gx[ngx++] = i;
+    }
+    const size_t bsz = (size_t)B * B;
+    double *blocks = (double *)malloc(sizeof(double) * bsz * (size_t)nplanes * (size_t)ngx * (size_t)ngy);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(g_threads)
+    for (int a = 0; a < ngy; ++a) {
+        cand *buf = (cand *)malloc(sizeof(cand) * (size_t)cap);
+        int32_t *pos = (int32_t *)malloc(sizeof(int32_t) * (size_t)cap);
+        double *keys = (double *)malloc(sizeof(double) * (size_t)cap);
+        for (int b = 0; b < ngx; ++b) {
+            const int n = match_one(&f, ref, w, h, w, gy[a], gx[b], buf, pos, keys);
+            for (int c = 0; c < nplanes; ++c)
+                nlm_average(p, src[c], w, range, gy[a], gx[b], pos, keys, n,
+                            blocks + (((size_t)a * ngx + b) * nplanes + c) * bsz);
+        }
+        free(buf);
+        free(pos);
+        free(keys);
+    }
+    for (int a = 0; a < ngy; ++a)
+        for (int b = 0; b < ngx; ++b) {
+            const int j = gy[a], i = gx[b];
             for (int c = 0; c < nplanes; ++c) {
-                nlm_average(p, src[c], w, range, j, i, pos, keys, n, blk);
+                const double *blk = blocks + (((size_t)a * ngx + b) * nplanes + c) * bsz;
                 for (int y = 0; y < B; ++y)
                     for (int x = 0; x < B; ++x) num[c][(size_t)(j + y) * w + i + x] += blk[y * B + x];
             }
             for (int y = 0; y < B; ++y)
                 for (int x = 0; x < B; ++x) den[(size_t)(j + y) * w + i + x] += 1.0;
         }
-    }
-    free(buf);
-    free(pos);
-    free(keys);
-    free(blk);
+    free(blocks);
+    free(gy);
+    free(gx);
     return 0;
 }
 

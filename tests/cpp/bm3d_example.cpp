@@ -56,6 +56,18 @@ int main()
         Plane po = filter(pi);
         std::printf("int frame R mean abs err %.3f levels; int plane Ceil %d out(0,0) %d\n", ierr / (W * H), po.Ceil(),
                     po(0, 0));
+        // NL-means of the same integer frame (NLMeans_Para defaults at sigma 20)
+        NLMeans nlm(NLMeans_Para(true, 20.0));
+        Frame gn = nlm(fi);
+        double nerr = 0, noisy = 0;
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                nerr += std::fabs(gn.R()(y, x) - 255.0 * (x < W / 2 ? 0.3 : 0.7));
+                noisy += std::fabs(fi.R()(y, x) - 255.0 * (x < W / 2 ? 0.3 : 0.7));
+            }
+        Plane_FL pn = nlm(src);
+        std::printf("nlm frame R mean abs err %.3f levels (noisy %.3f); nlm plane(0,0) %.6f\n", nerr / (W * H),
+                    noisy / (W * H), pn(0, 0));
         BM3D_Final_Para fp("np");
         fp.BlockSize = 11;  // not implemented by the GPU kernels: reported, not crashed
         BM3D_Final fin(fp);

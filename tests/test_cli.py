@@ -1,8 +1,9 @@
-"""The `isp_mw --bm3d` front end (isp_mw_amd/cli/isp_mw.cpp): the flag and
-file semantics of ISP_MW's BM3D_IO / FilterIO (include/BM3D.h:363-526,
-include/Filter.h:220-340), its PNG / PNM / PFM codecs against PIL (an
-independent implementation, test-only), and on the GPU its output against the
-library's integer Frame path and the oracle."""
+"""The `isp_mw --bm3d` / `--nlm` front end (isp_mw_amd/cli/isp_mw.cpp): the
+flag and file semantics of ISP_MW's BM3D_IO / NLMeans_IO / FilterIO
+(include/BM3D.h:363-526, include/NLMeans.h:62-171, include/Filter.h:220-340),
+its PNG / PNM / PFM codecs against PIL (an independent implementation,
+test-only), and on the GPU its output against the library's integer Frame
+paths and the oracle."""
 import json
 import os
 import subprocess
@@ -63,6 +64,33 @@ def test_output_naming_and_errors():
     assert r.returncode == 2 and "No argument specified for option -S" in r.stderr
     assert run("--bm3d", "x.png", "--bogus", "1", check=False).returncode == 2
     assert run("--bm3d", "missing.png", check=False).returncode == 2
+
+
+def nlm_params(*args):
+    return json.loads(run("--nlm", "in.png", "--print-params", *args).stdout)
+
+
+def test_nlm_defaults_follow_the_reference():
+    # NLMeans_Para (include/NLMeans.h:11-22), tag ".NLMeans"
+    p = nlm_params()
+    assert p["filter"] == "nlm" and p["correction"] is True and p["sigma"] == 8 and p["strength"] == 40
+    assert (p["BlockSize"], p["BlockStep"], p["GroupSize"], p["BMrange"], p["BMstep"]) == (8, 5, 16, 24, 3)
+    assert p["thMSE"] == 400 and p["output"] == "in.NLMeans.png"
+
+
+def test_nlm_flags_and_derived_values():
+    # strength and thMSE follow the final sigma and correction unless given (:148-149)
+    p = nlm_params("-S", "12", "-C", "FALSE")
+    assert (p["correction"], p["strength"], p["thMSE"]) == (False, 18, 300)
+    p = nlm_params("-C", "false", "-S", "12", "-H", "2.5", "-TH", "77", "-BS", "8", "-BSP", "3", "-GS", "8",
+                   "-MR", "9", "-MS", "1", "-T", "_x", "-F", "ppm")
+    assert (p["strength"], p["thMSE"], p["BlockStep"], p["GroupSize"], p["BMrange"], p["BMstep"]) == \
+        (2.5, 77, 3, 8, 9, 1)
+    assert p["output"] == "in_x.ppm"
+    for name in ("--NLMeans", "--nonlocalmeans", "--NLM"):
+        assert json.loads(run(name, "a.png", "--print-params").stdout)["filter"] == "nlm"
+    r = run("--nlm", "x.png", "-C", "maybe", check=False)
+    assert r.returncode == 2 and 'must be "true" or "false"' in r.stderr
 
 
 PIL = pytest.importorskip("PIL.Image")
@@ -170,3 +198,26 @@ def test_cli_against_oracle_and_guide(gpu, tmp_path):
     PIL.fromarray(np.asarray(PIL.open(p)), "RGB").save(g)
     out2 = run("--bm3d", p, "-P", "np", "-S", "25", "--ref", g, "-T", ".g").stdout.strip()
     np.testing.assert_array_equal(np.asarray(PIL.open(out2)), got.astype(np.uint8))
+
+
+@pytest.mark.gpu
+def test_cli_nlm_equals_library_and_oracle(gpu, tmp_path):
+    import oracle
+    from isp_mw_amd import NLMeans, NLMeans_Para
+    p, a = noisy_png(tmp_path, seed=5)
+    out = run("--nlm", p, "-S", "20").stdout.strip()
+    assert out.endswith("noisy.NLMeans.png")
+    got = np.asarray(PIL.open(out), np.int64)
+    para = NLMeans_Para(sigma=20.0)
+    frame = np.ascontiguousarray(np.transpose(a, (2, 0, 1)).astype(np.int32) * 257)
+    res = NLMeans(para)(frame, floor=0, ceil=65535)
+    want = np.transpose((res.astype(np.int64) + 128) // 257, (1, 2, 0))
+    np.testing.assert_array_equal(got, want)
+    op = oracle.nlm_params(True, 20.0)
+    ref16 = oracle.nlm_rgb_int(op, frame, floor=0, ceil=65535)
+    d = np.abs(got - np.transpose((ref16.astype(np.int64) + 128) // 257, (1, 2, 0)))
+    assert d.max() <= 1 and np.mean(d > 0) < 1e-3
+    # a float image is not a Frame
+    pfm = tmp_path / "f.pfm"
+    pfm.write_bytes(b"PF\n16 16\n-1.0\n" + np.zeros((16, 16, 3), "<f4").tobytes())
+    assert run("--nlm", pfm, check=False).returncode == 2

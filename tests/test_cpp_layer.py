@@ -1,5 +1,5 @@
-"""The header-only C++ layer (include/isp_bm3d.hpp) compiles and links against
-libisp_bm3d.so; on a GPU box the example program runs."""
+"""The header-only C++ layer (include/isp_bm3d.hpp: BM3D and NLMeans) compiles
+and links against libisp_bm3d.so; on a GPU box the example program runs."""
 import os
 import subprocess
 
@@ -36,3 +36,6 @@ def test_cpp_example_runs(gpu, tmp_path):
     ierr = float(out.stdout.split("int frame R mean abs err")[1].split()[0])
     assert ierr < 0.05 * 255
     assert "int plane Ceil 1023" in out.stdout
+    nerr = float(out.stdout.split("nlm frame R mean abs err")[1].split()[0])
+    noisy = float(out.stdout.split("(noisy")[1].split(")")[0])
+    assert nerr < noisy
