@@ -98,6 +98,11 @@ def _lib(fast: bool = False) -> ctypes.CDLL:
     lib.orc_match_all.argtypes = [sp, dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                   ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int32), dp]
     lib.orc_kernel.argtypes = [sp, ctypes.c_int, ctypes.c_int, pp, pp, pp, ctypes.c_int, ctypes.c_int]
+    ci = ctypes.c_int
+    lib.orc_match_stack.argtypes = [sp, pp, ci, ci, ci, ci, ci, ctypes.POINTER(ci), ctypes.POINTER(ctypes.c_int32),
+                                    ctypes.POINTER(ci)]
+    lib.orc_kernel_stack.argtypes = [sp, ci, ci, ci, ci, pp, pp, pp, ci, ci]
+    lib.orc_bm3d_stack.argtypes = [sp, sp, ci, ci, ci, ci, pp, pp, pp, ci, ci]
     lib.orc_bm3d_plane.argtypes = [sp, sp, dp, dp, dp, ctypes.c_int, ctypes.c_int]
     lib.orc_bm3d_rgb.argtypes = [sp, sp, ctypes.c_int, pp, pp, pp, ctypes.c_int, ctypes.c_int]
     lib.orc_rgb2opp.argtypes = [dp] * 6 + [ctypes.c_long]
@@ -364,3 +369,76 @@ def nlm_rgb_int(p: NlmParams, src, ref=None, floor: int = 0, ceil: int = 255) ->
     if rc < 0:
         raise ValueError(f"orc_nlm_rgb_int failed ({rc})")
     return np.stack(dst)
+
+
+def _stack_planes(frames):
+    """[T, H, W] or [T, 3, H, W] -> (nplanes, flat list of planes frame-major)."""
+    a = np.asarray(frames)
+    if a.ndim == 3:
+        return 1, [a[t] for t in range(a.shape[0])]
+    if a.ndim == 4 and a.shape[1] == 3:
+        return 3, [a[t, c] for t in range(a.shape[0]) for c in range(3)]
+    raise ValueError("expected [T, H, W] or [T, 3, H, W]")
+
+
+def match_stack(p, guides, radius: int, t: int):
+    """Multi-frame groups of frame t's grid over the guide planes [T, H, W].
+    Returns (counts[ngy, ngx], pos[ngy, ngx, G] as (y<<16)|x, frame[ngy, ngx, G])."""
+    sp = step_params(p)
+    g = np.asarray(guides)
+    T, h, w = g.shape
+    ga, gp = _planes([g[k] for k in range(T)])
+    ngx = len(grid(w, sp.block_size, sp.block_step))
+    ngy = len(grid(h, sp.block_size, sp.block_step))
+    G = sp.group_size
+    counts = np.zeros((ngy, ngx), dtype=np.int32)
+    pos = np.zeros((ngy, ngx, G), dtype=np.int32)
+    fz = np.zeros((ngy, ngx, G), dtype=np.int32)
+    rc = _lib().orc_match_stack(ctypes.byref(sp), gp, T, radius, t, w, h,
+                                counts.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                pos.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                fz.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    if rc != 0:
+        raise ValueError("orc_match_stack failed")
+    return counts, pos, fz
+
+
+def bm3d_stack(basic, final, frames, radius: int, ref=None, mode: int = 3) -> np.ndarray:
+    """BM3D of a stack of grey planes [T, H, W] or float RGB frames [T, 3, H, W]
+    with multi-frame grouping of temporal radius `radius` (orc_bm3d_stack)."""
+    b, f = step_params(basic), step_params(final)
+    n, src = _stack_planes(frames)
+    T = len(src) // n
+    h, w = src[0].shape
+    src_a, src_p = _planes(src)
+    ref_p = None
+    if ref is not None:
+        n2, refl = _stack_planes(ref)
+        assert n2 == n and len(refl) == len(src)
+        ref_a, ref_p = _planes(refl)
+    dst_a = [np.empty((h, w)) for _ in src]
+    dst_p = (ctypes.POINTER(ctypes.c_double) * len(dst_a))(*[_dptr(a) for a in dst_a])
+    rc = _lib().orc_bm3d_stack(ctypes.byref(b), ctypes.byref(f), mode, n, T, radius, dst_p, src_p, ref_p, w, h)
+    if rc < 0:
+        raise ValueError(f"orc_bm3d_stack failed ({rc})")
+    out = np.stack(dst_a)
+    return out.reshape((T, h, w) if n == 1 else (T, 3, h, w))
+
+
+def kernel_stack(p, src, ref, radius: int, wiener: bool) -> np.ndarray:
+    """One BM3D step over a stack (orc_kernel_stack): src / ref [T, H, W] or
+    [T, 3, H, W]; matching on ref's plane 0 of each frame."""
+    sp = step_params(p)
+    n, sl = _stack_planes(src)
+    _, rl = _stack_planes(ref)
+    T = len(sl) // n
+    h, w = sl[0].shape
+    sa, spp = _planes(sl)
+    ra, rpp = _planes(rl)
+    dst_a = [np.empty((h, w)) for _ in sl]
+    dst_p = (ctypes.POINTER(ctypes.c_double) * len(dst_a))(*[_dptr(a) for a in dst_a])
+    rc = _lib().orc_kernel_stack(ctypes.byref(sp), int(wiener), n, T, radius, dst_p, spp, rpp, w, h)
+    if rc < 0:
+        raise ValueError(f"orc_kernel_stack failed ({rc})")
+    out = np.stack(dst_a)
+    return out.reshape((T, h, w) if n == 1 else (T, 3, h, w))

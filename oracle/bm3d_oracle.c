@@ -286,6 +286,7 @@ typedef struct {
     double key;
     int seq;
     int y, x;
+    int z;   /* frame of a multi-frame candidate (0 otherwise) */
 } cand;
 
 static int cand_cmp(const void *a, const void *b)
@@ -413,25 +414,31 @@ int orc_match_all(const orc_step_params *p, const double *ref, int w, int h, int
 /* ------------------------------------------------------------------------ */
 /* Collaborative filtering (source/BM3D.cpp:339-385 and 411-457).           */
 
-static void gather(double *grp, const double *src, int stride, const int32_t *pos, int N, int B)
+static const int g_zero_frames[ORC_MAXN] = {0};
+
+/* Member z lies in frame fz[z] of the planes srcs[] (Pos3 gather, BlockGroup::From with a
+ * plane vector, include/Block.h:1141-1158). */
+static void gather(double *grp, const double *const *srcs, const int *fz, int stride, const int32_t *pos, int N,
+                   int B)
 {
     double *d = grp;
     for (int z = 0; z < N; ++z) {
         int y = pos[z] >> 16, x = pos[z] & 0xFFFF;
-        const double *s = src + (size_t)y * stride + x;
+        const double *s = srcs[fz[z]] + (size_t)y * stride + x;
         for (int u = 0; u < B; ++u)
             for (int v = 0; v < B; ++v) *d++ = s[(size_t)u * stride + v];
     }
 }
 
-/* Accumulates into num/den, whose row 0 is image row `row0`. */
-static void collaborative_filter(const filter_data *f, int c, double *num, double *den, int row0,
-                                 int stride, const double *src, const double *ref,
-                                 const int32_t *pos, int N, double *grp, double *gref)
+/* Accumulates into num/den of each member's frame, whose row 0 is image row
+ * `row0`.  A single-frame group has every fz[z] = 0. */
+static void collaborative_filter_pos3(const filter_data *f, int c, double *const *num, double *const *den, int row0,
+                                      int stride, const double *const *src, const double *const *ref,
+                                      const int32_t *pos, const int *fz, int N, double *grp, double *gref)
 {
     const int B = f->p.block_size;
     const int cnt = N * B * B;
-    gather(grp, src, stride, pos, N, B);
+    gather(grp, src, fz, stride, pos, N, B);
     orc_dct3d(grp, N, B, B, 0);
     double denWeight;
     if (!f->wiener) {
@@ -443,7 +450,7 @@ static void collaborative_filter(const filter_data *f, int c, double *num, doubl
         }
         denWeight = retained < 1 ? 1.0 : 1.0 / (double)retained;
     } else {
-        gather(gref, ref, stride, pos, N, B);
+        gather(gref, ref, fz, stride, pos, N, B);
         orc_dct3d(gref, N, B, B, 0);
         const double sigmaSquare = f->wienerSigmaSqr[c][N - 1];
         double L2 = 0;
@@ -460,16 +467,26 @@ static void collaborative_filter(const filter_data *f, int c, double *num, doubl
     const double *s = grp;
     for (int z = 0; z < N; ++z) {
         int y = (pos[z] >> 16) - row0, x = pos[z] & 0xFFFF;
-        double *dn = num + (size_t)y * stride + x;
+        double *dn = num[fz[z]] + (size_t)y * stride + x;
         for (int u = 0; u < B; ++u)
             for (int v = 0; v < B; ++v) dn[(size_t)u * stride + v] += *s++ * numWeight;
     }
     for (int z = 0; z < N; ++z) {
         int y = (pos[z] >> 16) - row0, x = pos[z] & 0xFFFF;
-        double *dd = den + (size_t)y * stride + x;
+        double *dd = den[fz[z]] + (size_t)y * stride + x;
         for (int u = 0; u < B; ++u)
             for (int v = 0; v < B; ++v) dd[(size_t)u * stride + v] += denWeight;
     }
+}
+
+/* Accumulates into num/den, whose row 0 is image row `row0`. */
+static void collaborative_filter(const filter_data *f, int c, double *num, double *den, int row0,
+                                 int stride, const double *src, const double *ref,
+                                 const int32_t *pos, int N, double *grp, double *gref)
+{
+    double *nm[1] = {num}, *dn[1] = {den};
+    const double *sr[1] = {src}, *rf[1] = {ref};
+    collaborative_filter_pos3(f, c, nm, dn, row0, stride, sr, rf, pos, g_zero_frames, N, grp, gref);
 }
 
 static int degenerate(const filter_data *f, int nplanes)
@@ -1044,5 +1061,243 @@ int orc_nlm_plane_int(const orc_nlm_params *p, int32_t *dst, const int32_t *src,
     const int rc = orc_nlm_plane(p, d, s, in[1] ? r : s, w, h);
     if (rc == 0) orc_range_to_int(dst, d, npx, floor_, neutral, ceil_);
     free(buf);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Multi-frame grouping.  The reference carries Pos3 group codes (frame, y, x)
+ * and gathers / aggregates over a vector of planes (BlockGroup,
+ * include/Block.h:843-846, 927-936, 1141-1208) but never forms such groups.
+ * This restatement defines them as the direct extension of
+ * BlockMatchingMulti (include/Block.h:578-711) to a stack of frames:
+ *   - the reference block is frame t's; candidates are every position of the
+ *     clipped search window in each frame z with |z - t| <= radius, visited
+ *     frame by frame in the order t, t-1, t+1, t-2, t+2, ... and row-major
+ *     within a frame (the scan order that breaks ties);
+ *   - the acceptance test is the reference's (SSE <= thSSE and SSE != 0),
+ *     the position (t, j, i) itself is the group's first member;
+ *   - members are filtered as one group by the step's collaborative filter
+ *     and each is aggregated into its own frame.
+ * With one frame or radius 0 every group and every output equals the
+ * single-frame filter's. */
+
+static int stack_frame_order(int t, int frames, int radius, int *order)
+{
+    int n = 0;
+    order[n++] = t;
+    for (int d = 1; d <= radius; ++d) {
+        if (t - d >= 0) order[n++] = t - d;
+        if (t + d < frames) order[n++] = t + d;
+    }
+    return n;
+}
+
+/* Group of reference block (t, j, i) over the guide planes guide[frame]. */
+static int match_stack(const filter_data *f, const double *const *guide, int frames, int radius, int t, int w, int h,
+                       int j, int i, cand *buf, int32_t *pos_out, int *fz_out)
+{
+    const orc_step_params *p = &f->p;
+    const int B = p->block_size;
+    pos_out[0] = (j << 16) | i;
+    fz_out[0] = t;
+    if (p->group_size == 1 || p->thMSE <= 0) return 1;
+    const double MSE2SSE = (double)(B * B) * 1.0 * 1.0 / (double)(255 * 255);
+    const double distMul = 1.0 / MSE2SSE;
+    const double thSSE = p->thMSE * MSE2SSE;
+    const int step = p->bm_step;
+    const int range = p->bm_range / step * step;
+    const int l = orc_search_boundary(i, 0, range, step), r = orc_search_boundary(i, w - B, range, step);
+    const int tp = orc_search_boundary(j, 0, range, step), b = orc_search_boundary(j, h - B, range, step);
+    const double *rb = guide[t] + (size_t)j * w + i;
+    int order[2 * 64 + 1];
+    const int nf = stack_frame_order(t, frames, radius, order);
+    int n = 0, seq = 0;
+    buf[n].key = 0.0; buf[n].seq = -1; buf[n].y = j; buf[n].x = i; buf[n].z = t; ++n;
+    for (int q = 0; q < nf; ++q) {
+        const int z = order[q];
+        const double *g = guide[z];
+        for (int y = tp; y <= b; y += step)
+            for (int x = l; x <= r; x += step) {
+                if (z == t && y == j && x == i) continue;
+                const double *cb = g + (size_t)y * w + x;
+                double dist = 0.0;
+                for (int u = 0; u < B; ++u)
+                    for (int v = 0; v < B; ++v) {
+                        const double d = rb[(size_t)u * w + v] - cb[(size_t)u * w + v];
+                        dist += d * d;
+                    }
+                if (dist <= thSSE && dist != 0) {
+                    buf[n].key = dist * distMul; buf[n].seq = seq; buf[n].y = y; buf[n].x = x; buf[n].z = z; ++n;
+                }
+                ++seq;
+            }
+    }
+    qsort(buf, (size_t)n, sizeof(cand), cand_cmp);
+    const int G = p->group_size;
+    if (G > 0 && n > G) n = G;
+    for (int k = 0; k < n; ++k) {
+        pos_out[k] = (buf[k].y << 16) | buf[k].x;
+        fz_out[k] = buf[k].z;
+    }
+    return n;
+}
+
+int orc_match_stack(const orc_step_params *p, const double *const *guide, int frames, int radius, int t, int w,
+                    int h, int *counts, int32_t *pos, int *fz)
+{
+    if (p->block_size <= 0 || w < p->block_size || h < p->block_size || p->bm_step <= 0 || p->group_size <= 0 ||
+        p->group_size > ORC_MAXN || frames < 1 || radius < 0 || radius > 64 || t < 0 || t >= frames)
+        return -1;
+    filter_data f;
+    filter_data_init(&f, p, 0);
+    const int B = p->block_size, G = p->group_size;
+    const int ngx = orc_grid(w, B, p->block_step, NULL, 0), ngy = orc_grid(h, B, p->block_step, NULL, 0);
+    int *gx = (int *)malloc(sizeof(int) * (size_t)ngx), *gy = (int *)malloc(sizeof(int) * (size_t)ngy);
+    orc_grid(w, B, p->block_step, gx, ngx);
+    orc_grid(h, B, p->block_step, gy, ngy);
+    cand *buf = (cand *)malloc(sizeof(cand) * (size_t)window_cap(p) * (size_t)(2 * radius + 1));
+    for (int a = 0; a < ngy; ++a)
+        for (int c = 0; c < ngx; ++c) {
+            const size_t g = (size_t)a * ngx + c;
+            counts[g] = match_stack(&f, guide, frames, radius, t, w, h, gy[a], gx[c], buf, pos + g * G, fz + g * G);
+        }
+    free(buf);
+    free(gx);
+    free(gy);
+    filter_data_free(&f);
+    return 0;
+}
+
+/* One BM3D step over a stack: dst/src/ref are [frame * nplanes + plane];
+ * matching on ref[frame * nplanes] (the luma guide), Wiener guides ref[...]. */
+int orc_kernel_stack(const orc_step_params *p, int wiener, int nplanes, int frames, int radius,
+                     double *const *dst, const double *const *src, const double *const *ref, int w, int h)
+{
+    if ((nplanes != 1 && nplanes != 3) || frames < 1 || radius < 0 || radius > 64) return -1;
+    init_tables();
+    filter_data f;
+    filter_data_init(&f, p, wiener);
+    const size_t npx = (size_t)w * h;
+    const int nall = frames * nplanes;
+    if (degenerate(&f, nplanes)) {
+        for (int k = 0; k < nall; ++k)
+            if (dst[k] != src[k]) memcpy(dst[k], src[k], sizeof(double) * npx);
+        filter_data_free(&f);
+        return 1;
+    }
+    const int B = p->block_size, G = p->group_size;
+    if (w < B || h < B || p->block_step <= 0 || p->bm_step <= 0 || G < 0 || G > ORC_MAXN) {
+        filter_data_free(&f);
+        return -2;
+    }
+    int active[3] = {0, 0, 0};
+    for (int c = 0; c < nplanes; ++c) active[c] = p->sigma[c] > 0;
+    const int ngx = orc_grid(w, B, p->block_step, NULL, 0), ngy = orc_grid(h, B, p->block_step, NULL, 0);
+    int *gx = (int *)malloc(sizeof(int) * (size_t)ngx), *gy = (int *)malloc(sizeof(int) * (size_t)ngy);
+    orc_grid(w, B, p->block_step, gx, ngx);
+    orc_grid(h, B, p->block_step, gy, ngy);
+    /* planes by frame: num/den/src/ref of plane c */
+    double **num = (double **)calloc((size_t)nall * 2, sizeof(double *)), **den = num + nall;
+    const double **sp = (const double **)malloc(sizeof(double *) * (size_t)frames * 3);
+    const double **rp = sp + frames, **gp = sp + 2 * frames;
+    for (int k = 0; k < nall; ++k)
+        if (active[k % nplanes]) {
+            num[k] = (double *)calloc(npx, sizeof(double));
+            den[k] = (double *)calloc(npx, sizeof(double));
+        }
+    for (int z = 0; z < frames; ++z) gp[z] = ref[z * nplanes];
+    const int Gcap = G > 0 ? G : ORC_MAXN;
+    cand *buf = (cand *)malloc(sizeof(cand) * (size_t)window_cap(p) * (size_t)(2 * radius + 1));
+    int32_t pos[ORC_MAXN];
+    int fz[ORC_MAXN];
+    double *grp = (double *)malloc(sizeof(double) * (size_t)Gcap * B * B);
+    double *gref = (double *)malloc(sizeof(double) * (size_t)Gcap * B * B);
+    double **nm = (double **)malloc(sizeof(double *) * (size_t)frames * 2), **dn = nm + frames;
+    /* frames in order, each frame's grid in the reference's raster order */
+    for (int t = 0; t < frames; ++t)
+        for (int a = 0; a < ngy; ++a)
+            for (int b = 0; b < ngx; ++b) {
+                int N = match_stack(&f, gp, frames, radius, t, w, h, gy[a], gx[b], buf, pos, fz);
+                if (G > 0 && N > G) N = G;
+                for (int c = 0; c < nplanes; ++c) {
+                    if (!active[c]) continue;
+                    for (int z = 0; z < frames; ++z) {
+                        sp[z] = src[z * nplanes + c];
+                        rp[z] = ref[z * nplanes + c];
+                        nm[z] = num[z * nplanes + c];
+                        dn[z] = den[z * nplanes + c];
+                    }
+                    collaborative_filter_pos3(&f, c, nm, dn, 0, w, sp, rp, pos, fz, N, grp, gref);
+                }
+            }
+    for (int k = 0; k < nall; ++k) {
+        if (active[k % nplanes]) {
+            for (size_t q = 0; q < npx; ++q) dst[k][q] = num[k][q] / den[k][q];
+            free(num[k]);
+            free(den[k]);
+        } else if (dst[k] != src[k]) {
+            memcpy(dst[k], src[k], sizeof(double) * npx);
+        }
+    }
+    free(num); free(sp); free(buf); free(grp); free(gref); free(nm); free(gx); free(gy);
+    filter_data_free(&f);
+    return 0;
+}
+
+/* Two-step (mode 3) or single-step (1 basic, 2 final) BM3D of a stack of grey
+ * planes (nplanes 1) or float RGB frames (nplanes 3, the Frame semantics:
+ * OPP without clip, matching on Y, OPP back with clip).  ref (or NULL): as
+ * for the single-frame forms, the step-1 matching guide (modes 1 and 3) or
+ * the Wiener guide (mode 2). */
+int orc_bm3d_stack(const orc_step_params *basic, const orc_step_params *final_, int mode, int nplanes, int frames,
+                   int radius, double *const *dst, const double *const *src, const double *const *ref, int w, int h)
+{
+    if ((mode != 1 && mode != 2 && mode != 3) || (nplanes != 1 && nplanes != 3) || frames < 1) return -1;
+    const size_t npx = (size_t)w * h;
+    const int nall = frames * nplanes;
+    double *buf = (double *)malloc(sizeof(double) * npx * (size_t)nall * 3);
+    double **pl = (double **)malloc(sizeof(double *) * (size_t)nall * 3);
+    double **s = pl, **t = pl + nall, **r = pl + 2 * nall;
+    for (int k = 0; k < nall; ++k) {
+        s[k] = buf + (size_t)k * npx;
+        t[k] = buf + (size_t)(nall + k) * npx;
+        r[k] = buf + (size_t)(2 * nall + k) * npx;
+    }
+    const int have_ref = ref && ref[0];
+    for (int z = 0; z < frames; ++z) {
+        if (nplanes == 3) {
+            const double *const *sz = src + z * 3;
+            orc_rgb2opp(s[z * 3], s[z * 3 + 1], s[z * 3 + 2], sz[0], sz[1], sz[2], (long)npx);
+            if (have_ref) {
+                const double *const *rz = ref + z * 3;
+                orc_rgb2opp(r[z * 3], r[z * 3 + 1], r[z * 3 + 2], rz[0], rz[1], rz[2], (long)npx);
+            }
+        } else {
+            memcpy(s[z], src[z], sizeof(double) * npx);
+            if (have_ref) memcpy(r[z], ref[z], sizeof(double) * npx);
+        }
+    }
+    const double *const *sc = (const double *const *)s;
+    int rc;
+    if (mode == 3) {
+        /* step 1 matches on the ref's luma (or src's); its planes filter src */
+        const double **rb = (const double **)malloc(sizeof(double *) * (size_t)nall);
+        for (int k = 0; k < nall; ++k) rb[k] = (have_ref && k % nplanes == 0) ? r[k] : s[k];
+        rc = orc_kernel_stack(basic, 0, nplanes, frames, radius, t, sc, rb, w, h);
+        free(rb);
+        if (rc >= 0) rc = orc_kernel_stack(final_, 1, nplanes, frames, radius, s, sc, (const double *const *)t, w, h);
+    } else {
+        const double **rb = (const double **)malloc(sizeof(double *) * (size_t)nall);
+        for (int k = 0; k < nall; ++k) rb[k] = have_ref && (mode == 2 || k % nplanes == 0) ? r[k] : s[k];
+        rc = orc_kernel_stack(mode == 1 ? basic : final_, mode == 2, nplanes, frames, radius, s, sc, rb, w, h);
+        free(rb);
+    }
+    for (int z = 0; z < frames && rc >= 0; ++z) {
+        if (nplanes == 3) orc_opp2rgb(dst[z * 3], dst[z * 3 + 1], dst[z * 3 + 2], s[z * 3], s[z * 3 + 1],
+                                      s[z * 3 + 2], (long)npx, 1);
+        else memcpy(dst[z], s[z], sizeof(double) * npx);
+    }
+    free(buf);
+    free(pl);
     return rc;
 }

@@ -122,6 +122,16 @@ void orc_dct3d(double *data, int n0, int n1, int n2, int inverse);
  * single-threaded loop order).  Returns the previous value. */
 int orc_set_threads(int n);
 
+/* Multi-frame grouping over a stack of frames (Pos3 groups, include/Block.h:843-846,
+ * 927-936, 1141-1208): candidates from frames |z - t| <= radius, visited t, t-1, t+1,
+ * ..., members aggregated into their own frames.  Planes are [frame * nplanes + c]. */
+int orc_match_stack(const orc_step_params *p, const double *const *guide, int frames, int radius, int t, int w,
+                    int h, int *counts, int32_t *pos, int *fz);
+int orc_kernel_stack(const orc_step_params *p, int wiener, int nplanes, int frames, int radius,
+                     double *const *dst, const double *const *src, const double *const *ref, int w, int h);
+int orc_bm3d_stack(const orc_step_params *basic, const orc_step_params *final_, int mode, int nplanes, int frames,
+                   int radius, double *const *dst, const double *const *src, const double *const *ref, int w, int h);
+
 #ifdef __cplusplus
 }
 #endif

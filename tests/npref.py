@@ -148,3 +148,77 @@ def nlm(p, src_planes, ref, value_range=1.0):
             nums[c][j:j + B, i:i + B] += est
         den[j:j + B, i:i + B] += 1
     return nums, den
+
+
+def brute_groups_stack(p, guides, radius, t, norm=NORM[0]):
+    """Multi-frame groups of frame t over guides [T, H, W]: every window
+    position of every frame z with |z - t| <= radius, frames visited in the
+    order t, t-1, t+1, t-2, ..., row-major within a frame; the same
+    acceptance test and (key, scan order) sort as brute_groups.  Returns
+    {(j, i): [(key, z, y, x), ...]}."""
+    guides = np.asarray(guides, np.float64)
+    T, h, w = guides.shape
+    st = p.BMstep
+    R = p.BMrange // st * st
+    th = p.thMSE * norm * (B * B / 255.0 ** 2)
+    mse2sse = B * B / 255.0 ** 2
+    order = [t] + [z for d in range(1, radius + 1) for z in (t - d, t + d) if 0 <= z < T]
+    out = {}
+    for j in grid(h, p.BlockStep):
+        for i in grid(w, p.BlockStep):
+            rb = guides[t, j:j + B, i:i + B]
+            if p.GroupSize == 1:
+                out[(j, i)] = [(0.0, t, j, i)]
+                continue
+            ys = [y for y in range(j - R, j + R + 1, st) if 0 <= y <= h - B]
+            xs = [x for x in range(i - R, i + R + 1, st) if 0 <= x <= w - B]
+            cands = []
+            for z in order:
+                # all window blocks of frame z at once: SSE by array reduction
+                blocks = np.stack([guides[z, y:y + B, x:x + B] for y in ys for x in xs])
+                d = ((blocks - rb) ** 2).reshape(len(ys) * len(xs), -1).sum(1)
+                for k, (y, x) in enumerate([(y, x) for y in ys for x in xs]):
+                    if z == t and (y, x) == (j, i):
+                        continue
+                    if d[k] <= th and d[k] != 0:
+                        cands.append((d[k] / mse2sse, len(cands), z, y, x))
+            cands.sort(key=lambda c: (c[0], c[1]))
+            g = [(0.0, t, j, i)] + [(k, z, y, x) for k, _, z, y, x in cands]
+            out[(j, i)] = g[:p.GroupSize] if p.GroupSize > 0 else g
+    return out
+
+
+def stack_step(p, src, ref, radius, wiener):
+    """One step of multi-frame BM3D on grey stacks src / ref [T, H, W] with the
+    orthonormal closed forms of filter_plane; every member aggregated into its
+    own frame.  Returns the [T, H, W] estimate."""
+    src = np.asarray(src, np.float64)
+    ref = np.asarray(ref, np.float64)
+    T, h, w = src.shape
+    sigma = p.sigma[0] / 255.0 * NORM[0]
+    num = np.zeros((T, h, w))
+    den = np.zeros((T, h, w))
+    for t in range(T):
+        for g in brute_groups_stack(p, ref, radius, t).values():
+            n = len(g)
+            O = sf.dctn(np.stack([src[z, y:y + B, x:x + B] for _, z, y, x in g]), type=2, norm="ortho")
+            if not wiener:
+                keep = np.abs(O) > p.lambda_ * sigma
+                O = np.where(keep, O, 0.0)
+                r = int(keep.sum())
+                wgt = 1.0 if r < 1 else 1.0 / r
+            else:
+                Or = sf.dctn(np.stack([ref[z, y:y + B, x:x + B] for _, z, y, x in g]), type=2, norm="ortho")
+                zeta = np.zeros((n, B, B))
+                zeta[0] += 1
+                zeta[:, 0, :] += 1
+                zeta[:, :, 0] += 1
+                Wc = Or ** 2 / (Or ** 2 + sigma ** 2 / 2.0 ** zeta)
+                O = O * Wc
+                l2 = float((Wc ** 2).sum())
+                wgt = 1.0 if l2 <= 0 else 1.0 / l2
+            blk = sf.idctn(O, type=2, norm="ortho")
+            for m, (_, z, y, x) in enumerate(g):
+                num[z, y:y + B, x:x + B] += wgt * blk[m]
+                den[z, y:y + B, x:x + B] += wgt
+    return num / den
