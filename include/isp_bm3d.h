@@ -128,6 +128,21 @@ int isp_bm3d_rgb_i32(isp_bm3d_ctx *ctx, int *dst_r, int *dst_g, int *dst_b, cons
                      const int *src_b, const int *ref_r, const int *ref_g, const int *ref_b, int width, int height,
                      size_t stride, int floor_, int ceil_, int mode, int mem, void *stream);
 
+/* Multi-frame grouping: BM3D of a stack of `frames` grey planes (nplanes 1)
+ * or float RGB frames (nplanes 3, the semantics of isp_bm3d_rgb_f32) in which
+ * the groups of frame t draw blocks from frames |z - t| <= radius (0..4) and
+ * each filtered block is aggregated into its own frame.  The reference has
+ * the Pos3 group codes for this (BlockGroup, include/Block.h:843-846, 927-936,
+ * 1141-1208) but forms no such groups: candidates are the single-frame search
+ * window's positions in every frame of the window, frames visited t, t-1,
+ * t+1, t-2, ..., with BlockMatchingMulti's test and (key, scan order) sort.
+ * With one frame or radius 0 the result equals the single-frame call on each
+ * frame.  dst, src and ref (or NULL) hold frames * nplanes plane pointers,
+ * frame-major ([t * nplanes + c]); mode, mem and stream as in
+ * isp_bm3d_plane_f32. */
+int isp_bm3d_stack_f32(isp_bm3d_ctx *ctx, float *const *dst, const float *const *src, const float *const *ref,
+                       int frames, int nplanes, int width, int height, size_t stride, int radius, int mode, int mem,
+                       void *stream);
 /* Block matching of one step only (BM3D_Base::BlockMatching,
  * source/BM3D.cpp:221-236, include/Block.h:658-703) on a float guide plane,
  * for group-index parity checks.  For grid point g = gy*ngx + gx:

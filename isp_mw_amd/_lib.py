@@ -30,6 +30,7 @@ EXPORTS = (
     "isp_bm3d_rgb_f32",
     "isp_bm3d_plane_i32",
     "isp_bm3d_rgb_i32",
+    "isp_bm3d_stack_f32",
     "isp_bm3d_match",
     "isp_bm3d_grid_size",
     "isp_bm3d_set_timing",
@@ -164,6 +165,7 @@ def lib() -> ctypes.CDLL:
     L.isp_bm3d_rgb_f32.argtypes = [vp] + [vp] * 9 + [i, i, sz, i, i, vp]
     L.isp_bm3d_plane_i32.argtypes = [vp, vp, vp, vp, i, i, sz, i, i, i, i, i, vp]
     L.isp_bm3d_rgb_i32.argtypes = [vp] + [vp] * 9 + [i, i, sz, i, i, i, i, vp]
+    L.isp_bm3d_stack_f32.argtypes = [vp, vp, vp, vp, i, i, i, i, sz, i, i, i, vp]
     L.isp_bm3d_match.argtypes = [vp, vp, i, i, sz, i, i, vp, vp, vp]
     L.isp_bm3d_grid_size.argtypes = [vp, i, i, i, ctypes.POINTER(i), ctypes.POINTER(i)]
     L.isp_bm3d_set_timing.argtypes = [vp, i]

@@ -240,6 +240,33 @@ class _Filter:
         L.check(rc, self._ctx)
         return dst
 
+    def process_stack(self, dst, src, radius: int = 1, ref=None):
+        """Multi-frame grouping over a stack: src / dst / ref are [T, H, W]
+        grey planes or [T, 3, H, W] float RGB frames (numpy: host memory,
+        torch: device memory).  The groups of frame t take blocks from frames
+        |z - t| <= radius; each filtered block returns to its own frame.
+        radius 0 (or T = 1) equals filtering every frame on its own."""
+        ptr, stream, mem, is_int, _ = io_args(dst, src, ref, None, None, None, 8)
+        if is_int:
+            raise ValueError("stacks are float frames")
+        if src.ndim == 3:
+            T, h, w = src.shape
+            n = 1
+            planes = lambda a: [a[t] for t in range(T)]  # noqa: E731
+        elif src.ndim == 4 and src.shape[1] == 3:
+            T, _, h, w = src.shape
+            n = 3
+            planes = lambda a: [a[t, c] for t in range(T) for c in range(3)]  # noqa: E731
+        else:
+            raise ValueError("expected [T, H, W] or [T, 3, H, W]")
+        arr = ctypes.c_void_p * (T * n)
+        d = arr(*[ptr(x) for x in planes(dst)])
+        s_ = arr(*[ptr(x) for x in planes(src)])
+        r = arr(*[ptr(x) for x in planes(ref)]) if ref is not None else None
+        L.check(L.lib().isp_bm3d_stack_f32(self._ctx, d, s_, r, T, n, w, h, w, int(radius), self._mode, mem, stream),
+                self._ctx)
+        return dst
+
     # FilterIF2::operator()(src[, ref]) (include/Filter.h:203-216)
     def __call__(self, src, ref=None, **quant):
         if _is_torch(src):

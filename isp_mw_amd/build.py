@@ -18,7 +18,7 @@ LIB = os.path.join(LIBDIR, "libisp_bm3d.so")
 CLI = os.path.join(LIBDIR, "isp_mw")
 CLI_SRC = os.path.join(HERE, "cli", "isp_mw.cpp")
 
-SOURCES = ["bm3d_kernels.hip", "nlm_kernels.hip", "bm3d_capi.cpp", "pilot.hip"]
+SOURCES = ["bm3d_kernels.hip", "nlm_kernels.hip", "stack_kernels.hip", "bm3d_capi.cpp", "pilot.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

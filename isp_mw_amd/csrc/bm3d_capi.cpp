@@ -2480,3 +2480,349 @@ int isp_nlm_rgb_i32(isp_bm3d_ctx *ctx, const isp_nlm_params *p, int *dst_r, int 
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Multi-frame grouping: BM3D over a stack of frames with Pos3 groups
+// (BlockGroup, include/Block.h:843-846, 927-936, 1141-1208).
+namespace {
+
+struct StackWork {
+    float *src[3];                       // plane c of frame t at src[c] + t * fs (YUV for RGB)
+    float *refp[3];                      // a separate ref stack's planes
+    float *basic[3];                     // step-1 estimate
+    unsigned long long *num[3], *den[3];
+    float *out[3];                       // host-mode staging (4-byte samples)
+    double *y64, *r64;                   // FP64 luma guides of RGB src / ref frames
+    int *lcnt[kMaxStackLists], *lpos[kMaxStackLists];
+    double *lkey[kMaxStackLists];
+    int *cnt, *pos, *slot;               // merged groups of one reference frame
+};
+
+size_t stack_layout(StackWork *w, void *base, size_t fs, int T, size_t refs, int G, int nlists)
+{
+    Carver c(base);
+    const size_t tf = fs * (size_t)T;
+    for (int k = 0; k < 3; ++k) w->src[k] = c.take<float>(tf);
+    for (int k = 0; k < 3; ++k) w->refp[k] = c.take<float>(tf);
+    for (int k = 0; k < 3; ++k) w->basic[k] = c.take<float>(tf);
+    for (int k = 0; k < 3; ++k) w->num[k] = c.take<unsigned long long>(tf);
+    for (int k = 0; k < 3; ++k) w->den[k] = c.take<unsigned long long>(tf);
+    for (int k = 0; k < 3; ++k) w->out[k] = c.take<float>(tf);
+    w->y64 = c.take<double>(tf);
+    w->r64 = c.take<double>(tf);
+    for (int l = 0; l < nlists; ++l) {
+        w->lcnt[l] = c.take<int>(refs);
+        w->lpos[l] = c.take<int>(refs * G);
+        w->lkey[l] = c.take<double>(refs * G);
+    }
+    w->cnt = c.take<int>(refs);
+    w->pos = c.take<int>(refs * G);
+    w->slot = c.take<int>(refs * G);
+    return c.off + 256;
+}
+
+// Frames of the window of t, own frame first, then t-1, t+1, t-2, ...
+int stack_order(int t, int T, int radius, int *order)
+{
+    int n = 0;
+    order[n++] = t;
+    for (int d = 1; d <= radius; ++d) {
+        if (t - d >= 0) order[n++] = t - d;
+        if (t + d < T) order[n++] = t + d;
+    }
+    return n;
+}
+
+// Planes of one step over the stack.
+struct StackIO {
+    int nplanes, T, radius, P;
+    size_t fs;
+    const float *src[3];                 // frame 0 of each plane
+    const float *guide[3];               // Wiener guides (final step)
+    const float *mg32;                   // matching guide frames, FP32 ...
+    const double *mg64;                  // ... or FP64
+    // finalize of frame t: out[t * nplanes + c], out_pitch
+    void *const *out;
+    size_t out_pitch;
+    int to_rgb;
+};
+
+// One BM3D step (BM3D_Base::Kernel with multi-frame groups), every frame as
+// reference frame in turn, then num/den of every frame.
+int stack_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, const StackIO &io, StackWork &w,
+               int W, int H, hipStream_t s)
+{
+    Derived d{};
+    int rc = derive(ctx, p, io.nplanes, W, H, d);
+    if (rc) return rc;
+    const size_t fs = io.fs;
+    const int T = io.T, P = io.P;
+    const bool mfma = ctx->precision != ISP_BM3D_PREC_FP32;
+    if (!d.degenerate) {
+        for (int c = 0; c < 3; ++c)
+            if (d.active[c]) {
+                StageTimer tm(ctx, ST_FILL, s);
+                HIP_CHECK(ctx, launch_fill(reinterpret_cast<float *>(w.num[c]), fs * T * 2, 0.f, s));
+                HIP_CHECK(ctx, launch_fill(reinterpret_cast<float *>(w.den[c]), fs * T * 2, 0.f, s));
+            }
+        const isp_bm3d_ctx::Tiles tl = tiles_for(ctx, W, H);
+        for (int t = 0; t < T; ++t) {
+            int order[kMaxStackLists];
+            const int nl = d.bypass ? 1 : stack_order(t, T, io.radius, order);
+            if (d.bypass) order[0] = t;
+            const int lo = std::max(0, t - io.radius), hi = std::min(T - 1, t + io.radius);
+            StackMergeArgs mg{};
+            mg.refs = d.ngx * d.ngy;
+            mg.G = p.group_size;
+            mg.nlists = nl;
+            for (int l = 0; l < nl; ++l) {
+                const int z = order[l];
+                const void *gt = io.mg64 ? (const void *)(io.mg64 + fs * t) : (const void *)(io.mg32 + fs * t);
+                const void *gz = io.mg64 ? (const void *)(io.mg64 + fs * z) : (const void *)(io.mg32 + fs * z);
+                MatchArgs a = build_match(ctx, p, d, W, H, gt, P, tl.match_x, tl.match_y, w.lcnt[l], w.lpos[l],
+                                          w.lkey[l]);
+                if (z != t) {
+                    a.search = gz;
+                    a.refW = (tl.match_x - 1) * p.block_step + kBlock;
+                    a.refH = (tl.match_y - 1) * p.block_step + kBlock;
+                    a.refPitch = a.refW | 1;
+                }
+                if ((rc = check_match_lds(ctx, p, a))) return rc;
+                StageTimer tm(ctx, wiener ? ST_MATCH_FINAL : ST_MATCH_BASIC, s);
+                HIP_CHECK(ctx, launch_match(a, io.mg64 != nullptr, s));
+                mg.cnt[l] = w.lcnt[l];
+                mg.pos[l] = w.lpos[l];
+                mg.key[l] = w.lkey[l];
+                mg.slot[l] = z - lo;
+            }
+            mg.cnt_out = w.cnt;
+            mg.pos_out = w.pos;
+            mg.slot_out = w.slot;
+            {
+                StageTimer tm(ctx, wiener ? ST_MATCH_FINAL : ST_MATCH_BASIC, s);
+                HIP_CHECK(ctx, launch_stack_merge(mg, s));
+            }
+            // the window's frames from slot 0 = frame lo
+            const float *sp[3], *gp[3];
+            unsigned long long *np_[3], *dp[3];
+            for (int c = 0; c < 3; ++c) {
+                sp[c] = io.src[c] ? io.src[c] + fs * lo : nullptr;
+                gp[c] = io.guide[c] ? io.guide[c] + fs * lo : nullptr;
+                np_[c] = w.num[c] + fs * lo;
+                dp[c] = w.den[c] + fs * lo;
+            }
+            int trx = mfma ? tl.mfma_x : tl.filter_x, try_ = mfma ? tl.mfma_y : tl.filter_y;
+            FilterArgs f{};
+            for (;;) {
+                f = build_filter(ctx, p, d, W, H, P, sp, gp, np_, dp, w.cnt, w.pos, trx, try_);
+                f.slot = w.slot;
+                f.nslots = hi - lo + 1;
+                f.fstride = fs;
+                if (filter_stack_lds_bytes(f, wiener, ctx->precision) <= 160 * 1024) break;
+                if (trx == 1 && try_ == 1)
+                    return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "radius %d needs more LDS than a CU has", io.radius);
+                if (trx >= try_ && trx > 1) trx = (trx + 1) / 2;
+                else try_ = (try_ + 1) / 2;
+            }
+            StageTimer tm(ctx, wiener ? ST_FILTER_FINAL : ST_FILTER_BASIC, s);
+            HIP_CHECK(ctx, launch_filter_stack(f, wiener, ctx->precision, s));
+        }
+    }
+    for (int t = 0; t < T; ++t) {
+        FinalizeArgs fa{};
+        fa.W = W;
+        fa.H = H;
+        fa.pitch = P;
+        fa.nplanes = io.nplanes;
+        fa.to_rgb = io.to_rgb;
+        fa.out_int = 0;
+        fa.clip = io.to_rgb;
+        fa.gain_y = fa.gain_c = 1.0;
+        fa.offset = 0.0;
+        fa.lo = 0.0;
+        fa.hi = 1.0;
+        fa.out_pitch = io.out_pitch;
+        for (int c = 0; c < 3; ++c) {
+            fa.src[c] = io.src[c] ? io.src[c] + fs * t : nullptr;
+            fa.num[c] = w.num[c] + fs * t;
+            fa.den[c] = w.den[c] + fs * t;
+            fa.active[c] = d.degenerate ? 0 : d.active[c];
+            fa.out[c] = (io.to_rgb || c < io.nplanes) ? io.out[t * (io.to_rgb ? 3 : io.nplanes) + c] : nullptr;
+        }
+        StageTimer tm(ctx, ST_FINALIZE, s);
+        HIP_CHECK(ctx, launch_finalize(fa, s));
+    }
+    return ISP_BM3D_OK;
+}
+
+int stack_common(isp_bm3d_ctx *ctx, float *const *dst, const float *const *src, const float *const *ref, int T,
+                 int nplanes, int W, int H, size_t stride, int radius, int mode, int mem, void *stream)
+{
+    int rc = check_common(ctx, W, H, stride, mode, mem);
+    if (rc) return rc;
+    if (T < 1 || T > 4096) return fail(ctx, ISP_BM3D_E_ARG, "frame count %d", T);
+    if (nplanes != 1 && nplanes != 3) return fail(ctx, ISP_BM3D_E_ARG, "nplanes %d (1 or 3)", nplanes);
+    if (radius < 0 || radius > kMaxStackRadius)
+        return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "temporal radius %d outside 0..%d", radius, kMaxStackRadius);
+    if (!dst || !src) return fail(ctx, ISP_BM3D_E_ARG, "null dst/src");
+    const int n = T * nplanes;
+    for (int k = 0; k < n; ++k)
+        if (!dst[k] || !src[k] || (ref && !ref[k])) return fail(ctx, ISP_BM3D_E_ARG, "null plane %d", k);
+    hipStream_t s = (hipStream_t)stream;
+    const bool host = mem == ISP_BM3D_MEM_HOST;
+    const bool has_ref = ref != nullptr;
+    // the single steps' early return (as isp_bm3d_plane_f32 / rgb_f32)
+    if (mode != ISP_BM3D_BOTH) {
+        const bool degen = nplanes == 3 ? frame_degenerate(ctx, mode) : [&] {
+            const isp_bm3d_step_params &p = mode == ISP_BM3D_BASIC ? ctx->params.basic : ctx->params.final_;
+            Derived d{};
+            return derive(ctx, p, 1, W, H, d) == ISP_BM3D_OK && d.degenerate;
+        }();
+        if (degen) {
+            for (int k = 0; k < n; ++k) {
+                if (host) host_copy_plane(dst[k], src[k], stride, W, H);
+                else if (dst[k] != src[k] && (rc = copy_plane(ctx, dst[k], stride, src[k], stride, W, H, s)))
+                    return rc;
+            }
+            return ISP_BM3D_OK;
+        }
+    }
+    const int P = ((W + 63) / 64) * 64;
+    const size_t fs = (size_t)P * H;
+    size_t refs = 1;
+    int G = 1;
+    for (const isp_bm3d_step_params *p : {&ctx->params.basic, &ctx->params.final_}) {
+        const int st = p->block_step > 0 ? p->block_step : 1;
+        refs = std::max(refs, (size_t)(W / st + 2) * (size_t)(H / st + 2));
+        G = std::max(G, std::min(std::max(p->group_size, 1), kMaxGroup));
+    }
+    const int nlists = 2 * radius + 1;
+    StackWork w{};
+    if ((rc = ensure_ws(ctx, stack_layout(&w, nullptr, fs, T, refs, G, nlists)))) return rc;
+    stack_layout(&w, ctx->ws, fs, T, refs, G, nlists);
+    for (int k = 0; k < 2; ++k) {
+        ctx->last_cnt[k] = nullptr;
+        ctx->last_refs[k] = 0;
+    }
+
+    // frames into the workspace: grey planes as they are, RGB as OPP planes
+    // plus the FP64 luma guide; a separate ref likewise
+    const HostStage hs{ctx, s, P, W, H};
+    const int nin = has_ref ? 2 * n : n;
+    if (host && (rc = HostStage::reserve(ctx, (size_t)nin * hs.plane_bytes()))) return rc;
+    for (int pass = 0; pass < (has_ref ? 2 : 1); ++pass) {
+        const float *const *in = pass == 0 ? src : ref;
+        float *const *dstp = pass == 0 ? w.src : w.refp;
+        double *g64 = pass == 0 ? w.y64 : w.r64;
+        for (int t = 0; t < T; ++t) {
+            const void *pl[3];
+            size_t ip = stride;
+            for (int c = 0; c < nplanes; ++c) {
+                pl[c] = in[t * nplanes + c];
+                if (host) {
+                    // RGB lands in the output staging planes, grey directly in place
+                    float *land = nplanes == 3 ? w.out[c] + fs * t : dstp[c] + fs * t;
+                    if ((rc = hs.upload(pass * n + t * nplanes + c, land, pl[c], stride))) return rc;
+                    pl[c] = land;
+                }
+            }
+            if (host) ip = (size_t)P;
+            if (nplanes == 3) {
+                OppArgs o{};
+                o.W = W;
+                o.H = H;
+                o.r = pl[0];
+                o.g = pl[1];
+                o.b = pl[2];
+                o.gainY = 1.0 / (1.0 * 3.0);
+                o.offsetY = -0.0 * 3.0 * o.gainY + 0.0;
+                o.gainU = 1.0 / (1.0 * 2.0);
+                o.gainV = 1.0 / (1.0 * 4.0);
+                o.offsetC = 0.0;
+                o.in_pitch = ip;
+                o.y = dstp[0] + fs * t;
+                o.u = dstp[1] + fs * t;
+                o.v = dstp[2] + fs * t;
+                o.pitch = P;
+                o.y64 = g64 + fs * t;
+                o.pitch64 = P;
+                StageTimer tm(ctx, ST_OPP, s);
+                HIP_CHECK(ctx, launch_opp(o, s));
+            } else if (!host) {
+                if ((rc = copy_plane(ctx, dstp[0] + fs * t, P, pl[0], stride, W, H, s))) return rc;
+            }
+        }
+    }
+
+    // outputs: the caller's planes (device) or the staging planes (host)
+    std::vector<void *> outs((size_t)n);
+    for (int t = 0; t < T; ++t)
+        for (int c = 0; c < nplanes; ++c)
+            outs[(size_t)t * nplanes + c] = host ? (void *)(w.out[c] + fs * t) : (void *)dst[t * nplanes + c];
+    std::vector<void *> basics((size_t)T * 3);
+    for (int t = 0; t < T; ++t)
+        for (int c = 0; c < 3; ++c) basics[(size_t)t * 3 + c] = w.basic[c] + fs * t;
+    StackIO io{};
+    io.nplanes = nplanes;
+    io.T = T;
+    io.radius = radius;
+    io.P = P;
+    io.fs = fs;
+    for (int c = 0; c < 3; ++c) io.src[c] = c < nplanes ? w.src[c] : nullptr;
+    const bool rgb = nplanes == 3;
+    auto final_out = [&](StackIO &x) {
+        x.out = outs.data();
+        x.out_pitch = host ? (size_t)P : stride;
+        x.to_rgb = rgb;
+    };
+    if (mode == ISP_BM3D_BASIC || mode == ISP_BM3D_FINAL) {
+        const bool fin = mode == ISP_BM3D_FINAL;
+        // step-1 matching on ref's luma; a final step's ref is its Wiener guide
+        for (int c = 0; c < 3; ++c) io.guide[c] = c < nplanes ? (has_ref && fin ? w.refp[c] : w.src[c]) : nullptr;
+        if (rgb) io.mg64 = has_ref ? w.r64 : w.y64;
+        else io.mg32 = has_ref ? w.refp[0] : w.src[0];
+        final_out(io);
+        rc = stack_step(ctx, fin ? ctx->params.final_ : ctx->params.basic, fin, io, w, W, H, s);
+    } else {
+        for (int c = 0; c < 3; ++c) io.guide[c] = c < nplanes ? w.src[c] : nullptr;
+        if (rgb) io.mg64 = has_ref ? w.r64 : w.y64;
+        else io.mg32 = has_ref ? w.refp[0] : w.src[0];
+        io.out = basics.data();
+        io.out_pitch = (size_t)P;
+        io.to_rgb = 0;
+        // step 1 writes plane c of frame t to basics[t * 3 + c]: index with 3 planes
+        StackIO io1 = io;
+        std::vector<void *> b1((size_t)n);
+        for (int t = 0; t < T; ++t)
+            for (int c = 0; c < nplanes; ++c) b1[(size_t)t * nplanes + c] = basics[(size_t)t * 3 + c];
+        io1.out = b1.data();
+        rc = stack_step(ctx, ctx->params.basic, false, io1, w, W, H, s);
+        if (!rc) {
+            StackIO io2 = io;
+            for (int c = 0; c < 3; ++c) io2.guide[c] = c < nplanes ? w.basic[c] : nullptr;
+            io2.mg64 = nullptr;
+            io2.mg32 = w.basic[0];
+            final_out(io2);
+            rc = stack_step(ctx, ctx->params.final_, true, io2, w, W, H, s);
+        }
+    }
+    if (rc) return rc;
+    if (host) {
+        for (int k = 0; k < n; ++k) {
+            const int t = k / nplanes, c = k % nplanes;
+            HIP_CHECK(ctx, hipMemcpyAsync(hs.slot(k), w.out[c] + fs * t, hs.plane_bytes(), hipMemcpyDeviceToHost, s));
+        }
+        HIP_CHECK(ctx, hipStreamSynchronize(s));
+        for (int k = 0; k < n; ++k) hs.download_finish(k, dst[k], stride);
+    }
+    return ISP_BM3D_OK;
+}
+
+}  // namespace
+
+extern "C" int isp_bm3d_stack_f32(isp_bm3d_ctx *ctx, float *const *dst, const float *const *src,
+                                  const float *const *ref, int frames, int nplanes, int width, int height,
+                                  size_t stride, int radius, int mode, int mem, void *stream)
+{
+    return stack_common(ctx, dst, src, ref, frames, nplanes, width, height, stride, radius, mode, mem, stream);
+}

@@ -61,6 +61,26 @@ struct MatchArgs {
     int *cnt_out;            // [ngy*ngx]
     int *pos_out;            // [ngy*ngx*G]
     double *key_out;         // [ngy*ngx*G] or nullptr
+    // Multi-frame grouping: candidates from another frame's plane `search`
+    // (same type and pitch as guide) against the reference blocks of guide.
+    // Such a list has no self entry: cnt_out = N <= G-1, pos_out[0..N).
+    const void *search;      // nullptr: the guide itself (the single-frame search)
+    int refW, refH, refPitch;  // staged reference-block tile of a cross-frame search
+};
+
+// Multi-frame grouping: merge of the per-frame candidate lists of one
+// reference frame into its groups.
+constexpr int kMaxStackRadius = 4;
+constexpr int kMaxStackLists = 2 * kMaxStackRadius + 1;
+struct StackMergeArgs {
+    int refs;                 // grid points
+    int G;                    // GroupSize: row length of every list and of the output
+    int nlists;               // list 0: the reference frame's own (self first); then by frame priority
+    const int *cnt[kMaxStackLists];
+    const int *pos[kMaxStackLists];
+    const double *key[kMaxStackLists];
+    int slot[kMaxStackLists]; // frame slot (window-relative) of each list's members
+    int *cnt_out, *pos_out, *slot_out;
 };
 
 struct FilterArgs {
@@ -98,6 +118,12 @@ struct FilterArgs {
     // cna[N][2][64][8] C_N as A of Y = C_N T (forward, group axis)
     // cnb[N][2][64][8] C_N as B of V^T = Y^T C_N (inverse, group axis)
     const uint16_t *kb, *ka, *cna, *cnb;
+    // Multi-frame groups (stack kernels only): member z lies in frame slot
+    // slot[refIdx*G + z]; slot s of src / guide / num / den is the pointer
+    // plus s * fstride elements.
+    const int *slot;
+    int nslots;
+    size_t fstride;
 };
 
 struct FinalizeArgs {
@@ -201,6 +227,10 @@ hipError_t launch_filter_mfma(const FilterArgs &a, bool wiener, int precision, h
                               const BatchLaunch *b = nullptr);
 size_t filter_mfma_lds_bytes(const FilterArgs &a, bool wiener);
 hipError_t launch_finalize(const FinalizeArgs &a, hipStream_t s, const BatchLaunch *b = nullptr);
+// multi-frame groups: merge, and the filters over a window of frame slots
+hipError_t launch_stack_merge(const StackMergeArgs &a, hipStream_t s);
+size_t filter_stack_lds_bytes(const FilterArgs &a, bool wiener, int precision);
+hipError_t launch_filter_stack(const FilterArgs &a, bool wiener, int precision, hipStream_t s);
 hipError_t launch_opp(const OppArgs &a, hipStream_t s, const BatchLaunch *b = nullptr);
 hipError_t launch_to_float(const ToFloatArgs &a, hipStream_t s);
 hipError_t launch_opp_inverse(float *r, float *g, float *b, const float *y, const float *u, const float *v, int W,

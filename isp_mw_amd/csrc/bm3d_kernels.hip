@@ -200,15 +200,20 @@ __global__ __launch_bounds__(kThreads) void match_kernel(MatchArgs a0, const uin
     int tx, ty;
     const MatchArgs &a = tile_unit<BATCHED>(a0, bargs, bmap, tx, ty);
     const GT *__restrict__ guide = static_cast<const GT *>(a.guide);
+    // candidates come from `search` (another frame) when it is set
+    const bool cross = a.search != nullptr;
+    const GT *__restrict__ sguide = cross ? static_cast<const GT *>(a.search) : guide;
     const int gpitch = a.gpitch;
     const int trx = a.trx, tnum = a.trx * a.try_;
     const int tx0 = tx * trx, ty0 = ty * a.try_;
-    const int ox = grid_pos(tx0, a.bstep, a.W - kBlock) - a.range;
-    const int oy = grid_pos(ty0, a.bstep, a.H - kBlock) - a.range;
+    const int rx0 = grid_pos(tx0, a.bstep, a.W - kBlock), ry0 = grid_pos(ty0, a.bstep, a.H - kBlock);
+    const int ox = rx0 - a.range, oy = ry0 - a.range;
 
     DT *tile = reinterpret_cast<DT *>(smem);
-    unsigned char *wb = smem + align16((size_t)a.tileH * a.tilePitch * sizeof(DT)) +
-                        (size_t)wave * match_wave_bytes(a.maxCand, sizeof(KT));
+    const size_t tileBytes = align16((size_t)a.tileH * a.tilePitch * sizeof(DT));
+    DT *rtile = reinterpret_cast<DT *>(smem + tileBytes);  // cross: the tile's reference blocks
+    const size_t rtileBytes = cross ? align16((size_t)a.refH * a.refPitch * sizeof(DT)) : 0;
+    unsigned char *wb = smem + tileBytes + rtileBytes + (size_t)wave * match_wave_bytes(a.maxCand, sizeof(KT));
     KT *keys = reinterpret_cast<KT *>(wb);
     KT *selk = keys + a.maxCand;
     uint32_t *idxs = reinterpret_cast<uint32_t *>(selk + 32);
@@ -220,8 +225,17 @@ __global__ __launch_bounds__(kThreads) void match_kernel(MatchArgs a0, const uin
         const int ty = k / a.tileW, tx = k - ty * a.tileW;
         const int yy = oy + ty, xx = ox + tx;
         DT v = 0;
-        if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) v = (DT)guide[(size_t)yy * gpitch + xx];
+        if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) v = (DT)sguide[(size_t)yy * gpitch + xx];
         tile[ty * a.tilePitch + tx] = v;
+    }
+    if (cross) {
+        for (int k = tid; k < a.refH * a.refW; k += kThreads) {
+            const int ty = k / a.refW, tx = k - ty * a.refW;
+            const int yy = ry0 + ty, xx = rx0 + tx;
+            DT v = 0;
+            if (yy < a.H && xx < a.W) v = (DT)guide[(size_t)yy * gpitch + xx];
+            rtile[ty * a.refPitch + tx] = v;
+        }
     }
     __syncthreads();
 
@@ -231,7 +245,7 @@ __global__ __launch_bounds__(kThreads) void match_kernel(MatchArgs a0, const uin
         const int i = grid_pos(gxi, a.bstep, a.W - kBlock), j = grid_pos(gyi, a.bstep, a.H - kBlock);
         const size_t refIdx = (size_t)gyi * a.ngx + gxi;
         const size_t base = refIdx * a.G;
-        if (a.bypass) {
+        if (a.bypass && !cross) {
             if (lane == 0) {
                 a.cnt_out[refIdx] = 1;
                 a.pos_out[base] = (j << 16) | i;
@@ -247,7 +261,8 @@ __global__ __launch_bounds__(kThreads) void match_kernel(MatchArgs a0, const uin
         const int nx = (rr - l) / st + 1, ny = (b - t) / st + 1;
         const int selfx = (i - l) / st, selfy = (j - t) / st;
         const int nsx = (nx + kStrip - 1) / kStrip, nstrips = nsx * ny;
-        const DT *tref = tile + (j - oy) * a.tilePitch + (i - ox);
+        const int rp = cross ? a.refPitch : a.tilePitch;
+        const DT *tref = cross ? rtile + (j - ry0) * rp + (i - rx0) : tile + (j - oy) * rp + (i - ox);
         const DT thSSE = (DT)a.thSSE, distMul = (DT)a.distMul;
 
         uint32_t cnt = 0;
@@ -265,7 +280,7 @@ __global__ __launch_bounds__(kThreads) void match_kernel(MatchArgs a0, const uin
                 for (int u = 0; u < kBlock; ++u) {
                     DT rv[kBlock];
 #pragma unroll
-                    for (int v = 0; v < kBlock; ++v) rv[v] = tref[u * a.tilePitch + v];
+                    for (int v = 0; v < kBlock; ++v) rv[v] = tref[u * rp + v];
                     if constexpr (ST1) {
                         DT cv[kBlock + kStrip - 1];
 #pragma unroll
@@ -291,7 +306,8 @@ __global__ __launch_bounds__(kThreads) void match_kernel(MatchArgs a0, const uin
 #pragma unroll
             for (int c = 0; c < kStrip; ++c) {
                 const int kx = kx0 + c;
-                const bool ok = valid && kx < nx && !(kx == selfx && ky == selfy) && acc[c] <= thSSE && acc[c] != 0;
+                const bool ok = valid && kx < nx && (cross || !(kx == selfx && ky == selfy)) && acc[c] <= thSSE &&
+                                acc[c] != 0;
                 const uint64_t m = __ballot(ok);
                 if (ok) {
                     const uint32_t slot = cnt + lanes_below(m);
@@ -411,10 +427,14 @@ __global__ __launch_bounds__(kThreads) void match_kernel(MatchArgs a0, const uin
         wave_sync();
 
         // ---- order the selection and write the group ----
+        // (a cross-frame list: the selection alone, without the self entry)
+        const int first = cross ? 0 : 1;
         if (lane == 0) {
-            a.cnt_out[refIdx] = (int)(1 + nsel);
-            a.pos_out[base] = (j << 16) | i;
-            if (a.key_out) a.key_out[base] = 0.0;
+            a.cnt_out[refIdx] = (int)(first + nsel);
+            if (!cross) {
+                a.pos_out[base] = (j << 16) | i;
+                if (a.key_out) a.key_out[base] = 0.0;
+            }
         }
         if ((uint32_t)lane < nsel) {
             const KT kq = selk[lane];
@@ -426,8 +446,8 @@ __global__ __launch_bounds__(kThreads) void match_kernel(MatchArgs a0, const uin
             }
             const int ky = (int)(iq / (uint32_t)nx), kx = (int)(iq - (uint32_t)ky * nx);
             const int y = t + ky * st, x = l + kx * st;
-            a.pos_out[base + 1 + rank] = (y << 16) | x;
-            if (a.key_out) a.key_out[base + 1 + rank] = KeyOf<DT>::value(kq);
+            a.pos_out[base + first + rank] = (y << 16) | x;
+            if (a.key_out) a.key_out[base + first + rank] = KeyOf<DT>::value(kq);
         }
         wave_sync();
     }
@@ -515,16 +535,19 @@ __device__ __forceinline__ double wave_sum(double v)
 // One 3D orthonormal DCT coefficient (k, u, v) of a group, in FP64 from the
 // original samples, computed by the whole wave (lane = pixel (y, x)):
 // sum_{z,y,x} CN[k][z] C8[u][y] C8[v][x] X[z][y][x].  Result in every lane.
+// slot (multi-frame groups, else null): member z's plane is src + slot[z] * fstride.
 __device__ __noinline__ double exact_coef_wave(int k, int u, int v, int N, int lane,
                                                const int *__restrict__ pos, const float *__restrict__ src,
                                                int pitch, const double *__restrict__ c8d,
-                                               const double *__restrict__ cnd)
+                                               const double *__restrict__ cnd, const int *__restrict__ slot,
+                                               size_t fstride)
 {
     const int y = lane >> 3, x = lane & 7;
     double s = 0.0;
     for (int z = 0; z < N; ++z) {
         const int pz = pos[z];
-        s = fma(cnd[k * N + z], (double)src[(size_t)((pz >> 16) + y) * pitch + (pz & 0xFFFF) + x], s);
+        const float *pl = slot ? src + (size_t)slot[z] * fstride : src;
+        s = fma(cnd[k * N + z], (double)pl[(size_t)((pz >> 16) + y) * pitch + (pz & 0xFFFF) + x], s);
     }
     return wave_sum(s * c8d[u * kBlock + y] * c8d[v * kBlock + x]);
 }
@@ -537,16 +560,20 @@ struct ExactCtx {
     const double *cnd;  // N x N for this group
     double thr64;
     float band;
+    const int *slot;    // multi-frame groups: member frame slots (else null)
+    size_t fstride;
 };
 
 __device__ __forceinline__ void gather(float *g, const float *__restrict__ plane, int pitch,
-                                       const int *__restrict__ pos, int N, int lane)
+                                       const int *__restrict__ pos, int N, int lane,
+                                       const int *__restrict__ slot = nullptr, size_t fstride = 0)
 {
     const int u = lane >> 3, v = lane & 7;
     const int o = gidx(0, lane);
     for (int z = 0; z < N; ++z) {
         const int pz = pos[z];
-        g[o + z * ZS] = plane[(size_t)((pz >> 16) + u) * pitch + (pz & 0xFFFF) + v];
+        const float *pl = slot ? plane + (size_t)slot[z] * fstride : plane;
+        g[o + z * ZS] = pl[(size_t)((pz >> 16) + u) * pitch + (pz & 0xFFFF) + v];
     }
 }
 
@@ -562,7 +589,7 @@ __device__ __forceinline__ float group_core(float *g, int N, int lane, const flo
     float yr[WIENER ? NP : 1];
     if (WIENER) {
         // the guide group first; its coefficients stay in registers
-        gather(g, guide, pitch, ex.pos, N, lane);
+        gather(g, guide, pitch, ex.pos, N, lane, ex.slot, ex.fstride);
         wave_sync();
         rows_pass(g, N, c8, lane, false);
         wave_sync();
@@ -573,7 +600,7 @@ __device__ __forceinline__ float group_core(float *g, int N, int lane, const flo
         for (int k = 0; k < NP; ++k) yr[k] = k < N ? g[o + k * ZS] : 0.f;
         wave_sync();
     }
-    gather(g, src, pitch, ex.pos, N, lane);
+    gather(g, src, pitch, ex.pos, N, lane, ex.slot, ex.fstride);
     wave_sync();
     rows_pass(g, N, c8, lane, false);
     wave_sync();
@@ -594,7 +621,7 @@ __device__ __forceinline__ float group_core(float *g, int N, int lane, const flo
                 const int L = __ffsll((unsigned long long)close) - 1;
                 close &= close - 1;
                 const double oc = exact_coef_wave(k, L >> 3, L & 7, N, lane, ex.pos, ex.src, ex.pitch,
-                                                  ex.c8d, ex.cnd);
+                                                  ex.c8d, ex.cnd, ex.slot, ex.fstride);
                 if (lane == L) zero = fabs(oc) <= ex.thr64;
             }
             if (zero) g[o + k * ZS] = 0.f;
@@ -652,7 +679,8 @@ __host__ __device__ inline size_t filter_lds(int FW, int FH, int G)
 // a pixel is the sum of the weights of the blocks whose origin lies in the
 // 8x8 window above-left of it: row box sums (into hsum), then column sums.
 __device__ __forceinline__ void flush_tile(const FilterArgs &a, int c, const u64 *accNum, const u64 *accDen,
-                                           u64 *hsum, int ox, int oy, int tid, int nthreads = kThreads)
+                                           u64 *hsum, int ox, int oy, int tid, int nthreads = kThreads,
+                                           size_t dst_off = 0)
 {
     const int FW = a.FW, FH = a.FH, DW = FW - kBlock + 1, DH = FH - kBlock + 1;
     for (int k = tid; k < DH * FW; k += nthreads) {
@@ -666,7 +694,7 @@ __device__ __forceinline__ void flush_tile(const FilterArgs &a, int c, const u64
         hsum[k] = s;
     }
     __syncthreads();
-    u64 *num = a.num[c], *den = a.den[c];
+    u64 *num = a.num[c] + dst_off, *den = a.den[c] + dst_off;
     for (int k = tid; k < FW * FH; k += nthreads) {
         const int py = k / FW, px = k - py * FW;
         const int yy = oy + py, xx = ox + px;
@@ -685,7 +713,9 @@ __device__ __forceinline__ void flush_tile(const FilterArgs &a, int c, const u64
     }
 }
 
-template <bool WIENER, bool BATCHED>
+// STACK: multi-frame groups; the tile's accumulators are kept per frame slot
+// of the window and flushed into each slot's planes.
+template <bool WIENER, bool BATCHED, bool STACK = false>
 __global__ __launch_bounds__(kThreads) void filter_kernel(FilterArgs a0, const uint32_t *__restrict__ bmap,
                                                           const FilterArgs *__restrict__ bargs)
 {
@@ -698,18 +728,22 @@ __global__ __launch_bounds__(kThreads) void filter_kernel(FilterArgs a0, const u
     const int ox = grid_pos(tx0, a.bstep, a.W - kBlock) - a.range;
     const int oy = grid_pos(ty0, a.bstep, a.H - kBlock) - a.range;
     const int FW = a.FW, FH = a.FH, DW = FW - kBlock + 1, DH = FH - kBlock + 1;
+    const int nslots = STACK ? a.nslots : 1;
+    const size_t accw = filter_acc_floats(FW, FH);   // 32-bit words per slot
 
-    u64 *accNum = reinterpret_cast<u64 *>(smem);
-    u64 *accDen = accNum + FW * FH;
-    float *gbase = reinterpret_cast<float *>(smem) + filter_acc_floats(FW, FH);
+    u64 *accNum0 = reinterpret_cast<u64 *>(smem);
+    float *gbase = reinterpret_cast<float *>(smem) + accw * nslots;
     float *g = gbase + (size_t)wave * a.G * ZS;
     u64 *hsum = reinterpret_cast<u64 *>(gbase);  // flush only, after every wave is done with its groups
     const int u = lane >> 3, v = lane & 7;
 
     for (int c = 0; c < 3; ++c) {
         if (!a.active[c]) continue;
-        for (int k = tid; k < FW * FH; k += kThreads) accNum[k] = 0;
-        for (int k = tid; k < DW * DH; k += kThreads) accDen[k] = 0;
+        for (int sl = 0; sl < nslots; ++sl) {
+            u64 *an = accNum0 + (accw / 2) * sl, *ad = an + FW * FH;
+            for (int k = tid; k < FW * FH; k += kThreads) an[k] = 0;
+            for (int k = tid; k < DW * DH; k += kThreads) ad[k] = 0;
+        }
         __syncthreads();
         const float *src = a.src[c];
         const float *gsrc = a.guide[c];
@@ -718,9 +752,11 @@ __global__ __launch_bounds__(kThreads) void filter_kernel(FilterArgs a0, const u
             if (gxi >= a.ngx || gyi >= a.ngy) continue;
             const size_t refIdx = (size_t)gyi * a.ngx + gxi;
             const int *pos = a.pos + refIdx * a.G;
+            const int *slot = STACK ? a.slot + refIdx * a.G : nullptr;
             const int N = a.cnt[refIdx];
             const float thr = a.thr[c], sig2 = a.sig2[c];
-            const ExactCtx ex{pos, src, a.pitch, a.c8d, a.cnd + a.cnd_off[N], a.thr64[c], a.band[c]};
+            const ExactCtx ex{pos, src, a.pitch, a.c8d, a.cnd + a.cnd_off[N], a.thr64[c], a.band[c], slot,
+                              STACK ? a.fstride : 0};
             const float *cn = a.cn + a.cn_off[N];
             float w;
             if (N <= 1) w = group_core<1, WIENER>(g, N, lane, a.c8, cn, src, gsrc, a.pitch, thr, sig2, ex);
@@ -733,17 +769,22 @@ __global__ __launch_bounds__(kThreads) void filter_kernel(FilterArgs a0, const u
             for (int z = 0; z < N; ++z) {
                 const int pz = pos[z];
                 const int py = (pz >> 16) - oy, px = (pz & 0xFFFF) - ox;
-                fix_add(&accNum[(py + u) * FW + px + v], w * g[o + z * ZS]);
+                u64 *an = accNum0 + (STACK ? (accw / 2) * slot[z] : 0);
+                fix_add(&an[(py + u) * FW + px + v], w * g[o + z * ZS]);
             }
             if (lane < N) {
                 const int pz = pos[lane];
-                fix_add(&accDen[((pz >> 16) - oy) * DW + (pz & 0xFFFF) - ox], w);
+                u64 *ad = accNum0 + (STACK ? (accw / 2) * slot[lane] : 0) + FW * FH;
+                fix_add(&ad[((pz >> 16) - oy) * DW + (pz & 0xFFFF) - ox], w);
             }
             wave_sync();
         }
         __syncthreads();
-        flush_tile(a, c, accNum, accDen, hsum, ox, oy, tid);
-        __syncthreads();
+        for (int sl = 0; sl < nslots; ++sl) {
+            const u64 *an = accNum0 + (accw / 2) * sl;
+            flush_tile(a, c, an, an + FW * FH, hsum, ox, oy, tid, kThreads, STACK ? a.fstride * sl : 0);
+            __syncthreads();
+        }
     }
 }
 
@@ -807,10 +848,12 @@ __device__ __forceinline__ typename MfmaT<T>::v8 lds_frag(const uint16_t *tab, i
 }
 
 // plane: the tile footprint staged in LDS (row pitch `pitch`, origin ox, oy).
+// slot (multi-frame groups, else null): member c's footprint is plane + slot[c] * slotsz.
 template <typename T>
 __device__ __forceinline__ void mfma_forward(const float *plane, int pitch, int ox, int oy,
                                              const int *__restrict__ pos, int N, int lane, const uint16_t *kb,
-                                             const typename MfmaT<T>::v8 (&cna)[2], f32x16 (&y)[2])
+                                             const typename MfmaT<T>::v8 (&cna)[2], f32x16 (&y)[2],
+                                             const int *__restrict__ slot = nullptr, int slotsz = 0)
 {
     typedef typename MfmaT<T>::v8 v8;
     const int c = lane & 31, h = lane >> 5;
@@ -818,7 +861,7 @@ __device__ __forceinline__ void mfma_forward(const float *plane, int pitch, int 
     v8 xa[4];
     if (c < N) {
         const int pz = pos[c];
-        const float *b = plane + ((pz >> 16) - oy) * pitch + (pz & 0xFFFF) - ox;
+        const float *b = plane + ((pz >> 16) - oy) * pitch + (pz & 0xFFFF) - ox + (slot ? slot[c] * slotsz : 0);
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const float *r = b + (2 * s + h) * pitch;
@@ -873,19 +916,16 @@ __device__ __forceinline__ void mfma_inverse(const f32x16 (&y)[2], const uint16_
 // and inverse A, 16 x 64 lanes x 8 elements of 16 bits), then the staged
 // source (and guide) footprint; the flush's row sums reuse the staging area.
 constexpr int kK64Elems = 16 * 64 * 8;
-__host__ __device__ inline size_t filter_mfma_stage_floats(int FW, int FH, bool wiener)
+__host__ __device__ inline size_t filter_mfma_lds(int FW, int FH, bool wiener, int nslots = 1)
 {
-    const int DH = FH - kBlock + 1;
-    size_t st = (size_t)FW * FH * (wiener ? 2 : 1);
-    const size_t hs = (size_t)DH * FW * 2;
-    return st > hs ? st : hs;
-}
-__host__ __device__ inline size_t filter_mfma_lds(int FW, int FH, bool wiener)
-{
-    return filter_acc_floats(FW, FH) * 4 + kK64Elems * 2 + filter_mfma_stage_floats(FW, FH, wiener) * 4;
+    const size_t st = (size_t)FW * FH * (wiener ? 2 : 1) * nslots;
+    const size_t hs = (size_t)(FH - kBlock + 1) * FW * 2;
+    return filter_acc_floats(FW, FH) * 4 * nslots + kK64Elems * 2 + (st > hs ? st : hs) * 4;
 }
 
-template <typename T, bool WIENER, bool BATCHED>
+// STACK: multi-frame groups; accumulators and staged footprints per frame
+// slot of the window.
+template <typename T, bool WIENER, bool BATCHED, bool STACK = false>
 __global__ __launch_bounds__(kMThreads) void filter_mfma_kernel(FilterArgs a0, const uint32_t *__restrict__ bmap,
                                                                 const FilterArgs *__restrict__ bargs)
 {
@@ -900,14 +940,16 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_kernel(FilterArgs a0, c
     const int ox = grid_pos(tx0, a.bstep, a.W - kBlock) - a.range;
     const int oy = grid_pos(ty0, a.bstep, a.H - kBlock) - a.range;
     const int FW = a.FW, FH = a.FH, DW = FW - kBlock + 1, DH = FH - kBlock + 1;
+    const int nslots = STACK ? a.nslots : 1;
+    const size_t accw = filter_acc_floats(FW, FH);   // 32-bit words per slot
+    const int fsz = FW * FH;                          // staged floats per slot and plane
 
-    u64 *accNum = reinterpret_cast<u64 *>(smem);
-    u64 *accDen = accNum + FW * FH;
-    uint16_t *kb = reinterpret_cast<uint16_t *>(reinterpret_cast<float *>(smem) + filter_acc_floats(FW, FH));
+    u64 *accNum0 = reinterpret_cast<u64 *>(smem);
+    uint16_t *kb = reinterpret_cast<uint16_t *>(reinterpret_cast<float *>(smem) + accw * nslots);
     uint16_t *ka = kb + kK64Elems / 2;
     float *stage = reinterpret_cast<float *>(kb + kK64Elems);
     float *lsrc = stage;
-    float *lgd = stage + FW * FH;
+    float *lgd = stage + (size_t)fsz * nslots;
     u64 *hsum = reinterpret_cast<u64 *>(stage);  // flush only
     {
         // the K64 fragments (kb then ka are contiguous in the table)
@@ -920,16 +962,20 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_kernel(FilterArgs a0, c
         if (!a.active[pl]) continue;
         const float *src = a.src[pl];
         const float *gsrc = a.guide[pl];
-        for (int k = tid; k < FW * FH; k += kMThreads) {
-            accNum[k] = 0;
-            const int py = k / FW, px = k - py * FW;
-            const int yy = oy + py, xx = ox + px;
-            const bool in = yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
-            const size_t o = (size_t)yy * a.pitch + xx;
-            lsrc[k] = in ? src[o] : 0.f;
-            if (WIENER) lgd[k] = in ? gsrc[o] : 0.f;
+        for (int sl = 0; sl < nslots; ++sl) {
+            u64 *an = accNum0 + (accw / 2) * sl, *ad = an + fsz;
+            const size_t fo = STACK ? a.fstride * sl : 0;
+            for (int k = tid; k < fsz; k += kMThreads) {
+                an[k] = 0;
+                const int py = k / FW, px = k - py * FW;
+                const int yy = oy + py, xx = ox + px;
+                const bool in = yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
+                const size_t o = (size_t)yy * a.pitch + xx + fo;
+                lsrc[sl * fsz + k] = in ? src[o] : 0.f;
+                if (WIENER) lgd[sl * fsz + k] = in ? gsrc[o] : 0.f;
+            }
+            for (int k = tid; k < DW * DH; k += kMThreads) ad[k] = 0;
         }
-        for (int k = tid; k < DW * DH; k += kMThreads) accDen[k] = 0;
         __syncthreads();
         const float thr = a.thr[pl], sig2 = a.sig2[pl];
         for (int r = wave; r < tnum; r += kMWaves) {
@@ -937,6 +983,7 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_kernel(FilterArgs a0, c
             if (gxi >= a.ngx || gyi >= a.ngy) continue;
             const size_t refIdx = (size_t)gyi * a.ngx + gxi;
             const int *pos = a.pos + refIdx * a.G;
+            const int *slot = STACK ? a.slot + refIdx * a.G : nullptr;
             const int N = a.cnt[refIdx];
             v8 cna[2], cnb[2];
 #pragma unroll
@@ -947,7 +994,7 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_kernel(FilterArgs a0, c
             f32x16 y[2];
             float w;
             if (!WIENER) {
-                mfma_forward<T>(lsrc, FW, ox, oy, pos, N, lane, kb, cna, y);
+                mfma_forward<T>(lsrc, FW, ox, oy, pos, N, lane, kb, cna, y, slot, fsz);
                 int kept = 0;
 #pragma unroll
                 for (int q = 0; q < 2; ++q)
@@ -960,8 +1007,8 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_kernel(FilterArgs a0, c
                 w = kept < 1 ? 1.f : 1.f / (float)kept;
             } else {
                 f32x16 yg[2];
-                mfma_forward<T>(lgd, FW, ox, oy, pos, N, lane, kb, cna, yg);
-                mfma_forward<T>(lsrc, FW, ox, oy, pos, N, lane, kb, cna, y);
+                mfma_forward<T>(lgd, FW, ox, oy, pos, N, lane, kb, cna, yg, slot, fsz);
+                mfma_forward<T>(lsrc, FW, ox, oy, pos, N, lane, kb, cna, y, slot, fsz);
                 float l2 = 0.f;
 #pragma unroll
                 for (int q = 0; q < 2; ++q) {
@@ -985,6 +1032,8 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_kernel(FilterArgs a0, c
             if (c < N) {
                 const int pz = pos[c];
                 const int py = (pz >> 16) - oy, px = (pz & 0xFFFF) - ox + 4 * h;
+                u64 *accNum = accNum0 + (STACK ? (accw / 2) * slot[c] : 0);
+                u64 *accDen = accNum + fsz;
 #pragma unroll
                 for (int rr = 0; rr < 2; ++rr)
 #pragma unroll
@@ -996,8 +1045,11 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_kernel(FilterArgs a0, c
             }
         }
         __syncthreads();
-        flush_tile(a, pl, accNum, accDen, hsum, ox, oy, tid, kMThreads);
-        __syncthreads();
+        for (int sl = 0; sl < nslots; ++sl) {
+            const u64 *an = accNum0 + (accw / 2) * sl;
+            flush_tile(a, pl, an, an + fsz, hsum, ox, oy, tid, kMThreads, STACK ? a.fstride * sl : 0);
+            __syncthreads();
+        }
     }
 }
 
@@ -1159,7 +1211,8 @@ hipError_t launch_match_g(const MatchArgs &a, hipStream_t s, const BatchLaunch *
 size_t match_lds_bytes(const MatchArgs &a)
 {
     const int db = a.exact ? 8 : 4;
-    return align16((size_t)a.tileH * a.tilePitch * db) + (size_t)kWaves * match_wave_bytes(a.maxCand, db);
+    const size_t rt = a.search ? align16((size_t)a.refH * a.refPitch * db) : 0;
+    return align16((size_t)a.tileH * a.tilePitch * db) + rt + (size_t)kWaves * match_wave_bytes(a.maxCand, db);
 }
 
 size_t filter_lds_bytes(const FilterArgs &a, bool) { return filter_lds(a.FW, a.FH, a.G); }
@@ -1194,6 +1247,31 @@ hipError_t launch_filter(const FilterArgs &a, bool wiener, hipStream_t s, const 
 }
 
 size_t filter_mfma_lds_bytes(const FilterArgs &a, bool wiener) { return filter_mfma_lds(a.FW, a.FH, wiener); }
+
+size_t filter_stack_lds_bytes(const FilterArgs &a, bool wiener, int precision)
+{
+    if (precision != 0) return filter_mfma_lds(a.FW, a.FH, wiener, a.nslots);
+    const int DH = a.FH - kBlock + 1;
+    size_t groups = (size_t)kWaves * a.G * ZS;
+    const size_t hs = (size_t)DH * a.FW * 2;
+    if (hs > groups) groups = hs;
+    return (filter_acc_floats(a.FW, a.FH) * a.nslots + groups) * 4;
+}
+
+hipError_t launch_filter_stack(const FilterArgs &a, bool wiener, int precision, hipStream_t s)
+{
+    const size_t lds = filter_stack_lds_bytes(a, wiener, precision);
+    const dim3 grid((a.ngx + a.trx - 1) / a.trx, (a.ngy + a.try_ - 1) / a.try_);
+    if (precision == 0)
+        return wiener ? launch_tiled(filter_kernel<true, false, true>, kThreads, lds, a, s, nullptr, grid)
+                      : launch_tiled(filter_kernel<false, false, true>, kThreads, lds, a, s, nullptr, grid);
+    if (precision == 1)
+        return wiener ? launch_tiled(filter_mfma_kernel<_Float16, true, false, true>, kMThreads, lds, a, s, nullptr, grid)
+                      : launch_tiled(filter_mfma_kernel<_Float16, false, false, true>, kMThreads, lds, a, s, nullptr,
+                                     grid);
+    return wiener ? launch_tiled(filter_mfma_kernel<__bf16, true, false, true>, kMThreads, lds, a, s, nullptr, grid)
+                  : launch_tiled(filter_mfma_kernel<__bf16, false, false, true>, kMThreads, lds, a, s, nullptr, grid);
+}
 
 template <typename T, bool W>
 hipError_t launch_mfma_t(const FilterArgs &a, size_t lds, dim3 grid, hipStream_t s, const BatchLaunch *b)

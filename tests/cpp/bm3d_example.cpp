@@ -56,9 +56,13 @@ int main()
         Plane po = filter(pi);
         std::printf("int frame R mean abs err %.3f levels; int plane Ceil %d out(0,0) %d\n", ierr / (W * H), po.Ceil(),
                     po(0, 0));
-        // NL-means of the same integer frame (NLMeans_Para defaults at sigma 20)
+        // NL-means of a grey integer frame (R = G = B; the frame above has a
+        // constant luma, on which every distance is 0 and nothing matches)
         NLMeans nlm(NLMeans_Para(true, 20.0));
-        Frame gn = nlm(fi);
+        Frame fg(W, H, 8, QuantRange::PC);
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) fg.R()(y, x) = fg.G()(y, x) = fg.B()(y, x) = fi.R()(y, x);
+        Frame gn = nlm(fg);
         double nerr = 0, noisy = 0;
         for (int y = 0; y < H; ++y)
             for (int x = 0; x < W; ++x) {
