@@ -416,6 +416,127 @@ def bench_full(args) -> int:
     return 0
 
 
+def bench_stack(args) -> int:
+    """Multi-frame grouping on the full-form frames: 8 frames of 3840x2160 RGB
+    (one seeded base image, frames shifted by random 0..7 px, AWGN sigma 25),
+    BlockStep 3, BMrange 19, GroupSize 32 in both steps, two-step BM3D whose
+    groups of frame t take blocks from frames t-1..t+1 (temporal radius 1),
+    in the timed precision's form.  Compared with the same form frame by frame
+    (radius 0) and, for PSNR, with the plain FP32 form at radius 1."""
+    import torch
+    from isp_mw_amd import BM3D, BM3D_Para, synth
+
+    FW_, FH_, NF, RADIUS = 3840, 2160, 8, 1
+    rng = np.random.default_rng(4)
+    base = synth.clean_rgb(FH_ + 8, FW_ + 8, 99)
+    cleans = np.empty((NF, 3, FH_, FW_), np.float32)
+    for k in range(NF):
+        dy, dx = rng.integers(0, 8, 2)
+        cleans[k] = base[:, dy:dy + FH_, dx:dx + FW_]
+    del base
+    noisy = cleans + (rng.standard_normal(cleans.shape) * (SIGMA / 255.0)).astype(np.float32)
+    para = BM3D_Para("np")
+    for st in (para.basic, para.final):
+        st.BlockStep, st.BMrange, st.BMstep, st.GroupSize = 3, 19, 1, 32
+    para.basic.lambda_ = 2.7
+    para.set_sigma(SIGMA)
+    x = torch.from_numpy(noisy).cuda()
+    y = torch.empty_like(x)
+    f = BM3D(para, precision=args.precision)
+
+    def run(radius, out, filt=f):
+        filt.process_stack(out, x, radius=radius)
+
+    run(RADIUS, y)
+    torch.cuda.synchronize()
+    f.set_timing(True)
+    f.timing(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run(RADIUS, y)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    stages = f.timing()
+    f.set_timing(False)
+    out = y.cpu().numpy()
+
+    def psnr_stack(a):
+        return float(np.mean([synth.psnr(a[k], cleans[k]) for k in range(NF)]))
+
+    psnr_r1 = psnr_stack(out)
+    # the same form frame by frame (radius 0)
+    y0 = torch.empty_like(x)
+    run(0, y0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run(0, y0)
+    torch.cuda.synchronize()
+    dt0 = time.perf_counter() - t0
+    psnr_r0 = psnr_stack(y0.cpu().numpy())
+    del y0
+    extra = {}
+    if args.precision != "fp32":
+        f32 = BM3D(para, precision="fp32")
+        y32 = torch.empty_like(x)
+        t0 = time.perf_counter()
+        run(RADIUS, y32, f32)
+        torch.cuda.synchronize()
+        extra = {"plain_fp32_radius1": {"value": round(NF * FW_ * FH_ / (time.perf_counter() - t0) / 1e6, 3),
+                                        "psnr_db": round(psnr_stack(y32.cpu().numpy()), 4)}}
+        extra["psnr_loss_vs_fp32_db"] = round(extra["plain_fp32_radius1"]["psnr_db"] - psnr_r1, 4)
+        del y32
+    px = float(NF * FW_ * FH_)
+    # issued matrix work of the dominant filter: 12 MFMA 32x32x16 per transform,
+    # 3 planes x (2 forward + 1 inverse) for the Wiener step, 3 x 2 for step 1
+    gx, gy = grid(FW_, 3), grid(FH_, 3)
+    refs = NF * gx.size * gy.size
+    per = {k: v[0] / v[1] for k, v in stages.items() if v[1]}
+    dom = max(("filter_final", "filter_basic", "match_final", "match_basic"), key=lambda k: stages[k][0])
+    ms_total = stages[dom][0] / args.steps
+    if dom.startswith("filter") and args.precision != "fp32":
+        ntr = 3 * (3 if dom == "filter_final" else 2) * refs
+        flops = ntr * 12 * 32 * 32 * 16 * 2.0
+        achieved = flops / (ms_total * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_F16_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_F16_TFLOPS, 4), "traffic": None, "kernel": dom,
+                "ms_per_step": round(ms_total, 3), "basis": "issued MFMA work per step (8 frames), all launches"}
+    else:
+        cand = float(window_len(gx, FW_, 19).sum() * window_len(gy, FH_, 19).sum()) * NF * (2 * RADIUS + 1)
+        flops = cand * B * B * 3 if dom.startswith("match") else refs * 3 * 3 * transform_flops(32)
+        peak = PEAK_FP32_TFLOPS
+        achieved = flops / (ms_total * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(achieved / peak, 4), "traffic": None, "kernel": dom,
+                "ms_per_step": round(ms_total, 3),
+                "basis": "matching: 3 flop per pixel per candidate over 3 frames; filter: dense transforms at N=32"}
+    res = {
+        "metric": "Mpix/s 8 x 4K RGB two-step BM3D with temporal groups (radius 1), R=19, G=32",
+        "value": round(px / dt / 1e6, 3),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 1,
+        "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32" if args.precision == "fp32" else f"{args.precision}+fp32",
+        "data": "synthetic RGB base, 8 frames shifted by random 0..7 px, AWGN sigma 25/255",
+        "config": {"workload": "8 frames 3840x2160 RGB float32 stack, temporal radius 1", "block_step": 3,
+                   "bm_range": 19, "group_size": [32, 32], "sigma": SIGMA, "precision": args.precision},
+        "roofline": roof,
+        "quality": {"psnr_noisy_db": round(psnr_stack(noisy), 4), "psnr_radius1_db": round(psnr_r1, 4),
+                    "psnr_radius0_db": round(psnr_r0, 4), "temporal_gain_db": round(psnr_r1 - psnr_r0, 4),
+                    **{k: v for k, v in extra.items() if k == "psnr_loss_vs_fp32_db"}},
+        "stages_ms_per_launch": {k: round(v, 4) for k, v in per.items()},
+        "radius0_same_form": {"value": round(px / dt0 / 1e6, 3), "ms": round(dt0 * 1e3, 1)},
+        **{k: v for k, v in extra.items() if k != "psnr_loss_vs_fp32_db"},
+    }
+    print(json.dumps(res), flush=True)
+    return 0
+
+
 def nlm_window(pos: np.ndarray, n: int, R: int, s: int) -> np.ndarray:
     # clipped search window with BMstep s (include/Block.h:716-751): k*s offsets
     # with |k*s| <= R and 0 <= pos + k*s <= n - B
@@ -553,11 +674,12 @@ def main() -> int:
                                                 "for profiler runs, whose averages the tuner's trials would mix)")
     ap.add_argument("--cpu-seconds", type=float, default=4.0, help="target seconds per CPU baseline run")
     ap.add_argument("--workload", default="rgb1080",
-                    choices=("rgb1080", "batch64", "pilot1080", "full4k", "nlm1080"),
+                    choices=("rgb1080", "batch64", "pilot1080", "full4k", "nlm1080", "stack4k"),
                     help="rgb1080: the headline 1080p RGB frame; batch64: 64 uneven grey images as one batch; "
                          "pilot1080: train the learned pilot, then pilot + Wiener on 1080p grey; "
                          "full4k: 8 x 4K RGB frames, pilot guides + one batched FP16 Wiener launch; "
-                         "nlm1080: NL-means of a 1080p 16-bit RGB frame")
+                         "nlm1080: NL-means of a 1080p 16-bit RGB frame; "
+                         "stack4k: 8 x 4K RGB frames with temporal groups (radius 1)")
     ap.add_argument("--pilot-steps", type=int, default=1250, help="training steps of the pilot workload")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy (3 launches): the known-traffic "
@@ -585,6 +707,8 @@ def main() -> int:
         return bench_full(args)
     if args.workload == "nlm1080":
         return bench_nlm(args)
+    if args.workload == "stack4k":
+        return bench_stack(args)
 
     import torch
     from isp_mw_amd import BM3D, synth
