@@ -2495,7 +2495,7 @@ struct StackWork {
     double *y64, *r64;                   // FP64 luma guides of RGB src / ref frames
     int *lcnt[kMaxStackLists], *lpos[kMaxStackLists];
     double *lkey[kMaxStackLists];
-    int *cnt, *pos, *slot;               // merged groups of one reference frame
+    int *cnt, *pos, *slot;               // merged groups of every reference frame (slot: member frame)
 };
 
 size_t stack_layout(StackWork *w, void *base, size_t fs, int T, size_t refs, int G, int nlists)
@@ -2515,9 +2515,9 @@ size_t stack_layout(StackWork *w, void *base, size_t fs, int T, size_t refs, int
         w->lpos[l] = c.take<int>(refs * G);
         w->lkey[l] = c.take<double>(refs * G);
     }
-    w->cnt = c.take<int>(refs);
-    w->pos = c.take<int>(refs * G);
-    w->slot = c.take<int>(refs * G);
+    w->cnt = c.take<int>(refs * T);
+    w->pos = c.take<int>(refs * G * T);
+    w->slot = c.take<int>(refs * G * T);
     return c.off + 256;
 }
 
@@ -2547,8 +2547,9 @@ struct StackIO {
     int to_rgb;
 };
 
-// One BM3D step (BM3D_Base::Kernel with multi-frame groups), every frame as
-// reference frame in turn, then num/den of every frame.
+// One BM3D step (BM3D_Base::Kernel with multi-frame groups): the groups of
+// every reference frame, one filter launch over the whole stack, then
+// num/den of every frame.
 int stack_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, const StackIO &io, StackWork &w,
                int W, int H, hipStream_t s)
 {
@@ -2559,20 +2560,14 @@ int stack_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, co
     const int T = io.T, P = io.P;
     const bool mfma = ctx->precision != ISP_BM3D_PREC_FP32;
     if (!d.degenerate) {
-        for (int c = 0; c < 3; ++c)
-            if (d.active[c]) {
-                StageTimer tm(ctx, ST_FILL, s);
-                HIP_CHECK(ctx, launch_fill(reinterpret_cast<float *>(w.num[c]), fs * T * 2, 0.f, s));
-                HIP_CHECK(ctx, launch_fill(reinterpret_cast<float *>(w.den[c]), fs * T * 2, 0.f, s));
-            }
         const isp_bm3d_ctx::Tiles tl = tiles_for(ctx, W, H);
+        const size_t refs = (size_t)d.ngx * d.ngy;
         for (int t = 0; t < T; ++t) {
             int order[kMaxStackLists];
             const int nl = d.bypass ? 1 : stack_order(t, T, io.radius, order);
             if (d.bypass) order[0] = t;
-            const int lo = std::max(0, t - io.radius), hi = std::min(T - 1, t + io.radius);
             StackMergeArgs mg{};
-            mg.refs = d.ngx * d.ngy;
+            mg.refs = (int)refs;
             mg.G = p.group_size;
             mg.nlists = nl;
             for (int l = 0; l < nl; ++l) {
@@ -2593,40 +2588,43 @@ int stack_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, co
                 mg.cnt[l] = w.lcnt[l];
                 mg.pos[l] = w.lpos[l];
                 mg.key[l] = w.lkey[l];
-                mg.slot[l] = z - lo;
+                mg.slot[l] = z;
             }
-            mg.cnt_out = w.cnt;
-            mg.pos_out = w.pos;
-            mg.slot_out = w.slot;
-            {
-                StageTimer tm(ctx, wiener ? ST_MATCH_FINAL : ST_MATCH_BASIC, s);
-                HIP_CHECK(ctx, launch_stack_merge(mg, s));
-            }
-            // the window's frames from slot 0 = frame lo
-            const float *sp[3], *gp[3];
-            unsigned long long *np_[3], *dp[3];
-            for (int c = 0; c < 3; ++c) {
-                sp[c] = io.src[c] ? io.src[c] + fs * lo : nullptr;
-                gp[c] = io.guide[c] ? io.guide[c] + fs * lo : nullptr;
-                np_[c] = w.num[c] + fs * lo;
-                dp[c] = w.den[c] + fs * lo;
-            }
-            int trx = mfma ? tl.mfma_x : tl.filter_x, try_ = mfma ? tl.mfma_y : tl.filter_y;
-            FilterArgs f{};
-            for (;;) {
-                f = build_filter(ctx, p, d, W, H, P, sp, gp, np_, dp, w.cnt, w.pos, trx, try_);
-                f.slot = w.slot;
-                f.nslots = hi - lo + 1;
-                f.fstride = fs;
-                if (filter_stack_lds_bytes(f, wiener, ctx->precision) <= 160 * 1024) break;
-                if (trx == 1 && try_ == 1)
-                    return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "radius %d needs more LDS than a CU has", io.radius);
-                if (trx >= try_ && trx > 1) trx = (trx + 1) / 2;
-                else try_ = (try_ + 1) / 2;
-            }
-            StageTimer tm(ctx, wiener ? ST_FILTER_FINAL : ST_FILTER_BASIC, s);
-            HIP_CHECK(ctx, launch_filter_stack(f, wiener, ctx->precision, s));
+            mg.cnt_out = w.cnt + refs * t;
+            mg.pos_out = w.pos + refs * p.group_size * t;
+            mg.slot_out = w.slot + refs * p.group_size * t;
+            StageTimer tm(ctx, wiener ? ST_MATCH_FINAL : ST_MATCH_BASIC, s);
+            HIP_CHECK(ctx, launch_stack_merge(mg, s));
         }
+        for (int c = 0; c < 3; ++c)
+            if (d.active[c]) {
+                StageTimer tm(ctx, ST_FILL, s);
+                HIP_CHECK(ctx, launch_fill(reinterpret_cast<float *>(w.num[c]), fs * T * 2, 0.f, s));
+                HIP_CHECK(ctx, launch_fill(reinterpret_cast<float *>(w.den[c]), fs * T * 2, 0.f, s));
+            }
+        // one launch over the stack: each workgroup walks the frames for its tile
+        int trx = mfma ? tl.mfma_x : tl.filter_x, try_ = mfma ? tl.mfma_y : tl.filter_y;
+        unsigned long long *np_[3], *dp[3];
+        for (int c = 0; c < 3; ++c) {
+            np_[c] = w.num[c];
+            dp[c] = w.den[c];
+        }
+        FilterArgs f{};
+        for (;;) {
+            f = build_filter(ctx, p, d, W, H, P, io.src, io.guide, np_, dp, w.cnt, w.pos, trx, try_);
+            f.slot = w.slot;
+            f.frames = T;
+            f.radius = io.radius;
+            f.nslots = std::min(2 * io.radius + 1, T);
+            f.fstride = fs;
+            if (filter_stack_lds_bytes(f, wiener, ctx->precision) <= 160 * 1024) break;
+            if (trx == 1 && try_ == 1)
+                return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "radius %d needs more LDS than a CU has", io.radius);
+            if (trx >= try_ && trx > 1) trx = (trx + 1) / 2;
+            else try_ = (try_ + 1) / 2;
+        }
+        StageTimer tm(ctx, wiener ? ST_FILTER_FINAL : ST_FILTER_BASIC, s);
+        HIP_CHECK(ctx, launch_filter_stack(f, wiener, ctx->precision, s));
     }
     for (int t = 0; t < T; ++t) {
         FinalizeArgs fa{};

@@ -118,11 +118,17 @@ struct FilterArgs {
     // cna[N][2][64][8] C_N as A of Y = C_N T (forward, group axis)
     // cnb[N][2][64][8] C_N as B of V^T = Y^T C_N (inverse, group axis)
     const uint16_t *kb, *ka, *cna, *cnb;
-    // Multi-frame groups (stack kernels only): member z lies in frame slot
-    // slot[refIdx*G + z]; slot s of src / guide / num / den is the pointer
-    // plus s * fstride elements.
+    // Multi-frame groups (stack kernels only).  One launch covers every
+    // frame of the stack: the groups of reference frame t are cnt / pos /
+    // slot + t * (ngx*ngy) (times G for pos and slot); member z of a group
+    // lies in frame slot[...] (absolute), whose src / guide / num / den
+    // planes are the pointers plus frame * fstride elements.  A workgroup
+    // walks the frames in order for its tile, with `nslots` accumulator
+    // slots used as a ring (frame f in slot f % nslots), and flushes frame
+    // f once frame f + radius is done.
     const int *slot;
     int nslots;
+    int frames, radius;
     size_t fstride;
 };
 

@@ -190,7 +190,10 @@ struct KeyOf<float> {
     static __device__ __forceinline__ double value(T k) { return (double)__uint_as_float(k); }
 };
 
-template <typename GT, typename DT, bool ST1, bool EXACT, bool BATCHED>
+// CROSS: candidates from another frame's plane (a.search) against the
+// reference blocks of a.guide, staged in a second LDS tile; the list has no
+// self entry.
+template <typename GT, typename DT, bool ST1, bool EXACT, bool BATCHED, bool CROSS = false>
 __global__ __launch_bounds__(kThreads) void match_kernel(MatchArgs a0, const uint32_t *__restrict__ bmap,
                                                          const MatchArgs *__restrict__ bargs)
 {
@@ -200,8 +203,7 @@ __global__ __launch_bounds__(kThreads) void match_kernel(MatchArgs a0, const uin
     int tx, ty;
     const MatchArgs &a = tile_unit<BATCHED>(a0, bargs, bmap, tx, ty);
     const GT *__restrict__ guide = static_cast<const GT *>(a.guide);
-    // candidates come from `search` (another frame) when it is set
-    const bool cross = a.search != nullptr;
+    constexpr bool cross = CROSS;
     const GT *__restrict__ sguide = cross ? static_cast<const GT *>(a.search) : guide;
     const int gpitch = a.gpitch;
     const int trx = a.trx, tnum = a.trx * a.try_;
@@ -535,18 +537,32 @@ __device__ __forceinline__ double wave_sum(double v)
 // One 3D orthonormal DCT coefficient (k, u, v) of a group, in FP64 from the
 // original samples, computed by the whole wave (lane = pixel (y, x)):
 // sum_{z,y,x} CN[k][z] C8[u][y] C8[v][x] X[z][y][x].  Result in every lane.
-// slot (multi-frame groups, else null): member z's plane is src + slot[z] * fstride.
 __device__ __noinline__ double exact_coef_wave(int k, int u, int v, int N, int lane,
                                                const int *__restrict__ pos, const float *__restrict__ src,
                                                int pitch, const double *__restrict__ c8d,
-                                               const double *__restrict__ cnd, const int *__restrict__ slot,
-                                               size_t fstride)
+                                               const double *__restrict__ cnd)
 {
     const int y = lane >> 3, x = lane & 7;
     double s = 0.0;
     for (int z = 0; z < N; ++z) {
         const int pz = pos[z];
-        const float *pl = slot ? src + (size_t)slot[z] * fstride : src;
+        s = fma(cnd[k * N + z], (double)src[(size_t)((pz >> 16) + y) * pitch + (pz & 0xFFFF) + x], s);
+    }
+    return wave_sum(s * c8d[u * kBlock + y] * c8d[v * kBlock + x]);
+}
+
+// The same for multi-frame groups: member z's plane is src + slot[z] * fstride.
+__device__ __noinline__ double exact_coef_wave_slots(int k, int u, int v, int N, int lane,
+                                                     const int *__restrict__ pos, const float *__restrict__ src,
+                                                     int pitch, const double *__restrict__ c8d,
+                                                     const double *__restrict__ cnd, const int *__restrict__ slot,
+                                                     size_t fstride)
+{
+    const int y = lane >> 3, x = lane & 7;
+    double s = 0.0;
+    for (int z = 0; z < N; ++z) {
+        const int pz = pos[z];
+        const float *pl = src + (size_t)slot[z] * fstride;
         s = fma(cnd[k * N + z], (double)pl[(size_t)((pz >> 16) + y) * pitch + (pz & 0xFFFF) + x], s);
     }
     return wave_sum(s * c8d[u * kBlock + y] * c8d[v * kBlock + x]);
@@ -565,21 +581,32 @@ struct ExactCtx {
 };
 
 __device__ __forceinline__ void gather(float *g, const float *__restrict__ plane, int pitch,
-                                       const int *__restrict__ pos, int N, int lane,
-                                       const int *__restrict__ slot = nullptr, size_t fstride = 0)
+                                       const int *__restrict__ pos, int N, int lane)
 {
     const int u = lane >> 3, v = lane & 7;
     const int o = gidx(0, lane);
     for (int z = 0; z < N; ++z) {
         const int pz = pos[z];
-        const float *pl = slot ? plane + (size_t)slot[z] * fstride : plane;
-        g[o + z * ZS] = pl[(size_t)((pz >> 16) + u) * pitch + (pz & 0xFFFF) + v];
+        g[o + z * ZS] = plane[(size_t)((pz >> 16) + u) * pitch + (pz & 0xFFFF) + v];
+    }
+}
+
+// multi-frame groups: member z's plane is plane + slot[z] * fstride
+__device__ __forceinline__ void gather_slots(float *g, const float *__restrict__ plane, int pitch,
+                                             const int *__restrict__ pos, int N, int lane,
+                                             const int *__restrict__ slot, size_t fstride)
+{
+    const int u = lane >> 3, v = lane & 7;
+    const int o = gidx(0, lane);
+    for (int z = 0; z < N; ++z) {
+        const int pz = pos[z];
+        g[o + z * ZS] = plane[(size_t)slot[z] * fstride + (size_t)((pz >> 16) + u) * pitch + (pz & 0xFFFF) + v];
     }
 }
 
 // Forward transform, shrinkage and inverse of one group of one plane; the
 // filtered blocks are left in g.  Returns the group's aggregation weight.
-template <int NP, bool WIENER>
+template <int NP, bool WIENER, bool SLOTS = false>
 __device__ __forceinline__ float group_core(float *g, int N, int lane, const float *__restrict__ c8,
                                             const float *__restrict__ cn, const float *__restrict__ src,
                                             const float *__restrict__ guide, int pitch, float thr, float sig2,
@@ -589,7 +616,8 @@ __device__ __forceinline__ float group_core(float *g, int N, int lane, const flo
     float yr[WIENER ? NP : 1];
     if (WIENER) {
         // the guide group first; its coefficients stay in registers
-        gather(g, guide, pitch, ex.pos, N, lane, ex.slot, ex.fstride);
+        if constexpr (SLOTS) gather_slots(g, guide, pitch, ex.pos, N, lane, ex.slot, ex.fstride);
+        else gather(g, guide, pitch, ex.pos, N, lane);
         wave_sync();
         rows_pass(g, N, c8, lane, false);
         wave_sync();
@@ -600,7 +628,8 @@ __device__ __forceinline__ float group_core(float *g, int N, int lane, const flo
         for (int k = 0; k < NP; ++k) yr[k] = k < N ? g[o + k * ZS] : 0.f;
         wave_sync();
     }
-    gather(g, src, pitch, ex.pos, N, lane, ex.slot, ex.fstride);
+    if constexpr (SLOTS) gather_slots(g, src, pitch, ex.pos, N, lane, ex.slot, ex.fstride);
+    else gather(g, src, pitch, ex.pos, N, lane);
     wave_sync();
     rows_pass(g, N, c8, lane, false);
     wave_sync();
@@ -620,8 +649,10 @@ __device__ __forceinline__ float group_core(float *g, int N, int lane, const flo
             while (close) {
                 const int L = __ffsll((unsigned long long)close) - 1;
                 close &= close - 1;
-                const double oc = exact_coef_wave(k, L >> 3, L & 7, N, lane, ex.pos, ex.src, ex.pitch,
-                                                  ex.c8d, ex.cnd, ex.slot, ex.fstride);
+                const double oc = SLOTS ? exact_coef_wave_slots(k, L >> 3, L & 7, N, lane, ex.pos, ex.src, ex.pitch,
+                                                                ex.c8d, ex.cnd, ex.slot, ex.fstride)
+                                        : exact_coef_wave(k, L >> 3, L & 7, N, lane, ex.pos, ex.src, ex.pitch,
+                                                          ex.c8d, ex.cnd);
                 if (lane == L) zero = fabs(oc) <= ex.thr64;
             }
             if (zero) g[o + k * ZS] = 0.f;
@@ -713,9 +744,7 @@ __device__ __forceinline__ void flush_tile(const FilterArgs &a, int c, const u64
     }
 }
 
-// STACK: multi-frame groups; the tile's accumulators are kept per frame slot
-// of the window and flushed into each slot's planes.
-template <bool WIENER, bool BATCHED, bool STACK = false>
+template <bool WIENER, bool BATCHED>
 __global__ __launch_bounds__(kThreads) void filter_kernel(FilterArgs a0, const uint32_t *__restrict__ bmap,
                                                           const FilterArgs *__restrict__ bargs)
 {
@@ -728,22 +757,18 @@ __global__ __launch_bounds__(kThreads) void filter_kernel(FilterArgs a0, const u
     const int ox = grid_pos(tx0, a.bstep, a.W - kBlock) - a.range;
     const int oy = grid_pos(ty0, a.bstep, a.H - kBlock) - a.range;
     const int FW = a.FW, FH = a.FH, DW = FW - kBlock + 1, DH = FH - kBlock + 1;
-    const int nslots = STACK ? a.nslots : 1;
-    const size_t accw = filter_acc_floats(FW, FH);   // 32-bit words per slot
 
-    u64 *accNum0 = reinterpret_cast<u64 *>(smem);
-    float *gbase = reinterpret_cast<float *>(smem) + accw * nslots;
+    u64 *accNum = reinterpret_cast<u64 *>(smem);
+    u64 *accDen = accNum + FW * FH;
+    float *gbase = reinterpret_cast<float *>(smem) + filter_acc_floats(FW, FH);
     float *g = gbase + (size_t)wave * a.G * ZS;
     u64 *hsum = reinterpret_cast<u64 *>(gbase);  // flush only, after every wave is done with its groups
     const int u = lane >> 3, v = lane & 7;
 
     for (int c = 0; c < 3; ++c) {
         if (!a.active[c]) continue;
-        for (int sl = 0; sl < nslots; ++sl) {
-            u64 *an = accNum0 + (accw / 2) * sl, *ad = an + FW * FH;
-            for (int k = tid; k < FW * FH; k += kThreads) an[k] = 0;
-            for (int k = tid; k < DW * DH; k += kThreads) ad[k] = 0;
-        }
+        for (int k = tid; k < FW * FH; k += kThreads) accNum[k] = 0;
+        for (int k = tid; k < DW * DH; k += kThreads) accDen[k] = 0;
         __syncthreads();
         const float *src = a.src[c];
         const float *gsrc = a.guide[c];
@@ -752,11 +777,9 @@ __global__ __launch_bounds__(kThreads) void filter_kernel(FilterArgs a0, const u
             if (gxi >= a.ngx || gyi >= a.ngy) continue;
             const size_t refIdx = (size_t)gyi * a.ngx + gxi;
             const int *pos = a.pos + refIdx * a.G;
-            const int *slot = STACK ? a.slot + refIdx * a.G : nullptr;
             const int N = a.cnt[refIdx];
             const float thr = a.thr[c], sig2 = a.sig2[c];
-            const ExactCtx ex{pos, src, a.pitch, a.c8d, a.cnd + a.cnd_off[N], a.thr64[c], a.band[c], slot,
-                              STACK ? a.fstride : 0};
+            const ExactCtx ex{pos, src, a.pitch, a.c8d, a.cnd + a.cnd_off[N], a.thr64[c], a.band[c]};
             const float *cn = a.cn + a.cn_off[N];
             float w;
             if (N <= 1) w = group_core<1, WIENER>(g, N, lane, a.c8, cn, src, gsrc, a.pitch, thr, sig2, ex);
@@ -769,22 +792,107 @@ __global__ __launch_bounds__(kThreads) void filter_kernel(FilterArgs a0, const u
             for (int z = 0; z < N; ++z) {
                 const int pz = pos[z];
                 const int py = (pz >> 16) - oy, px = (pz & 0xFFFF) - ox;
-                u64 *an = accNum0 + (STACK ? (accw / 2) * slot[z] : 0);
-                fix_add(&an[(py + u) * FW + px + v], w * g[o + z * ZS]);
+                fix_add(&accNum[(py + u) * FW + px + v], w * g[o + z * ZS]);
             }
             if (lane < N) {
                 const int pz = pos[lane];
-                u64 *ad = accNum0 + (STACK ? (accw / 2) * slot[lane] : 0) + FW * FH;
-                fix_add(&ad[((pz >> 16) - oy) * DW + (pz & 0xFFFF) - ox], w);
+                fix_add(&accDen[((pz >> 16) - oy) * DW + (pz & 0xFFFF) - ox], w);
             }
             wave_sync();
         }
         __syncthreads();
-        for (int sl = 0; sl < nslots; ++sl) {
-            const u64 *an = accNum0 + (accw / 2) * sl;
-            flush_tile(a, c, an, an + FW * FH, hsum, ox, oy, tid, kThreads, STACK ? a.fstride * sl : 0);
+        flush_tile(a, c, accNum, accDen, hsum, ox, oy, tid);
+        __syncthreads();
+    }
+}
+
+
+// STACK: multi-frame groups over a whole stack (see FilterArgs): the tile's
+// accumulators form a ring of frame slots; frame f is flushed into its planes
+// when reference frame f + radius is done.
+template <bool WIENER, bool BATCHED, bool STACK>
+__global__ __launch_bounds__(kThreads) void filter_stack_kernel(FilterArgs a0, const uint32_t *__restrict__ bmap,
+                                                                const FilterArgs *__restrict__ bargs)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int tx, ty;
+    const FilterArgs &a = tile_unit<BATCHED>(a0, bargs, bmap, tx, ty);
+    const int trx = a.trx, tnum = a.trx * a.try_;
+    const int tx0 = tx * trx, ty0 = ty * a.try_;
+    const int ox = grid_pos(tx0, a.bstep, a.W - kBlock) - a.range;
+    const int oy = grid_pos(ty0, a.bstep, a.H - kBlock) - a.range;
+    const int FW = a.FW, FH = a.FH, DW = FW - kBlock + 1, DH = FH - kBlock + 1;
+    const int nslots = STACK ? a.nslots : 1, T = STACK ? a.frames : 1, rad = STACK ? a.radius : 0;
+    const size_t accw = filter_acc_floats(FW, FH);   // 32-bit words per slot
+    const size_t refs = (size_t)a.ngx * a.ngy;
+
+    u64 *accNum0 = reinterpret_cast<u64 *>(smem);
+    float *gbase = reinterpret_cast<float *>(smem) + accw * nslots;
+    float *g = gbase + (size_t)wave * a.G * ZS;
+    u64 *hsum = reinterpret_cast<u64 *>(gbase);  // flush only, after every wave is done with its groups
+    const int u = lane >> 3, v = lane & 7;
+    auto slot_num = [&](int f) { return accNum0 + (accw / 2) * (STACK ? f % nslots : 0); };
+    auto zero_slot = [&](int f) {
+        u64 *an = slot_num(f), *ad = an + FW * FH;
+        for (int k = tid; k < FW * FH; k += kThreads) an[k] = 0;
+        for (int k = tid; k < DW * DH; k += kThreads) ad[k] = 0;
+    };
+    auto flush_frame = [&](int c, int f) {
+        const u64 *an = slot_num(f);
+        flush_tile(a, c, an, an + FW * FH, hsum, ox, oy, tid, kThreads, STACK ? a.fstride * f : 0);
+        __syncthreads();
+    };
+
+    for (int c = 0; c < 3; ++c) {
+        if (!a.active[c]) continue;
+        for (int f = 0; f < nslots; ++f) zero_slot(f);
+        __syncthreads();
+        const float *src = a.src[c];
+        const float *gsrc = a.guide[c];
+        for (int t = 0; t < T; ++t) {
+            if (STACK && t > rad) {
+                // frame t - rad - 1 is complete; its slot takes frame t + rad
+                flush_frame(c, t - rad - 1);
+                zero_slot(t - rad - 1);
+                __syncthreads();
+            }
+            for (int r = wave; r < tnum; r += kWaves) {
+                const int gxi = tx0 + (r % trx), gyi = ty0 + (r / trx);
+                if (gxi >= a.ngx || gyi >= a.ngy) continue;
+                const size_t refIdx = (size_t)t * refs + (size_t)gyi * a.ngx + gxi;
+                const int *pos = a.pos + refIdx * a.G;
+                const int *slot = STACK ? a.slot + refIdx * a.G : nullptr;
+                const int N = a.cnt[refIdx];
+                const float thr = a.thr[c], sig2 = a.sig2[c];
+                const ExactCtx ex{pos, src, a.pitch, a.c8d, a.cnd + a.cnd_off[N], a.thr64[c], a.band[c], slot,
+                                  STACK ? a.fstride : 0};
+                const float *cn = a.cn + a.cn_off[N];
+                float w;
+                if (N <= 1) w = group_core<1, WIENER, STACK>(g, N, lane, a.c8, cn, src, gsrc, a.pitch, thr, sig2, ex);
+                else if (N <= 2) w = group_core<2, WIENER, STACK>(g, N, lane, a.c8, cn, src, gsrc, a.pitch, thr, sig2, ex);
+                else if (N <= 4) w = group_core<4, WIENER, STACK>(g, N, lane, a.c8, cn, src, gsrc, a.pitch, thr, sig2, ex);
+                else if (N <= 8) w = group_core<8, WIENER, STACK>(g, N, lane, a.c8, cn, src, gsrc, a.pitch, thr, sig2, ex);
+                else if (N <= 16)
+                    w = group_core<16, WIENER, STACK>(g, N, lane, a.c8, cn, src, gsrc, a.pitch, thr, sig2, ex);
+                else w = group_core<32, WIENER, STACK>(g, N, lane, a.c8, cn, src, gsrc, a.pitch, thr, sig2, ex);
+                const int o = gidx(0, lane);
+                for (int z = 0; z < N; ++z) {
+                    const int pz = pos[z];
+                    const int py = (pz >> 16) - oy, px = (pz & 0xFFFF) - ox;
+                    u64 *an = slot_num(STACK ? slot[z] : 0);
+                    fix_add(&an[(py + u) * FW + px + v], w * g[o + z * ZS]);
+                }
+                if (lane < N) {
+                    const int pz = pos[lane];
+                    u64 *ad = slot_num(STACK ? slot[lane] : 0) + FW * FH;
+                    fix_add(&ad[((pz >> 16) - oy) * DW + (pz & 0xFFFF) - ox], w);
+                }
+                wave_sync();
+            }
             __syncthreads();
         }
+        for (int f = T - rad - 1 < 0 ? 0 : T - rad - 1; f < T; ++f) flush_frame(c, f);
     }
 }
 
@@ -853,7 +961,7 @@ template <typename T>
 __device__ __forceinline__ void mfma_forward(const float *plane, int pitch, int ox, int oy,
                                              const int *__restrict__ pos, int N, int lane, const uint16_t *kb,
                                              const typename MfmaT<T>::v8 (&cna)[2], f32x16 (&y)[2],
-                                             const int *__restrict__ slot = nullptr, int slotsz = 0)
+                                             const int *__restrict__ slot = nullptr, size_t slotsz = 0)
 {
     typedef typename MfmaT<T>::v8 v8;
     const int c = lane & 31, h = lane >> 5;
@@ -862,6 +970,7 @@ __device__ __forceinline__ void mfma_forward(const float *plane, int pitch, int 
     if (c < N) {
         const int pz = pos[c];
         const float *b = plane + ((pz >> 16) - oy) * pitch + (pz & 0xFFFF) - ox + (slot ? slot[c] * slotsz : 0);
+
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
             const float *r = b + (2 * s + h) * pitch;
@@ -916,16 +1025,16 @@ __device__ __forceinline__ void mfma_inverse(const f32x16 (&y)[2], const uint16_
 // and inverse A, 16 x 64 lanes x 8 elements of 16 bits), then the staged
 // source (and guide) footprint; the flush's row sums reuse the staging area.
 constexpr int kK64Elems = 16 * 64 * 8;
-__host__ __device__ inline size_t filter_mfma_lds(int FW, int FH, bool wiener, int nslots = 1)
+// stack: nslots accumulator slots, no staged footprint (member blocks are
+// read from HBM), the flush's row sums in a region of their own.
+__host__ __device__ inline size_t filter_mfma_lds(int FW, int FH, bool wiener, int nslots = 1, bool stack = false)
 {
-    const size_t st = (size_t)FW * FH * (wiener ? 2 : 1) * nslots;
+    const size_t st = stack ? 0 : (size_t)FW * FH * (wiener ? 2 : 1);
     const size_t hs = (size_t)(FH - kBlock + 1) * FW * 2;
     return filter_acc_floats(FW, FH) * 4 * nslots + kK64Elems * 2 + (st > hs ? st : hs) * 4;
 }
 
-// STACK: multi-frame groups; accumulators and staged footprints per frame
-// slot of the window.
-template <typename T, bool WIENER, bool BATCHED, bool STACK = false>
+template <typename T, bool WIENER, bool BATCHED>
 __global__ __launch_bounds__(kMThreads) void filter_mfma_kernel(FilterArgs a0, const uint32_t *__restrict__ bmap,
                                                                 const FilterArgs *__restrict__ bargs)
 {
@@ -940,16 +1049,14 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_kernel(FilterArgs a0, c
     const int ox = grid_pos(tx0, a.bstep, a.W - kBlock) - a.range;
     const int oy = grid_pos(ty0, a.bstep, a.H - kBlock) - a.range;
     const int FW = a.FW, FH = a.FH, DW = FW - kBlock + 1, DH = FH - kBlock + 1;
-    const int nslots = STACK ? a.nslots : 1;
-    const size_t accw = filter_acc_floats(FW, FH);   // 32-bit words per slot
-    const int fsz = FW * FH;                          // staged floats per slot and plane
 
-    u64 *accNum0 = reinterpret_cast<u64 *>(smem);
-    uint16_t *kb = reinterpret_cast<uint16_t *>(reinterpret_cast<float *>(smem) + accw * nslots);
+    u64 *accNum = reinterpret_cast<u64 *>(smem);
+    u64 *accDen = accNum + FW * FH;
+    uint16_t *kb = reinterpret_cast<uint16_t *>(reinterpret_cast<float *>(smem) + filter_acc_floats(FW, FH));
     uint16_t *ka = kb + kK64Elems / 2;
     float *stage = reinterpret_cast<float *>(kb + kK64Elems);
     float *lsrc = stage;
-    float *lgd = stage + (size_t)fsz * nslots;
+    float *lgd = stage + FW * FH;
     u64 *hsum = reinterpret_cast<u64 *>(stage);  // flush only
     {
         // the K64 fragments (kb then ka are contiguous in the table)
@@ -962,20 +1069,16 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_kernel(FilterArgs a0, c
         if (!a.active[pl]) continue;
         const float *src = a.src[pl];
         const float *gsrc = a.guide[pl];
-        for (int sl = 0; sl < nslots; ++sl) {
-            u64 *an = accNum0 + (accw / 2) * sl, *ad = an + fsz;
-            const size_t fo = STACK ? a.fstride * sl : 0;
-            for (int k = tid; k < fsz; k += kMThreads) {
-                an[k] = 0;
-                const int py = k / FW, px = k - py * FW;
-                const int yy = oy + py, xx = ox + px;
-                const bool in = yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
-                const size_t o = (size_t)yy * a.pitch + xx + fo;
-                lsrc[sl * fsz + k] = in ? src[o] : 0.f;
-                if (WIENER) lgd[sl * fsz + k] = in ? gsrc[o] : 0.f;
-            }
-            for (int k = tid; k < DW * DH; k += kMThreads) ad[k] = 0;
+        for (int k = tid; k < FW * FH; k += kMThreads) {
+            accNum[k] = 0;
+            const int py = k / FW, px = k - py * FW;
+            const int yy = oy + py, xx = ox + px;
+            const bool in = yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
+            const size_t o = (size_t)yy * a.pitch + xx;
+            lsrc[k] = in ? src[o] : 0.f;
+            if (WIENER) lgd[k] = in ? gsrc[o] : 0.f;
         }
+        for (int k = tid; k < DW * DH; k += kMThreads) accDen[k] = 0;
         __syncthreads();
         const float thr = a.thr[pl], sig2 = a.sig2[pl];
         for (int r = wave; r < tnum; r += kMWaves) {
@@ -983,7 +1086,6 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_kernel(FilterArgs a0, c
             if (gxi >= a.ngx || gyi >= a.ngy) continue;
             const size_t refIdx = (size_t)gyi * a.ngx + gxi;
             const int *pos = a.pos + refIdx * a.G;
-            const int *slot = STACK ? a.slot + refIdx * a.G : nullptr;
             const int N = a.cnt[refIdx];
             v8 cna[2], cnb[2];
 #pragma unroll
@@ -994,7 +1096,7 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_kernel(FilterArgs a0, c
             f32x16 y[2];
             float w;
             if (!WIENER) {
-                mfma_forward<T>(lsrc, FW, ox, oy, pos, N, lane, kb, cna, y, slot, fsz);
+                mfma_forward<T>(lsrc, FW, ox, oy, pos, N, lane, kb, cna, y);
                 int kept = 0;
 #pragma unroll
                 for (int q = 0; q < 2; ++q)
@@ -1007,8 +1109,8 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_kernel(FilterArgs a0, c
                 w = kept < 1 ? 1.f : 1.f / (float)kept;
             } else {
                 f32x16 yg[2];
-                mfma_forward<T>(lgd, FW, ox, oy, pos, N, lane, kb, cna, yg, slot, fsz);
-                mfma_forward<T>(lsrc, FW, ox, oy, pos, N, lane, kb, cna, y, slot, fsz);
+                mfma_forward<T>(lgd, FW, ox, oy, pos, N, lane, kb, cna, yg);
+                mfma_forward<T>(lsrc, FW, ox, oy, pos, N, lane, kb, cna, y);
                 float l2 = 0.f;
 #pragma unroll
                 for (int q = 0; q < 2; ++q) {
@@ -1032,8 +1134,6 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_kernel(FilterArgs a0, c
             if (c < N) {
                 const int pz = pos[c];
                 const int py = (pz >> 16) - oy, px = (pz & 0xFFFF) - ox + 4 * h;
-                u64 *accNum = accNum0 + (STACK ? (accw / 2) * slot[c] : 0);
-                u64 *accDen = accNum + fsz;
 #pragma unroll
                 for (int rr = 0; rr < 2; ++rr)
 #pragma unroll
@@ -1045,11 +1145,157 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_kernel(FilterArgs a0, c
             }
         }
         __syncthreads();
-        for (int sl = 0; sl < nslots; ++sl) {
-            const u64 *an = accNum0 + (accw / 2) * sl;
-            flush_tile(a, pl, an, an + fsz, hsum, ox, oy, tid, kMThreads, STACK ? a.fstride * sl : 0);
+        flush_tile(a, pl, accNum, accDen, hsum, ox, oy, tid, kMThreads);
+        __syncthreads();
+    }
+}
+
+// STACK: multi-frame groups over a whole stack (see FilterArgs and
+// filter_stack_kernel): a ring of accumulator slots, member blocks read from HBM.
+template <typename T, bool WIENER, bool BATCHED, bool STACK>
+__global__ __launch_bounds__(kMThreads) void filter_mfma_stack_kernel(FilterArgs a0,
+                                                                      const uint32_t *__restrict__ bmap,
+                                                                      const FilterArgs *__restrict__ bargs)
+{
+    typedef typename MfmaT<T>::v8 v8;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int c = lane & 31, h = lane >> 5;
+    int tx, ty;
+    const FilterArgs &a = tile_unit<BATCHED>(a0, bargs, bmap, tx, ty);
+    const int trx = a.trx, tnum = a.trx * a.try_;
+    const int tx0 = tx * trx, ty0 = ty * a.try_;
+    const int ox = grid_pos(tx0, a.bstep, a.W - kBlock) - a.range;
+    const int oy = grid_pos(ty0, a.bstep, a.H - kBlock) - a.range;
+    const int FW = a.FW, FH = a.FH, DW = FW - kBlock + 1, DH = FH - kBlock + 1;
+    const int nslots = STACK ? a.nslots : 1, NF = STACK ? a.frames : 1, rad = STACK ? a.radius : 0;
+    const size_t accw = filter_acc_floats(FW, FH);   // 32-bit words per slot
+    const int fsz = FW * FH;                          // footprint pixels
+    const size_t refs = (size_t)a.ngx * a.ngy;
+
+    u64 *accNum0 = reinterpret_cast<u64 *>(smem);
+    uint16_t *kb = reinterpret_cast<uint16_t *>(reinterpret_cast<float *>(smem) + accw * nslots);
+    uint16_t *ka = kb + kK64Elems / 2;
+    float *stage = reinterpret_cast<float *>(kb + kK64Elems);
+    float *lsrc = stage;          // single frame: the source footprint staged in LDS
+    float *lgd = stage + fsz;     // and the Wiener guide's
+    u64 *hsum = reinterpret_cast<u64 *>(stage);  // flush only
+    {
+        // the K64 fragments (kb then ka are contiguous in the table)
+        const uint4 *srcq = reinterpret_cast<const uint4 *>(a.kb);
+        uint4 *dstq = reinterpret_cast<uint4 *>(kb);
+        for (int k = tid; k < kK64Elems / 8; k += kMThreads) dstq[k] = srcq[k];
+    }
+    auto slot_num = [&](int f) { return accNum0 + (accw / 2) * (STACK ? f % nslots : 0); };
+    auto zero_slot = [&](int f) {
+        u64 *an = slot_num(f), *ad = an + fsz;
+        for (int k = tid; k < fsz; k += kMThreads) an[k] = 0;
+        for (int k = tid; k < DW * DH; k += kMThreads) ad[k] = 0;
+    };
+    auto flush_frame = [&](int pl, int f) {
+        const u64 *an = slot_num(f);
+        flush_tile(a, pl, an, an + fsz, hsum, ox, oy, tid, kMThreads, STACK ? a.fstride * f : 0);
+        __syncthreads();
+    };
+
+    for (int pl = 0; pl < 3; ++pl) {
+        if (!a.active[pl]) continue;
+        const float *src = a.src[pl];
+        const float *gsrc = a.guide[pl];
+        if (!STACK) {
+            for (int k = tid; k < fsz; k += kMThreads) {
+                const int py = k / FW, px = k - py * FW;
+                const int yy = oy + py, xx = ox + px;
+                const bool in = yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
+                const size_t o = (size_t)yy * a.pitch + xx;
+                lsrc[k] = in ? src[o] : 0.f;
+                if (WIENER) lgd[k] = in ? gsrc[o] : 0.f;
+            }
+        }
+        for (int f = 0; f < nslots; ++f) zero_slot(f);
+        __syncthreads();
+        const float thr = a.thr[pl], sig2 = a.sig2[pl];
+        // where member blocks are read: the staged footprint, or the frames in HBM
+        const float *fsrc = STACK ? src : lsrc, *fgd = STACK ? gsrc : lgd;
+        const int fp = STACK ? a.pitch : FW, fox = STACK ? 0 : ox, foy = STACK ? 0 : oy;
+        for (int t = 0; t < NF; ++t) {
+            if (STACK && t > rad) {
+                // frame t - rad - 1 is complete; its slot takes frame t + rad
+                flush_frame(pl, t - rad - 1);
+                zero_slot(t - rad - 1);
+                __syncthreads();
+            }
+            for (int r = wave; r < tnum; r += kMWaves) {
+                const int gxi = tx0 + (r % trx), gyi = ty0 + (r / trx);
+                if (gxi >= a.ngx || gyi >= a.ngy) continue;
+                const size_t refIdx = (size_t)t * refs + (size_t)gyi * a.ngx + gxi;
+                const int *pos = a.pos + refIdx * a.G;
+                const int *slot = STACK ? a.slot + refIdx * a.G : nullptr;
+                const size_t sstride = STACK ? a.fstride : 0;
+                const int N = a.cnt[refIdx];
+                v8 cna[2], cnb[2];
+#pragma unroll
+                for (int m = 0; m < 2; ++m) {
+                    cna[m] = load_frag<T>(a.cna, (N - 1) * 2 + m, lane);
+                    cnb[m] = load_frag<T>(a.cnb, (N - 1) * 2 + m, lane);
+                }
+                f32x16 y[2];
+                float w;
+                if (!WIENER) {
+                    mfma_forward<T>(fsrc, fp, fox, foy, pos, N, lane, kb, cna, y, slot, sstride);
+                    int kept = 0;
+#pragma unroll
+                    for (int q = 0; q < 2; ++q)
+#pragma unroll
+                        for (int k = 0; k < 16; ++k) {
+                            if (fabsf(y[q][k]) <= thr) y[q][k] = 0.f;
+                            else ++kept;
+                        }
+                    kept = wave_sum(kept);
+                    w = kept < 1 ? 1.f : 1.f / (float)kept;
+                } else {
+                    f32x16 yg[2];
+                    mfma_forward<T>(fgd, fp, fox, foy, pos, N, lane, kb, cna, yg, slot, sstride);
+                    mfma_forward<T>(fsrc, fp, fox, foy, pos, N, lane, kb, cna, y, slot, sstride);
+                    float l2 = 0.f;
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) {
+                        // column 32q + c = frequency (u, v): u = 4q + c/8, v = c%8
+                        const float su =
+                            sig2 * ((4 * q + (c >> 3)) == 0 ? 0.5f : 1.f) * ((c & 7) == 0 ? 0.5f : 1.f);
+#pragma unroll
+                        for (int k = 0; k < 16; ++k) {
+                            const int kz = 8 * (k >> 2) + 4 * h + (k & 3);  // group-axis frequency
+                            const float yy = yg[q][k] * yg[q][k];
+                            // 1-ulp reciprocal: the operands already carry 16-bit rounding
+                            const float wc = yy * __builtin_amdgcn_rcpf(yy + (kz == 0 ? 0.5f * su : su));
+                            y[q][k] *= wc;
+                            l2 = fmaf(wc, wc, l2);
+                        }
+                    }
+                    l2 = wave_sum(l2);
+                    w = l2 <= 0.f ? 1.f : 1.f / l2;
+                }
+                f32x16 xh[2];
+                mfma_inverse<T>(y, ka, lane, cnb, xh);
+                if (c < N) {
+                    const int pz = pos[c];
+                    const int py = (pz >> 16) - oy, px = (pz & 0xFFFF) - ox + 4 * h;
+                    u64 *accNum = slot_num(STACK ? slot[c] : 0);
+                    u64 *accDen = accNum + fsz;
+#pragma unroll
+                    for (int rr = 0; rr < 2; ++rr)
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+#pragma unroll
+                            for (int j = 0; j < 4; ++j)
+                                fix_add(&accNum[(py + 4 * rr + i) * FW + px + j], w * xh[rr][4 * i + j]);
+                    if (h == 0) fix_add(&accDen[py * DW + (pz & 0xFFFF) - ox], w);
+                }
+            }
             __syncthreads();
         }
+        for (int f = NF - rad - 1 < 0 ? 0 : NF - rad - 1; f < NF; ++f) flush_frame(pl, f);
     }
 }
 
@@ -1184,6 +1430,14 @@ hipError_t launch_match_t(const MatchArgs &a, hipStream_t s, const BatchLaunch *
 {
     const size_t lds = match_lds_bytes(a);
     hipError_t e;
+    if (a.search) {
+        if (b) return hipErrorInvalidValue;
+        auto k = match_kernel<GT, DT, ST1, EXACT, false, true>;
+        if ((e = set_lds(k, lds)) != hipSuccess) return e;
+        const dim3 grid((a.ngx + a.trx - 1) / a.trx, (a.ngy + a.try_ - 1) / a.try_);
+        hipLaunchKernelGGL(k, grid, dim3(kThreads), lds, s, a, nullptr, nullptr);
+        return hipGetLastError();
+    }
     if (b) {
         auto k = match_kernel<GT, DT, ST1, EXACT, true>;
         if ((e = set_lds(k, lds)) != hipSuccess) return e;
@@ -1250,7 +1504,7 @@ size_t filter_mfma_lds_bytes(const FilterArgs &a, bool wiener) { return filter_m
 
 size_t filter_stack_lds_bytes(const FilterArgs &a, bool wiener, int precision)
 {
-    if (precision != 0) return filter_mfma_lds(a.FW, a.FH, wiener, a.nslots);
+    if (precision != 0) return filter_mfma_lds(a.FW, a.FH, wiener, a.nslots, true);
     const int DH = a.FH - kBlock + 1;
     size_t groups = (size_t)kWaves * a.G * ZS;
     const size_t hs = (size_t)DH * a.FW * 2;
@@ -1263,14 +1517,14 @@ hipError_t launch_filter_stack(const FilterArgs &a, bool wiener, int precision, 
     const size_t lds = filter_stack_lds_bytes(a, wiener, precision);
     const dim3 grid((a.ngx + a.trx - 1) / a.trx, (a.ngy + a.try_ - 1) / a.try_);
     if (precision == 0)
-        return wiener ? launch_tiled(filter_kernel<true, false, true>, kThreads, lds, a, s, nullptr, grid)
-                      : launch_tiled(filter_kernel<false, false, true>, kThreads, lds, a, s, nullptr, grid);
+        return wiener ? launch_tiled(filter_stack_kernel<true, false, true>, kThreads, lds, a, s, nullptr, grid)
+                      : launch_tiled(filter_stack_kernel<false, false, true>, kThreads, lds, a, s, nullptr, grid);
     if (precision == 1)
-        return wiener ? launch_tiled(filter_mfma_kernel<_Float16, true, false, true>, kMThreads, lds, a, s, nullptr, grid)
-                      : launch_tiled(filter_mfma_kernel<_Float16, false, false, true>, kMThreads, lds, a, s, nullptr,
+        return wiener ? launch_tiled(filter_mfma_stack_kernel<_Float16, true, false, true>, kMThreads, lds, a, s, nullptr, grid)
+                      : launch_tiled(filter_mfma_stack_kernel<_Float16, false, false, true>, kMThreads, lds, a, s, nullptr,
                                      grid);
-    return wiener ? launch_tiled(filter_mfma_kernel<__bf16, true, false, true>, kMThreads, lds, a, s, nullptr, grid)
-                  : launch_tiled(filter_mfma_kernel<__bf16, false, false, true>, kMThreads, lds, a, s, nullptr, grid);
+    return wiener ? launch_tiled(filter_mfma_stack_kernel<__bf16, true, false, true>, kMThreads, lds, a, s, nullptr, grid)
+                  : launch_tiled(filter_mfma_stack_kernel<__bf16, false, false, true>, kMThreads, lds, a, s, nullptr, grid);
 }
 
 template <typename T, bool W>
