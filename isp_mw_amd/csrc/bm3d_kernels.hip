@@ -131,6 +131,16 @@ __device__ __forceinline__ void fix_add(u64 *acc, float v)
     atomicAdd(acc, ((u64)(long long)(int)hi << 32) + lo);
 }
 
+// fix_add(acc, w * x) with ws = w * 2^4 (the scale folded into the weight:
+// the product w * x * 2^4 rounds as (w * x) * 2^4, a power-of-two scaling)
+__device__ __forceinline__ void fix_add_scaled(u64 *acc, float x, float ws)
+{
+    const float s = x * ws;
+    const float hi = floorf(s);
+    const uint32_t lo = (uint32_t)((s - hi) * 4294967296.0f);
+    atomicAdd(acc, ((u64)(long long)(int)hi << 32) + lo);
+}
+
 __host__ __device__ inline size_t match_wave_bytes(int maxCand, int keyBytes)
 {
     // keys K[maxCand] + selk K[32] + idxs u32[maxCand] + memb u32[maxCand]
@@ -789,10 +799,11 @@ __global__ __launch_bounds__(kThreads) void filter_kernel(FilterArgs a0, const u
             else if (N <= 16) w = group_core<16, WIENER>(g, N, lane, a.c8, cn, src, gsrc, a.pitch, thr, sig2, ex);
             else w = group_core<32, WIENER>(g, N, lane, a.c8, cn, src, gsrc, a.pitch, thr, sig2, ex);
             const int o = gidx(0, lane);
+            const float ws = w * (kFixScale / 4294967296.0f);
             for (int z = 0; z < N; ++z) {
                 const int pz = pos[z];
                 const int py = (pz >> 16) - oy, px = (pz & 0xFFFF) - ox;
-                fix_add(&accNum[(py + u) * FW + px + v], w * g[o + z * ZS]);
+                fix_add_scaled(&accNum[(py + u) * FW + px + v], g[o + z * ZS], ws);
             }
             if (lane < N) {
                 const int pz = pos[lane];
@@ -877,11 +888,12 @@ __global__ __launch_bounds__(kThreads) void filter_stack_kernel(FilterArgs a0, c
                     w = group_core<16, WIENER, STACK>(g, N, lane, a.c8, cn, src, gsrc, a.pitch, thr, sig2, ex);
                 else w = group_core<32, WIENER, STACK>(g, N, lane, a.c8, cn, src, gsrc, a.pitch, thr, sig2, ex);
                 const int o = gidx(0, lane);
+                const float ws = w * (kFixScale / 4294967296.0f);
                 for (int z = 0; z < N; ++z) {
                     const int pz = pos[z];
                     const int py = (pz >> 16) - oy, px = (pz & 0xFFFF) - ox;
                     u64 *an = slot_num(STACK ? slot[z] : 0);
-                    fix_add(&an[(py + u) * FW + px + v], w * g[o + z * ZS]);
+                    fix_add_scaled(&an[(py + u) * FW + px + v], g[o + z * ZS], ws);
                 }
                 if (lane < N) {
                     const int pz = pos[lane];
@@ -998,6 +1010,69 @@ __device__ __forceinline__ void mfma_forward(const float *plane, int pitch, int 
     }
 }
 
+// The forward and inverse one column half q at a time (the same products as
+// mfma_forward / mfma_inverse, in the same order), so that fewer accumulators
+// are live at once: X's fragments, y_q = columns 32q..32q+31 of C_N X K64,
+// vt_q = y_q^T C_N, and the remaining inverse product from vt_0, vt_1.
+template <typename T>
+__device__ __forceinline__ void mfma_load_x(const float *plane, int pitch, int ox, int oy,
+                                            const int *__restrict__ pos, int N, int lane,
+                                            typename MfmaT<T>::v8 (&xa)[4], const int *__restrict__ slot = nullptr,
+                                            size_t slotsz = 0)
+{
+    const int c = lane & 31, h = lane >> 5;
+    if (c < N) {
+        const int pz = pos[c];
+        const float *b = plane + ((pz >> 16) - oy) * pitch + (pz & 0xFFFF) - ox + (slot ? slot[c] * slotsz : 0);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const float *r = b + (2 * s + h) * pitch;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) xa[s][e] = (T)r[e];
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) xa[s][e] = (T)0.f;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ f32x16 mfma_forward_half(const typename MfmaT<T>::v8 (&xa)[4], int q, const uint16_t *kb,
+                                                    int lane, const typename MfmaT<T>::v8 (&cna)[2])
+{
+    f32x16 t = (f32x16){};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) t = MfmaT<T>::mma(xa[s], lds_frag<T>(kb, s * 2 + q, lane), t);
+    f32x16 y = (f32x16){};
+#pragma unroll
+    for (int m = 0; m < 2; ++m) y = MfmaT<T>::mma(cna[m], acc_frag<T>(t, m), y);
+    return y;
+}
+
+template <typename T>
+__device__ __forceinline__ f32x16 mfma_inverse_half(const f32x16 &yq, const typename MfmaT<T>::v8 (&cnb)[2])
+{
+    f32x16 vt = (f32x16){};
+#pragma unroll
+    for (int m = 0; m < 2; ++m) vt = MfmaT<T>::mma(acc_frag<T>(yq, m), cnb[m], vt);
+    return vt;
+}
+
+template <typename T>
+__device__ __forceinline__ void mfma_inverse_rest(const f32x16 (&vt)[2], const uint16_t *ka, int lane,
+                                                  f32x16 (&xh)[2])
+{
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        xh[r] = (f32x16){};
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+            xh[r] = MfmaT<T>::mma(lds_frag<T>(ka, r * 4 + s, lane), acc_frag<T>(vt[s >> 1], s & 1), xh[r]);
+    }
+}
+
 // Inverse: xh[r] = rows 32r..32r+31 (pixels) of Xhat^T = K64 (Y'^T C_N).
 template <typename T>
 __device__ __forceinline__ void mfma_inverse(const f32x16 (&y)[2], const uint16_t *ka, int lane,
@@ -1034,8 +1109,9 @@ __host__ __device__ inline size_t filter_mfma_lds(int FW, int FH, bool wiener, i
     return filter_acc_floats(FW, FH) * 4 * nslots + kK64Elems * 2 + (st > hs ? st : hs) * 4;
 }
 
+// 4 waves per SIMD: two 8-wave workgroups per CU (at most 128 registers)
 template <typename T, bool WIENER, bool BATCHED>
-__global__ __launch_bounds__(kMThreads) void filter_mfma_kernel(FilterArgs a0, const uint32_t *__restrict__ bmap,
+__global__ __launch_bounds__(kMThreads, 4) void filter_mfma_kernel(FilterArgs a0, const uint32_t *__restrict__ bmap,
                                                                 const FilterArgs *__restrict__ bargs)
 {
     typedef typename MfmaT<T>::v8 v8;
@@ -1093,9 +1169,10 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_kernel(FilterArgs a0, c
                 cna[m] = load_frag<T>(a.cna, (N - 1) * 2 + m, lane);
                 cnb[m] = load_frag<T>(a.cnb, (N - 1) * 2 + m, lane);
             }
-            f32x16 y[2];
             float w;
+            f32x16 xh[2];
             if (!WIENER) {
+                f32x16 y[2];
                 mfma_forward<T>(lsrc, FW, ox, oy, pos, N, lane, kb, cna, y);
                 int kept = 0;
 #pragma unroll
@@ -1107,30 +1184,37 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_kernel(FilterArgs a0, c
                     }
                 kept = wave_sum(kept);
                 w = kept < 1 ? 1.f : 1.f / (float)kept;
+                mfma_inverse<T>(y, ka, lane, cnb, xh);
             } else {
-                f32x16 yg[2];
-                mfma_forward<T>(lgd, FW, ox, oy, pos, N, lane, kb, cna, yg);
-                mfma_forward<T>(lsrc, FW, ox, oy, pos, N, lane, kb, cna, y);
+                // one column half at a time: guide and source forward, gains,
+                // then that half's first inverse product
+                f32x16 vt[2];
                 float l2 = 0.f;
+                v8 xg[4], xs[4];
+                mfma_load_x<T>(lgd, FW, ox, oy, pos, N, lane, xg);
+                mfma_load_x<T>(lsrc, FW, ox, oy, pos, N, lane, xs);
 #pragma unroll
                 for (int q = 0; q < 2; ++q) {
+                    const f32x16 yg = mfma_forward_half<T>(xg, q, kb, lane, cna);
+                    f32x16 yq = mfma_forward_half<T>(xs, q, kb, lane, cna);
                     // column 32q + c = frequency (u, v): u = 4q + c/8, v = c%8
                     const float su = sig2 * ((4 * q + (c >> 3)) == 0 ? 0.5f : 1.f) * ((c & 7) == 0 ? 0.5f : 1.f);
 #pragma unroll
                     for (int k = 0; k < 16; ++k) {
                         const int kz = 8 * (k >> 2) + 4 * h + (k & 3);  // group-axis frequency
-                        const float yy = yg[q][k] * yg[q][k];
+                        const float yy = yg[k] * yg[k];
                         // 1-ulp reciprocal: the operands already carry 16-bit rounding
                         const float wc = yy * __builtin_amdgcn_rcpf(yy + (kz == 0 ? 0.5f * su : su));
-                        y[q][k] *= wc;
+                        yq[k] *= wc;
                         l2 = fmaf(wc, wc, l2);
                     }
+                    vt[q] = mfma_inverse_half<T>(yq, cnb);
                 }
                 l2 = wave_sum(l2);
                 w = l2 <= 0.f ? 1.f : 1.f / l2;
+                mfma_inverse_rest<T>(vt, ka, lane, xh);
             }
-            f32x16 xh[2];
-            mfma_inverse<T>(y, ka, lane, cnb, xh);
+            const float ws = w * (kFixScale / 4294967296.0f);
             if (c < N) {
                 const int pz = pos[c];
                 const int py = (pz >> 16) - oy, px = (pz & 0xFFFF) - ox + 4 * h;
@@ -1140,7 +1224,7 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_kernel(FilterArgs a0, c
                     for (int i = 0; i < 4; ++i)
 #pragma unroll
                         for (int j = 0; j < 4; ++j)
-                            fix_add(&accNum[(py + 4 * rr + i) * FW + px + j], w * xh[rr][4 * i + j]);
+                            fix_add_scaled(&accNum[(py + 4 * rr + i) * FW + px + j], xh[rr][4 * i + j], ws);
                 if (h == 0) fix_add(&accDen[py * DW + (pz & 0xFFFF) - ox], w);
             }
         }
@@ -1239,9 +1323,10 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_stack_kernel(FilterArgs
                     cna[m] = load_frag<T>(a.cna, (N - 1) * 2 + m, lane);
                     cnb[m] = load_frag<T>(a.cnb, (N - 1) * 2 + m, lane);
                 }
-                f32x16 y[2];
                 float w;
+                f32x16 xh[2];
                 if (!WIENER) {
+                    f32x16 y[2];
                     mfma_forward<T>(fsrc, fp, fox, foy, pos, N, lane, kb, cna, y, slot, sstride);
                     int kept = 0;
 #pragma unroll
@@ -1253,31 +1338,37 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_stack_kernel(FilterArgs
                         }
                     kept = wave_sum(kept);
                     w = kept < 1 ? 1.f : 1.f / (float)kept;
+                    mfma_inverse<T>(y, ka, lane, cnb, xh);
                 } else {
-                    f32x16 yg[2];
-                    mfma_forward<T>(fgd, fp, fox, foy, pos, N, lane, kb, cna, yg, slot, sstride);
-                    mfma_forward<T>(fsrc, fp, fox, foy, pos, N, lane, kb, cna, y, slot, sstride);
+                    // as filter_mfma_kernel: one column half at a time
+                    v8 xg[4], xs[4];
+                    mfma_load_x<T>(fgd, fp, fox, foy, pos, N, lane, xg, slot, sstride);
+                    mfma_load_x<T>(fsrc, fp, fox, foy, pos, N, lane, xs, slot, sstride);
+                    f32x16 vt[2];
                     float l2 = 0.f;
 #pragma unroll
                     for (int q = 0; q < 2; ++q) {
+                        const f32x16 yg = mfma_forward_half<T>(xg, q, kb, lane, cna);
+                        f32x16 yq = mfma_forward_half<T>(xs, q, kb, lane, cna);
                         // column 32q + c = frequency (u, v): u = 4q + c/8, v = c%8
                         const float su =
                             sig2 * ((4 * q + (c >> 3)) == 0 ? 0.5f : 1.f) * ((c & 7) == 0 ? 0.5f : 1.f);
 #pragma unroll
                         for (int k = 0; k < 16; ++k) {
                             const int kz = 8 * (k >> 2) + 4 * h + (k & 3);  // group-axis frequency
-                            const float yy = yg[q][k] * yg[q][k];
+                            const float yy = yg[k] * yg[k];
                             // 1-ulp reciprocal: the operands already carry 16-bit rounding
                             const float wc = yy * __builtin_amdgcn_rcpf(yy + (kz == 0 ? 0.5f * su : su));
-                            y[q][k] *= wc;
+                            yq[k] *= wc;
                             l2 = fmaf(wc, wc, l2);
                         }
+                        vt[q] = mfma_inverse_half<T>(yq, cnb);
                     }
                     l2 = wave_sum(l2);
                     w = l2 <= 0.f ? 1.f : 1.f / l2;
+                    mfma_inverse_rest<T>(vt, ka, lane, xh);
                 }
-                f32x16 xh[2];
-                mfma_inverse<T>(y, ka, lane, cnb, xh);
+                const float ws = w * (kFixScale / 4294967296.0f);
                 if (c < N) {
                     const int pz = pos[c];
                     const int py = (pz >> 16) - oy, px = (pz & 0xFFFF) - ox + 4 * h;
@@ -1289,7 +1380,7 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_stack_kernel(FilterArgs
                         for (int i = 0; i < 4; ++i)
 #pragma unroll
                             for (int j = 0; j < 4; ++j)
-                                fix_add(&accNum[(py + 4 * rr + i) * FW + px + j], w * xh[rr][4 * i + j]);
+                                fix_add_scaled(&accNum[(py + 4 * rr + i) * FW + px + j], xh[rr][4 * i + j], ws);
                     if (h == 0) fix_add(&accDen[py * DW + (pz & 0xFFFF) - ox], w);
                 }
             }
