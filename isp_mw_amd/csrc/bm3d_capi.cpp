@@ -482,6 +482,8 @@ FilterArgs build_filter(const isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, 
         f.ka = t + 8 * 64 * 8;
         f.cna = t + 16 * 64 * 8;
         f.cnb = f.cna + (size_t)kMaxGroup * 2 * 64 * 8;
+        f.cnap = f.cnb + (size_t)kMaxGroup * 2 * 64 * 8;
+        f.cnbp = f.cnap + (size_t)kPairTables * 2 * 64 * 8;
     }
     return f;
 }
@@ -694,6 +696,23 @@ std::vector<float> mfma_tables_f32(size_t *n_kb, size_t *n_ka, size_t *n_cn)
             for (int l = 0; l < 64; ++l)
                 for (int e = 0; e < 8; ++e) t.push_back((float)cn(N, perm(m, l >> 5, e), l & 31));
     *n_cn = (t.size() - *n_kb - *n_ka) / 2;
+    // two groups of N1, N2 <= 16 blocks in one operand: block-diagonal
+    // diag(C_N1, C_N2), group 1 on rows / columns 0..15, group 2 on 16..31
+    auto cpair = [&](int N1, int N2, int k, int z) -> double {
+        if (k < 16 && z < 16) return cn(N1, k, z);
+        if (k >= 16 && z >= 16 && N2 > 0) return cn(N2, k - 16, z - 16);
+        return 0.0;
+    };
+    for (int N1 = 1; N1 <= 16; ++N1)
+        for (int N2 = 0; N2 <= 16; ++N2)
+            for (int m = 0; m < 2; ++m)
+                for (int l = 0; l < 64; ++l)
+                    for (int e = 0; e < 8; ++e) t.push_back((float)cpair(N1, N2, l & 31, perm(m, l >> 5, e)));
+    for (int N1 = 1; N1 <= 16; ++N1)
+        for (int N2 = 0; N2 <= 16; ++N2)
+            for (int m = 0; m < 2; ++m)
+                for (int l = 0; l < 64; ++l)
+                    for (int e = 0; e < 8; ++e) t.push_back((float)cpair(N1, N2, perm(m, l >> 5, e), l & 31));
     return t;
 }
 

@@ -23,6 +23,7 @@ constexpr int kMTileRY = 8;
 constexpr int kMWaves = 8;
 constexpr int kMThreads = 64 * kMWaves;
 constexpr int kMaxTileRefs = 256;  // trx * try_ bound
+constexpr int kPairTables = 16 * 17;  // (N1, N2) operand tables of two groups in one MFMA operand
 
 // Batched launches: one launch covers the tiles (or rows) of many images.
 // Workgroup b reads unit = map[b]: image (unit >> 20) and, for tiled kernels,
@@ -117,7 +118,9 @@ struct FilterArgs {
     // ka[2][4][64][8]  K64 as A of X^T = K64 V^T (inverse, spatial)
     // cna[N][2][64][8] C_N as A of Y = C_N T (forward, group axis)
     // cnb[N][2][64][8] C_N as B of V^T = Y^T C_N (inverse, group axis)
-    const uint16_t *kb, *ka, *cna, *cnb;
+    // cnap / cnbp[(N1-1)*17 + N2][2][64][8]: the same for two groups of
+    // N1 (1..16) and N2 (0..16) blocks in one operand, diag(C_N1, C_N2)
+    const uint16_t *kb, *ka, *cna, *cnb, *cnap, *cnbp;
     // Multi-frame groups (stack kernels only).  One launch covers every
     // frame of the stack: the groups of reference frame t are cnt / pos /
     // slot + t * (ngx*ngy) (times G for pos and slot); member z of a group

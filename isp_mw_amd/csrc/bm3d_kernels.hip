@@ -1038,6 +1038,27 @@ __device__ __forceinline__ void mfma_load_x(const float *plane, int pitch, int o
     }
 }
 
+// The lane's rows of one member block (packed position pz, or zeros when pz < 0).
+template <typename T>
+__device__ __forceinline__ void mfma_load_block(const float *plane, int pitch, int ox, int oy, int pz, int h,
+                                                typename MfmaT<T>::v8 (&xa)[4])
+{
+    if (pz >= 0) {
+        const float *b = plane + ((pz >> 16) - oy) * pitch + (pz & 0xFFFF) - ox;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const float *r = b + (2 * s + h) * pitch;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) xa[s][e] = (T)r[e];
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) xa[s][e] = (T)0.f;
+    }
+}
+
 template <typename T>
 __device__ __forceinline__ f32x16 mfma_forward_half(const typename MfmaT<T>::v8 (&xa)[4], int q, const uint16_t *kb,
                                                     int lane, const typename MfmaT<T>::v8 (&cna)[2])
@@ -1157,6 +1178,9 @@ __global__ __launch_bounds__(kMThreads, 4) void filter_mfma_kernel(FilterArgs a0
         for (int k = tid; k < DW * DH; k += kMThreads) accDen[k] = 0;
         __syncthreads();
         const float thr = a.thr[pl], sig2 = a.sig2[pl];
+        if constexpr (WIENER) {
+            // one group per pass (a paired schedule costs the Wiener path its
+            // register budget: it spills)
         for (int r = wave; r < tnum; r += kMWaves) {
             const int gxi = tx0 + (r % trx), gyi = ty0 + (r / trx);
             if (gxi >= a.ngx || gyi >= a.ngy) continue;
@@ -1227,6 +1251,124 @@ __global__ __launch_bounds__(kMThreads, 4) void filter_mfma_kernel(FilterArgs a0
                             fix_add_scaled(&accNum[(py + 4 * rr + i) * FW + px + j], xh[rr][4 * i + j], ws);
                 if (h == 0) fix_add(&accDen[py * DW + (pz & 0xFFFF) - ox], w);
             }
+        }
+        } else {
+        // Reference blocks in pairs.  Two groups of at most 16 blocks share
+        // one 32-row operand (group 0 on rows 0..15, group 1 on 16..31, the
+        // group-axis DCT block-diagonal); otherwise each group takes a pass
+        // of its own on rows 0..N-1.
+        const int npairs = (tnum + 1) / 2;
+        for (int pi = wave; pi < npairs; pi += kMWaves) {
+            int NA = 0, NB = 0;
+            size_t refA = 0, refB = 0;
+            {
+                const int rA = 2 * pi, rB = rA + 1;
+                const int gxA = tx0 + (rA % trx), gyA = ty0 + (rA / trx);
+                if (gxA < a.ngx && gyA < a.ngy) {
+                    refA = (size_t)gyA * a.ngx + gxA;
+                    NA = a.cnt[refA];
+                }
+                const int gxB = tx0 + (rB % trx), gyB = ty0 + (rB / trx);
+                if (rB < tnum && gxB < a.ngx && gyB < a.ngy) {
+                    refB = (size_t)gyB * a.ngx + gxB;
+                    NB = a.cnt[refB];
+                }
+            }
+            const bool paired = NA > 0 && NB > 0 && NA <= 16 && NB <= 16;
+            const int passes = paired ? 1 : (NA > 0) + (NB > 0);
+            for (int ps = 0; ps < passes; ++ps) {
+                const bool first = paired || (ps == 0 && NA > 0);
+                const int N0 = first ? NA : NB, N1 = paired ? NB : 0;
+                const int *pos0 = a.pos + (first ? refA : refB) * a.G, *pos1 = a.pos + refB * a.G;
+                // this lane's member: group g, index mi
+                const int g = paired ? (c >> 4) : 0, mi = paired ? (c & 15) : c;
+                const int Ng = g ? N1 : N0;
+                const int pz = mi < Ng ? (g ? pos1 : pos0)[mi] : -1;
+                v8 cna[2], cnb[2];
+                {
+                    const uint16_t *ta = paired ? a.cnap : a.cna, *tb = paired ? a.cnbp : a.cnb;
+                    const int fi = paired ? (N0 - 1) * 17 + N1 : N0 - 1;
+#pragma unroll
+                    for (int m = 0; m < 2; ++m) {
+                        cna[m] = load_frag<T>(ta, fi * 2 + m, lane);
+                        cnb[m] = load_frag<T>(tb, fi * 2 + m, lane);
+                    }
+                }
+                // accumulator element k of this lane lies on row 8(k/4) + 4h + k%4,
+                // of group 1 when paired and k >= 8
+                float w0, w1;
+                f32x16 xh[2];
+                if (!WIENER) {
+                    v8 xs[4];
+                    mfma_load_block<T>(lsrc, FW, ox, oy, pz, h, xs);
+                    f32x16 y[2];
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) y[q] = mfma_forward_half<T>(xs, q, kb, lane, cna);
+                    int kept = 0;  // group 0 in the low 16 bits, group 1 above
+#pragma unroll
+                    for (int q = 0; q < 2; ++q)
+#pragma unroll
+                        for (int k = 0; k < 16; ++k) {
+                            if (fabsf(y[q][k]) <= thr) y[q][k] = 0.f;
+                            else kept += (paired && k >= 8) ? 65536 : 1;
+                        }
+                    kept = wave_sum(kept);
+                    const int k0 = kept & 0xFFFF, k1 = kept >> 16;
+                    w0 = k0 < 1 ? 1.f : 1.f / (float)k0;
+                    w1 = k1 < 1 ? 1.f : 1.f / (float)k1;
+                    mfma_inverse<T>(y, ka, lane, cnb, xh);
+                } else {
+                    // one column half at a time: guide and source forward, gains,
+                    // then that half's first inverse product
+                    f32x16 vt[2];
+                    float l20 = 0.f, l21 = 0.f;
+                    v8 xg[4], xs[4];
+                    mfma_load_block<T>(lgd, FW, ox, oy, pz, h, xg);
+                    mfma_load_block<T>(lsrc, FW, ox, oy, pz, h, xs);
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) {
+                        const f32x16 yg = mfma_forward_half<T>(xg, q, kb, lane, cna);
+                        f32x16 yq = mfma_forward_half<T>(xs, q, kb, lane, cna);
+                        // column 32q + c = frequency (u, v): u = 4q + c/8, v = c%8
+                        const float su =
+                            sig2 * ((4 * q + (c >> 3)) == 0 ? 0.5f : 1.f) * ((c & 7) == 0 ? 0.5f : 1.f);
+#pragma unroll
+                        for (int k = 0; k < 16; ++k) {
+                            // group-axis frequency (row within the group)
+                            const int kz = (8 * (k >> 2) + 4 * h + (k & 3)) & (paired ? 15 : 31);
+                            const float yy = yg[k] * yg[k];
+                            // 1-ulp reciprocal: the operands already carry 16-bit rounding
+                            const float wc = yy * __builtin_amdgcn_rcpf(yy + (kz == 0 ? 0.5f * su : su));
+                            yq[k] *= wc;
+                            if (paired && k >= 8) l21 = fmaf(wc, wc, l21);
+                            else l20 = fmaf(wc, wc, l20);
+                        }
+                        vt[q] = mfma_inverse_half<T>(yq, cnb);
+                    }
+                    l20 = wave_sum(l20);
+                    w0 = l20 <= 0.f ? 1.f : 1.f / l20;
+                    w1 = 1.f;
+                    if (paired) {
+                        l21 = wave_sum(l21);
+                        w1 = l21 <= 0.f ? 1.f : 1.f / l21;
+                    }
+                    mfma_inverse_rest<T>(vt, ka, lane, xh);
+                }
+                const float w = g ? w1 : w0;
+                const float ws = w * (kFixScale / 4294967296.0f);
+                if (pz >= 0) {
+                    const int py = (pz >> 16) - oy, px = (pz & 0xFFFF) - ox + 4 * h;
+#pragma unroll
+                    for (int rr = 0; rr < 2; ++rr)
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+#pragma unroll
+                            for (int j = 0; j < 4; ++j)
+                                fix_add_scaled(&accNum[(py + 4 * rr + i) * FW + px + j], xh[rr][4 * i + j], ws);
+                    if (h == 0) fix_add(&accDen[py * DW + (pz & 0xFFFF) - ox], w);
+                }
+            }
+        }
         }
         __syncthreads();
         flush_tile(a, pl, accNum, accDen, hsum, ox, oy, tid, kMThreads);
