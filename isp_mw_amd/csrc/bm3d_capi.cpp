@@ -419,7 +419,7 @@ MatchArgs build_match(const isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, co
     a.tileH = (try_ - 1) * p.block_step + 2 * d.range + kBlock;
     a.tilePitch = a.tileW | 1;
     const int side = 2 * (d.range / p.bm_step) + 1;
-    a.maxCand = side * side;
+    a.maxCand = std::min(side * side, kMatchListCap);
     a.cnt_out = cnt;
     a.pos_out = pos;
     a.key_out = keys;

@@ -15,6 +15,10 @@ constexpr int kMaxGroup = 32;    // largest GroupSize implemented
 constexpr int kTileRX = 8;       // matching and the FP32 filter
 constexpr int kTileRY = 4;
 constexpr int kStrip = 5;        // candidates per lane strip in matching
+// Capacity of a wave's accepted-candidate list in matching.  Windows with
+// more candidates reduce the list to its G-1 best whenever the next batch of
+// 64 strips could overflow it (capacity >= 31 + 64 * kStrip).
+constexpr int kMatchListCap = 384;
 constexpr int kWaves = 4;        // waves per workgroup
 constexpr int kThreads = 64 * kWaves;
 // MFMA filter: 8 waves over a tile of 8 x 8 reference blocks by default
@@ -58,7 +62,7 @@ struct MatchArgs {
     double thSSE;            // thMSE * normY * MSE2SSE
     double distMul;          // 1 / MSE2SSE
     int tileW, tileH, tilePitch;  // staged guide tile (doubles)
-    int maxCand;             // window size bound (per-wave candidate list)
+    int maxCand;             // per-wave candidate list capacity: min(window size, kMatchListCap)
     int *cnt_out;            // [ngy*ngx]
     int *pos_out;            // [ngy*ngx*G]
     double *key_out;         // [ngy*ngx*G] or nullptr

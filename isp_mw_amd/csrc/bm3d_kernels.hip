@@ -200,6 +200,122 @@ struct KeyOf<float> {
     static __device__ __forceinline__ double value(T k) { return (double)__uint_as_float(k); }
 };
 
+// The K smallest (key, scan index) of the wave's list keys / idxs[0..cnt)
+// into selk / seli, unordered; returns how many (min(cnt, K)).  An MSB-first
+// radix select on the key bits (256-bin LDS histograms, wave prefix scan);
+// ties within the last bucket are ranked exactly by scan index.
+template <typename KT, int BITS>
+__device__ __forceinline__ uint32_t select_smallest(const KT *keys, const uint32_t *idxs, uint32_t cnt, uint32_t K,
+                                                    KT *selk, uint32_t *seli, uint32_t *hist, uint32_t *memb,
+                                                    int lane)
+{
+    uint32_t nsel = 0;
+    if (cnt <= K) {
+        for (uint32_t e = lane; e < cnt; e += 64) {
+            selk[e] = keys[e];
+            seli[e] = idxs[e];
+        }
+        nsel = cnt;
+    } else {
+        KT prefix = 0, pmask = 0;
+        uint32_t need = K;
+        bool whole = false;
+        for (int shift = BITS - 8;; shift -= 8) {
+            for (int q = lane; q < 256; q += 64) hist[q] = 0;
+            wave_sync();
+            for (uint32_t e = lane; e < cnt; e += 64) {
+                const KT k = keys[e];
+                if ((k & pmask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
+            }
+            wave_sync();
+            const uint32_t h0 = hist[4 * lane], h1 = hist[4 * lane + 1];
+            const uint32_t h2 = hist[4 * lane + 2], h3 = hist[4 * lane + 3];
+            const uint32_t s = h0 + h1 + h2 + h3;
+            const uint32_t incl = wave_incl_scan(s, lane);
+            const uint32_t excl = incl - s;
+            const uint64_t bm = __ballot(excl < need && need <= incl);
+            const int L = __ffsll((unsigned long long)bm) - 1;
+            uint32_t dig = 0, blw = 0, mm = 0;
+            if (lane == L) {
+                const uint32_t hh[4] = {h0, h1, h2, h3};
+                uint32_t c = excl;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (mm == 0 && c + hh[q] >= need) {
+                        dig = 4 * lane + q;
+                        blw = c;
+                        mm = hh[q];
+                    }
+                    c += hh[q];
+                }
+            }
+            dig = __shfl(dig, L);
+            blw = __shfl(blw, L);
+            mm = __shfl(mm, L);
+            need -= blw;
+            prefix |= (KT)dig << shift;
+            pmask |= (KT)0xFF << shift;
+            if (mm == need) {
+                whole = true;
+                break;
+            }
+            if (mm <= 64 || shift == 0) break;
+            wave_sync();
+        }
+        uint32_t nm = 0;
+        for (uint32_t e0 = 0; e0 < cnt; e0 += 64) {
+            const uint32_t e = e0 + lane;
+            const bool inr = e < cnt;
+            const KT k = inr ? keys[e] : ~(KT)0;
+            const bool below = inr && (k & pmask) < prefix;
+            const bool inb = inr && (k & pmask) == prefix;
+            const bool take = below || (whole && inb);
+            const uint64_t tm = __ballot(take);
+            if (take) {
+                const uint32_t sl = nsel + lanes_below(tm);
+                selk[sl] = k;
+                seli[sl] = idxs[e];
+            }
+            nsel += (uint32_t)__popcll(tm);
+            if (!whole) {
+                const uint64_t mb = __ballot(inb);
+                if (inb) memb[nm + lanes_below(mb)] = e;
+                nm += (uint32_t)__popcll(mb);
+            }
+        }
+        if (!whole) {
+            wave_sync();
+            for (uint32_t q0 = 0; q0 < nm; q0 += 64) {
+                const uint32_t q = q0 + lane;
+                const bool v = q < nm;
+                KT kq = 0;
+                uint32_t iq = 0, rank = 0;
+                if (v) {
+                    const uint32_t e = memb[q];
+                    kq = keys[e];
+                    iq = idxs[e];
+                    for (uint32_t r2 = 0; r2 < nm; ++r2) {
+                        const uint32_t e2 = memb[r2];
+                        const KT k2 = keys[e2];
+                        const uint32_t i2 = idxs[e2];
+                        rank += (k2 < kq) || (k2 == kq && i2 < iq);
+                    }
+                }
+                const bool take = v && rank < need;
+                const uint64_t tm = __ballot(take);
+                if (take) {
+                    const uint32_t sl = nsel + lanes_below(tm);
+                    selk[sl] = kq;
+                    seli[sl] = iq;
+                }
+                nsel += (uint32_t)__popcll(tm);
+            }
+        }
+    }
+    wave_sync();
+    return nsel;
+}
+
 // CROSS: candidates from another frame's plane (a.search) against the
 // reference blocks of a.guide, staged in a second LDS tile; the list has no
 // self entry.
@@ -277,8 +393,24 @@ __global__ __launch_bounds__(kThreads) void match_kernel(MatchArgs a0, const uin
         const DT *tref = cross ? rtile + (j - ry0) * rp + (i - rx0) : tile + (j - oy) * rp + (i - ox);
         const DT thSSE = (DT)a.thSSE, distMul = (DT)a.distMul;
 
+        // The list holds a.maxCand entries (fewer than a large window's
+        // candidates): before a batch of strips that could overflow it, it is
+        // reduced to its K best, which still contain the window's K best.
+        const uint32_t K = (uint32_t)(a.G - 1);
+        const bool may_fill = (uint32_t)a.maxCand < (uint32_t)(nx * ny);
         uint32_t cnt = 0;
         for (int s0 = 0; s0 < nstrips; s0 += 64) {
+            if (may_fill && cnt + 64 * kStrip > (uint32_t)a.maxCand) {
+                wave_sync();
+                const uint32_t nk =
+                    select_smallest<KT, KeyOf<DT>::BITS>(keys, idxs, cnt, K, selk, seli, hist, memb, lane);
+                for (uint32_t e = lane; e < nk; e += 64) {
+                    keys[e] = selk[e];
+                    idxs[e] = seli[e];
+                }
+                cnt = nk;
+                wave_sync();
+            }
             const int sidx = s0 + lane;
             const bool valid = sidx < nstrips;
             const int ky = valid ? sidx / nsx : 0;
@@ -332,111 +464,8 @@ __global__ __launch_bounds__(kThreads) void match_kernel(MatchArgs a0, const uin
         wave_sync();
 
         // ---- selection of the K = G-1 smallest (key, scan index) ----
-        const uint32_t K = (uint32_t)(a.G - 1);
-        uint32_t nsel = 0;
-        if (cnt <= K) {
-            for (uint32_t e = lane; e < cnt; e += 64) {
-                selk[e] = keys[e];
-                seli[e] = idxs[e];
-            }
-            nsel = cnt;
-        } else {
-            KT prefix = 0, pmask = 0;
-            uint32_t need = K;
-            bool whole = false;
-            for (int shift = KeyOf<DT>::BITS - 8;; shift -= 8) {
-                for (int q = lane; q < 256; q += 64) hist[q] = 0;
-                wave_sync();
-                for (uint32_t e = lane; e < cnt; e += 64) {
-                    const KT k = keys[e];
-                    if ((k & pmask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
-                }
-                wave_sync();
-                const uint32_t h0 = hist[4 * lane], h1 = hist[4 * lane + 1];
-                const uint32_t h2 = hist[4 * lane + 2], h3 = hist[4 * lane + 3];
-                const uint32_t s = h0 + h1 + h2 + h3;
-                const uint32_t incl = wave_incl_scan(s, lane);
-                const uint32_t excl = incl - s;
-                const uint64_t bm = __ballot(excl < need && need <= incl);
-                const int L = __ffsll((unsigned long long)bm) - 1;
-                uint32_t dig = 0, blw = 0, mm = 0;
-                if (lane == L) {
-                    const uint32_t hh[4] = {h0, h1, h2, h3};
-                    uint32_t c = excl;
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        if (mm == 0 && c + hh[q] >= need) {
-                            dig = 4 * lane + q;
-                            blw = c;
-                            mm = hh[q];
-                        }
-                        c += hh[q];
-                    }
-                }
-                dig = __shfl(dig, L);
-                blw = __shfl(blw, L);
-                mm = __shfl(mm, L);
-                need -= blw;
-                prefix |= (KT)dig << shift;
-                pmask |= (KT)0xFF << shift;
-                if (mm == need) {
-                    whole = true;
-                    break;
-                }
-                if (mm <= 64 || shift == 0) break;
-                wave_sync();
-            }
-            uint32_t nm = 0;
-            for (uint32_t e0 = 0; e0 < cnt; e0 += 64) {
-                const uint32_t e = e0 + lane;
-                const bool inr = e < cnt;
-                const KT k = inr ? keys[e] : ~(KT)0;
-                const bool below = inr && (k & pmask) < prefix;
-                const bool inb = inr && (k & pmask) == prefix;
-                const bool take = below || (whole && inb);
-                const uint64_t tm = __ballot(take);
-                if (take) {
-                    const uint32_t sl = nsel + lanes_below(tm);
-                    selk[sl] = k;
-                    seli[sl] = idxs[e];
-                }
-                nsel += (uint32_t)__popcll(tm);
-                if (!whole) {
-                    const uint64_t mb = __ballot(inb);
-                    if (inb) memb[nm + lanes_below(mb)] = e;
-                    nm += (uint32_t)__popcll(mb);
-                }
-            }
-            if (!whole) {
-                wave_sync();
-                for (uint32_t q0 = 0; q0 < nm; q0 += 64) {
-                    const uint32_t q = q0 + lane;
-                    const bool v = q < nm;
-                    KT kq = 0;
-                    uint32_t iq = 0, rank = 0;
-                    if (v) {
-                        const uint32_t e = memb[q];
-                        kq = keys[e];
-                        iq = idxs[e];
-                        for (uint32_t r2 = 0; r2 < nm; ++r2) {
-                            const uint32_t e2 = memb[r2];
-                            const KT k2 = keys[e2];
-                            const uint32_t i2 = idxs[e2];
-                            rank += (k2 < kq) || (k2 == kq && i2 < iq);
-                        }
-                    }
-                    const bool take = v && rank < need;
-                    const uint64_t tm = __ballot(take);
-                    if (take) {
-                        const uint32_t sl = nsel + lanes_below(tm);
-                        selk[sl] = kq;
-                        seli[sl] = iq;
-                    }
-                    nsel += (uint32_t)__popcll(tm);
-                }
-            }
-        }
-        wave_sync();
+        const uint32_t nsel =
+            select_smallest<KT, KeyOf<DT>::BITS>(keys, idxs, cnt, K, selk, seli, hist, memb, lane);
 
         // ---- order the selection and write the group ----
         // (a cross-frame list: the selection alone, without the self entry)
