@@ -217,10 +217,30 @@ __device__ __forceinline__ uint32_t select_smallest(const KT *keys, const uint32
         }
         nsel = cnt;
     } else {
-        KT prefix = 0, pmask = 0;
+        // bits above the highest one in which the list's keys differ are
+        // common to all of them: the radix starts below them (the exponent
+        // bits of FP32 keys would otherwise put every key in one or two bins,
+        // whose atomics serialise)
+        KT kmin = ~(KT)0, kmax = 0;
+        for (uint32_t e = lane; e < cnt; e += 64) {
+            const KT k = keys[e];
+            kmin = k < kmin ? k : kmin;
+            kmax = k > kmax ? k : kmax;
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const KT x = __shfl_xor(kmin, o), y = __shfl_xor(kmax, o);
+            kmin = x < kmin ? x : kmin;
+            kmax = y > kmax ? y : kmax;
+        }
+        const KT diff = kmin ^ kmax;
+        const int top = diff == 0 ? 0 : (BITS == 64 ? 63 - __clzll((long long)diff) : 31 - __clz((int)diff));
+        int shift = top < 8 ? 0 : top - 7;
+        KT pmask = shift + 8 >= BITS ? (KT)0 : ~(KT)0 << (shift + 8);
+        KT prefix = kmin & pmask;
         uint32_t need = K;
         bool whole = false;
-        for (int shift = BITS - 8;; shift -= 8) {
+        for (;; shift = shift < 8 ? 0 : shift - 8) {
             for (int q = lane; q < 256; q += 64) hist[q] = 0;
             wave_sync();
             for (uint32_t e = lane; e < cnt; e += 64) {
