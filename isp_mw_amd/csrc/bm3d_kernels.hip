@@ -1174,7 +1174,8 @@ constexpr int kK64Elems = 16 * 64 * 8;
 // read from HBM), the flush's row sums in a region of their own.
 __host__ __device__ inline size_t filter_mfma_lds(int FW, int FH, bool wiener, int nslots = 1, bool stack = false)
 {
-    const size_t st = stack ? 0 : (size_t)FW * FH * (wiener ? 2 : 1);
+    // single frame: the staged footprint, then the Wiener schedule (2 ints per unit + count)
+    const size_t st = stack ? 0 : (size_t)FW * FH * (wiener ? 2 : 1) + (wiener ? 2 * kMaxTileRefs + 4 : 0);
     const size_t hs = (size_t)(FH - kBlock + 1) * FW * 2;
     return filter_acc_floats(FW, FH) * 4 * nslots + kK64Elems * 2 + (st > hs ? st : hs) * 4;
 }
@@ -1211,6 +1212,55 @@ __global__ __launch_bounds__(kMThreads, 4) void filter_mfma_kernel(FilterArgs a0
         for (int k = tid; k < kK64Elems / 8; k += kMThreads) dstq[k] = srcq[k];
     }
 
+    // Wiener step: the tile's schedule of units, consecutive reference blocks
+    // (2p, 2p+1) as one unit when both groups have at most 16 blocks, else one
+    // unit each; sched[2u], sched[2u+1] = (refA, refB or -1), sched[2 * kMaxTileRefs]
+    // = the number of units.
+    int *sched = reinterpret_cast<int *>(stage + (WIENER ? 2 : 1) * FW * FH);
+    if (WIENER && wave == 0) {
+        const int np = (tnum + 1) / 2;
+        for (int p0 = 0; p0 < np; p0 += 64) {
+            const int p = p0 + lane;
+            int refA = -1, refB = -1, NA = 0, NB = 0;
+            if (p < np) {
+                const int rA = 2 * p, rB = rA + 1;
+                const int gxA = tx0 + (rA % trx), gyA = ty0 + (rA / trx);
+                if (gxA < a.ngx && gyA < a.ngy) {
+                    refA = gyA * a.ngx + gxA;
+                    NA = a.cnt[refA];
+                }
+                const int gxB = tx0 + (rB % trx), gyB = ty0 + (rB / trx);
+                if (rB < tnum && gxB < a.ngx && gyB < a.ngy) {
+                    refB = gyB * a.ngx + gxB;
+                    NB = a.cnt[refB];
+                }
+            }
+            const bool pr = refA >= 0 && refB >= 0 && NA <= 16 && NB <= 16;
+            const uint32_t nu = pr ? 1u : (uint32_t)(refA >= 0) + (uint32_t)(refB >= 0);
+            const uint32_t incl = wave_incl_scan(nu, lane);
+            const int base = (p0 == 0 ? 0 : sched[2 * kMaxTileRefs]) + (int)(incl - nu);
+            if (pr) {
+                sched[2 * base] = refA;
+                sched[2 * base + 1] = refB;
+            } else {
+                int u = base;
+                if (refA >= 0) {
+                    sched[2 * u] = refA;
+                    sched[2 * u + 1] = -1;
+                    ++u;
+                }
+                if (refB >= 0) {
+                    sched[2 * u] = refB;
+                    sched[2 * u + 1] = -1;
+                }
+            }
+            const uint32_t total = __shfl(incl, 63);
+            wave_sync();
+            if (lane == 0) sched[2 * kMaxTileRefs] = (p0 == 0 ? 0 : sched[2 * kMaxTileRefs]) + (int)total;
+            wave_sync();
+        }
+    }
+
     for (int pl = 0; pl < 3; ++pl) {
         if (!a.active[pl]) continue;
         const float *src = a.src[pl];
@@ -1228,44 +1278,36 @@ __global__ __launch_bounds__(kMThreads, 4) void filter_mfma_kernel(FilterArgs a0
         __syncthreads();
         const float thr = a.thr[pl], sig2 = a.sig2[pl];
         if constexpr (WIENER) {
-            // one group per pass (a paired schedule costs the Wiener path its
-            // register budget: it spills)
-        for (int r = wave; r < tnum; r += kMWaves) {
-            const int gxi = tx0 + (r % trx), gyi = ty0 + (r / trx);
-            if (gxi >= a.ngx || gyi >= a.ngy) continue;
-            const size_t refIdx = (size_t)gyi * a.ngx + gxi;
-            const int *pos = a.pos + refIdx * a.G;
-            const int N = a.cnt[refIdx];
-            v8 cna[2], cnb[2];
+            // Units of work from the tile's schedule: a pair of groups of at
+            // most 16 blocks sharing one operand (as in step 1), or one group.
+            const int nunits = sched[2 * kMaxTileRefs];
+            for (int un = wave; un < nunits; un += kMWaves) {
+                const int2 unit = reinterpret_cast<const int2 *>(sched)[un];
+                const size_t refA = (size_t)unit.x;
+                const bool paired = unit.y >= 0;
+                const size_t refB = paired ? (size_t)unit.y : 0;
+                const int N0 = a.cnt[refA], N1 = paired ? a.cnt[refB] : 0;
+                const int *pos0 = a.pos + refA * a.G, *pos1 = a.pos + refB * a.G;
+                const int g = paired ? (c >> 4) : 0, mi = paired ? (c & 15) : c;
+                const int Ng = g ? N1 : N0;
+                const int pz = mi < Ng ? (g ? pos1 : pos0)[mi] : -1;
+                v8 cna[2], cnb[2];
+                {
+                    const uint16_t *ta = paired ? a.cnap : a.cna, *tb = paired ? a.cnbp : a.cnb;
+                    const int fi = paired ? (N0 - 1) * 17 + N1 : N0 - 1;
 #pragma unroll
-            for (int m = 0; m < 2; ++m) {
-                cna[m] = load_frag<T>(a.cna, (N - 1) * 2 + m, lane);
-                cnb[m] = load_frag<T>(a.cnb, (N - 1) * 2 + m, lane);
-            }
-            float w;
-            f32x16 xh[2];
-            if (!WIENER) {
-                f32x16 y[2];
-                mfma_forward<T>(lsrc, FW, ox, oy, pos, N, lane, kb, cna, y);
-                int kept = 0;
-#pragma unroll
-                for (int q = 0; q < 2; ++q)
-#pragma unroll
-                    for (int k = 0; k < 16; ++k) {
-                        if (fabsf(y[q][k]) <= thr) y[q][k] = 0.f;
-                        else ++kept;
+                    for (int m = 0; m < 2; ++m) {
+                        cna[m] = load_frag<T>(ta, fi * 2 + m, lane);
+                        cnb[m] = load_frag<T>(tb, fi * 2 + m, lane);
                     }
-                kept = wave_sum(kept);
-                w = kept < 1 ? 1.f : 1.f / (float)kept;
-                mfma_inverse<T>(y, ka, lane, cnb, xh);
-            } else {
+                }
                 // one column half at a time: guide and source forward, gains,
                 // then that half's first inverse product
                 f32x16 vt[2];
-                float l2 = 0.f;
+                float l20 = 0.f, l21 = 0.f;
                 v8 xg[4], xs[4];
-                mfma_load_x<T>(lgd, FW, ox, oy, pos, N, lane, xg);
-                mfma_load_x<T>(lsrc, FW, ox, oy, pos, N, lane, xs);
+                mfma_load_block<T>(lgd, FW, ox, oy, pz, h, xg);
+                mfma_load_block<T>(lsrc, FW, ox, oy, pz, h, xs);
 #pragma unroll
                 for (int q = 0; q < 2; ++q) {
                     const f32x16 yg = mfma_forward_half<T>(xg, q, kb, lane, cna);
@@ -1274,33 +1316,36 @@ __global__ __launch_bounds__(kMThreads, 4) void filter_mfma_kernel(FilterArgs a0
                     const float su = sig2 * ((4 * q + (c >> 3)) == 0 ? 0.5f : 1.f) * ((c & 7) == 0 ? 0.5f : 1.f);
 #pragma unroll
                     for (int k = 0; k < 16; ++k) {
-                        const int kz = 8 * (k >> 2) + 4 * h + (k & 3);  // group-axis frequency
+                        // group-axis frequency: the row within the group
+                        const int kz = (8 * (k >> 2) + 4 * h + (k & 3)) & (paired ? 15 : 31);
                         const float yy = yg[k] * yg[k];
                         // 1-ulp reciprocal: the operands already carry 16-bit rounding
                         const float wc = yy * __builtin_amdgcn_rcpf(yy + (kz == 0 ? 0.5f * su : su));
                         yq[k] *= wc;
-                        l2 = fmaf(wc, wc, l2);
+                        if (k >= 8 && paired) l21 = fmaf(wc, wc, l21);
+                        else l20 = fmaf(wc, wc, l20);
                     }
                     vt[q] = mfma_inverse_half<T>(yq, cnb);
                 }
-                l2 = wave_sum(l2);
-                w = l2 <= 0.f ? 1.f : 1.f / l2;
+                l20 = wave_sum(l20);
+                l21 = wave_sum(l21);
+                f32x16 xh[2];
                 mfma_inverse_rest<T>(vt, ka, lane, xh);
+                const float l2 = g ? l21 : l20;
+                const float w = l2 <= 0.f ? 1.f : 1.f / l2;
+                const float ws = w * (kFixScale / 4294967296.0f);
+                if (pz >= 0) {
+                    const int py = (pz >> 16) - oy, px = (pz & 0xFFFF) - ox + 4 * h;
+#pragma unroll
+                    for (int rr = 0; rr < 2; ++rr)
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+#pragma unroll
+                            for (int j = 0; j < 4; ++j)
+                                fix_add_scaled(&accNum[(py + 4 * rr + i) * FW + px + j], xh[rr][4 * i + j], ws);
+                    if (h == 0) fix_add(&accDen[py * DW + (pz & 0xFFFF) - ox], w);
+                }
             }
-            const float ws = w * (kFixScale / 4294967296.0f);
-            if (c < N) {
-                const int pz = pos[c];
-                const int py = (pz >> 16) - oy, px = (pz & 0xFFFF) - ox + 4 * h;
-#pragma unroll
-                for (int rr = 0; rr < 2; ++rr)
-#pragma unroll
-                    for (int i = 0; i < 4; ++i)
-#pragma unroll
-                        for (int j = 0; j < 4; ++j)
-                            fix_add_scaled(&accNum[(py + 4 * rr + i) * FW + px + j], xh[rr][4 * i + j], ws);
-                if (h == 0) fix_add(&accDen[py * DW + (pz & 0xFFFF) - ox], w);
-            }
-        }
         } else {
         // Reference blocks in pairs.  Two groups of at most 16 blocks share
         // one 32-row operand (group 0 on rows 0..15, group 1 on 16..31, the
