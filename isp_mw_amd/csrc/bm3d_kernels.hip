@@ -416,17 +416,31 @@ __global__ __launch_bounds__(kThreads) void match_kernel(MatchArgs a0, const uin
         // The list holds a.maxCand entries (fewer than a large window's
         // candidates): before a batch of strips that could overflow it, it is
         // reduced to its K best, which still contain the window's K best.
+        // After such a reduction to K entries, a later candidate whose key is
+        // not below the largest of them ranks behind all K (its scan index is
+        // larger) and is not listed: kbound.
         const uint32_t K = (uint32_t)(a.G - 1);
         const bool may_fill = (uint32_t)a.maxCand < (uint32_t)(nx * ny);
         uint32_t cnt = 0;
+        KT kbound = ~(KT)0;
         for (int s0 = 0; s0 < nstrips; s0 += 64) {
             if (may_fill && cnt + 64 * kStrip > (uint32_t)a.maxCand) {
                 wave_sync();
                 const uint32_t nk =
                     select_smallest<KT, KeyOf<DT>::BITS>(keys, idxs, cnt, K, selk, seli, hist, memb, lane);
+                KT kx = 0;
                 for (uint32_t e = lane; e < nk; e += 64) {
                     keys[e] = selk[e];
                     idxs[e] = seli[e];
+                    kx = selk[e] > kx ? selk[e] : kx;
+                }
+                if (nk == K) {
+#pragma unroll
+                    for (int o = 32; o > 0; o >>= 1) {
+                        const KT y = __shfl_xor(kx, o);
+                        kx = y > kx ? y : kx;
+                    }
+                    kbound = kx;
                 }
                 cnt = nk;
                 wave_sync();
@@ -470,12 +484,13 @@ __global__ __launch_bounds__(kThreads) void match_kernel(MatchArgs a0, const uin
 #pragma unroll
             for (int c = 0; c < kStrip; ++c) {
                 const int kx = kx0 + c;
+                const KT key = KeyOf<DT>::key(acc[c] * distMul);
                 const bool ok = valid && kx < nx && (cross || !(kx == selfx && ky == selfy)) && acc[c] <= thSSE &&
-                                acc[c] != 0;
+                                acc[c] != 0 && key < kbound;
                 const uint64_t m = __ballot(ok);
                 if (ok) {
                     const uint32_t slot = cnt + lanes_below(m);
-                    keys[slot] = KeyOf<DT>::key(acc[c] * distMul);
+                    keys[slot] = key;
                     idxs[slot] = (uint32_t)(ky * nx + kx);
                 }
                 cnt += (uint32_t)__popcll(m);
