@@ -12,6 +12,7 @@
 
 #include <cmath>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
@@ -25,6 +26,10 @@ struct isp_bm3d_ctx {
     isp_bm3d_params params;
     int device = 0;
     int precision = ISP_BM3D_PREC_FP32;
+    // FP32-distance matching by column walks where eligible; the environment
+    // variable ISP_BM3D_MATCH_WALK=0 at context creation keeps every search
+    // on the list kernel (the two give identical groups)
+    bool match_walk = true;
     float *d_tables = nullptr;  // C8 (64) then the padded C_N matrices
     int cn_off[kMaxGroup + 1] = {0};
     double *d_tables64 = nullptr;  // C8 (64) then the unpadded C_N matrices, FP64
@@ -423,6 +428,7 @@ MatchArgs build_match(const isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, co
     a.cnt_out = cnt;
     a.pos_out = pos;
     a.key_out = keys;
+    a.walk = ctx->match_walk && match_walk_eligible(a) ? 1 : 0;
     return a;
 }
 
@@ -970,6 +976,15 @@ int batch_args(isp_bm3d_batch *b)
             sp.f.push_back(fa);
         }
     }
+    // one matching kernel per batched launch: the column walk only when every
+    // image of the step can take it
+    for (int kind = 0; kind < 2; ++kind) {
+        BatchStepPlan &sp = b->st[kind];
+        bool all = true;
+        for (size_t k = 0; k < sp.m.size(); ++k)
+            if (!b->bi[k].d[kind].degenerate && !sp.m[k].walk) all = false;
+        for (MatchArgs &m : sp.m) m.walk = all ? m.walk : 0;
+    }
     return ISP_BM3D_OK;
 }
 
@@ -1358,6 +1373,7 @@ int isp_bm3d_create(isp_bm3d_ctx **out, const isp_bm3d_params *params, int devic
     if (device < 0 || device >= ndev) return fail(nullptr, ISP_BM3D_E_ARG, "device %d of %d", device, ndev);
     isp_bm3d_ctx *ctx = new isp_bm3d_ctx();
     ctx->params = *params;
+    if (const char *mw = std::getenv("ISP_BM3D_MATCH_WALK")) ctx->match_walk = std::strcmp(mw, "0") != 0;
     ctx->device = device;
     ctx->err = "no error";
     // Orthonormal DCT-II matrices: C8 [k][n], then C_N for N = 1..32, each
@@ -2597,6 +2613,7 @@ int stack_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, co
                                           w.lkey[l]);
                 if (z != t) {
                     a.search = gz;
+                    a.walk = 0;
                     a.refW = (tl.match_x - 1) * p.block_step + kBlock;
                     a.refH = (tl.match_y - 1) * p.block_step + kBlock;
                     a.refPitch = a.refW | 1;

@@ -71,7 +71,18 @@ struct MatchArgs {
     // Such a list has no self entry: cnt_out = N <= G-1, pos_out[0..N).
     const void *search;      // nullptr: the guide itself (the single-frame search)
     int refW, refH, refPitch;  // staged reference-block tile of a cross-frame search
+    int walk;                // 1: match_walk_kernel (see match_walk_eligible)
 };
+
+// The column-walk matching kernel takes FP32 distances of the single-frame
+// search with BlockStep below the block size (neighbouring blocks of a column
+// share rows) and windows of at most 128 lane strips of kStrip offsets.
+inline bool match_walk_eligible(const MatchArgs &a)
+{
+    if (a.exact || a.search || a.bstep >= 8 || a.step < 1) return false;
+    const int nm = 2 * (a.range / a.step) + 1;
+    return nm * ((nm + 4) / 5) <= 128;
+}
 
 // Multi-frame grouping: merge of the per-frame candidate lists of one
 // reference frame into its groups.

@@ -459,25 +459,36 @@ __global__ __launch_bounds__(kThreads) void match_kernel(MatchArgs a0, const uin
                     DT rv[kBlock];
 #pragma unroll
                     for (int v = 0; v < kBlock; ++v) rv[v] = tref[u * rp + v];
+                    // EXACT: one sum over the 64 terms in row-major order; FP32:
+                    // each row's 8 fused terms summed from 0, then the row sums
+                    // added in row order (the order of match_walk_kernel)
                     if constexpr (ST1) {
                         DT cv[kBlock + kStrip - 1];
 #pragma unroll
                         for (int q = 0; q < kBlock + kStrip - 1; ++q) cv[q] = tc[u * a.tilePitch + q];
 #pragma unroll
-                        for (int c = 0; c < kStrip; ++c)
+                        for (int c = 0; c < kStrip; ++c) {
+                            DT rs = 0;
 #pragma unroll
                             for (int v = 0; v < kBlock; ++v) {
                                 const DT d = rv[v] - cv[c + v];
-                                acc[c] = sq_acc<EXACT>(d, acc[c]);
+                                if constexpr (EXACT) acc[c] = sq_acc<true>(d, acc[c]);
+                                else rs = sq_acc<false>(d, rs);
                             }
+                            if constexpr (!EXACT) acc[c] += rs;
+                        }
                     } else {
 #pragma unroll
-                        for (int c = 0; c < kStrip; ++c)
+                        for (int c = 0; c < kStrip; ++c) {
+                            DT rs = 0;
 #pragma unroll
                             for (int v = 0; v < kBlock; ++v) {
                                 const DT d = rv[v] - tc[u * a.tilePitch + c * st + v];
-                                acc[c] = sq_acc<EXACT>(d, acc[c]);
+                                if constexpr (EXACT) acc[c] = sq_acc<true>(d, acc[c]);
+                                else rs = sq_acc<false>(d, rs);
                             }
+                            if constexpr (!EXACT) acc[c] += rs;
+                        }
                     }
                 }
             }
@@ -526,6 +537,379 @@ __global__ __launch_bounds__(kThreads) void match_kernel(MatchArgs a0, const uin
             if (a.key_out) a.key_out[base + first + rank] = KeyOf<DT>::value(kq);
         }
         wave_sync();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Block matching by column walks: FP32 distances, the single-frame search,
+// BlockStep below the block size, and windows of at most 2 x 64 lane strips.
+// One wave takes up to kWalk vertically consecutive reference blocks of one
+// grid column.  The windows of one column share their lattice of offsets
+// (dy, dx) (the range is a multiple of BMstep; each block's own clipping
+// decides which offsets are candidates).  Every lane owns two strips of
+// kStrip offsets for the whole walk.  Each image row's 8-term sum of squared
+// differences is formed once per offset and added to every block of the walk
+// that covers the row.  A block's distance is then the sum of its eight row
+// sums, added in row order from 0: the same wherever the block lies in a
+// walk, and the same as in match_kernel.  At BlockStep 3 a walk of 4 blocks
+// forms 17 row sums for 32 uses.  The keys stay in registers; the selection
+// reads them there.
+constexpr int kWalk = 4;
+constexpr int kWalkRounds = 2;
+constexpr uint32_t kNoKey = ~0u;
+
+// per wave: hist u32[256], memk u32[64], memi u32[64], selk u32[32], seli u32[32]
+__host__ __device__ constexpr size_t walk_wave_bytes() { return (256 + 64 + 64 + 32 + 32) * 4; }
+
+// The K smallest (key, index) of the wave's register keys into selk / seli,
+// unordered; returns how many (min(candidates, K)).  key[r][c] is kNoKey for
+// no candidate; its index is ibase[r] + c, and indices grow in (r, lane, c)
+// order.  The radix select of select_smallest with the histogram passes over
+// registers; the last bucket's members (at most 64) are ranked exactly in
+// LDS, or, when more of them share one key, taken in index order.
+__device__ __forceinline__ uint32_t select_walk(const uint32_t (&key)[kWalkRounds][kStrip],
+                                                const int (&ibase)[kWalkRounds], uint32_t K, uint32_t *selk,
+                                                uint32_t *seli, uint32_t *hist, uint32_t *memk, uint32_t *memi,
+                                                int lane)
+{
+    int n = 0;
+    uint32_t kmin = kNoKey, kmax = 0;
+#pragma unroll
+    for (int r = 0; r < kWalkRounds; ++r)
+#pragma unroll
+        for (int c = 0; c < kStrip; ++c) {
+            const uint32_t k = key[r][c];
+            if (k != kNoKey) {
+                ++n;
+                kmin = k < kmin ? k : kmin;
+                kmax = k > kmax ? k : kmax;
+            }
+        }
+    const uint32_t total = (uint32_t)wave_sum(n);
+    uint32_t nsel = 0;
+    if (total <= K) {
+#pragma unroll
+        for (int r = 0; r < kWalkRounds; ++r)
+#pragma unroll
+            for (int c = 0; c < kStrip; ++c) {
+                const bool take = key[r][c] != kNoKey;
+                const uint64_t tm = __ballot(take);
+                if (take) {
+                    const uint32_t sl = nsel + lanes_below(tm);
+                    selk[sl] = key[r][c];
+                    seli[sl] = (uint32_t)(ibase[r] + c);
+                }
+                nsel += (uint32_t)__popcll(tm);
+            }
+        wave_sync();
+        return nsel;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t x = __shfl_xor(kmin, o), y = __shfl_xor(kmax, o);
+        kmin = x < kmin ? x : kmin;
+        kmax = y > kmax ? y : kmax;
+    }
+    const uint32_t diff = kmin ^ kmax;
+    const int top = diff == 0 ? 0 : 31 - __clz((int)diff);
+    int shift = top < 8 ? 0 : top - 7;
+    uint32_t pmask = shift + 8 >= 32 ? 0u : ~0u << (shift + 8);
+    uint32_t prefix = kmin & pmask;
+    uint32_t need = K, mm = 0;
+    bool whole = false;
+    for (;; shift = shift < 8 ? 0 : shift - 8) {
+        for (int q = lane; q < 256; q += 64) hist[q] = 0;
+        wave_sync();
+#pragma unroll
+        for (int r = 0; r < kWalkRounds; ++r)
+#pragma unroll
+            for (int c = 0; c < kStrip; ++c) {
+                const uint32_t k = key[r][c];
+                if (k != kNoKey && (k & pmask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
+            }
+        wave_sync();
+        const uint32_t h0 = hist[4 * lane], h1 = hist[4 * lane + 1];
+        const uint32_t h2 = hist[4 * lane + 2], h3 = hist[4 * lane + 3];
+        const uint32_t s = h0 + h1 + h2 + h3;
+        const uint32_t incl = wave_incl_scan(s, lane);
+        const uint32_t excl = incl - s;
+        const uint64_t bm = __ballot(excl < need && need <= incl);
+        const int L = __ffsll((unsigned long long)bm) - 1;
+        uint32_t dig = 0, blw = 0;
+        mm = 0;
+        if (lane == L) {
+            const uint32_t hh[4] = {h0, h1, h2, h3};
+            uint32_t c = excl;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (mm == 0 && c + hh[q] >= need) {
+                    dig = 4 * lane + q;
+                    blw = c;
+                    mm = hh[q];
+                }
+                c += hh[q];
+            }
+        }
+        dig = __shfl(dig, L);
+        blw = __shfl(blw, L);
+        mm = __shfl(mm, L);
+        need -= blw;
+        prefix |= dig << shift;
+        pmask |= 0xFFu << shift;
+        if (mm == need) {
+            whole = true;
+            break;
+        }
+        if (mm <= 64 || shift == 0) break;
+        wave_sync();
+    }
+    // below the bucket (and the bucket itself when all of it is taken)
+    const bool rank = !whole && mm <= 64;
+    uint32_t nm = 0;
+#pragma unroll
+    for (int r = 0; r < kWalkRounds; ++r)
+#pragma unroll
+        for (int c = 0; c < kStrip; ++c) {
+            const uint32_t k = key[r][c];
+            const bool inr = k != kNoKey;
+            const bool below = inr && (k & pmask) < prefix;
+            const bool inb = inr && (k & pmask) == prefix;
+            const bool take = below || (whole && inb);
+            const uint64_t tm = __ballot(take);
+            if (take) {
+                const uint32_t sl = nsel + lanes_below(tm);
+                selk[sl] = k;
+                seli[sl] = (uint32_t)(ibase[r] + c);
+            }
+            nsel += (uint32_t)__popcll(tm);
+            if (rank) {
+                const uint64_t mb = __ballot(inb);
+                if (inb) {
+                    const uint32_t q = nm + lanes_below(mb);
+                    memk[q] = k;
+                    memi[q] = (uint32_t)(ibase[r] + c);
+                }
+                nm += (uint32_t)__popcll(mb);
+            }
+        }
+    if (rank) {
+        // at most 64 members: exact rank by (key, index)
+        wave_sync();
+        const bool v = (uint32_t)lane < nm;
+        uint32_t kq = 0, iq = 0, rk = 0;
+        if (v) {
+            kq = memk[lane];
+            iq = memi[lane];
+            for (uint32_t q = 0; q < nm; ++q) {
+                const uint32_t k2 = memk[q], i2 = memi[q];
+                rk += (k2 < kq) || (k2 == kq && i2 < iq);
+            }
+        }
+        const bool take = v && rk < need;
+        const uint64_t tm = __ballot(take);
+        if (take) {
+            const uint32_t sl = nsel + lanes_below(tm);
+            selk[sl] = kq;
+            seli[sl] = iq;
+        }
+        nsel += (uint32_t)__popcll(tm);
+    } else if (!whole) {
+        // more than 64 members, every key bit decided: one key, so the
+        // `need` smallest indices, which come first in (r, lane, c) order
+        uint32_t before = 0;
+#pragma unroll
+        for (int r = 0; r < kWalkRounds; ++r) {
+            uint32_t cl = 0;
+#pragma unroll
+            for (int c = 0; c < kStrip; ++c) cl += key[r][c] != kNoKey && (key[r][c] & pmask) == prefix;
+            const uint32_t incl = wave_incl_scan(cl, lane);
+            uint32_t p = before + incl - cl;
+#pragma unroll
+            for (int c = 0; c < kStrip; ++c) {
+                const uint32_t k = key[r][c];
+                if (k != kNoKey && (k & pmask) == prefix) {
+                    if (p < need) {
+                        selk[nsel + p] = k;
+                        seli[nsel + p] = (uint32_t)(ibase[r] + c);
+                    }
+                    ++p;
+                }
+            }
+            before += __shfl(incl, 63);
+        }
+        nsel += need;
+    }
+    wave_sync();
+    return nsel;
+}
+
+template <typename GT, bool ST1, bool BATCHED>
+__global__ __launch_bounds__(kThreads) void match_walk_kernel(MatchArgs a0, const uint32_t *__restrict__ bmap,
+                                                              const MatchArgs *__restrict__ bargs)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int tx, ty;
+    const MatchArgs &a = tile_unit<BATCHED>(a0, bargs, bmap, tx, ty);
+    const GT *__restrict__ guide = static_cast<const GT *>(a.guide);
+    const int gpitch = a.gpitch, pitch = a.tilePitch;
+    const int trx = a.trx, try_ = a.try_;
+    const int tx0 = tx * trx, ty0 = ty * try_;
+    const int rx0 = grid_pos(tx0, a.bstep, a.W - kBlock), ry0 = grid_pos(ty0, a.bstep, a.H - kBlock);
+    const int R = a.range, st = a.step;
+    const int ox = rx0 - R, oy = ry0 - R;
+
+    float *tile = reinterpret_cast<float *>(smem);
+    const size_t tileBytes = align16((size_t)a.tileH * pitch * sizeof(float));
+    uint32_t *hist = reinterpret_cast<uint32_t *>(smem + tileBytes + (size_t)wave * walk_wave_bytes());
+    uint32_t *memk = hist + 256, *memi = memk + 64, *selk = memi + 64, *seli = selk + 32;
+
+    for (int k = tid; k < a.tileH * a.tileW; k += kThreads) {
+        const int yy0 = k / a.tileW, xx0 = k - yy0 * a.tileW;
+        const int yy = oy + yy0, xx = ox + xx0;
+        float v = 0.f;
+        if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) v = (float)guide[(size_t)yy * gpitch + xx];
+        tile[yy0 * pitch + xx0] = v;
+    }
+    __syncthreads();
+
+    // this lane's strips: offsets (dy[r], dx[r] + c * st), c < kStrip; a strip
+    // past the lattice reads offsets -R (inside the tile) and keeps no key
+    const int NM = 2 * R / st + 1;
+    const int nsx = (NM + kStrip - 1) / kStrip, nstrips = NM * nsx;
+    int dy[kWalkRounds], dx[kWalkRounds], mx0[kWalkRounds];
+    bool sv[kWalkRounds];
+#pragma unroll
+    for (int r = 0; r < kWalkRounds; ++r) {
+        const int sidx = r * 64 + lane;
+        sv[r] = sidx < nstrips;
+        const int my = sv[r] ? sidx / nsx : 0;
+        mx0[r] = sv[r] ? (sidx - my * nsx) * kStrip : 0;
+        dy[r] = -R + my * st;
+        dx[r] = -R + mx0[r] * st;
+    }
+    const float thSSE = (float)a.thSSE, distMul = (float)a.distMul;
+    const uint32_t K = (uint32_t)(a.G - 1);
+    const int nseg = (try_ + kWalk - 1) / kWalk;
+
+    for (int w = wave; w < trx * nseg; w += kWaves) {
+        const int col = w % trx, seg = w / trx;
+        const int gx = tx0 + col, gy0 = ty0 + seg * kWalk;
+        const int nk = min(kWalk, min(try_ - seg * kWalk, a.ngy - gy0));
+        if (gx >= a.ngx || nk <= 0) continue;
+        const int i = grid_pos(gx, a.bstep, a.W - kBlock);
+        int jk[kWalk];
+#pragma unroll
+        for (int k = 0; k < kWalk; ++k) jk[k] = grid_pos(gy0 + min(k, nk - 1), a.bstep, a.H - kBlock);
+        if (a.bypass) {
+            if (lane < nk) {
+                const int j = grid_pos(gy0 + lane, a.bstep, a.H - kBlock);
+                const size_t refIdx = (size_t)(gy0 + lane) * a.ngx + gx;
+                a.cnt_out[refIdx] = 1;
+                a.pos_out[refIdx * a.G] = (j << 16) | i;
+                if (a.key_out) a.key_out[refIdx * a.G] = 0.0;
+            }
+            continue;
+        }
+
+        // ---- distances of the walk's blocks: row sums over the rows they cover ----
+        float D[kWalkRounds][kWalk][kStrip];
+#pragma unroll
+        for (int r = 0; r < kWalkRounds; ++r)
+#pragma unroll
+            for (int k = 0; k < kWalk; ++k)
+#pragma unroll
+                for (int c = 0; c < kStrip; ++c) D[r][k][c] = 0.f;
+        const float *tref = tile + (i - ox);
+        for (int y = jk[0]; y < jk[nk - 1] + kBlock; ++y) {
+            float rv[kBlock];
+#pragma unroll
+            for (int v = 0; v < kBlock; ++v) rv[v] = tref[(y - oy) * pitch + v];
+#pragma unroll
+            for (int r = 0; r < kWalkRounds; ++r) {
+                const float *tc = tile + (y + dy[r] - oy) * pitch + (i + dx[r] - ox);
+                float rs[kStrip];
+                if constexpr (ST1) {
+                    float cv[kBlock + kStrip - 1];
+#pragma unroll
+                    for (int q = 0; q < kBlock + kStrip - 1; ++q) cv[q] = tc[q];
+#pragma unroll
+                    for (int c = 0; c < kStrip; ++c) {
+                        rs[c] = 0.f;
+#pragma unroll
+                        for (int v = 0; v < kBlock; ++v) {
+                            const float d = rv[v] - cv[c + v];
+                            rs[c] = fmaf(d, d, rs[c]);
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int c = 0; c < kStrip; ++c) {
+                        rs[c] = 0.f;
+#pragma unroll
+                        for (int v = 0; v < kBlock; ++v) {
+                            const float d = rv[v] - tc[c * st + v];
+                            rs[c] = fmaf(d, d, rs[c]);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < kWalk; ++k)
+                    if (k < nk && y >= jk[k] && y < jk[k] + kBlock) {
+#pragma unroll
+                        for (int c = 0; c < kStrip; ++c) D[r][k][c] += rs[c];
+                    }
+            }
+        }
+
+        // ---- per block: keys, selection of the K = G-1 smallest, the group ----
+        const int l = search_boundary(i, 0, R, st), rr = search_boundary(i, a.W - kBlock, R, st);
+        const int nx = (rr - l) / st + 1;
+#pragma unroll
+        for (int k = 0; k < kWalk; ++k) {
+            if (k >= nk) break;
+            const int j = jk[k];
+            const int t = search_boundary(j, 0, R, st), b = search_boundary(j, a.H - kBlock, R, st);
+            const int ny = (b - t) / st + 1;
+            uint32_t key[kWalkRounds][kStrip];
+            int ibase[kWalkRounds];
+#pragma unroll
+            for (int r = 0; r < kWalkRounds; ++r) {
+                const int y = j + dy[r];
+                const bool vy = sv[r] && y >= t && y <= t + (ny - 1) * st;
+                const int ky = (y - t) / st;
+                ibase[r] = ky * nx + (i + dx[r] - l) / st;
+#pragma unroll
+                for (int c = 0; c < kStrip; ++c) {
+                    const int x = i + dx[r] + c * st;
+                    const float d = D[r][k][c];
+                    const bool ok = vy && mx0[r] + c < NM && x >= l && x <= l + (nx - 1) * st &&
+                                    !(x == i && y == j) && d <= thSSE && d != 0.f;
+                    key[r][c] = ok ? __float_as_uint(d * distMul) : kNoKey;
+                }
+            }
+            const uint32_t nsel = select_walk(key, ibase, K, selk, seli, hist, memk, memi, lane);
+
+            const size_t refIdx = (size_t)(gy0 + k) * a.ngx + gx;
+            const size_t base = refIdx * a.G;
+            if (lane == 0) {
+                a.cnt_out[refIdx] = (int)(1 + nsel);
+                a.pos_out[base] = (j << 16) | i;
+                if (a.key_out) a.key_out[base] = 0.0;
+            }
+            if ((uint32_t)lane < nsel) {
+                const uint32_t kq = selk[lane], iq = seli[lane];
+                uint32_t rank = 0;
+                for (uint32_t q = 0; q < nsel; ++q) {
+                    const uint32_t k2 = selk[q];
+                    rank += (k2 < kq) || (k2 == kq && seli[q] < iq);
+                }
+                const int cy = (int)(iq / (uint32_t)nx), cx = (int)(iq - (uint32_t)cy * nx);
+                a.pos_out[base + 1 + rank] = ((t + cy * st) << 16) | (l + cx * st);
+                if (a.key_out) a.key_out[base + 1 + rank] = (double)__uint_as_float(kq);
+            }
+            wave_sync();
+        }
     }
 }
 
@@ -1767,6 +2151,17 @@ hipError_t set_lds(K kernel, size_t bytes)
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
+template <typename K>
+hipError_t launch_tiled_m(K kernel, size_t lds, const MatchArgs &a, hipStream_t s, const BatchLaunch *b, dim3 grid)
+{
+    hipError_t e;
+    if ((e = set_lds(kernel, lds)) != hipSuccess) return e;
+    if (b) hipLaunchKernelGGL(kernel, dim3(b->units), dim3(kThreads), lds, s, a, b->map,
+                              static_cast<const MatchArgs *>(b->args));
+    else hipLaunchKernelGGL(kernel, grid, dim3(kThreads), lds, s, a, nullptr, nullptr);
+    return hipGetLastError();
+}
+
 template <typename GT, typename DT, bool EXACT, bool ST1>
 hipError_t launch_match_t(const MatchArgs &a, hipStream_t s, const BatchLaunch *b)
 {
@@ -1779,6 +2174,13 @@ hipError_t launch_match_t(const MatchArgs &a, hipStream_t s, const BatchLaunch *
         const dim3 grid((a.ngx + a.trx - 1) / a.trx, (a.ngy + a.try_ - 1) / a.try_);
         hipLaunchKernelGGL(k, grid, dim3(kThreads), lds, s, a, nullptr, nullptr);
         return hipGetLastError();
+    }
+    if constexpr (!EXACT) {
+        if (a.walk) {
+            const dim3 grid((a.ngx + a.trx - 1) / a.trx, (a.ngy + a.try_ - 1) / a.try_);
+            return b ? launch_tiled_m(match_walk_kernel<GT, ST1, true>, lds, a, s, b, grid)
+                     : launch_tiled_m(match_walk_kernel<GT, ST1, false>, lds, a, s, b, grid);
+        }
     }
     if (b) {
         auto k = match_kernel<GT, DT, ST1, EXACT, true>;
@@ -1806,6 +2208,7 @@ hipError_t launch_match_g(const MatchArgs &a, hipStream_t s, const BatchLaunch *
 // exact (the plain form): FP64 distances and keys; otherwise FP32
 size_t match_lds_bytes(const MatchArgs &a)
 {
+    if (a.walk) return align16((size_t)a.tileH * a.tilePitch * 4) + (size_t)kWaves * walk_wave_bytes();
     const int db = a.exact ? 8 : 4;
     const size_t rt = a.search ? align16((size_t)a.refH * a.refPitch * db) : 0;
     return align16((size_t)a.tileH * a.tilePitch * db) + rt + (size_t)kWaves * match_wave_bytes(a.maxCand, db);
