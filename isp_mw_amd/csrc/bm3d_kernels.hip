@@ -743,8 +743,9 @@ __device__ __forceinline__ uint32_t select_walk(const uint32_t (&key)[kWalkRound
     return nsel;
 }
 
+// 4 waves per SIMD: at most 128 registers
 template <typename GT, bool ST1, bool BATCHED>
-__global__ __launch_bounds__(kThreads) void match_walk_kernel(MatchArgs a0, const uint32_t *__restrict__ bmap,
+__global__ __launch_bounds__(kThreads, 4) void match_walk_kernel(MatchArgs a0, const uint32_t *__restrict__ bmap,
                                                               const MatchArgs *__restrict__ bargs)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -775,17 +776,18 @@ __global__ __launch_bounds__(kThreads) void match_walk_kernel(MatchArgs a0, cons
 
     // this lane's strips: offsets (dy[r], dx[r] + c * st), c < kStrip; a strip
     // past the lattice reads offsets -R (inside the tile) and keeps no key
-    const int NM = 2 * R / st + 1;
+    // lattice indices: offset (dy, dx) = (-R + my * st, -R + mx * st), my, mx < NM
+    const int NM = 2 * R / st + 1, Rs = R / st;
     const int nsx = (NM + kStrip - 1) / kStrip, nstrips = NM * nsx;
-    int dy[kWalkRounds], dx[kWalkRounds], mx0[kWalkRounds];
-    bool sv[kWalkRounds];
+    int dy[kWalkRounds], dx[kWalkRounds], my[kWalkRounds], mx0[kWalkRounds];
 #pragma unroll
     for (int r = 0; r < kWalkRounds; ++r) {
         const int sidx = r * 64 + lane;
-        sv[r] = sidx < nstrips;
-        const int my = sv[r] ? sidx / nsx : 0;
-        mx0[r] = sv[r] ? (sidx - my * nsx) * kStrip : 0;
-        dy[r] = -R + my * st;
+        const bool sv = sidx < nstrips;
+        // a strip past the lattice gets my = -1: never inside a window
+        my[r] = sv ? sidx / nsx : -1;
+        mx0[r] = sv ? (sidx - my[r] * nsx) * kStrip : 0;
+        dy[r] = sv ? -R + my[r] * st : -R;
         dx[r] = -R + mx0[r] * st;
     }
     const float thSSE = (float)a.thSSE, distMul = (float)a.distMul;
@@ -863,28 +865,37 @@ __global__ __launch_bounds__(kThreads) void match_walk_kernel(MatchArgs a0, cons
         }
 
         // ---- per block: keys, selection of the K = G-1 smallest, the group ----
+        // The column's window spans lattice columns [m0x, m0x + nx); each
+        // block's spans rows [m0y, m0y + ny).  Scan index = (my - m0y) * nx + mx - m0x.
         const int l = search_boundary(i, 0, R, st), rr = search_boundary(i, a.W - kBlock, R, st);
-        const int nx = (rr - l) / st + 1;
+        const int nx = (rr - l) / st + 1, m0x = (l - i + R) / st;
+        uint32_t vxm[kWalkRounds];  // strip positions inside the column's window, as bits
+#pragma unroll
+        for (int r = 0; r < kWalkRounds; ++r) {
+            vxm[r] = 0;
+#pragma unroll
+            for (int c = 0; c < kStrip; ++c)
+                vxm[r] |= (uint32_t)((unsigned)(mx0[r] + c - m0x) < (unsigned)nx) << c;
+            // the block itself (offset 0, 0) is no candidate
+            const int sc = Rs - mx0[r];
+            if (my[r] == Rs && sc >= 0 && sc < kStrip) vxm[r] &= ~(1u << sc);
+        }
 #pragma unroll
         for (int k = 0; k < kWalk; ++k) {
             if (k >= nk) break;
             const int j = jk[k];
             const int t = search_boundary(j, 0, R, st), b = search_boundary(j, a.H - kBlock, R, st);
-            const int ny = (b - t) / st + 1;
+            const int ny = (b - t) / st + 1, m0y = (t - j + R) / st;
             uint32_t key[kWalkRounds][kStrip];
             int ibase[kWalkRounds];
 #pragma unroll
             for (int r = 0; r < kWalkRounds; ++r) {
-                const int y = j + dy[r];
-                const bool vy = sv[r] && y >= t && y <= t + (ny - 1) * st;
-                const int ky = (y - t) / st;
-                ibase[r] = ky * nx + (i + dx[r] - l) / st;
+                const uint32_t vm = (unsigned)(my[r] - m0y) < (unsigned)ny ? vxm[r] : 0u;
+                ibase[r] = (my[r] - m0y) * nx + mx0[r] - m0x;
 #pragma unroll
                 for (int c = 0; c < kStrip; ++c) {
-                    const int x = i + dx[r] + c * st;
                     const float d = D[r][k][c];
-                    const bool ok = vy && mx0[r] + c < NM && x >= l && x <= l + (nx - 1) * st &&
-                                    !(x == i && y == j) && d <= thSSE && d != 0.f;
+                    const bool ok = ((vm >> c) & 1u) && d <= thSSE && d != 0.f;
                     key[r][c] = ok ? __float_as_uint(d * distMul) : kNoKey;
                 }
             }
@@ -897,16 +908,16 @@ __global__ __launch_bounds__(kThreads) void match_walk_kernel(MatchArgs a0, cons
                 a.pos_out[base] = (j << 16) | i;
                 if (a.key_out) a.key_out[base] = 0.0;
             }
+            // order of the selection: rank by (key, index) as one 48-bit value
+            // (indices are below 2^16: at most 640 offsets)
+            const uint64_t vq = (uint32_t)lane < nsel ? ((uint64_t)selk[lane] << 16) | seli[lane] : ~0ull;
+            uint32_t rank = 0;
+            for (uint32_t q = 0; q < nsel; ++q) rank += __shfl(vq, (int)q) < vq;
             if ((uint32_t)lane < nsel) {
-                const uint32_t kq = selk[lane], iq = seli[lane];
-                uint32_t rank = 0;
-                for (uint32_t q = 0; q < nsel; ++q) {
-                    const uint32_t k2 = selk[q];
-                    rank += (k2 < kq) || (k2 == kq && seli[q] < iq);
-                }
+                const uint32_t iq = (uint32_t)vq & 0xFFFFu;
                 const int cy = (int)(iq / (uint32_t)nx), cx = (int)(iq - (uint32_t)cy * nx);
                 a.pos_out[base + 1 + rank] = ((t + cy * st) << 16) | (l + cx * st);
-                if (a.key_out) a.key_out[base + 1 + rank] = (double)__uint_as_float(kq);
+                if (a.key_out) a.key_out[base + 1 + rank] = (double)__uint_as_float((uint32_t)(vq >> 16));
             }
             wave_sync();
         }
