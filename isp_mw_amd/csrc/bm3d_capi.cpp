@@ -91,8 +91,10 @@ hipEvent_t take_event(isp_bm3d_ctx *ctx)
         ctx->event_pool.pop_back();
         return e;
     }
+    // timestamps only: the caller's stream synchronisation makes results
+    // visible, so the events skip the system-scope fence (cheaper records)
     hipEvent_t e = nullptr;
-    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
     return e;
 }
 
