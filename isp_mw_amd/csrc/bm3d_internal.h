@@ -41,9 +41,14 @@ constexpr int kMaxBatchRows = 1 << 20;
 // Fixed-point scale of the aggregation sums: value * 2^36 rounded to int64.
 // A pixel collects at most 64 block origins x (2R/s+1)^2 groups of weight
 // <= 1 times |sample| <= 2, below 2^19 for every supported setting, so sums
-// stay below 2^55.
+// stay below 2^55.  Wiener weights 1 / sum(W^2) exceed 1 only where every
+// gain of the group is small; they are clamped to kMaxWienerWeight, which
+// keeps each addend's integer part (|w * x| * 2^4, |x| <= 2) inside int32.
+// The clamp acts only where sum(W^2) < 2^-16, groups whose guide carries
+// almost no energy and whose estimate is near zero.
 constexpr float kFixScale = 68719476736.0f;        // 2^36
 constexpr double kFixInv = 1.0 / 68719476736.0;    // 2^-36
+constexpr float kMaxWienerWeight = 65536.0f;       // 2^16
 
 // Positions are packed (y << 16) | x.
 struct MatchArgs {

@@ -117,18 +117,27 @@ __host__ __device__ constexpr size_t align16(size_t x) { return (x + 15) & ~size
 
 typedef unsigned long long u64;
 
+// floor(x) as int32 in one instruction (|x| < 2^31)
+__device__ __forceinline__ int floor_to_int(float x)
+{
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
 // Adds v to a 64-bit fixed-point accumulator in LDS (integer LDS atomics are
 // full rate; two's complement wraps negative values correctly).  The value
-// added is floor(v * 2^36), exactly: hi = floor(v * 2^4) is the part above
-// 2^32, and the fraction (v * 2^4 - hi) in [0, 1) scaled by 2^32 is an exact
-// FP32 value whose truncation is the low word.  Six ALU operations against
-// the eleven of a rounding float -> int64 conversion.
+// added is floor(v * 2^36): hi = floor(v * 2^4) is the part above 2^32, and
+// the fraction (v * 2^4 - hi) in [0, 1), from v_fract and scaled by 2^32, is
+// an exact FP32 value whose truncation is the low word.  (v_fract returns
+// 1 - 2^-24 where the fraction would round up to 1, i.e. for |v| < 2^-28
+// below zero: the low word is then 2^32 - 256 rather than 2^32 - 1.)  Five
+// ALU operations against the eleven of a rounding float -> int64 conversion.
 __device__ __forceinline__ void fix_add(u64 *acc, float v)
 {
     const float s = v * (kFixScale / 4294967296.0f);
-    const float hi = floorf(s);
-    const uint32_t lo = (uint32_t)((s - hi) * 4294967296.0f);
-    atomicAdd(acc, ((u64)(long long)(int)hi << 32) + lo);
+    const uint32_t lo = (uint32_t)(__builtin_amdgcn_fractf(s) * 4294967296.0f);
+    atomicAdd(acc, ((u64)(long long)floor_to_int(s) << 32) + lo);
 }
 
 // fix_add(acc, w * x) with ws = w * 2^4 (the scale folded into the weight:
@@ -136,9 +145,8 @@ __device__ __forceinline__ void fix_add(u64 *acc, float v)
 __device__ __forceinline__ void fix_add_scaled(u64 *acc, float x, float ws)
 {
     const float s = x * ws;
-    const float hi = floorf(s);
-    const uint32_t lo = (uint32_t)((s - hi) * 4294967296.0f);
-    atomicAdd(acc, ((u64)(long long)(int)hi << 32) + lo);
+    const uint32_t lo = (uint32_t)(__builtin_amdgcn_fractf(s) * 4294967296.0f);
+    atomicAdd(acc, ((u64)(long long)floor_to_int(s) << 32) + lo);
 }
 
 __host__ __device__ inline size_t match_wave_bytes(int maxCand, int keyBytes)
@@ -1143,7 +1151,7 @@ __device__ __forceinline__ float group_core(float *g, int N, int lane, const flo
             }
         }
         l2 = wave_sum(l2);
-        weight = l2 <= 0.f ? 1.f : 1.f / l2;
+        weight = l2 <= 0.f ? 1.f : fminf(1.f / l2, kMaxWienerWeight);
     }
     zpass<NP>(g, N, lane, cn, true);
     wave_sync();
@@ -1742,7 +1750,7 @@ __global__ __launch_bounds__(kMThreads, 4) void filter_mfma_kernel(FilterArgs a0
                 f32x16 xh[2];
                 mfma_inverse_rest<T>(vt, ka, lane, xh);
                 const float l2 = g ? l21 : l20;
-                const float w = l2 <= 0.f ? 1.f : 1.f / l2;
+                const float w = l2 <= 0.f ? 1.f : fminf(1.f / l2, kMaxWienerWeight);
                 const float ws = w * (kFixScale / 4294967296.0f);
                 if (pz >= 0) {
                     const int py = (pz >> 16) - oy, px = (pz & 0xFFFF) - ox + 4 * h;
@@ -1850,11 +1858,11 @@ __global__ __launch_bounds__(kMThreads, 4) void filter_mfma_kernel(FilterArgs a0
                         vt[q] = mfma_inverse_half<T>(yq, cnb);
                     }
                     l20 = wave_sum(l20);
-                    w0 = l20 <= 0.f ? 1.f : 1.f / l20;
+                    w0 = l20 <= 0.f ? 1.f : fminf(1.f / l20, kMaxWienerWeight);
                     w1 = 1.f;
                     if (paired) {
                         l21 = wave_sum(l21);
-                        w1 = l21 <= 0.f ? 1.f : 1.f / l21;
+                        w1 = l21 <= 0.f ? 1.f : fminf(1.f / l21, kMaxWienerWeight);
                     }
                     mfma_inverse_rest<T>(vt, ka, lane, xh);
                 }
@@ -2011,7 +2019,7 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_stack_kernel(FilterArgs
                         vt[q] = mfma_inverse_half<T>(yq, cnb);
                     }
                     l2 = wave_sum(l2);
-                    w = l2 <= 0.f ? 1.f : 1.f / l2;
+                    w = l2 <= 0.f ? 1.f : fminf(1.f / l2, kMaxWienerWeight);
                     mfma_inverse_rest<T>(vt, ka, lane, xh);
                 }
                 const float ws = w * (kFixScale / 4294967296.0f);

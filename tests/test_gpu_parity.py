@@ -190,6 +190,22 @@ def test_config0_512_grey(gpu):
 
 
 # ---------------------------------------------------------------- edges
+@pytest.mark.parametrize("prec", ["fp32", "fp16"])
+def test_wiener_guide_without_energy(gpu, prec):
+    # a Wiener guide of 1e-6 * clean: every gain is ~1e-10 and the group
+    # weight 1 / sum(W^2) ~ 1e17, beyond the fixed-point addend's integer
+    # range; weights are clamped (2^16), so the output stays the near-zero
+    # estimate the FP64 oracle gives
+    clean, n = synth.grey_pair(48, 64, 25.0, 21)
+    p = para()
+    guide = (clean * 1e-6).astype(np.float32)
+    got = BM3D_Final(p.final, precision=prec).process(np.empty_like(n), n, guide)
+    want = oracle.kernel(p.final, n, guide, wiener=True)
+    print(f"{prec}: max |want| = {np.abs(want).max():.3g}", flush=True)
+    assert np.all(np.isfinite(got))
+    assert_close(got, want, label=f"tiny Wiener guide {prec}")
+
+
 def test_degenerate_copy_through(gpu):
     _, n = synth.grey_pair(32, 40, 25.0, 1)
     p = para(sigma=0.0)
