@@ -42,6 +42,8 @@ struct isp_bm3d_ctx {
     // tile shapes (reference blocks per workgroup) of the single-image launches
     struct Tiles {
         int match_x = ispbm3d::kTileRX, match_y = ispbm3d::kTileRY;
+        // FP32-distance matching (the mixed forms) walks tile columns: tall tiles
+        int walk_x = ispbm3d::kWalkTileX, walk_y = ispbm3d::kWalkTileY;
         int filter_x = ispbm3d::kTileRX, filter_y = ispbm3d::kTileRY;
         int mfma_x = ispbm3d::kMTileRX, mfma_y = ispbm3d::kMTileRY;
     } tiles;
@@ -384,6 +386,10 @@ int size_class(int W, int H)
 isp_bm3d_ctx::Tiles tiles_for(const isp_bm3d_ctx *ctx, int W, int H)
 {
     isp_bm3d_ctx::Tiles t = ctx->tiles;
+    if (ctx->precision != ISP_BM3D_PREC_FP32) {
+        t.match_x = t.walk_x;
+        t.match_y = t.walk_y;
+    }
     auto it = ctx->tune.find(size_class(W, H));
     if (it != ctx->tune.end()) {
         t.match_x = it->second.match_tile[0];
@@ -422,7 +428,7 @@ MatchArgs build_match(const isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, co
     // the plain FP32 path reproduces the reference's tie order bit for bit;
     // the mixed-precision forms make no parity claim and use fused FMA
     a.exact = ctx->precision == ISP_BM3D_PREC_FP32;
-    a.tileW = (trx - 1) * p.block_step + 2 * d.range + kBlock + (kStrip - 1) * p.bm_step;
+    a.tileW = (trx - 1) * p.block_step + 2 * d.range + kBlock + (kStripMax - 1) * p.bm_step;
     a.tileH = (try_ - 1) * p.block_step + 2 * d.range + kBlock;
     a.tilePitch = a.tileW | 1;
     const int side = 2 * (d.range / p.bm_step) + 1;
@@ -2122,13 +2128,14 @@ int isp_bm3d_autotune(isp_bm3d_ctx *ctx, int W, int H, double sigma, int flags, 
         return r;
     };
     const bool mf = ctx->precision != ISP_BM3D_PREC_FP32;
-    const int mcands[][2] = {{8, 4}, {8, 8}, {16, 4}, {4, 8}, {16, 8}};
+    const int mcands_v[][2] = {{8, 4}, {8, 8}, {16, 4}, {4, 8}, {16, 8}};
+    const int mcands_m[][2] = {{4, 16}, {8, 16}, {4, 8}, {8, 8}, {8, 4}};
     const int fcands_m[][2] = {{8, 8}, {16, 8}, {8, 16}, {16, 16}};
     const int fcands_v[][2] = {{8, 4}, {8, 8}, {16, 4}, {4, 8}};
-    int best_m[2] = {ctx->tiles.match_x, ctx->tiles.match_y};
+    int best_m[2] = {mf ? ctx->tiles.walk_x : ctx->tiles.match_x, mf ? ctx->tiles.walk_y : ctx->tiles.match_y};
     int best_f[2] = {mf ? ctx->tiles.mfma_x : ctx->tiles.filter_x, mf ? ctx->tiles.mfma_y : ctx->tiles.filter_y};
     double best = 1e30, psnr_base = 0;
-    for (const auto &c : mcands) {
+    for (const auto &c : mf ? mcands_m : mcands_v) {
         double ms = 0;
         rc = measure(c, best_f, 0, &ms, nullptr);
         if (rc == ISP_BM3D_E_UNSUPPORTED) continue;

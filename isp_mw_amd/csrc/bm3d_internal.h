@@ -12,9 +12,12 @@ constexpr int kMaxGroup = 32;    // largest GroupSize implemented
 // Default tile shapes (reference blocks per workgroup, x by y); the tile
 // shape is a launch parameter (MatchArgs / FilterArgs trx, try_) that the
 // autotuner may change.
+constexpr int kWalkTileX = 4;    // matching by column walks: 4 columns of 16 blocks
+constexpr int kWalkTileY = 16;
 constexpr int kTileRX = 8;       // matching and the FP32 filter
 constexpr int kTileRY = 4;
 constexpr int kStrip = 5;        // candidates per lane strip in matching
+constexpr int kStripMax = 6;     // widest strip (column walks of one round); the tile's margin
 // Capacity of a wave's accepted-candidate list in matching.  Windows with
 // more candidates reduce the list to its G-1 best whenever the next batch of
 // 64 strips could overflow it (capacity >= 31 + 64 * kStrip).
@@ -80,11 +83,12 @@ struct MatchArgs {
 };
 
 // The column-walk matching kernel takes FP32 distances of the single-frame
-// search with BlockStep below the block size (neighbouring blocks of a column
-// share rows) and windows of at most 128 lane strips of kStrip offsets.
+// search with BlockStep 3..7 (neighbouring blocks of a column share rows, and
+// a row touches at most 4 of them) and windows of at most 128 lane strips of
+// kStrip offsets.
 inline bool match_walk_eligible(const MatchArgs &a)
 {
-    if (a.exact || a.search || a.bstep >= 8 || a.step < 1) return false;
+    if (a.exact || a.search || a.bstep < 3 || a.bstep >= 8 || a.step < 1) return false;
     const int nm = 2 * (a.range / a.step) + 1;
     return nm * ((nm + 4) / 5) <= 128;
 }

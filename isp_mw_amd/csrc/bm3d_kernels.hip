@@ -550,20 +550,25 @@ __global__ __launch_bounds__(kThreads) void match_kernel(MatchArgs a0, const uin
 
 // ---------------------------------------------------------------------------
 // Block matching by column walks: FP32 distances, the single-frame search,
-// BlockStep below the block size, and windows of at most 2 x 64 lane strips.
-// One wave takes up to kWalk vertically consecutive reference blocks of one
-// grid column.  The windows of one column share their lattice of offsets
-// (dy, dx) (the range is a multiple of BMstep; each block's own clipping
-// decides which offsets are candidates).  Every lane owns two strips of
-// kStrip offsets for the whole walk.  Each image row's 8-term sum of squared
-// differences is formed once per offset and added to every block of the walk
-// that covers the row.  A block's distance is then the sum of its eight row
-// sums, added in row order from 0: the same wherever the block lies in a
-// walk, and the same as in match_kernel.  At BlockStep 3 a walk of 4 blocks
-// forms 17 row sums for 32 uses.  The keys stay in registers; the selection
-// reads them there.
-constexpr int kWalk = 4;
-constexpr int kWalkRounds = 2;
+// BlockStep 3..7, and windows of at most 2 x 64 lane strips.  One wave walks
+// down one grid column of the tile.  The windows of one column share their
+// lattice of offsets (dy, dx) (the range is a multiple of BMstep; each
+// block's own clipping decides which offsets are candidates).  Every lane
+// owns RND strips of STRIP offsets for the whole walk (two of 5, or one of
+// 6 where 64 such strips cover the window).  Each image row's
+// 8-term sum of squared differences is formed once per offset and added to
+// every block of the column that covers the row (held in a ring of
+// kWalkSlots register slots).  A block's distance
+// is then the sum of its eight row sums, added in row order from 0: the same
+// wherever the block lies in a walk, and the same as in match_kernel.  A
+// block is selected as soon as its last row is in; its slot then takes the
+// block kWalkSlots further down.  At BlockStep 3 a column of 16 blocks forms
+// 53 row sums for 128 uses.  The keys stay in registers; the selection reads
+// them there.
+// Register slots of the ring: the blocks a row can touch.  Block k's rows
+// end at j_k + 7; block k + P starts at j_k + (P-1) s + 1 or later (the last
+// grid position may be clamped), so P slots suffice where (P-1) s >= 7:
+// P = 4 at BlockStep 3, P = 3 at BlockStep 4..7.
 constexpr uint32_t kNoKey = ~0u;
 
 // per wave: hist u32[256], memk u32[64], memi u32[64], selk u32[32], seli u32[32]
@@ -575,17 +580,18 @@ __host__ __device__ constexpr size_t walk_wave_bytes() { return (256 + 64 + 64 +
 // order.  The radix select of select_smallest with the histogram passes over
 // registers; the last bucket's members (at most 64) are ranked exactly in
 // LDS, or, when more of them share one key, taken in index order.
-__device__ __forceinline__ uint32_t select_walk(const uint32_t (&key)[kWalkRounds][kStrip],
-                                                const int (&ibase)[kWalkRounds], uint32_t K, uint32_t *selk,
+template <int RND, int STRIP>
+__device__ __forceinline__ uint32_t select_walk(const uint32_t (&key)[RND][STRIP],
+                                                const int (&ibase)[RND], uint32_t K, uint32_t *selk,
                                                 uint32_t *seli, uint32_t *hist, uint32_t *memk, uint32_t *memi,
                                                 int lane)
 {
     int n = 0;
     uint32_t kmin = kNoKey, kmax = 0;
 #pragma unroll
-    for (int r = 0; r < kWalkRounds; ++r)
+    for (int r = 0; r < RND; ++r)
 #pragma unroll
-        for (int c = 0; c < kStrip; ++c) {
+        for (int c = 0; c < STRIP; ++c) {
             const uint32_t k = key[r][c];
             if (k != kNoKey) {
                 ++n;
@@ -597,9 +603,9 @@ __device__ __forceinline__ uint32_t select_walk(const uint32_t (&key)[kWalkRound
     uint32_t nsel = 0;
     if (total <= K) {
 #pragma unroll
-        for (int r = 0; r < kWalkRounds; ++r)
+        for (int r = 0; r < RND; ++r)
 #pragma unroll
-            for (int c = 0; c < kStrip; ++c) {
+            for (int c = 0; c < STRIP; ++c) {
                 const bool take = key[r][c] != kNoKey;
                 const uint64_t tm = __ballot(take);
                 if (take) {
@@ -629,9 +635,9 @@ __device__ __forceinline__ uint32_t select_walk(const uint32_t (&key)[kWalkRound
         for (int q = lane; q < 256; q += 64) hist[q] = 0;
         wave_sync();
 #pragma unroll
-        for (int r = 0; r < kWalkRounds; ++r)
+        for (int r = 0; r < RND; ++r)
 #pragma unroll
-            for (int c = 0; c < kStrip; ++c) {
+            for (int c = 0; c < STRIP; ++c) {
                 const uint32_t k = key[r][c];
                 if (k != kNoKey && (k & pmask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
             }
@@ -675,9 +681,9 @@ __device__ __forceinline__ uint32_t select_walk(const uint32_t (&key)[kWalkRound
     const bool rank = !whole && mm <= 64;
     uint32_t nm = 0;
 #pragma unroll
-    for (int r = 0; r < kWalkRounds; ++r)
+    for (int r = 0; r < RND; ++r)
 #pragma unroll
-        for (int c = 0; c < kStrip; ++c) {
+        for (int c = 0; c < STRIP; ++c) {
             const uint32_t k = key[r][c];
             const bool inr = k != kNoKey;
             const bool below = inr && (k & pmask) < prefix;
@@ -726,14 +732,14 @@ __device__ __forceinline__ uint32_t select_walk(const uint32_t (&key)[kWalkRound
         // `need` smallest indices, which come first in (r, lane, c) order
         uint32_t before = 0;
 #pragma unroll
-        for (int r = 0; r < kWalkRounds; ++r) {
+        for (int r = 0; r < RND; ++r) {
             uint32_t cl = 0;
 #pragma unroll
-            for (int c = 0; c < kStrip; ++c) cl += key[r][c] != kNoKey && (key[r][c] & pmask) == prefix;
+            for (int c = 0; c < STRIP; ++c) cl += key[r][c] != kNoKey && (key[r][c] & pmask) == prefix;
             const uint32_t incl = wave_incl_scan(cl, lane);
             uint32_t p = before + incl - cl;
 #pragma unroll
-            for (int c = 0; c < kStrip; ++c) {
+            for (int c = 0; c < STRIP; ++c) {
                 const uint32_t k = key[r][c];
                 if (k != kNoKey && (k & pmask) == prefix) {
                     if (p < need) {
@@ -752,7 +758,9 @@ __device__ __forceinline__ uint32_t select_walk(const uint32_t (&key)[kWalkRound
 }
 
 // 4 waves per SIMD: at most 128 registers
-template <typename GT, bool ST1, bool BATCHED>
+// RND rounds of 64 lane strips of STRIP offsets: (2, 5) or, where 64 strips
+// of 6 cover the window, (1, 6)
+template <typename GT, bool ST1, bool BATCHED, int kWalkSlots, int RND, int STRIP>
 __global__ __launch_bounds__(kThreads, 4) void match_walk_kernel(MatchArgs a0, const uint32_t *__restrict__ bmap,
                                                               const MatchArgs *__restrict__ bargs)
 {
@@ -782,152 +790,159 @@ __global__ __launch_bounds__(kThreads, 4) void match_walk_kernel(MatchArgs a0, c
     }
     __syncthreads();
 
-    // this lane's strips: offsets (dy[r], dx[r] + c * st), c < kStrip; a strip
+    // this lane's strips: offsets (dy[r], dx[r] + c * st), c < STRIP; a strip
     // past the lattice reads offsets -R (inside the tile) and keeps no key
     // lattice indices: offset (dy, dx) = (-R + my * st, -R + mx * st), my, mx < NM
     const int NM = 2 * R / st + 1, Rs = R / st;
-    const int nsx = (NM + kStrip - 1) / kStrip, nstrips = NM * nsx;
-    int dy[kWalkRounds], dx[kWalkRounds], my[kWalkRounds], mx0[kWalkRounds];
+    const int nsx = (NM + STRIP - 1) / STRIP, nstrips = NM * nsx;
+    int dy[RND], dx[RND], my[RND], mx0[RND];
 #pragma unroll
-    for (int r = 0; r < kWalkRounds; ++r) {
+    for (int r = 0; r < RND; ++r) {
         const int sidx = r * 64 + lane;
         const bool sv = sidx < nstrips;
         // a strip past the lattice gets my = -1: never inside a window
         my[r] = sv ? sidx / nsx : -1;
-        mx0[r] = sv ? (sidx - my[r] * nsx) * kStrip : 0;
+        mx0[r] = sv ? (sidx - my[r] * nsx) * STRIP : 0;
         dy[r] = sv ? -R + my[r] * st : -R;
         dx[r] = -R + mx0[r] * st;
     }
     const float thSSE = (float)a.thSSE, distMul = (float)a.distMul;
     const uint32_t K = (uint32_t)(a.G - 1);
-    const int nseg = (try_ + kWalk - 1) / kWalk;
+    const int nk = min(try_, a.ngy - ty0);  // blocks of each column in this tile
 
-    for (int w = wave; w < trx * nseg; w += kWaves) {
-        const int col = w % trx, seg = w / trx;
-        const int gx = tx0 + col, gy0 = ty0 + seg * kWalk;
-        const int nk = min(kWalk, min(try_ - seg * kWalk, a.ngy - gy0));
+    for (int col = wave; col < trx; col += kWaves) {
+        const int gx = tx0 + col;
         if (gx >= a.ngx || nk <= 0) continue;
         const int i = grid_pos(gx, a.bstep, a.W - kBlock);
-        int jk[kWalk];
-#pragma unroll
-        for (int k = 0; k < kWalk; ++k) jk[k] = grid_pos(gy0 + min(k, nk - 1), a.bstep, a.H - kBlock);
         if (a.bypass) {
-            if (lane < nk) {
-                const int j = grid_pos(gy0 + lane, a.bstep, a.H - kBlock);
-                const size_t refIdx = (size_t)(gy0 + lane) * a.ngx + gx;
+            for (int k = lane; k < nk; k += 64) {
+                const int j = grid_pos(ty0 + k, a.bstep, a.H - kBlock);
+                const size_t refIdx = (size_t)(ty0 + k) * a.ngx + gx;
                 a.cnt_out[refIdx] = 1;
                 a.pos_out[refIdx * a.G] = (j << 16) | i;
                 if (a.key_out) a.key_out[refIdx * a.G] = 0.0;
             }
             continue;
         }
-
-        // ---- distances of the walk's blocks: row sums over the rows they cover ----
-        float D[kWalkRounds][kWalk][kStrip];
-#pragma unroll
-        for (int r = 0; r < kWalkRounds; ++r)
-#pragma unroll
-            for (int k = 0; k < kWalk; ++k)
-#pragma unroll
-                for (int c = 0; c < kStrip; ++c) D[r][k][c] = 0.f;
-        const float *tref = tile + (i - ox);
-        for (int y = jk[0]; y < jk[nk - 1] + kBlock; ++y) {
-            float rv[kBlock];
-#pragma unroll
-            for (int v = 0; v < kBlock; ++v) rv[v] = tref[(y - oy) * pitch + v];
-#pragma unroll
-            for (int r = 0; r < kWalkRounds; ++r) {
-                const float *tc = tile + (y + dy[r] - oy) * pitch + (i + dx[r] - ox);
-                float rs[kStrip];
-                if constexpr (ST1) {
-                    float cv[kBlock + kStrip - 1];
-#pragma unroll
-                    for (int q = 0; q < kBlock + kStrip - 1; ++q) cv[q] = tc[q];
-#pragma unroll
-                    for (int c = 0; c < kStrip; ++c) {
-                        rs[c] = 0.f;
-#pragma unroll
-                        for (int v = 0; v < kBlock; ++v) {
-                            const float d = rv[v] - cv[c + v];
-                            rs[c] = fmaf(d, d, rs[c]);
-                        }
-                    }
-                } else {
-#pragma unroll
-                    for (int c = 0; c < kStrip; ++c) {
-                        rs[c] = 0.f;
-#pragma unroll
-                        for (int v = 0; v < kBlock; ++v) {
-                            const float d = rv[v] - tc[c * st + v];
-                            rs[c] = fmaf(d, d, rs[c]);
-                        }
-                    }
-                }
-#pragma unroll
-                for (int k = 0; k < kWalk; ++k)
-                    if (k < nk && y >= jk[k] && y < jk[k] + kBlock) {
-#pragma unroll
-                        for (int c = 0; c < kStrip; ++c) D[r][k][c] += rs[c];
-                    }
-            }
-        }
-
-        // ---- per block: keys, selection of the K = G-1 smallest, the group ----
         // The column's window spans lattice columns [m0x, m0x + nx); each
         // block's spans rows [m0y, m0y + ny).  Scan index = (my - m0y) * nx + mx - m0x.
         const int l = search_boundary(i, 0, R, st), rr = search_boundary(i, a.W - kBlock, R, st);
         const int nx = (rr - l) / st + 1, m0x = (l - i + R) / st;
-        uint32_t vxm[kWalkRounds];  // strip positions inside the column's window, as bits
+        uint32_t vxm[RND];  // strip positions inside the column's window, as bits
 #pragma unroll
-        for (int r = 0; r < kWalkRounds; ++r) {
+        for (int r = 0; r < RND; ++r) {
             vxm[r] = 0;
 #pragma unroll
-            for (int c = 0; c < kStrip; ++c)
+            for (int c = 0; c < STRIP; ++c)
                 vxm[r] |= (uint32_t)((unsigned)(mx0[r] + c - m0x) < (unsigned)nx) << c;
             // the block itself (offset 0, 0) is no candidate
             const int sc = Rs - mx0[r];
-            if (my[r] == Rs && sc >= 0 && sc < kStrip) vxm[r] &= ~(1u << sc);
+            if (my[r] == Rs && sc >= 0 && sc < STRIP) vxm[r] &= ~(1u << sc);
         }
-#pragma unroll
-        for (int k = 0; k < kWalk; ++k) {
-            if (k >= nk) break;
-            const int j = jk[k];
-            const int t = search_boundary(j, 0, R, st), b = search_boundary(j, a.H - kBlock, R, st);
-            const int ny = (b - t) / st + 1, m0y = (t - j + R) / st;
-            uint32_t key[kWalkRounds][kStrip];
-            int ibase[kWalkRounds];
-#pragma unroll
-            for (int r = 0; r < kWalkRounds; ++r) {
-                const uint32_t vm = (unsigned)(my[r] - m0y) < (unsigned)ny ? vxm[r] : 0u;
-                ibase[r] = (my[r] - m0y) * nx + mx0[r] - m0x;
-#pragma unroll
-                for (int c = 0; c < kStrip; ++c) {
-                    const float d = D[r][k][c];
-                    const bool ok = ((vm >> c) & 1u) && d <= thSSE && d != 0.f;
-                    key[r][c] = ok ? __float_as_uint(d * distMul) : kNoKey;
-                }
-            }
-            const uint32_t nsel = select_walk(key, ibase, K, selk, seli, hist, memk, memi, lane);
 
-            const size_t refIdx = (size_t)(gy0 + k) * a.ngx + gx;
-            const size_t base = refIdx * a.G;
-            if (lane == 0) {
-                a.cnt_out[refIdx] = (int)(1 + nsel);
-                a.pos_out[base] = (j << 16) | i;
-                if (a.key_out) a.key_out[base] = 0.0;
+        float D[RND][kWalkSlots][STRIP];
+#pragma unroll
+        for (int r = 0; r < RND; ++r)
+#pragma unroll
+            for (int q = 0; q < kWalkSlots; ++q)
+#pragma unroll
+                for (int c = 0; c < STRIP; ++c) D[r][q][c] = 0.f;
+        const float *tref = tile + (i - ox);
+        int y = grid_pos(ty0, a.bstep, a.H - kBlock);
+        for (int m = 0; m < nk; m += kWalkSlots) {
+#pragma unroll
+            for (int ph = 0; ph < kWalkSlots; ++ph) {
+                const int k = m + ph;
+                if (k >= nk) break;
+                // blocks k .. k + kWalkSlots - 1 (slot (ph + d) % kWalkSlots)
+                int jd[kWalkSlots];
+#pragma unroll
+                for (int d = 0; d < kWalkSlots; ++d)
+                    jd[d] = k + d < nk ? grid_pos(ty0 + k + d, a.bstep, a.H - kBlock) : 0x3FFFFFFF;
+                const int j = jd[0];
+
+                // ---- the rows up to block k's last, added to the blocks covering them ----
+                for (; y < j + kBlock; ++y) {
+                    float rv[kBlock];
+#pragma unroll
+                    for (int v = 0; v < kBlock; ++v) rv[v] = tref[(y - oy) * pitch + v];
+#pragma unroll
+                    for (int r = 0; r < RND; ++r) {
+                        const float *tc = tile + (y + dy[r] - oy) * pitch + (i + dx[r] - ox);
+                        float rs[STRIP];
+                        if constexpr (ST1) {
+                            float cv[kBlock + STRIP - 1];
+#pragma unroll
+                            for (int q = 0; q < kBlock + STRIP - 1; ++q) cv[q] = tc[q];
+#pragma unroll
+                            for (int c = 0; c < STRIP; ++c) {
+                                rs[c] = 0.f;
+#pragma unroll
+                                for (int v = 0; v < kBlock; ++v) {
+                                    const float dd = rv[v] - cv[c + v];
+                                    rs[c] = fmaf(dd, dd, rs[c]);
+                                }
+                            }
+                        } else {
+#pragma unroll
+                            for (int c = 0; c < STRIP; ++c) {
+                                rs[c] = 0.f;
+#pragma unroll
+                                for (int v = 0; v < kBlock; ++v) {
+                                    const float dd = rv[v] - tc[c * st + v];
+                                    rs[c] = fmaf(dd, dd, rs[c]);
+                                }
+                            }
+                        }
+#pragma unroll
+                        for (int d = 0; d < kWalkSlots; ++d)
+                            if (y >= jd[d] && y < jd[d] + kBlock) {
+#pragma unroll
+                                for (int c = 0; c < STRIP; ++c)
+                                    D[r][(ph + d) % kWalkSlots][c] += rs[c];
+                            }
+                    }
+                }
+
+                // ---- block k is complete: keys, selection of the K = G-1 smallest, the group ----
+                const int t = search_boundary(j, 0, R, st), b = search_boundary(j, a.H - kBlock, R, st);
+                const int ny = (b - t) / st + 1, m0y = (t - j + R) / st;
+                uint32_t key[RND][STRIP];
+                int ibase[RND];
+#pragma unroll
+                for (int r = 0; r < RND; ++r) {
+                    const uint32_t vm = (unsigned)(my[r] - m0y) < (unsigned)ny ? vxm[r] : 0u;
+                    ibase[r] = (my[r] - m0y) * nx + mx0[r] - m0x;
+#pragma unroll
+                    for (int c = 0; c < STRIP; ++c) {
+                        const float dd = D[r][ph][c];
+                        const bool ok = ((vm >> c) & 1u) && dd <= thSSE && dd != 0.f;
+                        key[r][c] = ok ? __float_as_uint(dd * distMul) : kNoKey;
+                        D[r][ph][c] = 0.f;  // the slot takes block k + kWalkSlots
+                    }
+                }
+                const uint32_t nsel = select_walk<RND, STRIP>(key, ibase, K, selk, seli, hist, memk, memi, lane);
+
+                const size_t refIdx = (size_t)(ty0 + k) * a.ngx + gx;
+                const size_t base = refIdx * a.G;
+                if (lane == 0) {
+                    a.cnt_out[refIdx] = (int)(1 + nsel);
+                    a.pos_out[base] = (j << 16) | i;
+                    if (a.key_out) a.key_out[base] = 0.0;
+                }
+                // order of the selection: rank by (key, index) as one 48-bit value
+                // (indices are below 2^16: at most 640 offsets)
+                const uint64_t vq = (uint32_t)lane < nsel ? ((uint64_t)selk[lane] << 16) | seli[lane] : ~0ull;
+                uint32_t rank = 0;
+                for (uint32_t q = 0; q < nsel; ++q) rank += __shfl(vq, (int)q) < vq;
+                if ((uint32_t)lane < nsel) {
+                    const uint32_t iq = (uint32_t)vq & 0xFFFFu;
+                    const int cy = (int)(iq / (uint32_t)nx), cx = (int)(iq - (uint32_t)cy * nx);
+                    a.pos_out[base + 1 + rank] = ((t + cy * st) << 16) | (l + cx * st);
+                    if (a.key_out) a.key_out[base + 1 + rank] = (double)__uint_as_float((uint32_t)(vq >> 16));
+                }
+                wave_sync();
             }
-            // order of the selection: rank by (key, index) as one 48-bit value
-            // (indices are below 2^16: at most 640 offsets)
-            const uint64_t vq = (uint32_t)lane < nsel ? ((uint64_t)selk[lane] << 16) | seli[lane] : ~0ull;
-            uint32_t rank = 0;
-            for (uint32_t q = 0; q < nsel; ++q) rank += __shfl(vq, (int)q) < vq;
-            if ((uint32_t)lane < nsel) {
-                const uint32_t iq = (uint32_t)vq & 0xFFFFu;
-                const int cy = (int)(iq / (uint32_t)nx), cx = (int)(iq - (uint32_t)cy * nx);
-                a.pos_out[base + 1 + rank] = ((t + cy * st) << 16) | (l + cx * st);
-                if (a.key_out) a.key_out[base + 1 + rank] = (double)__uint_as_float((uint32_t)(vq >> 16));
-            }
-            wave_sync();
         }
     }
 }
@@ -2197,8 +2212,20 @@ hipError_t launch_match_t(const MatchArgs &a, hipStream_t s, const BatchLaunch *
     if constexpr (!EXACT) {
         if (a.walk) {
             const dim3 grid((a.ngx + a.trx - 1) / a.trx, (a.ngy + a.try_ - 1) / a.try_);
-            return b ? launch_tiled_m(match_walk_kernel<GT, ST1, true>, lds, a, s, b, grid)
-                     : launch_tiled_m(match_walk_kernel<GT, ST1, false>, lds, a, s, b, grid);
+            const int nm = 2 * (a.range / a.step) + 1;
+            const bool one = nm * ((nm + 5) / 6) <= 64;  // one round of strips of 6
+            if (a.bstep == 3) {
+                if (one)
+                    return b ? launch_tiled_m(match_walk_kernel<GT, ST1, true, 4, 1, 6>, lds, a, s, b, grid)
+                             : launch_tiled_m(match_walk_kernel<GT, ST1, false, 4, 1, 6>, lds, a, s, b, grid);
+                return b ? launch_tiled_m(match_walk_kernel<GT, ST1, true, 4, 2, 5>, lds, a, s, b, grid)
+                         : launch_tiled_m(match_walk_kernel<GT, ST1, false, 4, 2, 5>, lds, a, s, b, grid);
+            }
+            if (one)
+                return b ? launch_tiled_m(match_walk_kernel<GT, ST1, true, 3, 1, 6>, lds, a, s, b, grid)
+                         : launch_tiled_m(match_walk_kernel<GT, ST1, false, 3, 1, 6>, lds, a, s, b, grid);
+            return b ? launch_tiled_m(match_walk_kernel<GT, ST1, true, 3, 2, 5>, lds, a, s, b, grid)
+                     : launch_tiled_m(match_walk_kernel<GT, ST1, false, 3, 2, 5>, lds, a, s, b, grid);
         }
     }
     if (b) {

@@ -5,10 +5,12 @@ Both sum a candidate's distance the same way (each row's 8 fused terms from 0,
 then the row sums in row order), so they must give identical groups: counts,
 positions, order and keys, bit for bit, and therefore identical outputs.  The
 environment variable ISP_BM3D_MATCH_WALK=0 at context creation keeps every
-search on the list kernel.  Cases cover block steps 1..6, BMsteps 1..3,
-ranges, group sizes 2..32, grids that end in clamped positions, windows
-clipped on every side, quantised images full of equal distances, RGB frames,
-batches, multi-frame stacks and NL-means."""
+search on the list kernel.  The walk takes BlockStep 3..7 (smaller
+steps stay on the list kernel, so both runs agree trivially).  Cases cover
+block steps 1..7, BMsteps 1..3, ranges, group sizes 2..32, grids that end in
+clamped positions, windows clipped on every side, columns of 4 to 32 blocks
+per tile, quantised images full of equal distances, RGB frames, batches,
+multi-frame stacks and NL-means."""
 import numpy as np
 import pytest
 
@@ -55,6 +57,8 @@ CASES = [
     (70, 60, 6, 9, 1, 32, 32),
     (20, 24, 3, 12, 1, 16, 32),    # window larger than the image
     (33, 41, 4, 10, 1, 2, 4),      # K = 1 and 3
+    (75, 66, 7, 8, 2, 16, 16),
+    (100, 37, 3, 12, 1, 16, 32),   # columns of 30 blocks: two tiles, the second clamped
     (40, 40, 4, 16, 1, 16, 32),    # 231 strips: both runs on the list kernel
 ]
 
@@ -77,6 +81,24 @@ def test_groups_identical(gpu, monkeypatch, case, prec):
     for s in (0, 1):
         assert_same_groups(g0[s], g1[s], f"{case} step {s + 1}")
     np.testing.assert_array_equal(out0, out1)
+
+
+@pytest.mark.parametrize("tile", [(8, 4), (2, 32), (8, 8), (4, 16)])
+def test_tile_shapes_do_not_change_groups(gpu, monkeypatch, tile):
+    # walks of 4, 32, 8 and 16 blocks per column against the list kernel
+    p = para()
+    _, n = synth.grey_pair(90, 70, 25.0, 8)
+    print(f"match tile {tile}", flush=True)
+
+    def go():
+        f = BM3D(p, precision="fp16")
+        f.set_tuning(f.size_class(70, 90), {"match_tile": tile, "filter_tile": (8, 8), "final_group_size": 0})
+        return [f.match(n, final_step=s) for s in (False, True)]
+
+    g0 = with_walk(monkeypatch, False, go)
+    g1 = with_walk(monkeypatch, True, go)
+    for s in (0, 1):
+        assert_same_groups(g0[s], g1[s], f"tile {tile} step {s + 1}")
 
 
 @pytest.mark.parametrize("levels", [4, 16, 255])
