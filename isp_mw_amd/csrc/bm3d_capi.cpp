@@ -428,15 +428,16 @@ MatchArgs build_match(const isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, co
     // the plain FP32 path reproduces the reference's tie order bit for bit;
     // the mixed-precision forms make no parity claim and use fused FMA
     a.exact = ctx->precision == ISP_BM3D_PREC_FP32;
-    a.tileW = (trx - 1) * p.block_step + 2 * d.range + kBlock + (kStripMax - 1) * p.bm_step;
-    a.tileH = (try_ - 1) * p.block_step + 2 * d.range + kBlock;
-    a.tilePitch = a.tileW | 1;
     const int side = 2 * (d.range / p.bm_step) + 1;
     a.maxCand = std::min(side * side, kMatchListCap);
     a.cnt_out = cnt;
     a.pos_out = pos;
     a.key_out = keys;
     a.walk = ctx->match_walk && match_walk_eligible(a) ? 1 : 0;
+    // the staged tile: the window of every block plus the widest lane strip
+    a.tileW = (trx - 1) * p.block_step + 2 * d.range + kBlock + ((a.walk ? kStripMax : kStrip) - 1) * p.bm_step;
+    a.tileH = (try_ - 1) * p.block_step + 2 * d.range + kBlock;
+    a.tilePitch = a.tileW | 1;
     return a;
 }
 

@@ -1750,15 +1750,23 @@ __global__ __launch_bounds__(kMThreads, 4) void filter_mfma_kernel(FilterArgs a0
 #pragma unroll
                     for (int k = 0; k < 16; ++k) {
                         // group-axis frequency: the row within the group
-                        const int kz = (8 * (k >> 2) + 4 * h + (k & 3)) & (paired ? 15 : 31);
+                        // group-axis DC: row 8(k/4) + 4h + k%4 is 0 (or 16 when paired),
+                        // possible only where k % 4 == 0 (static) and h == 0
+                        const bool kz0 = (k & 3) == 0 && h == 0 && (paired ? ((k >> 2) & 1) == 0 : k == 0);
                         const float yy = yg[k] * yg[k];
                         // 1-ulp reciprocal: the operands already carry 16-bit rounding
-                        const float wc = yy * __builtin_amdgcn_rcpf(yy + (kz == 0 ? 0.5f * su : su));
+                        const float wc = yy * __builtin_amdgcn_rcpf(yy + (kz0 ? 0.5f * su : su));
                         yq[k] *= wc;
-                        if (k >= 8 && paired) l21 = fmaf(wc, wc, l21);
-                        else l20 = fmaf(wc, wc, l20);
+                        // rows 0..15 (elements k < 8) and 16..31 apart; one group
+                        // adds them below (the order of the stack kernel's sum)
+                        if (k < 8) l20 = fmaf(wc, wc, l20);
+                        else l21 = fmaf(wc, wc, l21);
                     }
                     vt[q] = mfma_inverse_half<T>(yq, cnb);
+                }
+                if (!paired) {
+                    l20 += l21;
+                    l21 = 0.f;
                 }
                 l20 = wave_sum(l20);
                 l21 = wave_sum(l21);
@@ -1862,10 +1870,12 @@ __global__ __launch_bounds__(kMThreads, 4) void filter_mfma_kernel(FilterArgs a0
 #pragma unroll
                         for (int k = 0; k < 16; ++k) {
                             // group-axis frequency (row within the group)
-                            const int kz = (8 * (k >> 2) + 4 * h + (k & 3)) & (paired ? 15 : 31);
+                            // group-axis DC: row 8(k/4) + 4h + k%4 is 0 (or 16 when paired),
+                            // possible only where k % 4 == 0 (static) and h == 0
+                            const bool kz0 = (k & 3) == 0 && h == 0 && (paired ? ((k >> 2) & 1) == 0 : k == 0);
                             const float yy = yg[k] * yg[k];
                             // 1-ulp reciprocal: the operands already carry 16-bit rounding
-                            const float wc = yy * __builtin_amdgcn_rcpf(yy + (kz == 0 ? 0.5f * su : su));
+                            const float wc = yy * __builtin_amdgcn_rcpf(yy + (kz0 ? 0.5f * su : su));
                             yq[k] *= wc;
                             if (paired && k >= 8) l21 = fmaf(wc, wc, l21);
                             else l20 = fmaf(wc, wc, l20);
@@ -2014,7 +2024,7 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_stack_kernel(FilterArgs
                     mfma_load_x<T>(fgd, fp, fox, foy, pos, N, lane, xg, slot, sstride);
                     mfma_load_x<T>(fsrc, fp, fox, foy, pos, N, lane, xs, slot, sstride);
                     f32x16 vt[2];
-                    float l2 = 0.f;
+                    float l2 = 0.f, l2b = 0.f;
 #pragma unroll
                     for (int q = 0; q < 2; ++q) {
                         const f32x16 yg = mfma_forward_half<T>(xg, q, kb, lane, cna);
@@ -2024,16 +2034,19 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_stack_kernel(FilterArgs
                             sig2 * ((4 * q + (c >> 3)) == 0 ? 0.5f : 1.f) * ((c & 7) == 0 ? 0.5f : 1.f);
 #pragma unroll
                         for (int k = 0; k < 16; ++k) {
-                            const int kz = 8 * (k >> 2) + 4 * h + (k & 3);  // group-axis frequency
+                            const bool kz0 = k == 0 && h == 0;  // group-axis DC (row 0)
                             const float yy = yg[k] * yg[k];
                             // 1-ulp reciprocal: the operands already carry 16-bit rounding
-                            const float wc = yy * __builtin_amdgcn_rcpf(yy + (kz == 0 ? 0.5f * su : su));
+                            const float wc = yy * __builtin_amdgcn_rcpf(yy + (kz0 ? 0.5f * su : su));
                             yq[k] *= wc;
-                            l2 = fmaf(wc, wc, l2);
+                            // rows 0..15 and 16..31 apart, then added (the order of
+                            // the single-frame kernel, whose operands pair groups)
+                            if (k < 8) l2 = fmaf(wc, wc, l2);
+                            else l2b = fmaf(wc, wc, l2b);
                         }
                         vt[q] = mfma_inverse_half<T>(yq, cnb);
                     }
-                    l2 = wave_sum(l2);
+                    l2 = wave_sum(l2 + l2b);
                     w = l2 <= 0.f ? 1.f : fminf(1.f / l2, kMaxWienerWeight);
                     mfma_inverse_rest<T>(vt, ka, lane, xh);
                 }
