@@ -23,6 +23,12 @@ namespace {
 
 typedef unsigned long long u64;
 
+// 8 waves per workgroup: one wave per group, and a tile's LDS accumulators
+// allow few workgroups per CU, so more waves per workgroup keep more groups
+// (and their sample loads) in flight
+constexpr int kNlmWaves = 8;
+constexpr int kNlmThreads = 64 * kNlmWaves;
+
 __device__ __forceinline__ int grid_pos(int k, int s, int last) { return min(k * s, last); }
 
 template <typename ST>
@@ -52,10 +58,10 @@ __device__ void nlm_tile(const NlmArgs &a, u64 *acc, int tx, int ty)
     const int FW = a.FW, FH = a.FH, np = a.nplanes;
     const int u = lane >> 3, v = lane & 7;
 
-    for (int k = tid; k < np * FW * FH; k += kThreads) acc[k] = 0;
+    for (int k = tid; k < np * FW * FH; k += kNlmThreads) acc[k] = 0;
     __syncthreads();
 
-    for (int r = wave; r < tnum; r += kWaves) {
+    for (int r = wave; r < tnum; r += kNlmWaves) {
         const int gxi = tx0 + (r % trx), gyi = ty0 + (r / trx);
         if (gxi >= a.ngx || gyi >= a.ngy) continue;
         const int gx = grid_pos(gxi, a.bstep, lastx), gy = grid_pos(gyi, a.bstep, lasty);
@@ -104,7 +110,7 @@ __device__ void nlm_tile(const NlmArgs &a, u64 *acc, int tx, int ty)
     }
     __syncthreads();
     // the tile's footprint into num; neighbouring tiles overlap by 8 - BlockStep
-    for (int k = tid; k < np * FW * FH; k += kThreads) {
+    for (int k = tid; k < np * FW * FH; k += kNlmThreads) {
         const u64 s = acc[k];
         if (s == 0) continue;
         const int c = k / (FW * FH), q = k - c * FW * FH;
@@ -116,7 +122,7 @@ __device__ void nlm_tile(const NlmArgs &a, u64 *acc, int tx, int ty)
 }
 
 template <typename ST>
-__global__ __launch_bounds__(kThreads) void nlm_kernel(NlmArgs a)
+__global__ __launch_bounds__(kNlmThreads) void nlm_kernel(NlmArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     nlm_tile<ST>(a, reinterpret_cast<u64 *>(smem), (int)blockIdx.x, (int)blockIdx.y);
@@ -171,7 +177,7 @@ hipError_t launch_nlm_t(const NlmArgs &a, hipStream_t s)
     hipError_t e;
     if ((e = set_lds(nlm_kernel<ST>, lds)) != hipSuccess) return e;
     const dim3 grid((a.ngx + a.trx - 1) / a.trx, (a.ngy + a.try_ - 1) / a.try_);
-    hipLaunchKernelGGL(nlm_kernel<ST>, grid, dim3(kThreads), lds, s, a);
+    hipLaunchKernelGGL(nlm_kernel<ST>, grid, dim3(kNlmThreads), lds, s, a);
     return hipGetLastError();
 }
 
