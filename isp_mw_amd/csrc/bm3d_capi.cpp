@@ -575,13 +575,13 @@ int run_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, cons
         rc = run_match(ctx, p, d, w, io.mguide32, io.mguide64, io.mguide64 ? io.pitch64 : io.pitch, W, H,
                        nullptr, s, wiener ? ST_MATCH_FINAL : ST_MATCH_BASIC);
         if (rc) return rc;
-        for (int c = 0; c < 3; ++c) {
-            if (d.active[c]) {
-                StageTimer tm(ctx, ST_FILL, s);
-                // zero bits of the 64-bit accumulators
-                HIP_CHECK(ctx, launch_fill(reinterpret_cast<float *>(w.num[c]), (size_t)io.pitch * H * 2, 0.f, s));
-                HIP_CHECK(ctx, launch_fill(reinterpret_cast<float *>(w.den[c]), (size_t)io.pitch * H * 2, 0.f, s));
-            }
+        {
+            StageTimer tm(ctx, ST_FILL, s);
+            // zero bits of the 64-bit accumulators: num[0..2] and den[0..2] are
+            // one contiguous span of the workspace (layout), cleared by one launch
+            char *lo = reinterpret_cast<char *>(w.num[0]);
+            char *hi = reinterpret_cast<char *>(w.den[2] + (size_t)io.pitch * H);
+            HIP_CHECK(ctx, launch_fill(reinterpret_cast<float *>(lo), (size_t)(hi - lo) / 4, 0.f, s));
         }
         const bool mfma = ctx->precision != ISP_BM3D_PREC_FP32;
         const int k = wiener ? 1 : 0;
