@@ -17,6 +17,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "../../include/isp_pilot.h"
 
 namespace pilot {
@@ -71,7 +73,7 @@ __global__ __launch_bounds__(THREADS) void conv3x3_kernel(const uint16_t *__rest
                                                           const uint16_t *__restrict__ mask,
                                                           const uint16_t *__restrict__ wts,
                                                           const float *__restrict__ bias, uint16_t *__restrict__ out,
-                                                          float *__restrict__ out32, int H, int W, int relu)
+                                                          float *__restrict__ out32, int H, int W, int relu, int N)
 {
     constexpr int KP = kpad(CIN), KC = KP / 32, NT = COUT / 16;
     constexpr int SP = CIN + 8, KS = KP + 8, CQ = CIN / 8;
@@ -79,70 +81,78 @@ __global__ __launch_bounds__(THREADS) void conv3x3_kernel(const uint16_t *__rest
     uint16_t *tin = reinterpret_cast<uint16_t *>(smem);  // [SH*SW][SP]
     uint16_t *tw = tin + SH * SW * SP;                    // [COUT][KS]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH, n = blockIdx.z;
-
-    for (int k = tid; k < SH * SW * CQ; k += THREADS) {
-        const int q = k % CQ, p = k / CQ, sx = p % SW, sy = p / SW;
-        const int yy = y0 + sy - 1, xx = x0 + sx - 1;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
-            const size_t o = (((size_t)n * H + yy) * W + xx) * CIN + 8 * q;
-            v = *reinterpret_cast<const uint4 *>(in + o);
-            if (mask) v = apply_mask(v, *reinterpret_cast<const uint4 *>(mask + o));
-        }
-        *reinterpret_cast<uint4 *>(tin + (size_t)p * SP + 8 * q) = v;
-    }
+    // persistent: the weights are staged once, then the workgroup walks the
+    // tiles blockIdx.x, blockIdx.x + gridDim.x, ... (neighbouring workgroups
+    // on neighbouring tiles)
     for (int k = tid; k < COUT * (KP / 8); k += THREADS) {
         const int row = k / (KP / 8), q = k - row * (KP / 8);
         *reinterpret_cast<uint4 *>(tw + (size_t)row * KS + 8 * q) =
             *reinterpret_cast<const uint4 *>(wts + (size_t)row * KP + 8 * q);
     }
-    __syncthreads();
+    const int ntx = (W + TW - 1) / TW, nty = (H + TH - 1) / TH, per = ntx * nty;
+    for (int t = blockIdx.x; t < per * N; t += gridDim.x) {
+        const int n = t / per, tr = t - n * per, tyi = tr / ntx;
+        const int x0 = (tr - tyi * ntx) * TW, y0 = tyi * TH;
+        __syncthreads();  // the previous tile's reads of tin are done
 
-    const int m = lane & 15, g = lane >> 4;
-    f32x4 acc[MT][NT];
-#pragma unroll
-    for (int a = 0; a < MT; ++a)
-#pragma unroll
-        for (int b = 0; b < NT; ++b) acc[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
+        for (int k = tid; k < SH * SW * CQ; k += THREADS) {
+            const int q = k % CQ, p = k / CQ, sx = p % SW, sy = p / SW;
+            const int yy = y0 + sy - 1, xx = x0 + sx - 1;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
+                const size_t o = (((size_t)n * H + yy) * W + xx) * CIN + 8 * q;
+                v = *reinterpret_cast<const uint4 *>(in + o);
+                if (mask) v = apply_mask(v, *reinterpret_cast<const uint4 *>(mask + o));
+            }
+            *reinterpret_cast<uint4 *>(tin + (size_t)p * SP + 8 * q) = v;
+        }
+        __syncthreads();
 
-    for (int kc = 0; kc < KC; ++kc) {
-        const int k = kc * 32 + 8 * g;
-        int tap = k / CIN;
-        const int ci0 = k - tap * CIN;  // in [0, CIN): the K padding reads tap 8, whose weights are 0
-        if (tap > 8) tap = 8;
-        const int ty = tap / 3, tx = tap - 3 * ty;
-        bf16x8 b[NT];
+        const int m = lane & 15, g = lane >> 4;
+        f32x4 acc[MT][NT];
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt)
-            b[nt] = *reinterpret_cast<const bf16x8 *>(tw + (size_t)(16 * nt + m) * KS + k);
+        for (int a = 0; a < MT; ++a)
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
-            const int r = RPW * wave + (mt >> 1), xs = (mt & 1) * 16 + m;
-            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(tin + ((size_t)(r + ty) * SW + xs + tx) * SP + ci0);
+            for (int b = 0; b < NT; ++b) acc[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
+
+        for (int kc = 0; kc < KC; ++kc) {
+            const int k = kc * 32 + 8 * g;
+            int tap = k / CIN;
+            const int ci0 = k - tap * CIN;  // in [0, CIN): the K padding reads tap 8, whose weights are 0
+            if (tap > 8) tap = 8;
+            const int ty = tap / 3, tx = tap - 3 * ty;
+            bf16x8 b[NT];
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt)
-                acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[nt], acc[mt][nt], 0, 0, 0);
+                b[nt] = *reinterpret_cast<const bf16x8 *>(tw + (size_t)(16 * nt + m) * KS + k);
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                const int r = RPW * wave + (mt >> 1), xs = (mt & 1) * 16 + m;
+                const bf16x8 a = *reinterpret_cast<const bf16x8 *>(tin + ((size_t)(r + ty) * SW + xs + tx) * SP + ci0);
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[nt], acc[mt][nt], 0, 0, 0);
+            }
         }
-    }
-    // D layout: lane holds rows 4g..4g+3 (pixels) of column lane % 16 (channel)
+        // D layout: lane holds rows 4g..4g+3 (pixels) of column lane % 16 (channel)
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt) {
-        const int r = RPW * wave + (mt >> 1), yy = y0 + r;
-        if (yy >= H) continue;
+        for (int mt = 0; mt < MT; ++mt) {
+            const int r = RPW * wave + (mt >> 1), yy = y0 + r;
+            if (yy >= H) continue;
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) {
-            const int co = 16 * nt + m;
-            const float bv = bias ? bias[co] : 0.f;
+            for (int nt = 0; nt < NT; ++nt) {
+                const int co = 16 * nt + m;
+                const float bv = bias ? bias[co] : 0.f;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int xx = x0 + (mt & 1) * 16 + 4 * g + i;
-                if (xx >= W) continue;
-                float v = acc[mt][nt][i] + bv;
-                if (relu) v = fmaxf(v, 0.f);
-                const size_t o = (((size_t)n * H + yy) * W + xx) * COUT + co;
-                out[o] = f2bf(v);
-                if (out32) out32[o] = v;
+                for (int i = 0; i < 4; ++i) {
+                    const int xx = x0 + (mt & 1) * 16 + 4 * g + i;
+                    if (xx >= W) continue;
+                    float v = acc[mt][nt][i] + bv;
+                    if (relu) v = fmaxf(v, 0.f);
+                    const size_t o = (((size_t)n * H + yy) * W + xx) * COUT + co;
+                    out[o] = f2bf(v);
+                    if (out32) out32[o] = v;
+                }
             }
         }
     }
@@ -336,9 +346,14 @@ hipError_t conv_t(const void *in, const void *mask, const void *w, const float *
     auto k = conv3x3_kernel<CIN, COUT>;
     hipError_t e = set_lds(k, lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k, dim3((wd + TW - 1) / TW, (h + TH - 1) / TH, n), dim3(THREADS), lds, s,
-                       (const uint16_t *)in, (const uint16_t *)mask, (const uint16_t *)w, bias, (uint16_t *)out,
-                       out32, h, wd, relu);
+    // two persistent workgroups per CU (the LDS of one is within half a CU's)
+    int dev = 0, cus = 0;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+    const long tiles = (long)((wd + TW - 1) / TW) * ((h + TH - 1) / TH) * n;
+    const int grid = (int)std::min<long>(tiles, 2L * std::max(cus, 1));
+    hipLaunchKernelGGL(k, dim3(grid), dim3(THREADS), lds, s, (const uint16_t *)in, (const uint16_t *)mask,
+                       (const uint16_t *)w, bias, (uint16_t *)out, out32, h, wd, relu, n);
     return hipGetLastError();
 }
 
