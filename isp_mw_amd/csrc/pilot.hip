@@ -158,6 +158,123 @@ __global__ __launch_bounds__(THREADS) void conv3x3_kernel(const uint16_t *__rest
     }
 }
 
+// Inference form of conv3x3_kernel (no ReLU mask, no FP32 copy of the
+// outputs): the next tile's input is loaded into registers while the current
+// tile's products run, and the outputs pass through LDS (bias, ReLU, bf16) so
+// that whole pixel rows go out as 16-byte stores.  The shared area holds the
+// staged input tile, then the output tile: max(SH*SW*SP, TH*TW*COUT) elements.
+template <int CIN, int COUT>
+__host__ __device__ constexpr int fwd_area()
+{
+    return SH * SW * (CIN + 8) > TH * TW * COUT ? SH * SW * (CIN + 8) : TH * TW * COUT;
+}
+
+template <int CIN, int COUT>
+__global__ __launch_bounds__(THREADS) void conv3x3_fwd_kernel(const uint16_t *__restrict__ in,
+                                                              const uint16_t *__restrict__ wts,
+                                                              const float *__restrict__ bias,
+                                                              uint16_t *__restrict__ out, int H, int W, int relu,
+                                                              int N)
+{
+    constexpr int KP = kpad(CIN), KC = KP / 32, NT = COUT / 16;
+    constexpr int SP = CIN + 8, KS = KP + 8, CQ = CIN / 8, CO8 = COUT / 8;
+    constexpr int NIN = SH * SW * CQ, NPF = (NIN + THREADS - 1) / THREADS;
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint16_t *tin = reinterpret_cast<uint16_t *>(smem);  // [SH*SW][SP], then the outputs [TH*TW][COUT]
+    uint16_t *tw = tin + fwd_area<CIN, COUT>();          // [COUT][KS]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int k = tid; k < COUT * (KP / 8); k += THREADS) {
+        const int row = k / (KP / 8), q = k - row * (KP / 8);
+        *reinterpret_cast<uint4 *>(tw + (size_t)row * KS + 8 * q) =
+            *reinterpret_cast<const uint4 *>(wts + (size_t)row * KP + 8 * q);
+    }
+    const int ntx = (W + TW - 1) / TW, nty = (H + TH - 1) / TH, per = ntx * nty, total = per * N;
+
+    // the input tile (with halo, zeros outside the image) of tile t into registers
+    uint4 pf[NPF];
+    auto fetch = [&](int t) {
+        const int n = t / per, tr = t - n * per, tyi = tr / ntx;
+        const int x0 = (tr - tyi * ntx) * TW, y0 = tyi * TH;
+#pragma unroll
+        for (int i = 0; i < NPF; ++i) {
+            const int k = tid + i * THREADS;
+            const int q = k % CQ, p = k / CQ, sx = p % SW, sy = p / SW;
+            const int yy = y0 + sy - 1, xx = x0 + sx - 1;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (k < NIN && yy >= 0 && yy < H && xx >= 0 && xx < W)
+                v = *reinterpret_cast<const uint4 *>(in + (((size_t)n * H + yy) * W + xx) * CIN + 8 * q);
+            pf[i] = v;
+        }
+    };
+    if ((int)blockIdx.x < total) fetch(blockIdx.x);
+
+    const int m = lane & 15, g = lane >> 4;
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+        const int n = t / per, tr = t - n * per, tyi = tr / ntx;
+        const int x0 = (tr - tyi * ntx) * TW, y0 = tyi * TH;
+        __syncthreads();  // the previous tile's output rows are out of the shared area
+#pragma unroll
+        for (int i = 0; i < NPF; ++i) {
+            const int k = tid + i * THREADS;
+            if (k < NIN) *reinterpret_cast<uint4 *>(tin + (size_t)(k / CQ) * SP + 8 * (k % CQ)) = pf[i];
+        }
+        __syncthreads();
+        if (t + (int)gridDim.x < total) fetch(t + gridDim.x);  // in flight during the products
+
+        f32x4 acc[MT][NT];
+#pragma unroll
+        for (int a = 0; a < MT; ++a)
+#pragma unroll
+            for (int b = 0; b < NT; ++b) acc[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
+        for (int kc = 0; kc < KC; ++kc) {
+            const int k = kc * 32 + 8 * g;
+            int tap = k / CIN;
+            const int ci0 = k - tap * CIN;  // in [0, CIN): the K padding reads tap 8, whose weights are 0
+            if (tap > 8) tap = 8;
+            const int ty = tap / 3, tx = tap - 3 * ty;
+            bf16x8 b[NT];
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+                b[nt] = *reinterpret_cast<const bf16x8 *>(tw + (size_t)(16 * nt + m) * KS + k);
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                const int r = RPW * wave + (mt >> 1), xs = (mt & 1) * 16 + m;
+                const bf16x8 a = *reinterpret_cast<const bf16x8 *>(tin + ((size_t)(r + ty) * SW + xs + tx) * SP + ci0);
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[nt], acc[mt][nt], 0, 0, 0);
+            }
+        }
+        __syncthreads();  // every wave is done reading the input tile
+        // D layout: lane holds rows 4g..4g+3 (pixels) of column lane % 16 (channel)
+        uint16_t *tout = tin;  // [TH*TW][COUT]
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            const int r = RPW * wave + (mt >> 1);
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const int co = 16 * nt + m;
+                const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int px = (mt & 1) * 16 + 4 * g + i;
+                    float v = acc[mt][nt][i] + bv;
+                    if (relu) v = fmaxf(v, 0.f);
+                    tout[(size_t)(r * TW + px) * COUT + co] = f2bf(v);
+                }
+            }
+        }
+        __syncthreads();
+        for (int k = tid; k < TH * TW * CO8; k += THREADS) {
+            const int pix = k / CO8, q = k - pix * CO8, r = pix / TW, px = pix - r * TW;
+            const int yy = y0 + r, xx = x0 + px;
+            if (yy < H && xx < W)
+                *reinterpret_cast<uint4 *>(out + (((size_t)n * H + yy) * W + xx) * COUT + 8 * q) =
+                    *reinterpret_cast<const uint4 *>(tout + (size_t)pix * COUT + 8 * q);
+        }
+    }
+}
+
 // Weight and bias gradients: dw[co][ci][tap] = sum_p dz[p][co] * in[p + off(tap)][ci],
 // db[co] = sum_p dz[p][co], with dz = dout * (mask > 0) (mask may be null).
 // A persistent grid walks the pixel tiles; workgroup b writes its sums to
@@ -343,15 +460,23 @@ hipError_t conv_t(const void *in, const void *mask, const void *w, const float *
 {
     constexpr int KP = kpad(CIN);
     const size_t lds = (size_t)SH * SW * (CIN + 8) * 2 + (size_t)COUT * (KP + 8) * 2;
-    auto k = conv3x3_kernel<CIN, COUT>;
-    hipError_t e = set_lds(k, lds);
-    if (e != hipSuccess) return e;
     // two persistent workgroups per CU (the LDS of one is within half a CU's)
+    hipError_t e;
     int dev = 0, cus = 0;
     if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
     if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
     const long tiles = (long)((wd + TW - 1) / TW) * ((h + TH - 1) / TH) * n;
     const int grid = (int)std::min<long>(tiles, 2L * std::max(cus, 1));
+    if (!mask && !out32) {
+        auto k = conv3x3_fwd_kernel<CIN, COUT>;
+        const size_t lf = (size_t)fwd_area<CIN, COUT>() * 2 + (size_t)COUT * (KP + 8) * 2;
+        if ((e = set_lds(k, lf)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k, dim3(grid), dim3(THREADS), lf, s, (const uint16_t *)in, (const uint16_t *)w, bias,
+                           (uint16_t *)out, h, wd, relu, n);
+        return hipGetLastError();
+    }
+    auto k = conv3x3_kernel<CIN, COUT>;
+    if ((e = set_lds(k, lds)) != hipSuccess) return e;
     hipLaunchKernelGGL(k, dim3(grid), dim3(THREADS), lds, s, (const uint16_t *)in, (const uint16_t *)mask,
                        (const uint16_t *)w, bias, (uint16_t *)out, out32, h, wd, relu, n);
     return hipGetLastError();
