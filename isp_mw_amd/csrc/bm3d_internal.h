@@ -84,13 +84,13 @@ struct MatchArgs {
 
 // The column-walk matching kernel takes FP32 distances of the single-frame
 // search with BlockStep 3..7 (neighbouring blocks of a column share rows, and
-// a row touches at most 4 of them) and windows of at most 128 lane strips of
-// kStrip offsets.
+// a row touches at most 4 of them).  Windows beyond 128 lane strips of kStrip
+// offsets are walked in chunks; scan indices must fit 16 bits (at most 255
+// offsets per axis).
 inline bool match_walk_eligible(const MatchArgs &a)
 {
     if (a.exact || a.search || a.bstep < 3 || a.bstep >= 8 || a.step < 1) return false;
-    const int nm = 2 * (a.range / a.step) + 1;
-    return nm * ((nm + 4) / 5) <= 128;
+    return 2 * (a.range / a.step) + 1 <= 255;
 }
 
 // Multi-frame grouping: merge of the per-frame candidate lists of one

@@ -573,6 +573,8 @@ constexpr uint32_t kNoKey = ~0u;
 
 // per wave: hist u32[256], memk u32[64], memi u32[64], selk u32[32], seli u32[32]
 __host__ __device__ constexpr size_t walk_wave_bytes() { return (256 + 64 + 64 + 32 + 32) * 4; }
+// per wave, MULTI only: [try_][2][32] kept pairs and [try_] counts
+__host__ __device__ inline size_t walk_prior_bytes(int try_) { return align16((size_t)try_ * 65 * 4); }
 
 // The K smallest (key, index) of the wave's register keys into selk / seli,
 // unordered; returns how many (min(candidates, K)).  key[r][c] is kNoKey for
@@ -759,8 +761,12 @@ __device__ __forceinline__ uint32_t select_walk(const uint32_t (&key)[RND][STRIP
 
 // 4 waves per SIMD: at most 128 registers
 // RND rounds of 64 lane strips of STRIP offsets: (2, 5) or, where 64 strips
-// of 6 cover the window, (1, 6)
-template <typename GT, bool ST1, bool BATCHED, int kWalkSlots, int RND, int STRIP>
+// of 6 cover the window, (1, 6).  MULTI: a window of more than 128 strips,
+// walked in chunks of 128; after each chunk a block keeps its K best
+// (key, index), in (key, index) order, in LDS, and the next chunk's selection
+// takes them as candidates ahead of its own (earlier chunks hold the smaller
+// scan indices, so the order of ties is kept).
+template <typename GT, bool ST1, bool BATCHED, int kWalkSlots, int RND, int STRIP, bool MULTI>
 __global__ __launch_bounds__(kThreads, 4) void match_walk_kernel(MatchArgs a0, const uint32_t *__restrict__ bmap,
                                                               const MatchArgs *__restrict__ bargs)
 {
@@ -780,6 +786,10 @@ __global__ __launch_bounds__(kThreads, 4) void match_walk_kernel(MatchArgs a0, c
     const size_t tileBytes = align16((size_t)a.tileH * pitch * sizeof(float));
     uint32_t *hist = reinterpret_cast<uint32_t *>(smem + tileBytes + (size_t)wave * walk_wave_bytes());
     uint32_t *memk = hist + 256, *memi = memk + 64, *selk = memi + 64, *seli = selk + 32;
+    // MULTI: each block's kept (key, index) pairs [try_][2][32] and their counts [try_]
+    uint32_t *prior = reinterpret_cast<uint32_t *>(smem + tileBytes + (size_t)kWaves * walk_wave_bytes() +
+                                                   (size_t)wave * walk_prior_bytes(try_));
+    uint32_t *pcnt = prior + (size_t)try_ * 64;
 
     for (int k = tid; k < a.tileH * a.tileW; k += kThreads) {
         const int yy0 = k / a.tileW, xx0 = k - yy0 * a.tileW;
@@ -795,24 +805,28 @@ __global__ __launch_bounds__(kThreads, 4) void match_walk_kernel(MatchArgs a0, c
     // lattice indices: offset (dy, dx) = (-R + my * st, -R + mx * st), my, mx < NM
     const int NM = 2 * R / st + 1, Rs = R / st;
     const int nsx = (NM + STRIP - 1) / STRIP, nstrips = NM * nsx;
-    int dy[RND], dx[RND], my[RND], mx0[RND];
-#pragma unroll
-    for (int r = 0; r < RND; ++r) {
-        const int sidx = r * 64 + lane;
-        const bool sv = sidx < nstrips;
-        // a strip past the lattice gets my = -1: never inside a window
-        my[r] = sv ? sidx / nsx : -1;
-        mx0[r] = sv ? (sidx - my[r] * nsx) * STRIP : 0;
-        dy[r] = sv ? -R + my[r] * st : -R;
-        dx[r] = -R + mx0[r] * st;
-    }
+    const int nchunks = MULTI ? (nstrips + RND * 64 - 1) / (RND * 64) : 1;
     const float thSSE = (float)a.thSSE, distMul = (float)a.distMul;
     const uint32_t K = (uint32_t)(a.G - 1);
     const int nk = min(try_, a.ngy - ty0);  // blocks of each column in this tile
 
-    for (int col = wave; col < trx; col += kWaves) {
+    for (int col = wave; col < trx; col += kWaves)
+    for (int h = 0; h < nchunks; ++h) {
         const int gx = tx0 + col;
         if (gx >= a.ngx || nk <= 0) continue;
+        if (MULTI && a.bypass && h > 0) continue;
+        // this chunk's strips of this lane
+        int dy[RND], dx[RND], my[RND], mx0[RND];
+#pragma unroll
+        for (int r = 0; r < RND; ++r) {
+            const int sidx = (h * RND + r) * 64 + lane;
+            const bool sv = sidx < nstrips;
+            // a strip past the lattice gets my = -1: never inside a window
+            my[r] = sv ? sidx / nsx : -1;
+            mx0[r] = sv ? (sidx - my[r] * nsx) * STRIP : 0;
+            dy[r] = sv ? -R + my[r] * st : -R;
+            dx[r] = -R + mx0[r] * st;
+        }
         const int i = grid_pos(gx, a.bstep, a.W - kBlock);
         if (a.bypass) {
             for (int k = lane; k < nk; k += 64) {
@@ -907,34 +921,55 @@ __global__ __launch_bounds__(kThreads, 4) void match_walk_kernel(MatchArgs a0, c
                 // ---- block k is complete: keys, selection of the K = G-1 smallest, the group ----
                 const int t = search_boundary(j, 0, R, st), b = search_boundary(j, a.H - kBlock, R, st);
                 const int ny = (b - t) / st + 1, m0y = (t - j + R) / st;
-                uint32_t key[RND][STRIP];
-                int ibase[RND];
+                // MULTI: round 0 holds the block's kept pairs from earlier chunks
+                constexpr int R0 = MULTI ? 1 : 0;
+                uint32_t key[RND + R0][STRIP];
+                int ibase[RND + R0];
+                if constexpr (MULTI) {
+                    const uint32_t np = h > 0 ? pcnt[k] : 0u;
+                    key[0][0] = (uint32_t)lane < np ? prior[k * 64 + lane] : kNoKey;
+                    ibase[0] = (uint32_t)lane < np ? (int)prior[k * 64 + 32 + lane] : 0;
+#pragma unroll
+                    for (int c = 1; c < STRIP; ++c) key[0][c] = kNoKey;
+                }
 #pragma unroll
                 for (int r = 0; r < RND; ++r) {
                     const uint32_t vm = (unsigned)(my[r] - m0y) < (unsigned)ny ? vxm[r] : 0u;
-                    ibase[r] = (my[r] - m0y) * nx + mx0[r] - m0x;
+                    ibase[R0 + r] = (my[r] - m0y) * nx + mx0[r] - m0x;
 #pragma unroll
                     for (int c = 0; c < STRIP; ++c) {
                         const float dd = D[r][ph][c];
                         const bool ok = ((vm >> c) & 1u) && dd <= thSSE && dd != 0.f;
-                        key[r][c] = ok ? __float_as_uint(dd * distMul) : kNoKey;
+                        key[R0 + r][c] = ok ? __float_as_uint(dd * distMul) : kNoKey;
                         D[r][ph][c] = 0.f;  // the slot takes block k + kWalkSlots
                     }
                 }
-                const uint32_t nsel = select_walk<RND, STRIP>(key, ibase, K, selk, seli, hist, memk, memi, lane);
+                const uint32_t nsel =
+                    select_walk<RND + R0, STRIP>(key, ibase, K, selk, seli, hist, memk, memi, lane);
 
                 const size_t refIdx = (size_t)(ty0 + k) * a.ngx + gx;
                 const size_t base = refIdx * a.G;
+                // order of the selection: rank by (key, index) as one 48-bit value
+                // (indices are below 2^16: at most 640 offsets per chunk, and
+                // MULTI windows are below 2^16 offsets)
+                const uint64_t vq = (uint32_t)lane < nsel ? ((uint64_t)selk[lane] << 16) | seli[lane] : ~0ull;
+                uint32_t rank = 0;
+                for (uint32_t q = 0; q < nsel; ++q) rank += __shfl(vq, (int)q) < vq;
+                if (MULTI && h + 1 < nchunks) {
+                    // kept for the next chunk, in (key, index) order
+                    if ((uint32_t)lane < nsel) {
+                        prior[k * 64 + rank] = (uint32_t)(vq >> 16);
+                        prior[k * 64 + 32 + rank] = (uint32_t)vq & 0xFFFFu;
+                    }
+                    if (lane == 0) pcnt[k] = nsel;
+                    wave_sync();
+                    continue;
+                }
                 if (lane == 0) {
                     a.cnt_out[refIdx] = (int)(1 + nsel);
                     a.pos_out[base] = (j << 16) | i;
                     if (a.key_out) a.key_out[base] = 0.0;
                 }
-                // order of the selection: rank by (key, index) as one 48-bit value
-                // (indices are below 2^16: at most 640 offsets)
-                const uint64_t vq = (uint32_t)lane < nsel ? ((uint64_t)selk[lane] << 16) | seli[lane] : ~0ull;
-                uint32_t rank = 0;
-                for (uint32_t q = 0; q < nsel; ++q) rank += __shfl(vq, (int)q) < vq;
                 if ((uint32_t)lane < nsel) {
                     const uint32_t iq = (uint32_t)vq & 0xFFFFu;
                     const int cy = (int)(iq / (uint32_t)nx), cx = (int)(iq - (uint32_t)cy * nx);
@@ -2226,19 +2261,26 @@ hipError_t launch_match_t(const MatchArgs &a, hipStream_t s, const BatchLaunch *
         if (a.walk) {
             const dim3 grid((a.ngx + a.trx - 1) / a.trx, (a.ngy + a.try_ - 1) / a.try_);
             const int nm = 2 * (a.range / a.step) + 1;
-            const bool one = nm * ((nm + 5) / 6) <= 64;  // one round of strips of 6
+            const bool one = nm * ((nm + 5) / 6) <= 64;    // one round of strips of 6
+            const bool multi = nm * ((nm + 4) / 5) > 128;  // chunks of 128 strips of 5
             if (a.bstep == 3) {
+                if (multi)
+                    return b ? launch_tiled_m(match_walk_kernel<GT, ST1, true, 4, 2, 5, true>, lds, a, s, b, grid)
+                             : launch_tiled_m(match_walk_kernel<GT, ST1, false, 4, 2, 5, true>, lds, a, s, b, grid);
                 if (one)
-                    return b ? launch_tiled_m(match_walk_kernel<GT, ST1, true, 4, 1, 6>, lds, a, s, b, grid)
-                             : launch_tiled_m(match_walk_kernel<GT, ST1, false, 4, 1, 6>, lds, a, s, b, grid);
-                return b ? launch_tiled_m(match_walk_kernel<GT, ST1, true, 4, 2, 5>, lds, a, s, b, grid)
-                         : launch_tiled_m(match_walk_kernel<GT, ST1, false, 4, 2, 5>, lds, a, s, b, grid);
+                    return b ? launch_tiled_m(match_walk_kernel<GT, ST1, true, 4, 1, 6, false>, lds, a, s, b, grid)
+                             : launch_tiled_m(match_walk_kernel<GT, ST1, false, 4, 1, 6, false>, lds, a, s, b, grid);
+                return b ? launch_tiled_m(match_walk_kernel<GT, ST1, true, 4, 2, 5, false>, lds, a, s, b, grid)
+                         : launch_tiled_m(match_walk_kernel<GT, ST1, false, 4, 2, 5, false>, lds, a, s, b, grid);
             }
+            if (multi)
+                return b ? launch_tiled_m(match_walk_kernel<GT, ST1, true, 3, 2, 5, true>, lds, a, s, b, grid)
+                         : launch_tiled_m(match_walk_kernel<GT, ST1, false, 3, 2, 5, true>, lds, a, s, b, grid);
             if (one)
-                return b ? launch_tiled_m(match_walk_kernel<GT, ST1, true, 3, 1, 6>, lds, a, s, b, grid)
-                         : launch_tiled_m(match_walk_kernel<GT, ST1, false, 3, 1, 6>, lds, a, s, b, grid);
-            return b ? launch_tiled_m(match_walk_kernel<GT, ST1, true, 3, 2, 5>, lds, a, s, b, grid)
-                     : launch_tiled_m(match_walk_kernel<GT, ST1, false, 3, 2, 5>, lds, a, s, b, grid);
+                return b ? launch_tiled_m(match_walk_kernel<GT, ST1, true, 3, 1, 6, false>, lds, a, s, b, grid)
+                         : launch_tiled_m(match_walk_kernel<GT, ST1, false, 3, 1, 6, false>, lds, a, s, b, grid);
+            return b ? launch_tiled_m(match_walk_kernel<GT, ST1, true, 3, 2, 5, false>, lds, a, s, b, grid)
+                     : launch_tiled_m(match_walk_kernel<GT, ST1, false, 3, 2, 5, false>, lds, a, s, b, grid);
         }
     }
     if (b) {
@@ -2267,7 +2309,12 @@ hipError_t launch_match_g(const MatchArgs &a, hipStream_t s, const BatchLaunch *
 // exact (the plain form): FP64 distances and keys; otherwise FP32
 size_t match_lds_bytes(const MatchArgs &a)
 {
-    if (a.walk) return align16((size_t)a.tileH * a.tilePitch * 4) + (size_t)kWaves * walk_wave_bytes();
+    if (a.walk) {
+        const int nm = 2 * (a.range / a.step) + 1;
+        const bool multi = nm * ((nm + 4) / 5) > 128;
+        return align16((size_t)a.tileH * a.tilePitch * 4) +
+               (size_t)kWaves * (walk_wave_bytes() + (multi ? walk_prior_bytes(a.try_) : 0));
+    }
     const int db = a.exact ? 8 : 4;
     const size_t rt = a.search ? align16((size_t)a.refH * a.refPitch * db) : 0;
     return align16((size_t)a.tileH * a.tilePitch * db) + rt + (size_t)kWaves * match_wave_bytes(a.maxCand, db);

@@ -6,7 +6,9 @@ then the row sums in row order), so they must give identical groups: counts,
 positions, order and keys, bit for bit, and therefore identical outputs.  The
 environment variable ISP_BM3D_MATCH_WALK=0 at context creation keeps every
 search on the list kernel.  The walk takes BlockStep 3..7 (smaller
-steps stay on the list kernel, so both runs agree trivially).  Cases cover
+steps stay on the list kernel, so both runs agree trivially); windows beyond
+128 lane strips are walked in chunks, each block keeping its best between
+chunks.  Cases cover
 block steps 1..7, BMsteps 1..3, ranges, group sizes 2..32, grids that end in
 clamped positions, windows clipped on every side, columns of 4 to 32 blocks
 per tile, quantised images full of equal distances, RGB frames, batches,
@@ -59,7 +61,9 @@ CASES = [
     (33, 41, 4, 10, 1, 2, 4),      # K = 1 and 3
     (75, 66, 7, 8, 2, 16, 16),
     (100, 37, 3, 12, 1, 16, 32),   # columns of 30 blocks: two tiles, the second clamped
-    (40, 40, 4, 16, 1, 16, 32),    # 231 strips: both runs on the list kernel
+    (40, 40, 4, 16, 1, 16, 32),    # 231 strips: walked in two chunks
+    (70, 90, 3, 19, 1, 16, 32),    # 312 strips: three chunks
+    (61, 83, 4, 19, 1, 32, 32),
 ]
 
 
@@ -101,11 +105,12 @@ def test_tile_shapes_do_not_change_groups(gpu, monkeypatch, tile):
         assert_same_groups(g0[s], g1[s], f"tile {tile} step {s + 1}")
 
 
+@pytest.mark.parametrize("R", [12, 19])
 @pytest.mark.parametrize("levels", [4, 16, 255])
-def test_equal_distances(gpu, monkeypatch, levels):
+def test_equal_distances(gpu, monkeypatch, levels, R):
     # quantised, noise-free images: long runs of equal keys exercise the
-    # selection's index order on ties
-    p = para(G1=16, G2=32)
+    # selection's index order on ties (at R 19 also across chunks)
+    p = para(R=R, G1=16, G2=32)
     img = quantised(72, 96, levels, levels)
 
     def go():
