@@ -112,6 +112,27 @@ def measured_traffic(prec: str, kernel: str):
     return None
 
 
+UTIL_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                         "bm3d_1080p_rgb_{prec}_utilisation.txt")
+
+
+def measured_util(prec: str, kernel: str):
+    """Pipe utilisation of `kernel` from the committed rocprofv3 SQ/GRBM
+    counter summary: lines `util <stage> <valu_busy> <mfma_busy>
+    <mfma_insts_per_launch> <lds_bank_conflict_per_cu_cycle> <ms>`."""
+    path = UTIL_FILE.format(prec=prec)
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            f = line.split()
+            if len(f) == 7 and f[0] == "util" and f[1] == kernel:
+                return {"valu_busy": float(f[2]), "mfma_busy": float(f[3]), "mfma_insts_per_launch": float(f[4]),
+                        "lds_bank_conflict_cycles_per_cu_cycle": float(f[5]), "ms_under_counters": float(f[6]),
+                        "source": os.path.relpath(path, os.path.dirname(UTIL_FILE))}
+    return None
+
+
 def pipelined_steps(fs, ys, x, steps: int, warmup: int):
     """K frames with len(fs) of them in flight: frame i runs on context
     i % n and its own stream (default priority), without stage events.
@@ -825,10 +846,10 @@ def main() -> int:
                 "engine": "fp32 VALU" if args.precision == "fp32" else f"{args.precision} MFMA 32x32x16",
                 "algorithmic_flops_per_launch": flops,
                 "flops_basis": "dense separable 2*B^2*N*(2B+N) per transform at each group's actual N"}
-        if args.precision != "fp32":
-            # issued matrix work: 12 MFMAs 32x32x16 per transform at any N
-            n_tr = (3 if dom == "filter_final" else 2) * 3 * refs
-            roof["issued_mfma_flops_per_launch"] = n_tr * 12 * 32 * 32 * 16 * 2.0
+        util = measured_util(args.precision, dom)
+        if args.precision != "fp32" and util is not None:
+            # issued matrix work: the counted 32x32x16 MFMAs of one launch
+            roof["issued_mfma_flops_per_launch"] = util["mfma_insts_per_launch"] * 32 * 32 * 16 * 2.0
             roof["issued_frac"] = round(roof["issued_mfma_flops_per_launch"] / (ms * 1e-3) / 1e12 / peak, 4)
     else:
         achieved = match_flops / (ms * 1e-3) / 1e12
@@ -838,6 +859,10 @@ def main() -> int:
                 "engine": "fp64 VALU", "algorithmic_flops_per_launch": match_flops,
                 "flops_basis": "3 flop (sub+FMA) per pixel per candidate, clipped windows"}
 
+    util = measured_util(args.precision, dom)
+    if util is not None:
+        # counter-measured pipe utilisation of the same kernel and form
+        roof["pipe_utilisation"] = util
     tr = measured_traffic(args.precision, dom)
     if tr is not None:
         roof["traffic"] = tr["bytes"]
