@@ -997,7 +997,18 @@ constexpr int ZS = kBlock * kRow;  // 72 floats per block
 
 __device__ __forceinline__ int gidx(int z, int pix) { return z * ZS + (pix >> 3) * kRow + (pix & 7); }
 
-__device__ __forceinline__ void rows_pass(float *g, int N, const float *__restrict__ c8, int lane,
+// Read-only tables seen through the constant address space: the kernel never
+// writes them, so wave-uniform reads become scalar loads (through a generic
+// pointer, the flush's global atomics count as possible writes and every
+// coefficient read stays a vector load per lane).
+typedef const __attribute__((address_space(4))) float cfloat;
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T *kconst(const T *p)
+{
+    return (const __attribute__((address_space(4))) T *)p;
+}
+
+__device__ __forceinline__ void rows_pass(float *g, int N, cfloat *__restrict__ c8, int lane,
                                           bool inverse)
 {
     for (int r = lane; r < N * kBlock; r += 64) {
@@ -1016,7 +1027,7 @@ __device__ __forceinline__ void rows_pass(float *g, int N, const float *__restri
     }
 }
 
-__device__ __forceinline__ void cols_pass(float *g, int N, const float *__restrict__ c8, int lane,
+__device__ __forceinline__ void cols_pass(float *g, int N, cfloat *__restrict__ c8, int lane,
                                           bool inverse)
 {
     for (int r = lane; r < N * kBlock; r += 64) {
@@ -1039,7 +1050,7 @@ __device__ __forceinline__ void cols_pass(float *g, int N, const float *__restri
 // held in registers and each output row is written back as soon as it is
 // formed.  inverse applies the transpose.
 template <int NP>
-__device__ __forceinline__ void zpass(float *g, int N, int lane, const float *__restrict__ cn, bool inverse)
+__device__ __forceinline__ void zpass(float *g, int N, int lane, cfloat *__restrict__ cn, bool inverse)
 {
     float x[NP];
     const int o = gidx(0, lane);
@@ -1107,14 +1118,28 @@ struct ExactCtx {
     size_t fstride;
 };
 
+// Members are read 8 at a time: lane z holds member z's position (and frame
+// slot), each load's address comes from it by readlane, and the 8 loads are
+// in flight together before their LDS stores (one dependent load per member,
+// waited on one at a time, left the plain filter waiting on memory).
+constexpr int kGatherBatch = 8;
+
 __device__ __forceinline__ void gather(float *g, const float *__restrict__ plane, int pitch,
                                        const int *__restrict__ pos, int N, int lane)
 {
     const int u = lane >> 3, v = lane & 7;
     const int o = gidx(0, lane);
-    for (int z = 0; z < N; ++z) {
-        const int pz = pos[z];
-        g[o + z * ZS] = plane[(size_t)((pz >> 16) + u) * pitch + (pz & 0xFFFF) + v];
+    const int pl = lane < N ? pos[lane] : 0;
+    for (int z0 = 0; z0 < N; z0 += kGatherBatch) {
+        float t[kGatherBatch];
+#pragma unroll
+        for (int j = 0; j < kGatherBatch; ++j) {
+            const int pz = __builtin_amdgcn_readlane(pl, min(z0 + j, N - 1));
+            t[j] = plane[(size_t)((pz >> 16) + u) * pitch + (pz & 0xFFFF) + v];
+        }
+#pragma unroll
+        for (int j = 0; j < kGatherBatch; ++j)
+            if (z0 + j < N) g[o + (z0 + j) * ZS] = t[j];
     }
 }
 
@@ -1125,20 +1150,31 @@ __device__ __forceinline__ void gather_slots(float *g, const float *__restrict__
 {
     const int u = lane >> 3, v = lane & 7;
     const int o = gidx(0, lane);
-    for (int z = 0; z < N; ++z) {
-        const int pz = pos[z];
-        g[o + z * ZS] = plane[(size_t)slot[z] * fstride + (size_t)((pz >> 16) + u) * pitch + (pz & 0xFFFF) + v];
+    const int pl = lane < N ? pos[lane] : 0, sl = lane < N ? slot[lane] : 0;
+    for (int z0 = 0; z0 < N; z0 += kGatherBatch) {
+        float t[kGatherBatch];
+#pragma unroll
+        for (int j = 0; j < kGatherBatch; ++j) {
+            const int z = min(z0 + j, N - 1);
+            const int pz = __builtin_amdgcn_readlane(pl, z);
+            const size_t base = (size_t)__builtin_amdgcn_readlane(sl, z) * fstride;
+            t[j] = plane[base + (size_t)((pz >> 16) + u) * pitch + (pz & 0xFFFF) + v];
+        }
+#pragma unroll
+        for (int j = 0; j < kGatherBatch; ++j)
+            if (z0 + j < N) g[o + (z0 + j) * ZS] = t[j];
     }
 }
 
 // Forward transform, shrinkage and inverse of one group of one plane; the
 // filtered blocks are left in g.  Returns the group's aggregation weight.
 template <int NP, bool WIENER, bool SLOTS = false>
-__device__ __forceinline__ float group_core(float *g, int N, int lane, const float *__restrict__ c8,
-                                            const float *__restrict__ cn, const float *__restrict__ src,
+__device__ __forceinline__ float group_core(float *g, int N, int lane, const float *__restrict__ c8g,
+                                            const float *__restrict__ cng, const float *__restrict__ src,
                                             const float *__restrict__ guide, int pitch, float thr, float sig2,
                                             const ExactCtx &ex)
 {
+    cfloat *c8 = kconst(c8g), *cn = kconst(cng);
     const int o = gidx(0, lane);
     float yr[WIENER ? NP : 1];
     if (WIENER) {
@@ -1276,7 +1312,7 @@ __global__ __launch_bounds__(kThreads) void filter_kernel(FilterArgs a0, const u
                                                           const FilterArgs *__restrict__ bargs)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     int tx, ty;
     const FilterArgs &a = tile_unit<BATCHED>(a0, bargs, bmap, tx, ty);
     const int trx = a.trx, tnum = a.trx * a.try_;
@@ -1304,7 +1340,7 @@ __global__ __launch_bounds__(kThreads) void filter_kernel(FilterArgs a0, const u
             if (gxi >= a.ngx || gyi >= a.ngy) continue;
             const size_t refIdx = (size_t)gyi * a.ngx + gxi;
             const int *pos = a.pos + refIdx * a.G;
-            const int N = a.cnt[refIdx];
+            const int N = kconst(a.cnt)[refIdx];
             const float thr = a.thr[c], sig2 = a.sig2[c];
             const ExactCtx ex{pos, src, a.pitch, a.c8d, a.cnd + a.cnd_off[N], a.thr64[c], a.band[c]};
             const float *cn = a.cn + a.cn_off[N];
@@ -1317,15 +1353,13 @@ __global__ __launch_bounds__(kThreads) void filter_kernel(FilterArgs a0, const u
             else w = group_core<32, WIENER>(g, N, lane, a.c8, cn, src, gsrc, a.pitch, thr, sig2, ex);
             const int o = gidx(0, lane);
             const float ws = w * (kFixScale / 4294967296.0f);
+            const int pl = lane < N ? pos[lane] : 0;
             for (int z = 0; z < N; ++z) {
-                const int pz = pos[z];
+                const int pz = __builtin_amdgcn_readlane(pl, z);
                 const int py = (pz >> 16) - oy, px = (pz & 0xFFFF) - ox;
                 fix_add_scaled(&accNum[(py + u) * FW + px + v], g[o + z * ZS], ws);
             }
-            if (lane < N) {
-                const int pz = pos[lane];
-                fix_add(&accDen[((pz >> 16) - oy) * DW + (pz & 0xFFFF) - ox], w);
-            }
+            if (lane < N) fix_add(&accDen[((pl >> 16) - oy) * DW + (pl & 0xFFFF) - ox], w);
             wave_sync();
         }
         __syncthreads();
@@ -1343,7 +1377,7 @@ __global__ __launch_bounds__(kThreads) void filter_stack_kernel(FilterArgs a0, c
                                                                 const FilterArgs *__restrict__ bargs)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     int tx, ty;
     const FilterArgs &a = tile_unit<BATCHED>(a0, bargs, bmap, tx, ty);
     const int trx = a.trx, tnum = a.trx * a.try_;
@@ -1391,7 +1425,7 @@ __global__ __launch_bounds__(kThreads) void filter_stack_kernel(FilterArgs a0, c
                 const size_t refIdx = (size_t)t * refs + (size_t)gyi * a.ngx + gxi;
                 const int *pos = a.pos + refIdx * a.G;
                 const int *slot = STACK ? a.slot + refIdx * a.G : nullptr;
-                const int N = a.cnt[refIdx];
+                const int N = kconst(a.cnt)[refIdx];
                 const float thr = a.thr[c], sig2 = a.sig2[c];
                 const ExactCtx ex{pos, src, a.pitch, a.c8d, a.cnd + a.cnd_off[N], a.thr64[c], a.band[c], slot,
                                   STACK ? a.fstride : 0};
@@ -1406,16 +1440,17 @@ __global__ __launch_bounds__(kThreads) void filter_stack_kernel(FilterArgs a0, c
                 else w = group_core<32, WIENER, STACK>(g, N, lane, a.c8, cn, src, gsrc, a.pitch, thr, sig2, ex);
                 const int o = gidx(0, lane);
                 const float ws = w * (kFixScale / 4294967296.0f);
+                const int pl = lane < N ? pos[lane] : 0;
+                const int sl = STACK && lane < N ? slot[lane] : 0;
                 for (int z = 0; z < N; ++z) {
-                    const int pz = pos[z];
+                    const int pz = __builtin_amdgcn_readlane(pl, z);
                     const int py = (pz >> 16) - oy, px = (pz & 0xFFFF) - ox;
-                    u64 *an = slot_num(STACK ? slot[z] : 0);
+                    u64 *an = slot_num(STACK ? __builtin_amdgcn_readlane(sl, z) : 0);
                     fix_add_scaled(&an[(py + u) * FW + px + v], g[o + z * ZS], ws);
                 }
                 if (lane < N) {
-                    const int pz = pos[lane];
-                    u64 *ad = slot_num(STACK ? slot[lane] : 0) + FW * FH;
-                    fix_add(&ad[((pz >> 16) - oy) * DW + (pz & 0xFFFF) - ox], w);
+                    u64 *ad = slot_num(sl) + FW * FH;
+                    fix_add(&ad[((pl >> 16) - oy) * DW + (pl & 0xFFFF) - ox], w);
                 }
                 wave_sync();
             }
