@@ -45,12 +45,24 @@ __device__ __forceinline__ void acc_add(u64 *acc, double v, double fix)
     atomicAdd(acc, (u64)(long long)floor(v * fix));
 }
 
+// lane l's value of v, l wave-uniform
+__device__ __forceinline__ double readlane_d(double v, int l)
+{
+    const u64 b = __double_as_longlong(v);
+    const u64 lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, l);
+    const u64 hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
+    return __longlong_as_double((long long)(lo | (hi << 32)));
+}
+
+// Members' samples are loaded this many at a time, then summed in member order
+constexpr int kNlmBatch = 8;
+
 template <typename ST>
 __device__ void nlm_tile(const NlmArgs &a, u64 *acc, int tx, int ty)
 {
     // separate mul and add roundings, as the reference's double code
 #pragma clang fp contract(off)
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int trx = a.trx, tnum = a.trx * a.try_;
     const int tx0 = tx * trx, ty0 = ty * a.try_;
     const int lastx = a.W - kBlock, lasty = a.H - kBlock;
@@ -78,22 +90,31 @@ __device__ void nlm_tile(const NlmArgs &a, u64 *acc, int tx, int ty)
         // weights of the group members, lane z holding member z's; their sum
         // in member order
         const double wz = lane < N ? exp(a.keys[refIdx * a.G + lane] * a.exp_mul) : 0.0;
+        const int pl = lane < N ? pos[lane] : 0;
         double wsum = 0.0;
-        for (int z = 0; z < N; ++z) wsum += __shfl(wz, z);
+        for (int z = 0; z < N; ++z) wsum += readlane_d(wz, z);
         const double rec = 1.0 / wsum;
         for (int c = 0; c < np; ++c) {
             const void *src = a.src[c];
             double s1 = 0.0, s2 = 0.0;
-            for (int z = 0; z < N; ++z) {
-                const double w = __shfl(wz, z);
-                const int pz = pos[z];
-                const double x = load<ST>(src, (size_t)((pz >> 16) + u) * a.spitch + (pz & 0xFFFF) + v);
-                if (a.correction) {
-                    const double t = x * w;
-                    s1 += t;
-                    s2 += x * t;
-                } else {
-                    s1 += x * w;
+            for (int z0 = 0; z0 < N; z0 += kNlmBatch) {
+                double xs[kNlmBatch];
+#pragma unroll
+                for (int j = 0; j < kNlmBatch; ++j) {
+                    const int pz = __builtin_amdgcn_readlane(pl, min(z0 + j, N - 1));
+                    xs[j] = load<ST>(src, (size_t)((pz >> 16) + u) * a.spitch + (pz & 0xFFFF) + v);
+                }
+#pragma unroll
+                for (int j = 0; j < kNlmBatch; ++j) {
+                    if (z0 + j >= N) break;
+                    const double w = readlane_d(wz, z0 + j), x = xs[j];
+                    if (a.correction) {
+                        const double t = x * w;
+                        s1 += t;
+                        s2 += x * t;
+                    } else {
+                        s1 += x * w;
+                    }
                 }
             }
             double out;
