@@ -17,6 +17,7 @@
 //                      fused with OPP->RGB and clip (include/Conversion.hpp:739-762).
 //   opp_kernel       - RGB->OPP without clip (include/Conversion.hpp:547-578).
 #include "bm3d_internal.h"
+#include "tile_order.h"
 
 namespace ispbm3d {
 namespace {
@@ -49,8 +50,7 @@ __device__ __forceinline__ const A &tile_unit(const A &a0, const A *__restrict__
         ty = (int)((u >> 10) & 1023u);
         return bargs[u >> kBatchImageShift];
     } else {
-        tx = (int)blockIdx.x;
-        ty = (int)blockIdx.y;
+        xcd_tile(tx, ty);
         return a0;
     }
 }

@@ -17,6 +17,7 @@
 // and add roundings, so outputs equal the reference's up to the last bits of
 // exp() and of the aggregation (2^-52 relative to the fixed-point scale).
 #include "bm3d_internal.h"
+#include "tile_order.h"
 
 namespace ispbm3d {
 namespace {
@@ -146,7 +147,9 @@ template <typename ST>
 __global__ __launch_bounds__(kNlmThreads) void nlm_kernel(NlmArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    nlm_tile<ST>(a, reinterpret_cast<u64 *>(smem), (int)blockIdx.x, (int)blockIdx.y);
+    int tx, ty;
+    xcd_tile(tx, ty);
+    nlm_tile<ST>(a, reinterpret_cast<u64 *>(smem), tx, ty);
 }
 
 // Number of grid positions min(k*s, last) in [x - 7, x]: the multiples of s
