@@ -1992,7 +1992,7 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_stack_kernel(FilterArgs
 {
     typedef typename MfmaT<T>::v8 v8;
     extern __shared__ __align__(16) unsigned char smem[];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int c = lane & 31, h = lane >> 5;
     int tx, ty;
     const FilterArgs &a = tile_unit<BATCHED>(a0, bargs, bmap, tx, ty);
@@ -2065,7 +2065,7 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_stack_kernel(FilterArgs
                 const int *pos = a.pos + refIdx * a.G;
                 const int *slot = STACK ? a.slot + refIdx * a.G : nullptr;
                 const size_t sstride = STACK ? a.fstride : 0;
-                const int N = a.cnt[refIdx];
+                const int N = kconst(a.cnt)[refIdx];
                 v8 cna[2], cnb[2];
 #pragma unroll
                 for (int m = 0; m < 2; ++m) {
