@@ -62,6 +62,23 @@ def test_grey_stack_against_oracle(gpu, radius):
     print(f"PSNR radius 0 {synth.psnr(r0, cl):.3f} dB", flush=True)
 
 
+def test_largest_radius_against_oracle(gpu):
+    # radius 4: nine lists per reference block, up to 9 x 31 = 279 candidates
+    # in the merge (more than one per lane of a wave), GroupSize 32 in both steps
+    p = para(R=8, G1=32, G2=32)
+    cl, g = grey_stack(9, 32, 40, 25.0, 10)
+    for prec in ("fp32", "fp16"):
+        got = BM3D(p, precision=prec).process_stack(np.empty_like(g), g, radius=4)
+        if prec == "fp32":
+            want = oracle.bm3d_stack(p.basic, p.final, g.astype(np.float64), 4)
+            check(got, want, cl, "grey stack radius 4")
+            ref32 = got
+        else:
+            d = synth.psnr(ref32, cl) - synth.psnr(got, cl)
+            print(f"fp16 stack radius 4: {d:.4f} dB below fp32", flush=True)
+            assert d <= 0.05
+
+
 def test_rgb_stack_against_oracle(gpu):
     p = para(R=8)
     cl, c = rgb_stack(3, 40, 56, 25.0, 6)
