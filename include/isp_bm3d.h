@@ -38,8 +38,13 @@ extern "C" {
 #define ISP_BM3D_MEM_HOST 0
 #define ISP_BM3D_MEM_DEVICE 1
 
-/* Arithmetic of the transforms and shrinkage.  Matching is always exact
- * (FP64 distances) and aggregation always FP32. */
+/* Arithmetic of the transforms and shrinkage, and of the matching distances.
+ * PREC_FP32: FP32 transforms and shrinkage (hard-threshold decisions within
+ * a narrow band of the threshold re-made in FP64), FP64 distances summed in
+ * the reference's order (its tie order exactly).  PREC_FP16 / PREC_BF16:
+ * 16-bit MFMA operands with FP32 accumulation, FP32 shrinkage, FP32
+ * distances.  Every form aggregates in 64-bit fixed point (exact,
+ * order-independent sums; see isp_bm3d_aggregation_limits). */
 #define ISP_BM3D_PREC_FP32 0 /* plain FP32 path                              */
 #define ISP_BM3D_PREC_FP16 1 /* FP16 MFMA transforms, FP32 accumulation      */
 #define ISP_BM3D_PREC_BF16 2 /* BF16 MFMA transforms, FP32 accumulation      */
@@ -173,6 +178,18 @@ int isp_bm3d_last_groups(isp_bm3d_ctx *ctx, int final_step, int *counts, int *po
 /* Grid sizes of a step for an image (source/BM3D.cpp:98-127). */
 int isp_bm3d_grid_size(const isp_bm3d_ctx *ctx, int width, int height, int final_step,
                        int *ngx, int *ngy);
+
+/* Headroom of the aggregation sums (no reference counterpart: the reference
+ * sums in FP64).  num and den are 64-bit fixed point with 36 fraction bits.
+ * For a step's parameters and a temporal radius (0 for single frames):
+ * *max_contributions = an upper bound on the (group, member) contributions
+ * to one pixel; *weight_cap = the cap applied to Wiener group weights
+ * 1 / sum(W^2), the largest power of two <= 2^16 with
+ * max_contributions * weight_cap * 4 < 2^26, so sums of samples and
+ * reconstructions of magnitude <= 4 cannot overflow.  Returns
+ * ISP_BM3D_E_UNSUPPORTED (weight_cap 0) when even weights <= 1 could. */
+int isp_bm3d_aggregation_limits(const isp_bm3d_step_params *p, int radius, double *max_contributions,
+                                double *weight_cap);
 
 /* OPP colour conversions of float device planes (MatrixConvert_RGB2YUV /
  * MatrixConvert_YUV2RGB with ColorMatrix::OPP, include/Conversion.hpp:547-578,

@@ -33,6 +33,7 @@ EXPORTS = (
     "isp_bm3d_stack_f32",
     "isp_bm3d_match",
     "isp_bm3d_grid_size",
+    "isp_bm3d_aggregation_limits",
     "isp_bm3d_set_timing",
     "isp_bm3d_get_timing",
     "isp_bm3d_reset_timing",
@@ -168,6 +169,8 @@ def lib() -> ctypes.CDLL:
     L.isp_bm3d_stack_f32.argtypes = [vp, vp, vp, vp, i, i, i, i, sz, i, i, i, vp]
     L.isp_bm3d_match.argtypes = [vp, vp, i, i, sz, i, i, vp, vp, vp]
     L.isp_bm3d_grid_size.argtypes = [vp, i, i, i, ctypes.POINTER(i), ctypes.POINTER(i)]
+    L.isp_bm3d_aggregation_limits.argtypes = [sp, i, ctypes.POINTER(ctypes.c_double),
+                                              ctypes.POINTER(ctypes.c_double)]
     L.isp_bm3d_set_timing.argtypes = [vp, i]
     L.isp_bm3d_get_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_long), i]

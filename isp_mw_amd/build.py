@@ -6,6 +6,7 @@ written in-tree to ``isp_mw_amd/lib/``.
 """
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -15,6 +16,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libisp_bm3d.so")
+OBJDIR = os.path.join(LIBDIR, "_obj")  # per-source objects (git-ignored *.o)
 CLI = os.path.join(LIBDIR, "isp_mw")
 CLI_SRC = os.path.join(HERE, "cli", "isp_mw.cpp")
 
@@ -42,15 +44,32 @@ def hipcc() -> str:
 
 
 def build(verbose: bool = False) -> str:
-    os.makedirs(LIBDIR, exist_ok=True)
+    """Compiles every source to an object in parallel, then links the library."""
+    os.makedirs(OBJDIR, exist_ok=True)
+    compile_flags = [f for f in HIPCC_FLAGS if f != "-shared"]
+
+    def compile_one(src: str) -> str:
+        obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
+        cmd = [hipcc(), *compile_flags, "-c", "-o", obj, os.path.join(CSRC, src)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError(f"hipcc failed compiling {src}")
+        return obj
+
+    jobs = max(1, min(len(SOURCES), int(os.environ.get("MAX_JOBS", "0") or 0) or (os.cpu_count() or 1)))
+    with concurrent.futures.ThreadPoolExecutor(jobs) as ex:
+        objs = list(ex.map(compile_one, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-o", tmp, *[os.path.join(CSRC, s) for s in SOURCES]]
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building libisp_bm3d.so")
+        raise RuntimeError("hipcc failed linking libisp_bm3d.so")
     os.replace(tmp, LIB)
     build_cli(verbose)
     return LIB

@@ -51,6 +51,10 @@
 
 namespace {
 
+// largest image side a file may declare (the library's own limits are
+// checked by the library: width < 65536, height < 32768)
+constexpr uint32_t kMaxSide = 65535;
+
 struct Image {
     int w = 0, h = 0;
     int channels = 3;         // after loading: always 3 (R, G, B)
@@ -101,8 +105,11 @@ Image read_png(const std::vector<unsigned char> &d)
         if (p + 12 + (size_t)len > d.size()) die("truncated PNG chunk " + type);
         const unsigned char *c = &d[p + 8];
         if (type == "IHDR") {
-            w = (int)be32(c);
-            h = (int)be32(c + 4);
+            if (len != 13) die("PNG IHDR chunk of " + std::to_string(len) + " bytes (13 expected)");
+            const uint32_t uw = be32(c), uh = be32(c + 4);
+            if (uw == 0 || uh == 0 || uw > kMaxSide || uh > kMaxSide) die("PNG dimensions out of range");
+            w = (int)uw;
+            h = (int)uh;
             bd = c[8];
             ct = c[9];
             il = c[12];
@@ -256,9 +263,21 @@ Header pnm_header(const std::vector<unsigned char> &d, int fields)
     }
     if ((int)toks.size() < fields) die("truncated image header");
     hd.magic = toks[0];
-    hd.w = std::stoi(toks[1]);
-    hd.h = std::stoi(toks[2]);
-    hd.maxval = std::stod(toks[3]);
+    auto side = [](const std::string &t) {
+        // digits only, at most kMaxSide: no sign, no overflow in w * h
+        if (t.empty() || t.size() > 6 || t.find_first_not_of("0123456789") != std::string::npos)
+            die("bad image dimension '" + t + "'");
+        const long v = std::stol(t);
+        if (v < 1 || v > (long)kMaxSide) die("image dimension " + t + " out of range");
+        return (int)v;
+    };
+    hd.w = side(toks[1]);
+    hd.h = side(toks[2]);
+    try {
+        hd.maxval = std::stod(toks[3]);
+    } catch (const std::exception &) {
+        die("bad maxval '" + toks[3] + "'");
+    }
     hd.offset = p;  // one whitespace byte after the last field was consumed
     return hd;
 }

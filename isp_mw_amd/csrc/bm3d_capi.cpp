@@ -176,6 +176,7 @@ struct Derived {
     float thr[3], sig2[3];
     double thr64[3];
     float band[3];
+    float wmax;  // cap of the Wiener group weight (aggregation headroom)
     int active[3];
     int ngx, ngy;
 };
@@ -199,9 +200,16 @@ int derive(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, int nplanes, int W,
         return fail(ctx, ISP_BM3D_E_ARG, "BlockStep and BMstep must be positive");
     if (W < kBlock || H < kBlock)
         return fail(ctx, ISP_BM3D_E_SIZE, "image %dx%d is smaller than one %dx%d block", W, H, kBlock, kBlock);
-    if (W > 65535 || H > 65535)
-        return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "image %dx%d: width and height must be < 65536", W, H);
+    // positions are packed (y << 16) | x in an int32 and decoded with an
+    // arithmetic shift: y must stay below 2^15
+    if (W > 65535 || H > 32767)
+        return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "image %dx%d: width must be < 65536 and height < 32768", W, H);
     d.range = p.bm_range / p.bm_step * p.bm_step;
+    d.wmax = wiener_weight_cap(kBlock, p.block_step, d.range, p.bm_step, p.group_size, 0);
+    if (!(d.wmax > 0))
+        return fail(ctx, ISP_BM3D_E_UNSUPPORTED,
+                    "BMrange %d / BlockStep %d / GroupSize %d: a pixel's aggregation sums could overflow",
+                    p.bm_range, p.block_step, p.group_size);
     d.bypass = p.group_size == 1 || thMSE <= 0;
     // include/Block.h:582-584 with range 1 (Plane_FL default Floor 0 / Ceil 1)
     const double MSE2SSE = (double)(kBlock * kBlock) * 1.0 * 1.0 / (double)(255 * 255);
@@ -481,6 +489,7 @@ FilterArgs build_filter(const isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, 
         f.thr64[c] = d.thr64[c];
         f.band[c] = d.band[c];
     }
+    f.wmax = d.wmax;
     f.FW = (trx - 1) * p.block_step + 2 * d.range + kBlock;
     f.FH = (try_ - 1) * p.block_step + 2 * d.range + kBlock;
     f.c8 = ctx->d_tables;
@@ -1538,6 +1547,18 @@ int isp_bm3d_last_groups(isp_bm3d_ctx *ctx, int final_step, int *counts, int *po
     return (int)refs;
 }
 
+int isp_bm3d_aggregation_limits(const isp_bm3d_step_params *p, int radius, double *max_contributions,
+                                double *weight_cap)
+{
+    if (!p || !max_contributions || !weight_cap || radius < 0 || p->block_step <= 0 || p->bm_step <= 0 ||
+        p->bm_range < 0 || p->group_size < 1)
+        return ISP_BM3D_E_ARG;
+    const int range = p->bm_range / p->bm_step * p->bm_step;
+    *max_contributions = aggregation_bound(kBlock, p->block_step, range, p->bm_step, p->group_size, radius);
+    *weight_cap = wiener_weight_cap(kBlock, p->block_step, range, p->bm_step, p->group_size, radius);
+    return *weight_cap > 0 ? ISP_BM3D_OK : ISP_BM3D_E_UNSUPPORTED;
+}
+
 int isp_bm3d_grid_size(const isp_bm3d_ctx *ctx, int W, int H, int final_step, int *ngx, int *ngy)
 {
     if (!ctx || !ngx || !ngy) return ISP_BM3D_E_ARG;
@@ -2266,8 +2287,9 @@ int nlm_common(isp_bm3d_ctx *ctx, const isp_nlm_params *pp, int nplanes, void *c
         return fail(ctx, ISP_BM3D_E_ARG, "BlockStep and BMstep must be positive");
     if (W < kBlock || H < kBlock)
         return fail(ctx, ISP_BM3D_E_SIZE, "image %dx%d is smaller than one %dx%d block", W, H, kBlock, kBlock);
-    if (W > 65535 || H > 65535)
-        return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "image %dx%d: width and height must be < 65536", W, H);
+    // positions are packed (y << 16) | x in an int32: y must stay below 2^15
+    if (W > 65535 || H > 32767)
+        return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "image %dx%d: width must be < 65536 and height < 32768", W, H);
 
     // matching: BlockMatchingMulti with the unscaled thMSE (source/NLMeans.cpp:203, 332)
     isp_bm3d_step_params mp{};
@@ -2661,6 +2683,8 @@ int stack_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, co
             f.slot = w.slot;
             f.frames = T;
             f.radius = io.radius;
+            // groups of 2 * radius + 1 reference frames reach each frame
+            f.wmax = wiener_weight_cap(kBlock, p.block_step, d.range, p.bm_step, p.group_size, io.radius);
             f.nslots = std::min(2 * io.radius + 1, T);
             f.fstride = fs;
             if (filter_stack_lds_bytes(f, wiener, ctx->precision) <= 160 * 1024) break;
@@ -2669,6 +2693,9 @@ int stack_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, co
             if (trx >= try_ && trx > 1) trx = (trx + 1) / 2;
             else try_ = (try_ + 1) / 2;
         }
+        if (!(f.wmax > 0))
+            return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "radius %d: a pixel's aggregation sums could overflow",
+                        io.radius);
         StageTimer tm(ctx, wiener ? ST_FILTER_FINAL : ST_FILTER_BASIC, s);
         HIP_CHECK(ctx, launch_filter_stack(f, wiener, ctx->precision, s));
     }

@@ -41,17 +41,51 @@ constexpr int kMaxBatchImages = 4096;
 constexpr int kMaxBatchTiles = 1024;  // tiles per image per axis
 constexpr int kMaxBatchRows = 1 << 20;
 
-// Fixed-point scale of the aggregation sums: value * 2^36 rounded to int64.
-// A pixel collects at most 64 block origins x (2R/s+1)^2 groups of weight
-// <= 1 times |sample| <= 2, below 2^19 for every supported setting, so sums
-// stay below 2^55.  Wiener weights 1 / sum(W^2) exceed 1 only where every
-// gain of the group is small; they are clamped to kMaxWienerWeight, which
-// keeps each addend's integer part (|w * x| * 2^4, |x| <= 2) inside int32.
-// The clamp acts only where sum(W^2) < 2^-16, groups whose guide carries
-// almost no energy and whose estimate is near zero.
+// Fixed-point scale of the aggregation sums: value * 2^36, two's complement
+// in 64 bits, so a pixel's sums must stay below 2^63 / 2^36 = 2^27 in value.
+// A pixel p is covered by block origins o with o <= p < o + B on each axis;
+// o can be a member of the group of every reference block whose window holds
+// it, once per group.  Per axis the references whose windows reach
+// [p - B + 1, p] lie in a span of 2R + B positions, so at most
+// floor((2R + B - 1) / s) + 2 grid points (the +2: both ends, and the grid's
+// last position clamped to n - B), and a group holds at most min(G, B^2) of
+// the origins covering p (fewer at BMstep > 1).  That gives
+// aggregation_bound().  Hard-threshold weights are <= 1; Wiener weights
+// 1 / sum(W^2) exceed 1 only where every
+// gain of the group is small, and are capped at wiener_weight_cap(): the
+// largest power of two (at most 2^16) for which bound * cap * kSampleBound
+// stays below 2^26, a factor 2 inside the 2^27 limit.  With samples of
+// magnitude <= kSampleBound, num and den then cannot overflow.  The cap acts
+// only on groups whose guide carries almost no energy (sum(W^2) < 1 / cap),
+// whose estimates are near zero.
 constexpr float kFixScale = 68719476736.0f;        // 2^36
 constexpr double kFixInv = 1.0 / 68719476736.0;    // 2^-36
-constexpr float kMaxWienerWeight = 65536.0f;       // 2^16
+constexpr double kSampleBound = 4.0;               // |samples| and |reconstructions| the sums allow
+constexpr float kMaxWienerWeight = 65536.0f;       // 2^16, the cap's upper end
+
+// Largest number of (group, member) contributions to one pixel's sums for a
+// step with BlockSize B, BlockStep s, BMrange R (rounded to BMstep st),
+// GroupSize G, and 2 * radius + 1 frames whose groups reach a frame
+// (multi-frame groups).  A window's members lie on a lattice of step st, so at
+// most ceil(B / st) of them per axis cover a pixel.
+inline double aggregation_bound(int B, int s, int R, int st, int G, int radius)
+{
+    const double axis = (double)((2 * R + B - 1) / s + 2);
+    const int per_axis = (B + st - 1) / st;
+    const int members = per_axis * per_axis;
+    return axis * axis * (double)(G < members ? G : members) * (double)(2 * radius + 1);
+}
+
+// Cap of the Wiener group weight (a power of two in [1, 2^16]); 0 when even
+// weights <= 1 could overflow the sums.
+inline float wiener_weight_cap(int B, int s, int R, int st, int G, int radius)
+{
+    const double bound = aggregation_bound(B, s, R, st, G, radius);
+    if (bound * kSampleBound >= 67108864.0) return 0.f;  // 2^26
+    float cap = kMaxWienerWeight;
+    while (cap > 1.f && (double)cap * bound * kSampleBound >= 67108864.0) cap *= 0.5f;
+    return cap;
+}
 
 // Positions are packed (y << 16) | x.
 struct MatchArgs {
@@ -126,6 +160,7 @@ struct FilterArgs {
     int active[3];            // plane filtered?
     float thr[3];             // lambda * sigma_c (orthonormal domain)
     float sig2[3];            // sigma_c^2 (orthonormal domain)
+    float wmax;               // cap of the Wiener group weight (wiener_weight_cap)
     int FW, FH;               // accumulation footprint of a tile
     const float *c8;          // 8x8 orthonormal DCT-II matrix, [k][n]
     const float *cn;          // per-N orthonormal DCT-II matrices, padded to NP x NP

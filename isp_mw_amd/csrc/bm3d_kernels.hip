@@ -1114,6 +1114,7 @@ struct ExactCtx {
     const double *cnd;  // N x N for this group
     double thr64;
     float band;
+    float wmax;         // cap of the Wiener group weight (FilterArgs::wmax)
     const int *slot;    // multi-frame groups: member frame slots (else null)
     size_t fstride;
 };
@@ -1237,7 +1238,7 @@ __device__ __forceinline__ float group_core(float *g, int N, int lane, const flo
             }
         }
         l2 = wave_sum(l2);
-        weight = l2 <= 0.f ? 1.f : fminf(1.f / l2, kMaxWienerWeight);
+        weight = l2 <= 0.f ? 1.f : fminf(1.f / l2, ex.wmax);
     }
     zpass<NP>(g, N, lane, cn, true);
     wave_sync();
@@ -1342,7 +1343,7 @@ __global__ __launch_bounds__(kThreads) void filter_kernel(FilterArgs a0, const u
             const int *pos = a.pos + refIdx * a.G;
             const int N = kconst(a.cnt)[refIdx];
             const float thr = a.thr[c], sig2 = a.sig2[c];
-            const ExactCtx ex{pos, src, a.pitch, a.c8d, a.cnd + a.cnd_off[N], a.thr64[c], a.band[c]};
+            const ExactCtx ex{pos, src, a.pitch, a.c8d, a.cnd + a.cnd_off[N], a.thr64[c], a.band[c], a.wmax};
             const float *cn = a.cn + a.cn_off[N];
             float w;
             if (N <= 1) w = group_core<1, WIENER>(g, N, lane, a.c8, cn, src, gsrc, a.pitch, thr, sig2, ex);
@@ -1427,7 +1428,7 @@ __global__ __launch_bounds__(kThreads) void filter_stack_kernel(FilterArgs a0, c
                 const int *slot = STACK ? a.slot + refIdx * a.G : nullptr;
                 const int N = kconst(a.cnt)[refIdx];
                 const float thr = a.thr[c], sig2 = a.sig2[c];
-                const ExactCtx ex{pos, src, a.pitch, a.c8d, a.cnd + a.cnd_off[N], a.thr64[c], a.band[c], slot,
+                const ExactCtx ex{pos, src, a.pitch, a.c8d, a.cnd + a.cnd_off[N], a.thr64[c], a.band[c], a.wmax, slot,
                                   STACK ? a.fstride : 0};
                 const float *cn = a.cn + a.cn_off[N];
                 float w;
@@ -1843,7 +1844,7 @@ __global__ __launch_bounds__(kMThreads, 4) void filter_mfma_kernel(FilterArgs a0
                 f32x16 xh[2];
                 mfma_inverse_rest<T>(vt, ka, lane, xh);
                 const float l2 = g ? l21 : l20;
-                const float w = l2 <= 0.f ? 1.f : fminf(1.f / l2, kMaxWienerWeight);
+                const float w = l2 <= 0.f ? 1.f : fminf(1.f / l2, a.wmax);
                 const float ws = w * (kFixScale / 4294967296.0f);
                 if (pz >= 0) {
                     const int py = (pz >> 16) - oy, px = (pz & 0xFFFF) - ox + 4 * h;
@@ -1953,11 +1954,11 @@ __global__ __launch_bounds__(kMThreads, 4) void filter_mfma_kernel(FilterArgs a0
                         vt[q] = mfma_inverse_half<T>(yq, cnb);
                     }
                     l20 = wave_sum(l20);
-                    w0 = l20 <= 0.f ? 1.f : fminf(1.f / l20, kMaxWienerWeight);
+                    w0 = l20 <= 0.f ? 1.f : fminf(1.f / l20, a.wmax);
                     w1 = 1.f;
                     if (paired) {
                         l21 = wave_sum(l21);
-                        w1 = l21 <= 0.f ? 1.f : fminf(1.f / l21, kMaxWienerWeight);
+                        w1 = l21 <= 0.f ? 1.f : fminf(1.f / l21, a.wmax);
                     }
                     mfma_inverse_rest<T>(vt, ka, lane, xh);
                 }
@@ -2117,7 +2118,7 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_stack_kernel(FilterArgs
                         vt[q] = mfma_inverse_half<T>(yq, cnb);
                     }
                     l2 = wave_sum(l2 + l2b);
-                    w = l2 <= 0.f ? 1.f : fminf(1.f / l2, kMaxWienerWeight);
+                    w = l2 <= 0.f ? 1.f : fminf(1.f / l2, a.wmax);
                     mfma_inverse_rest<T>(vt, ka, lane, xh);
                 }
                 const float ws = w * (kFixScale / 4294967296.0f);

@@ -156,6 +156,29 @@ def test_pnm_and_pfm_round_trips(tmp_path):
     np.testing.assert_array_equal(np.frombuffer(data[len(hdr):], "<f4").reshape(9, 11, 3)[::-1], f)
 
 
+def test_bad_headers_rejected(tmp_path):
+    # a short IHDR chunk, and PNM / PFM dimensions that are negative, zero or
+    # too large: the reader stops with an error instead of reading past the data
+    import struct
+    import zlib
+
+    def chunk(t, data):
+        return struct.pack(">I", len(data)) + t + data + struct.pack(">I", zlib.crc32(t + data) & 0xFFFFFFFF)
+    png = b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">II", 4, 4) + b"\x08\x02\x00") + chunk(b"IEND", b"")
+    bad = [("short.png", png),
+           ("huge.png", b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", 1 << 31, 4, 8, 2, 0, 0, 0))),
+           ("neg.ppm", b"P6\n-4 4\n255\n" + bytes(48)),
+           ("zero.pgm", b"P5\n0 4\n255\n"),
+           ("wrap.ppm", b"P6\n4294967297 1\n255\n" + bytes(3)),
+           ("big.pfm", b"PF\n100000 100000\n-1.0\n" + bytes(12))]
+    for name, data in bad:
+        f = tmp_path / name
+        f.write_bytes(data)
+        r = run("--bm3d", f, "--copy", check=False)
+        assert r.returncode != 0, name
+        assert "Traceback" not in r.stderr
+
+
 # ---------------------------------------------------------------- GPU
 def noisy_png(tmp_path, h=48, w=64, seed=3):
     from isp_mw_amd import synth
@@ -193,11 +216,24 @@ def test_cli_against_oracle_and_guide(gpu, tmp_path):
     d = np.abs(got - want)
     print(f"cli vs oracle: max {d.max()} levels, {np.mean(d > 0):.2e} of samples differ", flush=True)
     assert d.max() <= 1 and np.mean(d > 0) < 1e-3
-    # a guide frame (--ref) changes step 1's matching only
+    # a guide frame (--ref) other than the input: it steers step 1's matching
+    # (its luma), so the output follows the oracle run with the same guide
+    from isp_mw_amd import synth
+    clean = synth.rgb_pair(48, 64, 25.0, 4)[0]
+    ga = np.clip(np.rint(np.transpose(clean, (1, 2, 0)) * 255), 0, 255).astype(np.uint8)
     g = tmp_path / "guide.png"
-    PIL.fromarray(np.asarray(PIL.open(p)), "RGB").save(g)
+    PIL.fromarray(ga, "RGB").save(g)
     out2 = run("--bm3d", p, "-P", "np", "-S", "25", "--ref", g, "-T", ".g").stdout.strip()
-    np.testing.assert_array_equal(np.asarray(PIL.open(out2)), got.astype(np.uint8))
+    got2 = np.asarray(PIL.open(out2), np.int64)
+    guide16 = np.transpose(ga, (2, 0, 1)).astype(np.int32) * 257
+    ref16g = oracle.bm3d_rgb_int(para.basic, para.final, frame, ref=guide16, mode=3, floor=0, ceil=65535)
+    want2 = np.transpose((ref16g.astype(np.int64) + 128) // 257, (1, 2, 0))
+    d2 = np.abs(got2 - want2)
+    changed = float(np.mean(got2 != got))
+    print(f"cli --ref vs oracle: max {d2.max()} levels, {np.mean(d2 > 0):.2e} differ; "
+          f"{changed:.3f} of samples changed by the guide", flush=True)
+    assert d2.max() <= 1 and np.mean(d2 > 0) < 1e-3
+    assert changed > 0.05  # an ignored guide would leave the output as it was
 
 
 @pytest.mark.gpu

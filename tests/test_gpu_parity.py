@@ -190,20 +190,30 @@ def test_config0_512_grey(gpu):
 
 
 # ---------------------------------------------------------------- edges
+WIENER_CAP_CASES = {
+    "np s3 R12 G32": dict(),
+    "s1 R19 G32": dict(step=1, R=19, G2=32),   # the most contributions per pixel: weight cap 128
+    "high": dict(profile="high", step=None, R=None),
+}
+
+
+@pytest.mark.parametrize("case", list(WIENER_CAP_CASES))
 @pytest.mark.parametrize("prec", ["fp32", "fp16"])
-def test_wiener_guide_without_energy(gpu, prec):
+def test_wiener_guide_without_energy(gpu, prec, case):
     # a Wiener guide of 1e-6 * clean: every gain is ~1e-10 and the group
-    # weight 1 / sum(W^2) ~ 1e17, beyond the fixed-point addend's integer
-    # range; weights are clamped (2^16), so the output stays the near-zero
-    # estimate the FP64 oracle gives
+    # weight 1 / sum(W^2) ~ 1e17, beyond what the 64-bit fixed-point sums
+    # hold; every weight is capped (isp_bm3d_aggregation_limits), so every
+    # pixel's sums reach their largest value without overflowing, and the
+    # output stays the near-zero estimate the FP64 oracle gives
+    print(f"tiny Wiener guide {prec} {case}", flush=True)
     clean, n = synth.grey_pair(48, 64, 25.0, 21)
-    p = para()
+    p = para(**WIENER_CAP_CASES[case])
     guide = (clean * 1e-6).astype(np.float32)
     got = BM3D_Final(p.final, precision=prec).process(np.empty_like(n), n, guide)
     want = oracle.kernel(p.final, n, guide, wiener=True)
     print(f"{prec}: max |want| = {np.abs(want).max():.3g}", flush=True)
     assert np.all(np.isfinite(got))
-    assert_close(got, want, label=f"tiny Wiener guide {prec}")
+    assert_close(got, want, label=f"tiny Wiener guide {prec} {case}")
 
 
 def test_degenerate_copy_through(gpu):
@@ -228,6 +238,10 @@ def test_errors(gpu):
     with pytest.raises(L.BM3DError) as e:
         BM3D(p)(np.zeros((6, 20), np.float32))
     assert e.value.code == L.E_SIZE
+    # positions pack the row in 15 bits: a 16 x 40000 plane is refused, not misread
+    with pytest.raises(L.BM3DError) as e:
+        BM3D(para())(np.zeros((40000, 16), np.float32))
+    assert e.value.code == L.E_UNSUPPORTED
     p = para("vn", step=None, R=None)
     with pytest.raises(L.BM3DError) as e:
         BM3D(p)(np.zeros((32, 32), np.float32))  # final BlockSize 11

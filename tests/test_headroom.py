@@ -1,0 +1,105 @@
+"""Headroom of the 64-bit fixed-point aggregation sums (CPU only).
+
+The library sums num and den as value * 2^36 in 64-bit two's complement, so a
+pixel's sums must stay below 2^27.  isp_bm3d_aggregation_limits() states an
+upper bound on the (group, member) contributions to one pixel and the cap it
+puts on Wiener group weights.  Here the bound is checked against a brute-force
+count over every reference block's clipped search window (the oracle's grid
+and _SearchBoundary, include/Block.h:716-751), and the worst case -- every
+weight at the cap, every sample at the allowed magnitude -- is summed exactly
+in Python integers.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+from isp_mw_amd import _lib as L
+
+SAMPLE_BOUND = 4  # |samples| and |reconstructions| the sums allow
+
+
+def limits(block_step, bm_range, group_size, bm_step=1, radius=0):
+    p = L.StepParams()
+    p.block_size, p.block_step, p.group_size, p.bm_range, p.bm_step = 8, block_step, group_size, bm_range, bm_step
+    bound, cap = ctypes.c_double(), ctypes.c_double()
+    rc = L.lib().isp_bm3d_aggregation_limits(ctypes.byref(p), radius, ctypes.byref(bound), ctypes.byref(cap))
+    return rc, bound.value, cap.value
+
+
+def axis_cover(n, s, R, st):
+    """cover[k, q]: members of grid point k's window (one axis) whose block covers q."""
+    g = oracle.grid(n, 8, s)
+    Rr = R // st * st
+    cov = np.zeros((len(g), n), np.int64)
+    for k, r in enumerate(g):
+        lo = oracle.search_boundary(int(r), 0, Rr, st)
+        hi = oracle.search_boundary(int(r), n - 8, Rr, st)
+        for o in range(lo, hi + 1, st):
+            cov[k, o:o + 8] += 1
+    return cov
+
+
+def brute_max(W, H, s, R, G, st=1):
+    """max over pixels of sum over reference blocks of min(G, members covering the pixel)."""
+    cy, cx = axis_cover(H, s, R, st), axis_cover(W, s, R, st)
+    best = 0
+    for q in range(H):
+        m = np.minimum(cy[:, q][:, None, None] * cx[None, :, :], G)  # [gy, gx, W]
+        best = max(best, int(m.sum(axis=(0, 1)).max()))
+    return best
+
+
+CASES = [
+    # (BlockStep, BMrange, GroupSize, BMstep, W, H)
+    (3, 12, 32, 1, 72, 70),    # the benchmark's final step
+    (3, 12, 16, 1, 72, 70),    # its basic step
+    (1, 19, 32, 1, 60, 58),    # the worst case the north_star's forms reach: BlockStep 1, BMrange 19
+    (2, 16, 32, 1, 64, 66),    # profile high, final step
+    (8, 9, 16, 1, 50, 41),     # profile fast
+    (3, 10, 32, 3, 61, 64),    # BMstep 3
+    (5, 24, 16, 3, 90, 80),    # NL-means-like window
+]
+
+
+@pytest.mark.parametrize("s,R,G,st,W,H", CASES)
+def test_bound_covers_brute_force(s, R, G, st, W, H):
+    rc, bound, cap = limits(s, R, G, st)
+    assert rc == L.OK
+    got = brute_max(W, H, s, R, G, st)
+    print(f"s={s} R={R} G={G} st={st}: brute-force max {got}, bound {bound:.0f}, weight cap {cap:g}")
+    assert got <= bound
+    assert got >= 0.3 * bound  # the bound is not vacuous on an image larger than the windows
+
+
+@pytest.mark.parametrize("s,R,G,st,W,H", CASES)
+def test_worst_case_sums_fit(s, R, G, st, W, H):
+    rc, bound, cap = limits(s, R, G, st)
+    assert rc == L.OK
+    assert cap >= 1 and cap <= 65536 and float(cap).is_integer() and (int(cap) & (int(cap) - 1)) == 0
+    # every contribution at the cap with the largest allowed sample, exactly
+    worst = int(bound) * int(cap) * SAMPLE_BOUND * (1 << 36)
+    assert worst < (1 << 62)  # a factor 2 inside the signed 64-bit range
+    # the next power of two would not fit the 2^26 margin (the cap is the largest one)
+    if cap < 65536:
+        assert int(bound) * int(cap) * 2 * SAMPLE_BOUND >= (1 << 26)
+    # each addend's integer part (|w * x| * 2^4) fits int32
+    assert cap * SAMPLE_BOUND * 16 < 2 ** 31
+
+
+def test_expected_caps():
+    # the benchmark's final step: 12^2 reference blocks x 32 members
+    assert limits(3, 12, 32)[1:] == (4608.0, 2048.0)
+    # BlockStep 1, BMrange 19, G 32: 47^2 x 32 contributions
+    assert limits(1, 19, 32)[1:] == (70688.0, 128.0)
+    # three frames of multi-frame groups reach each frame
+    assert limits(3, 12, 32, radius=1)[1:] == (3 * 4608.0, 1024.0)
+
+
+def test_overflowing_parameters_are_rejected():
+    # BlockStep 1 with a 300-pixel window: even weights <= 1 could overflow
+    rc, bound, cap = limits(1, 300, 64)
+    assert rc == L.E_UNSUPPORTED and cap == 0
+    assert bound * SAMPLE_BOUND >= 2 ** 26
+    assert limits(0, 12, 32)[0] == L.E_ARG
