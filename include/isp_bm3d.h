@@ -106,6 +106,24 @@ int isp_bm3d_set_precision(isp_bm3d_ctx *ctx, int precision);
 int isp_bm3d_plane_f32(isp_bm3d_ctx *ctx, float *dst, const float *src, const float *ref,
                        int width, int height, size_t stride, int mode, int mem, void *stream);
 
+/* isp_bm3d_plane_f32 on device buffers that allocates nothing.  No
+ * reference counterpart: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The call's workspace (the step-1 estimate, one plane of
+ * fixed-point sums, the group lists) lives in a device array of 96 MiB per
+ * GPU that the library holds from context creation until the process ends;
+ * no hipMalloc, hipFree or other allocating routine runs in the call.  A plane
+ * whose workspace does not fit (about 2 Mpixel at GroupSize 32 and
+ * BlockStep 3, e.g. 1920x1080 at stride 1920) returns ISP_BM3D_E_UNSUPPORTED.
+ * dst, src and ref (or NULL) are device pointers with row stride `stride`;
+ * the call is stream-ordered on `stream` and stage timing is off during it.
+ * Every call on one GPU shares that GPU's array, which keeps the last call's
+ * contents: calls that overlap on one GPU (other streams, other threads,
+ * other contexts) race and must be serialised by the caller.  The groups it
+ * forms are not kept for isp_bm3d_last_groups. */
+int isp_bm3d_plane_f32_noalloc(isp_bm3d_ctx *ctx, float *dst, const float *src, const float *ref,
+                               int width, int height, size_t stride, int mode, void *stream);
+
 /* Float RGB frame (planar R, G, B in [0,1]) with the semantics of
  * process_Frame (source/BM3D.cpp:282-313 for BASIC / FINAL, :501-527 for
  * BOTH): OPP conversion without clip, matching on Y, OPP inverse with clip to

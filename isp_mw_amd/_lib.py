@@ -27,6 +27,7 @@ EXPORTS = (
     "isp_bm3d_last_error",
     "isp_bm3d_set_precision",
     "isp_bm3d_plane_f32",
+    "isp_bm3d_plane_f32_noalloc",
     "isp_bm3d_rgb_f32",
     "isp_bm3d_plane_i32",
     "isp_bm3d_rgb_i32",
@@ -163,6 +164,7 @@ def lib() -> ctypes.CDLL:
     L.isp_bm3d_last_error.restype = ctypes.c_char_p
     L.isp_bm3d_set_precision.argtypes = [vp, i]
     L.isp_bm3d_plane_f32.argtypes = [vp, vp, vp, vp, i, i, sz, i, i, vp]
+    L.isp_bm3d_plane_f32_noalloc.argtypes = [vp, vp, vp, vp, i, i, sz, i, vp]
     L.isp_bm3d_rgb_f32.argtypes = [vp] + [vp] * 9 + [i, i, sz, i, i, vp]
     L.isp_bm3d_plane_i32.argtypes = [vp, vp, vp, vp, i, i, sz, i, i, i, i, i, vp]
     L.isp_bm3d_rgb_i32.argtypes = [vp] + [vp] * 9 + [i, i, sz, i, i, i, i, vp]

@@ -20,7 +20,7 @@ OBJDIR = os.path.join(LIBDIR, "_obj")  # per-source objects (git-ignored *.o)
 CLI = os.path.join(LIBDIR, "isp_mw")
 CLI_SRC = os.path.join(HERE, "cli", "isp_mw.cpp")
 
-SOURCES = ["bm3d_kernels.hip", "nlm_kernels.hip", "stack_kernels.hip", "bm3d_capi.cpp", "pilot.hip"]
+SOURCES = ["bm3d_kernels.hip", "nlm_kernels.hip", "stack_kernels.hip", "bm3d_capi.cpp", "pilot.hip", "scratch.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

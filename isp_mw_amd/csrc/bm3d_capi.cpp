@@ -37,6 +37,9 @@ struct isp_bm3d_ctx {
     uint16_t *d_mfma[2] = {nullptr, nullptr};  // MFMA operand tables, [0] FP16, [1] BF16
     void *ws = nullptr;
     size_t ws_bytes = 0;
+    // this device's copy of the allocation-free entry point's scratch array
+    // (scratch.hip), looked up at creation; null if the lookup failed
+    void *scratch = nullptr;
     char *pin = nullptr;  // page-locked staging for host-memory calls
     size_t pin_bytes = 0;
     // tile shapes (reference blocks per workgroup) of the single-image launches
@@ -1457,6 +1460,9 @@ int isp_bm3d_create(isp_bm3d_ctx **out, const isp_bm3d_params *params, int devic
         delete ctx;
         return rc;
     }
+    // the scratch array's address on this device (the device is current):
+    // the allocation-free entry point then needs no runtime lookup
+    if (scratch_address(device, &ctx->scratch) != hipSuccess) ctx->scratch = nullptr;
     *out = ctx;
     return ISP_BM3D_OK;
 }
@@ -1735,6 +1741,87 @@ int plane_common(isp_bm3d_ctx *ctx, void *dst, const void *src, const void *ref,
     return ISP_BM3D_OK;
 }
 
+// Work of one float device plane carved from the scratch array: the step-1
+// estimate, one plane of num / den (adjacent, cleared by one fill), and one
+// group list shared by the two steps (step 2 matches after step 1's filter
+// has read its groups, in stream order).  Returns the bytes needed.
+size_t noalloc_layout(Work *w, void *base, int P, int H, size_t refs, int G)
+{
+    Carver c(base);
+    const size_t np = (size_t)P * H;
+    w->basic[0] = c.take<float>(np);
+    w->num[0] = c.take<unsigned long long>(np);
+    w->den[0] = c.take<unsigned long long>(np);
+    for (int k = 1; k < 3; ++k) {
+        // inactive planes: never read or written with one plane; the fill
+        // of run_step clears [num[0], den[2] + P * H) = num[0] and den[0]
+        w->basic[k] = w->basic[0];
+        w->num[k] = w->num[0];
+        w->den[k] = w->den[0];
+    }
+    w->cnt[0] = w->cnt[1] = c.take<int>(refs);
+    w->pos[0] = w->pos[1] = c.take<int>(refs * (size_t)G);
+    return c.off;
+}
+
+int plane_noalloc(isp_bm3d_ctx *ctx, float *dst, const float *src, const float *ref, int W, int H,
+                  size_t stride, int mode, void *stream)
+{
+    int rc = check_common(ctx, W, H, stride, mode, ISP_BM3D_MEM_DEVICE);
+    if (rc) return rc;
+    if (!dst || !src) return fail(ctx, ISP_BM3D_E_ARG, "null dst/src");
+    if (!ctx->scratch)
+        return fail(ctx, ISP_BM3D_E_HIP, "the scratch array's address was not found on device %d", ctx->device);
+    if (W < kBlock || H < kBlock)
+        return fail(ctx, ISP_BM3D_E_SIZE, "image %dx%d is smaller than one %dx%d block", W, H, kBlock, kBlock);
+    const int P = (int)stride;
+    size_t refs = 1;
+    int G = 1;
+    for (const isp_bm3d_step_params *p : {&ctx->params.basic, &ctx->params.final_}) {
+        if (p->block_step <= 0) continue;
+        const size_t ngx = (size_t)(W - kBlock) / p->block_step + 2, ngy = (size_t)(H - kBlock) / p->block_step + 2;
+        refs = std::max(refs, ngx * ngy);
+        G = std::max(G, std::min(std::max(p->group_size, 1), kMaxGroup));
+    }
+    Work w{};
+    const size_t need = noalloc_layout(&w, nullptr, P, H, refs, G);
+    if (need > kScratchBytes)
+        return fail(ctx, ISP_BM3D_E_UNSUPPORTED,
+                    "a %dx%d plane (stride %zu) needs %zu bytes of scratch; the allocation-free entry point "
+                    "has %zu",
+                    W, H, stride, need, kScratchBytes);
+    noalloc_layout(&w, ctx->scratch, P, H, refs, G);
+    // stage timers would create events: off for this call
+    struct TimingOff {
+        isp_bm3d_ctx *c;
+        bool was;
+        ~TimingOff() { c->timing = was; }
+    } off{ctx, ctx->timing};
+    ctx->timing = false;
+    hipStream_t s = (hipStream_t)stream;
+    const float *fref = ref ? ref : src;
+    StepIO io{};
+    io.nplanes = 1;
+    io.pitch = P;
+    io.pitch64 = P;
+    io.src[0] = src;
+    if (mode == ISP_BM3D_BASIC || mode == ISP_BM3D_FINAL) {
+        const bool fin = mode == ISP_BM3D_FINAL;
+        io.guide[0] = fref;
+        io.mguide32 = fref;
+        set_plain_out(io, dst, stride);
+        return run_step(ctx, fin ? ctx->params.final_ : ctx->params.basic, fin, io, w, W, H, s);
+    }
+    io.mguide32 = fref;
+    io.guide[0] = fref;
+    set_plain_out(io, w.basic[0], (size_t)P);
+    if ((rc = run_step(ctx, ctx->params.basic, false, io, w, W, H, s))) return rc;
+    io.mguide32 = w.basic[0];
+    io.guide[0] = w.basic[0];
+    set_plain_out(io, dst, stride);
+    return run_step(ctx, ctx->params.final_, true, io, w, W, H, s);
+}
+
 // BM3D_Base::process_Frame's wrapper test for a single step (source/BM3D.cpp:284-290)
 bool frame_degenerate(const isp_bm3d_ctx *ctx, int mode)
 {
@@ -1908,6 +1995,12 @@ int isp_bm3d_plane_f32(isp_bm3d_ctx *ctx, float *dst, const float *src, const fl
 {
     const Fmt f{false, 0, 0, 1};
     return plane_common(ctx, dst, src, ref, W, H, stride, f, mode, mem, stream);
+}
+
+int isp_bm3d_plane_f32_noalloc(isp_bm3d_ctx *ctx, float *dst, const float *src, const float *ref, int W, int H,
+                               size_t stride, int mode, void *stream)
+{
+    return plane_noalloc(ctx, dst, src, ref, W, H, stride, mode, stream);
 }
 
 int isp_bm3d_plane_i32(isp_bm3d_ctx *ctx, int *dst, const int *src, const int *ref, int W, int H, size_t stride,

@@ -87,6 +87,13 @@ inline float wiener_weight_cap(int B, int s, int R, int st, int G, int radius)
     return cap;
 }
 
+// Device scratch of the allocation-free entry point (scratch.hip): a
+// file-scope device array of kScratchBytes per GPU.  scratch_address() gives
+// the array's address on `device`, which must be the current device; the
+// runtime is asked once per device.
+constexpr size_t kScratchBytes = size_t(96) << 20;  // 96 MiB
+hipError_t scratch_address(int device, void **out);
+
 // Positions are packed (y << 16) | x.
 struct MatchArgs {
     const void *guide;       // matching guide plane (float or double by launch)
