@@ -20,7 +20,8 @@ OBJDIR = os.path.join(LIBDIR, "_obj")  # per-source objects (git-ignored *.o)
 CLI = os.path.join(LIBDIR, "isp_mw")
 CLI_SRC = os.path.join(HERE, "cli", "isp_mw.cpp")
 
-SOURCES = ["bm3d_kernels.hip", "nlm_kernels.hip", "stack_kernels.hip", "bm3d_capi.cpp", "pilot.hip", "scratch.hip"]
+SOURCES = ["bm3d_kernels.hip", "nlm_kernels.hip", "stack_kernels.hip", "bm3d_capi.cpp", "pilot.hip", "scratch.hip",
+           "generic_kernels.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -63,7 +64,7 @@ def build(verbose: bool = False) -> str:
     with concurrent.futures.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(compile_one, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-Wl,--no-undefined", "-o", tmp, *objs]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)

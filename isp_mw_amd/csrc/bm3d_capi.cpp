@@ -35,6 +35,9 @@ struct isp_bm3d_ctx {
     double *d_tables64 = nullptr;  // C8 (64) then the unpadded C_N matrices, FP64
     int cnd_off[kMaxGroup + 1] = {0};
     uint16_t *d_mfma[2] = {nullptr, nullptr};  // MFMA operand tables, [0] FP16, [1] BF16
+    // generic kernels (BlockSize != 8 or GroupSize > 32), per step [basic, final]:
+    // the B x B orthonormal DCT-II, then C_N for N = 1..G, FP64
+    double *d_gen[2] = {nullptr, nullptr};
     void *ws = nullptr;
     size_t ws_bytes = 0;
     // this device's copy of the allocation-free entry point's scratch array
@@ -177,11 +180,14 @@ struct Derived {
     bool bypass;
     double thSSE, distMul;
     float thr[3], sig2[3];
-    double thr64[3];
+    double thr64[3], sig2_64[3];
     float band[3];
     float wmax;  // cap of the Wiener group weight (aggregation headroom)
     int active[3];
     int ngx, ngy;
+    int B;         // BlockSize
+    bool generic;  // BlockSize != 8 or GroupSize > 32: the generic kernels
+    const double *gen_tab;  // their transform tables (GenFilterArgs::cb, then cn)
 };
 
 int derive(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, int nplanes, int W, int H, Derived &d)
@@ -194,28 +200,44 @@ int derive(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, int nplanes, int W,
     d.degenerate = nosigma || p.group_size == 0 || p.block_size <= 0 || p.bm_range <= 0 ||
                    p.bm_range < p.bm_step || thMSE <= 0;
     if (d.degenerate) return ISP_BM3D_OK;
-    if (p.block_size != kBlock)
-        return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "BlockSize %d: the GPU kernels implement BlockSize 8",
-                    p.block_size);
-    if (p.group_size < 0 || p.group_size > kMaxGroup)
-        return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "GroupSize %d outside 1..%d", p.group_size, kMaxGroup);
+    const int B = p.block_size;
+    d.B = B;
+    // BlockSize 8 with GroupSize <= 32 runs the tuned kernels; anything else
+    // within the generic kernels' bounds runs those
+    d.generic = B != kBlock || p.group_size > kMaxGroup;
+    if (B > kGenMaxBlock)
+        return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "BlockSize %d outside 1..%d", B, kGenMaxBlock);
+    if (p.group_size < 0 || p.group_size > kGenMaxGroup)
+        return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "GroupSize %d outside 1..%d", p.group_size, kGenMaxGroup);
+    if (d.generic && p.group_size * B * B > kGenMaxElems)
+        return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "GroupSize %d x BlockSize %d^2 above %d samples per group",
+                    p.group_size, B, kGenMaxElems);
+    d.gen_tab = nullptr;
+    for (int k = 0; d.generic && k < 2; ++k) {
+        // the tables built at creation for a step of this BlockSize hold C_N up to its GroupSize
+        const isp_bm3d_step_params &q = k ? ctx->params.final_ : ctx->params.basic;
+        if (ctx->d_gen[k] && q.block_size == B && p.group_size <= q.group_size) d.gen_tab = ctx->d_gen[k];
+    }
+    if (d.generic && !d.gen_tab)
+        return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "no transform tables for BlockSize %d / GroupSize %d", B,
+                    p.group_size);
     if (p.block_step <= 0 || p.bm_step <= 0)
         return fail(ctx, ISP_BM3D_E_ARG, "BlockStep and BMstep must be positive");
-    if (W < kBlock || H < kBlock)
-        return fail(ctx, ISP_BM3D_E_SIZE, "image %dx%d is smaller than one %dx%d block", W, H, kBlock, kBlock);
+    if (W < B || H < B)
+        return fail(ctx, ISP_BM3D_E_SIZE, "image %dx%d is smaller than one %dx%d block", W, H, B, B);
     // positions are packed (y << 16) | x in an int32 and decoded with an
     // arithmetic shift: y must stay below 2^15
     if (W > 65535 || H > 32767)
         return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "image %dx%d: width must be < 65536 and height < 32768", W, H);
     d.range = p.bm_range / p.bm_step * p.bm_step;
-    d.wmax = wiener_weight_cap(kBlock, p.block_step, d.range, p.bm_step, p.group_size, 0);
+    d.wmax = wiener_weight_cap(B, p.block_step, d.range, p.bm_step, p.group_size, 0);
     if (!(d.wmax > 0))
         return fail(ctx, ISP_BM3D_E_UNSUPPORTED,
                     "BMrange %d / BlockStep %d / GroupSize %d: a pixel's aggregation sums could overflow",
                     p.bm_range, p.block_step, p.group_size);
     d.bypass = p.group_size == 1 || thMSE <= 0;
     // include/Block.h:582-584 with range 1 (Plane_FL default Floor 0 / Ceil 1)
-    const double MSE2SSE = (double)(kBlock * kBlock) * 1.0 * 1.0 / (double)(255 * 255);
+    const double MSE2SSE = (double)(B * B) * 1.0 * 1.0 / (double)(255 * 255);
     d.distMul = 1.0 / MSE2SSE;
     d.thSSE = thMSE * MSE2SSE;
     for (int c = 0; c < 3; ++c) {
@@ -223,14 +245,34 @@ int derive(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, int nplanes, int W,
         const double sigma = p.sigma[c] / 255.0 * norm[c];
         d.thr[c] = (float)(sigma * p.lambda);
         d.sig2[c] = (float)(sigma * sigma);
+        d.sig2_64[c] = sigma * sigma;
         d.thr64[c] = sigma * p.lambda;
         // FP32 coefficients of [0,1] data carry absolute errors of a few 1e-6;
         // decisions closer than this to the threshold are re-made in FP64.
         d.band[c] = (float)(3e-5 + 1e-4 * d.thr64[c]);
     }
-    d.ngx = (W - kBlock) / p.block_step + 1 + ((W - kBlock) % p.block_step ? 1 : 0);
-    d.ngy = (H - kBlock) / p.block_step + 1 + ((H - kBlock) % p.block_step ? 1 : 0);
+    d.ngx = (W - B) / p.block_step + 1 + ((W - B) % p.block_step ? 1 : 0);
+    d.ngy = (H - B) / p.block_step + 1 + ((H - B) % p.block_step ? 1 : 0);
     return ISP_BM3D_OK;
+}
+
+// Largest GroupSize of the context's two steps (the group lists' row length
+// in the workspace), at least kMaxGroup.
+int max_group(const isp_bm3d_ctx *ctx)
+{
+    int G = kMaxGroup;
+    for (const isp_bm3d_step_params *p : {&ctx->params.basic, &ctx->params.final_})
+        G = std::max(G, std::min(p->group_size, kGenMaxGroup));
+    return G;
+}
+
+// Smallest BlockSize of the two steps (the grid is densest for it).
+int min_block(const isp_bm3d_ctx *ctx)
+{
+    int B = kBlock;
+    for (const isp_bm3d_step_params *p : {&ctx->params.basic, &ctx->params.final_})
+        if (p->block_size > 0) B = std::min(B, p->block_size);
+    return B;
 }
 
 // Workspace carving: a bump allocator over one device allocation.
@@ -333,7 +375,7 @@ struct Work {
     double *keys;
 };
 
-size_t layout(Work *w, void *base, int P, int H, size_t refs, bool keys)
+size_t layout(Work *w, void *base, int P, int H, size_t refs, int G, bool keys)
 {
     Carver c(base);
     const size_t np = (size_t)P * H;
@@ -347,9 +389,9 @@ size_t layout(Work *w, void *base, int P, int H, size_t refs, bool keys)
     w->ref64 = c.take<double>(np);
     for (int k = 0; k < 2; ++k) {
         w->cnt[k] = c.take<int>(refs);
-        w->pos[k] = c.take<int>(refs * kMaxGroup);
+        w->pos[k] = c.take<int>(refs * (size_t)G);
     }
-    w->keys = keys ? c.take<double>(refs * kMaxGroup) : nullptr;
+    w->keys = keys ? c.take<double>(refs * (size_t)G) : nullptr;
     return c.off + 256;
 }
 
@@ -363,10 +405,11 @@ int prepare(isp_bm3d_ctx *ctx, Work &w, int P, int H, int W, bool keys)
         const size_t gx = (size_t)(W / s + 2), gy = (size_t)(H / s + 2);
         refs = std::max(refs, gx * gy);
     }
-    const size_t bytes = layout(&w, nullptr, P, H, refs, keys);
+    const int G = max_group(ctx);
+    const size_t bytes = layout(&w, nullptr, P, H, refs, G, keys);
     int rc = ensure_ws(ctx, bytes);
     if (rc) return rc;
-    layout(&w, ctx->ws, P, H, refs, keys);
+    layout(&w, ctx->ws, P, H, refs, G, keys);
     return ISP_BM3D_OK;
 }
 
@@ -556,11 +599,48 @@ FinalizeArgs build_finalize(int W, int H, const StepIO &io, const Derived &d, un
     return fa;
 }
 
+// Matching arguments of the generic kernel: any BlockSize, FP64 distances in
+// the reference's order whatever the context's precision; the tile shrinks
+// until its LDS fits a CU.
+int build_match_generic(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, const Derived &d, int W, int H,
+                        const void *guide, int gpitch, int *cnt, int *pos, double *keys, MatchArgs &a)
+{
+    int trx = kTileRX, try_ = kTileRY;
+    for (;;) {
+        a = build_match(ctx, p, d, W, H, guide, gpitch, trx, try_, cnt, pos, keys);
+        a.B = d.B;
+        a.exact = 1;
+        a.walk = 0;
+        a.tileW = (trx - 1) * p.block_step + 2 * d.range + d.B + (kStrip - 1) * p.bm_step;
+        a.tileH = (try_ - 1) * p.block_step + 2 * d.range + d.B;
+        a.tilePitch = a.tileW | 1;
+        if (match_generic_lds_bytes(a) <= 160 * 1024) return ISP_BM3D_OK;
+        if (trx == 1 && try_ == 1)
+            return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "BMrange %d / BlockSize %d need %zu bytes of LDS for matching",
+                        p.bm_range, d.B, match_generic_lds_bytes(a));
+        if (trx >= try_ && trx > 1) trx = (trx + 1) / 2;
+        else try_ = (try_ + 1) / 2;
+    }
+}
+
 int run_match(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, const Derived &d, Work &w,
               const float *g32, const double *g64, int gpitch, int W, int H, double *keys,
               hipStream_t s, int stage)
 {
     const int k = stage == ST_MATCH_FINAL ? 1 : 0;
+    if (d.generic) {
+        MatchArgs a{};
+        int rc = build_match_generic(ctx, p, d, W, H, g64 ? (const void *)g64 : (const void *)g32, gpitch,
+                                     w.cnt[k], w.pos[k], keys, a);
+        if (rc) return rc;
+        ctx->last_cnt[k] = w.cnt[k];
+        ctx->last_pos[k] = w.pos[k];
+        ctx->last_refs[k] = (size_t)d.ngx * d.ngy;
+        ctx->last_G[k] = p.group_size;
+        StageTimer tm(ctx, stage, s);
+        HIP_CHECK(ctx, launch_match_generic(a, g64 != nullptr, s));
+        return ISP_BM3D_OK;
+    }
     const isp_bm3d_ctx::Tiles t = tiles_for(ctx, W, H);
     const MatchArgs a = build_match(ctx, p, d, W, H, g64 ? (const void *)g64 : (const void *)g32, gpitch,
                                     t.match_x, t.match_y, w.cnt[k], w.pos[k], keys);
@@ -595,16 +675,44 @@ int run_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, cons
             char *hi = reinterpret_cast<char *>(w.den[2] + (size_t)io.pitch * H);
             HIP_CHECK(ctx, launch_fill(reinterpret_cast<float *>(lo), (size_t)(hi - lo) / 4, 0.f, s));
         }
-        const bool mfma = ctx->precision != ISP_BM3D_PREC_FP32;
         const int k = wiener ? 1 : 0;
-        const isp_bm3d_ctx::Tiles t = tiles_for(ctx, W, H);
-        const FilterArgs f = build_filter(ctx, p, d, W, H, io.pitch, io.src, io.guide, w.num, w.den, w.cnt[k],
-                                          w.pos[k], mfma ? t.mfma_x : t.filter_x, mfma ? t.mfma_y : t.filter_y);
-        const size_t lds = filter_lds(ctx, f, wiener);
-        if (lds > 160 * 1024)
-            return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "parameters need %zu bytes of LDS for filtering", lds);
-        StageTimer tm(ctx, wiener ? ST_FILTER_FINAL : ST_FILTER_BASIC, s);
-        HIP_CHECK(ctx, launch_filter_any(ctx, f, wiener, s));
+        if (d.generic) {
+            GenFilterArgs g{};
+            g.W = W;
+            g.H = H;
+            g.pitch = io.pitch;
+            g.ngx = d.ngx;
+            g.ngy = d.ngy;
+            g.B = d.B;
+            g.G = p.group_size;
+            g.cnt = w.cnt[k];
+            g.pos = w.pos[k];
+            for (int c = 0; c < 3; ++c) {
+                g.src[c] = io.src[c];
+                g.guide[c] = io.guide[c];
+                g.num[c] = w.num[c];
+                g.den[c] = w.den[c];
+                g.active[c] = d.active[c];
+                g.thr[c] = d.thr64[c];
+                g.sig2[c] = d.sig2_64[c];
+            }
+            g.wmax = d.wmax;
+            g.cb = d.gen_tab;
+            g.cn = d.gen_tab + (size_t)d.B * d.B;
+            StageTimer tm(ctx, wiener ? ST_FILTER_FINAL : ST_FILTER_BASIC, s);
+            HIP_CHECK(ctx, launch_filter_generic(g, wiener, s));
+        } else {
+            const bool mfma = ctx->precision != ISP_BM3D_PREC_FP32;
+            const isp_bm3d_ctx::Tiles t = tiles_for(ctx, W, H);
+            const FilterArgs f = build_filter(ctx, p, d, W, H, io.pitch, io.src, io.guide, w.num, w.den,
+                                              w.cnt[k], w.pos[k], mfma ? t.mfma_x : t.filter_x,
+                                              mfma ? t.mfma_y : t.filter_y);
+            const size_t lds = filter_lds(ctx, f, wiener);
+            if (lds > 160 * 1024)
+                return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "parameters need %zu bytes of LDS for filtering", lds);
+            StageTimer tm(ctx, wiener ? ST_FILTER_FINAL : ST_FILTER_BASIC, s);
+            HIP_CHECK(ctx, launch_filter_any(ctx, f, wiener, s));
+        }
     }
     StageTimer tm(ctx, ST_FINALIZE, s);
     HIP_CHECK(ctx, launch_finalize(fa, s));
@@ -865,6 +973,8 @@ int batch_params(isp_bm3d_batch *b, int k)
     for (int s = 0; s < 2; ++s) {
         int rc = derive(ctx, s ? x.prm.final_ : x.prm.basic, b->nplanes, x.W, x.H, x.d[s]);
         if (rc) return rc;
+        if (x.d[s].generic)
+            return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "batches need BlockSize 8 and GroupSize <= %d", kMaxGroup);
     }
     return ISP_BM3D_OK;
 }
@@ -1460,6 +1570,30 @@ int isp_bm3d_create(isp_bm3d_ctx **out, const isp_bm3d_params *params, int devic
         delete ctx;
         return rc;
     }
+    // transform tables of the generic kernels, for a step whose BlockSize is
+    // not 8 or whose GroupSize is above 32 (the reference plans a transform
+    // for every group length of every step, source/BM3D.cpp:9-24)
+    for (int k = 0; k < 2; ++k) {
+        const isp_bm3d_step_params &q = k ? ctx->params.final_ : ctx->params.basic;
+        const int B = q.block_size, G = q.group_size;
+        if (!(B != kBlock || G > kMaxGroup) || B < 1 || B > kGenMaxBlock || G < 1 || G > kGenMaxGroup) continue;
+        std::vector<double> gt;
+        auto dct = [&](int n) {
+            for (int r = 0; r < n; ++r)
+                for (int c = 0; c < n; ++c)
+                    gt.push_back((double)((r == 0 ? std::sqrt(1.0L / n) : std::sqrt(2.0L / n)) *
+                                          std::cos(PI * (2 * c + 1) * r / (2.0L * n))));
+        };
+        dct(B);
+        for (int N = 1; N <= G; ++N) dct(N);
+        if ((e = hipMalloc(&ctx->d_gen[k], gt.size() * sizeof(double))) != hipSuccess ||
+            (e = hipMemcpy(ctx->d_gen[k], gt.data(), gt.size() * sizeof(double), hipMemcpyHostToDevice)) !=
+                hipSuccess) {
+            int rc = fail(nullptr, ISP_BM3D_E_HIP, "create: %s", hipGetErrorString(e));
+            isp_bm3d_destroy(ctx);
+            return rc;
+        }
+    }
     // the scratch array's address on this device (the device is current):
     // the allocation-free entry point then needs no runtime lookup
     if (scratch_address(device, &ctx->scratch) != hipSuccess) ctx->scratch = nullptr;
@@ -1523,8 +1657,10 @@ int isp_bm3d_destroy(isp_bm3d_ctx *ctx)
     if (ctx->pin) (void)hipHostFree(ctx->pin);
     if (ctx->d_tables) (void)hipFree(ctx->d_tables);
     if (ctx->d_tables64) (void)hipFree(ctx->d_tables64);
-    for (int k = 0; k < 2; ++k)
+    for (int k = 0; k < 2; ++k) {
         if (ctx->d_mfma[k]) (void)hipFree(ctx->d_mfma[k]);
+        if (ctx->d_gen[k]) (void)hipFree(ctx->d_gen[k]);
+    }
     delete ctx;
     return ISP_BM3D_OK;
 }
@@ -1772,17 +1908,16 @@ int plane_noalloc(isp_bm3d_ctx *ctx, float *dst, const float *src, const float *
     if (!dst || !src) return fail(ctx, ISP_BM3D_E_ARG, "null dst/src");
     if (!ctx->scratch)
         return fail(ctx, ISP_BM3D_E_HIP, "the scratch array's address was not found on device %d", ctx->device);
-    if (W < kBlock || H < kBlock)
-        return fail(ctx, ISP_BM3D_E_SIZE, "image %dx%d is smaller than one %dx%d block", W, H, kBlock, kBlock);
     const int P = (int)stride;
     size_t refs = 1;
-    int G = 1;
     for (const isp_bm3d_step_params *p : {&ctx->params.basic, &ctx->params.final_}) {
         if (p->block_step <= 0) continue;
-        const size_t ngx = (size_t)(W - kBlock) / p->block_step + 2, ngy = (size_t)(H - kBlock) / p->block_step + 2;
+        const int B = p->block_size > 0 ? p->block_size : kBlock;
+        const size_t ngx = (size_t)std::max(W - B, 0) / p->block_step + 2;
+        const size_t ngy = (size_t)std::max(H - B, 0) / p->block_step + 2;
         refs = std::max(refs, ngx * ngy);
-        G = std::max(G, std::min(std::max(p->group_size, 1), kMaxGroup));
     }
+    const int G = max_group(ctx);
     Work w{};
     const size_t need = noalloc_layout(&w, nullptr, P, H, refs, G);
     if (need > kScratchBytes)
@@ -2199,6 +2334,8 @@ int isp_bm3d_autotune(isp_bm3d_ctx *ctx, int W, int H, double sigma, int flags, 
     Derived d{};
     int rc = derive(ctx, ctx->params.basic, 1, W, H, d);
     if (rc) return rc;
+    if (d.generic)
+        return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "autotuning needs BlockSize 8 and GroupSize <= %d", kMaxGroup);
     const long tiles = (long)((d.ngx + 7) / 8) * ((d.ngy + 7) / 8);
     const int K = (int)std::max(1L, std::min(32L, (4096 + tiles - 1) / std::max(1L, tiles)));
     const size_t np = (size_t)W * H;
@@ -2716,6 +2853,9 @@ int stack_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, co
     Derived d{};
     int rc = derive(ctx, p, io.nplanes, W, H, d);
     if (rc) return rc;
+    if (d.generic)
+        return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "multi-frame groups need BlockSize 8 and GroupSize <= %d",
+                    kMaxGroup);
     const size_t fs = io.fs;
     const int T = io.T, P = io.P;
     const bool mfma = ctx->precision != ISP_BM3D_PREC_FP32;

@@ -121,6 +121,7 @@ struct MatchArgs {
     const void *search;      // nullptr: the guide itself (the single-frame search)
     int refW, refH, refPitch;  // staged reference-block tile of a cross-frame search
     int walk;                // 1: match_walk_kernel (see match_walk_eligible)
+    int B;                   // BlockSize (match_generic_kernel; the other kernels use kBlock)
 };
 
 // The column-walk matching kernel takes FP32 distances of the single-frame
@@ -278,6 +279,35 @@ struct NlmFinalizeArgs {
     void *out[3];
     size_t out_pitch;
 };
+
+// Generic kernels (generic_kernels.hip) for the parameters the tuned kernels
+// do not take: BlockSize != 8 or GroupSize > 32.  FP64 distances, transforms
+// and shrinkage; one workgroup per group; aggregation by global atomics into
+// the same fixed-point planes.
+constexpr int kGenMaxBlock = 16;
+constexpr int kGenMaxGroup = 64;
+constexpr int kGenMaxElems = 8192;  // G * B^2 of one group (two FP64 groups in LDS for Wiener)
+
+struct GenFilterArgs {
+    int W, H, pitch;
+    int ngx, ngy;
+    int B, G;
+    const int *cnt, *pos;      // groups from matching, G entries per grid point
+    const float *src[3];
+    const float *guide[3];     // Wiener guides
+    unsigned long long *num[3], *den[3];  // zeroed 64-bit fixed-point sums (kFixScale)
+    int active[3];
+    double thr[3];             // lambda * sigma_c (orthonormal domain)
+    double sig2[3];            // sigma_c^2
+    float wmax;                // cap of the Wiener group weight
+    const double *cb;          // B x B orthonormal DCT-II [k][n]
+    const double *cn;          // C_N [k][n] for N = 1..G, at (N-1) N (2N-1) / 6
+};
+
+size_t match_generic_lds_bytes(const MatchArgs &a);
+hipError_t launch_match_generic(const MatchArgs &a, bool guide64, hipStream_t s);
+size_t filter_generic_lds_bytes(const GenFilterArgs &a, bool wiener);
+hipError_t launch_filter_generic(const GenFilterArgs &a, bool wiener, hipStream_t s);
 
 size_t nlm_lds_bytes(const NlmArgs &a);
 hipError_t launch_nlm(const NlmArgs &a, hipStream_t s);

@@ -73,10 +73,14 @@ int main()
         std::printf("nlm frame R mean abs err %.3f levels (noisy %.3f); nlm plane(0,0) %.6f\n", nerr / (W * H),
                     noisy / (W * H), pn(0, 0));
         BM3D_Final_Para fp("np");
-        fp.BlockSize = 11;  // not implemented by the GPU kernels: reported, not crashed
+        fp.BlockSize = 11;  // profile vn's final BlockSize: the generic kernels
         BM3D_Final fin(fp);
+        fin(src, dst);
+        std::printf("final step with BlockSize 11: out(0,0) %.6f\n", dst(0, 0));
+        fp.BlockSize = 17;  // beyond every kernel: reported, not crashed
+        BM3D_Final fin17(fp);
         try {
-            fin(src, dst);
+            fin17(src, dst);
             std::printf("unexpected success\n");
             return 1;
         } catch (const Error &e) {

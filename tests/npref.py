@@ -20,8 +20,8 @@ NORM = np.array([np.sqrt(1 / 3), np.sqrt(1 / 2), np.sqrt(3 / 8)])
 B = 8
 
 
-def grid(n, s):
-    last = n - B
+def grid(n, s, b=B):
+    last = n - b
     out, j = [], 0
     while True:
         if j >= last + s:
@@ -37,7 +37,9 @@ def brute_groups(p, ref, norm=NORM[0]):
     """All candidates of the clipped window (BMstep grid through the reference
     position), SSE <= thSSE and != 0, sorted by (key, scan order), self first,
     truncated to GroupSize.  Returns {(j, i): [(key, y, x), ...]}.  thMSE is
-    scaled by `norm` (BM3D's luma OPP norm; NL-means passes 1)."""
+    scaled by `norm` (BM3D's luma OPP norm; NL-means passes 1).  The block
+    size is p.BlockSize (8 when p has none)."""
+    B = int(getattr(p, "BlockSize", 8))
     ref = np.asarray(ref, np.float64)
     h, w = ref.shape
     st = p.BMstep
@@ -46,8 +48,8 @@ def brute_groups(p, ref, norm=NORM[0]):
     mse2sse = B * B / 255.0 ** 2
     th = thMSE * mse2sse
     out = {}
-    for j in grid(h, p.BlockStep):
-        for i in grid(w, p.BlockStep):
+    for j in grid(h, p.BlockStep, B):
+        for i in grid(w, p.BlockStep, B):
             rb = ref[j:j + B, i:i + B]
             if p.GroupSize == 1:
                 out[(j, i)] = [(0.0, j, i)]
@@ -70,6 +72,7 @@ def brute_groups(p, ref, norm=NORM[0]):
 
 def filter_plane(p, src, ref, groups, c, wiener):
     """Aggregated num/den of one plane for the given groups."""
+    B = int(getattr(p, "BlockSize", 8))
     src = np.asarray(src, np.float64)
     ref = np.asarray(ref, np.float64)
     h, w = src.shape

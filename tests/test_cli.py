@@ -203,6 +203,31 @@ def test_cli_equals_library_integer_frame_path(gpu, tmp_path):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("flags", [("-P", "vn", "-S", "20"),
+                                   ("-P", "np", "-S", "20", "-BS1", "6", "-GS1", "40", "-BS2", "10", "-GS2", "48")],
+                         ids=["vn", "BS-GS"])
+def test_cli_block_and_group_sizes(gpu, tmp_path, flags):
+    # BlockSize other than 8 and GroupSize above 32 through the front end
+    # (include/BM3D.h:416-452), equal to the library's integer Frame path
+    from isp_mw_amd import BM3D
+    p, a = noisy_png(tmp_path, seed=6)
+    out = run("--bm3d", p, *flags).stdout.strip()
+    got = np.asarray(PIL.open(out), np.int64)
+    prm = params(*flags)
+    from isp_mw_amd import BM3D_Para
+    para = BM3D_Para(prm["profile"])
+    for st, d in ((para.basic, prm["basic"]), (para.final, prm["final"])):
+        st.BlockSize, st.BlockStep, st.GroupSize = d["BlockSize"], d["BlockStep"], d["GroupSize"]
+        st.BMrange, st.BMstep, st.thMSE = d["BMrange"], d["BMstep"], d["thMSE"]
+        st.sigma = list(d["sigma"])
+    para.basic.lambda_ = prm["basic"]["lambda"]
+    frame = np.ascontiguousarray(np.transpose(a, (2, 0, 1)).astype(np.int32) * 257)
+    res = BM3D(para)(frame, floor=0, ceil=65535)
+    want = np.transpose((res.astype(np.int64) + 128) // 257, (1, 2, 0))
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.gpu
 def test_cli_against_oracle_and_guide(gpu, tmp_path):
     import oracle
     from isp_mw_amd import BM3D_Para

@@ -31,6 +31,7 @@ def test_cpp_example_runs(gpu, tmp_path):
     print(out.stdout, out.stderr)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "expected error -2" in out.stdout
+    assert "final step with BlockSize 11" in out.stdout
     err = float(out.stdout.split("mean abs err")[1].split()[0])
     assert err < 0.05
     ierr = float(out.stdout.split("int frame R mean abs err")[1].split()[0])

@@ -100,7 +100,7 @@ def test_plane_vs_oracle(gpu, mode, shape):
     assert_close(got, ref, clean, label=f"plane {mode} {w}x{h}")
 
 
-@pytest.mark.parametrize("profile", ["fast", "lc", "np", "high"])
+@pytest.mark.parametrize("profile", ["fast", "lc", "np", "high", "vn"])
 def test_profiles_vs_oracle(gpu, profile):
     say(f"profile {profile}")
     clean, n = synth.grey_pair(72, 88, 20.0, 31)
@@ -243,8 +243,9 @@ def test_errors(gpu):
         BM3D(para())(np.zeros((40000, 16), np.float32))
     assert e.value.code == L.E_UNSUPPORTED
     p = para("vn", step=None, R=None)
+    p.final.BlockSize = 17  # above the generic kernels' 16
     with pytest.raises(L.BM3DError) as e:
-        BM3D(p)(np.zeros((32, 32), np.float32))  # final BlockSize 11
+        BM3D(p)(np.zeros((32, 32), np.float32))
     assert e.value.code == L.E_UNSUPPORTED
 
 

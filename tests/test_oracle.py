@@ -103,6 +103,62 @@ def test_match_vs_brute_force(shape, kw):
             np.testing.assert_allclose(keys[a, b, :N], [k for k, _, _ in g], rtol=1e-12, atol=1e-12)
 
 
+# BlockSize other than 8 (profile vn's final step: 11) and GroupSize above 32
+BLOCK_CASES = [((40, 37), 11, dict(step=6, R=8, G1=32, G2=32)),
+               ((31, 33), 5, dict(step=2, R=6, G1=48, G2=48)),
+               ((38, 36), 16, dict(step=5, R=5, G1=12, G2=12)),
+               ((30, 29), 8, dict(step=1, R=4, G1=64, G2=64, sigma=50.0))]
+
+
+def block_para(B, kw):
+    p = para(**kw)
+    p.basic.BlockSize = p.final.BlockSize = B
+    return p
+
+
+def smooth_pair(h, w, seed):
+    """A smooth image with weak noise (10/255): most window candidates pass
+    the default threshold, so groups fill up to GroupSize."""
+    y, x = np.mgrid[0:h, 0:w]
+    clean = 0.4 + 0.1 * np.sin(x / 7.0) * np.cos(y / 9.0) + 0.002 * x
+    n = clean + np.random.default_rng(seed).standard_normal((h, w)) * (10 / 255)
+    return clean, n.astype(np.float32)
+
+
+@pytest.mark.parametrize("shape,B,kw", BLOCK_CASES)
+def test_match_vs_brute_force_any_block_size(shape, B, kw):
+    h, w = shape
+    _, n = smooth_pair(h, w, h * 7 + w)
+    p = block_para(B, kw)
+    counts, pos, keys = oracle.match_all(p.basic, n)
+    bf = npref.brute_groups(p.basic, n)
+    gy, gx = npref.grid(h, p.basic.BlockStep, B), npref.grid(w, p.basic.BlockStep, B)
+    assert list(gy) == list(oracle.grid(h, B, p.basic.BlockStep))
+    assert counts.shape == (len(gy), len(gx))
+    assert counts.max() == p.basic.GroupSize  # full groups are exercised
+    for a, j in enumerate(gy):
+        for b, i in enumerate(gx):
+            g = bf[(j, i)]
+            N = counts[a, b]
+            assert N == len(g)
+            got = [(int(v) >> 16, int(v) & 0xFFFF) for v in pos[a, b, :N]]
+            assert got == [(y, x) for _, y, x in g]
+            np.testing.assert_allclose(keys[a, b, :N], [k for k, _, _ in g], rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("wiener", [False, True])
+@pytest.mark.parametrize("shape,B,kw", BLOCK_CASES)
+def test_kernel_vs_orthonormal_numpy_any_block_size(shape, B, kw, wiener):
+    h, w = shape
+    clean, n = smooth_pair(h, w, h + 3 * w)
+    p = block_para(B, kw)
+    st = p.final if wiener else p.basic
+    ref = (clean + 0.02 * np.sin(np.arange(w) / 3.0)).astype(np.float32) if wiener else n
+    got = oracle.kernel(st, n, ref, wiener=wiener)
+    want = npref.kernel(st, [n], [ref], wiener)[0]
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-10)
+
+
 def test_match_invariants():
     _, n = synth.grey_pair(64, 72, 25.0, 3)
     p = para()
