@@ -302,6 +302,13 @@ def bench_pilot(args) -> int:
     g_p = piloted.guide(x).cpu().numpy()
     g_b = plain.guide(x).cpu().numpy()
     p_noisy, p_plain, p_pilot = synth.psnr(n, clean), synth.psnr(out_b, clean), synth.psnr(out_p, clean)
+    # the same comparison on other seeded frames of the recipe: the pilot's
+    # loss depends on the image (its gratings)
+    others = {}
+    for sd in (77, 5, 11):
+        c2, n2 = synth.grey_pair(H, W, SIGMA, sd)
+        x2 = torch.from_numpy(n2).cuda()
+        others[str(sd)] = round(synth.psnr(piloted(x2).cpu().numpy(), c2) - synth.psnr(plain(x2).cpu().numpy(), c2), 3)
     # the network's flops: 2*9*cin*cout per pixel per layer
     from isp_mw_amd.pilot import LAYERS
     fl = 2.0 * 9 * sum(ci * co for ci, co, _, _, _ in LAYERS) * W * H
@@ -328,7 +335,8 @@ def bench_pilot(args) -> int:
         "quality": {"psnr_noisy_db": round(p_noisy, 3), "psnr_plain_two_step_db": round(p_plain, 3),
                     "psnr_pilot_db": round(p_pilot, 3), "gain_db": round(p_pilot - p_plain, 3),
                     "psnr_guide_pilot_db": round(synth.psnr(g_p, clean), 3),
-                    "psnr_guide_step1_db": round(synth.psnr(g_b, clean), 3)},
+                    "psnr_guide_step1_db": round(synth.psnr(g_b, clean), 3),
+                    "gain_db_other_seeds": others},
         "training": {"target_generation_s": round(gen_s, 2), "train_s": round(train_s, 2),
                      "ms_per_step": round(train_s / args.pilot_steps * 1e3, 3),
                      "loss_first": losses[0], "loss_last_mean": float(np.mean(losses[-50:]))},
@@ -340,10 +348,12 @@ def bench_pilot(args) -> int:
 def bench_full(args) -> int:
     """Full form: 8 frames of 3840x2160 RGB (one seeded base image, frames
     shifted by random 0..7 px), sigma 25, BMrange 19, GroupSize 32 in both
-    steps.  The learned pilot (trained as in the pilot workload) gives each
-    frame's guide, per RGB channel; the Wiener step runs on all 8 frames in one
+    steps.  The RGB learned pilot (the three OPP planes of a frame at once,
+    trained on step-1 targets with these parameters) gives the guides of all 8
+    frames in one network pass; the Wiener step runs on all 8 frames in one
     batched graph launch with FP16 matrix-core transforms.  Compared with the
-    plain FP32 GPU two-step path on every frame."""
+    plain FP32 GPU two-step path on every frame, and with step 1 as the guide
+    in the same batched FP16 form."""
     import torch
     from isp_mw_amd import BM3D, BM3D_Final, BM3D_Para, BM3DBatch, pilot as P, synth
 
@@ -362,23 +372,24 @@ def bench_full(args) -> int:
         st.BlockStep, st.BMrange, st.BMstep, st.GroupSize = 3, 19, 1, 32
     para.basic.lambda_ = 2.7
     para.set_sigma(SIGMA)
-    # the pilot
+    # the pilot: the RGB network, trained on step-1 targets of these parameters
     t0 = time.perf_counter()
-    noisy_t, sig_t, tgt_t = P.make_training_set(20000, crop=128, seed=0)
-    net = P.PilotCNN(seed=0)
+    noisy_t, sig_t, tgt_t = P.make_rgb_training_set(20000, crop=128, seed=0, para=para)
+    net = P.PilotCNN(seed=0, channels=3)
     P.train(net, noisy_t, sig_t, tgt_t, args.pilot_steps, batch=32, lr=1e-3, seed=0)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
     del noisy_t, tgt_t
-    xs = [torch.from_numpy(f).cuda() for f in frames]
-    guides = [torch.empty_like(x) for x in xs]
+    x_all = torch.from_numpy(np.stack(frames)).cuda()
+    xs = list(x_all)
+    guide_all = torch.empty_like(x_all)
+    guides = list(guide_all)
     outs = [torch.empty_like(x) for x in xs]
     fin = BM3D_Final(para.final, precision="fp16")
     batch = BM3DBatch(fin, outs, xs, refs=guides)
 
     def guides_of_frames():
-        for x, g in zip(xs, guides):
-            g.copy_(P.rgb_guide(net, x, SIGMA))   # the network on the frame's OPP planes
+        P.rgb_guide(net, x_all, SIGMA, out=guide_all)   # one network pass over the 8 frames' OPP planes
 
     def full_form():
         guides_of_frames()
@@ -411,6 +422,17 @@ def bench_full(args) -> int:
     torch.cuda.synchronize()
     dt_plain = time.perf_counter() - t0
     psnr_plain = float(np.mean([synth.psnr(o.cpu().numpy(), c) for o, c in zip(ref_out, cleans)]))
+    # step 1 as the guide in the same batched FP16 form (the method without the network)
+    from isp_mw_amd import BM3D_Basic
+    est1 = [torch.empty_like(x) for x in xs]
+    b1 = BM3DBatch(BM3D_Basic(para.basic, precision="fp16"), est1, xs)
+    b1.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b1.run()
+    torch.cuda.synchronize()
+    dt_step1 = (time.perf_counter() - t0) / args.steps
     # the FP16 two-step path without the pilot, batched, for reference
     both16 = BM3D(para, precision="fp16")
     out16 = [torch.empty_like(x) for x in xs]
@@ -425,8 +447,7 @@ def bench_full(args) -> int:
     px = float(NF * FW_ * FH_)
     # algorithmic work of the full form: the CNN on 3 planes per frame, and the
     # Wiener step's 3 transforms x 3 planes at full N plus its matching
-    from isp_mw_amd.pilot import LAYERS
-    cnn = 2.0 * 9 * sum(ci * co for ci, co, _, _, _ in LAYERS) * 3 * px
+    cnn = 2.0 * 9 * sum(ci * co for ci, co, _, _, _ in P.layers(3)) * px
     gx, gy = grid(FW_, 3), grid(FH_, 3)
     refs = gx.size * gy.size
     cand = float(window_len(gx, FW_, 19).sum() * window_len(gy, FH_, 19).sum() - refs)
@@ -450,7 +471,11 @@ def bench_full(args) -> int:
         "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_F16_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_F16_TFLOPS, 4), "traffic": None,
                      "basis": "CNN flops + Wiener transforms at full N + matching, over the whole form"},
-        "stages_ms": {"pilot_cnn_8_frames": round(dt_pilot * 1e3, 2), "wiener_batch": round(dt_wiener * 1e3, 2)},
+        "stages_ms": {"pilot_cnn_8_frames": round(dt_pilot * 1e3, 2), "wiener_batch": round(dt_wiener * 1e3, 2),
+                      "step1_batch_8_frames": round(dt_step1 * 1e3, 2)},
+        "pilot_gain": {"guide_speedup_vs_step1": round(dt_step1 / dt_pilot, 2),
+                       "note": "the pilot's guides of the 8 frames against batched FP16 step 1 (the same guides' "
+                               "role) with these parameters"},
         "quality": {"psnr_full_form_db": round(psnr_full, 3), "psnr_plain_fp32_two_step_db": round(psnr_plain, 3),
                     "psnr_delta_vs_plain_fp32_db": round(psnr_full - psnr_plain, 3),
                     "psnr_fp16_two_step_batch_db": round(psnr16, 3)},

@@ -43,7 +43,8 @@ int isp_pilot_conv3x3(const void *in, const void *mask, const void *weights, con
                       float *out_f32, int n, int h, int w, int cin_pad, int cout_pad, int relu, void *stream);
 
 /* Weight and bias gradients of a convolution with real channel counts cin,
- * cout: dw [cout][cin][9], db [cout] (may be NULL), with dz = dout * (mask > 0)
+ * cout (48 with cout_pad 48; 1 or 3, one residual per plane, with cout_pad
+ * 16): dw [cout][cin][9], db [cout] (may be NULL), with dz = dout * (mask > 0)
  * (mask may be NULL).  partial: scratch of isp_pilot_wgrad_partial_floats
  * floats.  The sums are reduced in a fixed order (run to run identical). */
 size_t isp_pilot_wgrad_partial_floats(int n, int h, int w, int cin, int cout);
@@ -56,16 +57,26 @@ int isp_pilot_pack(const float *w, void *wf, void *wb, int cin, int cout, int ci
 int isp_pilot_adam(float *p, const float *g, float *m, float *v, long n, float lr, float beta1, float beta2,
                    float eps, int step, void *stream);
 
-/* Network input [n][h][w][16]: channel 0 the noisy plane plus offset[i] (offset
- * may be NULL: 0; chroma planes centred on 0 pass 0.5), channel 1 sigma[i]/255
- * (sigma per plane in 0..255 units), the rest 0.  The residual output makes
- * the offset cancel: out = noisy - r. */
+/* Network input [n][h][w][16] of n images of `channels` planes each (noisy:
+ * [n][channels][h][w], 1 <= channels <= 15): channel c < channels the
+ * noisy plane c plus offset[i * channels + c] (offset may be NULL: 0; chroma
+ * planes centred on 0 pass 0.5), channel `channels` sigma[i]/255 (one sigma
+ * per image, 0..255 units), the rest 0.  The residual output makes the
+ * offsets cancel: out = noisy - r. */
+int isp_pilot_input_planes(const float *noisy, int channels, const float *sigma, const float *offset, void *x, int n,
+                           int h, int w, void *stream);
+/* isp_pilot_input_planes with one plane per image */
 int isp_pilot_input(const float *noisy, const float *sigma, const float *offset, void *x, int n, int h, int w,
                     void *stream);
 
-/* out = noisy - residual (residual: channel 0 of res_f32 [n][h][w][16]).  With a
- * target: *loss += mean((out - target)^2) and dres [n][h][w][16] (bf16) gets
- * d(loss)/d(residual) in channel 0.  out may be NULL. */
+/* out = noisy - residual per plane (plane c of image i: residual channel c of
+ * res_f32 [n][h][w][16]; noisy, out, target [n][channels][h][w]).  With a
+ * target: *loss += mean over every plane of (out - target)^2 and dres
+ * [n][h][w][16] (bf16) gets d(loss)/d(residual) in channels 0..channels-1.
+ * out may be NULL. */
+int isp_pilot_output_planes(const float *noisy, const float *res_f32, float *out, const float *target, void *dres,
+                            float *loss, int channels, int n, int h, int w, void *stream);
+/* isp_pilot_output_planes with one plane per image */
 int isp_pilot_output(const float *noisy, const float *res_f32, float *out, const float *target, void *dres,
                      float *loss, int n, int h, int w, void *stream);
 

@@ -405,44 +405,62 @@ __global__ void adam_kernel(float *__restrict__ p, const float *__restrict__ g, 
     }
 }
 
-// network input: channel 0 the noisy sample plus the plane's offset, channel 1
-// sigma/255, 2..15 zero
-__global__ void input_kernel(const float *__restrict__ noisy, const float *__restrict__ sigma,
+// network input [n][h][w][16]: channels 0..C-1 the noisy samples of the
+// image's C planes plus each plane's offset, channel C sigma/255, the rest zero
+__global__ void input_kernel(const float *__restrict__ noisy, int C, const float *__restrict__ sigma,
                              const float *__restrict__ offset, uint16_t *__restrict__ x, int N, int H, int W)
 {
-    const long total = (long)N * H * W;
+    const long hw = (long)H * W, total = (long)N * hw;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
-        const int n = (int)(i / ((long)H * W));
+        const int n = (int)(i / hw);
+        const long p = i - (long)n * hw;
+        uint32_t v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int c = 0; c < C; ++c) {
+            const long q = (long)n * C + c;
+            const uint32_t b = f2bf(noisy[q * hw + p] + (offset ? offset[q] : 0.f));
+            v[c >> 1] |= b << (16 * (c & 1));
+        }
+        v[C >> 1] |= (uint32_t)f2bf(sigma[n] / 255.f) << (16 * (C & 1));
         uint4 *d = reinterpret_cast<uint4 *>(x + (size_t)i * 16);
-        const float v = noisy[i] + (offset ? offset[n] : 0.f);
-        const uint32_t c01 = (uint32_t)f2bf(v) | ((uint32_t)f2bf(sigma[n] / 255.f) << 16);
-        d[0] = make_uint4(c01, 0, 0, 0);
-        d[1] = make_uint4(0, 0, 0, 0);
+        d[0] = make_uint4(v[0], v[1], v[2], v[3]);
+        d[1] = make_uint4(v[4], v[5], v[6], v[7]);
     }
 }
 
-// out = noisy - residual; with a target also the loss sum and d(loss)/d(residual)
-// for loss = mean((out - target)^2).
-__global__ void output_kernel(const float *__restrict__ noisy, const float *__restrict__ res32,
+// out = noisy - residual per plane (residual channel c of res32 for plane c);
+// with a target also the loss sum and d(loss)/d(residual) for
+// loss = mean over the N * C planes of (out - target)^2.
+__global__ void output_kernel(const float *__restrict__ noisy, const float *__restrict__ res32, int C,
                               float *__restrict__ out, const float *__restrict__ target, uint16_t *__restrict__ dres,
-                              float *__restrict__ loss, long total)
+                              float *__restrict__ loss, int N, int H, int W)
 {
+    const long hw = (long)H * W, total = (long)N * hw;
+    const float count = (float)(total * C);
     float part = 0.f;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
-        const float o = noisy[i] - res32[(size_t)i * 16];
-        if (out) out[i] = o;
+        const int n = (int)(i / hw);
+        const long p = i - (long)n * hw;
+        uint32_t g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int c = 0; c < C; ++c) {
+            const long q = ((long)n * C + c) * hw + p;
+            const float o = noisy[q] - res32[(size_t)i * 16 + c];
+            if (out) out[q] = o;
+            if (target) {
+                const float d = o - target[q];
+                part += d * d;
+                g[c >> 1] |= (uint32_t)f2bf(-2.f * d / count) << (16 * (c & 1));
+            }
+        }
         if (target) {
-            const float d = o - target[i];
-            part += d * d;
             uint4 *q = reinterpret_cast<uint4 *>(dres + (size_t)i * 16);
-            q[0] = make_uint4((uint32_t)f2bf(-2.f * d / (float)total), 0, 0, 0);
-            q[1] = make_uint4(0, 0, 0, 0);
+            q[0] = make_uint4(g[0], g[1], g[2], g[3]);
+            q[1] = make_uint4(g[4], g[5], g[6], g[7]);
         }
     }
     if (target && loss) {
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
-        if ((threadIdx.x & 63) == 0) atomicAdd(loss, part / (float)total);
+        if ((threadIdx.x & 63) == 0) atomicAdd(loss, part / count);
     }
 }
 
@@ -495,7 +513,9 @@ hipError_t wgrad_t(const void *in, const void *dout, const void *mask, float *pa
     return hipGetLastError();
 }
 
-int cb_of(int coutp) { return coutp == 48 ? 12 : 1; }
+// output channels per work item of the weight gradient: 12 of a hidden
+// layer's 48; the last layer's 1 (grey) or 3 (OPP planes) in one item
+int cb_of(int cout) { return cout > 12 ? 12 : cout; }
 
 int wgrad_blocks(int n, int h, int w)
 {
@@ -525,7 +545,7 @@ int isp_pilot_conv3x3(const void *in, const void *mask, const void *weights, con
 
 size_t isp_pilot_wgrad_partial_floats(int n, int h, int w, int cin, int cout)
 {
-    const int cb = cout > 1 ? 12 : 1;
+    const int cb = cb_of(cout);
     const size_t stride = (size_t)((cout + cb - 1) / cb) * cin * 9 * cb + cout;
     return stride * (size_t)wgrad_blocks(n, h, w);
 }
@@ -535,19 +555,23 @@ int isp_pilot_conv3x3_wgrad(const void *in, const void *dout, const void *mask, 
 {
     if (!in || !dout || !partial || !dw || n <= 0 || h <= 0 || w <= 0) return ISP_PILOT_E_ARG;
     if (cin < 1 || cin > cin_pad || cout < 1 || cout > cout_pad) return ISP_PILOT_E_ARG;
-    if ((cout_pad == 48) != (cout > 1)) return ISP_PILOT_E_UNSUPPORTED;  // CB 12 for 48 outputs, 1 for the last layer
+    const int cb = cb_of(cout);
+    // CB 12 for the 48 outputs of a hidden layer, 1 or 3 for the last layer
+    if ((cout_pad == 48) != (cb == 12) || (cout_pad == 16 && cout != 1 && cout != 3)) return ISP_PILOT_E_UNSUPPORTED;
     hipStream_t s = (hipStream_t)stream;
     const int blocks = wgrad_blocks(n, h, w);
     // work items per thread the instantiations hold (items = ceil(cout/CB) * cin * 9)
     const int ipt = cin_pad == 16 ? 1 : (cout_pad == 48 ? 7 : 2);
-    if (((cout + cb_of(cout_pad) - 1) / cb_of(cout_pad)) * cin * 9 > ipt * THREADS) return ISP_PILOT_E_UNSUPPORTED;
+    if (((cout + cb - 1) / cb) * cin * 9 > ipt * THREADS) return ISP_PILOT_E_UNSUPPORTED;
     hipError_t e;
     if (cin_pad == 16 && cout_pad == 48) e = wgrad_t<16, 48, 12, 1>(in, dout, mask, partial, blocks, n, h, w, cin, cout, s);
     else if (cin_pad == 48 && cout_pad == 48) e = wgrad_t<48, 48, 12, 7>(in, dout, mask, partial, blocks, n, h, w, cin, cout, s);
-    else if (cin_pad == 48 && cout_pad == 16) e = wgrad_t<48, 16, 1, 2>(in, dout, mask, partial, blocks, n, h, w, cin, cout, s);
+    else if (cin_pad == 48 && cout_pad == 16 && cout == 1)
+        e = wgrad_t<48, 16, 1, 2>(in, dout, mask, partial, blocks, n, h, w, cin, cout, s);
+    else if (cin_pad == 48 && cout_pad == 16 && cout == 3)
+        e = wgrad_t<48, 16, 3, 2>(in, dout, mask, partial, blocks, n, h, w, cin, cout, s);
     else return ISP_PILOT_E_UNSUPPORTED;
     if (e != hipSuccess) return ISP_PILOT_E_HIP;
-    const int cb = cb_of(cout_pad);
     const int stride = ((cout + cb - 1) / cb) * cin * 9 * cb + cout;
     hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((stride + 255) / 256), dim3(256), 0, s, partial, blocks, cin, cout, cb,
                        dw, db);
@@ -578,26 +602,39 @@ int isp_pilot_adam(float *p, const float *g, float *m, float *v, long n, float l
     return hipGetLastError() == hipSuccess ? ISP_PILOT_OK : ISP_PILOT_E_HIP;
 }
 
+int isp_pilot_input_planes(const float *noisy, int channels, const float *sigma, const float *offset, void *x, int n,
+                           int h, int w, void *stream)
+{
+    if (!noisy || !sigma || !x || n <= 0 || h <= 0 || w <= 0 || channels < 1 || channels > 15) return ISP_PILOT_E_ARG;
+    const long total = (long)n * h * w;
+    const long blocks = (total + 255) / 256 < 4096 ? (total + 255) / 256 : 4096;
+    hipLaunchKernelGGL(input_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, noisy, channels,
+                       sigma, offset, (uint16_t *)x, n, h, w);
+    return hipGetLastError() == hipSuccess ? ISP_PILOT_OK : ISP_PILOT_E_HIP;
+}
+
 int isp_pilot_input(const float *noisy, const float *sigma, const float *offset, void *x, int n, int h, int w,
                     void *stream)
 {
-    if (!noisy || !sigma || !x || n <= 0 || h <= 0 || w <= 0) return ISP_PILOT_E_ARG;
+    return isp_pilot_input_planes(noisy, 1, sigma, offset, x, n, h, w, stream);
+}
+
+int isp_pilot_output_planes(const float *noisy, const float *res_f32, float *out, const float *target, void *dres,
+                            float *loss, int channels, int n, int h, int w, void *stream)
+{
+    if (!noisy || !res_f32 || n <= 0 || h <= 0 || w <= 0 || (target && !dres) || channels < 1 || channels > 16)
+        return ISP_PILOT_E_ARG;
     const long total = (long)n * h * w;
     const long blocks = (total + 255) / 256 < 4096 ? (total + 255) / 256 : 4096;
-    hipLaunchKernelGGL(input_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, noisy, sigma, offset,
-                       (uint16_t *)x, n, h, w);
+    hipLaunchKernelGGL(output_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, noisy, res_f32,
+                       channels, out, target, (uint16_t *)dres, loss, n, h, w);
     return hipGetLastError() == hipSuccess ? ISP_PILOT_OK : ISP_PILOT_E_HIP;
 }
 
 int isp_pilot_output(const float *noisy, const float *res_f32, float *out, const float *target, void *dres, float *loss,
                      int n, int h, int w, void *stream)
 {
-    if (!noisy || !res_f32 || n <= 0 || h <= 0 || w <= 0 || (target && !dres)) return ISP_PILOT_E_ARG;
-    const long total = (long)n * h * w;
-    const long blocks = (total + 255) / 256 < 4096 ? (total + 255) / 256 : 4096;
-    hipLaunchKernelGGL(output_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, noisy, res_f32, out,
-                       target, (uint16_t *)dres, loss, total);
-    return hipGetLastError() == hipSuccess ? ISP_PILOT_OK : ISP_PILOT_E_HIP;
+    return isp_pilot_output_planes(noisy, res_f32, out, target, dres, loss, 1, n, h, w, stream);
 }
 
 }  // extern "C"

@@ -19,8 +19,14 @@ from __future__ import annotations
 
 import numpy as np
 
-# (cin, cout, relu) of the 8 layers: noisy + sigma map in, residual out
-LAYERS = [(2, 48, True)] + [(48, 48, True)] * 6 + [(48, 1, False)]
+
+def layers(channels: int = 1):
+    """(cin, cout, relu) of the 8 layers of a network on images of `channels`
+    planes: the planes + a sigma map in, one residual per plane out."""
+    return [(channels + 1, 48, True)] + [(48, 48, True)] * 6 + [(48, channels, False)]
+
+
+LAYERS = layers(1)
 
 
 def bf16_round(a: np.ndarray) -> np.ndarray:
@@ -63,46 +69,62 @@ def conv3x3_dinput(dz: np.ndarray, w: np.ndarray) -> np.ndarray:
     return conv3x3(dz, np.ascontiguousarray(wf))
 
 
-def net_input(noisy: np.ndarray, sigma: np.ndarray) -> np.ndarray:
-    """[N,H,W] noisy planes and sigma per image (0..255) -> [N,H,W,2]."""
-    s = np.broadcast_to((np.asarray(sigma, np.float64) / 255.0)[:, None, None], noisy.shape)
-    return np.stack([noisy, s], axis=-1).astype(np.float64)
+def planes(noisy) -> np.ndarray:
+    """[N,H,W] (one plane per image) or [N,C,H,W] -> [N,C,H,W] float64."""
+    a = np.asarray(noisy, np.float64)
+    return a[:, None] if a.ndim == 3 else a
 
 
-def forward(params, noisy, sigma, bf16: bool = False):
+def net_input(noisy: np.ndarray, sigma: np.ndarray, offset=None) -> np.ndarray:
+    """Noisy planes [N,H,W] or [N,C,H,W] (plus offset [N,C] per plane, or
+    None) and sigma per image (0..255) -> [N,H,W,C+1]."""
+    x = planes(noisy)
+    if offset is not None:
+        x = x + np.asarray(offset, np.float64).reshape(x.shape[0], x.shape[1])[:, :, None, None]
+    N, C, H, W = x.shape
+    s = np.broadcast_to((np.asarray(sigma, np.float64) / 255.0)[:, None, None, None], (N, 1, H, W))
+    return np.moveaxis(np.concatenate([x, s], axis=1), 1, -1)
+
+
+def forward(params, noisy, sigma, bf16: bool = False, offset=None):
     """params: list of (w [cout,cin,9], b [cout]).  Returns (out, acts, r)
-    with acts[0] the input and acts[l+1] the output of layer l."""
+    with acts[0] the input and acts[l+1] the output of layer l; out and r
+    have the shape of noisy."""
     rnd = bf16_round if bf16 else (lambda a: a)
-    a = rnd(net_input(noisy, sigma))
+    C = params[-1][0].shape[0]
+    a = rnd(net_input(noisy, sigma, offset))
     acts = [a]
     r = None
-    for (w, b), (_, _, relu) in zip(params, LAYERS):
+    for (w, b), (_, _, relu) in zip(params, layers(C)):
         z = conv3x3(a, rnd(w), b)
         if relu:
             a = rnd(np.maximum(z, 0.0))
             acts.append(a)
         else:
-            r = z[..., 0]
+            r = np.moveaxis(z, -1, 1)  # [N,C,H,W]
+    r = r.reshape(np.shape(noisy))
     out = np.asarray(noisy, np.float64) - r
     return out, acts, r
 
 
-def loss(params, noisy, sigma, target, bf16: bool = False) -> float:
-    out, _, _ = forward(params, noisy, sigma, bf16)
+def loss(params, noisy, sigma, target, bf16: bool = False, offset=None) -> float:
+    out, _, _ = forward(params, noisy, sigma, bf16, offset)
     return float(np.mean((out - target) ** 2))
 
 
-def loss_and_grads(params, noisy, sigma, target, bf16: bool = False):
+def loss_and_grads(params, noisy, sigma, target, bf16: bool = False, offset=None):
     """Loss and [(dw, db)] per layer (backward of forward())."""
     rnd = bf16_round if bf16 else (lambda a: a)
-    out, acts, _ = forward(params, noisy, sigma, bf16)
+    C = params[-1][0].shape[0]
+    out, acts, _ = forward(params, noisy, sigma, bf16, offset)
     d = out - target
     L = float(np.mean(d ** 2))
-    g = rnd((-2.0 * d / d.size)[..., None])     # d loss / d r, [N,H,W,1]
+    g = rnd(np.moveaxis(planes(-2.0 * d / d.size), 1, -1))     # d loss / d r, [N,H,W,C]
     grads = [None] * len(params)
+    net = layers(C)
     for l in range(len(params) - 1, -1, -1):
         w, _ = params[l]
-        _, _, relu = LAYERS[l]
+        _, _, relu = net[l]
         dz = g * (acts[l + 1] > 0) if relu else g
         grads[l] = conv3x3_wgrad(acts[l], dz)
         if l > 0:

@@ -368,10 +368,10 @@ def test_int_degenerate_copy_through():
 
 
 # --- the learned pilot's FP64 oracle (oracle/pilot_oracle.py)
-def _pilot_params(rng, scale=1.0):
+def _pilot_params(rng, scale=1.0, channels=1):
     from oracle import pilot_oracle as po
     out = []
-    for cin, cout, relu in po.LAYERS:
+    for cin, cout, relu in po.layers(channels):
         out.append((rng.standard_normal((cout, cin, 9)) * np.sqrt(2.0 / (cin * 9)) * scale,
                     rng.standard_normal(cout) * 0.05))
     return out
@@ -404,14 +404,18 @@ def test_pilot_conv_adjoint_identity():
     assert abs(lhs - rhs) < 1e-9 * max(1.0, abs(lhs))
 
 
-def test_pilot_gradients_match_finite_differences():
+@pytest.mark.parametrize("channels", [1, 3])
+def test_pilot_gradients_match_finite_differences(channels):
     from oracle import pilot_oracle as po
-    rng = np.random.default_rng(2)
-    params = _pilot_params(rng)
-    noisy = rng.uniform(0, 1, (1, 5, 6))
+    rng = np.random.default_rng(2 + channels)
+    params = _pilot_params(rng, channels=channels)
+    shape = (1, 5, 6) if channels == 1 else (1, channels, 5, 6)
+    noisy = rng.uniform(0, 1, shape)
     sigma = np.array([25.0])
-    target = rng.uniform(0, 1, (1, 5, 6))
-    _, grads = po.loss_and_grads(params, noisy, sigma, target)
+    target = rng.uniform(0, 1, shape)
+    off = None if channels == 1 else np.array([[0.0, 0.5, 0.5]])
+    L0, grads = po.loss_and_grads(params, noisy, sigma, target, offset=off)
+    assert abs(L0 - po.loss(params, noisy, sigma, target, offset=off)) < 1e-15
     h = 1e-6
     for l in range(len(params)):
         w, b = params[l]
@@ -420,15 +424,46 @@ def test_pilot_gradients_match_finite_differences():
             wm = [(ww.copy(), bb.copy()) for ww, bb in params]
             wp[l][0][idx] += h
             wm[l][0][idx] -= h
-            fd = (po.loss(wp, noisy, sigma, target) - po.loss(wm, noisy, sigma, target)) / (2 * h)
+            fd = (po.loss(wp, noisy, sigma, target, offset=off) - po.loss(wm, noisy, sigma, target, offset=off)) / (2 * h)
             assert abs(fd - grads[l][0][idx]) <= 1e-6 * max(1.0, abs(fd)) + 1e-9, (l, idx, fd, grads[l][0][idx])
         j = int(rng.integers(0, b.shape[0]))
         bp = [(ww.copy(), bb.copy()) for ww, bb in params]
         bm = [(ww.copy(), bb.copy()) for ww, bb in params]
         bp[l][1][j] += h
         bm[l][1][j] -= h
-        fd = (po.loss(bp, noisy, sigma, target) - po.loss(bm, noisy, sigma, target)) / (2 * h)
+        fd = (po.loss(bp, noisy, sigma, target, offset=off) - po.loss(bm, noisy, sigma, target, offset=off)) / (2 * h)
         assert abs(fd - grads[l][1][j]) <= 1e-6 * max(1.0, abs(fd)) + 1e-9, (l, j, fd, grads[l][1][j])
+
+
+def test_pilot_multichannel_equals_separate_single_channel_parts():
+    # a 3-plane network whose weights couple no planes (block-diagonal first
+    # and last layers over three disjoint groups of 16 hidden channels, zero
+    # cross terms) computes three independent single-plane networks
+    from oracle import pilot_oracle as po
+    rng = np.random.default_rng(9)
+    C, H, W = 3, 6, 7
+    noisy = rng.uniform(0, 1, (2, C, H, W))
+    sigma = np.array([20.0, 35.0])
+    grey = _pilot_params(rng, channels=1)
+    rgb = []
+    for l, (cin, cout, relu) in enumerate(po.layers(C)):
+        w = np.zeros((cout, cin, 9))
+        b = np.zeros(cout)
+        for c in range(C):
+            o = slice(16 * c, 16 * c + 16) if cout == 48 else slice(c, c + 1)
+            i = slice(16 * c, 16 * c + 16) if cin == 48 else [c, C]
+            gw, gb = grey[l]
+            go = slice(0, 16) if cout == 48 else slice(0, 1)
+            gi = slice(0, 16) if cin == 48 else [0, 1]
+            w[o, i] = gw[go][:, gi]
+            b[o] = gb[go]
+        rgb.append((w, b))
+    out3, _, _ = po.forward(rgb, noisy, sigma)
+    for c in range(C):
+        g16 = [(w[:16] if w.shape[0] == 48 else w[:1], b[:16] if b.shape[0] == 48 else b[:1]) for w, b in grey]
+        g16 = [(w[:, :16] if w.shape[1] == 48 else w, b) for w, b in g16]
+        out1, _, _ = po.forward(g16, noisy[:, c], sigma)
+        np.testing.assert_allclose(out3[:, c], out1, atol=1e-12)
 
 
 def test_bf16_round_known_values():
