@@ -8,12 +8,12 @@ gratings, channels mixed, AWGN sigma 25/255).  Parameters: BlockSize 8,
 BlockStep 3 (both steps), BMrange 12, BMstep 1, GroupSize 16 / 32, lambda 2.7,
 thMSE from thMSE_Default (profile "np" otherwise).
 
-One step = one two-step BM3D of the frame with input and output resident in
-HBM.  The value's timed region runs K frames with --streams of them in flight
-(default 2: one context and one default-priority stream each, as a frame
-sequence is processed), without instrumentation; the same K frames one after
-another and once more with per-stage HIP events follow, the latter giving the
-stage times and the roofline.  Prints one JSON line (rank 0, one GPU).
+One step = two-step BM3D of a batch of --frames-per-step frames (default 24)
+with inputs and outputs resident in HBM, processed as a frame sequence is:
+--streams of them in flight (default 2: one context and one default-priority
+stream each), without instrumentation.  The same frames one after another,
+and once more with per-stage HIP events (stage times and the roofline),
+follow.  Prints one JSON line (rank 0, one GPU).
 """
 from __future__ import annotations
 
@@ -63,32 +63,35 @@ def transform_flops(n) -> np.ndarray:
     return 2.0 * B * B * n * (2 * B + n)
 
 
-def cpu_baseline(para, clean_rgb_noisy, threads: int, target_s: float):
-    """Time the oracle (FP64 C restatement, multi-threaded row bands) on a crop."""
+def cpu_baseline(para, clean_rgb_noisy, threads: int, target_s: float, runs: int = 3):
+    """Time the oracle (FP64 C restatement, row bands over `threads` threads)
+    on a crop of the frame sized to about target_s per run; median of `runs`."""
     import oracle
     fast = oracle.host_supports_fast()
     oracle.set_threads(threads, fast=fast)
     noisy = clean_rgb_noisy
     # calibrate on a small crop, then size the timed crop to ~target_s per run
-    hc = 64
+    hc = 32 if threads == 1 else 64
     t0 = time.perf_counter()
     oracle.bm3d_rgb(para.basic, para.final, noisy[:, :hc, :], fast=fast)
     rate = W * hc / max(time.perf_counter() - t0, 1e-6)
-    hc = int(min(H, max(64, rate * target_s / W)))
+    hc = int(min(H, max(32, rate * target_s / W)))
     times = []
-    for _ in range(2):
+    for _ in range(runs):
         t0 = time.perf_counter()
         oracle.bm3d_rgb(para.basic, para.final, noisy[:, :hc, :], fast=fast)
         times.append(time.perf_counter() - t0)
-    dt = min(times)
+    dt = float(np.median(times))
+    oracle.set_threads(1, fast=fast)
     return {
         "value": round(W * hc / dt / 1e6, 4),
         "unit": "Mpix/s",
         "cores": threads,
         "kind": "port",
         "sample": f"oracle (FP64 C restatement, {'x86-64-v3' if fast else 'baseline x86-64'} build, "
-                  f"{threads} threads, row bands) two-step BM3D on the top {W}x{hc} crop of the same "
-                  f"frame, best of 2 runs of {dt:.2f} s",
+                  f"{threads} thread{'s, row bands' if threads > 1 else ''}) two-step BM3D on the top "
+                  f"{W}x{hc} crop of the same frame, median of {runs} runs "
+                  f"({', '.join(f'{t:.2f}' for t in times)} s)",
     }
 
 
@@ -109,6 +112,33 @@ def measured_traffic(prec: str, kernel: str):
             if len(f) == 4 and f[0] == "traffic" and f[1] == kernel:
                 return {"bytes": float(f[2]) + float(f[3]), "fetch_bytes": float(f[2]),
                         "write_bytes": float(f[3]), "source": os.path.relpath(path, os.path.dirname(TRAFFIC_FILE))}
+    return None
+
+
+STATS_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                          "bm3d_1080p_rgb_{prec}_kernel_stats.txt")
+# kernel-name fragments of each stage's kernel in the committed rocprofv3 stats
+STAGE_KERNELS = {
+    "fp32": {"filter_final": "filter_kernel<true, false>", "filter_basic": "filter_kernel<false, false>",
+             "match_basic": "match_kernel<double, double, true, true, false, false>",
+             "match_final": "match_kernel<float, double, true, true, false, false>"},
+    "mixed": {"filter_final": "filter_mfma_kernelIDF16_Lb1ELb0E", "filter_basic": "filter_mfma_kernelIDF16_Lb0ELb0E",
+              "match_basic": "match_walk_kernel<double, true, false", "match_final": "match_walk_kernel<float, true, false"},
+}
+
+
+def profiled_ms(prec: str, stage: str):
+    """Average duration (ms) of a stage's kernel in the committed rocprofv3
+    kernel-trace summary of the same form, or None."""
+    path = STATS_FILE.format(prec=prec)
+    frag = STAGE_KERNELS["fp32" if prec == "fp32" else "mixed"].get(stage)
+    if prec == "bf16" or frag is None or not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith('"') and frag in line:
+                f = line.rsplit('",', 1)[1].split(",")
+                return round(float(f[2]) / 1e6, 4)
     return None
 
 
@@ -134,7 +164,7 @@ def measured_util(prec: str, kernel: str):
 
 
 def pipelined_steps(fs, ys, x, steps: int, warmup: int):
-    """K frames with len(fs) of them in flight: frame i runs on context
+    """`steps` frames with len(fs) of them in flight: frame i runs on context
     i % n and its own stream (default priority), without stage events.
     Returns seconds per frame."""
     import torch
@@ -744,6 +774,9 @@ def main() -> int:
     ap.add_argument("--tiles", default="", help="mx,my,fx,fy: given tile shapes of the timed form (no autotuning; "
                                                 "for profiler runs, whose averages the tuner's trials would mix)")
     ap.add_argument("--cpu-seconds", type=float, default=4.0, help="target seconds per CPU baseline run")
+    ap.add_argument("--frames-per-step", type=int, default=24,
+                    help="frames of the headline workload per step (a step times a batch of frames, so that "
+                         "the default 20 steps keep the GPU busy for about a second)")
     ap.add_argument("--streams", type=int, default=2,
                     help="frames in flight in the timed region of the headline workload, one context and one "
                          "stream each (1: one frame after another)")
@@ -811,11 +844,13 @@ def main() -> int:
         for g in fs[1:]:
             g.set_tuning(g.size_class(W, H), {k: v for k, v in tuning.items() if k != "given"})
     ys = [y] + [torch.empty_like(x) for _ in range(nstr - 1)]
-    dt = pipelined_steps(fs, ys, x, args.steps, args.warmup)
+    F = max(1, args.frames_per_step)
+    nfr = args.steps * F
+    dt = pipelined_steps(fs, ys, x, nfr, args.warmup)          # seconds per frame
     value = W * H / dt / 1e6
     contexts_equal = all(bool(torch.equal(ys[0], o)) for o in ys[1:])
-    dt1 = pipelined_steps(fs[:1], ys[:1], x, args.steps, 1) if nstr > 1 else dt
-    dti, stages = timed_steps(f, y, x, args.steps, 1)
+    dt1 = pipelined_steps(fs[:1], ys[:1], x, nfr, 1) if nstr > 1 else dt
+    dti, stages = timed_steps(f, y, x, nfr, 1)
     out = y.cpu().numpy()
     psnr_out = synth.psnr(out, clean)
     psnr_in = synth.psnr(noisy, clean)
@@ -827,7 +862,7 @@ def main() -> int:
         if args.tune and not args.tiles:
             f32.autotune(W, H, SIGMA, rgb=True)
         y32 = torch.empty_like(x)
-        dt32, st32 = timed_steps(f32, y32, x, max(2, args.steps // 2), 1)
+        dt32, st32 = timed_steps(f32, y32, x, max(2, nfr // 4), 1)
         ref32 = y32.cpu().numpy()
         quality_extra = {"psnr_fp32_db": round(synth.psnr(ref32, clean), 4),
                          "psnr_loss_vs_fp32_db": round(synth.psnr(ref32, clean) - psnr_out, 4),
@@ -863,6 +898,7 @@ def main() -> int:
     ms = stages[dom][0] / stages[dom][1]
     if dom.startswith("filter"):
         flops = filt2_actual if dom == "filter_final" else filt1_actual
+        full = filt2_full if dom == "filter_final" else refs * 3 * 2 * transform_flops(16)
         achieved = flops / (ms * 1e-3) / 1e12
         peak = PEAK_FP32_TFLOPS if args.precision == "fp32" else PEAK_F16_TFLOPS
         roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
@@ -870,7 +906,12 @@ def main() -> int:
                 "ms_per_launch": round(ms, 4),
                 "engine": "fp32 VALU" if args.precision == "fp32" else f"{args.precision} MFMA 32x32x16",
                 "algorithmic_flops_per_launch": flops,
-                "flops_basis": "dense separable 2*B^2*N*(2B+N) per transform at each group's actual N"}
+                "flops_basis": "dense separable 2*B^2*N*(2B+N) per transform at each group's actual N",
+                # the same count with every group at its full GroupSize (the work-count basis that
+                # states the path's work independently of the image)
+                "achieved_full_n": round(full / (ms * 1e-3) / 1e12, 3),
+                "frac_full_n": round(full / (ms * 1e-3) / 1e12 / peak, 4),
+                "algorithmic_flops_per_launch_full_n": full}
         util = measured_util(args.precision, dom)
         if args.precision != "fp32" and util is not None:
             # issued matrix work: the counted 32x32x16 MFMAs of one launch
@@ -888,6 +929,12 @@ def main() -> int:
     if util is not None:
         # counter-measured pipe utilisation of the same kernel and form
         roof["pipe_utilisation"] = util
+    rp = profiled_ms(args.precision, dom)
+    if rp is not None:
+        # the committed rocprofv3 kernel-trace average of the same kernel: it times
+        # a short profiled run (one frame at a time), the events time this run
+        roof["rocprof_ms_per_launch"] = rp
+        roof["rocprof_over_event_ms"] = round(rp / ms, 3)
     tr = measured_traffic(args.precision, dom)
     if tr is not None:
         roof["traffic"] = tr["bytes"]
@@ -901,10 +948,12 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(dt * 1e3, 3),
+        "ms_per_step": round(dt * F * 1e3, 3),
+        "frames_per_step": F,
+        "ms_per_frame": round(dt * 1e3, 4),
         "frames_in_flight": nstr,
-        "one_frame_at_a_time": {"value": round(W * H / dt1 / 1e6, 3), "ms_per_step": round(dt1 * 1e3, 3)},
-        "instrumented": {"value": round(W * H / dti / 1e6, 3), "ms_per_step": round(dti * 1e3, 3),
+        "one_frame_at_a_time": {"value": round(W * H / dt1 / 1e6, 3), "ms_per_frame": round(dt1 * 1e3, 4)},
+        "instrumented": {"value": round(W * H / dti / 1e6, 3), "ms_per_frame": round(dti * 1e3, 4),
                          "note": "per-stage HIP events on the launch stream; the stage times and the roofline"},
         "contexts_equal": contexts_equal,
         "higher_is_better": True,
@@ -913,8 +962,8 @@ def main() -> int:
         "dtype": "fp32" if args.precision == "fp32" else f"{args.precision}+fp32",
         "data": "synthetic (seeded rectangles+gradient+gratings, mixed RGB, AWGN sigma 25/255)",
         "config": {"workload": "two-step BM3D, 1920x1080 RGB float32, sigma=25, OPP colour",
-                   "block": 8, "block_step": [3, 3], "bm_range": 12, "bm_step": 1, "group_size": [16, 32],
-                   "lambda": 2.7, "precision": args.precision},
+                   "frames_per_step": F, "block": 8, "block_step": 3, "bm_range": 12, "bm_step": 1,
+                   "group_size": 32, "group_size_basic": 16, "lambda": 2.7, "precision": args.precision},
         "roofline": roof,
         "quality": {"psnr_noisy_db": round(psnr_in, 3), "psnr_out_db": round(psnr_out, 3),
                     "mean_group_size": [round(float(n1.mean()), 3), round(float(n2.mean()), 3)],
@@ -932,6 +981,9 @@ def main() -> int:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
         res["cpu_baseline"] = cpu_baseline(para, noisy, threads, args.cpu_seconds)
         res["gpu_over_cpu"] = round(value / res["cpu_baseline"]["value"], 1)
+        # the reference's own behaviour: one thread (source/BM3D.cpp runs single-threaded)
+        res["cpu_baseline_single_thread"] = cpu_baseline(para, noisy, 1, args.cpu_seconds)
+        res["gpu_over_cpu_single_thread"] = round(value / res["cpu_baseline_single_thread"]["value"], 1)
     print(json.dumps(res), flush=True)
     return 0
 

@@ -63,12 +63,15 @@ def test_committed_profiles_parse(prec, stage):
     assert u is not None and 0 < u["valu_busy"] <= 1 and 0 <= u["mfma_busy"] <= 1
     assert (u["mfma_insts_per_launch"] > 0) == (prec == "fp16" and stage.startswith("filter"))
     assert u["ms_under_counters"] > 0
+    ms = bench.profiled_ms(prec, stage)
+    assert ms is not None and 0.05 < ms < 50
 
 
 def test_unknown_stage_reads_nothing():
     assert bench.measured_traffic("fp16", "no_such_stage") is None
     assert bench.measured_util("fp16", "no_such_stage") is None
     assert bench.measured_util("int8", "filter_final") is None
+    assert bench.profiled_ms("fp16", "no_such_stage") is None
 
 
 def test_more_than_one_gpu_is_refused():
