@@ -197,6 +197,25 @@ int isp_bm3d_last_groups(isp_bm3d_ctx *ctx, int final_step, int *counts, int *po
 int isp_bm3d_grid_size(const isp_bm3d_ctx *ctx, int width, int height, int final_step,
                        int *ngx, int *ngy);
 
+/* What a call of this size would do, computed on the host alone (no device,
+ * no context; no reference counterpart): per step [basic, final], whether it
+ * copies src through (degenerate parameters), whether it runs the generic
+ * kernels (BlockSize != 8 or GroupSize > 32), its reference grid and its
+ * Wiener weight cap; the device workspace a plane / frame call of this size
+ * allocates, and the scratch isp_bm3d_plane_f32_noalloc needs (stride =
+ * width) against the scratch it has.  Returns ISP_BM3D_OK, or the status the
+ * filter calls would return for these parameters and sizes. */
+typedef struct isp_bm3d_plan_info {
+    int degenerate[2];
+    int generic[2];
+    int ngx[2], ngy[2];
+    double weight_cap[2];
+    size_t workspace_bytes;
+    size_t noalloc_bytes;
+    size_t noalloc_capacity;
+} isp_bm3d_plan_info;
+int isp_bm3d_plan(const isp_bm3d_params *params, int width, int height, int nplanes, isp_bm3d_plan_info *info);
+
 /* Headroom of the aggregation sums (no reference counterpart: the reference
  * sums in FP64).  num and den are 64-bit fixed point with 36 fraction bits.
  * For a step's parameters and a temporal radius (0 for single frames):

@@ -10,6 +10,9 @@ import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libisp_bm3d.so")
+# ISP_BM3D_LIB names another build of the same library (the host-sanitized
+# libisp_bm3d_san.so of the sanitizer tests); it must exist like the default
+LIB_PATH = os.environ.get("ISP_BM3D_LIB") or LIB_PATH
 
 OK = 0
 E_ARG, E_UNSUPPORTED, E_SIZE, E_HIP, E_PROFILE = -1, -2, -3, -4, -5
@@ -35,6 +38,7 @@ EXPORTS = (
     "isp_bm3d_match",
     "isp_bm3d_grid_size",
     "isp_bm3d_aggregation_limits",
+    "isp_bm3d_plan",
     "isp_bm3d_set_timing",
     "isp_bm3d_get_timing",
     "isp_bm3d_reset_timing",
@@ -76,6 +80,21 @@ class StepParams(ctypes.Structure):
         ("bm_step", ctypes.c_int),
         ("thMSE", ctypes.c_double),
         ("lambda_", ctypes.c_double),
+    ]
+
+
+class PlanInfo(ctypes.Structure):
+    """isp_bm3d_plan_info."""
+
+    _fields_ = [
+        ("degenerate", ctypes.c_int * 2),
+        ("generic", ctypes.c_int * 2),
+        ("ngx", ctypes.c_int * 2),
+        ("ngy", ctypes.c_int * 2),
+        ("weight_cap", ctypes.c_double * 2),
+        ("workspace_bytes", ctypes.c_size_t),
+        ("noalloc_bytes", ctypes.c_size_t),
+        ("noalloc_capacity", ctypes.c_size_t),
     ]
 
 
@@ -173,6 +192,7 @@ def lib() -> ctypes.CDLL:
     L.isp_bm3d_grid_size.argtypes = [vp, i, i, i, ctypes.POINTER(i), ctypes.POINTER(i)]
     L.isp_bm3d_aggregation_limits.argtypes = [sp, i, ctypes.POINTER(ctypes.c_double),
                                               ctypes.POINTER(ctypes.c_double)]
+    L.isp_bm3d_plan.argtypes = [pp, i, i, i, ctypes.POINTER(PlanInfo)]
     L.isp_bm3d_set_timing.argtypes = [vp, i]
     L.isp_bm3d_get_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_long), i]

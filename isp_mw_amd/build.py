@@ -18,6 +18,13 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libisp_bm3d.so")
 OBJDIR = os.path.join(LIBDIR, "_obj")  # per-source objects (git-ignored *.o)
 CLI = os.path.join(LIBDIR, "isp_mw")
+# host-side AddressSanitizer + UndefinedBehaviorSanitizer builds for the CPU
+# tests (tests/test_sanitizers.py): the front end, and the library whose C-ABI
+# host code (bm3d_capi.cpp) is instrumented; device code is never instrumented
+CLI_SAN = os.path.join(LIBDIR, "isp_mw_san")
+LIB_SAN = os.path.join(LIBDIR, "libisp_bm3d_san.so")
+HOST_SANITIZE = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+                 "-Xarch_host", "-fno-sanitize-recover=undefined", "-Xarch_host", "-fno-omit-frame-pointer"]
 CLI_SRC = os.path.join(HERE, "cli", "isp_mw.cpp")
 
 SOURCES = ["bm3d_kernels.hip", "nlm_kernels.hip", "stack_kernels.hip", "bm3d_capi.cpp", "pilot.hip", "scratch.hip",
@@ -60,9 +67,20 @@ def build(verbose: bool = False) -> str:
             raise RuntimeError(f"hipcc failed compiling {src}")
         return obj
 
-    jobs = max(1, min(len(SOURCES), int(os.environ.get("MAX_JOBS", "0") or 0) or (os.cpu_count() or 1)))
+    def compile_capi_san() -> str:
+        obj = os.path.join(OBJDIR, "bm3d_capi_san.o")
+        cmd = [hipcc(), *compile_flags, *HOST_SANITIZE, "-g", "-c", "-o", obj, os.path.join(CSRC, "bm3d_capi.cpp")]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError("hipcc failed compiling the sanitized bm3d_capi.cpp")
+        return obj
+
+    jobs = max(1, min(len(SOURCES) + 1, int(os.environ.get("MAX_JOBS", "0") or 0) or (os.cpu_count() or 1)))
     with concurrent.futures.ThreadPoolExecutor(jobs) as ex:
+        san = ex.submit(compile_capi_san)
         objs = list(ex.map(compile_one, SOURCES))
+        san_obj = san.result()
     tmp = LIB + ".tmp"
     cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-Wl,--no-undefined", "-o", tmp, *objs]
     if verbose:
@@ -72,13 +90,27 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("hipcc failed linking libisp_bm3d.so")
     os.replace(tmp, LIB)
+    # the same library with the sanitized host code; the sanitizer runtime
+    # comes from the process (preloaded), so its symbols stay undefined here
+    tmp = LIB_SAN + ".tmp"
+    objs_san = [san_obj if o.endswith("bm3d_capi.o") else o for o in objs]
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs_san]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("hipcc failed linking libisp_bm3d_san.so")
+    os.replace(tmp, LIB_SAN)
     build_cli(verbose)
+    build_cli(verbose, sanitize=True)
     return LIB
 
 
-def build_cli(verbose: bool = False) -> str:
-    tmp = CLI + ".tmp"
-    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), CLI_SRC,
+def build_cli(verbose: bool = False, sanitize: bool = False) -> str:
+    out = CLI_SAN if sanitize else CLI
+    tmp = out + ".tmp"
+    flags = ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
+             "-fno-sanitize-recover=undefined"] if sanitize else ["-O2"]
+    cmd = ["g++", *flags, "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), CLI_SRC,
            "-L", LIBDIR, "-lisp_bm3d", "-lz", "-Wl,-rpath,$ORIGIN", "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
@@ -86,8 +118,8 @@ def build_cli(verbose: bool = False) -> str:
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("g++ failed building isp_mw")
-    os.replace(tmp, CLI)
-    return CLI
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

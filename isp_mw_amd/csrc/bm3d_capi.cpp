@@ -2125,6 +2125,58 @@ int rgb_common(isp_bm3d_ctx *ctx, void *const *dst, const void *const *src, cons
 
 extern "C" {
 
+int isp_bm3d_plan(const isp_bm3d_params *params, int W, int H, int nplanes, isp_bm3d_plan_info *info)
+{
+    if (!params || !info || (nplanes != 1 && nplanes != 3))
+        return fail(nullptr, ISP_BM3D_E_ARG, "plan: null argument or nplanes %d", nplanes);
+    if (W <= 0 || H <= 0) return fail(nullptr, ISP_BM3D_E_SIZE, "plan: empty image %dx%d", W, H);
+    std::memset(info, 0, sizeof(*info));
+    // a context holding only the parameters: derive() reads them and whether
+    // generic tables would exist (their contents are never read here)
+    isp_bm3d_ctx c;
+    c.params = *params;
+    static const double no_table[1] = {0.0};
+    for (int k = 0; k < 2; ++k) {
+        const isp_bm3d_step_params &q = k ? c.params.final_ : c.params.basic;
+        if ((q.block_size != kBlock || q.group_size > kMaxGroup) && q.block_size >= 1 &&
+            q.block_size <= kGenMaxBlock && q.group_size >= 1 && q.group_size <= kGenMaxGroup)
+            c.d_gen[k] = const_cast<double *>(no_table);
+    }
+    for (int k = 0; k < 2; ++k) {
+        Derived d{};
+        const int rc = derive(&c, k ? c.params.final_ : c.params.basic, nplanes, W, H, d);
+        c.d_gen[k] = nullptr;
+        if (rc) {
+            g_create_err = c.err;
+            for (int j = 0; j < 2; ++j) c.d_gen[j] = nullptr;
+            return rc;
+        }
+        info->degenerate[k] = d.degenerate;
+        info->generic[k] = !d.degenerate && d.generic;
+        info->ngx[k] = d.degenerate ? 0 : d.ngx;
+        info->ngy[k] = d.degenerate ? 0 : d.ngy;
+        info->weight_cap[k] = d.degenerate ? 0.0 : (double)d.wmax;
+    }
+    for (int j = 0; j < 2; ++j) c.d_gen[j] = nullptr;
+    size_t refs = 1;
+    for (const isp_bm3d_step_params *p : {&c.params.basic, &c.params.final_}) {
+        const int s = p->block_step > 0 ? p->block_step : 1;
+        refs = std::max(refs, (size_t)(W / s + 2) * (size_t)(H / s + 2));
+    }
+    Work w{};
+    info->workspace_bytes = layout(&w, nullptr, ((W + 63) / 64) * 64, H, refs, max_group(&c), false);
+    size_t nrefs = 1;
+    for (const isp_bm3d_step_params *p : {&c.params.basic, &c.params.final_}) {
+        if (p->block_step <= 0) continue;
+        const int B = p->block_size > 0 ? p->block_size : kBlock;
+        nrefs = std::max(nrefs, ((size_t)std::max(W - B, 0) / p->block_step + 2) *
+                                    ((size_t)std::max(H - B, 0) / p->block_step + 2));
+    }
+    info->noalloc_bytes = noalloc_layout(&w, nullptr, W, H, nrefs, max_group(&c));
+    info->noalloc_capacity = kScratchBytes;
+    return ISP_BM3D_OK;
+}
+
 int isp_bm3d_plane_f32(isp_bm3d_ctx *ctx, float *dst, const float *src, const float *ref, int W, int H,
                        size_t stride, int mode, int mem, void *stream)
 {

@@ -83,7 +83,12 @@ def nlm_params(correction=True, sigma=8.0, strength=None, BlockSize=8, BlockStep
 
 
 def _lib(fast: bool = False) -> ctypes.CDLL:
-    name = "liborc_fast.so" if fast else "liborc.so"
+    # ISP_ORACLE_SANITIZED=1: the ASan/UBSan build for both (the process must
+    # preload the sanitizer runtime; tests/test_sanitizers.py)
+    if os.environ.get("ISP_ORACLE_SANITIZED") == "1":
+        name = "liborc_san.so"
+    else:
+        name = "liborc_fast.so" if fast else "liborc.so"
     if name in _libs:
         return _libs[name]
     path = os.path.join(_BUILD, name)

@@ -77,3 +77,34 @@ def test_batch_and_tuning_argument_checks():
     assert lib.isp_bm3d_size_class(257, 256) == 17
     assert lib.isp_bm3d_size_class(1920, 1080) == 21
     assert lib.isp_bm3d_size_class(0, 5) == L.E_ARG
+
+
+def test_plan_on_the_host():
+    """isp_bm3d_plan: grid sizes, generic steps and the no-alloc scratch without a GPU."""
+    lib = L.lib()
+    p = L.Params()
+    info = L.PlanInfo()
+    assert lib.isp_bm3d_params_default(ctypes.byref(p), b"np") == L.OK
+    for st in (p.basic, p.final_):
+        st.block_step, st.bm_range = 3, 12
+    p.basic.group_size, p.final_.group_size = 16, 32
+    assert lib.isp_bm3d_plan(ctypes.byref(p), 1920, 1080, 3, ctypes.byref(info)) == L.OK
+    assert (info.ngx[0], info.ngy[0], info.ngx[1], info.ngy[1]) == (639, 359, 639, 359)
+    assert list(info.generic) == [0, 0] and list(info.degenerate) == [0, 0]
+    assert info.weight_cap[1] == 2048.0
+    assert info.noalloc_bytes <= info.noalloc_capacity == 96 << 20  # a 1080p plane fits the scratch
+    assert lib.isp_bm3d_plan(ctypes.byref(p), 3840, 2160, 1, ctypes.byref(info)) == L.OK
+    assert info.noalloc_bytes > info.noalloc_capacity
+    assert lib.isp_bm3d_params_default(ctypes.byref(p), b"vn") == L.OK
+    assert lib.isp_bm3d_plan(ctypes.byref(p), 640, 480, 1, ctypes.byref(info)) == L.OK
+    assert list(info.generic) == [0, 1]          # final BlockSize 11: the generic kernels
+    assert info.ngx[1] == (640 - 11) // 6 + 1 + (1 if (640 - 11) % 6 else 0)
+    p.final_.block_size = 17
+    assert lib.isp_bm3d_plan(ctypes.byref(p), 640, 480, 1, ctypes.byref(info)) == L.E_UNSUPPORTED
+    assert b"BlockSize 17" in lib.isp_bm3d_last_error(None)
+    p.final_.block_size = 11
+    assert lib.isp_bm3d_plan(ctypes.byref(p), 16, 40000, 1, ctypes.byref(info)) == L.E_UNSUPPORTED
+    assert lib.isp_bm3d_plan(ctypes.byref(p), 10, 10, 1, ctypes.byref(info)) == L.E_SIZE
+    p.basic.sigma[0] = p.final_.sigma[0] = 0.0
+    assert lib.isp_bm3d_plan(ctypes.byref(p), 10, 10, 1, ctypes.byref(info)) == L.OK
+    assert list(info.degenerate) == [1, 1]       # sigma 0: both steps copy through
