@@ -85,7 +85,8 @@ int isp_bm3d_thmse_default(isp_bm3d_step_params *p, int final_step);
 /* Creates a filter context on HIP device `device` (BM3D(const BM3D_Para&),
  * include/BM3D.h:349-351; tables of source/BM3D.cpp:9-73).  The context owns
  * its device tables and workspace; it is immutable after creation except for
- * the precision set with isp_bm3d_set_precision(). */
+ * the precision (isp_bm3d_set_precision), the variant modes
+ * (isp_bm3d_set_variant), tile tuning and timing. */
 int isp_bm3d_create(isp_bm3d_ctx **ctx, const isp_bm3d_params *params, int device);
 int isp_bm3d_destroy(isp_bm3d_ctx *ctx);
 
@@ -95,6 +96,22 @@ const char *isp_bm3d_last_error(const isp_bm3d_ctx *ctx);
 
 /* Transform arithmetic; ISP_BM3D_PREC_FP32 by default. */
 int isp_bm3d_set_precision(isp_bm3d_ctx *ctx, int precision);
+
+/* Variant modes of the filter, off by default (flags 0: the reference's
+ * semantics).  No reference counterpart: the reference transforms the group
+ * axis with a DCT of the group's length and aggregates each block with one
+ * weight (source/BM3D.cpp:14-24, 378-384, 450-456).
+ *   ISP_BM3D_VARIANT_HAAR: the orthonormal Haar transform along the group
+ *     axis, each group truncated to the largest power of two of its members;
+ *   ISP_BM3D_VARIANT_KAISER: each block aggregated with a separable Kaiser
+ *     window of shape kaiser_beta (2.0 in the north_star's configs).
+ * Both keep the orthonormal thresholds (lambda sigma) and Wiener noise
+ * (sigma^2 / 2^zeta).  Variant calls run the generic FP64 kernels in every
+ * precision (BlockSize <= 16, GroupSize <= 64); batches, multi-frame stacks
+ * and autotuning refuse them. */
+#define ISP_BM3D_VARIANT_HAAR 1
+#define ISP_BM3D_VARIANT_KAISER 2
+int isp_bm3d_set_variant(isp_bm3d_ctx *ctx, int flags, double kaiser_beta);
 
 /* One float plane (Plane_FL): FilterIF2::process(dst, src, ref) for
  * BM3D_Basic (mode BASIC, source/BM3D.cpp:80-136,339-385), BM3D_Final (mode

@@ -19,6 +19,7 @@ E_ARG, E_UNSUPPORTED, E_SIZE, E_HIP, E_PROFILE = -1, -2, -3, -4, -5
 BASIC, FINAL, BOTH = 1, 2, 3
 MEM_HOST, MEM_DEVICE = 0, 1
 PREC_FP32, PREC_FP16, PREC_BF16 = 0, 1, 2
+VARIANT_HAAR, VARIANT_KAISER = 1, 2
 
 # Every function include/isp_bm3d.h declares.
 EXPORTS = (
@@ -29,6 +30,7 @@ EXPORTS = (
     "isp_bm3d_destroy",
     "isp_bm3d_last_error",
     "isp_bm3d_set_precision",
+    "isp_bm3d_set_variant",
     "isp_bm3d_plane_f32",
     "isp_bm3d_plane_f32_noalloc",
     "isp_bm3d_rgb_f32",
@@ -182,6 +184,7 @@ def lib() -> ctypes.CDLL:
     L.isp_bm3d_last_error.argtypes = [vp]
     L.isp_bm3d_last_error.restype = ctypes.c_char_p
     L.isp_bm3d_set_precision.argtypes = [vp, i]
+    L.isp_bm3d_set_variant.argtypes = [vp, i, ctypes.c_double]
     L.isp_bm3d_plane_f32.argtypes = [vp, vp, vp, vp, i, i, sz, i, i, vp]
     L.isp_bm3d_plane_f32_noalloc.argtypes = [vp, vp, vp, vp, i, i, sz, i, vp]
     L.isp_bm3d_rgb_f32.argtypes = [vp] + [vp] * 9 + [i, i, sz, i, i, vp]

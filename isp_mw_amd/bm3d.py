@@ -201,6 +201,14 @@ class _Filter:
         prec = {"fp32": L.PREC_FP32, "fp16": L.PREC_FP16, "bf16": L.PREC_BF16}[precision]
         L.check(lib.isp_bm3d_set_precision(self._ctx, prec), self._ctx)
 
+    def set_variant(self, haar: bool = False, kaiser_beta: float = 0.0) -> None:
+        """The configs' variant modes (no reference counterpart): the Haar
+        transform along the group axis (groups truncated to a power of two),
+        and Kaiser-windowed aggregation of shape kaiser_beta (0: off).  Runs
+        the generic FP64 kernels; set_variant() restores the reference."""
+        flags = (L.VARIANT_HAAR if haar else 0) | (L.VARIANT_KAISER if kaiser_beta else 0)
+        L.check(L.lib().isp_bm3d_set_variant(self._ctx, flags, float(kaiser_beta)), self._ctx)
+
     def close(self) -> None:
         if getattr(self, "_ctx", None) and self._ctx.value:
             L.lib().isp_bm3d_destroy(self._ctx)

@@ -38,6 +38,13 @@ struct isp_bm3d_ctx {
     // generic kernels (BlockSize != 8 or GroupSize > 32), per step [basic, final]:
     // the B x B orthonormal DCT-II, then C_N for N = 1..G, FP64
     double *d_gen[2] = {nullptr, nullptr};
+    // variant modes (isp_bm3d_set_variant): every step on the generic kernels
+    // with these tables per step: B x B DCT-II, C_N for N = 1..G (the Haar
+    // matrix for a power-of-two N in the Haar mode), then the B x B window
+    int variant = 0;
+    double kaiser_beta = 0.0;
+    double *d_var[2] = {nullptr, nullptr};
+    size_t var_kaiser_off[2] = {0, 0};
     void *ws = nullptr;
     size_t ws_bytes = 0;
     // this device's copy of the allocation-free entry point's scratch array
@@ -186,8 +193,10 @@ struct Derived {
     int active[3];
     int ngx, ngy;
     int B;         // BlockSize
-    bool generic;  // BlockSize != 8 or GroupSize > 32: the generic kernels
+    bool generic;  // BlockSize != 8 or GroupSize > 32 (or a variant mode): the generic kernels
     const double *gen_tab;  // their transform tables (GenFilterArgs::cb, then cn)
+    int haar;               // variant: Haar along the group axis
+    const double *kaiser;   // variant: B x B aggregation window, or null
 };
 
 int derive(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, int nplanes, int W, int H, Derived &d)
@@ -204,7 +213,7 @@ int derive(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, int nplanes, int W,
     d.B = B;
     // BlockSize 8 with GroupSize <= 32 runs the tuned kernels; anything else
     // within the generic kernels' bounds runs those
-    d.generic = B != kBlock || p.group_size > kMaxGroup;
+    d.generic = B != kBlock || p.group_size > kMaxGroup || ctx->variant != 0;
     if (B > kGenMaxBlock)
         return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "BlockSize %d outside 1..%d", B, kGenMaxBlock);
     if (p.group_size < 0 || p.group_size > kGenMaxGroup)
@@ -213,10 +222,20 @@ int derive(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, int nplanes, int W,
         return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "GroupSize %d x BlockSize %d^2 above %d samples per group",
                     p.group_size, B, kGenMaxElems);
     d.gen_tab = nullptr;
+    d.haar = 0;
+    d.kaiser = nullptr;
     for (int k = 0; d.generic && k < 2; ++k) {
-        // the tables built at creation for a step of this BlockSize hold C_N up to its GroupSize
+        // the tables built for a step of this BlockSize hold C_N up to its GroupSize
         const isp_bm3d_step_params &q = k ? ctx->params.final_ : ctx->params.basic;
-        if (ctx->d_gen[k] && q.block_size == B && p.group_size <= q.group_size) d.gen_tab = ctx->d_gen[k];
+        if (q.block_size != B || p.group_size > q.group_size) continue;
+        if (ctx->variant) {
+            if (!ctx->d_var[k]) continue;
+            d.gen_tab = ctx->d_var[k];
+            d.haar = (ctx->variant & ISP_BM3D_VARIANT_HAAR) != 0;
+            d.kaiser = (ctx->variant & ISP_BM3D_VARIANT_KAISER) ? ctx->d_var[k] + ctx->var_kaiser_off[k] : nullptr;
+        } else if (ctx->d_gen[k]) {
+            d.gen_tab = ctx->d_gen[k];
+        }
     }
     if (d.generic && !d.gen_tab)
         return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "no transform tables for BlockSize %d / GroupSize %d", B,
@@ -699,6 +718,8 @@ int run_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, cons
             g.wmax = d.wmax;
             g.cb = d.gen_tab;
             g.cn = d.gen_tab + (size_t)d.B * d.B;
+            g.haar = d.haar;
+            g.kaiser = d.kaiser;
             StageTimer tm(ctx, wiener ? ST_FILTER_FINAL : ST_FILTER_BASIC, s);
             HIP_CHECK(ctx, launch_filter_generic(g, wiener, s));
         } else {
@@ -1660,8 +1681,85 @@ int isp_bm3d_destroy(isp_bm3d_ctx *ctx)
     for (int k = 0; k < 2; ++k) {
         if (ctx->d_mfma[k]) (void)hipFree(ctx->d_mfma[k]);
         if (ctx->d_gen[k]) (void)hipFree(ctx->d_gen[k]);
+        if (ctx->d_var[k]) (void)hipFree(ctx->d_var[k]);
     }
     delete ctx;
+    return ISP_BM3D_OK;
+}
+
+int isp_bm3d_set_variant(isp_bm3d_ctx *ctx, int flags, double kaiser_beta)
+{
+    if (!ctx) return fail(nullptr, ISP_BM3D_E_ARG, "null context");
+    if (flags & ~(ISP_BM3D_VARIANT_HAAR | ISP_BM3D_VARIANT_KAISER))
+        return fail(ctx, ISP_BM3D_E_ARG, "unknown variant flags %d", flags);
+    if ((flags & ISP_BM3D_VARIANT_KAISER) && !(kaiser_beta >= 0 && kaiser_beta <= 100))
+        return fail(ctx, ISP_BM3D_E_ARG, "Kaiser beta %g outside 0..100", kaiser_beta);
+    HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    HIP_CHECK(ctx, hipDeviceSynchronize());
+    for (int k = 0; k < 2; ++k) {
+        if (ctx->d_var[k]) HIP_CHECK(ctx, hipFree(ctx->d_var[k]));
+        ctx->d_var[k] = nullptr;
+    }
+    ctx->variant = 0;
+    ctx->kaiser_beta = 0.0;
+    if (!flags) return ISP_BM3D_OK;
+    const long double PI = 3.141592653589793238462643383279502884L;
+    for (int k = 0; k < 2; ++k) {
+        const isp_bm3d_step_params &q = k ? ctx->params.final_ : ctx->params.basic;
+        const int B = q.block_size, G = q.group_size;
+        if (B < 1 || B > kGenMaxBlock || G < 1 || G > kGenMaxGroup) continue;  // degenerate or refused later
+        std::vector<double> t;
+        auto dct = [&](int n) {
+            for (int r = 0; r < n; ++r)
+                for (int c = 0; c < n; ++c)
+                    t.push_back((double)((r == 0 ? std::sqrt(1.0L / n) : std::sqrt(2.0L / n)) *
+                                         std::cos(PI * (2 * c + 1) * r / (2.0L * n))));
+        };
+        // orthonormal Haar matrix: H_1 = [1], H_2m = [H_m (x) [1 1]; I_m (x) [1 -1]] / sqrt 2
+        auto haar = [&](int n) {
+            std::vector<long double> h(1, 1.0L);
+            for (int m = 1; m < n; m *= 2) {
+                std::vector<long double> g(4 * m * m, 0.0L);
+                const long double r = 1.0L / std::sqrt(2.0L);
+                for (int i = 0; i < m; ++i)
+                    for (int c = 0; c < m; ++c) g[i * 2 * m + 2 * c] = g[i * 2 * m + 2 * c + 1] = h[i * m + c] * r;
+                for (int i = 0; i < m; ++i) {
+                    g[(m + i) * 2 * m + 2 * i] = r;
+                    g[(m + i) * 2 * m + 2 * i + 1] = -r;
+                }
+                h.swap(g);
+            }
+            for (long double v : h) t.push_back((double)v);
+        };
+        dct(B);
+        for (int N = 1; N <= G; ++N) {
+            if ((flags & ISP_BM3D_VARIANT_HAAR) && (N & (N - 1)) == 0) haar(N);
+            else dct(N);
+        }
+        ctx->var_kaiser_off[k] = t.size();
+        // Kaiser window: I0(beta sqrt(1 - s^2)) / I0(beta), s = (2i - (B - 1)) / (B - 1)
+        auto i0 = [](long double x) {
+            long double sum = 1.0L, term = 1.0L;
+            for (int m = 1; m < 200; ++m) {
+                term *= (x / (2.0L * m)) * (x / (2.0L * m));
+                sum += term;
+                if (term < 1e-21L * sum) break;
+            }
+            return sum;
+        };
+        std::vector<long double> kw(B, 1.0L);
+        if (flags & ISP_BM3D_VARIANT_KAISER)
+            for (int i = 0; i < B && B > 1; ++i) {
+                const long double sv = (2.0L * i - (B - 1)) / (long double)(B - 1);
+                kw[i] = i0(kaiser_beta * std::sqrt(std::max(0.0L, 1.0L - sv * sv))) / i0(kaiser_beta);
+            }
+        for (int y = 0; y < B; ++y)
+            for (int x = 0; x < B; ++x) t.push_back((double)(kw[y] * kw[x]));
+        HIP_CHECK(ctx, hipMalloc(&ctx->d_var[k], t.size() * sizeof(double)));
+        HIP_CHECK(ctx, hipMemcpy(ctx->d_var[k], t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    ctx->variant = flags;
+    ctx->kaiser_beta = kaiser_beta;
     return ISP_BM3D_OK;
 }
 

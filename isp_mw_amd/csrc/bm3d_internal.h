@@ -302,6 +302,12 @@ struct GenFilterArgs {
     float wmax;                // cap of the Wiener group weight
     const double *cb;          // B x B orthonormal DCT-II [k][n]
     const double *cn;          // C_N [k][n] for N = 1..G, at (N-1) N (2N-1) / 6
+    // variant modes (no reference counterpart): haar: C_N of a power-of-two N
+    // is the orthonormal Haar matrix and a group keeps the largest power of
+    // two of its members; kaiser (B x B, or null): each block's aggregation
+    // weight times the window
+    int haar;
+    const double *kaiser;
 };
 
 size_t match_generic_lds_bytes(const MatchArgs &a);

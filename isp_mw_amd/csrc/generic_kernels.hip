@@ -258,7 +258,8 @@ __global__ __launch_bounds__(kGenThreads) void filter_generic_kernel(GenFilterAr
     const int refIdx = blockIdx.x;
     if (refIdx >= a.ngx * a.ngy) return;
     const int B = a.B, B2 = B * B;
-    const int N = a.cnt[refIdx];
+    int N = a.cnt[refIdx];
+    if (a.haar) N = 1 << (31 - __clz(N));  // the largest power of two of the members
     const int E = N * B2;
     const int *pos = a.pos + (size_t)refIdx * a.G;
     double *X = reinterpret_cast<double *>(smem);
@@ -316,8 +317,14 @@ __global__ __launch_bounds__(kGenThreads) void filter_generic_kernel(GenFilterAr
             const int z = e / B2, yx = e - z * B2, y = yx / B, x = yx - y * B;
             const int pz = pos[z];
             const size_t o = (size_t)((pz >> 16) + y) * a.pitch + (pz & 0xFFFF) + x;
-            fix_add_global(num + o, weight * X[e]);
-            atomicAdd(den + o, wfix);
+            if (a.kaiser) {
+                const double wk = weight * a.kaiser[yx];
+                fix_add_global(num + o, wk * X[e]);
+                fix_add_global(den + o, wk);
+            } else {
+                fix_add_global(num + o, weight * X[e]);
+                atomicAdd(den + o, wfix);
+            }
         }
         __syncthreads();
     }

@@ -99,6 +99,7 @@ def _lib(fast: bool = False) -> ctypes.CDLL:
     pp = ctypes.POINTER(dp)
     sp = ctypes.POINTER(StepParams)
     lib.orc_grid.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+    lib.orc_set_variant.argtypes = [ctypes.c_int, ctypes.c_double]
     lib.orc_search_boundary.argtypes = [ctypes.c_int] * 4
     lib.orc_match_all.argtypes = [sp, dp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                   ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int32), dp]
@@ -168,6 +169,14 @@ def step_params(p) -> StepParams:
 
 def set_threads(n: int, fast: bool = False) -> int:
     return _lib(fast).orc_set_threads(int(n))
+
+
+def set_variant(haar: bool = False, kaiser_beta: float = 0.0) -> None:
+    """The configs' variant modes for later calls (no reference counterpart):
+    Haar along the group axis with groups truncated to a power of two, and
+    Kaiser-windowed aggregation.  set_variant() restores the reference."""
+    for fast in (False, True):
+        _lib(fast).orc_set_variant(int(bool(haar)), float(kaiser_beta))
 
 
 def grid(n: int, block_size: int, block_step: int) -> np.ndarray:

@@ -430,12 +430,186 @@ static void gather(double *grp, const double *const *srcs, const int *fz, int st
     }
 }
 
+/* ------------------------------------------------------------------------ */
+/* Variant modes of the north_star's configs, which the reference does not   */
+/* have (it uses a DCT along the group and a uniform weight per block):       */
+/*   Haar: the group axis is transformed by the orthonormal Haar matrix, and  */
+/*         a group is truncated to the largest power of two of its members;  */
+/*   Kaiser: each filtered block is aggregated with a Kaiser window of shape  */
+/*         beta (w * k[u] k[v] in num, w * k[u] k[v] in den).                 */
+/* Both run in the orthonormal formulation (a uniform lambda*sigma threshold; */
+/* Wiener noise sigma^2 / 2^zeta, zeta counting the zero indices, the DC row  */
+/* of the Haar matrix being index 0), which the reference's unnormalised      */
+/* tables reduce to in the DCT mode.                                          */
+
+static int g_haar = 0;
+static double g_kaiser_beta = 0.0;
+
+int orc_set_variant(int haar, double kaiser_beta)
+{
+    g_haar = haar != 0;
+    g_kaiser_beta = kaiser_beta;
+    return 0;
+}
+
+/* Orthonormal Haar matrix of a power-of-two n, row-major:
+ * H_1 = [1], H_2m = [H_m (x) [1 1]; I_m (x) [1 -1]] / sqrt(2). */
+static void haar_matrix(double *h, int n)
+{
+    h[0] = 1.0;
+    for (int m = 1; m < n; m *= 2) {
+        double t[ORC_MAXN * ORC_MAXN];
+        const double r = (double)(1.0L / sqrtl(2.0L));
+        for (int i = 0; i < 2 * m * 2 * m; ++i) t[i] = 0.0;
+        for (int r0 = 0; r0 < m; ++r0)
+            for (int c = 0; c < m; ++c) {
+                t[r0 * 2 * m + 2 * c] = h[r0 * m + c] * r;
+                t[r0 * 2 * m + 2 * c + 1] = h[r0 * m + c] * r;
+            }
+        for (int r0 = 0; r0 < m; ++r0) {
+            t[(m + r0) * 2 * m + 2 * r0] = r;
+            t[(m + r0) * 2 * m + 2 * r0 + 1] = -r;
+        }
+        for (int i = 0; i < 2 * m * 2 * m; ++i) h[i] = t[i];
+    }
+}
+
+/* Orthonormal DCT-II matrix of length n, row-major */
+static void dct_matrix(double *c, int n)
+{
+    for (int k = 0; k < n; ++k)
+        for (int j = 0; j < n; ++j)
+            c[k * n + j] = (double)((k == 0 ? sqrtl(1.0L / n) : sqrtl(2.0L / n)) *
+                                    cosl(3.14159265358979323846264338327950288L * (2 * j + 1) * k / (2.0L * n)));
+}
+
+/* Modified Bessel function of the first kind, order 0 (its power series) */
+static double bessel_i0(double x)
+{
+    double sum = 1.0, term = 1.0;
+    for (int m = 1; m < 200; ++m) {
+        term *= (x / (2.0 * m)) * (x / (2.0 * m));
+        sum += term;
+        if (term < 1e-18 * sum) break;
+    }
+    return sum;
+}
+
+/* Kaiser window of length n and shape beta: I0(beta sqrt(1 - t^2)) / I0(beta),
+ * t = (2i - (n - 1)) / (n - 1); 1 for n = 1 */
+static void kaiser_window(double *k, int n, double beta)
+{
+    for (int i = 0; i < n; ++i) {
+        if (n == 1) {
+            k[i] = 1.0;
+            continue;
+        }
+        const double t = (2.0 * i - (n - 1)) / (double)(n - 1);
+        k[i] = bessel_i0(beta * sqrt(fmax(0.0, 1.0 - t * t))) / bessel_i0(beta);
+    }
+}
+
+/* y = M x along one axis of g[n0][n1][n2] (axis 0, 1 or 2), M n x n row-major;
+ * transposed (x = M^T y) when inverse. */
+static void axis_product(double *g, int n0, int n1, int n2, int axis, const double *M, int inverse)
+{
+    const int n = axis == 0 ? n0 : (axis == 1 ? n1 : n2);
+    const size_t stride = axis == 0 ? (size_t)n1 * n2 : (axis == 1 ? (size_t)n2 : 1);
+    const int lines0 = axis == 0 ? 1 : n0, lines1 = axis == 2 ? n1 : (axis == 1 ? n2 : n1 * n2);
+    double in[ORC_MAXN];
+    for (int a = 0; a < lines0; ++a)
+        for (int b = 0; b < lines1; ++b) {
+            double *p;
+            if (axis == 0) p = g + b;
+            else if (axis == 1) p = g + (size_t)a * n1 * n2 + b;
+            else p = g + (size_t)a * n1 * n2 + (size_t)b * n2;
+            for (int i = 0; i < n; ++i) in[i] = p[(size_t)i * stride];
+            for (int k = 0; k < n; ++k) {
+                double s = 0.0;
+                for (int i = 0; i < n; ++i) s += (inverse ? M[i * n + k] : M[k * n + i]) * in[i];
+                p[(size_t)k * stride] = s;
+            }
+        }
+}
+
+static void variant_transform(double *g, int N, int B, int inverse)
+{
+    double cz[ORC_MAXN * ORC_MAXN], cb[ORC_MAXN * ORC_MAXN];
+    if (g_haar) haar_matrix(cz, N);
+    else dct_matrix(cz, N);
+    dct_matrix(cb, B);
+    if (!inverse) {
+        axis_product(g, N, B, B, 2, cb, 0);
+        axis_product(g, N, B, B, 1, cb, 0);
+        axis_product(g, N, B, B, 0, cz, 0);
+    } else {
+        axis_product(g, N, B, B, 0, cz, 1);
+        axis_product(g, N, B, B, 1, cb, 1);
+        axis_product(g, N, B, B, 2, cb, 1);
+    }
+}
+
+static void collaborative_filter_variant(const filter_data *f, int c, double *const *num, double *const *den,
+                                         int row0, int stride, const double *const *src, const double *const *ref,
+                                         const int32_t *pos, const int *fz, int N, double *grp, double *gref)
+{
+    const int B = f->p.block_size;
+    if (g_haar)
+        while (N & (N - 1)) N &= N - 1;  /* the largest power of two of the members */
+    const int cnt = N * B * B;
+    const double sigma = f->sigma[c];
+    gather(grp, src, fz, stride, pos, N, B);
+    variant_transform(grp, N, B, 0);
+    double w;
+    if (!f->wiener) {
+        const double thr = sigma * f->p.lambda;
+        int retained = 0;
+        for (int k = 0; k < cnt; ++k) {
+            if (fabs(grp[k]) <= thr) grp[k] = 0;
+            else ++retained;
+        }
+        w = retained < 1 ? 1.0 : 1.0 / (double)retained;
+    } else {
+        gather(gref, ref, fz, stride, pos, N, B);
+        variant_transform(gref, N, B, 0);
+        double L2 = 0;
+        for (int k = 0; k < cnt; ++k) {
+            const int z = k / (B * B), u = (k / B) % B, v = k % B;
+            const int zeta = (z == 0) + (u == 0) + (v == 0);
+            const double yy = gref[k] * gref[k];
+            const double wc = yy / (yy + sigma * sigma / (double)(1 << zeta));
+            grp[k] *= wc;
+            L2 += wc * wc;
+        }
+        w = L2 <= 0 ? 1.0 : 1.0 / L2;
+    }
+    variant_transform(grp, N, B, 1);
+    double kw[ORC_MAXN];
+    kaiser_window(kw, B, g_kaiser_beta);
+    const double *s = grp;
+    for (int z = 0; z < N; ++z) {
+        int y = (pos[z] >> 16) - row0, x = pos[z] & 0xFFFF;
+        double *dn = num[fz[z]] + (size_t)y * stride + x;
+        double *dd = den[fz[z]] + (size_t)y * stride + x;
+        for (int u = 0; u < B; ++u)
+            for (int v = 0; v < B; ++v) {
+                const double ww = w * kw[u] * kw[v];
+                dn[(size_t)u * stride + v] += *s++ * ww;
+                dd[(size_t)u * stride + v] += ww;
+            }
+    }
+}
+
 /* Accumulates into num/den of each member's frame, whose row 0 is image row
  * `row0`.  A single-frame group has every fz[z] = 0. */
 static void collaborative_filter_pos3(const filter_data *f, int c, double *const *num, double *const *den, int row0,
                                       int stride, const double *const *src, const double *const *ref,
                                       const int32_t *pos, const int *fz, int N, double *grp, double *gref)
 {
+    if (g_haar || g_kaiser_beta != 0.0) {
+        collaborative_filter_variant(f, c, num, den, row0, stride, src, ref, pos, fz, N, grp, gref);
+        return;
+    }
     const int B = f->p.block_size;
     const int cnt = N * B * B;
     gather(grp, src, fz, stride, pos, N, B);

@@ -31,6 +31,14 @@ typedef struct orc_step_params {
     double sigma[3];
 } orc_step_params;
 
+/* Variant modes (no reference counterpart; the configs' "1D Haar" and
+ * "Kaiser window beta=2.0"), for every later filter call of the process:
+ * haar != 0 transforms the group axis with the orthonormal Haar matrix and
+ * truncates each group to the largest power of two of its members;
+ * kaiser_beta != 0 aggregates each block with a Kaiser window of that shape.
+ * (0, 0) restores the reference's semantics. */
+int orc_set_variant(int haar, double kaiser_beta);
+
 /* Reference grid of one axis (source/BM3D.cpp:98-127).  Writes at most
  * `cap` positions, returns the count. */
 int orc_grid(int n, int block_size, int block_step, int *out, int cap);

@@ -70,9 +70,32 @@ def brute_groups(p, ref, norm=NORM[0]):
     return out
 
 
-def filter_plane(p, src, ref, groups, c, wiener):
-    """Aggregated num/den of one plane for the given groups."""
+def haar_basis(n):
+    """Orthonormal Haar matrix of a power-of-two n from its basis functions:
+    row 0 the constant 1/sqrt(n); row 2^j + m (0 <= m < 2^j) supported on
+    [m n / 2^j, (m + 1) n / 2^j), +sqrt(2^j / n) on the first half and
+    -sqrt(2^j / n) on the second."""
+    h = np.zeros((n, n))
+    h[0] = 1.0 / np.sqrt(n)
+    k = 1
+    j = 0
+    while k < n:
+        for m in range(2 ** j):
+            L = n // 2 ** j
+            a = m * L
+            h[k, a:a + L // 2] = np.sqrt(2 ** j / n)
+            h[k, a + L // 2:a + L] = -np.sqrt(2 ** j / n)
+            k += 1
+        j += 1
+    return h
+
+
+def filter_plane(p, src, ref, groups, c, wiener, haar=False, kaiser_beta=0.0):
+    """Aggregated num/den of one plane for the given groups.  haar: the group
+    axis by the Haar matrix, groups truncated to a power of two; kaiser_beta:
+    blocks aggregated with numpy's Kaiser window of that shape."""
     B = int(getattr(p, "BlockSize", 8))
+    win = np.outer(np.kaiser(B, kaiser_beta), np.kaiser(B, kaiser_beta)) if kaiser_beta else np.ones((B, B))
     src = np.asarray(src, np.float64)
     ref = np.asarray(ref, np.float64)
     h, w = src.shape
@@ -80,16 +103,25 @@ def filter_plane(p, src, ref, groups, c, wiener):
     num = np.zeros((h, w))
     den = np.zeros((h, w))
     for g in groups.values():
+        if haar:
+            g = g[:2 ** int(np.floor(np.log2(len(g))))]
         n = len(g)
         grp = np.stack([src[y:y + B, x:x + B] for _, y, x in g])
-        O = sf.dctn(grp, type=2, norm="ortho")
+        if haar:
+            O = np.einsum("kz,zuv->kuv", haar_basis(n), sf.dctn(grp, type=2, norm="ortho", axes=(1, 2)))
+        else:
+            O = sf.dctn(grp, type=2, norm="ortho")
         if not wiener:
             keep = np.abs(O) > p.lambda_ * sigma
             O = np.where(keep, O, 0.0)
             r = int(keep.sum())
             wgt = 1.0 if r < 1 else 1.0 / r
         else:
-            Or = sf.dctn(np.stack([ref[y:y + B, x:x + B] for _, y, x in g]), type=2, norm="ortho")
+            gr = np.stack([ref[y:y + B, x:x + B] for _, y, x in g])
+            if haar:
+                Or = np.einsum("kz,zuv->kuv", haar_basis(n), sf.dctn(gr, type=2, norm="ortho", axes=(1, 2)))
+            else:
+                Or = sf.dctn(gr, type=2, norm="ortho")
             zeta = np.zeros((n, B, B))
             zeta[0] += 1
             zeta[:, 0, :] += 1
@@ -98,21 +130,24 @@ def filter_plane(p, src, ref, groups, c, wiener):
             O = O * Wc
             l2 = float((Wc ** 2).sum())
             wgt = 1.0 if l2 <= 0 else 1.0 / l2
-        blk = sf.idctn(O, type=2, norm="ortho")
+        if haar:
+            blk = sf.idctn(np.einsum("kz,kuv->zuv", haar_basis(n), O), type=2, norm="ortho", axes=(1, 2))
+        else:
+            blk = sf.idctn(O, type=2, norm="ortho")
         for z, (_, y, x) in enumerate(g):
-            num[y:y + B, x:x + B] += wgt * blk[z]
-            den[y:y + B, x:x + B] += wgt
+            num[y:y + B, x:x + B] += wgt * win * blk[z]
+            den[y:y + B, x:x + B] += wgt * win
     return num, den
 
 
-def kernel(p, src_planes, ref_planes, wiener):
+def kernel(p, src_planes, ref_planes, wiener, haar=False, kaiser_beta=0.0):
     """BM3D_Base::Kernel semantics: match on ref_planes[0], filter each plane
     with sigma > 0, others copied from src."""
     groups = brute_groups(p, ref_planes[0])
     out = []
     for c, s in enumerate(src_planes):
         if p.sigma[c] > 0:
-            num, den = filter_plane(p, s, ref_planes[c], groups, c, wiener)
+            num, den = filter_plane(p, s, ref_planes[c], groups, c, wiener, haar, kaiser_beta)
             out.append(num / den)
         else:
             out.append(np.asarray(s, np.float64).copy())

@@ -473,3 +473,67 @@ def test_bf16_round_known_values():
     # 1 + 2^-8 is a tie between 1 and 1 + 2^-7: to even (1); 1 + 1.5 * 2^-8 rounds up
     np.testing.assert_array_equal(got[:4], [1.0, 1.0, 1.0 + 2 ** -7, -2.5])
     assert abs(got[4] - 1e-3) <= 1e-3 * 2 ** -8 and got[5] == 0.0
+
+
+# --- the configs' variant modes (Haar along the group axis, Kaiser aggregation)
+@pytest.fixture
+def variant():
+    def set_(haar=False, beta=0.0):
+        oracle.set_variant(haar, beta)
+    yield set_
+    oracle.set_variant(False, 0.0)
+
+
+VARIANTS = [(True, 0.0), (False, 2.0), (True, 2.0)]
+
+
+@pytest.mark.parametrize("wiener", [False, True])
+@pytest.mark.parametrize("haar,beta", VARIANTS)
+def test_variant_kernel_vs_numpy(variant, haar, beta, wiener):
+    # independent: numpy's Kaiser window, the Haar basis functions written out,
+    # scipy's orthonormal DCT; groups of up to 16 (truncated to 8, 4, 2, 1)
+    clean, n = smooth_pair(34, 38, 3) if haar else synth.grey_pair(30, 33, 25.0, 4)
+    p = para(step=3, R=6, G1=16, G2=16)
+    st = p.final if wiener else p.basic
+    ref = (clean + 0.01).astype(np.float32) if wiener else n
+    variant(haar, beta)
+    got = oracle.kernel(st, n, ref, wiener=wiener)
+    want = npref.kernel(st, [n], [ref], wiener, haar=haar, kaiser_beta=beta)[0]
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-10)
+    counts, _, _ = oracle.match_all(st, ref)
+    assert counts.max() > 8  # truncation from above 8 members is exercised
+
+
+def test_variant_haar_of_two_equals_the_dct(variant):
+    # the orthonormal Haar matrix of length 2 is the length-2 DCT-II: with
+    # groups of at most 2 the Haar mode is the reference's filter
+    clean, n = synth.grey_pair(40, 36, 25.0, 6)
+    p = para(step=2, R=5, G1=2, G2=2)
+    want = oracle.bm3d_plane(p.basic, p.final, n)
+    variant(True, 0.0)
+    np.testing.assert_allclose(oracle.bm3d_plane(p.basic, p.final, n), want, rtol=0, atol=1e-12)
+
+
+def test_variant_constant_image_and_window(variant):
+    # a constant image: every group is the block alone, every estimate the
+    # constant, so any aggregation window returns it
+    src = np.full((36, 40), 0.3)
+    p = para()
+    variant(True, 2.0)
+    np.testing.assert_allclose(oracle.kernel(p.basic, src, src, wiener=False), 0.3, atol=1e-14)
+    # numpy's Kaiser window (the variant's weights) is symmetric, peaks at the
+    # centre and equals 1/I0(beta) at the ends
+    k = np.kaiser(8, 2.0)
+    np.testing.assert_allclose(k, k[::-1])
+    assert abs(k[0] - 1.0 / np.i0(2.0)) < 1e-15
+
+
+def test_variant_off_is_the_reference(variant):
+    clean, n = synth.grey_pair(32, 32, 25.0, 7)
+    p = para(step=3, R=5, G1=8, G2=8)
+    want = oracle.bm3d_plane(p.basic, p.final, n)
+    variant(True, 2.0)
+    other = oracle.bm3d_plane(p.basic, p.final, n)
+    variant(False, 0.0)
+    np.testing.assert_array_equal(oracle.bm3d_plane(p.basic, p.final, n), want)
+    assert np.abs(other - want).max() > 1e-4   # the variants do change the output
