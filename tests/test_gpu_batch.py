@@ -189,3 +189,62 @@ def test_rgb_autotune_keeps_results(gpu):
     t = f.autotune(200, 120, 25.0, rgb=True)
     say(f"rgb autotune 200x120: {t}")
     np.testing.assert_array_equal(f(x).cpu().numpy(), before)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16"])
+def test_configs2_scale_batch(gpu, prec):
+    """configs[2] at its own scale: 64 grey images, W uniform in [256, 1024],
+    H in [256, 768], per-image sigma in [5, 50], each generated like the
+    512x512 case, in one batch (one launch per stage, replayed as a hipGraph)
+    with autotuned tiles and final GroupSize per size class.  Every image
+    equals a single call on it with the same tiles and GroupSize, bit for
+    bit; two images (the smallest and a large one) against the oracle."""
+    import torch
+    rng = np.random.default_rng(2024)
+    n_img = 64
+    Ws = rng.integers(256, 1025, n_img)
+    Hs = rng.integers(256, 769, n_img)
+    sig = rng.uniform(5.0, 50.0, n_img)
+    pairs = [synth.grey_pair(int(h), int(w), float(s), 1000 + k) for k, (w, h, s) in enumerate(zip(Ws, Hs, sig))]
+    p = para()
+    f = BM3D(p, precision=prec)
+    classes = {}
+    for w, h in zip(Ws, Hs):
+        classes.setdefault(f.size_class(int(w), int(h)), []).append((int(w), int(h)))
+    for cls, dims in classes.items():
+        w, h = sorted(dims)[len(dims) // 2]
+        t = f.autotune(w, h, 25.0, auto_group=True)
+        say(f"size class {cls} ({len(dims)} images): tiles {t['match_tile']} / {t['filter_tile']}, "
+            f"final GroupSize {t['final_group_size'] or p.final.GroupSize}")
+    srcs = [torch.from_numpy(nz).cuda() for _, nz in pairs]
+    dsts = [torch.empty_like(x) for x in srcs]
+    b = BM3DBatch(f, dsts, srcs, sigmas=list(sig), auto_group=True)
+    say(f"batch {prec}: {b.info()}")
+    b.run()
+    torch.cuda.synchronize()
+    worst = 0.0
+    for k, (x, d) in enumerate(zip(srcs, dsts)):
+        one = BM3DBatch(f, [torch.empty_like(x)], [x], sigmas=[float(sig[k])], auto_group=True)
+        one.run()
+        torch.cuda.synchronize()
+        diff = float((one._keep[0][0] - d).abs().max())
+        worst = max(worst, diff)
+        assert diff == 0.0, (k, diff)
+        one.close()
+    say(f"64 images: max |batch - single image| = {worst}")
+    # against the oracle: the single-image semantics with that image's sigma and GroupSize
+    for k in (int(np.argmin(Ws * Hs)), 7):
+        q = para(sigma=float(sig[k]))
+        tg = f.get_tuning(f.size_class(int(Ws[k]), int(Hs[k])))
+        if tg and tg["final_group_size"]:
+            q.final.GroupSize = tg["final_group_size"]
+        clean, nz = pairs[k]
+        ref = oracle.bm3d_plane(q.basic, q.final, nz)
+        got = dsts[k].cpu().numpy()
+        err = float(np.abs(got - ref).max())
+        d = synth.psnr(got, clean) - synth.psnr(ref, clean)
+        say(f"image {k} {Ws[k]}x{Hs[k]} sigma {sig[k]:.1f}: max |batch - oracle| {err:.3g}, dPSNR {d:+.4f} dB")
+        if prec == "fp32":
+            assert err <= 2e-3 and abs(d) <= 0.01
+        else:
+            assert d >= -0.05
