@@ -234,12 +234,12 @@ typedef struct isp_bm3d_plan_info {
 int isp_bm3d_plan(const isp_bm3d_params *params, int width, int height, int nplanes, isp_bm3d_plan_info *info);
 
 /* Headroom of the aggregation sums (no reference counterpart: the reference
- * sums in FP64).  num and den are 64-bit fixed point with 36 fraction bits.
+ * sums in FP64).  num and den are 64-bit fixed point with 32 fraction bits.
  * For a step's parameters and a temporal radius (0 for single frames):
  * *max_contributions = an upper bound on the (group, member) contributions
  * to one pixel; *weight_cap = the cap applied to Wiener group weights
  * 1 / sum(W^2), the largest power of two <= 2^16 with
- * max_contributions * weight_cap * 4 < 2^26, so sums of samples and
+ * max_contributions * weight_cap * 4 < 2^30, so sums of samples and
  * reconstructions of magnitude <= 4 cannot overflow.  Returns
  * ISP_BM3D_E_UNSUPPORTED (weight_cap 0) when even weights <= 1 could. */
 int isp_bm3d_aggregation_limits(const isp_bm3d_step_params *p, int radius, double *max_contributions,

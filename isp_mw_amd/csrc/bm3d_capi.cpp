@@ -615,6 +615,8 @@ FinalizeArgs build_finalize(int W, int H, const StepIO &io, const Derived &d, un
         fa.den[c] = den[c];
         fa.active[c] = d.degenerate ? 0 : d.active[c];
     }
+    // the tiled filter kernels add den per block origin, the generic ones per pixel
+    fa.den_box = d.generic ? 0 : kBlock;
     return fa;
 }
 
@@ -3096,6 +3098,7 @@ int stack_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, co
         fa.lo = 0.0;
         fa.hi = 1.0;
         fa.out_pitch = io.out_pitch;
+        fa.den_box = kBlock;  // the stack filters add den per block origin
         for (int c = 0; c < 3; ++c) {
             fa.src[c] = io.src[c] ? io.src[c] + fs * t : nullptr;
             fa.num[c] = w.num[c] + fs * t;

@@ -105,37 +105,33 @@ __host__ __device__ constexpr size_t align16(size_t x) { return (x + 15) & ~size
 
 typedef unsigned long long u64;
 
-// floor(x) as int32 in one instruction (|x| < 2^31)
-__device__ __forceinline__ int floor_to_int(float x)
+// Fixed-point addends of the aggregation sums (LDS integer atomics are full
+// rate; two's complement wraps negative values correctly).  The value added
+// for x with weight w is x * w * 2^32 rounded to the nearest integer (ties to
+// even), formed by one FP64 fused multiply-add: doubles in [2^56, 2^57) are
+// spaced 16 apart, so t = fma(x, w * 2^36, 1.5 * 2^56) equals
+// 1.5 * 2^56 + 16 * round(x * w * 2^32) while |x * w * 2^32| < 2^51, and the
+// bit pattern of t minus that of 1.5 * 2^56 is that integer in two's
+// complement.  The constant's low word is zero, so only the high word is
+// subtracted: three ALU operations (FP32 -> FP64, FP64 FMA, 32-bit add) where
+// an integer and a fraction part take five.
+constexpr double kFixMagic = 108086391056891904.0;     // 1.5 * 2^56
+constexpr u64 kFixMagicBits = 0x4378000000000000ull;   // its bit pattern
+
+// the FMA operand of weight w: w * 2^32 * 16 (exact: a power-of-two scaling)
+__device__ __forceinline__ double fix_weight(float w) { return (double)w * (16.0 * (double)kFixScale); }
+
+__device__ __forceinline__ u64 fix_bits(float x, double wfix)
 {
-    int r;
-    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
-    return r;
+    const double t = __builtin_fma((double)x, wfix, kFixMagic);
+    return (u64)__double_as_longlong(t) - kFixMagicBits;
 }
 
-// Adds v to a 64-bit fixed-point accumulator in LDS (integer LDS atomics are
-// full rate; two's complement wraps negative values correctly).  The value
-// added is floor(v * 2^36): hi = floor(v * 2^4) is the part above 2^32, and
-// the fraction (v * 2^4 - hi) in [0, 1), from v_fract and scaled by 2^32, is
-// an exact FP32 value whose truncation is the low word.  (v_fract returns
-// 1 - 2^-24 where the fraction would round up to 1, i.e. for |v| < 2^-28
-// below zero: the low word is then 2^32 - 256 rather than 2^32 - 1.)  Five
-// ALU operations against the eleven of a rounding float -> int64 conversion.
-__device__ __forceinline__ void fix_add(u64 *acc, float v)
-{
-    const float s = v * (kFixScale / 4294967296.0f);
-    const uint32_t lo = (uint32_t)(__builtin_amdgcn_fractf(s) * 4294967296.0f);
-    atomicAdd(acc, ((u64)(long long)floor_to_int(s) << 32) + lo);
-}
+// adds round(w * 2^32)
+__device__ __forceinline__ void fix_add(u64 *acc, float w) { atomicAdd(acc, fix_bits(1.f, fix_weight(w))); }
 
-// fix_add(acc, w * x) with ws = w * 2^4 (the scale folded into the weight:
-// the product w * x * 2^4 rounds as (w * x) * 2^4, a power-of-two scaling)
-__device__ __forceinline__ void fix_add_scaled(u64 *acc, float x, float ws)
-{
-    const float s = x * ws;
-    const uint32_t lo = (uint32_t)(__builtin_amdgcn_fractf(s) * 4294967296.0f);
-    atomicAdd(acc, ((u64)(long long)floor_to_int(s) << 32) + lo);
-}
+// adds round(x * w * 2^32), wfix = fix_weight(w)
+__device__ __forceinline__ void fix_add_scaled(u64 *acc, float x, double wfix) { atomicAdd(acc, fix_bits(x, wfix)); }
 
 __host__ __device__ inline size_t match_wave_bytes(int maxCand, int keyBytes)
 {

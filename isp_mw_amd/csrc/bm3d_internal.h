@@ -41,8 +41,8 @@ constexpr int kMaxBatchImages = 4096;
 constexpr int kMaxBatchTiles = 1024;  // tiles per image per axis
 constexpr int kMaxBatchRows = 1 << 20;
 
-// Fixed-point scale of the aggregation sums: value * 2^36, two's complement
-// in 64 bits, so a pixel's sums must stay below 2^63 / 2^36 = 2^27 in value.
+// Fixed-point scale of the aggregation sums: value * 2^32, two's complement
+// in 64 bits, so a pixel's sums must stay below 2^63 / 2^32 = 2^31 in value.
 // A pixel p is covered by block origins o with o <= p < o + B on each axis;
 // o can be a member of the group of every reference block whose window holds
 // it, once per group.  Per axis the references whose windows reach
@@ -54,12 +54,15 @@ constexpr int kMaxBatchRows = 1 << 20;
 // 1 / sum(W^2) exceed 1 only where every
 // gain of the group is small, and are capped at wiener_weight_cap(): the
 // largest power of two (at most 2^16) for which bound * cap * kSampleBound
-// stays below 2^26, a factor 2 inside the 2^27 limit.  With samples of
-// magnitude <= kSampleBound, num and den then cannot overflow.  The cap acts
+// stays below 2^30, a factor 2 inside the 2^31 limit.  With samples of
+// magnitude <= kSampleBound, num and den then cannot overflow, and each
+// addend (|value| <= 4 * 2^16 = 2^18) lies inside the 2^19 for which the
+// FP64 rounding of the device helper (fix_bits) is exact.  The cap acts
 // only on groups whose guide carries almost no energy (sum(W^2) < 1 / cap),
 // whose estimates are near zero.
-constexpr float kFixScale = 68719476736.0f;        // 2^36
-constexpr double kFixInv = 1.0 / 68719476736.0;    // 2^-36
+constexpr float kFixScale = 4294967296.0f;         // 2^32
+constexpr double kFixInv = 1.0 / 4294967296.0;     // 2^-32
+constexpr double kFixSumLimit = 1073741824.0;      // 2^30: bound * cap * kSampleBound stays below
 constexpr double kSampleBound = 4.0;               // |samples| and |reconstructions| the sums allow
 constexpr float kMaxWienerWeight = 65536.0f;       // 2^16, the cap's upper end
 
@@ -81,9 +84,9 @@ inline double aggregation_bound(int B, int s, int R, int st, int G, int radius)
 inline float wiener_weight_cap(int B, int s, int R, int st, int G, int radius)
 {
     const double bound = aggregation_bound(B, s, R, st, G, radius);
-    if (bound * kSampleBound >= 67108864.0) return 0.f;  // 2^26
+    if (bound * kSampleBound >= kFixSumLimit) return 0.f;
     float cap = kMaxWienerWeight;
-    while (cap > 1.f && (double)cap * bound * kSampleBound >= 67108864.0) cap *= 0.5f;
+    while (cap > 1.f && (double)cap * bound * kSampleBound >= kFixSumLimit) cap *= 0.5f;
     return cap;
 }
 
@@ -206,6 +209,9 @@ struct FinalizeArgs {
     int W, H, pitch;
     const unsigned long long *num[3];
     const unsigned long long *den[3];
+    int den_box;              // 0: den per pixel; kBlock: den per block origin,
+                              // a pixel's den the sum over the 8 x 8 origins
+                              // whose block covers it
     const float *src[3];      // copied through for inactive planes
     int active[3];
     int nplanes;

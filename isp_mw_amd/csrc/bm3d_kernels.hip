@@ -928,8 +928,7 @@ __device__ __forceinline__ float group_core(float *g, int N, int lane, const flo
 }
 
 // LDS of the filter kernel: accumulators of the tile footprint (num per pixel,
-// den per block origin) and one group buffer per wave; the row box sums of
-// the flush reuse the group buffers.
+// den per block origin) and one group buffer per wave.
 // accumulators as 32-bit words: num per pixel and den per origin, 64 bits each
 __host__ __device__ inline size_t filter_acc_floats(int FW, int FH)
 {
@@ -939,49 +938,38 @@ __host__ __device__ inline size_t filter_acc_floats(int FW, int FH)
 
 __host__ __device__ inline size_t filter_lds(int FW, int FH, int G)
 {
-    const int DH = FH - kBlock + 1;
-    size_t groups = (size_t)kWaves * G * ZS;
-    const size_t hs = (size_t)DH * FW * 2;
-    if (hs > groups) groups = hs;
-    return (filter_acc_floats(FW, FH) + groups) * 4;
+    return (filter_acc_floats(FW, FH) + (size_t)kWaves * G * ZS) * 4;
 }
 
 
-// Adds a tile's LDS accumulators into the num/den planes of plane c (all in
-// 64-bit fixed point, so the sums do not depend on the order of the adds).  den of
-// a pixel is the sum of the weights of the blocks whose origin lies in the
-// 8x8 window above-left of it: row box sums (into hsum), then column sums.
-__device__ __forceinline__ void flush_tile(const FilterArgs &a, int c, const u64 *accNum, const u64 *accDen,
-                                           u64 *hsum, int ox, int oy, int tid, int nthreads = kThreads,
-                                           size_t dst_off = 0)
+// Adds a tile's LDS accumulators into the planes of plane c, in 64-bit fixed
+// point (the sums do not depend on the order of the adds): num per pixel of
+// the footprint, den per block origin (finalize_kernel sums a pixel's den over
+// the 8x8 origins whose block covers it).  Zero entries are not added.  Rows
+// go to the waves and columns to the lanes (no division by the footprint
+// width).
+__device__ __forceinline__ void flush_tile(const FilterArgs &a, int c, const u64 *accNum, const u64 *accDen, int ox,
+                                           int oy, int tid, int nthreads = kThreads, size_t dst_off = 0)
 {
     const int FW = a.FW, FH = a.FH, DW = FW - kBlock + 1, DH = FH - kBlock + 1;
-    for (int k = tid; k < DH * FW; k += nthreads) {
-        const int dy = k / FW, px = k - dy * FW;
-        u64 s = 0;
-#pragma unroll
-        for (int bq = 0; bq < kBlock; ++bq) {
-            const int q = px - bq;
-            if (q >= 0 && q < DW) s += accDen[dy * DW + q];
-        }
-        hsum[k] = s;
-    }
-    __syncthreads();
+    const int lane = tid & 63, wave = tid >> 6, nw = nthreads >> 6;
     u64 *num = a.num[c] + dst_off, *den = a.den[c] + dst_off;
-    for (int k = tid; k < FW * FH; k += nthreads) {
-        const int py = k / FW, px = k - py * FW;
-        const int yy = oy + py, xx = ox + px;
-        if (yy < 0 || yy >= a.H || xx < 0 || xx >= a.W) continue;
-        u64 d = 0;
-#pragma unroll
-        for (int aq = 0; aq < kBlock; ++aq) {
-            const int q = py - aq;
-            if (q >= 0 && q < DH) d += hsum[q * FW + px];
+    for (int py = wave; py < FH; py += nw) {
+        const int yy = oy + py;
+        if (yy < 0 || yy >= a.H) continue;
+        for (int px = lane; px < FW; px += 64) {
+            const int xx = ox + px;
+            const u64 v = accNum[py * FW + px];
+            if (v != 0 && xx >= 0 && xx < a.W) atomicAdd(&num[(size_t)yy * a.pitch + xx], v);
         }
-        if (d != 0) {
-            const size_t o = (size_t)yy * a.pitch + xx;
-            atomicAdd(&num[o], accNum[k]);
-            atomicAdd(&den[o], d);
+    }
+    for (int dy = wave; dy < DH; dy += nw) {
+        const int yy = oy + dy;
+        if (yy < 0 || yy > a.H - kBlock) continue;
+        for (int dx = lane; dx < DW; dx += 64) {
+            const int xx = ox + dx;
+            const u64 d = accDen[dy * DW + dx];
+            if (d != 0 && xx >= 0 && xx <= a.W - kBlock) atomicAdd(&den[(size_t)yy * a.pitch + xx], d);
         }
     }
 }
@@ -1004,7 +992,6 @@ __global__ __launch_bounds__(kThreads) void filter_kernel(FilterArgs a0, const u
     u64 *accDen = accNum + FW * FH;
     float *gbase = reinterpret_cast<float *>(smem) + filter_acc_floats(FW, FH);
     float *g = gbase + (size_t)wave * a.G * ZS;
-    u64 *hsum = reinterpret_cast<u64 *>(gbase);  // flush only, after every wave is done with its groups
     const int u = lane >> 3, v = lane & 7;
 
     for (int c = 0; c < 3; ++c) {
@@ -1031,7 +1018,7 @@ __global__ __launch_bounds__(kThreads) void filter_kernel(FilterArgs a0, const u
             else if (N <= 16) w = group_core<16, WIENER>(g, N, lane, a.c8, cn, src, gsrc, a.pitch, thr, sig2, ex);
             else w = group_core<32, WIENER>(g, N, lane, a.c8, cn, src, gsrc, a.pitch, thr, sig2, ex);
             const int o = gidx(0, lane);
-            const float ws = w * (kFixScale / 4294967296.0f);
+            const double ws = fix_weight(w);
             const int pl = lane < N ? pos[lane] : 0;
             for (int z = 0; z < N; ++z) {
                 const int pz = __builtin_amdgcn_readlane(pl, z);
@@ -1042,7 +1029,7 @@ __global__ __launch_bounds__(kThreads) void filter_kernel(FilterArgs a0, const u
             wave_sync();
         }
         __syncthreads();
-        flush_tile(a, c, accNum, accDen, hsum, ox, oy, tid);
+        flush_tile(a, c, accNum, accDen, ox, oy, tid);
         __syncthreads();
     }
 }
@@ -1071,7 +1058,6 @@ __global__ __launch_bounds__(kThreads) void filter_stack_kernel(FilterArgs a0, c
     u64 *accNum0 = reinterpret_cast<u64 *>(smem);
     float *gbase = reinterpret_cast<float *>(smem) + accw * nslots;
     float *g = gbase + (size_t)wave * a.G * ZS;
-    u64 *hsum = reinterpret_cast<u64 *>(gbase);  // flush only, after every wave is done with its groups
     const int u = lane >> 3, v = lane & 7;
     auto slot_num = [&](int f) { return accNum0 + (accw / 2) * (STACK ? f % nslots : 0); };
     auto zero_slot = [&](int f) {
@@ -1081,7 +1067,7 @@ __global__ __launch_bounds__(kThreads) void filter_stack_kernel(FilterArgs a0, c
     };
     auto flush_frame = [&](int c, int f) {
         const u64 *an = slot_num(f);
-        flush_tile(a, c, an, an + FW * FH, hsum, ox, oy, tid, kThreads, STACK ? a.fstride * f : 0);
+        flush_tile(a, c, an, an + FW * FH, ox, oy, tid, kThreads, STACK ? a.fstride * f : 0);
         __syncthreads();
     };
 
@@ -1118,7 +1104,7 @@ __global__ __launch_bounds__(kThreads) void filter_stack_kernel(FilterArgs a0, c
                     w = group_core<16, WIENER, STACK>(g, N, lane, a.c8, cn, src, gsrc, a.pitch, thr, sig2, ex);
                 else w = group_core<32, WIENER, STACK>(g, N, lane, a.c8, cn, src, gsrc, a.pitch, thr, sig2, ex);
                 const int o = gidx(0, lane);
-                const float ws = w * (kFixScale / 4294967296.0f);
+                const double ws = fix_weight(w);
                 const int pl = lane < N ? pos[lane] : 0;
                 const int sl = STACK && lane < N ? slot[lane] : 0;
                 for (int z = 0; z < N; ++z) {
@@ -1269,24 +1255,38 @@ __device__ __forceinline__ void mfma_load_x(const float *plane, int pitch, int o
     }
 }
 
-// The lane's rows of one member block (packed position pz, or zeros when pz < 0).
+// The staged footprint of the single-frame MFMA filter holds, for every
+// pixel x of a row, the 32-bit word (T(p[x]), T(p[x + 1])): a block row of 8
+// samples at any x is the words x, x + 2, x + 4, x + 6, read without
+// conversion (the samples are converted once, when the footprint is staged).
 template <typename T>
-__device__ __forceinline__ void mfma_load_block(const float *plane, int pitch, int ox, int oy, int pz, int h,
+__device__ __forceinline__ uint32_t pack_pair(float a, float b)
+{
+    typedef T t2 __attribute__((ext_vector_type(2)));
+    const t2 v = {(T)a, (T)b};
+    return __builtin_bit_cast(uint32_t, v);
+}
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// The lane's rows of one member block from a paired-word footprint (packed
+// position pz, or zeros when pz < 0).
+template <typename T>
+__device__ __forceinline__ void mfma_load_pairs(const uint32_t *plane, int pitch, int ox, int oy, int pz, int h,
                                                 typename MfmaT<T>::v8 (&xa)[4])
 {
+    typedef typename MfmaT<T>::v8 v8;
     if (pz >= 0) {
-        const float *b = plane + ((pz >> 16) - oy) * pitch + (pz & 0xFFFF) - ox;
+        const uint32_t *b = plane + ((pz >> 16) - oy) * pitch + (pz & 0xFFFF) - ox;
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            const float *r = b + (2 * s + h) * pitch;
-#pragma unroll
-            for (int e = 0; e < 8; ++e) xa[s][e] = (T)r[e];
+            const uint32_t *r = b + (2 * s + h) * pitch;
+            const u32x4 q = {r[0], r[2], r[4], r[6]};
+            xa[s] = __builtin_bit_cast(v8, q);
         }
     } else {
 #pragma unroll
-        for (int s = 0; s < 4; ++s)
-#pragma unroll
-            for (int e = 0; e < 8; ++e) xa[s][e] = (T)0.f;
+        for (int s = 0; s < 4; ++s) xa[s] = __builtin_bit_cast(v8, (u32x4){0u, 0u, 0u, 0u});
     }
 }
 
@@ -1346,20 +1346,17 @@ __device__ __forceinline__ void mfma_inverse(const f32x16 (&y)[2], const uint16_
     }
 }
 
-// LDS: accumulators, flush row sums, then the K64 operand fragments
-// (forward B and inverse A, 16 x 64 lanes x 8 elements of 16 bits).
 // LDS of the MFMA filter: accumulators, the K64 operand fragments (forward B
 // and inverse A, 16 x 64 lanes x 8 elements of 16 bits), then the staged
-// source (and guide) footprint; the flush's row sums reuse the staging area.
+// source (and guide) footprint, one 32-bit word per pixel.
 constexpr int kK64Elems = 16 * 64 * 8;
-// stack: nslots accumulator slots, no staged footprint (member blocks are
-// read from HBM), the flush's row sums in a region of their own.
+// stack: nslots accumulator slots and no staged footprint (member blocks are
+// read from HBM).
 __host__ __device__ inline size_t filter_mfma_lds(int FW, int FH, bool wiener, int nslots = 1, bool stack = false)
 {
     // single frame: the staged footprint, then the Wiener schedule (2 ints per unit + count)
     const size_t st = stack ? 0 : (size_t)FW * FH * (wiener ? 2 : 1) + (wiener ? 2 * kMaxTileRefs + 4 : 0);
-    const size_t hs = (size_t)(FH - kBlock + 1) * FW * 2;
-    return filter_acc_floats(FW, FH) * 4 * nslots + kK64Elems * 2 + (st > hs ? st : hs) * 4;
+    return filter_acc_floats(FW, FH) * 4 * nslots + kK64Elems * 2 + st * 4;
 }
 
 // 4 waves per SIMD: two 8-wave workgroups per CU (at most 128 registers)
@@ -1383,10 +1380,9 @@ __global__ __launch_bounds__(kMThreads, 4) void filter_mfma_kernel(FilterArgs a0
     u64 *accDen = accNum + FW * FH;
     uint16_t *kb = reinterpret_cast<uint16_t *>(reinterpret_cast<float *>(smem) + filter_acc_floats(FW, FH));
     uint16_t *ka = kb + kK64Elems / 2;
-    float *stage = reinterpret_cast<float *>(kb + kK64Elems);
-    float *lsrc = stage;
-    float *lgd = stage + FW * FH;
-    u64 *hsum = reinterpret_cast<u64 *>(stage);  // flush only
+    uint32_t *stage = reinterpret_cast<uint32_t *>(kb + kK64Elems);
+    uint32_t *lsrc = stage;  // paired words (mfma_load_pairs)
+    uint32_t *lgd = stage + FW * FH;
     {
         // the K64 fragments (kb then ka are contiguous in the table)
         const uint4 *srcq = reinterpret_cast<const uint4 *>(a.kb);
@@ -1447,14 +1443,19 @@ __global__ __launch_bounds__(kMThreads, 4) void filter_mfma_kernel(FilterArgs a0
         if (!a.active[pl]) continue;
         const float *src = a.src[pl];
         const float *gsrc = a.guide[pl];
-        for (int k = tid; k < FW * FH; k += kMThreads) {
-            accNum[k] = 0;
-            const int py = k / FW, px = k - py * FW;
-            const int yy = oy + py, xx = ox + px;
-            const bool in = yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
-            const size_t o = (size_t)yy * a.pitch + xx;
-            lsrc[k] = in ? src[o] : 0.f;
-            if (WIENER) lgd[k] = in ? gsrc[o] : 0.f;
+        // rows to the waves, columns to the lanes
+        for (int py = wave; py < FH; py += kMWaves) {
+            const int yy = oy + py;
+            const bool rin = yy >= 0 && yy < a.H;
+            for (int px = lane; px < FW; px += 64) {
+                const int k = py * FW + px, xx = ox + px;
+                accNum[k] = 0;
+                const bool in0 = rin && xx >= 0 && xx < a.W;
+                const bool in1 = rin && px + 1 < FW && xx + 1 >= 0 && xx + 1 < a.W;
+                const size_t o = (size_t)yy * a.pitch + xx;
+                lsrc[k] = pack_pair<T>(in0 ? src[o] : 0.f, in1 ? src[o + 1] : 0.f);
+                if (WIENER) lgd[k] = pack_pair<T>(in0 ? gsrc[o] : 0.f, in1 ? gsrc[o + 1] : 0.f);
+            }
         }
         for (int k = tid; k < DW * DH; k += kMThreads) accDen[k] = 0;
         __syncthreads();
@@ -1488,8 +1489,8 @@ __global__ __launch_bounds__(kMThreads, 4) void filter_mfma_kernel(FilterArgs a0
                 f32x16 vt[2];
                 float l20 = 0.f, l21 = 0.f;
                 v8 xg[4], xs[4];
-                mfma_load_block<T>(lgd, FW, ox, oy, pz, h, xg);
-                mfma_load_block<T>(lsrc, FW, ox, oy, pz, h, xs);
+                mfma_load_pairs<T>(lgd, FW, ox, oy, pz, h, xg);
+                mfma_load_pairs<T>(lsrc, FW, ox, oy, pz, h, xs);
 #pragma unroll
                 for (int q = 0; q < 2; ++q) {
                     const f32x16 yg = mfma_forward_half<T>(xg, q, kb, lane, cna);
@@ -1523,7 +1524,7 @@ __global__ __launch_bounds__(kMThreads, 4) void filter_mfma_kernel(FilterArgs a0
                 mfma_inverse_rest<T>(vt, ka, lane, xh);
                 const float l2 = g ? l21 : l20;
                 const float w = l2 <= 0.f ? 1.f : fminf(1.f / l2, a.wmax);
-                const float ws = w * (kFixScale / 4294967296.0f);
+                const double ws = fix_weight(w);
                 if (pz >= 0) {
                     const int py = (pz >> 16) - oy, px = (pz & 0xFFFF) - ox + 4 * h;
 #pragma unroll
@@ -1584,7 +1585,7 @@ __global__ __launch_bounds__(kMThreads, 4) void filter_mfma_kernel(FilterArgs a0
                 f32x16 xh[2];
                 if (!WIENER) {
                     v8 xs[4];
-                    mfma_load_block<T>(lsrc, FW, ox, oy, pz, h, xs);
+                    mfma_load_pairs<T>(lsrc, FW, ox, oy, pz, h, xs);
                     f32x16 y[2];
 #pragma unroll
                     for (int q = 0; q < 2; ++q) y[q] = mfma_forward_half<T>(xs, q, kb, lane, cna);
@@ -1607,8 +1608,8 @@ __global__ __launch_bounds__(kMThreads, 4) void filter_mfma_kernel(FilterArgs a0
                     f32x16 vt[2];
                     float l20 = 0.f, l21 = 0.f;
                     v8 xg[4], xs[4];
-                    mfma_load_block<T>(lgd, FW, ox, oy, pz, h, xg);
-                    mfma_load_block<T>(lsrc, FW, ox, oy, pz, h, xs);
+                    mfma_load_pairs<T>(lgd, FW, ox, oy, pz, h, xg);
+                    mfma_load_pairs<T>(lsrc, FW, ox, oy, pz, h, xs);
 #pragma unroll
                     for (int q = 0; q < 2; ++q) {
                         const f32x16 yg = mfma_forward_half<T>(xg, q, kb, lane, cna);
@@ -1641,7 +1642,7 @@ __global__ __launch_bounds__(kMThreads, 4) void filter_mfma_kernel(FilterArgs a0
                     mfma_inverse_rest<T>(vt, ka, lane, xh);
                 }
                 const float w = g ? w1 : w0;
-                const float ws = w * (kFixScale / 4294967296.0f);
+                const double ws = fix_weight(w);
                 if (pz >= 0) {
                     const int py = (pz >> 16) - oy, px = (pz & 0xFFFF) - ox + 4 * h;
 #pragma unroll
@@ -1657,7 +1658,7 @@ __global__ __launch_bounds__(kMThreads, 4) void filter_mfma_kernel(FilterArgs a0
         }
         }
         __syncthreads();
-        flush_tile(a, pl, accNum, accDen, hsum, ox, oy, tid, kMThreads);
+        flush_tile(a, pl, accNum, accDen, ox, oy, tid, kMThreads);
         __syncthreads();
     }
 }
@@ -1691,7 +1692,6 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_stack_kernel(FilterArgs
     float *stage = reinterpret_cast<float *>(kb + kK64Elems);
     float *lsrc = stage;          // single frame: the source footprint staged in LDS
     float *lgd = stage + fsz;     // and the Wiener guide's
-    u64 *hsum = reinterpret_cast<u64 *>(stage);  // flush only
     {
         // the K64 fragments (kb then ka are contiguous in the table)
         const uint4 *srcq = reinterpret_cast<const uint4 *>(a.kb);
@@ -1706,7 +1706,7 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_stack_kernel(FilterArgs
     };
     auto flush_frame = [&](int pl, int f) {
         const u64 *an = slot_num(f);
-        flush_tile(a, pl, an, an + fsz, hsum, ox, oy, tid, kMThreads, STACK ? a.fstride * f : 0);
+        flush_tile(a, pl, an, an + fsz, ox, oy, tid, kMThreads, STACK ? a.fstride * f : 0);
         __syncthreads();
     };
 
@@ -1799,7 +1799,7 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_stack_kernel(FilterArgs
                     w = l2 <= 0.f ? 1.f : fminf(1.f / l2, a.wmax);
                     mfma_inverse_rest<T>(vt, ka, lane, xh);
                 }
-                const float ws = w * (kFixScale / 4294967296.0f);
+                const double ws = fix_weight(w);
                 if (c < N) {
                     const int pz = pos[c];
                     const int py = (pz >> 16) - oy, px = (pz & 0xFFFF) - ox + 4 * h;
@@ -1825,21 +1825,65 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_stack_kernel(FilterArgs
 // num/den, then either the plane output (float, or int with RangeConvert
 // rounding and clip, include/Conversion.hpp:241-256) or OPP -> RGB
 // (MatrixConvert_YUV2RGB, include/Conversion.hpp:660-762) with clip, float or
-// int.  FP64 arithmetic.
+// int.  FP64 arithmetic.  With den per block origin (den_box = 8), a pixel's
+// den is the sum over the origins (oy, ox) with y - 8 < oy <= y and
+// x - 8 < ox <= x: the workgroup sums each column of its row segment (and
+// the 7 columns left of it) over those rows, 8 independent loads per column,
+// then each pixel adds 8 column sums from LDS.  Integer sums: exact in any
+// order.
 template <bool BATCHED>
-__global__ void finalize_kernel(FinalizeArgs a0, const uint32_t *__restrict__ bmap,
-                                const FinalizeArgs *__restrict__ bargs)
+__global__ __launch_bounds__(256) void finalize_kernel(FinalizeArgs a0, const uint32_t *__restrict__ bmap,
+                                                       const FinalizeArgs *__restrict__ bargs)
 {
     // separate mul and add roundings, as the reference's double code
 #pragma clang fp contract(off)
+    constexpr int kCols = 256 + kBlock - 1;
+    __shared__ u64 cols[3][kCols];
     int y;
     const FinalizeArgs &a = row_unit<BATCHED>(a0, bargs, bmap, y);
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= a.W) return;
+    const int x0 = blockIdx.x * 256;
+    if (x0 >= a.W) return;  // the whole workgroup (batches launch the widest image's row)
+    const int x = x0 + (int)threadIdx.x;
+    const bool in = x < a.W;
     const size_t o = (size_t)y * a.pitch + x;
+    if (a.den_box) {
+        for (int c = 0; c < a.nplanes; ++c) {
+            if (!a.active[c]) continue;
+            const u64 *den = a.den[c];
+            for (int j = threadIdx.x; j < kCols; j += 256) {
+                const int ox = x0 - (kBlock - 1) + j;
+                u64 sum = 0;
+                if (ox >= 0 && ox <= a.W - kBlock) {
+                    u64 v[kBlock];
+#pragma unroll
+                    for (int r = 0; r < kBlock; ++r) {
+                        const int oy = y - (kBlock - 1) + r;
+                        v[r] = oy >= 0 && oy <= a.H - kBlock ? den[(size_t)oy * a.pitch + ox] : 0;
+                    }
+#pragma unroll
+                    for (int r = 0; r < kBlock; ++r) sum += v[r];
+                }
+                cols[c][j] = sum;
+            }
+        }
+        __syncthreads();
+    }
+    if (!in) return;
     double val[3] = {0.0, 0.0, 0.0};
-    for (int c = 0; c < a.nplanes; ++c)
-        val[c] = a.active[c] ? (double)(long long)a.num[c][o] / (double)(long long)a.den[c][o] : (double)a.src[c][o];
+    for (int c = 0; c < a.nplanes; ++c) {
+        if (!a.active[c]) {
+            val[c] = (double)a.src[c][o];
+            continue;
+        }
+        u64 d = 0;
+        if (a.den_box) {
+#pragma unroll
+            for (int j = 0; j < kBlock; ++j) d += cols[c][threadIdx.x + j];
+        } else {
+            d = a.den[c][o];
+        }
+        val[c] = (double)(long long)a.num[c][o] / (double)(long long)d;
+    }
     const size_t oo = (size_t)y * a.out_pitch + x;
     double res[3];
     int n = a.nplanes;
@@ -2070,11 +2114,7 @@ size_t filter_mfma_lds_bytes(const FilterArgs &a, bool wiener) { return filter_m
 size_t filter_stack_lds_bytes(const FilterArgs &a, bool wiener, int precision)
 {
     if (precision != 0) return filter_mfma_lds(a.FW, a.FH, wiener, a.nslots, true);
-    const int DH = a.FH - kBlock + 1;
-    size_t groups = (size_t)kWaves * a.G * ZS;
-    const size_t hs = (size_t)DH * a.FW * 2;
-    if (hs > groups) groups = hs;
-    return (filter_acc_floats(a.FW, a.FH) * a.nslots + groups) * 4;
+    return (filter_acc_floats(a.FW, a.FH) * a.nslots + (size_t)kWaves * a.G * ZS) * 4;
 }
 
 hipError_t launch_filter_stack(const FilterArgs &a, bool wiener, int precision, hipStream_t s)
@@ -2119,6 +2159,7 @@ hipError_t launch_rows(K k1, K kb, const A &a, hipStream_t s, const BatchLaunch 
 
 hipError_t launch_finalize(const FinalizeArgs &a, hipStream_t s, const BatchLaunch *b)
 {
+    if (a.den_box != 0 && a.den_box != kBlock) return hipErrorInvalidValue;
     return launch_rows(finalize_kernel<false>, finalize_kernel<true>, a, s, b);
 }
 

@@ -245,7 +245,7 @@ __device__ __forceinline__ T block_sum(T v, T *red)
     return s;
 }
 
-// floor(v * 2^36) added to a 64-bit fixed-point accumulator in HBM.
+// floor(v * 2^32) added to a 64-bit fixed-point accumulator in HBM.
 __device__ __forceinline__ void fix_add_global(u64 *acc, double v)
 {
     atomicAdd(acc, (u64)__double2ll_rd(v * (double)kFixScale));

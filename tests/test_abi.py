@@ -91,7 +91,7 @@ def test_plan_on_the_host():
     assert lib.isp_bm3d_plan(ctypes.byref(p), 1920, 1080, 3, ctypes.byref(info)) == L.OK
     assert (info.ngx[0], info.ngy[0], info.ngx[1], info.ngy[1]) == (639, 359, 639, 359)
     assert list(info.generic) == [0, 0] and list(info.degenerate) == [0, 0]
-    assert info.weight_cap[1] == 2048.0
+    assert info.weight_cap[1] == 32768.0
     assert info.noalloc_bytes <= info.noalloc_capacity == 96 << 20  # a 1080p plane fits the scratch
     assert lib.isp_bm3d_plan(ctypes.byref(p), 3840, 2160, 1, ctypes.byref(info)) == L.OK
     assert info.noalloc_bytes > info.noalloc_capacity

@@ -192,7 +192,7 @@ def test_config0_512_grey(gpu):
 # ---------------------------------------------------------------- edges
 WIENER_CAP_CASES = {
     "np s3 R12 G32": dict(),
-    "s1 R19 G32": dict(step=1, R=19, G2=32),   # the most contributions per pixel: weight cap 128
+    "s1 R19 G32": dict(step=1, R=19, G2=32),   # the most contributions per pixel: weight cap 2048
     "high": dict(profile="high", step=None, R=None),
 }
 

@@ -1,7 +1,7 @@
 """Headroom of the 64-bit fixed-point aggregation sums (CPU only).
 
-The library sums num and den as value * 2^36 in 64-bit two's complement, so a
-pixel's sums must stay below 2^27.  isp_bm3d_aggregation_limits() states an
+The library sums num and den as value * 2^32 in 64-bit two's complement, so a
+pixel's sums must stay below 2^31.  isp_bm3d_aggregation_limits() states an
 upper bound on the (group, member) contributions to one pixel and the cap it
 puts on Wiener group weights.  Here the bound is checked against a brute-force
 count over every reference block's clipped search window (the oracle's grid
@@ -79,27 +79,28 @@ def test_worst_case_sums_fit(s, R, G, st, W, H):
     assert rc == L.OK
     assert cap >= 1 and cap <= 65536 and float(cap).is_integer() and (int(cap) & (int(cap) - 1)) == 0
     # every contribution at the cap with the largest allowed sample, exactly
-    worst = int(bound) * int(cap) * SAMPLE_BOUND * (1 << 36)
+    worst = int(bound) * int(cap) * SAMPLE_BOUND * (1 << 32)
     assert worst < (1 << 62)  # a factor 2 inside the signed 64-bit range
-    # the next power of two would not fit the 2^26 margin (the cap is the largest one)
+    # the next power of two would not fit the 2^30 margin (the cap is the largest one)
     if cap < 65536:
-        assert int(bound) * int(cap) * 2 * SAMPLE_BOUND >= (1 << 26)
-    # each addend's integer part (|w * x| * 2^4) fits int32
-    assert cap * SAMPLE_BOUND * 16 < 2 ** 31
+        assert int(bound) * int(cap) * 2 * SAMPLE_BOUND >= (1 << 30)
+    # each addend |w * x| * 2^32 stays inside the 2^51 where the FP64 rounding
+    # 1.5 * 2^56 + 16 * round(v) is exact
+    assert cap * SAMPLE_BOUND * 2 ** 32 < 2 ** 51
 
 
 def test_expected_caps():
     # the benchmark's final step: 12^2 reference blocks x 32 members
-    assert limits(3, 12, 32)[1:] == (4608.0, 2048.0)
+    assert limits(3, 12, 32)[1:] == (4608.0, 32768.0)
     # BlockStep 1, BMrange 19, G 32: 47^2 x 32 contributions
-    assert limits(1, 19, 32)[1:] == (70688.0, 128.0)
+    assert limits(1, 19, 32)[1:] == (70688.0, 2048.0)
     # three frames of multi-frame groups reach each frame
-    assert limits(3, 12, 32, radius=1)[1:] == (3 * 4608.0, 1024.0)
+    assert limits(3, 12, 32, radius=1)[1:] == (3 * 4608.0, 16384.0)
 
 
 def test_overflowing_parameters_are_rejected():
-    # BlockStep 1 with a 300-pixel window: even weights <= 1 could overflow
-    rc, bound, cap = limits(1, 300, 64)
+    # BlockStep 1 with a 1200-pixel window: even weights <= 1 could overflow
+    rc, bound, cap = limits(1, 1200, 64)
     assert rc == L.E_UNSUPPORTED and cap == 0
-    assert bound * SAMPLE_BOUND >= 2 ** 26
+    assert bound * SAMPLE_BOUND >= 2 ** 30
     assert limits(0, 12, 32)[0] == L.E_ARG
