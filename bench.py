@@ -869,6 +869,16 @@ def main() -> int:
                          "max_abs_vs_fp32": float(np.abs(out - ref32).max())}
         fp32_form = {"value": round(W * H / dt32 / 1e6, 3), "ms_per_step": round(dt32 * 1e3, 3),
                      "stages_ms": {k: round(v[0] / v[1], 4) for k, v in st32.items() if v[1]}}
+        # the FP32-throughout form (FP32 distances, no FP64 re-decision) beside it
+        ff = BM3D(para, device=0, precision="fp32fast")
+        yf = torch.empty_like(x)
+        dtf, stf = timed_steps(ff, yf, x, max(2, nfr // 4), 1)
+        of = yf.cpu().numpy()
+        fp32_form["fp32fast"] = {"value": round(W * H / dtf / 1e6, 3), "ms_per_step": round(dtf * 1e3, 3),
+                                 "stages_ms": {k: round(v[0] / v[1], 4) for k, v in stf.items() if v[1]},
+                                 "psnr_db": round(synth.psnr(of, clean), 4),
+                                 "max_abs_vs_fp32": float(np.abs(of - ref32).max())}
+        quality_extra["psnr_loss_vs_fp32fast_db"] = round(synth.psnr(of, clean) - psnr_out, 4)
     # end-to-end rate with host buffers (PCIe both ways); never the headline value
     hin = noisy
     hout = np.empty_like(hin)

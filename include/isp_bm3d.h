@@ -41,13 +41,16 @@ extern "C" {
 /* Arithmetic of the transforms and shrinkage, and of the matching distances.
  * PREC_FP32: FP32 transforms and shrinkage (hard-threshold decisions within
  * a narrow band of the threshold re-made in FP64), FP64 distances summed in
- * the reference's order (its tie order exactly).  PREC_FP16 / PREC_BF16:
- * 16-bit MFMA operands with FP32 accumulation, FP32 shrinkage, FP32
- * distances.  Every form aggregates in 64-bit fixed point (exact,
+ * the reference's order (its tie order exactly).  PREC_FP32_FAST: the same
+ * FP32 transforms and shrinkage without the FP64 re-decision, and FP32
+ * distances (fused multiply-add) as the mixed forms match.  PREC_FP16 /
+ * PREC_BF16: 16-bit MFMA operands with FP32 accumulation, FP32 shrinkage,
+ * FP32 distances.  Every form aggregates in 64-bit fixed point (exact,
  * order-independent sums; see isp_bm3d_aggregation_limits). */
-#define ISP_BM3D_PREC_FP32 0 /* plain FP32 path                              */
-#define ISP_BM3D_PREC_FP16 1 /* FP16 MFMA transforms, FP32 accumulation      */
-#define ISP_BM3D_PREC_BF16 2 /* BF16 MFMA transforms, FP32 accumulation      */
+#define ISP_BM3D_PREC_FP32 0      /* plain FP32 path, the reference's groups exactly */
+#define ISP_BM3D_PREC_FP16 1      /* FP16 MFMA transforms, FP32 accumulation        */
+#define ISP_BM3D_PREC_BF16 2      /* BF16 MFMA transforms, FP32 accumulation        */
+#define ISP_BM3D_PREC_FP32_FAST 3 /* FP32 throughout: FP32 distances, FP32 filter   */
 
 /* Mirror of BM3D_Para_Base (include/BM3D.h:15-45). */
 typedef struct isp_bm3d_step_params {

@@ -267,8 +267,9 @@ int derive(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, int nplanes, int W,
         d.sig2_64[c] = sigma * sigma;
         d.thr64[c] = sigma * p.lambda;
         // FP32 coefficients of [0,1] data carry absolute errors of a few 1e-6;
-        // decisions closer than this to the threshold are re-made in FP64.
-        d.band[c] = (float)(3e-5 + 1e-4 * d.thr64[c]);
+        // the plain form re-makes decisions closer than this to the threshold
+        // in FP64; the FP32-throughout form does not (a negative band)
+        d.band[c] = ctx->precision == ISP_BM3D_PREC_FP32_FAST ? -1.f : (float)(3e-5 + 1e-4 * d.thr64[c]);
     }
     d.ngx = (W - B) / p.block_step + 1 + ((W - B) % p.block_step ? 1 : 0);
     d.ngy = (H - B) / p.block_step + 1 + ((H - B) % p.block_step ? 1 : 0);
@@ -455,11 +456,20 @@ int size_class(int W, int H)
     return k;
 }
 
+// The context's engines: the FP16/BF16 matrix-core filter (engine 1 or 2)
+// or the FP32 vector filter (engine 0); matching with FP64 distances in the
+// reference's order (the plain FP32 form) or FP32 distances (every other).
+inline int filter_engine(const isp_bm3d_ctx *ctx)
+{
+    return ctx->precision == ISP_BM3D_PREC_FP16 ? 1 : ctx->precision == ISP_BM3D_PREC_BF16 ? 2 : 0;
+}
+inline bool exact_matching(const isp_bm3d_ctx *ctx) { return ctx->precision == ISP_BM3D_PREC_FP32; }
+
 // Tile shapes of an image: the tuning entry of its size class, else the defaults.
 isp_bm3d_ctx::Tiles tiles_for(const isp_bm3d_ctx *ctx, int W, int H)
 {
     isp_bm3d_ctx::Tiles t = ctx->tiles;
-    if (ctx->precision != ISP_BM3D_PREC_FP32) {
+    if (!exact_matching(ctx)) {
         t.match_x = t.walk_x;
         t.match_y = t.walk_y;
     }
@@ -467,7 +477,7 @@ isp_bm3d_ctx::Tiles tiles_for(const isp_bm3d_ctx *ctx, int W, int H)
     if (it != ctx->tune.end()) {
         t.match_x = it->second.match_tile[0];
         t.match_y = it->second.match_tile[1];
-        if (ctx->precision != ISP_BM3D_PREC_FP32) {
+        if (filter_engine(ctx)) {
             t.mfma_x = it->second.filter_tile[0];
             t.mfma_y = it->second.filter_tile[1];
         } else {
@@ -500,7 +510,7 @@ MatchArgs build_match(const isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, co
     a.distMul = d.distMul;
     // the plain FP32 path reproduces the reference's tie order bit for bit;
     // the mixed-precision forms make no parity claim and use fused FMA
-    a.exact = ctx->precision == ISP_BM3D_PREC_FP32;
+    a.exact = exact_matching(ctx);
     const int side = 2 * (d.range / p.bm_step) + 1;
     a.maxCand = std::min(side * side, kMatchListCap);
     a.cnt_out = cnt;
@@ -565,8 +575,8 @@ FilterArgs build_filter(const isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, 
     }
     f.c8d = ctx->d_tables64;
     f.cnd = ctx->d_tables64;
-    if (ctx->precision != ISP_BM3D_PREC_FP32) {
-        const uint16_t *t = ctx->d_mfma[ctx->precision == ISP_BM3D_PREC_FP16 ? 0 : 1];
+    if (filter_engine(ctx)) {
+        const uint16_t *t = ctx->d_mfma[filter_engine(ctx) - 1];
         f.kb = t;
         f.ka = t + 8 * 64 * 8;
         f.cna = t + 16 * 64 * 8;
@@ -579,13 +589,13 @@ FilterArgs build_filter(const isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, 
 
 size_t filter_lds(const isp_bm3d_ctx *ctx, const FilterArgs &f, bool wiener)
 {
-    return ctx->precision != ISP_BM3D_PREC_FP32 ? filter_mfma_lds_bytes(f, wiener) : filter_lds_bytes(f, wiener);
+    return filter_engine(ctx) ? filter_mfma_lds_bytes(f, wiener) : filter_lds_bytes(f, wiener);
 }
 
 hipError_t launch_filter_any(const isp_bm3d_ctx *ctx, const FilterArgs &f, bool wiener, hipStream_t s,
                              const BatchLaunch *b = nullptr)
 {
-    if (ctx->precision != ISP_BM3D_PREC_FP32) return launch_filter_mfma(f, wiener, ctx->precision, s, b);
+    if (filter_engine(ctx)) return launch_filter_mfma(f, wiener, filter_engine(ctx), s, b);
     return launch_filter(f, wiener, s, b);
 }
 
@@ -725,7 +735,7 @@ int run_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, cons
             StageTimer tm(ctx, wiener ? ST_FILTER_FINAL : ST_FILTER_BASIC, s);
             HIP_CHECK(ctx, launch_filter_generic(g, wiener, s));
         } else {
-            const bool mfma = ctx->precision != ISP_BM3D_PREC_FP32;
+            const bool mfma = filter_engine(ctx) != 0;
             const isp_bm3d_ctx::Tiles t = tiles_for(ctx, W, H);
             const FilterArgs f = build_filter(ctx, p, d, W, H, io.pitch, io.src, io.guide, w.num, w.den,
                                               w.cnt[k], w.pos[k], mfma ? t.mfma_x : t.filter_x,
@@ -1006,7 +1016,7 @@ int batch_params(isp_bm3d_batch *b, int k)
 int batch_args(isp_bm3d_batch *b)
 {
     isp_bm3d_ctx *ctx = b->ctx;
-    const bool mfma = ctx->precision != ISP_BM3D_PREC_FP32;
+    const bool mfma = filter_engine(ctx) != 0;
     int kinds[2];
     const int nst = batch_kinds(b, kinds);
     const bool final_only = b->mode == ISP_BM3D_FINAL;
@@ -1287,7 +1297,7 @@ size_t batch_layout(isp_bm3d_batch *b, void *base)
             x.cnt[s] = used ? c.take<int>(refs) : nullptr;
             x.pos[s] = used ? c.take<int>(refs * (size_t)std::max(G, 1)) : nullptr;
             if (used) {
-                const bool mf = b->ctx->precision != ISP_BM3D_PREC_FP32;
+                const bool mf = filter_engine(b->ctx) != 0;
                 const int tfx = mf ? x.tiles.mfma_x : x.tiles.filter_x, tfy = mf ? x.tiles.mfma_y : x.tiles.filter_y;
                 mt[s] += (size_t)((d.ngx + x.tiles.match_x - 1) / x.tiles.match_x) *
                          ((d.ngy + x.tiles.match_y - 1) / x.tiles.match_y);
@@ -1768,7 +1778,7 @@ int isp_bm3d_set_variant(isp_bm3d_ctx *ctx, int flags, double kaiser_beta)
 int isp_bm3d_set_precision(isp_bm3d_ctx *ctx, int precision)
 {
     if (!ctx) return fail(nullptr, ISP_BM3D_E_ARG, "null context");
-    if (precision < ISP_BM3D_PREC_FP32 || precision > ISP_BM3D_PREC_BF16)
+    if (precision < ISP_BM3D_PREC_FP32 || precision > ISP_BM3D_PREC_FP32_FAST)
         return fail(ctx, ISP_BM3D_E_ARG, "bad precision %d", precision);
     ctx->precision = precision;
     return ISP_BM3D_OK;
@@ -2531,15 +2541,15 @@ int isp_bm3d_autotune(isp_bm3d_ctx *ctx, int W, int H, double sigma, int flags, 
         (void)isp_bm3d_batch_destroy(b);
         return r;
     };
-    const bool mf = ctx->precision != ISP_BM3D_PREC_FP32;
+    const bool mf = filter_engine(ctx) != 0, walk = !exact_matching(ctx);
     const int mcands_v[][2] = {{8, 4}, {8, 8}, {16, 4}, {4, 8}, {16, 8}};
     const int mcands_m[][2] = {{4, 16}, {8, 16}, {4, 8}, {8, 8}, {8, 4}};
     const int fcands_m[][2] = {{8, 8}, {16, 8}, {8, 16}, {16, 16}};
     const int fcands_v[][2] = {{8, 4}, {8, 8}, {16, 4}, {4, 8}};
-    int best_m[2] = {mf ? ctx->tiles.walk_x : ctx->tiles.match_x, mf ? ctx->tiles.walk_y : ctx->tiles.match_y};
+    int best_m[2] = {walk ? ctx->tiles.walk_x : ctx->tiles.match_x, walk ? ctx->tiles.walk_y : ctx->tiles.match_y};
     int best_f[2] = {mf ? ctx->tiles.mfma_x : ctx->tiles.filter_x, mf ? ctx->tiles.mfma_y : ctx->tiles.filter_y};
     double best = 1e30, psnr_base = 0;
-    for (const auto &c : mf ? mcands_m : mcands_v) {
+    for (const auto &c : walk ? mcands_m : mcands_v) {
         double ms = 0;
         rc = measure(c, best_f, 0, &ms, nullptr);
         if (rc == ISP_BM3D_E_UNSUPPORTED) continue;
@@ -3010,7 +3020,7 @@ int stack_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, co
                     kMaxGroup);
     const size_t fs = io.fs;
     const int T = io.T, P = io.P;
-    const bool mfma = ctx->precision != ISP_BM3D_PREC_FP32;
+    const bool mfma = filter_engine(ctx) != 0;
     if (!d.degenerate) {
         const isp_bm3d_ctx::Tiles tl = tiles_for(ctx, W, H);
         const size_t refs = (size_t)d.ngx * d.ngy;
@@ -3072,7 +3082,7 @@ int stack_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, co
             f.wmax = wiener_weight_cap(kBlock, p.block_step, d.range, p.bm_step, p.group_size, io.radius);
             f.nslots = std::min(2 * io.radius + 1, T);
             f.fstride = fs;
-            if (filter_stack_lds_bytes(f, wiener, ctx->precision) <= 160 * 1024) break;
+            if (filter_stack_lds_bytes(f, wiener, filter_engine(ctx)) <= 160 * 1024) break;
             if (trx == 1 && try_ == 1)
                 return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "radius %d needs more LDS than a CU has", io.radius);
             if (trx >= try_ && trx > 1) trx = (trx + 1) / 2;
@@ -3082,7 +3092,7 @@ int stack_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, co
             return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "radius %d: a pixel's aggregation sums could overflow",
                         io.radius);
         StageTimer tm(ctx, wiener ? ST_FILTER_FINAL : ST_FILTER_BASIC, s);
-        HIP_CHECK(ctx, launch_filter_stack(f, wiener, ctx->precision, s));
+        HIP_CHECK(ctx, launch_filter_stack(f, wiener, filter_engine(ctx), s));
     }
     for (int t = 0; t < T; ++t) {
         FinalizeArgs fa{};

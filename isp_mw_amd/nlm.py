@@ -37,8 +37,8 @@ class NLMeans(_Filter):
     """NLMeans (include/NLMeans.h:25-59): FilterIF2 process / __call__ on a
     float plane [H, W], an int plane [H, W] or an int RGB frame [3, H, W];
     numpy arrays are host memory, torch tensors device memory.
-    precision "fp32" forms the reference's groups exactly; "fp16" / "bf16"
-    match with FP32 distances."""
+    precision "fp32" forms the reference's groups exactly; "fp32fast", "fp16"
+    and "bf16" match with FP32 distances."""
 
     def __init__(self, para: NLMeans_Para | None = None, device: int = 0, precision: str = "fp32"):
         super().__init__(c_params_default("fast"), device, precision)

@@ -1,5 +1,6 @@
-"""Parity of the headline form (FP16 / BF16 MFMA transforms, FP32 distances in
-matching) against the FP64 oracle.
+"""Parity of the FP32-distance forms against the FP64 oracle: the headline
+form (FP16 / BF16 MFMA transforms) and the FP32-throughout form (fp32fast:
+FP32 distances, FP32 transforms without FP64 re-decisions).
 
 Matching: the FP32-distance kernels (column walks for BlockStep 3..7, the list
 kernel otherwise) must reproduce the oracle's ordered groups
@@ -24,6 +25,12 @@ MAX_DPSNR_MIXED = 0.05
 # bf16 keeps 8 mantissa bits against fp16's 11)
 MAX_ABS_MIXED = {"fp16": 0.02, "bf16": 0.03}
 MEAN_ABS_MIXED = {"fp16": 5e-4, "bf16": 1.5e-3}
+# fp32fast against the oracle at 1080p RGB, about twice the measured values
+# (sigma 25: max 3.3e-3, 21 of 6.2M samples beyond 2e-3, mean 1.2e-6; sigma
+# 15: max 1.6e-3, mean 3.3e-7): FP32 group selection (5 + 2 of 229,401 groups
+# differ) and hard-threshold decisions without the FP64 re-decision
+MAX_ABS_FAST = 0.007
+MEAN_ABS_FAST = 3e-6
 
 
 def threads():
@@ -42,7 +49,7 @@ def key_check(c, keys, oc, ok, G):
 
 
 @pytest.mark.parametrize("walk", [True, False], ids=["walk", "list"])
-@pytest.mark.parametrize("prec", ["fp16", "bf16"])
+@pytest.mark.parametrize("prec", ["fp16", "bf16", "fp32fast"])
 def test_fp32_distance_groups_512_grey(gpu, prec, walk, monkeypatch):
     """configs[0] shape: 512x512 grey, BlockStep 3, BMrange 12, G 16 / 32."""
     monkeypatch.setenv("ISP_BM3D_MATCH_WALK", "1" if walk else "0")
@@ -65,7 +72,7 @@ def test_fp32_distance_groups_512_grey(gpu, prec, walk, monkeypatch):
         assert rel < 1e-4
 
 
-@pytest.mark.parametrize("prec", ["fp16", "bf16"])
+@pytest.mark.parametrize("prec", ["fp16", "bf16", "fp32fast"])
 def test_fp32_distance_groups_1080p_rgb(gpu, prec):
     """Groups of the benchmark frame as the fp16 / bf16 form forms them: step 1
     on the FP64 OPP luma of the noisy frame, step 2 (BM3D_Final) on a given
@@ -109,7 +116,7 @@ def test_fp16_1080p_rgb_vs_oracle(gpu, sigma):
     p = para(sigma=sigma)
     x = torch.from_numpy(noisy).to(gpu)
     outs = {}
-    for prec in ("fp32", "fp16", "bf16"):
+    for prec in ("fp32", "fp32fast", "fp16", "bf16"):
         outs[prec] = BM3D(p, precision=prec)(x).cpu().numpy().astype(np.float64)
     say(f"1080p sigma {sigma}: oracle")
     oracle.set_threads(threads())
@@ -122,6 +129,17 @@ def test_fp16_1080p_rgb_vs_oracle(gpu, sigma):
     p32 = synth.psnr(outs["fp32"], clean)
     say(f"1080p sigma {sigma}: PSNR noisy {synth.psnr(noisy, clean):.3f} oracle {p_ref:.4f} fp32 {p32:.4f}")
     assert abs(p32 - p_ref) <= 0.01
+    out = outs["fp32fast"]
+    pf = synth.psnr(out, clean)
+    mx = [float(np.abs(out[c] - ref[c]).max()) for c in range(3)]
+    mean = [float(np.abs(out[c] - ref[c]).mean()) for c in range(3)]
+    big = int(sum((np.abs(out[c] - ref[c]) > 2e-3).sum() for c in range(3)))
+    say(f"  fp32fast: PSNR {pf:.4f} ({pf - p_ref:+.5f} dB vs oracle); |out - oracle| per channel "
+        f"R,G,B max {mx[0]:.3g} {mx[1]:.3g} {mx[2]:.3g} mean {mean[0]:.3g} {mean[1]:.3g} {mean[2]:.3g}; "
+        f"{big} samples beyond 2e-3")
+    assert abs(pf - p_ref) <= 0.01
+    assert max(mx) <= MAX_ABS_FAST
+    assert max(mean) <= MEAN_ABS_FAST
     for prec in ("fp16", "bf16"):
         out = outs[prec]
         pm = synth.psnr(out, clean)
