@@ -1484,35 +1484,50 @@ __global__ __launch_bounds__(kMThreads, 4) void filter_mfma_kernel(FilterArgs a0
                         cnb[m] = load_frag<T>(tb, fi * 2 + m, lane);
                     }
                 }
-                // one column half at a time: guide and source forward, gains,
-                // then that half's first inverse product
-                f32x16 vt[2];
+                // the guide's spectrum first, one column half at a time, and
+                // from it the gains (the guide's operand is dead before the
+                // source's is loaded); then the source's halves, shrunk and
+                // taken through the first inverse product
+                f32x16 wg[2], vt[2];
                 float l20 = 0.f, l21 = 0.f;
-                v8 xg[4], xs[4];
-                mfma_load_pairs<T>(lgd, FW, ox, oy, pz, h, xg);
-                mfma_load_pairs<T>(lsrc, FW, ox, oy, pz, h, xs);
+                {
+                    v8 xg[4];
+                    mfma_load_pairs<T>(lgd, FW, ox, oy, pz, h, xg);
 #pragma unroll
-                for (int q = 0; q < 2; ++q) {
-                    const f32x16 yg = mfma_forward_half<T>(xg, q, kb, lane, cna);
-                    f32x16 yq = mfma_forward_half<T>(xs, q, kb, lane, cna);
-                    // column 32q + c = frequency (u, v): u = 4q + c/8, v = c%8
-                    const float su = sig2 * ((4 * q + (c >> 3)) == 0 ? 0.5f : 1.f) * ((c & 7) == 0 ? 0.5f : 1.f);
+                    for (int q = 0; q < 2; ++q) {
+                        wg[q] = mfma_forward_half<T>(xg, q, kb, lane, cna);
+                        // column 32q + c = frequency (u, v): u = 4q + c/8, v = c%8
+                        const float su =
+                            sig2 * ((4 * q + (c >> 3)) == 0 ? 0.5f : 1.f) * ((c & 7) == 0 ? 0.5f : 1.f);
 #pragma unroll
-                    for (int k = 0; k < 16; ++k) {
-                        // group-axis frequency: the row within the group
-                        // group-axis DC: row 8(k/4) + 4h + k%4 is 0 (or 16 when paired),
-                        // possible only where k % 4 == 0 (static) and h == 0
-                        const bool kz0 = (k & 3) == 0 && h == 0 && (paired ? ((k >> 2) & 1) == 0 : k == 0);
-                        const float yy = yg[k] * yg[k];
-                        // 1-ulp reciprocal: the operands already carry 16-bit rounding
-                        const float wc = yy * __builtin_amdgcn_rcpf(yy + (kz0 ? 0.5f * su : su));
-                        yq[k] *= wc;
-                        // rows 0..15 (elements k < 8) and 16..31 apart; one group
-                        // adds them below (the order of the stack kernel's sum)
-                        if (k < 8) l20 = fmaf(wc, wc, l20);
-                        else l21 = fmaf(wc, wc, l21);
+                        for (int k = 0; k < 16; ++k) {
+                            // group-axis frequency: the row within the group
+                            // group-axis DC: row 8(k/4) + 4h + k%4 is 0 (or 16 when paired),
+                            // possible only where k % 4 == 0 (static) and h == 0
+                            const bool kz0 = (k & 3) == 0 && h == 0 && (paired ? ((k >> 2) & 1) == 0 : k == 0);
+                            const float yy = wg[q][k] * wg[q][k];
+                            // 1-ulp reciprocal: the operands already carry 16-bit rounding
+                            const float wc = yy * __builtin_amdgcn_rcpf(yy + (kz0 ? 0.5f * su : su));
+                            wg[q][k] = wc;
+                            // rows 0..15 (elements k < 8) and 16..31 apart; one group
+                            // adds them below (the order of the stack kernel's sum)
+                            if (k < 8) l20 = fmaf(wc, wc, l20);
+                            else l21 = fmaf(wc, wc, l21);
+                        }
                     }
-                    vt[q] = mfma_inverse_half<T>(yq, cnb);
+                }
+                // keep the source's loads after the gains (register pressure)
+                __builtin_amdgcn_sched_barrier(0);
+                {
+                    v8 xs[4];
+                    mfma_load_pairs<T>(lsrc, FW, ox, oy, pz, h, xs);
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) {
+                        f32x16 yq = mfma_forward_half<T>(xs, q, kb, lane, cna);
+#pragma unroll
+                        for (int k = 0; k < 16; ++k) yq[k] *= wg[q][k];
+                        vt[q] = mfma_inverse_half<T>(yq, cnb);
+                    }
                 }
                 if (!paired) {
                     l20 += l21;
@@ -1581,66 +1596,26 @@ __global__ __launch_bounds__(kMThreads, 4) void filter_mfma_kernel(FilterArgs a0
                 }
                 // accumulator element k of this lane lies on row 8(k/4) + 4h + k%4,
                 // of group 1 when paired and k >= 8
-                float w0, w1;
+                // (the Wiener step takes the schedule above: here hard thresholds only)
                 f32x16 xh[2];
-                if (!WIENER) {
-                    v8 xs[4];
-                    mfma_load_pairs<T>(lsrc, FW, ox, oy, pz, h, xs);
-                    f32x16 y[2];
+                v8 xs[4];
+                mfma_load_pairs<T>(lsrc, FW, ox, oy, pz, h, xs);
+                f32x16 y[2];
 #pragma unroll
-                    for (int q = 0; q < 2; ++q) y[q] = mfma_forward_half<T>(xs, q, kb, lane, cna);
-                    int kept = 0;  // group 0 in the low 16 bits, group 1 above
+                for (int q = 0; q < 2; ++q) y[q] = mfma_forward_half<T>(xs, q, kb, lane, cna);
+                int kept = 0;  // group 0 in the low 16 bits, group 1 above
 #pragma unroll
-                    for (int q = 0; q < 2; ++q)
+                for (int q = 0; q < 2; ++q)
 #pragma unroll
-                        for (int k = 0; k < 16; ++k) {
-                            if (fabsf(y[q][k]) <= thr) y[q][k] = 0.f;
-                            else kept += (paired && k >= 8) ? 65536 : 1;
-                        }
-                    kept = wave_sum(kept);
-                    const int k0 = kept & 0xFFFF, k1 = kept >> 16;
-                    w0 = k0 < 1 ? 1.f : 1.f / (float)k0;
-                    w1 = k1 < 1 ? 1.f : 1.f / (float)k1;
-                    mfma_inverse<T>(y, ka, lane, cnb, xh);
-                } else {
-                    // one column half at a time: guide and source forward, gains,
-                    // then that half's first inverse product
-                    f32x16 vt[2];
-                    float l20 = 0.f, l21 = 0.f;
-                    v8 xg[4], xs[4];
-                    mfma_load_pairs<T>(lgd, FW, ox, oy, pz, h, xg);
-                    mfma_load_pairs<T>(lsrc, FW, ox, oy, pz, h, xs);
-#pragma unroll
-                    for (int q = 0; q < 2; ++q) {
-                        const f32x16 yg = mfma_forward_half<T>(xg, q, kb, lane, cna);
-                        f32x16 yq = mfma_forward_half<T>(xs, q, kb, lane, cna);
-                        // column 32q + c = frequency (u, v): u = 4q + c/8, v = c%8
-                        const float su =
-                            sig2 * ((4 * q + (c >> 3)) == 0 ? 0.5f : 1.f) * ((c & 7) == 0 ? 0.5f : 1.f);
-#pragma unroll
-                        for (int k = 0; k < 16; ++k) {
-                            // group-axis frequency (row within the group)
-                            // group-axis DC: row 8(k/4) + 4h + k%4 is 0 (or 16 when paired),
-                            // possible only where k % 4 == 0 (static) and h == 0
-                            const bool kz0 = (k & 3) == 0 && h == 0 && (paired ? ((k >> 2) & 1) == 0 : k == 0);
-                            const float yy = yg[k] * yg[k];
-                            // 1-ulp reciprocal: the operands already carry 16-bit rounding
-                            const float wc = yy * __builtin_amdgcn_rcpf(yy + (kz0 ? 0.5f * su : su));
-                            yq[k] *= wc;
-                            if (paired && k >= 8) l21 = fmaf(wc, wc, l21);
-                            else l20 = fmaf(wc, wc, l20);
-                        }
-                        vt[q] = mfma_inverse_half<T>(yq, cnb);
+                    for (int k = 0; k < 16; ++k) {
+                        if (fabsf(y[q][k]) <= thr) y[q][k] = 0.f;
+                        else kept += (paired && k >= 8) ? 65536 : 1;
                     }
-                    l20 = wave_sum(l20);
-                    w0 = l20 <= 0.f ? 1.f : fminf(1.f / l20, a.wmax);
-                    w1 = 1.f;
-                    if (paired) {
-                        l21 = wave_sum(l21);
-                        w1 = l21 <= 0.f ? 1.f : fminf(1.f / l21, a.wmax);
-                    }
-                    mfma_inverse_rest<T>(vt, ka, lane, xh);
-                }
+                kept = wave_sum(kept);
+                const int k0 = kept & 0xFFFF, k1 = kept >> 16;
+                const float w0 = k0 < 1 ? 1.f : 1.f / (float)k0;
+                const float w1 = k1 < 1 ? 1.f : 1.f / (float)k1;
+                mfma_inverse<T>(y, ka, lane, cnb, xh);
                 const float w = g ? w1 : w0;
                 const double ws = fix_weight(w);
                 if (pz >= 0) {
