@@ -1360,8 +1360,10 @@ __host__ __device__ inline size_t filter_mfma_lds(int FW, int FH, bool wiener, i
 }
 
 // 4 waves per SIMD: two 8-wave workgroups per CU (at most 128 registers)
-template <typename T, bool WIENER, bool BATCHED>
-__global__ __launch_bounds__(kMThreads, 4) void filter_mfma_kernel(FilterArgs a0, const uint32_t *__restrict__ bmap,
+// NW waves per workgroup: 8 (two workgroups per CU) or 16 (one workgroup of a
+// larger tile per CU); 4 waves per SIMD either way, 128 registers.
+template <typename T, bool WIENER, bool BATCHED, int NW = kMWaves>
+__global__ __launch_bounds__(NW * 64, 4) void filter_mfma_kernel(FilterArgs a0, const uint32_t *__restrict__ bmap,
                                                                 const FilterArgs *__restrict__ bargs)
 {
     typedef typename MfmaT<T>::v8 v8;
@@ -1387,7 +1389,7 @@ __global__ __launch_bounds__(kMThreads, 4) void filter_mfma_kernel(FilterArgs a0
         // the K64 fragments (kb then ka are contiguous in the table)
         const uint4 *srcq = reinterpret_cast<const uint4 *>(a.kb);
         uint4 *dstq = reinterpret_cast<uint4 *>(kb);
-        for (int k = tid; k < kK64Elems / 8; k += kMThreads) dstq[k] = srcq[k];
+        for (int k = tid; k < kK64Elems / 8; k += NW * 64) dstq[k] = srcq[k];
     }
 
     // Wiener step: the tile's schedule of units, consecutive reference blocks
@@ -1444,7 +1446,7 @@ __global__ __launch_bounds__(kMThreads, 4) void filter_mfma_kernel(FilterArgs a0
         const float *src = a.src[pl];
         const float *gsrc = a.guide[pl];
         // rows to the waves, columns to the lanes
-        for (int py = wave; py < FH; py += kMWaves) {
+        for (int py = wave; py < FH; py += NW) {
             const int yy = oy + py;
             const bool rin = yy >= 0 && yy < a.H;
             for (int px = lane; px < FW; px += 64) {
@@ -1457,14 +1459,14 @@ __global__ __launch_bounds__(kMThreads, 4) void filter_mfma_kernel(FilterArgs a0
                 if (WIENER) lgd[k] = pack_pair<T>(in0 ? gsrc[o] : 0.f, in1 ? gsrc[o + 1] : 0.f);
             }
         }
-        for (int k = tid; k < DW * DH; k += kMThreads) accDen[k] = 0;
+        for (int k = tid; k < DW * DH; k += NW * 64) accDen[k] = 0;
         __syncthreads();
         const float thr = a.thr[pl], sig2 = a.sig2[pl];
         if constexpr (WIENER) {
             // Units of work from the tile's schedule: a pair of groups of at
             // most 16 blocks sharing one operand (as in step 1), or one group.
             const int nunits = sched[2 * kMaxTileRefs];
-            for (int un = wave; un < nunits; un += kMWaves) {
+            for (int un = wave; un < nunits; un += NW) {
                 const int2 unit = reinterpret_cast<const int2 *>(sched)[un];
                 const size_t refA = (size_t)unit.x;
                 const bool paired = unit.y >= 0;
@@ -1558,7 +1560,7 @@ __global__ __launch_bounds__(kMThreads, 4) void filter_mfma_kernel(FilterArgs a0
         // group-axis DCT block-diagonal); otherwise each group takes a pass
         // of its own on rows 0..N-1.
         const int npairs = (tnum + 1) / 2;
-        for (int pi = wave; pi < npairs; pi += kMWaves) {
+        for (int pi = wave; pi < npairs; pi += NW) {
             int NA = 0, NB = 0;
             size_t refA = 0, refB = 0;
             {
@@ -1633,7 +1635,7 @@ __global__ __launch_bounds__(kMThreads, 4) void filter_mfma_kernel(FilterArgs a0
         }
         }
         __syncthreads();
-        flush_tile(a, pl, accNum, accDen, ox, oy, tid, kMThreads);
+        flush_tile(a, pl, accNum, accDen, ox, oy, tid, NW * 64);
         __syncthreads();
     }
 }
@@ -2107,11 +2109,17 @@ hipError_t launch_filter_stack(const FilterArgs &a, bool wiener, int precision, 
                   : launch_tiled(filter_mfma_stack_kernel<__bf16, false, false, true>, kMThreads, lds, a, s, nullptr, grid);
 }
 
+// Tiles of at least kMWideRefs reference blocks take 16-wave workgroups
+// (single launches; a batch's images share one workgroup size and take 8).
+constexpr int kMWideRefs = 128;
+
 template <typename T, bool W>
 hipError_t launch_mfma_t(const FilterArgs &a, size_t lds, dim3 grid, hipStream_t s, const BatchLaunch *b)
 {
-    return b ? launch_tiled(filter_mfma_kernel<T, W, true>, kMThreads, lds, a, s, b, grid)
-             : launch_tiled(filter_mfma_kernel<T, W, false>, kMThreads, lds, a, s, b, grid);
+    if (b) return launch_tiled(filter_mfma_kernel<T, W, true>, kMThreads, lds, a, s, b, grid);
+    if (a.trx * a.try_ >= kMWideRefs)
+        return launch_tiled(filter_mfma_kernel<T, W, false, 2 * kMWaves>, 2 * kMThreads, lds, a, s, b, grid);
+    return launch_tiled(filter_mfma_kernel<T, W, false>, kMThreads, lds, a, s, b, grid);
 }
 
 hipError_t launch_filter_mfma(const FilterArgs &a, bool wiener, int precision, hipStream_t s, const BatchLaunch *b)

@@ -65,3 +65,18 @@ def test_mixed_error_vs_fp64_oracle_per_channel(gpu):
         errs = [float(np.abs(out[c] - ref[c]).max()) for c in range(3)]
         print(f"{prec}: max|out-oracle| per channel R,G,B = {errs}", flush=True)
         assert max(errs) < {"fp32": 2e-3, "fp16": 0.03, "bf16": 0.05}[prec]
+
+
+@pytest.mark.parametrize("prec", ["fp16", "bf16"])
+@pytest.mark.parametrize("tile", [(16, 8), (8, 16), (16, 16), (4, 4)])
+def test_filter_tile_shapes_do_not_change_results(gpu, prec, tile):
+    """Filter tiles of 128 reference blocks or more run 16-wave workgroups,
+    smaller ones 8-wave workgroups; the fixed-point sums are integers, so the
+    output equals the default 8x8 tile's bit for bit."""
+    clean, n = synth.rgb_pair(150, 200, 25.0, 31)
+    p = para()
+    want = BM3D(p, precision=prec)(n)
+    f = BM3D(p, precision=prec)
+    f.set_tuning(f.size_class(200, 150), {"match_tile": (4, 16), "filter_tile": tile, "final_group_size": 0})
+    print(f"{prec} filter tile {tile}", flush=True)
+    np.testing.assert_array_equal(f(n), want)
