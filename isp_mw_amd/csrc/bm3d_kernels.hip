@@ -2109,17 +2109,22 @@ hipError_t launch_filter_stack(const FilterArgs &a, bool wiener, int precision, 
                   : launch_tiled(filter_mfma_stack_kernel<__bf16, false, false, true>, kMThreads, lds, a, s, nullptr, grid);
 }
 
-// Tiles of at least kMWideRefs reference blocks take 16-wave workgroups
-// (single launches; a batch's images share one workgroup size and take 8).
+// 16-wave workgroups where only one workgroup fits a CU's 160 KB of LDS
+// (large footprints: BMrange 19 at the default tile), so that a CU still
+// holds 4 waves per SIMD, and for single launches of tiles of at least
+// kMWideRefs reference blocks.  The workgroup size never changes results.
 constexpr int kMWideRefs = 128;
+constexpr size_t kLdsPerCu = 160 * 1024;
 
 template <typename T, bool W>
 hipError_t launch_mfma_t(const FilterArgs &a, size_t lds, dim3 grid, hipStream_t s, const BatchLaunch *b)
 {
-    if (b) return launch_tiled(filter_mfma_kernel<T, W, true>, kMThreads, lds, a, s, b, grid);
-    if (a.trx * a.try_ >= kMWideRefs)
-        return launch_tiled(filter_mfma_kernel<T, W, false, 2 * kMWaves>, 2 * kMThreads, lds, a, s, b, grid);
-    return launch_tiled(filter_mfma_kernel<T, W, false>, kMThreads, lds, a, s, b, grid);
+    const bool wide = 2 * lds > kLdsPerCu || (!b && a.trx * a.try_ >= kMWideRefs);
+    if (b)
+        return wide ? launch_tiled(filter_mfma_kernel<T, W, true, 2 * kMWaves>, 2 * kMThreads, lds, a, s, b, grid)
+                    : launch_tiled(filter_mfma_kernel<T, W, true>, kMThreads, lds, a, s, b, grid);
+    return wide ? launch_tiled(filter_mfma_kernel<T, W, false, 2 * kMWaves>, 2 * kMThreads, lds, a, s, b, grid)
+                : launch_tiled(filter_mfma_kernel<T, W, false>, kMThreads, lds, a, s, b, grid);
 }
 
 hipError_t launch_filter_mfma(const FilterArgs &a, bool wiener, int precision, hipStream_t s, const BatchLaunch *b)

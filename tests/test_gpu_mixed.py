@@ -80,3 +80,24 @@ def test_filter_tile_shapes_do_not_change_results(gpu, prec, tile):
     f.set_tuning(f.size_class(200, 150), {"match_tile": (4, 16), "filter_tile": tile, "final_group_size": 0})
     print(f"{prec} filter tile {tile}", flush=True)
     np.testing.assert_array_equal(f(n), want)
+
+
+@pytest.mark.parametrize("prec", ["fp16", "bf16"])
+def test_wide_workgroups_at_bmrange_19(gpu, prec):
+    """At BMrange 19 the default 8x8 tile's footprint leaves room for one
+    workgroup per CU, which then takes 16 waves; a 4x2 tile fits two 8-wave
+    workgroups.  Both give the same bits, single calls and batches alike."""
+    import torch
+    from isp_mw_amd import BM3DBatch
+    clean, n = synth.rgb_pair(120, 160, 25.0, 33)
+    p = para(R=19, G1=32, G2=32)
+    wide = BM3D(p, precision=prec)(n)
+    f = BM3D(p, precision=prec)
+    f.set_tuning(f.size_class(160, 120), {"match_tile": (4, 16), "filter_tile": (4, 2), "final_group_size": 0})
+    np.testing.assert_array_equal(f(n), wide)
+    g = BM3D(p, precision=prec)
+    x = torch.from_numpy(n).cuda()
+    y = torch.empty_like(x)
+    BM3DBatch(g, [y], [x]).run()
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(y.cpu().numpy(), wide)
