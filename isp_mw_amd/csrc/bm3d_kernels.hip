@@ -1642,8 +1642,10 @@ __global__ __launch_bounds__(NW * 64, 4) void filter_mfma_kernel(FilterArgs a0, 
 
 // STACK: multi-frame groups over a whole stack (see FilterArgs and
 // filter_stack_kernel): a ring of accumulator slots, member blocks read from HBM.
-template <typename T, bool WIENER, bool BATCHED, bool STACK>
-__global__ __launch_bounds__(kMThreads) void filter_mfma_stack_kernel(FilterArgs a0,
+// NW waves per workgroup as in filter_mfma_kernel (16 where one workgroup
+// fills a CU's LDS), at most 128 registers.
+template <typename T, bool WIENER, bool BATCHED, bool STACK, int NW = kMWaves>
+__global__ __launch_bounds__(NW * 64, 4) void filter_mfma_stack_kernel(FilterArgs a0,
                                                                       const uint32_t *__restrict__ bmap,
                                                                       const FilterArgs *__restrict__ bargs)
 {
@@ -1673,17 +1675,17 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_stack_kernel(FilterArgs
         // the K64 fragments (kb then ka are contiguous in the table)
         const uint4 *srcq = reinterpret_cast<const uint4 *>(a.kb);
         uint4 *dstq = reinterpret_cast<uint4 *>(kb);
-        for (int k = tid; k < kK64Elems / 8; k += kMThreads) dstq[k] = srcq[k];
+        for (int k = tid; k < kK64Elems / 8; k += NW * 64) dstq[k] = srcq[k];
     }
     auto slot_num = [&](int f) { return accNum0 + (accw / 2) * (STACK ? f % nslots : 0); };
     auto zero_slot = [&](int f) {
         u64 *an = slot_num(f), *ad = an + fsz;
-        for (int k = tid; k < fsz; k += kMThreads) an[k] = 0;
-        for (int k = tid; k < DW * DH; k += kMThreads) ad[k] = 0;
+        for (int k = tid; k < fsz; k += NW * 64) an[k] = 0;
+        for (int k = tid; k < DW * DH; k += NW * 64) ad[k] = 0;
     };
     auto flush_frame = [&](int pl, int f) {
         const u64 *an = slot_num(f);
-        flush_tile(a, pl, an, an + fsz, ox, oy, tid, kMThreads, STACK ? a.fstride * f : 0);
+        flush_tile(a, pl, an, an + fsz, ox, oy, tid, NW * 64, STACK ? a.fstride * f : 0);
         __syncthreads();
     };
 
@@ -1692,7 +1694,7 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_stack_kernel(FilterArgs
         const float *src = a.src[pl];
         const float *gsrc = a.guide[pl];
         if (!STACK) {
-            for (int k = tid; k < fsz; k += kMThreads) {
+            for (int k = tid; k < fsz; k += NW * 64) {
                 const int py = k / FW, px = k - py * FW;
                 const int yy = oy + py, xx = ox + px;
                 const bool in = yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
@@ -1714,7 +1716,7 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_stack_kernel(FilterArgs
                 zero_slot(t - rad - 1);
                 __syncthreads();
             }
-            for (int r = wave; r < tnum; r += kMWaves) {
+            for (int r = wave; r < tnum; r += NW) {
                 const int gxi = tx0 + (r % trx), gyi = ty0 + (r / trx);
                 if (gxi >= a.ngx || gyi >= a.ngy) continue;
                 const size_t refIdx = (size_t)t * refs + (size_t)gyi * a.ngx + gxi;
@@ -1745,32 +1747,45 @@ __global__ __launch_bounds__(kMThreads) void filter_mfma_stack_kernel(FilterArgs
                     w = kept < 1 ? 1.f : 1.f / (float)kept;
                     mfma_inverse<T>(y, ka, lane, cnb, xh);
                 } else {
-                    // as filter_mfma_kernel: one column half at a time
-                    v8 xg[4], xs[4];
-                    mfma_load_x<T>(fgd, fp, fox, foy, pos, N, lane, xg, slot, sstride);
-                    mfma_load_x<T>(fsrc, fp, fox, foy, pos, N, lane, xs, slot, sstride);
-                    f32x16 vt[2];
+                    // as filter_mfma_kernel: the guide's spectrum and the gains
+                    // first, then the source's halves
+                    f32x16 wg[2], vt[2];
                     float l2 = 0.f, l2b = 0.f;
+                    {
+                        v8 xg[4];
+                        mfma_load_x<T>(fgd, fp, fox, foy, pos, N, lane, xg, slot, sstride);
 #pragma unroll
-                    for (int q = 0; q < 2; ++q) {
-                        const f32x16 yg = mfma_forward_half<T>(xg, q, kb, lane, cna);
-                        f32x16 yq = mfma_forward_half<T>(xs, q, kb, lane, cna);
-                        // column 32q + c = frequency (u, v): u = 4q + c/8, v = c%8
-                        const float su =
-                            sig2 * ((4 * q + (c >> 3)) == 0 ? 0.5f : 1.f) * ((c & 7) == 0 ? 0.5f : 1.f);
+                        for (int q = 0; q < 2; ++q) {
+                            wg[q] = mfma_forward_half<T>(xg, q, kb, lane, cna);
+                            // column 32q + c = frequency (u, v): u = 4q + c/8, v = c%8
+                            const float su =
+                                sig2 * ((4 * q + (c >> 3)) == 0 ? 0.5f : 1.f) * ((c & 7) == 0 ? 0.5f : 1.f);
 #pragma unroll
-                        for (int k = 0; k < 16; ++k) {
-                            const bool kz0 = k == 0 && h == 0;  // group-axis DC (row 0)
-                            const float yy = yg[k] * yg[k];
-                            // 1-ulp reciprocal: the operands already carry 16-bit rounding
-                            const float wc = yy * __builtin_amdgcn_rcpf(yy + (kz0 ? 0.5f * su : su));
-                            yq[k] *= wc;
-                            // rows 0..15 and 16..31 apart, then added (the order of
-                            // the single-frame kernel, whose operands pair groups)
-                            if (k < 8) l2 = fmaf(wc, wc, l2);
-                            else l2b = fmaf(wc, wc, l2b);
+                            for (int k = 0; k < 16; ++k) {
+                                const bool kz0 = k == 0 && h == 0;  // group-axis DC (row 0)
+                                const float yy = wg[q][k] * wg[q][k];
+                                // 1-ulp reciprocal: the operands already carry 16-bit rounding
+                                const float wc = yy * __builtin_amdgcn_rcpf(yy + (kz0 ? 0.5f * su : su));
+                                wg[q][k] = wc;
+                                // rows 0..15 and 16..31 apart, then added (the order of
+                                // the single-frame kernel, whose operands pair groups)
+                                if (k < 8) l2 = fmaf(wc, wc, l2);
+                                else l2b = fmaf(wc, wc, l2b);
+                            }
                         }
-                        vt[q] = mfma_inverse_half<T>(yq, cnb);
+                    }
+                    // keep the source's loads after the gains (register pressure)
+                    __builtin_amdgcn_sched_barrier(0);
+                    {
+                        v8 xs[4];
+                        mfma_load_x<T>(fsrc, fp, fox, foy, pos, N, lane, xs, slot, sstride);
+#pragma unroll
+                        for (int q = 0; q < 2; ++q) {
+                            f32x16 yq = mfma_forward_half<T>(xs, q, kb, lane, cna);
+#pragma unroll
+                            for (int k = 0; k < 16; ++k) yq[k] *= wg[q][k];
+                            vt[q] = mfma_inverse_half<T>(yq, cnb);
+                        }
                     }
                     l2 = wave_sum(l2 + l2b);
                     w = l2 <= 0.f ? 1.f : fminf(1.f / l2, a.wmax);
@@ -2094,6 +2109,19 @@ size_t filter_stack_lds_bytes(const FilterArgs &a, bool wiener, int precision)
     return (filter_acc_floats(a.FW, a.FH) * a.nslots + (size_t)kWaves * a.G * ZS) * 4;
 }
 
+// LDS of one CU
+constexpr size_t kLdsPerCu = 160 * 1024;
+
+template <typename T, bool W>
+hipError_t launch_mfma_stack_t(const FilterArgs &a, size_t lds, dim3 grid, hipStream_t s)
+{
+    // 16 waves where one workgroup fills a CU's LDS (as launch_mfma_t)
+    if (2 * lds > kLdsPerCu)
+        return launch_tiled(filter_mfma_stack_kernel<T, W, false, true, 2 * kMWaves>, 2 * kMThreads, lds, a, s,
+                            nullptr, grid);
+    return launch_tiled(filter_mfma_stack_kernel<T, W, false, true>, kMThreads, lds, a, s, nullptr, grid);
+}
+
 hipError_t launch_filter_stack(const FilterArgs &a, bool wiener, int precision, hipStream_t s)
 {
     const size_t lds = filter_stack_lds_bytes(a, wiener, precision);
@@ -2102,11 +2130,9 @@ hipError_t launch_filter_stack(const FilterArgs &a, bool wiener, int precision, 
         return wiener ? launch_tiled(filter_stack_kernel<true, false, true>, kThreads, lds, a, s, nullptr, grid)
                       : launch_tiled(filter_stack_kernel<false, false, true>, kThreads, lds, a, s, nullptr, grid);
     if (precision == 1)
-        return wiener ? launch_tiled(filter_mfma_stack_kernel<_Float16, true, false, true>, kMThreads, lds, a, s, nullptr, grid)
-                      : launch_tiled(filter_mfma_stack_kernel<_Float16, false, false, true>, kMThreads, lds, a, s, nullptr,
-                                     grid);
-    return wiener ? launch_tiled(filter_mfma_stack_kernel<__bf16, true, false, true>, kMThreads, lds, a, s, nullptr, grid)
-                  : launch_tiled(filter_mfma_stack_kernel<__bf16, false, false, true>, kMThreads, lds, a, s, nullptr, grid);
+        return wiener ? launch_mfma_stack_t<_Float16, true>(a, lds, grid, s)
+                      : launch_mfma_stack_t<_Float16, false>(a, lds, grid, s);
+    return wiener ? launch_mfma_stack_t<__bf16, true>(a, lds, grid, s) : launch_mfma_stack_t<__bf16, false>(a, lds, grid, s);
 }
 
 // 16-wave workgroups where only one workgroup fits a CU's 160 KB of LDS
@@ -2114,7 +2140,6 @@ hipError_t launch_filter_stack(const FilterArgs &a, bool wiener, int precision, 
 // holds 4 waves per SIMD, and for single launches of tiles of at least
 // kMWideRefs reference blocks.  The workgroup size never changes results.
 constexpr int kMWideRefs = 128;
-constexpr size_t kLdsPerCu = 160 * 1024;
 
 template <typename T, bool W>
 hipError_t launch_mfma_t(const FilterArgs &a, size_t lds, dim3 grid, hipStream_t s, const BatchLaunch *b)
