@@ -26,6 +26,7 @@ import time
 import numpy as np
 
 W, H, SIGMA, SEED = 1920, 1080, 25.0, 1
+ROOT = os.path.dirname(os.path.abspath(__file__))
 B = 8
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
@@ -288,6 +289,23 @@ def bench_batch(args) -> int:
     return 0
 
 
+def pilot_network(P, kind: str, steps: int):
+    """(network, training record): the shipped weights of `kind` (trained by
+    isp_mw_amd.train_pilot) when steps is 0, else a network trained here by
+    the same recipe for `steps` steps."""
+    import torch
+    from isp_mw_amd import train_pilot
+    if steps <= 0:
+        path = P.weights_path(kind)
+        return P.pretrained(kind), dict(P.PilotCNN.meta_of(path), source=os.path.relpath(path, ROOT))
+    t0 = time.perf_counter()
+    net, meta = train_pilot.train_kind(kind, steps, log_every=max(1, steps // 10))
+    torch.cuda.synchronize()
+    meta.update(source="trained in this run", ms_per_step=round(meta["train_s"] / steps * 1e3, 3),
+                total_s=round(time.perf_counter() - t0, 1))
+    return net, meta
+
+
 def bench_pilot(args) -> int:
     """Learned pilot: train the residual CNN on 20k random 128x128 crops of
     synthetic images (sigma in [10, 40]) against GPU step-1 targets (Adam,
@@ -296,17 +314,7 @@ def bench_pilot(args) -> int:
     import torch
     from isp_mw_amd import pilot as P, synth
 
-    t0 = time.perf_counter()
-    noisy, sigma, target = P.make_training_set(20000, crop=128, seed=0)
-    torch.cuda.synchronize()
-    gen_s = time.perf_counter() - t0
-    net = P.PilotCNN(seed=0)
-    t0 = time.perf_counter()
-    losses = P.train(net, noisy, sigma, target, args.pilot_steps, batch=32, lr=1e-3, seed=0,
-                     log_every=max(1, args.pilot_steps // 10))
-    torch.cuda.synchronize()
-    train_s = time.perf_counter() - t0
-    del noisy, target
+    net, training = pilot_network(P, "grey", args.pilot_steps)
     clean, n = synth.grey_pair(H, W, SIGMA, SEED)
     x = torch.from_numpy(n).cuda()
     para = P.default_para(SIGMA)
@@ -356,8 +364,7 @@ def bench_pilot(args) -> int:
         "dtype": "bf16 network, " + ("fp32" if args.precision == "fp32" else f"{args.precision}+fp32") + " Wiener",
         "data": "synthetic (seeded rectangles+gradient+gratings, AWGN)",
         "config": {"workload": "pilot CNN (8 layers 3x3, 48 ch, ReLU, residual) as Wiener guide, 1920x1080 grey",
-                   "train_crops": 20000, "crop": 128, "batch": 32, "train_steps": args.pilot_steps,
-                   "sigma_train": [10, 40], "sigma": SIGMA, "bm3d_sigma_param": round(SIGMA * P.GREY_SIGMA_SCALE, 3),
+                   "sigma": SIGMA, "bm3d_sigma_param": round(SIGMA * P.GREY_SIGMA_SCALE, 3),
                    "precision": args.precision},
         "plain_two_step": {"value": round(W * H / dt_plain / 1e6, 3), "ms": round(dt_plain * 1e3, 3)},
         "guide_ms": {"pilot_cnn": round(dt_guide_pilot * 1e3, 3), "bm3d_step1": round(dt_guide_step1 * 1e3, 3)},
@@ -367,9 +374,7 @@ def bench_pilot(args) -> int:
                     "psnr_guide_pilot_db": round(synth.psnr(g_p, clean), 3),
                     "psnr_guide_step1_db": round(synth.psnr(g_b, clean), 3),
                     "gain_db_other_seeds": others},
-        "training": {"target_generation_s": round(gen_s, 2), "train_s": round(train_s, 2),
-                     "ms_per_step": round(train_s / args.pilot_steps * 1e3, 3),
-                     "loss_first": losses[0], "loss_last_mean": float(np.mean(losses[-50:]))},
+        "training": training,
     }
     print(json.dumps(res), flush=True)
     return 0
@@ -403,13 +408,7 @@ def bench_full(args) -> int:
     para.basic.lambda_ = 2.7
     para.set_sigma(SIGMA)
     # the pilot: the RGB network, trained on step-1 targets of these parameters
-    t0 = time.perf_counter()
-    noisy_t, sig_t, tgt_t = P.make_rgb_training_set(20000, crop=128, seed=0, para=para)
-    net = P.PilotCNN(seed=0, channels=3)
-    P.train(net, noisy_t, sig_t, tgt_t, args.pilot_steps, batch=32, lr=1e-3, seed=0)
-    torch.cuda.synchronize()
-    train_s = time.perf_counter() - t0
-    del noisy_t, tgt_t
+    net, training = pilot_network(P, "rgb", args.pilot_steps)
     x_all = torch.from_numpy(np.stack(frames)).cuda()
     xs = list(x_all)
     guide_all = torch.empty_like(x_all)
@@ -511,7 +510,7 @@ def bench_full(args) -> int:
                     "psnr_fp16_two_step_batch_db": round(psnr16, 3)},
         "plain_fp32_two_step": {"value": round(px / dt_plain / 1e6, 3), "ms": round(dt_plain * 1e3, 1)},
         "fp16_two_step_batch": {"value": round(px / dt16 / 1e6, 3), "ms": round(dt16 * 1e3, 1)},
-        "pilot_training_s": round(train_s, 1),
+        "training": training,
     }
     print(json.dumps(res), flush=True)
     return 0
@@ -787,7 +786,9 @@ def main() -> int:
                          "full4k: 8 x 4K RGB frames, pilot guides + one batched FP16 Wiener launch; "
                          "nlm1080: NL-means of a 1080p 16-bit RGB frame; "
                          "stack4k: 8 x 4K RGB frames with temporal groups (radius 1)")
-    ap.add_argument("--pilot-steps", type=int, default=1250, help="training steps of the pilot workload")
+    ap.add_argument("--pilot-steps", type=int, default=0,
+                    help="pilot workloads: 0 uses the shipped weights (isp_mw_amd/weights); N > 0 trains a network "
+                         "for N steps by the same recipe first")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy (3 launches): the known-traffic "
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings")

@@ -107,7 +107,7 @@ int isp_bm3d_set_precision(isp_bm3d_ctx *ctx, int precision);
  *   ISP_BM3D_VARIANT_HAAR: the orthonormal Haar transform along the group
  *     axis, each group truncated to the largest power of two of its members;
  *   ISP_BM3D_VARIANT_KAISER: each block aggregated with a separable Kaiser
- *     window of shape kaiser_beta (2.0 in the north_star's configs).
+ *     window of shape kaiser_beta (2.0 is the usual choice).
  * Both keep the orthonormal thresholds (lambda sigma) and Wiener noise
  * (sigma^2 / 2^zeta).  Variant calls run the generic FP64 kernels in every
  * precision (BlockSize <= 16, GroupSize <= 64); batches, multi-frame stacks

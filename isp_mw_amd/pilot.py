@@ -19,6 +19,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import os
 import sys
 
 import numpy as np
@@ -250,22 +251,41 @@ class PilotCNN:
         self.step = int(st.get("step", 0))
         self.pack()
 
-    def save(self, path) -> None:
-        """Weights and Adam state (moments and step count) to an .npz file, so
-        that training can resume from it (numpy arrays only, no pickles)."""
+    def save(self, path, optimizer: bool = True, meta: dict | None = None) -> None:
+        """Weights, and with optimizer the Adam state (moments and step
+        count), to an .npz file, so that training can resume from it (numpy
+        arrays only, no pickles).  meta: a JSON-able record of how the weights
+        were made, kept as a string."""
+        import json
         st = self.state()
-        np.savez(path, params=st["params"], m=st["m"], v=st["v"], step=np.int64(st["step"]),
-                 channels=np.int64(self.channels))
+        arrays = dict(params=st["params"], step=np.int64(st["step"]), channels=np.int64(self.channels))
+        if optimizer:
+            arrays.update(m=st["m"], v=st["v"])
+        if meta is not None:
+            arrays["meta"] = np.array(json.dumps(meta, sort_keys=True))
+        np.savez(path, **arrays)
+
+    @staticmethod
+    def meta_of(path) -> dict:
+        """The meta record save() stored with a checkpoint ({} when none)."""
+        import json
+        with np.load(path, allow_pickle=False) as z:
+            return json.loads(str(z["meta"])) if "meta" in z.files else {}
 
     @classmethod
     def load(cls, path, device: int = 0) -> "PilotCNN":
-        """A network restored from save(): the same inference, and training
-        continues exactly where it stopped."""
+        """A network restored from save(): the same inference, and, when the
+        file holds the Adam state, training continues exactly where it
+        stopped (otherwise it restarts Adam from zero moments)."""
         with np.load(path, allow_pickle=False) as z:
-            st = {k: z[k] for k in z.files}
+            st = {k: z[k] for k in z.files if k != "meta"}
         net = cls(device=device, channels=int(st["channels"]))
-        net.load_state({"params": st["params"], "m": st["m"], "v": st["v"], "step": int(st["step"]),
-                        "channels": int(st["channels"])})
+        state = {"params": st["params"], "step": int(st["step"]), "channels": int(st["channels"])}
+        if "m" in st:
+            state.update(m=st["m"], v=st["v"])
+        else:
+            state["step"] = 0
+        net.load_state(state)
         return net
 
     def layer_params(self):
@@ -324,20 +344,37 @@ def default_para(sigma: float = 25.0) -> BM3D_Para:
     return p
 
 
+def dihedral(t, k: int):
+    """Crops [N,H,W] or [N,C,H,W] (square) under the dihedral transform k in
+    0..7: a transpose when k & 4, then k & 3 quarter turns."""
+    import torch
+    if k & 4:
+        t = t.transpose(-2, -1)
+    return torch.rot90(t, k & 3, (-2, -1)) if k & 3 else t
+
+
 def train(net: PilotCNN, noisy, sigma, target, steps: int, batch: int = 32, lr: float = 1e-3, seed: int = 0,
-          log_every: int = 0) -> list[float]:
-    """Adam on random mini-batches; returns the loss of every step."""
+          log_every: int = 0, augment: bool = True) -> list[float]:
+    """Adam on random mini-batches; returns the loss of every step.  With
+    augment, each mini-batch (noisy crops and their step-1 targets alike) is
+    taken under one of the 8 dihedral transforms of the square crops, drawn
+    from the same seeded generator."""
     import torch
     g = torch.Generator(device="cpu").manual_seed(seed)
     n = noisy.shape[0]
+    if augment and noisy.shape[-1] != noisy.shape[-2]:
+        raise ValueError("dihedral augmentation needs square crops")
     losses = []
     # an RGB network's inputs are OPP planes: chroma offset by 0.5, as in rgb_guide
     off = None if net.channels == 1 else torch.tensor(list(OPP_OFFSETS) * batch, device=noisy.device)
     for it in range(steps):
         idx = torch.randint(0, n, (batch,), generator=g).to(noisy.device)
+        a, t = noisy[idx], target[idx]
+        if augment:
+            k = int(torch.randint(0, 8, (1,), generator=g))
+            a, t = dihedral(a, k), dihedral(t, k)
         lr_t = lr * (0.5 * (1 + math.cos(math.pi * it / max(1, steps))))  # cosine decay
-        loss = net.loss_and_grads(noisy[idx].contiguous(), sigma[idx].contiguous(), target[idx].contiguous(),
-                                  offset=off)
+        loss = net.loss_and_grads(a.contiguous(), sigma[idx].contiguous(), t.contiguous(), offset=off)
         net.adam(lr_t)
         losses.append(loss)
         if log_every and (it + 1) % log_every == 0:
@@ -447,6 +484,24 @@ def make_rgb_training_set(n_crops: int, crop: int = 128, sigma_range=(10.0, 40.0
     return to_opp(rgb), torch.from_numpy(sig).to(dev), to_opp(est)
 
 
+# the shipped networks (trained by ``python -m isp_mw_amd.train_pilot KIND``)
+WEIGHTS_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights")
+WEIGHT_FILES = {"grey": "pilot_grey.npz", "rgb": "pilot_rgb_r19.npz"}
+
+
+def weights_path(kind: str) -> str:
+    """Path of the shipped weights of `kind`: "grey" (grey planes, the grey
+    parameters of default_para) or "rgb" (OPP planes of RGB frames, step-1
+    targets at BMrange 19, GroupSize 32)."""
+    return os.path.join(WEIGHTS_DIR, WEIGHT_FILES[kind])
+
+
+def pretrained(kind: str = "grey", device: int = 0) -> PilotCNN:
+    """The shipped network of `kind`; PilotCNN.meta_of(weights_path(kind))
+    gives the recipe it was trained with and its final training loss."""
+    return PilotCNN.load(weights_path(kind), device=device)
+
+
 class PilotBM3D:
     """Two-step BM3D on a grey plane [H,W] or an RGB frame [3,H,W] with the
     pilot's estimate as the Wiener guide.  Without a network it runs plain
@@ -476,6 +531,7 @@ class PilotBM3D:
         return self.final.process(out, noisy, g)
 
 
-__all__ = ["PilotCNN", "PilotBM3D", "make_training_set", "make_rgb_training_set", "train", "default_para",
+__all__ = ["PilotCNN", "PilotBM3D", "make_training_set", "make_rgb_training_set", "train", "dihedral",
+           "pretrained", "weights_path", "default_para",
            "rgb_para", "rgb_guide", "to_opp", "from_opp", "layers", "LAYERS", "GREY_SIGMA_SCALE", "OPP_NORMS",
            "OPP_OFFSETS", "BM3D"]

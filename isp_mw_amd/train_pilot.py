@@ -1,0 +1,77 @@
+"""Train the learned pilot's shipped weights on the GPU.
+
+    python -m isp_mw_amd.train_pilot grey [--steps 30000] [--out PATH]
+    python -m isp_mw_amd.train_pilot rgb  [--steps 30000] [--out PATH]
+
+grey: 20k random 128x128 crops of seeded synthetic grey images (sigma uniform
+in [10, 40]), targets from GPU BM3D step 1 at the grey parameters
+(``default_para``).  rgb: the same on RGB crops as OPP planes, targets from
+GPU step 1 at the full form's parameters (BMrange 19, GroupSize 32).  Both:
+Adam at batch 32 with cosine decay of the learning rate (from 1e-3 for grey,
+5e-4 for rgb: at 1e-3 the RGB network's loss jumped from 1e-4 to the noise
+level near step 4000 and stayed there), dihedral augmentation, seed 0;
+every convolution, gradient and update runs in libisp_bm3d.so's kernels.
+The weights (no optimizer state) go to isp_mw_amd/weights/ with a record of
+the recipe and the final training loss.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import pilot as P
+
+CROPS, CROP, BATCH, SEED = 20000, 128, 32, 0
+LR = {"grey": 1e-3, "rgb": 5e-4}
+
+
+def recipe(kind: str, steps: int) -> dict:
+    para = "default_para(25)" if kind == "grey" else "rgb_para(25, bm_range=19, group_sizes=(32, 32))"
+    return {"kind": kind, "crops": CROPS, "crop": CROP, "sigma_range": [10.0, 40.0], "target": f"GPU step 1, {para}",
+            "optimizer": "adam", "batch": BATCH, "lr": LR[kind], "schedule": "cosine", "augment": "dihedral",
+            "steps": steps, "seed": SEED}
+
+
+def train_kind(kind: str, steps: int, device: int = 0, log_every: int = 0):
+    """(network, meta) trained by the recipe of `kind`."""
+    import torch
+    t0 = time.perf_counter()
+    if kind == "grey":
+        noisy, sigma, target = P.make_training_set(CROPS, crop=CROP, seed=SEED, device=device)
+        net = P.PilotCNN(seed=SEED, device=device)
+    else:
+        para = P.rgb_para(25.0, bm_range=19, group_sizes=(32, 32))
+        noisy, sigma, target = P.make_rgb_training_set(CROPS, crop=CROP, seed=SEED, para=para, device=device)
+        net = P.PilotCNN(seed=SEED, device=device, channels=3)
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    losses = P.train(net, noisy, sigma, target, steps, batch=BATCH, lr=LR[kind], seed=SEED, log_every=log_every,
+                     augment=True)
+    torch.cuda.synchronize()
+    meta = recipe(kind, steps)
+    meta.update(data_s=round(t_data, 1), train_s=round(time.perf_counter() - t0, 1),
+                loss_first=float(np.mean(losses[:50])), loss_last=float(np.mean(losses[-500:])))
+    return net, meta
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("kind", choices=sorted(P.WEIGHT_FILES))
+    ap.add_argument("--steps", type=int, default=30000)
+    ap.add_argument("--out", default=None)
+    args = ap.parse_args(argv)
+    net, meta = train_kind(args.kind, args.steps, log_every=max(1, args.steps // 20))
+    out = args.out or P.weights_path(args.kind)
+    os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
+    net.save(out, optimizer=False, meta=meta)
+    print(f"{args.kind}: {meta} -> {out}", flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -192,8 +192,8 @@ def test_rgb_autotune_keeps_results(gpu):
 
 
 @pytest.mark.parametrize("prec", ["fp32", "fp16"])
-def test_configs2_scale_batch(gpu, prec):
-    """configs[2] at its own scale: 64 grey images, W uniform in [256, 1024],
+def test_batch64_scale(gpu, prec):
+    """The many-image batch at its full scale: 64 grey images, W uniform in [256, 1024],
     H in [256, 768], per-image sigma in [5, 50], each generated like the
     512x512 case, in one batch (one launch per stage, replayed as a hipGraph)
     with autotuned tiles and final GroupSize per size class.  Every image

@@ -51,7 +51,7 @@ def key_check(c, keys, oc, ok, G):
 @pytest.mark.parametrize("walk", [True, False], ids=["walk", "list"])
 @pytest.mark.parametrize("prec", ["fp16", "bf16", "fp32fast"])
 def test_fp32_distance_groups_512_grey(gpu, prec, walk, monkeypatch):
-    """configs[0] shape: 512x512 grey, BlockStep 3, BMrange 12, G 16 / 32."""
+    """The 512x512 grey case: BlockStep 3, BMrange 12, G 16 / 32."""
     monkeypatch.setenv("ISP_BM3D_MATCH_WALK", "1" if walk else "0")
     clean, n = synth.grey_pair(512, 512, 25.0, 5)
     p = para()
@@ -108,7 +108,7 @@ def test_fp32_distance_groups_1080p_rgb(gpu, prec):
 
 @pytest.mark.parametrize("sigma", [25.0, 15.0])
 def test_fp16_1080p_rgb_vs_oracle(gpu, sigma):
-    """configs[1] (sigma 15) and the benchmark (sigma 25): the mixed forms at
+    """The mixed-precision 1080p RGB case at sigma 15 and the benchmark (sigma 25): the mixed forms at
     1920x1080 RGB against the FP64 oracle, per channel, and against the plain
     FP32 GPU form in PSNR."""
     import torch

@@ -94,7 +94,7 @@ def test_variant_refused_by_batches_and_stacks(gpu):
 
 
 def test_variant_quality_512(gpu):
-    """configs[0] with the configs' own transform and window (Haar, Kaiser 2),
+    """The 512x512 grey case with the variant transform and window (Haar, Kaiser 2),
     beside the reference's semantics: both are reported, the variant must
     denoise about as well."""
     clean, n = synth.grey_pair(512, 512, 25.0, 5)

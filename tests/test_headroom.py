@@ -55,7 +55,7 @@ CASES = [
     # (BlockStep, BMrange, GroupSize, BMstep, W, H)
     (3, 12, 32, 1, 72, 70),    # the benchmark's final step
     (3, 12, 16, 1, 72, 70),    # its basic step
-    (1, 19, 32, 1, 60, 58),    # the worst case the north_star's forms reach: BlockStep 1, BMrange 19
+    (1, 19, 32, 1, 60, 58),    # the worst case the 4K multi-frame form reaches: BlockStep 1, BMrange 19
     (2, 16, 32, 1, 64, 66),    # profile high, final step
     (8, 9, 16, 1, 50, 41),     # profile fast
     (3, 10, 32, 3, 61, 64),    # BMstep 3
