@@ -42,11 +42,12 @@ int isp_pilot_kpad(int cin_pad);
 int isp_pilot_conv3x3(const void *in, const void *mask, const void *weights, const float *bias, void *out,
                       float *out_f32, int n, int h, int w, int cin_pad, int cout_pad, int relu, void *stream);
 
-/* Weight and bias gradients of a convolution with real channel counts cin,
- * cout (48 with cout_pad 48; 1 or 3, one residual per plane, with cout_pad
- * 16): dw [cout][cin][9], db [cout] (may be NULL), with dz = dout * (mask > 0)
- * (mask may be NULL).  partial: scratch of isp_pilot_wgrad_partial_floats
- * floats.  The sums are reduced in a fixed order (run to run identical). */
+/* Weight and bias gradients of a convolution with real channel counts
+ * cin <= cin_pad, cout <= cout_pad (the same pads as isp_pilot_conv3x3), on
+ * the matrix cores: dw [cout][cin][9], db [cout] (may be NULL), with
+ * dz = dout * (mask > 0) (mask may be NULL).  partial: scratch of
+ * isp_pilot_wgrad_partial_floats floats.  The per-workgroup sums are added
+ * in a fixed order (run to run identical). */
 size_t isp_pilot_wgrad_partial_floats(int n, int h, int w, int cin, int cout);
 int isp_pilot_conv3x3_wgrad(const void *in, const void *dout, const void *mask, float *partial, float *dw, float *db,
                             int n, int h, int w, int cin_pad, int cout_pad, int cin, int cout, void *stream);

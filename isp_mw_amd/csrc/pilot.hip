@@ -35,11 +35,14 @@ constexpr int RPW = TH / 4;              // rows per wave
 constexpr int MT = RPW * (TW / 16);      // 16-pixel M-tiles per wave
 constexpr int SH = TH + 2, SW = TW + 2;  // staged input tile with halo
 constexpr int THREADS = 256;             // 4 waves x RPW rows
-constexpr int GTH = 8, GTW = 32;         // pixel tile of the weight-gradient kernel
+// pixel tile of the weight-gradient kernel: 4 rows of 32 (one 32-pixel row per
+// MFMA K chunk), staged as 6 rows with the halo
+constexpr int WTH = 4, WTW = 32, WPIX = WTH * WTW, WSR = WTH + 2;
+constexpr int WCP = WSR * WTW + 8;  // pitch of one channel of a shifted input copy (elements)
+constexpr int WZP = WPIX + 8;       // pitch of one channel of the transposed output gradient
+constexpr int WBLOCKS = 512;        // persistent grid of the weight gradient (two workgroups per CU)
 
 __host__ __device__ constexpr int kpad(int cin) { return (9 * cin + 31) / 32 * 32; }
-
-__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
 __device__ __forceinline__ uint16_t f2bf(float f)
 {
@@ -275,103 +278,172 @@ __global__ __launch_bounds__(THREADS) void conv3x3_fwd_kernel(const uint16_t *__
     }
 }
 
-// Weight and bias gradients: dw[co][ci][tap] = sum_p dz[p][co] * in[p + off(tap)][ci],
-// db[co] = sum_p dz[p][co], with dz = dout * (mask > 0) (mask may be null).
-// A persistent grid walks the pixel tiles; workgroup b writes its sums to
-// partial[b][*] (work item it = ((cb * cin) + ci) * 9 + tap holds CB outputs
-// co = cb*CB + j, then cout bias sums), reduced in a fixed order afterwards.
-template <int CIN, int COUT, int CB, int IPT>
-__global__ __launch_bounds__(THREADS) void wgrad_kernel(const uint16_t *__restrict__ in,
-                                                        const uint16_t *__restrict__ dout,
-                                                        const uint16_t *__restrict__ mask, float *__restrict__ partial,
-                                                        int N, int H, int W, int cin, int cout)
+// Weight and bias gradients on the matrix cores:
+//   dw[co][tap][ci] = sum_p dz[p][co] * in[p + off(tap)][ci],  db[co] = sum_p dz[p][co],
+// with dz = dout * (mask > 0) (mask may be null), as the product D = A B with
+// A[co][p] = dz (M = output channels), B[p][(tap, ci)] = the shifted input
+// (N = tap * CINP + ci, the packed forward weights' K order) and the pixels as
+// the reduction axis, 32 per v_mfma_f32_16x16x32_bf16 (one tile row).  A
+// workgroup stages a 4 x 32 pixel tile: dz transposed to [co][p], and the
+// input (with its 1-pixel halo) transposed to [ci][row][x] in three copies
+// shifted by kx = 0, 1, 2, so that every operand fragment (8 consecutive
+// pixels of one channel) is one aligned 16-byte read.  The bias gradient is
+// the same product with B = 1.  A persistent grid walks the tiles; workgroup
+// b writes its sums to partial[b] ([cout][9][cin], then the cout bias sums),
+// added over the workgroups in a fixed order by wgrad_sum_kernel.
+template <int CINP, int COUTP>
+__host__ __device__ constexpr size_t wgrad_lds_bytes()
 {
-    constexpr int ASH = GTH + 2, ASW = GTW + 2;
+    return ((size_t)3 * CINP * WCP + (size_t)COUTP * WZP) * 2;
+}
+
+template <int CINP, int COUTP>
+__global__ __launch_bounds__(THREADS) void wgrad_mfma_kernel(const uint16_t *__restrict__ in,
+                                                             const uint16_t *__restrict__ dout,
+                                                             const uint16_t *__restrict__ mask,
+                                                             float *__restrict__ partial, int N, int H, int W, int cin,
+                                                             int cout)
+{
+    constexpr int MT = COUTP / 16;          // 16-channel tiles of the outputs
+    constexpr int NTT = 9 * CINP / 16;      // 16-wide tiles of (tap, ci); one tap each (CINP % 16 == 0)
+    constexpr int NTW = (NTT + 3) / 4;      // per wave (wave w takes tiles w, w + 4, ...)
+    constexpr int CQI = CINP / 8, CQO = COUTP / 8;
     extern __shared__ __align__(16) unsigned char smem[];
-    uint16_t *ta = reinterpret_cast<uint16_t *>(smem);  // [ASH*ASW][CIN]
-    uint16_t *tz = ta + ASH * ASW * CIN;                 // [GTH*GTW][COUT]
-    const int tid = threadIdx.x;
-    const int ncb = (cout + CB - 1) / CB;
-    const int items = ncb * cin * 9;
-    const int tx_n = (W + GTW - 1) / GTW, ty_n = (H + GTH - 1) / GTH;
+    uint16_t *xin = reinterpret_cast<uint16_t *>(smem);  // [3][CINP][WCP]: copy kx, channel, row * WTW + x
+    uint16_t *dzt = xin + 3 * CINP * WCP;                 // [COUTP][WZP]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int l16 = lane & 15, g = lane >> 4;
+    const int tx_n = (W + WTW - 1) / WTW, ty_n = (H + WTH - 1) / WTH;
     const long ntiles = (long)N * tx_n * ty_n;
-    float acc[IPT][CB];
+    f32x4 acc[NTW][MT];
 #pragma unroll
-    for (int j = 0; j < IPT; ++j)
+    for (int i = 0; i < NTW; ++i)
 #pragma unroll
-        for (int c = 0; c < CB; ++c) acc[j][c] = 0.f;
-    float bacc = 0.f;
+        for (int m = 0; m < MT; ++m) acc[i][m] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    f32x4 bacc = (f32x4){0.f, 0.f, 0.f, 0.f};
+    bf16x8 ones;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) ones[e] = (__bf16)1.f;
 
     for (long t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int n = (int)(t / ((long)tx_n * ty_n));
         const int rem = (int)(t - (long)n * tx_n * ty_n);
-        const int y0 = (rem / tx_n) * GTH, x0 = (rem % tx_n) * GTW;
-        __syncthreads();
-        for (int k = tid; k < ASH * ASW * (CIN / 8); k += THREADS) {
-            const int q = k % (CIN / 8), p = k / (CIN / 8), sx = p % ASW, sy = p / ASW;
-            const int yy = y0 + sy - 1, xx = x0 + sx - 1;
-            uint4 v = make_uint4(0, 0, 0, 0);
-            if (yy >= 0 && yy < H && xx >= 0 && xx < W)
-                v = *reinterpret_cast<const uint4 *>(in + (((size_t)n * H + yy) * W + xx) * CIN + 8 * q);
-            *reinterpret_cast<uint4 *>(ta + (size_t)p * CIN + 8 * q) = v;
-        }
-        for (int k = tid; k < GTH * GTW * (COUT / 8); k += THREADS) {
-            const int q = k % (COUT / 8), p = k / (COUT / 8), px = p % GTW, py = p / GTW;
-            const int yy = y0 + py, xx = x0 + px;
+        const int y0 = (rem / tx_n) * WTH, x0 = (rem % tx_n) * WTW;
+        __syncthreads();  // the previous tile's operand reads are done
+        for (int k = tid; k < WPIX * CQO; k += THREADS) {
+            const int q = k % CQO, p = k / CQO, yy = y0 + p / WTW, xx = x0 + p % WTW;
             uint4 v = make_uint4(0, 0, 0, 0);
             if (yy < H && xx < W) {
-                const size_t o = (((size_t)n * H + yy) * W + xx) * COUT + 8 * q;
+                const size_t o = (((size_t)n * H + yy) * W + xx) * COUTP + 8 * q;
                 v = *reinterpret_cast<const uint4 *>(dout + o);
                 if (mask) v = apply_mask(v, *reinterpret_cast<const uint4 *>(mask + o));
             }
-            *reinterpret_cast<uint4 *>(tz + (size_t)p * COUT + 8 * q) = v;
-        }
-        __syncthreads();
+            const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+            uint16_t *d = dzt + (size_t)(8 * q) * WZP + p;
 #pragma unroll
-        for (int j = 0; j < IPT; ++j) {
-            const int it = tid + j * THREADS;
-            if (it >= items) continue;
-            const int tap = it % 9, ci = (it / 9) % cin, cb = it / (9 * cin);
-            const int ky = tap / 3, kx = tap - 3 * ky;
-            for (int p = 0; p < GTH * GTW; ++p) {
-                const int py = p / GTW, px = p - py * GTW;
-                const float a = bf2f(ta[((py + ky) * ASW + px + kx) * CIN + ci]);
-                const uint16_t *z = tz + (size_t)p * COUT + cb * CB;
-#pragma unroll
-                for (int c = 0; c < CB; ++c) acc[j][c] = fmaf(a, bf2f(z[c]), acc[j][c]);
+            for (int j = 0; j < 4; ++j) {
+                d[(2 * j) * WZP] = (uint16_t)(w4[j] & 0xFFFFu);
+                d[(2 * j + 1) * WZP] = (uint16_t)(w4[j] >> 16);
             }
         }
-        if (tid < cout)
-            for (int p = 0; p < GTH * GTW; ++p) bacc += bf2f(tz[(size_t)p * COUT + tid]);
-    }
-    const int stride = items * CB + cout;
-    float *dst = partial + (size_t)blockIdx.x * stride;
+        // staged pixel (sy, sx) = input (y0 + sy - 1, x0 + sx - 1); copy kx holds x = sx - kx in [0, WTW)
+        for (int k = tid; k < WSR * (WTW + 2) * CQI; k += THREADS) {
+            const int q = k % CQI, sp = k / CQI, sx = sp % (WTW + 2), sy = sp / (WTW + 2);
+            const int yy = y0 + sy - 1, xx = x0 + sx - 1;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+                v = *reinterpret_cast<const uint4 *>(in + (((size_t)n * H + yy) * W + xx) * CINP + 8 * q);
+            const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-        const int it = tid + j * THREADS;
-        if (it >= items) continue;
+            for (int kx = 0; kx < 3; ++kx) {
+                const int x = sx - kx;
+                if (x < 0 || x >= WTW) continue;
+                uint16_t *d = xin + (size_t)(kx * CINP + 8 * q) * WCP + sy * WTW + x;
 #pragma unroll
-        for (int c = 0; c < CB; ++c) dst[it * CB + c] = acc[j][c];
+                for (int j = 0; j < 4; ++j) {
+                    d[(2 * j) * WCP] = (uint16_t)(w4[j] & 0xFFFFu);
+                    d[(2 * j + 1) * WCP] = (uint16_t)(w4[j] >> 16);
+                }
+            }
+        }
+        __syncthreads();
+        // A fragments: output channels 16m + l16, pixels 8g..8g+7 of tile row kc
+        bf16x8 a[MT][WTH];
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int kc = 0; kc < WTH; ++kc)
+                a[m][kc] = *reinterpret_cast<const bf16x8 *>(dzt + (size_t)(16 * m + l16) * WZP + kc * WTW + 8 * g);
+#pragma unroll
+        for (int i = 0; i < NTW; ++i) {
+            const int nt = wave + 4 * i;
+            if (nt >= NTT) continue;  // wave-uniform
+            const int tap = (16 * nt) / CINP, ci0 = 16 * nt - tap * CINP, ky = tap / 3, kx = tap - 3 * ky;
+            const uint16_t *xb = xin + (size_t)(kx * CINP + ci0 + l16) * WCP + 8 * g;
+#pragma unroll
+            for (int kc = 0; kc < WTH; ++kc) {
+                const bf16x8 b = *reinterpret_cast<const bf16x8 *>(xb + (kc + ky) * WTW);
+#pragma unroll
+                for (int m = 0; m < MT; ++m) acc[i][m] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m][kc], b, acc[i][m], 0, 0, 0);
+            }
+        }
+        if (wave < MT) {
+#pragma unroll
+            for (int m = 0; m < MT; ++m)
+                if (m == wave) {
+#pragma unroll
+                    for (int kc = 0; kc < WTH; ++kc) bacc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m][kc], ones, bacc, 0, 0, 0);
+                }
+        }
     }
-    if (tid < cout) dst[items * CB + tid] = bacc;
+    // D layout: lane holds rows 4g..4g+3 (output channels) of column l16
+    float *dst = partial + (size_t)blockIdx.x * ((size_t)cout * 9 * cin + cout);
+#pragma unroll
+    for (int i = 0; i < NTW; ++i) {
+        const int nt = wave + 4 * i;
+        if (nt >= NTT) continue;
+        const int kk = 16 * nt + l16, tap = kk / CINP, ci = kk - tap * CINP;
+        if (ci >= cin) continue;
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int co = 16 * m + 4 * g + r;
+                if (co < cout) dst[((size_t)co * 9 + tap) * cin + ci] = acc[i][m][r];
+            }
+    }
+    if (wave < MT && l16 == 0) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int co = 16 * wave + 4 * g + r;
+            if (co < cout) dst[(size_t)cout * 9 * cin + co] = bacc[r];
+        }
+    }
 }
 
-// dw[co][ci][tap] and db[co] from the partial sums of `blocks` workgroups.
-__global__ void wgrad_reduce_kernel(const float *__restrict__ partial, int blocks, int cin, int cout, int CB,
-                                    float *__restrict__ dw, float *__restrict__ db)
+// dw[co][ci][tap] and db[co]: the partial sums of `blocks` workgroups, summed
+// in a fixed order (wave w adds blocks w, w + 4, ...; the four wave sums are
+// then added in wave order).
+__global__ __launch_bounds__(256) void wgrad_sum_kernel(const float *__restrict__ partial, int blocks, int cin,
+                                                        int cout, float *__restrict__ dw, float *__restrict__ db)
 {
-    const int ncb = (cout + CB - 1) / CB, items = ncb * cin * 9, stride = items * CB + cout;
-    const int o = blockIdx.x * blockDim.x + threadIdx.x;
-    if (o >= stride) return;
+    __shared__ float part[4][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nw = cout * 9 * cin, stride = nw + cout;
+    const int o = blockIdx.x * 64 + lane;
     float s = 0.f;
-    for (int b = 0; b < blocks; ++b) s += partial[(size_t)b * stride + o];
-    if (o >= items * CB) {
-        if (db) db[o - items * CB] = s;
+    if (o < stride)
+        for (int b = wave; b < blocks; b += 4) s += partial[(size_t)b * stride + o];
+    part[wave][lane] = s;
+    __syncthreads();
+    if (wave != 0 || o >= stride) return;
+    s = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
+    if (o >= nw) {
+        if (db) db[o - nw] = s;
         return;
     }
-    const int it = o / CB, c = o - it * CB;
-    const int tap = it % 9, ci = (it / 9) % cin, co = (it / (9 * cin)) * CB + c;
-    if (co < cout) dw[((size_t)co * cin + ci) * 9 + tap] = s;
+    const int co = o / (9 * cin), r = o - co * 9 * cin, tap = r / cin, ci = r - tap * cin;
+    dw[((size_t)co * cin + ci) * 9 + tap] = s;
 }
 
 __global__ void pack_kernel(const float *__restrict__ w, uint16_t *__restrict__ wf, uint16_t *__restrict__ wb, int cin,
@@ -500,27 +572,23 @@ hipError_t conv_t(const void *in, const void *mask, const void *w, const float *
     return hipGetLastError();
 }
 
-template <int CIN, int COUT, int CB, int IPT>
+template <int CINP, int COUTP>
 hipError_t wgrad_t(const void *in, const void *dout, const void *mask, float *partial, int blocks, int n, int h,
                    int wd, int cin, int cout, hipStream_t s)
 {
-    const size_t lds = ((size_t)(GTH + 2) * (GTW + 2) * CIN + (size_t)GTH * GTW * COUT) * 2;
-    auto k = wgrad_kernel<CIN, COUT, CB, IPT>;
+    auto k = wgrad_mfma_kernel<CINP, COUTP>;
+    constexpr size_t lds = wgrad_lds_bytes<CINP, COUTP>();
     hipError_t e = set_lds(k, lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k, dim3(blocks), dim3(THREADS), lds, s, (const uint16_t *)in, (const uint16_t *)dout,
-                       (const uint16_t *)mask, partial, n, h, wd, cin, cout);
+    hipLaunchKernelGGL(k, dim3(blocks), dim3(THREADS), lds, s, (const uint16_t *)in,
+                       (const uint16_t *)dout, (const uint16_t *)mask, partial, n, h, wd, cin, cout);
     return hipGetLastError();
 }
 
-// output channels per work item of the weight gradient: 12 of a hidden
-// layer's 48; the last layer's 1 (grey) or 3 (OPP planes) in one item
-int cb_of(int cout) { return cout > 12 ? 12 : cout; }
-
 int wgrad_blocks(int n, int h, int w)
 {
-    const long tiles = (long)n * ((w + GTW - 1) / GTW) * ((h + GTH - 1) / GTH);
-    return (int)(tiles < 1024 ? tiles : 1024);
+    const long tiles = (long)n * ((w + WTW - 1) / WTW) * ((h + WTH - 1) / WTH);
+    return (int)(tiles < WBLOCKS ? tiles : WBLOCKS);
 }
 
 }  // namespace
@@ -545,9 +613,8 @@ int isp_pilot_conv3x3(const void *in, const void *mask, const void *weights, con
 
 size_t isp_pilot_wgrad_partial_floats(int n, int h, int w, int cin, int cout)
 {
-    const int cb = cb_of(cout);
-    const size_t stride = (size_t)((cout + cb - 1) / cb) * cin * 9 * cb + cout;
-    return stride * (size_t)wgrad_blocks(n, h, w);
+    if (n <= 0 || h <= 0 || w <= 0 || cin < 1 || cout < 1) return 0;
+    return ((size_t)cout * 9 * cin + cout) * (size_t)wgrad_blocks(n, h, w);
 }
 
 int isp_pilot_conv3x3_wgrad(const void *in, const void *dout, const void *mask, float *partial, float *dw, float *db,
@@ -555,26 +622,16 @@ int isp_pilot_conv3x3_wgrad(const void *in, const void *dout, const void *mask, 
 {
     if (!in || !dout || !partial || !dw || n <= 0 || h <= 0 || w <= 0) return ISP_PILOT_E_ARG;
     if (cin < 1 || cin > cin_pad || cout < 1 || cout > cout_pad) return ISP_PILOT_E_ARG;
-    const int cb = cb_of(cout);
-    // CB 12 for the 48 outputs of a hidden layer, 1 or 3 for the last layer
-    if ((cout_pad == 48) != (cb == 12) || (cout_pad == 16 && cout != 1 && cout != 3)) return ISP_PILOT_E_UNSUPPORTED;
     hipStream_t s = (hipStream_t)stream;
     const int blocks = wgrad_blocks(n, h, w);
-    // work items per thread the instantiations hold (items = ceil(cout/CB) * cin * 9)
-    const int ipt = cin_pad == 16 ? 1 : (cout_pad == 48 ? 7 : 2);
-    if (((cout + cb - 1) / cb) * cin * 9 > ipt * THREADS) return ISP_PILOT_E_UNSUPPORTED;
     hipError_t e;
-    if (cin_pad == 16 && cout_pad == 48) e = wgrad_t<16, 48, 12, 1>(in, dout, mask, partial, blocks, n, h, w, cin, cout, s);
-    else if (cin_pad == 48 && cout_pad == 48) e = wgrad_t<48, 48, 12, 7>(in, dout, mask, partial, blocks, n, h, w, cin, cout, s);
-    else if (cin_pad == 48 && cout_pad == 16 && cout == 1)
-        e = wgrad_t<48, 16, 1, 2>(in, dout, mask, partial, blocks, n, h, w, cin, cout, s);
-    else if (cin_pad == 48 && cout_pad == 16 && cout == 3)
-        e = wgrad_t<48, 16, 3, 2>(in, dout, mask, partial, blocks, n, h, w, cin, cout, s);
+    if (cin_pad == 16 && cout_pad == 48) e = wgrad_t<16, 48>(in, dout, mask, partial, blocks, n, h, w, cin, cout, s);
+    else if (cin_pad == 48 && cout_pad == 48) e = wgrad_t<48, 48>(in, dout, mask, partial, blocks, n, h, w, cin, cout, s);
+    else if (cin_pad == 48 && cout_pad == 16) e = wgrad_t<48, 16>(in, dout, mask, partial, blocks, n, h, w, cin, cout, s);
     else return ISP_PILOT_E_UNSUPPORTED;
     if (e != hipSuccess) return ISP_PILOT_E_HIP;
-    const int stride = ((cout + cb - 1) / cb) * cin * 9 * cb + cout;
-    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((stride + 255) / 256), dim3(256), 0, s, partial, blocks, cin, cout, cb,
-                       dw, db);
+    const int stride = cout * 9 * cin + cout;
+    hipLaunchKernelGGL(wgrad_sum_kernel, dim3((stride + 63) / 64), dim3(256), 0, s, partial, blocks, cin, cout, dw, db);
     return hipGetLastError() == hipSuccess ? ISP_PILOT_OK : ISP_PILOT_E_HIP;
 }
 
