@@ -880,6 +880,16 @@ def main() -> int:
                                  "psnr_db": round(synth.psnr(of, clean), 4),
                                  "max_abs_vs_fp32": float(np.abs(of - ref32).max())}
         quality_extra["psnr_loss_vs_fp32fast_db"] = round(synth.psnr(of, clean) - psnr_out, 4)
+        # the same arithmetic with FP32-accurate products on the matrix cores
+        fm = BM3D(para, device=0, precision="fp32mfma")
+        ym = torch.empty_like(x)
+        dtm, stm = timed_steps(fm, ym, x, max(2, nfr // 4), 1)
+        om = ym.cpu().numpy()
+        fp32_form["fp32mfma"] = {"value": round(W * H / dtm / 1e6, 3), "ms_per_step": round(dtm * 1e3, 3),
+                                 "stages_ms": {k: round(v[0] / v[1], 4) for k, v in stm.items() if v[1]},
+                                 "psnr_db": round(synth.psnr(om, clean), 4),
+                                 "max_abs_vs_fp32": float(np.abs(om - ref32).max()),
+                                 "max_abs_vs_fp32fast": float(np.abs(om - of).max())}
     # end-to-end rate with host buffers (PCIe both ways); never the headline value
     hin = noisy
     hout = np.empty_like(hin)

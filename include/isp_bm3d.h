@@ -45,12 +45,18 @@ extern "C" {
  * FP32 transforms and shrinkage without the FP64 re-decision, and FP32
  * distances (fused multiply-add) as the mixed forms match.  PREC_FP16 /
  * PREC_BF16: 16-bit MFMA operands with FP32 accumulation, FP32 shrinkage,
- * FP32 distances.  Every form aggregates in 64-bit fixed point (exact,
- * order-independent sums; see isp_bm3d_aggregation_limits). */
+ * FP32 distances.  PREC_FP32_MFMA: the FP32_FAST form's arithmetic with the
+ * transform products on the FP16 matrix cores, each operand split into
+ * fp16(a) + fp16(a - fp16(a)) (22 significant bits) and each product taken
+ * as three MFMAs with FP32 accumulation (single frames and batches; the
+ * multi-frame entry point returns E_UNSUPPORTED).  Every form aggregates in
+ * 64-bit fixed point (exact, order-independent sums; see
+ * isp_bm3d_aggregation_limits). */
 #define ISP_BM3D_PREC_FP32 0      /* plain FP32 path, the reference's groups exactly */
 #define ISP_BM3D_PREC_FP16 1      /* FP16 MFMA transforms, FP32 accumulation        */
 #define ISP_BM3D_PREC_BF16 2      /* BF16 MFMA transforms, FP32 accumulation        */
 #define ISP_BM3D_PREC_FP32_FAST 3 /* FP32 throughout: FP32 distances, FP32 filter   */
+#define ISP_BM3D_PREC_FP32_MFMA 4 /* FP32-accurate products on the matrix cores     */
 
 /* Mirror of BM3D_Para_Base (include/BM3D.h:15-45). */
 typedef struct isp_bm3d_step_params {

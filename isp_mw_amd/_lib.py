@@ -18,7 +18,7 @@ OK = 0
 E_ARG, E_UNSUPPORTED, E_SIZE, E_HIP, E_PROFILE = -1, -2, -3, -4, -5
 BASIC, FINAL, BOTH = 1, 2, 3
 MEM_HOST, MEM_DEVICE = 0, 1
-PREC_FP32, PREC_FP16, PREC_BF16, PREC_FP32_FAST = 0, 1, 2, 3
+PREC_FP32, PREC_FP16, PREC_BF16, PREC_FP32_FAST, PREC_FP32_MFMA = 0, 1, 2, 3, 4
 VARIANT_HAAR, VARIANT_KAISER = 1, 2
 
 # Every function include/isp_bm3d.h declares.

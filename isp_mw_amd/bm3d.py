@@ -198,7 +198,8 @@ class _Filter:
         lib = L.lib()
         L.check(lib.isp_bm3d_create(ctypes.byref(self._ctx), ctypes.byref(params), int(device)))
         self.device = device
-        prec = {"fp32": L.PREC_FP32, "fp16": L.PREC_FP16, "bf16": L.PREC_BF16, "fp32fast": L.PREC_FP32_FAST}[precision]
+        prec = {"fp32": L.PREC_FP32, "fp16": L.PREC_FP16, "bf16": L.PREC_BF16, "fp32fast": L.PREC_FP32_FAST,
+                "fp32mfma": L.PREC_FP32_MFMA}[precision]
         L.check(lib.isp_bm3d_set_precision(self._ctx, prec), self._ctx)
 
     def set_variant(self, haar: bool = False, kaiser_beta: float = 0.0) -> None:

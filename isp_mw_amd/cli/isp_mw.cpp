@@ -7,10 +7,10 @@
 //   isp_mw --bm3d <input> [-T tag] [-F format] [-P profile] [--ref path]
 //          [-S sigma]... [-BS1 n] [-BSP1 n] [-GS1 n] [-MR1 n] [-MS1 n] [-TH1 x] [-LD x]
 //          [-BS2 n] [-BSP2 n] [-GS2 n] [-MR2 n] [-MS2 n] [-TH2 x]
-//          [-D device] [--precision fp32|fp32fast|fp16|bf16] [--print-params]
+//          [-D device] [--precision fp32|fp32fast|fp32mfma|fp16|bf16] [--print-params]
 //   isp_mw --nlm|--nlmeans|--nonlocalmeans <input> [-T tag] [-F format] [--ref path]
 //          [-C true|false] [-S sigma] [-H strength] [-BS n] [-BSP n] [-GS n] [-MR n] [-MS n]
-//          [-TH x] [-D device] [--precision fp32|fp32fast|fp16|bf16] [--print-params]
+//          [-TH x] [-D device] [--precision fp32|fp32fast|fp32mfma|fp16|bf16] [--print-params]
 //
 // BM3D as the reference: the profile defaults to "fast"; -S may repeat (one
 // value per plane, the last one repeated to three) and sets both steps; thMSE
@@ -572,6 +572,7 @@ int precision_code(const std::string &p)
     if (p == "fp16") return ISP_BM3D_PREC_FP16;
     if (p == "bf16") return ISP_BM3D_PREC_BF16;
     if (p == "fp32fast") return ISP_BM3D_PREC_FP32_FAST;
+    if (p == "fp32mfma") return ISP_BM3D_PREC_FP32_MFMA;
     die("unknown precision " + p);
 }
 

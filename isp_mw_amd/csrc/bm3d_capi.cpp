@@ -34,7 +34,9 @@ struct isp_bm3d_ctx {
     int cn_off[kMaxGroup + 1] = {0};
     double *d_tables64 = nullptr;  // C8 (64) then the unpadded C_N matrices, FP64
     int cnd_off[kMaxGroup + 1] = {0};
-    uint16_t *d_mfma[2] = {nullptr, nullptr};  // MFMA operand tables, [0] FP16, [1] BF16
+    // MFMA operand tables: [0] FP16, [1] BF16, [2] the low halves of [0]
+    // (fp16(v - fp16(v)), the FP32_MFMA form's second operand halves)
+    uint16_t *d_mfma[3] = {nullptr, nullptr, nullptr};
     // generic kernels (BlockSize != 8 or GroupSize > 32), per step [basic, final]:
     // the B x B orthonormal DCT-II, then C_N for N = 1..G, FP64
     double *d_gen[2] = {nullptr, nullptr};
@@ -269,7 +271,7 @@ int derive(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, int nplanes, int W,
         // FP32 coefficients of [0,1] data carry absolute errors of a few 1e-6;
         // the plain form re-makes decisions closer than this to the threshold
         // in FP64; the FP32-throughout form does not (a negative band)
-        d.band[c] = ctx->precision == ISP_BM3D_PREC_FP32_FAST ? -1.f : (float)(3e-5 + 1e-4 * d.thr64[c]);
+        d.band[c] = ctx->precision == ISP_BM3D_PREC_FP32 ? (float)(3e-5 + 1e-4 * d.thr64[c]) : -1.f;
     }
     d.ngx = (W - B) / p.block_step + 1 + ((W - B) % p.block_step ? 1 : 0);
     d.ngy = (H - B) / p.block_step + 1 + ((H - B) % p.block_step ? 1 : 0);
@@ -456,12 +458,18 @@ int size_class(int W, int H)
     return k;
 }
 
-// The context's engines: the FP16/BF16 matrix-core filter (engine 1 or 2)
-// or the FP32 vector filter (engine 0); matching with FP64 distances in the
+// The context's engines: the FP16/BF16 matrix-core filter (engine 1 or 2),
+// the split FP16 matrix-core filter of the FP32_MFMA form (engine 3) or the
+// FP32 vector filter (engine 0); matching with FP64 distances in the
 // reference's order (the plain FP32 form) or FP32 distances (every other).
 inline int filter_engine(const isp_bm3d_ctx *ctx)
 {
-    return ctx->precision == ISP_BM3D_PREC_FP16 ? 1 : ctx->precision == ISP_BM3D_PREC_BF16 ? 2 : 0;
+    switch (ctx->precision) {
+    case ISP_BM3D_PREC_FP16: return 1;
+    case ISP_BM3D_PREC_BF16: return 2;
+    case ISP_BM3D_PREC_FP32_MFMA: return 3;
+    default: return 0;
+    }
 }
 inline bool exact_matching(const isp_bm3d_ctx *ctx) { return ctx->precision == ISP_BM3D_PREC_FP32; }
 
@@ -483,6 +491,25 @@ isp_bm3d_ctx::Tiles tiles_for(const isp_bm3d_ctx *ctx, int W, int H)
         } else {
             t.filter_x = it->second.filter_tile[0];
             t.filter_y = it->second.filter_tile[1];
+        }
+    }
+    if (filter_engine(ctx) == 3) {
+        // the FP32_MFMA form stages both halves of its footprint and of the
+        // K64 fragments: where a step's LDS would exceed a CU's, the tile
+        // shrinks (rows first) until it fits (tiles never change results)
+        auto fits = [&](int tx, int ty) {
+            for (const isp_bm3d_step_params *q : {&ctx->params.basic, &ctx->params.final_}) {
+                FilterArgs probe{};
+                probe.FW = (tx - 1) * q->block_step + 2 * q->bm_range + kBlock;
+                probe.FH = (ty - 1) * q->block_step + 2 * q->bm_range + kBlock;
+                probe.kbl = ctx->d_mfma[2];
+                if (filter_mfma_lds_bytes(probe, true) > 160 * 1024) return false;
+            }
+            return true;
+        };
+        while (t.mfma_x * t.mfma_y > 1 && !fits(t.mfma_x, t.mfma_y)) {
+            if (t.mfma_y >= t.mfma_x) t.mfma_y = (t.mfma_y + 1) / 2;
+            else t.mfma_x = (t.mfma_x + 1) / 2;
         }
     }
     return t;
@@ -575,14 +602,24 @@ FilterArgs build_filter(const isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, 
     }
     f.c8d = ctx->d_tables64;
     f.cnd = ctx->d_tables64;
-    if (filter_engine(ctx)) {
-        const uint16_t *t = ctx->d_mfma[filter_engine(ctx) - 1];
+    if (const int eng = filter_engine(ctx)) {
+        // the FP32_MFMA form: FP16 high halves plus the low-half tables
+        const uint16_t *t = ctx->d_mfma[eng == 3 ? 0 : eng - 1];
         f.kb = t;
         f.ka = t + 8 * 64 * 8;
         f.cna = t + 16 * 64 * 8;
         f.cnb = f.cna + (size_t)kMaxGroup * 2 * 64 * 8;
         f.cnap = f.cnb + (size_t)kMaxGroup * 2 * 64 * 8;
         f.cnbp = f.cnap + (size_t)kPairTables * 2 * 64 * 8;
+        if (eng == 3) {
+            const uint16_t *l = ctx->d_mfma[2];
+            f.kbl = l;
+            f.kal = l + 8 * 64 * 8;
+            f.cnal = l + 16 * 64 * 8;
+            f.cnbl = f.cnal + (size_t)kMaxGroup * 2 * 64 * 8;
+            f.cnapl = f.cnbl + (size_t)kMaxGroup * 2 * 64 * 8;
+            f.cnbpl = f.cnapl + (size_t)kPairTables * 2 * 64 * 8;
+        }
     }
     return f;
 }
@@ -1587,9 +1624,10 @@ int isp_bm3d_create(isp_bm3d_ctx **out, const isp_bm3d_params *params, int devic
     }
     size_t n_kb, n_ka, n_cn;
     std::vector<float> mt = mfma_tables_f32(&n_kb, &n_ka, &n_cn);
-    for (int k = 0; k < 2; ++k) {
+    for (int k = 0; k < 3; ++k) {
         std::vector<uint16_t> bits(mt.size());
-        for (size_t i = 0; i < mt.size(); ++i) bits[i] = k == 0 ? to_f16(mt[i]) : to_bf16(mt[i]);
+        for (size_t i = 0; i < mt.size(); ++i)
+            bits[i] = k == 0 ? to_f16(mt[i]) : k == 1 ? to_bf16(mt[i]) : to_f16(mt[i] - (float)(_Float16)mt[i]);
         if ((e = hipMalloc(&ctx->d_mfma[k], bits.size() * 2)) != hipSuccess ||
             (e = hipMemcpy(ctx->d_mfma[k], bits.data(), bits.size() * 2, hipMemcpyHostToDevice)) != hipSuccess)
             break;
@@ -1598,7 +1636,7 @@ int isp_bm3d_create(isp_bm3d_ctx **out, const isp_bm3d_params *params, int devic
         int rc = fail(nullptr, ISP_BM3D_E_HIP, "create: %s", hipGetErrorString(e));
         (void)hipFree(ctx->d_tables);
         (void)hipFree(ctx->d_tables64);
-        for (int k = 0; k < 2; ++k)
+        for (int k = 0; k < 3; ++k)
             if (ctx->d_mfma[k]) (void)hipFree(ctx->d_mfma[k]);
         delete ctx;
         return rc;
@@ -1690,6 +1728,7 @@ int isp_bm3d_destroy(isp_bm3d_ctx *ctx)
     if (ctx->pin) (void)hipHostFree(ctx->pin);
     if (ctx->d_tables) (void)hipFree(ctx->d_tables);
     if (ctx->d_tables64) (void)hipFree(ctx->d_tables64);
+    if (ctx->d_mfma[2]) (void)hipFree(ctx->d_mfma[2]);
     for (int k = 0; k < 2; ++k) {
         if (ctx->d_mfma[k]) (void)hipFree(ctx->d_mfma[k]);
         if (ctx->d_gen[k]) (void)hipFree(ctx->d_gen[k]);
@@ -1778,7 +1817,7 @@ int isp_bm3d_set_variant(isp_bm3d_ctx *ctx, int flags, double kaiser_beta)
 int isp_bm3d_set_precision(isp_bm3d_ctx *ctx, int precision)
 {
     if (!ctx) return fail(nullptr, ISP_BM3D_E_ARG, "null context");
-    if (precision < ISP_BM3D_PREC_FP32 || precision > ISP_BM3D_PREC_FP32_FAST)
+    if (precision < ISP_BM3D_PREC_FP32 || precision > ISP_BM3D_PREC_FP32_MFMA)
         return fail(ctx, ISP_BM3D_E_ARG, "bad precision %d", precision);
     ctx->precision = precision;
     return ISP_BM3D_OK;
@@ -3132,6 +3171,8 @@ int stack_common(isp_bm3d_ctx *ctx, float *const *dst, const float *const *src, 
     if (radius < 0 || radius > kMaxStackRadius)
         return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "temporal radius %d outside 0..%d", radius, kMaxStackRadius);
     if (!dst || !src) return fail(ctx, ISP_BM3D_E_ARG, "null dst/src");
+    if (filter_engine(ctx) == 3)
+        return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "multi-frame groups run in the fp32, fp32fast, fp16 and bf16 forms");
     const int n = T * nplanes;
     for (int k = 0; k < n; ++k)
         if (!dst[k] || !src[k] || (ref && !ref[k])) return fail(ctx, ISP_BM3D_E_ARG, "null plane %d", k);

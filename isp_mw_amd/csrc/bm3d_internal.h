@@ -191,6 +191,9 @@ struct FilterArgs {
     // cnap / cnbp[(N1-1)*17 + N2][2][64][8]: the same for two groups of
     // N1 (1..16) and N2 (0..16) blocks in one operand, diag(C_N1, C_N2)
     const uint16_t *kb, *ka, *cna, *cnb, *cnap, *cnbp;
+    // The FP32_MFMA form only (else null): the low halves of the FP16 tables
+    // above, fp16(v - fp16(v)), in the same layouts.
+    const uint16_t *kbl, *kal, *cnal, *cnbl, *cnapl, *cnbpl;
     // Multi-frame groups (stack kernels only).  One launch covers every
     // frame of the stack: the groups of reference frame t are cnt / pos /
     // slot + t * (ngx*ngy) (times G for pos and slot); member z of a group
@@ -339,7 +342,9 @@ struct BatchLaunch {
 // guide64: the guide is FP64 (else FP32).  `b` null: a single-image launch.
 hipError_t launch_match(const MatchArgs &a, bool guide64, hipStream_t s, const BatchLaunch *b = nullptr);
 hipError_t launch_filter(const FilterArgs &a, bool wiener, hipStream_t s, const BatchLaunch *b = nullptr);
-// precision: 1 = FP16, 2 = BF16 operands with FP32 accumulation
+// precision: 1 = FP16, 2 = BF16 operands with FP32 accumulation, 3 = FP16
+// operands split in high and low halves (FP32-accurate products; needs the
+// low-half tables in FilterArgs)
 hipError_t launch_filter_mfma(const FilterArgs &a, bool wiener, int precision, hipStream_t s,
                               const BatchLaunch *b = nullptr);
 size_t filter_mfma_lds_bytes(const FilterArgs &a, bool wiener);

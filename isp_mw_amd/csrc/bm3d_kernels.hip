@@ -16,6 +16,8 @@
 //   finalize_kernel  - num/den (source/BM3D.cpp:132-135, 204-217), optionally
 //                      fused with OPP->RGB and clip (include/Conversion.hpp:739-762).
 //   opp_kernel       - RGB->OPP without clip (include/Conversion.hpp:547-578).
+#include <type_traits>
+
 #include "bm3d_device.h"
 
 namespace ispbm3d {
@@ -1346,26 +1348,114 @@ __device__ __forceinline__ void mfma_inverse(const f32x16 (&y)[2], const uint16_
     }
 }
 
+// FP32-accurate products on the FP16 matrix cores (the FP32_MFMA form).  An
+// operand a splits into a_hi = fp16(a) and a_lo = fp16(a - a_hi), 22
+// significant bits together, and A B = A_hi B_hi + A_hi B_lo + A_lo B_hi in
+// FP32 accumulation, less A_lo B_lo (about 2^-22 of the product).  The small
+// terms go in first.
+__device__ __forceinline__ void split_frag(const f32x16 &acc, int m, f16x8 &hi, f16x8 &lo)
+{
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const float v = acc[8 * m + e];
+        const _Float16 h = (_Float16)v;
+        hi[e] = h;
+        lo[e] = (_Float16)(v - (float)h);
+    }
+}
+
+__device__ __forceinline__ f32x16 mma3(const f16x8 &ah, const f16x8 &al, const f16x8 &bh, const f16x8 &bl,
+                                       f32x16 c)
+{
+    c = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, c, 0, 0, 0);
+}
+
+// the low halves of a pair of samples, as mfma_load_pairs reads them
+__device__ __forceinline__ uint32_t pack_lo_pair(float a, float b)
+{
+    return pack_pair<_Float16>(a - (float)(_Float16)a, b - (float)(_Float16)b);
+}
+
+// mfma_forward_half with every product split
+__device__ __forceinline__ f32x16 split_forward_half(const f16x8 (&xh)[4], const f16x8 (&xl)[4], int q,
+                                                     const uint16_t *kb, const uint16_t *kbl, int lane,
+                                                     const f16x8 (&cah)[2], const f16x8 (&cal)[2])
+{
+    f32x16 t = (f32x16){};
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+        t = mma3(xh[s], xl[s], lds_frag<_Float16>(kb, s * 2 + q, lane), lds_frag<_Float16>(kbl, s * 2 + q, lane), t);
+    f32x16 y = (f32x16){};
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+        f16x8 th, tl;
+        split_frag(t, m, th, tl);
+        y = mma3(cah[m], cal[m], th, tl, y);
+    }
+    return y;
+}
+
+// mfma_inverse_half with every product split
+__device__ __forceinline__ f32x16 split_inverse_half(const f32x16 &yq, const f16x8 (&cbh)[2], const f16x8 (&cbl)[2])
+{
+    f32x16 vt = (f32x16){};
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+        f16x8 yh, yl;
+        split_frag(yq, m, yh, yl);
+        vt = mma3(yh, yl, cbh[m], cbl[m], vt);
+    }
+    return vt;
+}
+
+// mfma_inverse_rest with every product split
+__device__ __forceinline__ void split_inverse_rest(const f32x16 (&vt)[2], const uint16_t *ka, const uint16_t *kal,
+                                                   int lane, f32x16 (&xh)[2])
+{
+    f16x8 vh[4], vl[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) split_frag(vt[s >> 1], s & 1, vh[s], vl[s]);
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        xh[r] = (f32x16){};
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+            xh[r] = mma3(lds_frag<_Float16>(ka, r * 4 + s, lane), lds_frag<_Float16>(kal, r * 4 + s, lane), vh[s],
+                         vl[s], xh[r]);
+    }
+}
+
 // LDS of the MFMA filter: accumulators, the K64 operand fragments (forward B
 // and inverse A, 16 x 64 lanes x 8 elements of 16 bits), then the staged
 // source (and guide) footprint, one 32-bit word per pixel.
 constexpr int kK64Elems = 16 * 64 * 8;
 // stack: nslots accumulator slots and no staged footprint (member blocks are
 // read from HBM).
-__host__ __device__ inline size_t filter_mfma_lds(int FW, int FH, bool wiener, int nslots = 1, bool stack = false)
+__host__ __device__ inline size_t filter_mfma_lds(int FW, int FH, bool wiener, int nslots = 1, bool stack = false,
+                                                  bool split = false)
 {
-    // single frame: the staged footprint, then the Wiener schedule (2 ints per unit + count)
-    const size_t st = stack ? 0 : (size_t)FW * FH * (wiener ? 2 : 1) + (wiener ? 2 * kMaxTileRefs + 4 : 0);
-    return filter_acc_floats(FW, FH) * 4 * nslots + kK64Elems * 2 + st * 4;
+    // single frame: the staged footprint (split: its high then its low
+    // halves), then the Wiener schedule (2 ints per unit + count)
+    const size_t st = stack ? 0
+                            : (size_t)FW * FH * (wiener ? 2 : 1) * (split ? 2 : 1) +
+                                  (wiener ? 2 * kMaxTileRefs + 4 : 0);
+    return filter_acc_floats(FW, FH) * 4 * nslots + kK64Elems * 2 * (split ? 2 : 1) + st * 4;
 }
 
 // 4 waves per SIMD: two 8-wave workgroups per CU (at most 128 registers)
 // NW waves per workgroup: 8 (two workgroups per CU) or 16 (one workgroup of a
 // larger tile per CU); 4 waves per SIMD either way, 128 registers.
-template <typename T, bool WIENER, bool BATCHED, int NW = kMWaves>
-__global__ __launch_bounds__(NW * 64, 4) void filter_mfma_kernel(FilterArgs a0, const uint32_t *__restrict__ bmap,
-                                                                const FilterArgs *__restrict__ bargs)
+// SPLIT (T = _Float16): the FP32_MFMA form, every product split in high and
+// low halves (split_frag, mma3); the K64 fragments and the staged footprint
+// hold both halves; 2 waves per SIMD (256 registers).
+template <typename T, bool WIENER, bool BATCHED, int NW = kMWaves, bool SPLIT = false>
+__global__ __launch_bounds__(NW * 64, SPLIT ? 2 : 4) void filter_mfma_kernel(FilterArgs a0,
+                                                                            const uint32_t *__restrict__ bmap,
+                                                                            const FilterArgs *__restrict__ bargs)
 {
+    static_assert(!SPLIT || std::is_same<T, _Float16>::value, "the split form runs on FP16 operands");
     typedef typename MfmaT<T>::v8 v8;
     extern __shared__ __align__(16) unsigned char smem[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1382,21 +1472,28 @@ __global__ __launch_bounds__(NW * 64, 4) void filter_mfma_kernel(FilterArgs a0, 
     u64 *accDen = accNum + FW * FH;
     uint16_t *kb = reinterpret_cast<uint16_t *>(reinterpret_cast<float *>(smem) + filter_acc_floats(FW, FH));
     uint16_t *ka = kb + kK64Elems / 2;
-    uint32_t *stage = reinterpret_cast<uint32_t *>(kb + kK64Elems);
+    uint16_t *kbl = kb + kK64Elems, *kal = kbl + kK64Elems / 2;  // SPLIT: the low halves
+    uint32_t *stage = reinterpret_cast<uint32_t *>(kb + kK64Elems * (SPLIT ? 2 : 1));
     uint32_t *lsrc = stage;  // paired words (mfma_load_pairs)
     uint32_t *lgd = stage + FW * FH;
+    uint32_t *lsrcl = stage + (WIENER ? 2 : 1) * FW * FH, *lgdl = lsrcl + FW * FH;  // SPLIT: low halves
     {
         // the K64 fragments (kb then ka are contiguous in the table)
         const uint4 *srcq = reinterpret_cast<const uint4 *>(a.kb);
         uint4 *dstq = reinterpret_cast<uint4 *>(kb);
         for (int k = tid; k < kK64Elems / 8; k += NW * 64) dstq[k] = srcq[k];
+        if constexpr (SPLIT) {
+            const uint4 *srcl = reinterpret_cast<const uint4 *>(a.kbl);
+            uint4 *dstl = reinterpret_cast<uint4 *>(kbl);
+            for (int k = tid; k < kK64Elems / 8; k += NW * 64) dstl[k] = srcl[k];
+        }
     }
 
     // Wiener step: the tile's schedule of units, consecutive reference blocks
     // (2p, 2p+1) as one unit when both groups have at most 16 blocks, else one
     // unit each; sched[2u], sched[2u+1] = (refA, refB or -1), sched[2 * kMaxTileRefs]
     // = the number of units.
-    int *sched = reinterpret_cast<int *>(stage + (WIENER ? 2 : 1) * FW * FH);
+    int *sched = reinterpret_cast<int *>(stage + (WIENER ? 2 : 1) * FW * FH * (SPLIT ? 2 : 1));
     if (WIENER && wave == 0) {
         const int np = (tnum + 1) / 2;
         for (int p0 = 0; p0 < np; p0 += 64) {
@@ -1455,8 +1552,14 @@ __global__ __launch_bounds__(NW * 64, 4) void filter_mfma_kernel(FilterArgs a0, 
                 const bool in0 = rin && xx >= 0 && xx < a.W;
                 const bool in1 = rin && px + 1 < FW && xx + 1 >= 0 && xx + 1 < a.W;
                 const size_t o = (size_t)yy * a.pitch + xx;
-                lsrc[k] = pack_pair<T>(in0 ? src[o] : 0.f, in1 ? src[o + 1] : 0.f);
-                if (WIENER) lgd[k] = pack_pair<T>(in0 ? gsrc[o] : 0.f, in1 ? gsrc[o + 1] : 0.f);
+                const float s0 = in0 ? src[o] : 0.f, s1 = in1 ? src[o + 1] : 0.f;
+                lsrc[k] = pack_pair<T>(s0, s1);
+                if constexpr (SPLIT) lsrcl[k] = pack_lo_pair(s0, s1);
+                if (WIENER) {
+                    const float g0 = in0 ? gsrc[o] : 0.f, g1 = in1 ? gsrc[o + 1] : 0.f;
+                    lgd[k] = pack_pair<T>(g0, g1);
+                    if constexpr (SPLIT) lgdl[k] = pack_lo_pair(g0, g1);
+                }
             }
         }
         for (int k = tid; k < DW * DH; k += NW * 64) accDen[k] = 0;
@@ -1476,7 +1579,7 @@ __global__ __launch_bounds__(NW * 64, 4) void filter_mfma_kernel(FilterArgs a0, 
                 const int g = paired ? (c >> 4) : 0, mi = paired ? (c & 15) : c;
                 const int Ng = g ? N1 : N0;
                 const int pz = mi < Ng ? (g ? pos1 : pos0)[mi] : -1;
-                v8 cna[2], cnb[2];
+                v8 cna[2], cnb[2], cnal[2], cnbl[2];
                 {
                     const uint16_t *ta = paired ? a.cnap : a.cna, *tb = paired ? a.cnbp : a.cnb;
                     const int fi = paired ? (N0 - 1) * 17 + N1 : N0 - 1;
@@ -1484,6 +1587,14 @@ __global__ __launch_bounds__(NW * 64, 4) void filter_mfma_kernel(FilterArgs a0, 
                     for (int m = 0; m < 2; ++m) {
                         cna[m] = load_frag<T>(ta, fi * 2 + m, lane);
                         cnb[m] = load_frag<T>(tb, fi * 2 + m, lane);
+                    }
+                    if constexpr (SPLIT) {
+                        const uint16_t *tal = paired ? a.cnapl : a.cnal, *tbl = paired ? a.cnbpl : a.cnbl;
+#pragma unroll
+                        for (int m = 0; m < 2; ++m) {
+                            cnal[m] = load_frag<T>(tal, fi * 2 + m, lane);
+                            cnbl[m] = load_frag<T>(tbl, fi * 2 + m, lane);
+                        }
                     }
                 }
                 // the guide's spectrum first, one column half at a time, and
@@ -1493,11 +1604,13 @@ __global__ __launch_bounds__(NW * 64, 4) void filter_mfma_kernel(FilterArgs a0, 
                 f32x16 wg[2], vt[2];
                 float l20 = 0.f, l21 = 0.f;
                 {
-                    v8 xg[4];
+                    v8 xg[4], xgl[4];
                     mfma_load_pairs<T>(lgd, FW, ox, oy, pz, h, xg);
+                    if constexpr (SPLIT) mfma_load_pairs<T>(lgdl, FW, ox, oy, pz, h, xgl);
 #pragma unroll
                     for (int q = 0; q < 2; ++q) {
-                        wg[q] = mfma_forward_half<T>(xg, q, kb, lane, cna);
+                        if constexpr (SPLIT) wg[q] = split_forward_half(xg, xgl, q, kb, kbl, lane, cna, cnal);
+                        else wg[q] = mfma_forward_half<T>(xg, q, kb, lane, cna);
                         // column 32q + c = frequency (u, v): u = 4q + c/8, v = c%8
                         const float su =
                             sig2 * ((4 * q + (c >> 3)) == 0 ? 0.5f : 1.f) * ((c & 7) == 0 ? 0.5f : 1.f);
@@ -1521,14 +1634,18 @@ __global__ __launch_bounds__(NW * 64, 4) void filter_mfma_kernel(FilterArgs a0, 
                 // keep the source's loads after the gains (register pressure)
                 __builtin_amdgcn_sched_barrier(0);
                 {
-                    v8 xs[4];
+                    v8 xs[4], xsl[4];
                     mfma_load_pairs<T>(lsrc, FW, ox, oy, pz, h, xs);
+                    if constexpr (SPLIT) mfma_load_pairs<T>(lsrcl, FW, ox, oy, pz, h, xsl);
 #pragma unroll
                     for (int q = 0; q < 2; ++q) {
-                        f32x16 yq = mfma_forward_half<T>(xs, q, kb, lane, cna);
+                        f32x16 yq;
+                        if constexpr (SPLIT) yq = split_forward_half(xs, xsl, q, kb, kbl, lane, cna, cnal);
+                        else yq = mfma_forward_half<T>(xs, q, kb, lane, cna);
 #pragma unroll
                         for (int k = 0; k < 16; ++k) yq[k] *= wg[q][k];
-                        vt[q] = mfma_inverse_half<T>(yq, cnb);
+                        if constexpr (SPLIT) vt[q] = split_inverse_half(yq, cnb, cnbl);
+                        else vt[q] = mfma_inverse_half<T>(yq, cnb);
                     }
                 }
                 if (!paired) {
@@ -1538,7 +1655,8 @@ __global__ __launch_bounds__(NW * 64, 4) void filter_mfma_kernel(FilterArgs a0, 
                 l20 = wave_sum(l20);
                 l21 = wave_sum(l21);
                 f32x16 xh[2];
-                mfma_inverse_rest<T>(vt, ka, lane, xh);
+                if constexpr (SPLIT) split_inverse_rest(vt, ka, kal, lane, xh);
+                else mfma_inverse_rest<T>(vt, ka, lane, xh);
                 const float l2 = g ? l21 : l20;
                 const float w = l2 <= 0.f ? 1.f : fminf(1.f / l2, a.wmax);
                 const double ws = fix_weight(w);
@@ -1586,7 +1704,7 @@ __global__ __launch_bounds__(NW * 64, 4) void filter_mfma_kernel(FilterArgs a0, 
                 const int g = paired ? (c >> 4) : 0, mi = paired ? (c & 15) : c;
                 const int Ng = g ? N1 : N0;
                 const int pz = mi < Ng ? (g ? pos1 : pos0)[mi] : -1;
-                v8 cna[2], cnb[2];
+                v8 cna[2], cnb[2], cnal[2], cnbl[2];
                 {
                     const uint16_t *ta = paired ? a.cnap : a.cna, *tb = paired ? a.cnbp : a.cnb;
                     const int fi = paired ? (N0 - 1) * 17 + N1 : N0 - 1;
@@ -1595,16 +1713,28 @@ __global__ __launch_bounds__(NW * 64, 4) void filter_mfma_kernel(FilterArgs a0, 
                         cna[m] = load_frag<T>(ta, fi * 2 + m, lane);
                         cnb[m] = load_frag<T>(tb, fi * 2 + m, lane);
                     }
+                    if constexpr (SPLIT) {
+                        const uint16_t *tal = paired ? a.cnapl : a.cnal, *tbl = paired ? a.cnbpl : a.cnbl;
+#pragma unroll
+                        for (int m = 0; m < 2; ++m) {
+                            cnal[m] = load_frag<T>(tal, fi * 2 + m, lane);
+                            cnbl[m] = load_frag<T>(tbl, fi * 2 + m, lane);
+                        }
+                    }
                 }
                 // accumulator element k of this lane lies on row 8(k/4) + 4h + k%4,
                 // of group 1 when paired and k >= 8
                 // (the Wiener step takes the schedule above: here hard thresholds only)
                 f32x16 xh[2];
-                v8 xs[4];
+                v8 xs[4], xsl[4];
                 mfma_load_pairs<T>(lsrc, FW, ox, oy, pz, h, xs);
+                if constexpr (SPLIT) mfma_load_pairs<T>(lsrcl, FW, ox, oy, pz, h, xsl);
                 f32x16 y[2];
 #pragma unroll
-                for (int q = 0; q < 2; ++q) y[q] = mfma_forward_half<T>(xs, q, kb, lane, cna);
+                for (int q = 0; q < 2; ++q) {
+                    if constexpr (SPLIT) y[q] = split_forward_half(xs, xsl, q, kb, kbl, lane, cna, cnal);
+                    else y[q] = mfma_forward_half<T>(xs, q, kb, lane, cna);
+                }
                 int kept = 0;  // group 0 in the low 16 bits, group 1 above
 #pragma unroll
                 for (int q = 0; q < 2; ++q)
@@ -1617,7 +1747,14 @@ __global__ __launch_bounds__(NW * 64, 4) void filter_mfma_kernel(FilterArgs a0, 
                 const int k0 = kept & 0xFFFF, k1 = kept >> 16;
                 const float w0 = k0 < 1 ? 1.f : 1.f / (float)k0;
                 const float w1 = k1 < 1 ? 1.f : 1.f / (float)k1;
-                mfma_inverse<T>(y, ka, lane, cnb, xh);
+                if constexpr (SPLIT) {
+                    f32x16 vt[2];
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) vt[q] = split_inverse_half(y[q], cnb, cnbl);
+                    split_inverse_rest(vt, ka, kal, lane, xh);
+                } else {
+                    mfma_inverse<T>(y, ka, lane, cnb, xh);
+                }
                 const float w = g ? w1 : w0;
                 const double ws = fix_weight(w);
                 if (pz >= 0) {
@@ -2101,7 +2238,10 @@ hipError_t launch_filter(const FilterArgs &a, bool wiener, hipStream_t s, const 
              : launch_tiled(filter_kernel<false, false>, kThreads, lds, a, s, b, grid);
 }
 
-size_t filter_mfma_lds_bytes(const FilterArgs &a, bool wiener) { return filter_mfma_lds(a.FW, a.FH, wiener); }
+size_t filter_mfma_lds_bytes(const FilterArgs &a, bool wiener)
+{
+    return filter_mfma_lds(a.FW, a.FH, wiener, 1, false, a.kbl != nullptr);
+}
 
 size_t filter_stack_lds_bytes(const FilterArgs &a, bool wiener, int precision)
 {
@@ -2152,10 +2292,24 @@ hipError_t launch_mfma_t(const FilterArgs &a, size_t lds, dim3 grid, hipStream_t
                 : launch_tiled(filter_mfma_kernel<T, W, false>, kMThreads, lds, a, s, b, grid);
 }
 
+// The split (FP32_MFMA) form: 8-wave workgroups at 2 waves per SIMD, one
+// workgroup per CU (its LDS holds both halves of the K64 fragments and of the
+// staged footprint).
+template <bool W>
+hipError_t launch_mfma_split_t(const FilterArgs &a, size_t lds, dim3 grid, hipStream_t s, const BatchLaunch *b)
+{
+    if (b) return launch_tiled(filter_mfma_kernel<_Float16, W, true, kMWaves, true>, kMThreads, lds, a, s, b, grid);
+    return launch_tiled(filter_mfma_kernel<_Float16, W, false, kMWaves, true>, kMThreads, lds, a, s, b, grid);
+}
+
 hipError_t launch_filter_mfma(const FilterArgs &a, bool wiener, int precision, hipStream_t s, const BatchLaunch *b)
 {
     const size_t lds = filter_mfma_lds_bytes(a, wiener);
     const dim3 grid((a.ngx + a.trx - 1) / a.trx, (a.ngy + a.try_ - 1) / a.try_);
+    if (precision == 3) {
+        if (!a.kbl || !a.kal || !a.cnal || !a.cnbl || !a.cnapl || !a.cnbpl) return hipErrorInvalidValue;
+        return wiener ? launch_mfma_split_t<true>(a, lds, grid, s, b) : launch_mfma_split_t<false>(a, lds, grid, s, b);
+    }
     if (precision == 1)
         return wiener ? launch_mfma_t<_Float16, true>(a, lds, grid, s, b) : launch_mfma_t<_Float16, false>(a, lds, grid, s, b);
     return wiener ? launch_mfma_t<__bf16, true>(a, lds, grid, s, b) : launch_mfma_t<__bf16, false>(a, lds, grid, s, b);

@@ -31,7 +31,7 @@ def single(p_sigma, n, prec, cls=BM3D, **kw):
     return f(x).cpu().numpy()
 
 
-@pytest.mark.parametrize("prec", ["fp32", "fp16", "fp32fast"])
+@pytest.mark.parametrize("prec", ["fp32", "fp16", "fp32fast", "fp32mfma"])
 def test_grey_batch_equals_single_calls(gpu, prec):
     import torch
     imgs = [synth.grey_pair(h, w, s or 25.0, 50 + k)[1] for k, ((h, w), s) in enumerate(zip(SIZES, SIGMAS))]

@@ -1,6 +1,7 @@
 """Parity of the FP32-distance forms against the FP64 oracle: the headline
-form (FP16 / BF16 MFMA transforms) and the FP32-throughout form (fp32fast:
-FP32 distances, FP32 transforms without FP64 re-decisions).
+form (FP16 / BF16 MFMA transforms), the FP32-throughout form (fp32fast:
+FP32 distances, FP32 transforms without FP64 re-decisions) and the same
+arithmetic with FP32-accurate products on the matrix cores (fp32mfma).
 
 Matching: the FP32-distance kernels (column walks for BlockStep 3..7, the list
 kernel otherwise) must reproduce the oracle's ordered groups
@@ -72,7 +73,7 @@ def test_fp32_distance_groups_512_grey(gpu, prec, walk, monkeypatch):
         assert rel < 1e-4
 
 
-@pytest.mark.parametrize("prec", ["fp16", "bf16", "fp32fast"])
+@pytest.mark.parametrize("prec", ["fp16", "bf16", "fp32fast", "fp32mfma"])
 def test_fp32_distance_groups_1080p_rgb(gpu, prec):
     """Groups of the benchmark frame as the fp16 / bf16 form forms them: step 1
     on the FP64 OPP luma of the noisy frame, step 2 (BM3D_Final) on a given
@@ -116,7 +117,7 @@ def test_fp16_1080p_rgb_vs_oracle(gpu, sigma):
     p = para(sigma=sigma)
     x = torch.from_numpy(noisy).to(gpu)
     outs = {}
-    for prec in ("fp32", "fp32fast", "fp16", "bf16"):
+    for prec in ("fp32", "fp32fast", "fp32mfma", "fp16", "bf16"):
         outs[prec] = BM3D(p, precision=prec)(x).cpu().numpy().astype(np.float64)
     say(f"1080p sigma {sigma}: oracle")
     oracle.set_threads(threads())
@@ -129,17 +130,18 @@ def test_fp16_1080p_rgb_vs_oracle(gpu, sigma):
     p32 = synth.psnr(outs["fp32"], clean)
     say(f"1080p sigma {sigma}: PSNR noisy {synth.psnr(noisy, clean):.3f} oracle {p_ref:.4f} fp32 {p32:.4f}")
     assert abs(p32 - p_ref) <= 0.01
-    out = outs["fp32fast"]
-    pf = synth.psnr(out, clean)
-    mx = [float(np.abs(out[c] - ref[c]).max()) for c in range(3)]
-    mean = [float(np.abs(out[c] - ref[c]).mean()) for c in range(3)]
-    big = int(sum((np.abs(out[c] - ref[c]) > 2e-3).sum() for c in range(3)))
-    say(f"  fp32fast: PSNR {pf:.4f} ({pf - p_ref:+.5f} dB vs oracle); |out - oracle| per channel "
-        f"R,G,B max {mx[0]:.3g} {mx[1]:.3g} {mx[2]:.3g} mean {mean[0]:.3g} {mean[1]:.3g} {mean[2]:.3g}; "
-        f"{big} samples beyond 2e-3")
-    assert abs(pf - p_ref) <= 0.01
-    assert max(mx) <= MAX_ABS_FAST
-    assert max(mean) <= MEAN_ABS_FAST
+    for prec in ("fp32fast", "fp32mfma"):
+        out = outs[prec]
+        pf = synth.psnr(out, clean)
+        mx = [float(np.abs(out[c] - ref[c]).max()) for c in range(3)]
+        mean = [float(np.abs(out[c] - ref[c]).mean()) for c in range(3)]
+        big = int(sum((np.abs(out[c] - ref[c]) > 2e-3).sum() for c in range(3)))
+        say(f"  {prec}: PSNR {pf:.4f} ({pf - p_ref:+.5f} dB vs oracle); |out - oracle| per channel "
+            f"R,G,B max {mx[0]:.3g} {mx[1]:.3g} {mx[2]:.3g} mean {mean[0]:.3g} {mean[1]:.3g} {mean[2]:.3g}; "
+            f"{big} samples beyond 2e-3")
+        assert abs(pf - p_ref) <= 0.01
+        assert max(mx) <= MAX_ABS_FAST
+        assert max(mean) <= MEAN_ABS_FAST
     for prec in ("fp16", "bf16"):
         out = outs[prec]
         pm = synth.psnr(out, clean)
