@@ -39,19 +39,20 @@ extern "C" {
 #define ISP_BM3D_MEM_DEVICE 1
 
 /* Arithmetic of the transforms and shrinkage, and of the matching distances.
- * PREC_FP32: FP32 transforms and shrinkage (hard-threshold decisions within
- * a narrow band of the threshold re-made in FP64), FP64 distances summed in
- * the reference's order (its tie order exactly).  PREC_FP32_FAST: the same
- * FP32 transforms and shrinkage without the FP64 re-decision, and FP32
- * distances (fused multiply-add) as the mixed forms match.  PREC_FP16 /
- * PREC_BF16: 16-bit MFMA operands with FP32 accumulation, FP32 shrinkage,
- * FP32 distances.  PREC_FP32_MFMA: the FP32_FAST form's arithmetic with the
- * transform products on the FP16 matrix cores, each operand split into
- * fp16(a) + fp16(a - fp16(a)) (22 significant bits) and each product taken
- * as three MFMAs with FP32 accumulation (single frames and batches; the
- * multi-frame entry point returns E_UNSUPPORTED).  Every form aggregates in
- * 64-bit fixed point (exact, order-independent sums; see
- * isp_bm3d_aggregation_limits). */
+ * PREC_FP32: FP32-accurate transforms and FP32 shrinkage (hard-threshold
+ * decisions within a narrow band of the threshold re-made in FP64), FP64
+ * distances summed in the reference's order (its tie order exactly).
+ * PREC_FP32_FAST: FP32 transforms on the vector ALUs and FP32 shrinkage
+ * without the FP64 re-decision, and FP32 distances (fused multiply-add) as
+ * the mixed forms match.  PREC_FP16 / PREC_BF16: 16-bit MFMA operands with
+ * FP32 accumulation, FP32 shrinkage, FP32 distances.  PREC_FP32_MFMA: the
+ * FP32_FAST form's arithmetic with the transform products on the FP16 matrix
+ * cores, each operand split into fp16(a) + fp16(a - fp16(a)) (22 significant
+ * bits) and each product taken as three MFMAs with FP32 accumulation.  The
+ * plain form's transforms are the same split products (the environment
+ * variable ISP_BM3D_FP32_VALU=1 puts them back on the FP32 vector ALUs).
+ * Every form aggregates in 64-bit fixed point (exact, order-independent sums;
+ * see isp_bm3d_aggregation_limits). */
 #define ISP_BM3D_PREC_FP32 0      /* plain FP32 path, the reference's groups exactly */
 #define ISP_BM3D_PREC_FP16 1      /* FP16 MFMA transforms, FP32 accumulation        */
 #define ISP_BM3D_PREC_BF16 2      /* BF16 MFMA transforms, FP32 accumulation        */

@@ -26,6 +26,7 @@ struct isp_bm3d_ctx {
     isp_bm3d_params params;
     int device = 0;
     int precision = ISP_BM3D_PREC_FP32;
+    bool plain_valu = false;  // the plain form's filter on the vector ALUs (ISP_BM3D_FP32_VALU)
     // FP32-distance matching by column walks where eligible; the environment
     // variable ISP_BM3D_MATCH_WALK=0 at context creation keeps every search
     // on the list kernel (the two give identical groups)
@@ -459,18 +460,24 @@ int size_class(int W, int H)
 }
 
 // The context's engines: the FP16/BF16 matrix-core filter (engine 1 or 2),
-// the split FP16 matrix-core filter of the FP32_MFMA form (engine 3) or the
-// FP32 vector filter (engine 0); matching with FP64 distances in the
-// reference's order (the plain FP32 form) or FP32 distances (every other).
+// the split FP16 matrix-core filter with FP32-accurate products (engine 3:
+// the FP32_MFMA form, and the plain form, which adds the FP64 re-decision
+// near the hard threshold; ISP_BM3D_FP32_VALU=1 puts the plain form back on
+// the vector filter) or the FP32 vector filter (engine 0: the FP32_FAST
+// form); matching with FP64 distances in the reference's order (the plain
+// form) or FP32 distances (every other).
 inline int filter_engine(const isp_bm3d_ctx *ctx)
 {
     switch (ctx->precision) {
     case ISP_BM3D_PREC_FP16: return 1;
     case ISP_BM3D_PREC_BF16: return 2;
     case ISP_BM3D_PREC_FP32_MFMA: return 3;
+    case ISP_BM3D_PREC_FP32: return ctx->plain_valu ? 0 : 3;
     default: return 0;
     }
 }
+// Multi-frame groups: the same engines on the stack kernels.
+inline int stack_engine(const isp_bm3d_ctx *ctx) { return filter_engine(ctx); }
 inline bool exact_matching(const isp_bm3d_ctx *ctx) { return ctx->precision == ISP_BM3D_PREC_FP32; }
 
 // Tile shapes of an image: the tuning entry of its size class, else the defaults.
@@ -1575,6 +1582,7 @@ int isp_bm3d_create(isp_bm3d_ctx **out, const isp_bm3d_params *params, int devic
     isp_bm3d_ctx *ctx = new isp_bm3d_ctx();
     ctx->params = *params;
     if (const char *mw = std::getenv("ISP_BM3D_MATCH_WALK")) ctx->match_walk = std::strcmp(mw, "0") != 0;
+    if (const char *pv = std::getenv("ISP_BM3D_FP32_VALU")) ctx->plain_valu = std::strcmp(pv, "0") != 0;
     ctx->device = device;
     ctx->err = "no error";
     // Orthonormal DCT-II matrices: C8 [k][n], then C_N for N = 1..32, each
@@ -3059,7 +3067,7 @@ int stack_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, co
                     kMaxGroup);
     const size_t fs = io.fs;
     const int T = io.T, P = io.P;
-    const bool mfma = filter_engine(ctx) != 0;
+    const bool mfma = stack_engine(ctx) != 0;
     if (!d.degenerate) {
         const isp_bm3d_ctx::Tiles tl = tiles_for(ctx, W, H);
         const size_t refs = (size_t)d.ngx * d.ngy;
@@ -3121,7 +3129,7 @@ int stack_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, co
             f.wmax = wiener_weight_cap(kBlock, p.block_step, d.range, p.bm_step, p.group_size, io.radius);
             f.nslots = std::min(2 * io.radius + 1, T);
             f.fstride = fs;
-            if (filter_stack_lds_bytes(f, wiener, filter_engine(ctx)) <= 160 * 1024) break;
+            if (filter_stack_lds_bytes(f, wiener, stack_engine(ctx)) <= 160 * 1024) break;
             if (trx == 1 && try_ == 1)
                 return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "radius %d needs more LDS than a CU has", io.radius);
             if (trx >= try_ && trx > 1) trx = (trx + 1) / 2;
@@ -3131,7 +3139,7 @@ int stack_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, co
             return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "radius %d: a pixel's aggregation sums could overflow",
                         io.radius);
         StageTimer tm(ctx, wiener ? ST_FILTER_FINAL : ST_FILTER_BASIC, s);
-        HIP_CHECK(ctx, launch_filter_stack(f, wiener, filter_engine(ctx), s));
+        HIP_CHECK(ctx, launch_filter_stack(f, wiener, stack_engine(ctx), s));
     }
     for (int t = 0; t < T; ++t) {
         FinalizeArgs fa{};
@@ -3171,8 +3179,6 @@ int stack_common(isp_bm3d_ctx *ctx, float *const *dst, const float *const *src, 
     if (radius < 0 || radius > kMaxStackRadius)
         return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "temporal radius %d outside 0..%d", radius, kMaxStackRadius);
     if (!dst || !src) return fail(ctx, ISP_BM3D_E_ARG, "null dst/src");
-    if (filter_engine(ctx) == 3)
-        return fail(ctx, ISP_BM3D_E_UNSUPPORTED, "multi-frame groups run in the fp32, fp32fast, fp16 and bf16 forms");
     const int n = T * nplanes;
     for (int k = 0; k < n; ++k)
         if (!dst[k] || !src[k] || (ref && !ref[k])) return fail(ctx, ISP_BM3D_E_ARG, "null plane %d", k);

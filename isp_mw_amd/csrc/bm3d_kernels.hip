@@ -1257,6 +1257,35 @@ __device__ __forceinline__ void mfma_load_x(const float *plane, int pitch, int o
     }
 }
 
+// mfma_load_x in high and low halves (the FP32_MFMA / plain forms' stacks)
+__device__ __forceinline__ void mfma_load_x_split(const float *plane, int pitch, int ox, int oy,
+                                                  const int *__restrict__ pos, int N, int lane, f16x8 (&xh)[4],
+                                                  f16x8 (&xl)[4], const int *__restrict__ slot = nullptr,
+                                                  size_t slotsz = 0)
+{
+    const int c = lane & 31, h = lane >> 5;
+    if (c < N) {
+        const int pz = pos[c];
+        const float *b = plane + ((pz >> 16) - oy) * pitch + (pz & 0xFFFF) - ox + (slot ? slot[c] * slotsz : 0);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const float *r = b + (2 * s + h) * pitch;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const float v = r[e];
+                const _Float16 hv = (_Float16)v;
+                xh[s][e] = hv;
+                xl[s][e] = (_Float16)(v - (float)hv);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) xh[s][e] = xl[s][e] = (_Float16)0.f;
+    }
+}
+
 // The staged footprint of the single-frame MFMA filter holds, for every
 // pixel x of a row, the 32-bit word (T(p[x]), T(p[x + 1])): a block row of 8
 // samples at any x is the words x, x + 2, x + 4, x + 6, read without
@@ -1735,12 +1764,48 @@ __global__ __launch_bounds__(NW * 64, SPLIT ? 2 : 4) void filter_mfma_kernel(Fil
                     if constexpr (SPLIT) y[q] = split_forward_half(xs, xsl, q, kb, kbl, lane, cna, cnal);
                     else y[q] = mfma_forward_half<T>(xs, q, kb, lane, cna);
                 }
+                // SPLIT with band >= 0 (the plain form): decisions within
+                // band of the threshold are re-made from the coefficient in
+                // FP64 (exact_coef_wave, the whole wave per coefficient);
+                // bit 16q + k of a lane's masks stands for its element k of half q
+                uint32_t nearm = 0, zerom = 0;
+                if constexpr (SPLIT) {
+                    const float band = a.band[pl];
+                    if (band >= 0.f) {
+#pragma unroll
+                        for (int q = 0; q < 2; ++q)
+#pragma unroll
+                            for (int k = 0; k < 16; ++k)
+                                if (fabsf(fabsf(y[q][k]) - thr) <= band) nearm |= 1u << (16 * q + k);
+                        uint64_t lanes = __ballot(nearm != 0);
+                        while (lanes) {
+                            const int L = __ffsll((unsigned long long)lanes) - 1;
+                            lanes &= lanes - 1;
+                            uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)nearm, L);
+                            while (m) {
+                                const int bit = __ffs(m) - 1;
+                                m &= m - 1;
+                                const int q = bit >> 4, k = bit & 15, cL = L & 31, hL = L >> 5;
+                                // element k of lane L: group-axis row 8(k/4) + 4h + k%4, column 32q + c
+                                const int row = 8 * (k >> 2) + 4 * hL + (k & 3);
+                                const int gg = paired ? (row >> 4) : 0, kz = paired ? (row & 15) : row;
+                                const int Ng = gg ? N1 : N0;
+                                const double oc = exact_coef_wave(kz, 4 * q + (cL >> 3), cL & 7, Ng, lane,
+                                                                  gg ? pos1 : pos0, src, a.pitch, a.c8d,
+                                                                  a.cnd + a.cnd_off[Ng]);
+                                if (lane == L && fabs(oc) <= a.thr64[pl]) zerom |= 1u << bit;
+                            }
+                        }
+                    }
+                }
                 int kept = 0;  // group 0 in the low 16 bits, group 1 above
 #pragma unroll
                 for (int q = 0; q < 2; ++q)
 #pragma unroll
                     for (int k = 0; k < 16; ++k) {
-                        if (fabsf(y[q][k]) <= thr) y[q][k] = 0.f;
+                        const uint32_t b = 1u << (16 * q + k);
+                        const bool zero = (nearm & b) ? (zerom & b) != 0 : fabsf(y[q][k]) <= thr;
+                        if (zero) y[q][k] = 0.f;
                         else kept += (paired && k >= 8) ? 65536 : 1;
                     }
                 kept = wave_sum(kept);
@@ -1781,8 +1846,10 @@ __global__ __launch_bounds__(NW * 64, SPLIT ? 2 : 4) void filter_mfma_kernel(Fil
 // filter_stack_kernel): a ring of accumulator slots, member blocks read from HBM.
 // NW waves per workgroup as in filter_mfma_kernel (16 where one workgroup
 // fills a CU's LDS), at most 128 registers.
-template <typename T, bool WIENER, bool BATCHED, bool STACK, int NW = kMWaves>
-__global__ __launch_bounds__(NW * 64, 4) void filter_mfma_stack_kernel(FilterArgs a0,
+// SPLIT: as filter_mfma_kernel's (split products, the FP64 re-decision of
+// the plain form), members read from HBM in high and low halves.
+template <typename T, bool WIENER, bool BATCHED, bool STACK, int NW = kMWaves, bool SPLIT = false>
+__global__ __launch_bounds__(NW * 64, SPLIT ? 2 : 4) void filter_mfma_stack_kernel(FilterArgs a0,
                                                                       const uint32_t *__restrict__ bmap,
                                                                       const FilterArgs *__restrict__ bargs)
 {
@@ -1802,10 +1869,12 @@ __global__ __launch_bounds__(NW * 64, 4) void filter_mfma_stack_kernel(FilterArg
     const int fsz = FW * FH;                          // footprint pixels
     const size_t refs = (size_t)a.ngx * a.ngy;
 
+    static_assert(!SPLIT || (STACK && std::is_same<T, _Float16>::value), "split stacks run on FP16 operands");
     u64 *accNum0 = reinterpret_cast<u64 *>(smem);
     uint16_t *kb = reinterpret_cast<uint16_t *>(reinterpret_cast<float *>(smem) + accw * nslots);
     uint16_t *ka = kb + kK64Elems / 2;
-    float *stage = reinterpret_cast<float *>(kb + kK64Elems);
+    uint16_t *kbl = kb + kK64Elems, *kal = kbl + kK64Elems / 2;  // SPLIT: the low halves
+    float *stage = reinterpret_cast<float *>(kb + kK64Elems * (SPLIT ? 2 : 1));
     float *lsrc = stage;          // single frame: the source footprint staged in LDS
     float *lgd = stage + fsz;     // and the Wiener guide's
     {
@@ -1813,6 +1882,11 @@ __global__ __launch_bounds__(NW * 64, 4) void filter_mfma_stack_kernel(FilterArg
         const uint4 *srcq = reinterpret_cast<const uint4 *>(a.kb);
         uint4 *dstq = reinterpret_cast<uint4 *>(kb);
         for (int k = tid; k < kK64Elems / 8; k += NW * 64) dstq[k] = srcq[k];
+        if constexpr (SPLIT) {
+            const uint4 *srcl = reinterpret_cast<const uint4 *>(a.kbl);
+            uint4 *dstl = reinterpret_cast<uint4 *>(kbl);
+            for (int k = tid; k < kK64Elems / 8; k += NW * 64) dstl[k] = srcl[k];
+        }
     }
     auto slot_num = [&](int f) { return accNum0 + (accw / 2) * (STACK ? f % nslots : 0); };
     auto zero_slot = [&](int f) {
@@ -1861,39 +1935,90 @@ __global__ __launch_bounds__(NW * 64, 4) void filter_mfma_stack_kernel(FilterArg
                 const int *slot = STACK ? a.slot + refIdx * a.G : nullptr;
                 const size_t sstride = STACK ? a.fstride : 0;
                 const int N = kconst(a.cnt)[refIdx];
-                v8 cna[2], cnb[2];
+                v8 cna[2], cnb[2], cnal[2], cnbl[2];
 #pragma unroll
                 for (int m = 0; m < 2; ++m) {
                     cna[m] = load_frag<T>(a.cna, (N - 1) * 2 + m, lane);
                     cnb[m] = load_frag<T>(a.cnb, (N - 1) * 2 + m, lane);
+                    if constexpr (SPLIT) {
+                        cnal[m] = load_frag<T>(a.cnal, (N - 1) * 2 + m, lane);
+                        cnbl[m] = load_frag<T>(a.cnbl, (N - 1) * 2 + m, lane);
+                    }
                 }
                 float w;
                 f32x16 xh[2];
                 if (!WIENER) {
                     f32x16 y[2];
-                    mfma_forward<T>(fsrc, fp, fox, foy, pos, N, lane, kb, cna, y, slot, sstride);
+                    if constexpr (SPLIT) {
+                        v8 xs[4], xsl[4];
+                        mfma_load_x_split(fsrc, fp, fox, foy, pos, N, lane, xs, xsl, slot, sstride);
+#pragma unroll
+                        for (int q = 0; q < 2; ++q) y[q] = split_forward_half(xs, xsl, q, kb, kbl, lane, cna, cnal);
+                    } else {
+                        mfma_forward<T>(fsrc, fp, fox, foy, pos, N, lane, kb, cna, y, slot, sstride);
+                    }
+                    // SPLIT with band >= 0 (the plain form): near-threshold
+                    // decisions re-made in FP64 (as in filter_mfma_kernel)
+                    uint32_t nearm = 0, zerom = 0;
+                    if constexpr (SPLIT) {
+                        const float band = a.band[pl];
+                        if (band >= 0.f) {
+#pragma unroll
+                            for (int q = 0; q < 2; ++q)
+#pragma unroll
+                                for (int k = 0; k < 16; ++k)
+                                    if (fabsf(fabsf(y[q][k]) - thr) <= band) nearm |= 1u << (16 * q + k);
+                            uint64_t lanes = __ballot(nearm != 0);
+                            while (lanes) {
+                                const int L = __ffsll((unsigned long long)lanes) - 1;
+                                lanes &= lanes - 1;
+                                uint32_t m = (uint32_t)__builtin_amdgcn_readlane((int)nearm, L);
+                                while (m) {
+                                    const int bit = __ffs(m) - 1;
+                                    m &= m - 1;
+                                    const int q = bit >> 4, k = bit & 15, cL = L & 31, hL = L >> 5;
+                                    const int row = 8 * (k >> 2) + 4 * hL + (k & 3);
+                                    const double oc = exact_coef_wave_slots(row, 4 * q + (cL >> 3), cL & 7, N, lane,
+                                                                            pos, src, a.pitch, a.c8d,
+                                                                            a.cnd + a.cnd_off[N], slot, sstride);
+                                    if (lane == L && fabs(oc) <= a.thr64[pl]) zerom |= 1u << bit;
+                                }
+                            }
+                        }
+                    }
                     int kept = 0;
 #pragma unroll
                     for (int q = 0; q < 2; ++q)
 #pragma unroll
                         for (int k = 0; k < 16; ++k) {
-                            if (fabsf(y[q][k]) <= thr) y[q][k] = 0.f;
+                            const uint32_t b = 1u << (16 * q + k);
+                            const bool zero = (nearm & b) ? (zerom & b) != 0 : fabsf(y[q][k]) <= thr;
+                            if (zero) y[q][k] = 0.f;
                             else ++kept;
                         }
                     kept = wave_sum(kept);
                     w = kept < 1 ? 1.f : 1.f / (float)kept;
-                    mfma_inverse<T>(y, ka, lane, cnb, xh);
+                    if constexpr (SPLIT) {
+                        f32x16 vt[2];
+#pragma unroll
+                        for (int q = 0; q < 2; ++q) vt[q] = split_inverse_half(y[q], cnb, cnbl);
+                        split_inverse_rest(vt, ka, kal, lane, xh);
+                    } else {
+                        mfma_inverse<T>(y, ka, lane, cnb, xh);
+                    }
                 } else {
                     // as filter_mfma_kernel: the guide's spectrum and the gains
                     // first, then the source's halves
                     f32x16 wg[2], vt[2];
                     float l2 = 0.f, l2b = 0.f;
                     {
-                        v8 xg[4];
-                        mfma_load_x<T>(fgd, fp, fox, foy, pos, N, lane, xg, slot, sstride);
+                        v8 xg[4], xgl[4];
+                        if constexpr (SPLIT) mfma_load_x_split(fgd, fp, fox, foy, pos, N, lane, xg, xgl, slot, sstride);
+                        else mfma_load_x<T>(fgd, fp, fox, foy, pos, N, lane, xg, slot, sstride);
 #pragma unroll
                         for (int q = 0; q < 2; ++q) {
-                            wg[q] = mfma_forward_half<T>(xg, q, kb, lane, cna);
+                            if constexpr (SPLIT) wg[q] = split_forward_half(xg, xgl, q, kb, kbl, lane, cna, cnal);
+                            else wg[q] = mfma_forward_half<T>(xg, q, kb, lane, cna);
                             // column 32q + c = frequency (u, v): u = 4q + c/8, v = c%8
                             const float su =
                                 sig2 * ((4 * q + (c >> 3)) == 0 ? 0.5f : 1.f) * ((c & 7) == 0 ? 0.5f : 1.f);
@@ -1914,19 +2039,24 @@ __global__ __launch_bounds__(NW * 64, 4) void filter_mfma_stack_kernel(FilterArg
                     // keep the source's loads after the gains (register pressure)
                     __builtin_amdgcn_sched_barrier(0);
                     {
-                        v8 xs[4];
-                        mfma_load_x<T>(fsrc, fp, fox, foy, pos, N, lane, xs, slot, sstride);
+                        v8 xs[4], xsl[4];
+                        if constexpr (SPLIT) mfma_load_x_split(fsrc, fp, fox, foy, pos, N, lane, xs, xsl, slot, sstride);
+                        else mfma_load_x<T>(fsrc, fp, fox, foy, pos, N, lane, xs, slot, sstride);
 #pragma unroll
                         for (int q = 0; q < 2; ++q) {
-                            f32x16 yq = mfma_forward_half<T>(xs, q, kb, lane, cna);
+                            f32x16 yq;
+                            if constexpr (SPLIT) yq = split_forward_half(xs, xsl, q, kb, kbl, lane, cna, cnal);
+                            else yq = mfma_forward_half<T>(xs, q, kb, lane, cna);
 #pragma unroll
                             for (int k = 0; k < 16; ++k) yq[k] *= wg[q][k];
-                            vt[q] = mfma_inverse_half<T>(yq, cnb);
+                            if constexpr (SPLIT) vt[q] = split_inverse_half(yq, cnb, cnbl);
+                            else vt[q] = mfma_inverse_half<T>(yq, cnb);
                         }
                     }
                     l2 = wave_sum(l2 + l2b);
                     w = l2 <= 0.f ? 1.f : fminf(1.f / l2, a.wmax);
-                    mfma_inverse_rest<T>(vt, ka, lane, xh);
+                    if constexpr (SPLIT) split_inverse_rest(vt, ka, kal, lane, xh);
+                    else mfma_inverse_rest<T>(vt, ka, lane, xh);
                 }
                 const double ws = fix_weight(w);
                 if (c < N) {
@@ -2245,7 +2375,7 @@ size_t filter_mfma_lds_bytes(const FilterArgs &a, bool wiener)
 
 size_t filter_stack_lds_bytes(const FilterArgs &a, bool wiener, int precision)
 {
-    if (precision != 0) return filter_mfma_lds(a.FW, a.FH, wiener, a.nslots, true);
+    if (precision != 0) return filter_mfma_lds(a.FW, a.FH, wiener, a.nslots, true, precision == 3);
     return (filter_acc_floats(a.FW, a.FH) * a.nslots + (size_t)kWaves * a.G * ZS) * 4;
 }
 
@@ -2272,6 +2402,14 @@ hipError_t launch_filter_stack(const FilterArgs &a, bool wiener, int precision, 
     if (precision == 1)
         return wiener ? launch_mfma_stack_t<_Float16, true>(a, lds, grid, s)
                       : launch_mfma_stack_t<_Float16, false>(a, lds, grid, s);
+    if (precision == 3) {
+        // split products (8-wave workgroups at 2 waves per SIMD)
+        if (!a.kbl || !a.kal || !a.cnal || !a.cnbl) return hipErrorInvalidValue;
+        return wiener ? launch_tiled(filter_mfma_stack_kernel<_Float16, true, false, true, kMWaves, true>, kMThreads,
+                                     lds, a, s, nullptr, grid)
+                      : launch_tiled(filter_mfma_stack_kernel<_Float16, false, false, true, kMWaves, true>, kMThreads,
+                                     lds, a, s, nullptr, grid);
+    }
     return wiener ? launch_mfma_stack_t<__bf16, true>(a, lds, grid, s) : launch_mfma_stack_t<__bf16, false>(a, lds, grid, s);
 }
 

@@ -6,14 +6,15 @@ Its products differ from FP32 VALU sums by rounding only (about 1e-6 of the
 coefficients), so its outputs must track the fp32fast form's: the same
 matching, decisions that differ only where a coefficient lies within rounding
 of the hard threshold.  Against the FP64 oracle it is held to the fp32fast
-bars (tests/test_gpu_headline.py covers 1920x1080 RGB).
+bars (tests/test_gpu_headline.py covers 1920x1080 RGB).  The plain form's
+transforms run on the same split products with its FP64 re-decision; its
+vector-ALU kernel stays selectable and is checked beside it.
 """
 import numpy as np
 import pytest
 
 import oracle
 from isp_mw_amd import BM3D, BM3D_Basic, BM3DBatch, synth
-from isp_mw_amd import _lib as L
 from tests.gpu_common import para
 
 pytestmark = pytest.mark.gpu
@@ -89,11 +90,23 @@ def test_batch_of_rgb_frames_equals_single_calls(gpu):
         np.testing.assert_array_equal(dsts[k].cpu().numpy().astype(np.float64), want)
 
 
-def test_multi_frame_groups_are_refused(gpu):
+def test_plain_form_on_either_engine(gpu, monkeypatch):
+    """The plain form's transforms on the split matrix-core products (the
+    default) and on the FP32 vector ALUs (ISP_BM3D_FP32_VALU=1, read when a
+    context is made): both within the plain bar of the oracle, and within
+    rounding of each other (the FP64 re-decision makes the same hard-threshold
+    calls on both)."""
     import torch
-    frames = torch.from_numpy(np.stack([synth.grey_pair(64, 64, 25.0, k)[1] for k in range(3)])).to(gpu)
-    f = BM3D(para(), precision="fp32mfma")
-    with pytest.raises(L.BM3DError) as e:
-        f.process_stack(torch.empty_like(frames), frames, radius=1)
-    say(f"stack in the fp32mfma form: {e.value}")
-    assert e.value.code == L.E_UNSUPPORTED
+    from tests.gpu_common import assert_close
+    clean, n = synth.rgb_pair(96, 128, 25.0, 6)
+    p = para()
+    x = torch.from_numpy(n).to(gpu)
+    mx = run(p, x, "fp32")
+    monkeypatch.setenv("ISP_BM3D_FP32_VALU", "1")
+    va = run(p, x, "fp32")
+    want = oracle.bm3d_rgb(p.basic, p.final, n)
+    assert_close(mx, want, clean, label="plain form, split products")
+    assert_close(va, want, clean, label="plain form, FP32 vector ALUs")
+    d = float(np.abs(mx - va).max())
+    say(f"plain form: max |split - vector| {d:.3g}")
+    assert d <= 1e-5

@@ -35,7 +35,7 @@ def check(got, want, clean, label):
     assert err <= MAX_ABS and d <= MAX_DPSNR, (label, err, d)
 
 
-@pytest.mark.parametrize("prec", ["fp32", "fp16", "fp32fast"])
+@pytest.mark.parametrize("prec", ["fp32", "fp16", "fp32fast", "fp32mfma"])
 def test_one_frame_and_radius_zero_equal_single_frame_calls(gpu, prec):
     p = para(R=8)
     _, g = grey_stack(3, 48, 64, 25.0, 3)
