@@ -120,10 +120,13 @@ STATS_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles"
                           "bm3d_1080p_rgb_{prec}_kernel_stats.txt")
 # kernel-name fragments of each stage's kernel in the committed rocprofv3 stats
 STAGE_KERNELS = {
-    "fp32": {"filter_final": "filter_kernel<true, false>", "filter_basic": "filter_kernel<false, false>",
+    # the plain form's filter: the split-operand instantiations of the MFMA kernel
+    "fp32": {"filter_final": "filter_mfma_kernelIDF16_Lb1ELb0ELi8ELb1E",
+             "filter_basic": "filter_mfma_kernelIDF16_Lb0ELb0ELi8ELb1E",
              "match_basic": "match_kernel<double, double, true, true, false, false>",
              "match_final": "match_kernel<float, double, true, true, false, false>"},
-    "mixed": {"filter_final": "filter_mfma_kernelIDF16_Lb1ELb0E", "filter_basic": "filter_mfma_kernelIDF16_Lb0ELb0E",
+    "mixed": {"filter_final": "filter_mfma_kernelIDF16_Lb1ELb0ELi8ELb0E",
+              "filter_basic": "filter_mfma_kernelIDF16_Lb0ELb0ELi8ELb0E",
               "match_basic": "match_walk_kernel<double, true, false", "match_final": "match_walk_kernel<float, true, false"},
 }
 
@@ -604,7 +607,8 @@ def bench_stack(args) -> int:
     else:
         cand = float(window_len(gx, FW_, 19).sum() * window_len(gy, FH_, 19).sum()) * NF * (2 * RADIUS + 1)
         flops = cand * B * B * 3 if dom.startswith("match") else refs * 3 * 3 * transform_flops(32)
-        peak = PEAK_FP32_TFLOPS
+        # the plain form's filter: FP32-accurate products at three FP16 MFMAs each
+        peak = PEAK_FP32_TFLOPS if dom.startswith("match") else PEAK_F16_TFLOPS / 3
         achieved = flops / (ms_total * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
                 "frac": round(achieved / peak, 4), "traffic": None, "kernel": dom,
@@ -921,11 +925,15 @@ def main() -> int:
         flops = filt2_actual if dom == "filter_final" else filt1_actual
         full = filt2_full if dom == "filter_final" else refs * 3 * 2 * transform_flops(16)
         achieved = flops / (ms * 1e-3) / 1e12
-        peak = PEAK_FP32_TFLOPS if args.precision == "fp32" else PEAK_F16_TFLOPS
-        roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+        # the plain form's FP32-accurate products take three FP16 MFMAs each
+        # (split operands): its peak is a third of the dense FP16 one
+        split = args.precision == "fp32"
+        peak = PEAK_F16_TFLOPS / 3 if split else PEAK_F16_TFLOPS
+        roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": round(peak, 1), "unit": "TFLOP/s",
                 "frac": round(achieved / peak, 4), "traffic": None, "kernel": dom,
                 "ms_per_launch": round(ms, 4),
-                "engine": "fp32 VALU" if args.precision == "fp32" else f"{args.precision} MFMA 32x32x16",
+                "engine": ("fp16 MFMA 32x32x16, three per FP32-accurate product (hi/lo split operands)" if split
+                           else f"{args.precision} MFMA 32x32x16"),
                 "algorithmic_flops_per_launch": flops,
                 "flops_basis": "dense separable 2*B^2*N*(2B+N) per transform at each group's actual N",
                 # the same count with every group at its full GroupSize (the work-count basis that
@@ -934,10 +942,11 @@ def main() -> int:
                 "frac_full_n": round(full / (ms * 1e-3) / 1e12 / peak, 4),
                 "algorithmic_flops_per_launch_full_n": full}
         util = measured_util(args.precision, dom)
-        if args.precision != "fp32" and util is not None:
-            # issued matrix work: the counted 32x32x16 MFMAs of one launch
+        if util is not None and util.get("mfma_insts_per_launch"):
+            # issued matrix work: the counted 32x32x16 MFMAs of one launch, against the dense FP16 peak
             roof["issued_mfma_flops_per_launch"] = util["mfma_insts_per_launch"] * 32 * 32 * 16 * 2.0
-            roof["issued_frac"] = round(roof["issued_mfma_flops_per_launch"] / (ms * 1e-3) / 1e12 / peak, 4)
+            roof["issued_frac"] = round(roof["issued_mfma_flops_per_launch"] / (ms * 1e-3) / 1e12 / PEAK_F16_TFLOPS,
+                                        4)
     else:
         achieved = match_flops / (ms * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",

@@ -61,7 +61,8 @@ def test_committed_profiles_parse(prec, stage):
     assert t["bytes"] == t["fetch_bytes"] + t["write_bytes"]
     u = bench.measured_util(prec, stage)
     assert u is not None and 0 < u["valu_busy"] <= 1 and 0 <= u["mfma_busy"] <= 1
-    assert (u["mfma_insts_per_launch"] > 0) == (prec == "fp16" and stage.startswith("filter"))
+    # both forms filter on the matrix cores (fp32: split-FP16 products); matching is vector work
+    assert (u["mfma_insts_per_launch"] > 0) == stage.startswith("filter")
     assert u["ms_under_counters"] > 0
     ms = bench.profiled_ms(prec, stage)
     assert ms is not None and 0.05 < ms < 50
