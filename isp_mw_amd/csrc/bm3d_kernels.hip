@@ -1257,35 +1257,6 @@ __device__ __forceinline__ void mfma_load_x(const float *plane, int pitch, int o
     }
 }
 
-// mfma_load_x in high and low halves (the FP32_MFMA / plain forms' stacks)
-__device__ __forceinline__ void mfma_load_x_split(const float *plane, int pitch, int ox, int oy,
-                                                  const int *__restrict__ pos, int N, int lane, f16x8 (&xh)[4],
-                                                  f16x8 (&xl)[4], const int *__restrict__ slot = nullptr,
-                                                  size_t slotsz = 0)
-{
-    const int c = lane & 31, h = lane >> 5;
-    if (c < N) {
-        const int pz = pos[c];
-        const float *b = plane + ((pz >> 16) - oy) * pitch + (pz & 0xFFFF) - ox + (slot ? slot[c] * slotsz : 0);
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const float *r = b + (2 * s + h) * pitch;
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const float v = r[e];
-                const _Float16 hv = (_Float16)v;
-                xh[s][e] = hv;
-                xl[s][e] = (_Float16)(v - (float)hv);
-            }
-        }
-    } else {
-#pragma unroll
-        for (int s = 0; s < 4; ++s)
-#pragma unroll
-            for (int e = 0; e < 8; ++e) xh[s][e] = xl[s][e] = (_Float16)0.f;
-    }
-}
-
 // The staged footprint of the single-frame MFMA filter holds, for every
 // pixel x of a row, the 32-bit word (T(p[x]), T(p[x + 1])): a block row of 8
 // samples at any x is the words x, x + 2, x + 4, x + 6, read without
@@ -1382,14 +1353,67 @@ __device__ __forceinline__ void mfma_inverse(const f32x16 (&y)[2], const uint16_
 // significant bits together, and A B = A_hi B_hi + A_hi B_lo + A_lo B_hi in
 // FP32 accumulation, less A_lo B_lo (about 2^-22 of the product).  The small
 // terms go in first.
+// A pair of FP32 values as packed FP16 high halves h and low halves
+// l = fp16(v - hi) (the subtraction is exact).  The low half is written as
+// fma(v, one, -hi) with `one` opaque to the compiler (an empty asm emits no
+// instruction), so that it becomes a mixed-precision FMA (v_fma_mix) rather
+// than a conversion back to FP32, a subtraction and a second conversion.
+typedef _Float16 h16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void split_pair(float v0, float v1, uint32_t &h, uint32_t &l)
+{
+    float one = 1.0f;
+    asm("" : "+v"(one));
+    const h16x2 hh = __builtin_convertvector((f32x2v){v0, v1}, h16x2);
+    const h16x2 ll = {(_Float16)__builtin_fmaf(v0, one, -(float)hh[0]),
+                      (_Float16)__builtin_fmaf(v1, one, -(float)hh[1])};
+    h = __builtin_bit_cast(uint32_t, hh);
+    l = __builtin_bit_cast(uint32_t, ll);
+}
+
 __device__ __forceinline__ void split_frag(const f32x16 &acc, int m, f16x8 &hi, f16x8 &lo)
 {
+    u32x4 H, L;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        const float v = acc[8 * m + e];
-        const _Float16 h = (_Float16)v;
-        hi[e] = h;
-        lo[e] = (_Float16)(v - (float)h);
+    for (int e = 0; e < 8; e += 2) {
+        uint32_t h, l;
+        split_pair(acc[8 * m + e], acc[8 * m + e + 1], h, l);
+        H[e / 2] = h;
+        L[e / 2] = l;
+    }
+    hi = __builtin_bit_cast(f16x8, H);
+    lo = __builtin_bit_cast(f16x8, L);
+}
+
+// mfma_load_x in high and low halves (the FP32_MFMA / plain forms' stacks)
+__device__ __forceinline__ void mfma_load_x_split(const float *plane, int pitch, int ox, int oy,
+                                                  const int *__restrict__ pos, int N, int lane, f16x8 (&xh)[4],
+                                                  f16x8 (&xl)[4], const int *__restrict__ slot = nullptr,
+                                                  size_t slotsz = 0)
+{
+    const int c = lane & 31, h = lane >> 5;
+    if (c < N) {
+        const int pz = pos[c];
+        const float *b = plane + ((pz >> 16) - oy) * pitch + (pz & 0xFFFF) - ox + (slot ? slot[c] * slotsz : 0);
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const float *r = b + (2 * s + h) * pitch;
+            u32x4 H, L;
+#pragma unroll
+            for (int e = 0; e < 8; e += 2) {
+                uint32_t hh, ll;
+                split_pair(r[e], r[e + 1], hh, ll);
+                H[e / 2] = hh;
+                L[e / 2] = ll;
+            }
+            xh[s] = __builtin_bit_cast(f16x8, H);
+            xl[s] = __builtin_bit_cast(f16x8, L);
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) xh[s][e] = xl[s][e] = (_Float16)0.f;
     }
 }
 
@@ -1404,7 +1428,9 @@ __device__ __forceinline__ f32x16 mma3(const f16x8 &ah, const f16x8 &al, const f
 // the low halves of a pair of samples, as mfma_load_pairs reads them
 __device__ __forceinline__ uint32_t pack_lo_pair(float a, float b)
 {
-    return pack_pair<_Float16>(a - (float)(_Float16)a, b - (float)(_Float16)b);
+    uint32_t h, l;
+    split_pair(a, b, h, l);
+    return l;
 }
 
 // mfma_forward_half with every product split
