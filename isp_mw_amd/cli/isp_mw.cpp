@@ -69,6 +69,8 @@ struct Image {
     throw std::runtime_error(m);
 }
 
+int precision_code(const std::string &p);
+
 std::vector<unsigned char> read_file(const std::string &path)
 {
     std::ifstream f(path, std::ios::binary);
@@ -453,7 +455,10 @@ Options parse(const std::vector<std::string> &a)
         else if (s == "-MS2" || s == "--BMstep2") f.bm_step = to_int(s, a, i);
         else if (s == "-TH2" || s == "--thMSE2") f.thMSE = to_double(s, a, i), th2 = true;
         else if (s == "-D" || s == "--device") o.device = to_int(s, a, i);
-        else if (s == "--precision") o.precision = to_str(s, a, i);
+        else if (s == "--precision") {
+            o.precision = to_str(s, a, i);
+            (void)precision_code(o.precision);  // unknown names end the run here
+        }
         else if (s == "--print-params") o.print_params = true;
         else if (s == "--copy") o.copy_only = true;
         else if (!s.empty() && s[0] == '-') die("unknown option " + s);
@@ -497,7 +502,10 @@ Options parse_nlm(const std::vector<std::string> &a)
         else if (s == "-MS" || s == "--BMstep") p.bm_step = to_int(s, a, i);
         else if (s == "-TH" || s == "--thMSE") p.thMSE = to_double(s, a, i), th_def = true;
         else if (s == "-D" || s == "--device") o.device = to_int(s, a, i);
-        else if (s == "--precision") o.precision = to_str(s, a, i);
+        else if (s == "--precision") {
+            o.precision = to_str(s, a, i);
+            (void)precision_code(o.precision);  // unknown names end the run here
+        }
         else if (s == "--print-params") o.print_params = true;
         else if (s == "--copy") o.copy_only = true;
         else if (!s.empty() && s[0] == '-') die("unknown option " + s);

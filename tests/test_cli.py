@@ -55,6 +55,12 @@ def test_explicit_values_win():
     assert (f["BlockStep"], f["GroupSize"], f["BMrange"], f["BMstep"]) == (4, 24, 11, 1)
 
 
+def test_precision_names():
+    for prec in ("fp32", "fp32fast", "fp32mfma", "fp16", "bf16"):
+        assert params("--precision", prec)["precision"] == prec
+    assert run("--bm3d", "in.png", "--print-params", "--precision", "fp8", check=False).returncode != 0
+
+
 def test_output_naming_and_errors():
     p = json.loads(run("--bm3d", "dir/a.b.png", "--print-params", "-T", "_x", "-F", "ppm").stdout)
     assert p["output"] == "dir/a.b_x.ppm"
@@ -190,14 +196,15 @@ def noisy_png(tmp_path, h=48, w=64, seed=3):
 
 
 @pytest.mark.gpu
-def test_cli_equals_library_integer_frame_path(gpu, tmp_path):
+@pytest.mark.parametrize("prec", ["fp32", "fp32mfma", "fp16"])
+def test_cli_equals_library_integer_frame_path(gpu, tmp_path, prec):
     from isp_mw_amd import BM3D, BM3D_Para
     p, a = noisy_png(tmp_path)
-    out = run("--bm3d", p, "-P", "np", "-S", "25").stdout.strip()
+    out = run("--bm3d", p, "-P", "np", "-S", "25", "--precision", prec, "-T", f".{prec}").stdout.strip()
     got = np.asarray(PIL.open(out), np.int64)
     para = BM3D_Para("np").set_sigma(25.0)
     frame = np.ascontiguousarray(np.transpose(a, (2, 0, 1)).astype(np.int32) * 257)
-    res = BM3D(para)(frame, floor=0, ceil=65535)
+    res = BM3D(para, precision=prec)(frame, floor=0, ceil=65535)
     want = np.transpose((res.astype(np.int64) + 128) // 257, (1, 2, 0))
     np.testing.assert_array_equal(got, want)
 
