@@ -42,6 +42,16 @@ int isp_pilot_kpad(int cin_pad);
 int isp_pilot_conv3x3(const void *in, const void *mask, const void *weights, const float *bias, void *out,
                       float *out_f32, int n, int h, int w, int cin_pad, int cout_pad, int relu, void *stream);
 
+/* The last layer at inference, fused with the output stage: for a
+ * (cin_pad, cout_pad) = (48, 16) convolution whose first `channels` outputs
+ * are the residuals of the image's planes, est[i][c][y][x] = noisy[i][c][y][x]
+ * - (conv + bias)[c] (noisy and est FP32 [n][channels][h][w]), with the same
+ * values as isp_pilot_conv3x3 into out_f32 followed by
+ * isp_pilot_output_planes. */
+int isp_pilot_conv3x3_residual(const void *in, const void *weights, const float *bias, const float *noisy,
+                               float *est, int channels, int n, int h, int w, int cin_pad, int cout_pad,
+                               void *stream);
+
 /* Weight and bias gradients of a convolution with real channel counts
  * cin <= cin_pad, cout <= cout_pad (the same pads as isp_pilot_conv3x3), on
  * the matrix cores: dw [cout][cin][9], db [cout] (may be NULL), with

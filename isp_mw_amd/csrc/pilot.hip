@@ -172,12 +172,17 @@ __host__ __device__ constexpr int fwd_area()
     return SH * SW * (CIN + 8) > TH * TW * COUT ? SH * SW * (CIN + 8) : TH * TW * COUT;
 }
 
-template <int CIN, int COUT>
+// RESID: the network's last layer at inference.  Instead of the bf16 output
+// tile, output channels c < creal (the residual of plane c) are staged in LDS
+// as FP32 [c][TH*TW] and leave as est[n][c][y][x] = noisy[n][c][y][x] -
+// (conv + bias), planar FP32, consecutive lanes on consecutive pixels.
+template <int CIN, int COUT, bool RESID = false>
 __global__ __launch_bounds__(THREADS) void conv3x3_fwd_kernel(const uint16_t *__restrict__ in,
                                                               const uint16_t *__restrict__ wts,
                                                               const float *__restrict__ bias,
                                                               uint16_t *__restrict__ out, int H, int W, int relu,
-                                                              int N)
+                                                              int N, const float *__restrict__ noisy = nullptr,
+                                                              float *__restrict__ est = nullptr, int creal = 0)
 {
     constexpr int KP = kpad(CIN), KC = KP / 32, NT = COUT / 16;
     constexpr int SP = CIN + 8, KS = KP + 8, CQ = CIN / 8, CO8 = COUT / 8;
@@ -249,6 +254,34 @@ __global__ __launch_bounds__(THREADS) void conv3x3_fwd_kernel(const uint16_t *__
             }
         }
         __syncthreads();  // every wave is done reading the input tile
+        if constexpr (RESID) {
+            float *tres = reinterpret_cast<float *>(tin);  // [creal][TH*TW]
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                const int r = RPW * wave + (mt >> 1);
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) {
+                    const int co = 16 * nt + m;
+                    if (co >= creal) continue;
+                    const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int px = (mt & 1) * 16 + 4 * g + i;
+                        tres[co * (TH * TW) + r * TW + px] = acc[mt][nt][i] + bv;
+                    }
+                }
+            }
+            __syncthreads();
+            for (int k = tid; k < creal * TH * TW; k += THREADS) {
+                const int c = k / (TH * TW), pix = k - c * (TH * TW), r = pix / TW, px = pix - r * TW;
+                const int yy = y0 + r, xx = x0 + px;
+                if (yy < H && xx < W) {
+                    const size_t o = (((size_t)n * creal + c) * H + yy) * W + xx;
+                    est[o] = noisy[o] - tres[k];
+                }
+            }
+            continue;
+        }
         // D layout: lane holds rows 4g..4g+3 (pixels) of column lane % 16 (channel)
         uint16_t *tout = tin;  // [TH*TW][COUT]
 #pragma unroll
@@ -562,7 +595,7 @@ hipError_t conv_t(const void *in, const void *mask, const void *w, const float *
         const size_t lf = (size_t)fwd_area<CIN, COUT>() * 2 + (size_t)COUT * (KP + 8) * 2;
         if ((e = set_lds(k, lf)) != hipSuccess) return e;
         hipLaunchKernelGGL(k, dim3(grid), dim3(THREADS), lf, s, (const uint16_t *)in, (const uint16_t *)w, bias,
-                           (uint16_t *)out, h, wd, relu, n);
+                           (uint16_t *)out, h, wd, relu, n, nullptr, nullptr, 0);
         return hipGetLastError();
     }
     auto k = conv3x3_kernel<CIN, COUT>;
@@ -597,6 +630,30 @@ int wgrad_blocks(int n, int h, int w)
 using namespace pilot;
 
 extern "C" {
+
+int isp_pilot_conv3x3_residual(const void *in, const void *weights, const float *bias, const float *noisy,
+                               float *est, int channels, int n, int h, int w, int cin_pad, int cout_pad, void *stream)
+{
+    if (!in || !weights || !noisy || !est || n <= 0 || h <= 0 || w <= 0 || channels < 1 || channels > 16)
+        return ISP_PILOT_E_ARG;
+    if (cin_pad != 48 || cout_pad != 16) return ISP_PILOT_E_UNSUPPORTED;
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e;
+    int dev = 0, cus = 0;
+    if ((e = hipGetDevice(&dev)) != hipSuccess || (e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                                                                            dev)) != hipSuccess)
+        return ISP_PILOT_E_HIP;
+    constexpr int KP = kpad(48);
+    const long tiles = (long)((w + TW - 1) / TW) * ((h + TH - 1) / TH) * n;
+    const int grid = (int)std::min<long>(tiles, 2L * std::max(cus, 1));
+    auto k = conv3x3_fwd_kernel<48, 16, true>;
+    const size_t lf = (size_t)fwd_area<48, 16>() * 2 + (size_t)16 * (KP + 8) * 2;
+    static_assert(fwd_area<48, 16>() * 2 >= 16 * TH * TW * 4, "the FP32 residual tile fits the shared area");
+    if ((e = set_lds(k, lf)) != hipSuccess) return ISP_PILOT_E_HIP;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(THREADS), lf, s, (const uint16_t *)in, (const uint16_t *)weights, bias,
+                       nullptr, h, w, 0, n, noisy, est, channels);
+    return hipGetLastError() == hipSuccess ? ISP_PILOT_OK : ISP_PILOT_E_HIP;
+}
 
 int isp_pilot_conv3x3(const void *in, const void *mask, const void *weights, const float *bias, void *out,
                       float *out_f32, int n, int h, int w, int cin_pad, int cout_pad, int relu, void *stream)

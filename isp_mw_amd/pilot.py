@@ -58,6 +58,7 @@ def _plib():
         lib.isp_pilot_output.argtypes = [vp, vp, vp, vp, vp, vp, i, i, i, vp]
         lib.isp_pilot_input_planes.argtypes = [vp, i, vp, vp, vp, i, i, i, vp]
         lib.isp_pilot_output_planes.argtypes = [vp, vp, vp, vp, vp, vp, i, i, i, i, vp]
+        lib.isp_pilot_conv3x3_residual.argtypes = [vp, vp, vp, vp, vp, i, i, i, i, i, i, vp]
         lib._pilot_typed = True
     return lib
 
@@ -152,11 +153,13 @@ class PilotCNN:
             raise ValueError(f"expected {self.channels} planes per image, got {C}")
         return N, H, W
 
-    def forward(self, noisy, sigma, keep: bool = True, offset=None):
+    def forward(self, noisy, sigma, keep: bool = True, offset=None, est=None):
         """noisy [N,H,W] (grey) or [N,3,H,W] (OPP frames) fp32 device, sigma [N]
         fp32 device (0..255), offset one fp32 value per plane ([N] or [N,3]) on
         the device, or None.  Returns (acts, res32): acts[0] the input,
-        acts[l+1] layer l's output."""
+        acts[l+1] layer l's output.  With est (a tensor shaped like noisy),
+        the last layer writes the estimate noisy - residual into it directly
+        (isp_pilot_conv3x3_residual) and res32 is None."""
         torch = self.torch
         lib = _plib()
         s = self._stream()
@@ -166,10 +169,15 @@ class PilotCNN:
         _check(lib.isp_pilot_input_planes(_ptr(noisy), self.channels, _ptr(sigma), _ptr(offset), _ptr(x), N, H, W,
                                           s), "input")
         acts = [x]
-        res32 = self._buf("res32", P * 16, torch.float32)
+        res32 = None if est is not None else self._buf("res32", P * 16, torch.float32)
         cur = x
         for l, (cin, cout, cinp, coutp, relu) in enumerate(self.layers):
             last = l == len(self.layers) - 1
+            if last and est is not None:
+                _check(lib.isp_pilot_conv3x3_residual(_ptr(cur), _ptr(self.wf[l]), _ptr(self.bias_padded(l)),
+                                                      _ptr(noisy), _ptr(est), self.channels, N, H, W, cinp, coutp,
+                                                      s), "residual")
+                return acts, None
             name = f"act{l + 1}" if keep else f"ping{l % 2}"
             out = self._buf(name if not last else "res16", P * coutp, torch.int16)
             _check(lib.isp_pilot_conv3x3(_ptr(cur), None, _ptr(self.wf[l]), _ptr(self.bias_padded(l)), _ptr(out),
@@ -230,10 +238,9 @@ class PilotCNN:
             o = np.broadcast_to(o.reshape(-1), (N * C,)) if o.size == N * C or o.size == 1 else \
                 np.broadcast_to(o.reshape(1, C), (N, C)).reshape(-1)
             off = torch.tensor(np.ascontiguousarray(o), device=self.dev)
-        _, res32 = self.forward(x.contiguous(), sg, keep=False, offset=off)
+        x = x.contiguous()
         out = torch.empty_like(x)
-        _check(_plib().isp_pilot_output_planes(_ptr(x), _ptr(res32), _ptr(out), None, None, None, C, N, H, W,
-                                               self._stream()), "output")
+        self.forward(x, sg, keep=False, offset=off, est=out)
         return out[0] if one else out
 
     def state(self) -> dict:

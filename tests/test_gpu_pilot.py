@@ -460,3 +460,26 @@ def test_pilot_guides_an_rgb_frame(P, rgb_net):
     say(f"1080p RGB: noisy {pn:.2f} dB, two-step {pp:.3f} dB, pilot + Wiener {po_:.3f} dB ({po_ - pp:+.4f})")
     assert po_ >= pp - RGB_ENVELOPE_DB
     assert po_ > pn + 8.0
+
+
+@pytest.mark.parametrize("channels", [1, 3])
+def test_fused_residual_layer_equals_conv_then_output(P, channels):
+    """Inference's last layer fused with the output stage
+    (isp_pilot_conv3x3_residual) against the training path's convolution with
+    an FP32 copy followed by isp_pilot_output_planes: the same values, bit for bit."""
+    import torch
+    rng = np.random.default_rng(40 + channels)
+    N, H, W = 2, 37, 70
+    net = P.PilotCNN(seed=5, channels=channels)
+    shape = (N, H, W) if channels == 1 else (N, channels, H, W)
+    noisy = torch.from_numpy(rng.uniform(0, 1, shape).astype(np.float32)).cuda()
+    sigma = torch.full((N,), 20.0, device="cuda")
+    _, res32 = net.forward(noisy, sigma, keep=False)
+    want = torch.empty_like(noisy)
+    lib = P._plib()
+    assert lib.isp_pilot_output_planes(P._ptr(noisy), P._ptr(res32), P._ptr(want), None, None, None, channels, N, H, W,
+                                       None) == 0
+    got = torch.empty_like(noisy)
+    net.forward(noisy, sigma, keep=False, est=got)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(got.cpu().numpy(), want.cpu().numpy())
