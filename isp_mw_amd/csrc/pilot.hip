@@ -28,7 +28,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // output tile of a convolution workgroup: 8 x 32 pixels keeps the 48-channel
-// kernel's LDS (81,856 bytes with padding) within half of the 160 KB of a CU,
+// kernel's LDS (77,184 bytes) within half of the 160 KB of a CU,
 // so two workgroups overlap their staging with each other's products
 constexpr int TH = 8, TW = 32;
 constexpr int RPW = TH / 4;              // rows per wave
@@ -43,6 +43,15 @@ constexpr int WZP = WPIX + 8;       // pitch of one channel of the transposed ou
 constexpr int WBLOCKS = 512;        // persistent grid of the weight gradient (two workgroups per CU)
 
 __host__ __device__ constexpr int kpad(int cin) { return (9 * cin + 31) / 32 * 32; }
+// LDS row pitches (elements) of the staged input tile (one row per pixel) and
+// of the weights (one row per output channel).  A 16-byte LDS read is served
+// per half-wave (32 lanes) from a 512-byte bank window, so the 16 pixels x 2
+// K-groups of 8 channels that half a wave reads are conflict-free when a row
+// is an odd multiple of 32 bytes: the input rows are unpadded (CIN = 16 or
+// 48), the weight rows KP + 16 (KP is a multiple of 32).  The staging writes
+// of the input tile are then contiguous.
+__host__ __device__ constexpr int in_pitch(int cin) { return cin; }
+__host__ __device__ constexpr int w_pitch(int cin) { return kpad(cin) + 16; }
 
 __device__ __forceinline__ uint16_t f2bf(float f)
 {
@@ -68,9 +77,7 @@ __device__ __forceinline__ uint4 apply_mask(uint4 v, uint4 m)
 
 // One output tile of TH x TW pixels and all COUT channels per workgroup; wave
 // w computes rows RPW*w .. RPW*w+RPW-1 (MT M-tiles of 16 pixels) against
-// COUT/16 N-tiles.
-// LDS rows are padded by 8 elements so that the 16-byte operand reads of 8
-// neighbouring lanes fall in distinct banks.
+// COUT/16 N-tiles.  LDS row pitches: in_pitch, w_pitch.
 template <int CIN, int COUT>
 __global__ __launch_bounds__(THREADS) void conv3x3_kernel(const uint16_t *__restrict__ in,
                                                           const uint16_t *__restrict__ mask,
@@ -79,7 +86,7 @@ __global__ __launch_bounds__(THREADS) void conv3x3_kernel(const uint16_t *__rest
                                                           float *__restrict__ out32, int H, int W, int relu, int N)
 {
     constexpr int KP = kpad(CIN), KC = KP / 32, NT = COUT / 16;
-    constexpr int SP = CIN + 8, KS = KP + 8, CQ = CIN / 8;
+    constexpr int SP = in_pitch(CIN), KS = w_pitch(CIN), CQ = CIN / 8;
     extern __shared__ __align__(16) unsigned char smem[];
     uint16_t *tin = reinterpret_cast<uint16_t *>(smem);  // [SH*SW][SP]
     uint16_t *tw = tin + SH * SW * SP;                    // [COUT][KS]
@@ -162,14 +169,29 @@ __global__ __launch_bounds__(THREADS) void conv3x3_kernel(const uint16_t *__rest
 }
 
 // Inference form of conv3x3_kernel (no ReLU mask, no FP32 copy of the
-// outputs): the next tile's input is loaded into registers while the current
-// tile's products run, and the outputs pass through LDS (bias, ReLU, bf16) so
-// that whole pixel rows go out as 16-byte stores.  The shared area holds the
-// staged input tile, then the output tile: max(SH*SW*SP, TH*TW*COUT) elements.
+// outputs).  The products run as D[co][pixel] = W X, the weights being the A
+// operand, so that a lane's four accumulator values are four consecutive
+// output channels of one pixel: after bias, ReLU and the bf16 rounding (one
+// v_cvt_pk_bf16_f32 per pair) they leave as one 8-byte store, and the three
+// stores of a pixel's 48 channels fill its 96 bytes.  The operand fragments
+// of K chunk kc + 1 are read from LDS while chunk kc's products run, and the
+// next tile's input is loaded into registers during the whole tile.  Every
+// thread's tile-relative offsets are computed once; tiles clear of the image
+// border use them without bounds tests.  The shared area holds the staged
+// input tile (with RESID, then the FP32 residual tile):
+// max(SH*SW*SP, TH*TW*16 FP32 as bf16 elements).
 template <int CIN, int COUT>
 __host__ __device__ constexpr int fwd_area()
 {
-    return SH * SW * (CIN + 8) > TH * TW * COUT ? SH * SW * (CIN + 8) : TH * TW * COUT;
+    return SH * SW * in_pitch(CIN) > TH * TW * COUT ? SH * SW * in_pitch(CIN) : TH * TW * COUT;
+}
+
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t pack_bf16(float a, float b)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, bf16x2));  // round to nearest even
 }
 
 // RESID: the network's last layer at inference.  Instead of the bf16 output
@@ -177,25 +199,51 @@ __host__ __device__ constexpr int fwd_area()
 // as FP32 [c][TH*TW] and leave as est[n][c][y][x] = noisy[n][c][y][x] -
 // (conv + bias), planar FP32, consecutive lanes on consecutive pixels.
 template <int CIN, int COUT, bool RESID = false>
-__global__ __launch_bounds__(THREADS) void conv3x3_fwd_kernel(const uint16_t *__restrict__ in,
-                                                              const uint16_t *__restrict__ wts,
-                                                              const float *__restrict__ bias,
-                                                              uint16_t *__restrict__ out, int H, int W, int relu,
-                                                              int N, const float *__restrict__ noisy = nullptr,
-                                                              float *__restrict__ est = nullptr, int creal = 0)
+__global__ __launch_bounds__(THREADS, 2) void conv3x3_fwd_kernel(const uint16_t *__restrict__ in,
+                                                                 const uint16_t *__restrict__ wts,
+                                                                 const float *__restrict__ bias,
+                                                                 uint16_t *__restrict__ out, int H, int W, int relu,
+                                                                 int N, const float *__restrict__ noisy = nullptr,
+                                                                 float *__restrict__ est = nullptr, int creal = 0)
 {
     constexpr int KP = kpad(CIN), KC = KP / 32, NT = COUT / 16;
-    constexpr int SP = CIN + 8, KS = KP + 8, CQ = CIN / 8, CO8 = COUT / 8;
+    constexpr int SP = in_pitch(CIN), KS = w_pitch(CIN), CQ = CIN / 8;
     constexpr int NIN = SH * SW * CQ, NPF = (NIN + THREADS - 1) / THREADS;
     extern __shared__ __align__(16) unsigned char smem[];
-    uint16_t *tin = reinterpret_cast<uint16_t *>(smem);  // [SH*SW][SP], then the outputs [TH*TW][COUT]
+    uint16_t *tin = reinterpret_cast<uint16_t *>(smem);  // [SH*SW][SP]
     uint16_t *tw = tin + fwd_area<CIN, COUT>();          // [COUT][KS]
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m = lane & 15, g = lane >> 4;
     for (int k = tid; k < COUT * (KP / 8); k += THREADS) {
         const int row = k / (KP / 8), q = k - row * (KP / 8);
         *reinterpret_cast<uint4 *>(tw + (size_t)row * KS + 8 * q) =
             *reinterpret_cast<const uint4 *>(wts + (size_t)row * KP + 8 * q);
     }
+    // this lane's output channels 16 nt + 4 g + i
+    float bv[NT][4];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) bv[nt][i] = bias ? bias[16 * nt + 4 * g + i] : 0.f;
+    // the 16-byte input pieces of this thread: k = tid + i THREADS is piece
+    // k % CQ of halo pixel k / CQ at (sy, sx); ioff is its offset from the
+    // tile's top-left halo pixel, ipos packs sy << 8 | sx of pieces 2j, 2j + 1
+    // in its low and high halves
+    int ioff[NPF], ipos[(NPF + 1) / 2];
+#pragma unroll
+    for (int i = 0; i < NPF; ++i) {
+        const int k = tid + i * THREADS, p = k / CQ, sy = p / SW, sx = p - sy * SW;
+        ioff[i] = (sy * W + sx) * CIN + 8 * (k - p * CQ);
+        if (i & 1) ipos[i >> 1] |= (sy << 8 | sx) << 16;
+        else ipos[i >> 1] = sy << 8 | sx;
+    }
+    // the last round's pieces beyond the tile are neither loaded nor staged
+    const bool last_in = NIN % THREADS == 0 || tid + (NPF - 1) * THREADS < NIN;
+    // output pixel of M-tile mt: row RPW wave + mt / 2, column 16 (mt % 2) + m;
+    // ooff is its offset (elements) from the tile's top-left pixel, channel 4 g
+    int ooff[MT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) ooff[mt] = ((RPW * wave + (mt >> 1)) * W + (mt & 1) * 16 + m) * COUT + 4 * g;
     const int ntx = (W + TW - 1) / TW, nty = (H + TH - 1) / TH, per = ntx * nty, total = per * N;
 
     // the input tile (with halo, zeros outside the image) of tile t into registers
@@ -203,29 +251,50 @@ __global__ __launch_bounds__(THREADS) void conv3x3_fwd_kernel(const uint16_t *__
     auto fetch = [&](int t) {
         const int n = t / per, tr = t - n * per, tyi = tr / ntx;
         const int x0 = (tr - tyi * ntx) * TW, y0 = tyi * TH;
+        // element index of the top-left halo pixel (negative on the first row or column)
+        const long long base = (((long long)n * H + y0 - 1) * W + x0 - 1) * CIN;
+        const bool inner = y0 >= 1 && y0 + SH - 1 <= H && x0 >= 1 && x0 + SW - 1 <= W;
 #pragma unroll
         for (int i = 0; i < NPF; ++i) {
-            const int k = tid + i * THREADS;
-            const int q = k % CQ, p = k / CQ, sx = p % SW, sy = p / SW;
-            const int yy = y0 + sy - 1, xx = x0 + sx - 1;
-            uint4 v = make_uint4(0, 0, 0, 0);
-            if (k < NIN && yy >= 0 && yy < H && xx >= 0 && xx < W)
-                v = *reinterpret_cast<const uint4 *>(in + (((size_t)n * H + yy) * W + xx) * CIN + 8 * q);
-            pf[i] = v;
+            bool ok = i < NPF - 1 || last_in;
+            if (!inner) {
+                const int pos = (ipos[i >> 1] >> (16 * (i & 1))) & 0xFFFF;
+                const int yy = y0 + (pos >> 8) - 1, xx = x0 + (pos & 255) - 1;
+                ok = ok && yy >= 0 && yy < H && xx >= 0 && xx < W;
+            }
+            pf[i] = ok ? *reinterpret_cast<const uint4 *>(in + (base + ioff[i])) : make_uint4(0, 0, 0, 0);
         }
     };
     if ((int)blockIdx.x < total) fetch(blockIdx.x);
 
-    const int m = lane & 15, g = lane >> 4;
+    // operand fragments of K chunk kc: weights (A, rows 16 nt + m) and input
+    // pixels (B, columns m of the M-tiles), K elements 8 g .. 8 g + 7
+    auto frags = [&](int kc, bf16x8 *fw, bf16x8 *fx) {
+        const int k = kc * 32 + 8 * g;
+        int tap = k / CIN;
+        const int ci0 = k - tap * CIN;  // in [0, CIN): the K padding reads tap 8, whose weights are 0
+        if (tap > 8) tap = 8;
+        const int ty = tap / 3, tx = tap - 3 * ty;
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+            fw[nt] = *reinterpret_cast<const bf16x8 *>(tw + (size_t)(16 * nt + m) * KS + k);
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            const int r = RPW * wave + (mt >> 1), xs = (mt & 1) * 16 + m;
+            fx[mt] = *reinterpret_cast<const bf16x8 *>(tin + ((size_t)(r + ty) * SW + xs + tx) * SP + ci0);
+        }
+    };
+
     for (int t = blockIdx.x; t < total; t += gridDim.x) {
         const int n = t / per, tr = t - n * per, tyi = tr / ntx;
         const int x0 = (tr - tyi * ntx) * TW, y0 = tyi * TH;
-        __syncthreads();  // the previous tile's output rows are out of the shared area
+        __syncthreads();  // every wave is done with the previous tile's shared area
 #pragma unroll
-        for (int i = 0; i < NPF; ++i) {
-            const int k = tid + i * THREADS;
-            if (k < NIN) *reinterpret_cast<uint4 *>(tin + (size_t)(k / CQ) * SP + 8 * (k % CQ)) = pf[i];
-        }
+        for (int i = 0; i < NPF; ++i)
+            if (i < NPF - 1 || last_in) {
+                const int k = tid + i * THREADS;
+                *reinterpret_cast<uint4 *>(tin + (size_t)(k / CQ) * SP + 8 * (k % CQ)) = pf[i];
+            }
         __syncthreads();
         if (t + (int)gridDim.x < total) fetch(t + gridDim.x);  // in flight during the products
 
@@ -234,42 +303,36 @@ __global__ __launch_bounds__(THREADS) void conv3x3_fwd_kernel(const uint16_t *__
         for (int a = 0; a < MT; ++a)
 #pragma unroll
             for (int b = 0; b < NT; ++b) acc[a][b] = (f32x4){0.f, 0.f, 0.f, 0.f};
+        bf16x8 fw[2][NT], fx[2][MT];
+        frags(0, fw[0], fx[0]);
+#pragma unroll
         for (int kc = 0; kc < KC; ++kc) {
-            const int k = kc * 32 + 8 * g;
-            int tap = k / CIN;
-            const int ci0 = k - tap * CIN;  // in [0, CIN): the K padding reads tap 8, whose weights are 0
-            if (tap > 8) tap = 8;
-            const int ty = tap / 3, tx = tap - 3 * ty;
-            bf16x8 b[NT];
+            if (kc + 1 < KC) frags(kc + 1, fw[(kc + 1) & 1], fx[(kc + 1) & 1]);
+            // the reads stay ahead of this chunk's products (the scheduler
+            // would otherwise sink them next to their use)
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt)
-                b[nt] = *reinterpret_cast<const bf16x8 *>(tw + (size_t)(16 * nt + m) * KS + k);
-#pragma unroll
-            for (int mt = 0; mt < MT; ++mt) {
-                const int r = RPW * wave + (mt >> 1), xs = (mt & 1) * 16 + m;
-                const bf16x8 a = *reinterpret_cast<const bf16x8 *>(tin + ((size_t)(r + ty) * SW + xs + tx) * SP + ci0);
+            for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt)
-                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[nt], acc[mt][nt], 0, 0, 0);
-            }
+                    acc[mt][nt] =
+                        __builtin_amdgcn_mfma_f32_16x16x32_bf16(fw[kc & 1][nt], fx[kc & 1][mt], acc[mt][nt], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
         }
-        __syncthreads();  // every wave is done reading the input tile
+        // D layout: lane holds rows 4g..4g+3 (channels) of column lane % 16 (pixel)
         if constexpr (RESID) {
+            __syncthreads();  // every wave is done reading the input tile
             float *tres = reinterpret_cast<float *>(tin);  // [creal][TH*TW]
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
-                const int r = RPW * wave + (mt >> 1);
+                const int pix = (RPW * wave + (mt >> 1)) * TW + (mt & 1) * 16 + m;
 #pragma unroll
-                for (int nt = 0; nt < NT; ++nt) {
-                    const int co = 16 * nt + m;
-                    if (co >= creal) continue;
-                    const float bv = bias ? bias[co] : 0.f;
+                for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
-                        const int px = (mt & 1) * 16 + 4 * g + i;
-                        tres[co * (TH * TW) + r * TW + px] = acc[mt][nt][i] + bv;
+                        const int co = 16 * nt + 4 * g + i;
+                        if (co < creal) tres[co * (TH * TW) + pix] = acc[mt][nt][i] + bv[nt][i];
                     }
-                }
             }
             __syncthreads();
             for (int k = tid; k < creal * TH * TW; k += THREADS) {
@@ -282,31 +345,22 @@ __global__ __launch_bounds__(THREADS) void conv3x3_fwd_kernel(const uint16_t *__
             }
             continue;
         }
-        // D layout: lane holds rows 4g..4g+3 (pixels) of column lane % 16 (channel)
-        uint16_t *tout = tin;  // [TH*TW][COUT]
+        uint16_t *dst = out + (((size_t)n * H + y0) * W + x0) * COUT;
+        const bool whole = y0 + TH <= H && x0 + TW <= W;
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) {
-            const int r = RPW * wave + (mt >> 1);
+            if (!whole && (y0 + RPW * wave + (mt >> 1) >= H || x0 + (mt & 1) * 16 + m >= W)) continue;
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) {
-                const int co = 16 * nt + m;
-                const float bv = bias ? bias[co] : 0.f;
+                float v[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    const int px = (mt & 1) * 16 + 4 * g + i;
-                    float v = acc[mt][nt][i] + bv;
-                    if (relu) v = fmaxf(v, 0.f);
-                    tout[(size_t)(r * TW + px) * COUT + co] = f2bf(v);
+                    v[i] = acc[mt][nt][i] + bv[nt][i];
+                    if (relu) v[i] = fmaxf(v[i], 0.f);
                 }
+                *reinterpret_cast<uint2 *>(dst + ooff[mt] + 16 * nt) =
+                    make_uint2(pack_bf16(v[0], v[1]), pack_bf16(v[2], v[3]));
             }
-        }
-        __syncthreads();
-        for (int k = tid; k < TH * TW * CO8; k += THREADS) {
-            const int pix = k / CO8, q = k - pix * CO8, r = pix / TW, px = pix - r * TW;
-            const int yy = y0 + r, xx = x0 + px;
-            if (yy < H && xx < W)
-                *reinterpret_cast<uint4 *>(out + (((size_t)n * H + yy) * W + xx) * COUT + 8 * q) =
-                    *reinterpret_cast<const uint4 *>(tout + (size_t)pix * COUT + 8 * q);
         }
     }
 }
@@ -581,8 +635,7 @@ template <int CIN, int COUT>
 hipError_t conv_t(const void *in, const void *mask, const void *w, const float *bias, void *out, float *out32, int n,
                   int h, int wd, int relu, hipStream_t s)
 {
-    constexpr int KP = kpad(CIN);
-    const size_t lds = (size_t)SH * SW * (CIN + 8) * 2 + (size_t)COUT * (KP + 8) * 2;
+    const size_t lds = (size_t)SH * SW * in_pitch(CIN) * 2 + (size_t)COUT * w_pitch(CIN) * 2;
     // two persistent workgroups per CU (the LDS of one is within half a CU's)
     hipError_t e;
     int dev = 0, cus = 0;
@@ -592,7 +645,7 @@ hipError_t conv_t(const void *in, const void *mask, const void *w, const float *
     const int grid = (int)std::min<long>(tiles, 2L * std::max(cus, 1));
     if (!mask && !out32) {
         auto k = conv3x3_fwd_kernel<CIN, COUT>;
-        const size_t lf = (size_t)fwd_area<CIN, COUT>() * 2 + (size_t)COUT * (KP + 8) * 2;
+        const size_t lf = (size_t)fwd_area<CIN, COUT>() * 2 + (size_t)COUT * w_pitch(CIN) * 2;
         if ((e = set_lds(k, lf)) != hipSuccess) return e;
         hipLaunchKernelGGL(k, dim3(grid), dim3(THREADS), lf, s, (const uint16_t *)in, (const uint16_t *)w, bias,
                            (uint16_t *)out, h, wd, relu, n, nullptr, nullptr, 0);
@@ -643,11 +696,10 @@ int isp_pilot_conv3x3_residual(const void *in, const void *weights, const float 
     if ((e = hipGetDevice(&dev)) != hipSuccess || (e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
                                                                             dev)) != hipSuccess)
         return ISP_PILOT_E_HIP;
-    constexpr int KP = kpad(48);
     const long tiles = (long)((w + TW - 1) / TW) * ((h + TH - 1) / TH) * n;
     const int grid = (int)std::min<long>(tiles, 2L * std::max(cus, 1));
     auto k = conv3x3_fwd_kernel<48, 16, true>;
-    const size_t lf = (size_t)fwd_area<48, 16>() * 2 + (size_t)16 * (KP + 8) * 2;
+    const size_t lf = (size_t)fwd_area<48, 16>() * 2 + (size_t)16 * w_pitch(48) * 2;
     static_assert(fwd_area<48, 16>() * 2 >= 16 * TH * TW * 4, "the FP32 residual tile fits the shared area");
     if ((e = set_lds(k, lf)) != hipSuccess) return ISP_PILOT_E_HIP;
     hipLaunchKernelGGL(k, dim3(grid), dim3(THREADS), lf, s, (const uint16_t *)in, (const uint16_t *)weights, bias,

@@ -90,6 +90,44 @@ def test_conv_forward_vs_oracle(P, cinp, coutp, cin, cout):
         assert np.all(got16[..., cout:] == 0)
 
 
+@pytest.mark.parametrize("cinp,coutp,cin,cout", CONFIGS[:3])
+def test_inference_conv_vs_oracle(P, cinp, coutp, cin, cout):
+    """The inference convolution (no FP32 copy: the form every layer but the
+    last of a forward pass uses) at a size with tiles clear of the border
+    (their loads and stores skip the bounds tests) and partial tiles on both
+    axes: within one BF16 rounding of the oracle, and bit for bit the BF16
+    outputs of the training form."""
+    import torch
+    rng = np.random.default_rng(7 * cinp + coutp)
+    N, H, W = 2, 53, 141
+    x = po.bf16_round(rng.uniform(-1, 1, (N, H, W, cin)))
+    w = rng.standard_normal((cout, cin, 9)) * np.sqrt(2.0 / (cin * 9))
+    b = rng.standard_normal(cout) * 0.1
+    wf, _ = packed(P, torch, w, cin, cout, cinp, coutp)
+    xt = dev(bits(pad_ch(x, cinp)), torch)
+    bt = dev(pad_ch(b[None], coutp)[0].astype(np.float32), torch)
+    out = torch.empty(N * H * W * coutp, dtype=torch.int16, device="cuda")
+    train = torch.empty_like(out)
+    out32 = torch.empty(N * H * W * coutp, dtype=torch.float32, device="cuda")
+    lib = P._plib()
+    for relu in (0, 1):
+        out.fill_(-1)
+        assert lib.isp_pilot_conv3x3(P._ptr(xt), None, P._ptr(wf), P._ptr(bt), P._ptr(out), None, N, H, W,
+                                     cinp, coutp, relu, None) == 0
+        assert lib.isp_pilot_conv3x3(P._ptr(xt), None, P._ptr(wf), P._ptr(bt), P._ptr(train), P._ptr(out32), N, H,
+                                     W, cinp, coutp, relu, None) == 0
+        torch.cuda.synchronize()
+        ref = po.conv3x3(x, po.bf16_round(w), b)
+        if relu:
+            ref = np.maximum(ref, 0)
+        got = unbits(out).reshape(N, H, W, coutp)
+        err = np.abs(got[..., :cout] - ref).max()
+        say(f"inference conv {cinp}->{coutp} relu {relu}: max |bf16 - oracle| = {err:.3g}")
+        np.testing.assert_allclose(got[..., :cout], ref, rtol=2.0 ** -8, atol=1e-6)
+        assert np.all(got[..., cout:] == 0)
+        np.testing.assert_array_equal(out.cpu().numpy(), train.cpu().numpy())
+
+
 def test_conv_masked_input_gradient(P):
     import torch
     rng = np.random.default_rng(5)
