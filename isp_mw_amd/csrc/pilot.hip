@@ -38,8 +38,12 @@ constexpr int THREADS = 256;             // 4 waves x RPW rows
 // pixel tile of the weight-gradient kernel: 4 rows of 32 (one 32-pixel row per
 // MFMA K chunk), staged as 6 rows with the halo
 constexpr int WTH = 4, WTW = 32, WPIX = WTH * WTW, WSR = WTH + 2;
-constexpr int WCP = WSR * WTW + 8;  // pitch of one channel of a shifted input copy (elements)
-constexpr int WZP = WPIX + 8;       // pitch of one channel of the transposed output gradient
+// channel pitches (elements) of the shifted input copies and of the transposed
+// output gradient: odd multiples of 32 bytes (208 and 144 elements), so the
+// 16-byte operand reads of half a wave (16 channels x 2 pixel groups) are
+// conflict-free (see in_pitch)
+constexpr int WCP = WSR * WTW + 16;
+constexpr int WZP = WPIX + 16;
 constexpr int WBLOCKS = 512;        // persistent grid of the weight gradient (two workgroups per CU)
 
 __host__ __device__ constexpr int kpad(int cin) { return (9 * cin + 31) / 32 * 32; }
