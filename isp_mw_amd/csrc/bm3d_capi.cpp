@@ -521,8 +521,9 @@ isp_bm3d_ctx::Tiles tiles_for(const isp_bm3d_ctx *ctx, int W, int H)
 // STRIP (mod 64) that is st * STRIP * s (mod 64): distinct over a read's 64
 // lanes where st * STRIP is odd, so the reads are conflict-free over LDS's 64
 // four-byte banks (for FP64 elements, over each half-wave's 32 pairs).
-// Measured on the column walk's 1080p tile, pitch 59 against 89: 20.2
-// against 4.5 clocks per CU per 64-lane pair read.  STRIP and nsx follow the
+// Measured with the column walk's lanes at BMrange 12 (nsx = 5): 4.5 clocks
+// per CU per 64-lane pair read at pitches 89 and 153, 8.5 at 61 and 63, 20.2
+// at 59 and 91.  STRIP and nsx follow the
 // kernel the step takes (the column walk: strips of 6 where one round of 64
 // covers the window, else 5; the list kernel: kStrip), for a block whose
 // window is not clipped by the image border.  The odd pitch tileW | 1 is
@@ -3115,6 +3116,7 @@ int stack_step(isp_bm3d_ctx *ctx, const isp_bm3d_step_params &p, bool wiener, co
                 if (z != t) {
                     a.search = gz;
                     a.walk = 0;
+                    a.tilePitch = match_pitch(a, d, p);
                     a.refW = (tl.match_x - 1) * p.block_step + kBlock;
                     a.refH = (tl.match_y - 1) * p.block_step + kBlock;
                     a.refPitch = a.refW | 1;
