@@ -515,32 +515,55 @@ isp_bm3d_ctx::Tiles tiles_for(const isp_bm3d_ctx *ctx, int W, int H)
 
 // Arguments of one matching launch on one image (guide: FP32 or FP64 plane).
 // Row pitch (elements) of matching's staged tile.  Lane s of a read (strip s
-// of nsx strips of STRIP offsets per lattice row of the window) reads the
-// tile at element (s / nsx) * st * pitch + (s % nsx) * STRIP * st + const
-// (consecutive elements of its strip, a pair at a time).  With pitch = nsx *
-// STRIP (mod 64) that is st * STRIP * s (mod 64): distinct over a read's 64
-// lanes where st * STRIP is odd, so the reads are conflict-free over LDS's 64
-// four-byte banks (for FP64 elements, over each half-wave's 32 pairs).
-// Measured with the column walk's lanes at BMrange 12 (nsx = 5): 4.5 clocks
-// per CU per 64-lane pair read at pitches 89 and 153, 8.5 at 61 and 63, 20.2
-// at 59 and 91.  STRIP and nsx follow the
-// kernel the step takes (the column walk: strips of 6 where one round of 64
-// covers the window, else 5; the list kernel: kStrip), for a block whose
-// window is not clipped by the image border.  The odd pitch tileW | 1 is
-// kept where the rule would lower the workgroups a CU holds.
+// of the nsx strips of STRIP offsets per lattice row of the window) reads
+// consecutive elements of the tile from element (s / nsx) * st * pitch +
+// (s % nsx) * STRIP * st + const.  Bank model, fitted to a probe of these
+// reads at every pitch from 41 to 167 (8 waves per CU; correlation 0.999 for
+// the column walk's lanes at BMrange 12, 0.996 for NL-means' strips of 6 at
+// BMstep 3, 0.97 for the FP64 list kernel): 4-byte elements are read by
+// groups of 32 lanes over 32 banks, 8-byte elements by groups of 16 lanes,
+// and a group costs its largest number of lanes on one bank.  The pitch is
+// the one of tileW .. tileW + 31 with the smallest model cost over the
+// window's strips (tileW | 1 unless another is strictly cheaper); the model's
+// choices are the measured fastest in all three probed cases (column walk:
+// 2.55 clocks per CU per element read at pitch 57 or 89, 4.41 at 47, 10.1 at
+// 59).  STRIP follows the kernel the step takes (the column walk: strips of 6
+// where one round of 64 covers the window, else 5; the list kernel: kStrip),
+// nsx a block whose window the image border does not clip.  The odd pitch
+// tileW | 1 is kept where the chosen one would lower the workgroups a CU holds.
 int match_pitch(const MatchArgs &a, const Derived &d, const isp_bm3d_step_params &p)
 {
-    const int NM = 2 * d.range / p.bm_step + 1, nm = 2 * (d.range / p.bm_step) + 1;
+    const int st = p.bm_step, NM = 2 * d.range / st + 1, nm = 2 * (d.range / st) + 1;
     int strip = kStrip;
     if (a.walk) {
         const bool multi = nm * ((nm + 4) / 5) > 128, one = nm * ((nm + 5) / 6) <= 64;
         strip = !multi && one ? 6 : 5;
     }
-    const int nsx = (NM + strip - 1) / strip, want = nsx * strip % 64;
+    const int nsx = (NM + strip - 1) / strip, nstrips = std::min(NM * nsx, 256);
+    const int words = !a.walk && a.exact ? 2 : 1, G = 32 / words;  // FP64 tiles in the exact list kernel
+    auto cost = [&](int P) {
+        int t = 0;
+        for (int g0 = 0; g0 < nstrips; g0 += G) {
+            int cnt[32] = {}, mx = 0;
+            for (int s = g0; s < std::min(g0 + G, nstrips); ++s) {
+                const long o = ((long)(s / nsx) * st * P + (long)(s % nsx) * strip * st) * words;
+                for (int k = 0; k < words; ++k) mx = std::max(mx, ++cnt[(o + k) % 32]);
+            }
+            t += mx;
+        }
+        return t;
+    };
     MatchArgs c = a;
     c.tilePitch = a.tileW | 1;
+    int bc = cost(c.tilePitch);
     const size_t l0 = match_lds_bytes(c);
-    c.tilePitch = a.tileW + ((want - a.tileW) % 64 + 64) % 64;
+    for (int P = a.tileW; P < a.tileW + 32; ++P) {
+        const int cp = cost(P);
+        if (cp < bc) {
+            bc = cp;
+            c.tilePitch = P;
+        }
+    }
     const size_t l1 = match_lds_bytes(c);
     constexpr size_t cu_lds = 160 * 1024;
     return std::min<size_t>(4, cu_lds / l1) >= std::min<size_t>(4, cu_lds / l0) ? c.tilePitch : (a.tileW | 1);
