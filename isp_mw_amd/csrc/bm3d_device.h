@@ -25,6 +25,17 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
+// Lane q's value of v, q the same in every lane: v_readlane from registers
+// (a shuffle or an LDS array would wait on an LDS round trip per call).
+__device__ __forceinline__ uint32_t lane_value(uint32_t v, uint32_t q)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)q);
+}
+__device__ __forceinline__ uint64_t lane_value(uint64_t v, uint32_t q)
+{
+    return ((uint64_t)lane_value((uint32_t)(v >> 32), q) << 32) | lane_value((uint32_t)v, q);
+}
+
 // The image and tile (or row) a workgroup works on: its own launch's
 // arguments and blockIdx, or, in a batched launch, the map entry of this
 // workgroup and that image's arguments.
@@ -71,14 +82,44 @@ __device__ __forceinline__ int wave_sum(int v)
     return v;
 }
 
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane)
+// Inclusive prefix sum over the wave's 64 lanes by DPP, without LDS: row
+// shifts by 1, 2, 4, 8 (a lane with no source in its row of 16 adds 0) scan
+// each row; row_bcast:15 then adds row 0's total to row 1 and row 2's to row
+// 3, and row_bcast:31 the total of rows 0-1 to rows 2 and 3.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int)
 {
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t t = __shfl_up(v, o);
-        if (lane >= o) v += t;
-    }
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);   // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);   // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true);   // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true);   // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15, rows 1 and 3
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31, rows 2 and 3
     return v;
+}
+
+// Minimum / maximum over the wave by DPP, in every lane: the row scans of
+// wave_incl_scan with min / max (a lane with no source takes the identity),
+// then lane 63's value.
+__device__ __forceinline__ uint32_t wave_min(uint32_t v)
+{
+    const int id = (int)~0u;
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(id, (int)v, 0x111, 0xf, 0xf, false));
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(id, (int)v, 0x112, 0xf, 0xf, false));
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(id, (int)v, 0x114, 0xf, 0xf, false));
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(id, (int)v, 0x118, 0xf, 0xf, false));
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(id, (int)v, 0x142, 0xa, 0xf, false));
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(id, (int)v, 0x143, 0xc, 0xf, false));
+    return lane_value(v, 63);
+}
+__device__ __forceinline__ uint32_t wave_max(uint32_t v)
+{
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false));
+    return lane_value(v, 63);
 }
 
 // Block::_SearchBoundary (include/Block.h:716-745); range already a multiple of step.
@@ -261,9 +302,9 @@ __device__ __forceinline__ uint32_t select_smallest(const KT *keys, const uint32
                     c += hh[q];
                 }
             }
-            dig = __shfl(dig, L);
-            blw = __shfl(blw, L);
-            mm = __shfl(mm, L);
+            dig = lane_value(dig, (uint32_t)L);
+            blw = lane_value(blw, (uint32_t)L);
+            mm = lane_value(mm, (uint32_t)L);
             need -= blw;
             prefix |= (KT)dig << shift;
             pmask |= (KT)0xFF << shift;
@@ -306,10 +347,21 @@ __device__ __forceinline__ uint32_t select_smallest(const KT *keys, const uint32
                     const uint32_t e = memb[q];
                     kq = keys[e];
                     iq = idxs[e];
-                    for (uint32_t r2 = 0; r2 < nm; ++r2) {
-                        const uint32_t e2 = memb[r2];
-                        const KT k2 = keys[e2];
-                        const uint32_t i2 = idxs[e2];
+                }
+                // the bucket's members 64 at a time, one per lane, each
+                // compared with this lane's from registers
+                for (uint32_t r0 = 0; r0 < nm; r0 += 64) {
+                    KT kr = 0;
+                    uint32_t ir = 0;
+                    if (r0 + lane < nm) {
+                        const uint32_t e2 = memb[r0 + lane];
+                        kr = keys[e2];
+                        ir = idxs[e2];
+                    }
+                    const uint32_t nr = nm - r0 < 64u ? nm - r0 : 64u;
+                    for (uint32_t r2 = 0; r2 < nr; ++r2) {
+                        const KT k2 = lane_value(kr, r2);
+                        const uint32_t i2 = lane_value(ir, r2);
                         rank += (k2 < kq) || (k2 == kq && i2 < iq);
                     }
                 }

@@ -211,14 +211,19 @@ __global__ __launch_bounds__(kThreads) void match_kernel(MatchArgs a0, const uin
                 if (a.key_out) a.key_out[base] = 0.0;
             }
         }
+        // each lane holds one selected entry; its rank from the others' by
+        // lane_value
+        KT kq = 0;
+        uint32_t iq = 0, rank = 0;
         if ((uint32_t)lane < nsel) {
-            const KT kq = selk[lane];
-            const uint32_t iq = seli[lane];
-            uint32_t rank = 0;
-            for (uint32_t q = 0; q < nsel; ++q) {
-                const KT k2 = selk[q];
-                rank += (k2 < kq) || (k2 == kq && seli[q] < iq);
-            }
+            kq = selk[lane];
+            iq = seli[lane];
+        }
+        for (uint32_t q = 0; q < nsel; ++q) {
+            const KT k2 = lane_value(kq, q);
+            rank += (k2 < kq) || (k2 == kq && lane_value(iq, q) < iq);
+        }
+        if ((uint32_t)lane < nsel) {
             const int ky = (int)(iq / (uint32_t)nx), kx = (int)(iq - (uint32_t)ky * nx);
             const int y = t + ky * st, x = l + kx * st;
             a.pos_out[base + first + rank] = (y << 16) | x;
@@ -281,7 +286,7 @@ __device__ __forceinline__ uint32_t select_walk(const uint32_t (&key)[RND][STRIP
                 kmax = k > kmax ? k : kmax;
             }
         }
-    const uint32_t total = (uint32_t)wave_sum(n);
+    const uint32_t total = lane_value(wave_incl_scan((uint32_t)n, lane), 63);
     uint32_t nsel = 0;
     if (total <= K) {
 #pragma unroll
@@ -300,12 +305,8 @@ __device__ __forceinline__ uint32_t select_walk(const uint32_t (&key)[RND][STRIP
         wave_sync();
         return nsel;
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const uint32_t x = __shfl_xor(kmin, o), y = __shfl_xor(kmax, o);
-        kmin = x < kmin ? x : kmin;
-        kmax = y > kmax ? y : kmax;
-    }
+    kmin = wave_min(kmin);
+    kmax = wave_max(kmax);
     const uint32_t diff = kmin ^ kmax;
     const int top = diff == 0 ? 0 : 31 - __clz((int)diff);
     int shift = top < 8 ? 0 : top - 7;
@@ -346,9 +347,9 @@ __device__ __forceinline__ uint32_t select_walk(const uint32_t (&key)[RND][STRIP
                 c += hh[q];
             }
         }
-        dig = __shfl(dig, L);
-        blw = __shfl(blw, L);
-        mm = __shfl(mm, L);
+        dig = lane_value(dig, (uint32_t)L);
+        blw = lane_value(blw, (uint32_t)L);
+        mm = lane_value(mm, (uint32_t)L);
         need -= blw;
         prefix |= dig << shift;
         pmask |= 0xFFu << shift;
@@ -389,17 +390,17 @@ __device__ __forceinline__ uint32_t select_walk(const uint32_t (&key)[RND][STRIP
             }
         }
     if (rank) {
-        // at most 64 members: exact rank by (key, index)
+        // at most 64 members, one per lane: exact rank by (key, index)
         wave_sync();
         const bool v = (uint32_t)lane < nm;
         uint32_t kq = 0, iq = 0, rk = 0;
         if (v) {
             kq = memk[lane];
             iq = memi[lane];
-            for (uint32_t q = 0; q < nm; ++q) {
-                const uint32_t k2 = memk[q], i2 = memi[q];
-                rk += (k2 < kq) || (k2 == kq && i2 < iq);
-            }
+        }
+        for (uint32_t q = 0; q < nm; ++q) {
+            const uint32_t k2 = lane_value(kq, q), i2 = lane_value(iq, q);
+            rk += (k2 < kq) || (k2 == kq && i2 < iq);
         }
         const bool take = v && rk < need;
         const uint64_t tm = __ballot(take);
@@ -431,7 +432,7 @@ __device__ __forceinline__ uint32_t select_walk(const uint32_t (&key)[RND][STRIP
                     ++p;
                 }
             }
-            before += __shfl(incl, 63);
+            before += lane_value(incl, 63);
         }
         nsel += need;
     }
@@ -634,7 +635,7 @@ __global__ __launch_bounds__(kThreads, 4) void match_walk_kernel(MatchArgs a0, c
                 // MULTI windows are below 2^16 offsets)
                 const uint64_t vq = (uint32_t)lane < nsel ? ((uint64_t)selk[lane] << 16) | seli[lane] : ~0ull;
                 uint32_t rank = 0;
-                for (uint32_t q = 0; q < nsel; ++q) rank += __shfl(vq, (int)q) < vq;
+                for (uint32_t q = 0; q < nsel; ++q) rank += lane_value(vq, q) < vq;
                 if (MULTI && h + 1 < nchunks) {
                     // kept for the next chunk, in (key, index) order
                     if ((uint32_t)lane < nsel) {

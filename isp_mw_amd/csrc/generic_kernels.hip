@@ -156,14 +156,19 @@ __global__ __launch_bounds__(kThreads) void match_generic_kernel(MatchArgs a)
             a.pos_out[base] = (j << 16) | i;
             if (a.key_out) a.key_out[base] = 0.0;
         }
+        // each lane holds one selected entry (nsel <= 63); its rank from the
+        // others' by lane_value
+        KT kq = 0;
+        uint32_t iq = 0, rank = 0;
         if ((uint32_t)lane < nsel) {
-            const KT kq = selk[lane];
-            const uint32_t iq = seli[lane];
-            uint32_t rank = 0;
-            for (uint32_t q = 0; q < nsel; ++q) {
-                const KT k2 = selk[q];
-                rank += (k2 < kq) || (k2 == kq && seli[q] < iq);
-            }
+            kq = selk[lane];
+            iq = seli[lane];
+        }
+        for (uint32_t q = 0; q < nsel; ++q) {
+            const KT k2 = lane_value(kq, q);
+            rank += (k2 < kq) || (k2 == kq && lane_value(iq, q) < iq);
+        }
+        if ((uint32_t)lane < nsel) {
             const int ky = (int)(iq / (uint32_t)nx), kx = (int)(iq - (uint32_t)ky * nx);
             const int y = t + ky * st, x = l + kx * st;
             a.pos_out[base + 1 + rank] = (y << 16) | x;
