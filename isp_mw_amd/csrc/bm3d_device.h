@@ -343,25 +343,16 @@ __device__ __forceinline__ uint32_t select_smallest(const KT *keys, const uint32
                 const bool v = q < nm;
                 KT kq = 0;
                 uint32_t iq = 0, rank = 0;
+                // (the bucket can hold hundreds of equal keys here, where
+                // LDS broadcast reads pipeline better than per-member readlanes)
                 if (v) {
                     const uint32_t e = memb[q];
                     kq = keys[e];
                     iq = idxs[e];
-                }
-                // the bucket's members 64 at a time, one per lane, each
-                // compared with this lane's from registers
-                for (uint32_t r0 = 0; r0 < nm; r0 += 64) {
-                    KT kr = 0;
-                    uint32_t ir = 0;
-                    if (r0 + lane < nm) {
-                        const uint32_t e2 = memb[r0 + lane];
-                        kr = keys[e2];
-                        ir = idxs[e2];
-                    }
-                    const uint32_t nr = nm - r0 < 64u ? nm - r0 : 64u;
-                    for (uint32_t r2 = 0; r2 < nr; ++r2) {
-                        const KT k2 = lane_value(kr, r2);
-                        const uint32_t i2 = lane_value(ir, r2);
+                    for (uint32_t r2 = 0; r2 < nm; ++r2) {
+                        const uint32_t e2 = memb[r2];
+                        const KT k2 = keys[e2];
+                        const uint32_t i2 = idxs[e2];
                         rank += (k2 < kq) || (k2 == kq && i2 < iq);
                     }
                 }

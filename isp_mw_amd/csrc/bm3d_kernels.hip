@@ -211,17 +211,31 @@ __global__ __launch_bounds__(kThreads) void match_kernel(MatchArgs a0, const uin
                 if (a.key_out) a.key_out[base] = 0.0;
             }
         }
-        // each lane holds one selected entry; its rank from the others' by
-        // lane_value
+        // each lane holds one selected entry and counts the entries before
+        // it: from the other lanes' registers by lane_value at BMstep 1, from
+        // LDS otherwise.  Measured (same build, A/B): at BMstep 1 (the plain
+        // form's FP64 matching at 1080p) lane_value took 1.46 + 1.16 ->
+        // 1.42 + 1.12 ms; at BMstep 3 (NL-means' exact form) it made
+        // nlm_match 1.13 -> 1.49 ms, a difference not traced to the loop
         KT kq = 0;
         uint32_t iq = 0, rank = 0;
         if ((uint32_t)lane < nsel) {
             kq = selk[lane];
             iq = seli[lane];
         }
-        for (uint32_t q = 0; q < nsel; ++q) {
-            const KT k2 = lane_value(kq, q);
-            rank += (k2 < kq) || (k2 == kq && lane_value(iq, q) < iq);
+        if constexpr (ST1) {
+            // a uniform trip count and no short-circuit: no divergent loop
+            const uint32_t nq = (uint32_t)__builtin_amdgcn_readfirstlane((int)nsel);
+            for (uint32_t q = 0; q < nq; ++q) {
+                const KT k2 = lane_value(kq, q);
+                const uint32_t i2 = lane_value(iq, q);
+                rank += (uint32_t)((k2 < kq) | ((k2 == kq) & (i2 < iq)));
+            }
+        } else if ((uint32_t)lane < nsel) {
+            for (uint32_t q = 0; q < nsel; ++q) {
+                const KT k2 = selk[q];
+                rank += (k2 < kq) || (k2 == kq && seli[q] < iq);
+            }
         }
         if ((uint32_t)lane < nsel) {
             const int ky = (int)(iq / (uint32_t)nx), kx = (int)(iq - (uint32_t)ky * nx);
@@ -400,7 +414,7 @@ __device__ __forceinline__ uint32_t select_walk(const uint32_t (&key)[RND][STRIP
         }
         for (uint32_t q = 0; q < nm; ++q) {
             const uint32_t k2 = lane_value(kq, q), i2 = lane_value(iq, q);
-            rk += (k2 < kq) || (k2 == kq && i2 < iq);
+            rk += (uint32_t)((k2 < kq) | ((k2 == kq) & (i2 < iq)));
         }
         const bool take = v && rk < need;
         const uint64_t tm = __ballot(take);

@@ -164,9 +164,12 @@ __global__ __launch_bounds__(kThreads) void match_generic_kernel(MatchArgs a)
             kq = selk[lane];
             iq = seli[lane];
         }
-        for (uint32_t q = 0; q < nsel; ++q) {
+        // a uniform trip count and no short-circuit: no divergent loop
+        const uint32_t nq = (uint32_t)__builtin_amdgcn_readfirstlane((int)nsel);
+        for (uint32_t q = 0; q < nq; ++q) {
             const KT k2 = lane_value(kq, q);
-            rank += (k2 < kq) || (k2 == kq && lane_value(iq, q) < iq);
+            const uint32_t i2 = lane_value(iq, q);
+            rank += (uint32_t)((k2 < kq) | ((k2 == kq) & (i2 < iq)));
         }
         if ((uint32_t)lane < nsel) {
             const int ky = (int)(iq / (uint32_t)nx), kx = (int)(iq - (uint32_t)ky * nx);
