@@ -156,22 +156,14 @@ __global__ __launch_bounds__(kThreads) void match_generic_kernel(MatchArgs a)
             a.pos_out[base] = (j << 16) | i;
             if (a.key_out) a.key_out[base] = 0.0;
         }
-        // each lane holds one selected entry (nsel <= 63); its rank from the
-        // others' by lane_value
-        KT kq = 0;
-        uint32_t iq = 0, rank = 0;
         if ((uint32_t)lane < nsel) {
-            kq = selk[lane];
-            iq = seli[lane];
-        }
-        // a uniform trip count and no short-circuit: no divergent loop
-        const uint32_t nq = (uint32_t)__builtin_amdgcn_readfirstlane((int)nsel);
-        for (uint32_t q = 0; q < nq; ++q) {
-            const KT k2 = lane_value(kq, q);
-            const uint32_t i2 = lane_value(iq, q);
-            rank += (uint32_t)((k2 < kq) | ((k2 == kq) & (i2 < iq)));
-        }
-        if ((uint32_t)lane < nsel) {
+            const KT kq = selk[lane];
+            const uint32_t iq = seli[lane];
+            uint32_t rank = 0;
+            for (uint32_t q = 0; q < nsel; ++q) {
+                const KT k2 = selk[q];
+                rank += (k2 < kq) || (k2 == kq && seli[q] < iq);
+            }
             const int ky = (int)(iq / (uint32_t)nx), kx = (int)(iq - (uint32_t)ky * nx);
             const int y = t + ky * st, x = l + kx * st;
             a.pos_out[base + 1 + rank] = (y << 16) | x;
