@@ -60,7 +60,9 @@ def test_committed_profiles_parse(prec, stage):
     assert t is not None and t["fetch_bytes"] > 0 and t["write_bytes"] > 0
     assert t["bytes"] == t["fetch_bytes"] + t["write_bytes"]
     u = bench.measured_util(prec, stage)
-    assert u is not None and 0 < u["valu_busy"] <= 1 and 0 <= u["mfma_busy"] <= 1
+    # valu_busy sums SQ_ACTIVE_INST_VALU over a SIMD's waves, whose VALU
+    # instructions overlap in the pipeline: it passes 1 a little at the issue limit
+    assert u is not None and 0 < u["valu_busy"] <= 1.25 and 0 <= u["mfma_busy"] <= 1
     # both forms filter on the matrix cores (fp32: split-FP16 products); matching is vector work
     assert (u["mfma_insts_per_launch"] > 0) == stage.startswith("filter")
     assert u["ms_under_counters"] > 0
