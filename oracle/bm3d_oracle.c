@@ -431,7 +431,7 @@ static void gather(double *grp, const double *const *srcs, const int *fz, int st
 }
 
 /* ------------------------------------------------------------------------ */
-/* Variant modes of the north_star's configs, which the reference does not   */
+/* Opt-in variant modes of the filter, which the reference does not          */
 /* have (it uses a DCT along the group and a uniform weight per block):       */
 /*   Haar: the group axis is transformed by the orthonormal Haar matrix, and  */
 /*         a group is truncated to the largest power of two of its members;  */
